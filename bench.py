@@ -1,0 +1,196 @@
+"""Benchmark: rational-quaternion Julia field evaluation at 512^3 on one MI355X.
+
+One step = one evaluation of the whole 512^3 field (config-2 roots: 32 top + 32
+bottom Gaussian roots, powers in {1,2}, maxIter 4, escape 2000) with the FP32
+bulk + FP64 band kernel pair. The plain all-FP64 kernel is timed beside it.
+Inputs (roots) and the output field are resident in HBM during the timed region.
+
+Prints one JSON line on stdout; progress goes to stderr.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+PEAK_FP32_TFLOPS = 157.3   # MI355X published FP32 vector peak
+PEAK_FP64_TFLOPS = 78.6    # MI355X published FP64 vector peak
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def basic_ops_per_voxel_iter(nt, nb):
+    """Algorithmic IEEE basic ops per voxel-iteration of the reference method:
+    per root term 4 (subtract) + 23 (pow) + 28 (Hamilton product); division 40,
+    scaling 4, magnitude 8 (QUATERNION.cpp:154-200, 375-393)."""
+    return 55 * (nt + nb) - 4 if nb > 0 else 55 * nt - 16
+
+
+def cpu_baseline(w, gpu64, gpu32, slabs, threads):
+    import oracle as O
+    res = w.res
+    g = O.make_grid(res, w.center, w.lengths)
+    top = O.poly(w.top.roots, w.top.powers, w.top.power_scalar)
+    bot = O.poly(w.bottom.roots, w.bottom.powers, w.bottom.power_scalar)
+    p = O.make_params(w.max_iter, w.escape, w.exp_scaling, w.slice, O.BLACK_HOLE)
+    zs = np.linspace(0, res[2] - 1, slabs).round().astype(int)
+    plane = res[0] * res[1]
+    t0 = time.perf_counter()
+    flips64 = flips32 = 0
+    maxerr = 0.0
+    for z in zs:
+        f = O.field_map_zrange(g, top, bot, p, int(z), int(z) + 1, nthreads=threads)
+        a = gpu64[z * plane:(z + 1) * plane]
+        b = gpu32[z * plane:(z + 1) * plane]
+        flips64 += int(np.count_nonzero((f.astype(np.float32) > 0) != (a.astype(np.float32) > 0)))
+        flips32 += int(np.count_nonzero((f.astype(np.float32) > 0) != (b.astype(np.float32) > 0)))
+        fin = np.isfinite(f) & np.isfinite(a)
+        if fin.any():
+            maxerr = max(maxerr, float(np.max(np.abs(f[fin] - a[fin]) / np.maximum(1.0, np.abs(f[fin])))))
+    dt = time.perf_counter() - t0
+    nominal = len(zs) * plane * w.max_iter
+    return dict(value=nominal / dt / 1e9, seconds=dt, slabs=len(zs), flips_fp64=flips64, flips_fp32band=flips32,
+                max_rel_err_fp64=maxerr)
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--res", type=int, default=512)
+    ap.add_argument("--fp64-steps", type=int, default=2)
+    ap.add_argument("--cpu-slabs", type=int, default=24)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--band", type=float, default=0.05)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU: this build runs on a single MI355X and has no multi-GPU path "
+              "(the sharding design is in DESIGN.md); run with --gpus 1", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+    from quijibo_amd import api, workloads as W
+    from quijibo_amd._lib import FP32_BAND, FP64, MAP_BLACK_HOLE
+
+    torch.cuda.set_device(0)
+    w = W.headline(args.res)
+    nt, nb = w.top.n, w.bottom.n
+    grid = api.make_grid(w.res, w.center, w.lengths)
+    params = api.make_params(w.max_iter, w.escape, w.exp_scaling, w.slice, MAP_BLACK_HOLE)
+    r = api.Rational(api.POLYNOMIAL_4D(w.top.roots, w.top.powers), api.POLYNOMIAL_4D(w.bottom.roots, w.bottom.powers))
+    ws = api.Workspace()
+    out32 = torch.empty(w.voxels, dtype=torch.float64, device="cuda")
+    out64 = torch.empty(w.voxels, dtype=torch.float64, device="cuda")
+    nominal = w.voxels * w.max_iter
+    stream = torch.cuda.current_stream()
+
+    # plain FP64 kernel
+    log("fp64: warmup")
+    api.field_eval(grid, r, params, FP64, out=out64, workspace=ws)
+    torch.cuda.synchronize()
+    ws.reset_stats()
+    api._lib.load().qj_workspace_timing(ws.handle, 1)
+    t0 = time.perf_counter()
+    for _ in range(args.fp64_steps):
+        api.field_eval(grid, r, params, FP64, out=out64, workspace=ws)
+    torch.cuda.synchronize()
+    t64 = (time.perf_counter() - t0) / args.fp64_steps
+    st64 = ws.stats()
+    k64 = _kernel_ms(api, ws)
+    log("fp64: %.2f ms/step" % (t64 * 1e3))
+
+    # FP32 bulk + FP64 band
+    for _ in range(args.warmup):
+        api.field_eval(grid, r, params, FP32_BAND, args.band, out=out32, workspace=ws)
+    torch.cuda.synchronize()
+    ws.reset_stats()
+    api._lib.load().qj_workspace_timing(ws.handle, 1)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        api.field_eval(grid, r, params, FP32_BAND, args.band, out=out32, workspace=ws)
+    torch.cuda.synchronize()
+    t32 = (time.perf_counter() - t0) / args.steps
+    st32 = ws.stats()
+    k32 = _kernel_ms(api, ws)
+    api._lib.load().qj_workspace_timing(ws.handle, 0)
+    log("fp32+band: %.2f ms/step" % (t32 * 1e3))
+
+    f64 = out64.cpu().numpy()
+    f32 = out32.cpu().numpy()
+    flips_vs_fp64 = int(np.count_nonzero((f64.astype(np.float32) > 0) != (f32.astype(np.float32) > 0)))
+    fin = np.isfinite(f64) & np.isfinite(f32)
+    maxdf = float(np.max(np.abs(f64[fin] - f32[fin]))) if fin.any() else 0.0
+
+    value = nominal / t32 / 1e9
+    value64 = nominal / t64 / 1e9
+    iters_per_launch = st32["iterations"] / args.steps
+    bulk_ms = k32["bulk_ms"] / max(1, k32["launches"])
+    band_ms = k32["band_ms"] / max(1, k32["launches"])
+    flops = iters_per_launch * basic_ops_per_voxel_iter(nt, nb)
+    achieved = flops / (bulk_ms * 1e-3) / 1e12
+    roofline = {"bound": "valu", "kernel": "k_grid32", "achieved": round(achieved, 3), "peak": PEAK_FP32_TFLOPS,
+                "unit": "TFLOP/s", "frac": round(achieved / PEAK_FP32_TFLOPS, 4), "traffic": None,
+                "algorithmic_flops_per_launch": flops, "kernel_ms": round(bulk_ms, 4),
+                "executed_voxel_iters_per_launch": iters_per_launch}
+    fp64_ms = k64["bulk_ms"] / max(1, k64["launches"])
+    fp64_flops = st64["iterations"] / args.fp64_steps * basic_ops_per_voxel_iter(nt, nb)
+    cpu = None
+    if not args.no_cpu:
+        threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or os.cpu_count()
+        log("cpu baseline: %d slabs on %d threads" % (args.cpu_slabs, threads))
+        cb = cpu_baseline(w, f64, f32, args.cpu_slabs, threads)
+        cpu = {"value": round(cb["value"], 6), "unit": "Gvoxel-iter/s", "cores": threads, "kind": "port",
+               "sample": "%d of %d z-slabs (%dx%d voxels each) of the same 512^3 workload, oracle C restatement "
+                         "(FP64, OpenMP), %.1f s" % (cb["slabs"], w.res[2], w.res[0], w.res[1], cb["seconds"]),
+               "parity_on_sample": {"sign_flips_fp64_gpu": cb["flips_fp64"],
+                                    "sign_flips_fp32band_gpu": cb["flips_fp32band"],
+                                    "max_rel_err_fp64_gpu": cb["max_rel_err_fp64"]}}
+    line = {
+        "metric": "Gvoxel-iter/s at 512^3 (FP32+FP64 band vs FP64) + p50/p99 request latency",
+        "value": round(value, 4),
+        "unit": "Gvoxel-iter/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(t32 * 1e3, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32+f64",
+        "data": "synthetic (seeded roots, numpy default_rng(1002))",
+        "config": {"workload": "c2_rational_32x32_%d: %d^3 grid on [-1.5,1.5]^3, 32 top + 32 bottom roots, "
+                               "powers U{1,2}, maxIter 4, escape 2000, band |F|<%g" % (args.res, args.res, args.band),
+                   "res": args.res, "n_top": nt, "n_bottom": nb, "max_iter": w.max_iter, "band": args.band},
+        "fp64_plain": {"value": round(value64, 4), "ms_per_step": round(t64 * 1e3, 3), "kernel_ms": round(fp64_ms, 3),
+                       "achieved_tflops": round(fp64_flops / (fp64_ms * 1e-3) / 1e12, 3),
+                       "peak": PEAK_FP64_TFLOPS},
+        "speedup_vs_fp64": round(value / value64, 3),
+        "band": {"voxels": st32["band_voxels"] // args.steps, "fraction": st32["band_voxels"] / args.steps / w.voxels,
+                 "band_kernel_ms": round(band_ms, 4)},
+        "parity": {"sign_flips_vs_fp64_gpu": flips_vs_fp64, "max_abs_dF_vs_fp64_gpu": maxdf},
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line), flush=True)
+
+
+def _kernel_ms(api, ws):
+    import ctypes
+    b = ctypes.c_double()
+    d = ctypes.c_double()
+    n = ctypes.c_int64()
+    api.check(api._lib.load().qj_workspace_kernel_ms(ws.handle, ctypes.byref(b), ctypes.byref(d), ctypes.byref(n)))
+    return {"bulk_ms": b.value, "band_ms": d.value, "launches": n.value}
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,253 @@
+/*
+ * quijibo-mi355x C ABI: the drop-in boundary for QUIJIBO's rational-quaternion
+ * Julia field path on MI355X (gfx950).
+ *
+ * Plain C: pointers, sizes and status codes only. Device pointers are HIP
+ * device addresses; `stream` is a hipStream_t (NULL = the null stream). Every
+ * call that takes a stream only enqueues work on it unless stated otherwise.
+ * Functions return QJ_OK or an error code; qj_last_error() gives the message.
+ * The reference reports numeric trouble as inf/NaN in the field and calls
+ * exit(0) on fatal conditions; here fatal conditions are status codes.
+ *
+ * Reference interfaces replaced (theodorekim/QUIJIBO):
+ *   OPTIMIZE_3D::computeLogScaledPowerRationalMap    src/solvers/OPTIMIZE_3D.h:161, .cpp:1272-1370
+ *   OPTIMIZE_3D::computeRationalMapAP                src/solvers/OPTIMIZE_3D.cpp:1374-1568
+ *   OPTIMIZE_3D::computeLogScaledPowerRationalMapCached  src/solvers/OPTIMIZE_3D.cpp:1572-1718
+ *   the dormant GPU seam setTopRoots/setConsts/runMap  src/solvers/OPTIMIZE_3D.cpp:3842-3858
+ *   TRIANGLE_MESH(const FIELD_3D&) / computeStagedMarchingCubes  src/geometry/TRIANGLE_MESH.h:42, .cpp:989-1114
+ *   OPTIMIZE_3D::computeScoreAP, computeExpGradientAP, compute{Top,Bottom}GradientAP,
+ *     compute{Top,Bottom}BulkScalarGradientAP       src/solvers/OPTIMIZE_3D.cpp:3563, 3589, 8905, 10012, 9250, 10360
+ *   OPTIMIZE_3D::computeLogRationalCenteredGradient (root-position gradient) src/solvers/OPTIMIZE_3D.cpp:2872-2951
+ */
+#ifndef QJ_C_API_H
+#define QJ_C_API_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef int32_t qj_status;
+enum {
+  QJ_OK = 0,
+  QJ_ERR_INVALID = 1,     /* bad argument (sizes, null pointers, unsupported combination) */
+  QJ_ERR_HIP = 2,         /* a HIP runtime call failed */
+  QJ_ERR_NOMEM = 3,       /* device allocation failed */
+  QJ_ERR_CAPACITY = 4,    /* request larger than the configured capacity */
+  QJ_ERR_NUMERIC = 5,     /* inconsistent numeric state (the reference would exit(0)) */
+  QJ_ERR_SHUTDOWN = 6     /* server is stopping */
+};
+
+/* A FIELD_3D geometry (FIELD_3D.h:58,73-80; FIELD_3D.cpp:28-57): cell-centred,
+ * x fastest, data[z*xRes*yRes + y*xRes + x]. rotation is a quaternion
+ * (w,x,y,z) applied about center (FIELD_3D.cpp:818-841); identity (1,0,0,0). */
+typedef struct qj_grid {
+  int32_t res[3];
+  double center[3];
+  double lengths[3];
+  double rotation[4];
+} qj_grid;
+
+/* A POLYNOMIAL_4D in factored form (POLYNOMIAL_4D.h:137-150): roots (n x 4,
+ * slot order w,x,y,z), per-root powers and the uniform power scalar. Host
+ * pointers. Term i is (q - root_i)^(power_scalar * powers[i]). */
+typedef struct qj_poly {
+  int32_t n_roots;
+  const double* roots;
+  const double* powers;
+  double power_scalar;
+} qj_poly;
+
+/* map flags */
+#define QJ_MAP_BLACK_HOLE 1u  /* stop iterating when |q| < 10*FLT_MIN (OPTIMIZE_3D.cpp:1358-1359) */
+#define QJ_MAP_SKIP_ORIGIN 2u /* leave cells with |q0| < 1e-8 unwritten (OPTIMIZE_3D.cpp:1428, 1619) */
+#define QJ_MAP_ADAPTIVE 4u    /* flag overflowed cells and recompute them with extended range
+                                 (computeRationalMapAP's long double re-pass, OPTIMIZE_3D.cpp:1477-1567) */
+
+/* Map parameters: the OPTIMIZE_3D members the map reads (OPTIMIZE_3D.h:67,73,81)
+ * plus the escape radius each reference variant hard-codes (2000 at
+ * OPTIMIZE_3D.cpp:1291, 20 at :1388 and :1589). */
+typedef struct qj_map_params {
+  int32_t max_iter;
+  double escape;
+  double exp_scaling;
+  double slice;
+  uint32_t flags;
+} qj_map_params;
+
+/* Reference presets for qj_map_params.flags/escape */
+#define QJ_PRESET_SCALED_POWER_ESCAPE 2000.0 /* computeLogScaledPowerRationalMap, flags = BLACK_HOLE */
+#define QJ_PRESET_AP_ESCAPE 20.0             /* computeRationalMapAP, flags = SKIP_ORIGIN|ADAPTIVE */
+
+typedef enum qj_precision {
+  QJ_FP64 = 0,      /* every voxel in FP64, reference expression order */
+  QJ_FP32_BAND = 1  /* FP32 bulk; FP64 re-evaluation of the band |F| < band, of
+                       non-finite results and of voxels whose escape or black-hole
+                       test was within FP32's error of its threshold */
+} qj_precision;
+
+/* Counters accumulated by the evaluation calls on a workspace since the last
+ * qj_workspace_reset_stats. */
+typedef struct qj_eval_stats {
+  int64_t voxels;            /* voxels evaluated (bulk pass) */
+  int64_t iterations;        /* executed voxel-iterations in the bulk pass */
+  int64_t band_voxels;       /* voxels re-evaluated in FP64 (QJ_FP32_BAND) */
+  int64_t band_iterations;   /* executed voxel-iterations of the FP64 re-evaluation */
+  int64_t adaptive_voxels;   /* cells flagged as overflowed (QJ_MAP_ADAPTIVE) */
+  int64_t nonfinite;         /* non-finite results written */
+} qj_eval_stats;
+
+typedef struct qj_rational qj_rational;   /* top/bottom root sets resident in HBM */
+typedef struct qj_workspace qj_workspace; /* device scratch + counters for one stream */
+
+/* ---- library ---- */
+const char* qj_version(void);
+const char* qj_last_error(void); /* thread-local message of the last failing call */
+const char* qj_status_string(qj_status s);
+qj_status qj_init(int32_t device); /* selects the device, loads kernels; idempotent */
+
+/* ---- root sets: POLYNOMIAL_4D pair on device (replaces setTopRoots,
+ *      OPTIMIZE_3D.cpp:3842-3850) ---- */
+qj_status qj_rational_create(const qj_poly* top, const qj_poly* bottom, qj_rational** out);
+/* Replace roots/powers in place (same root counts), enqueued on `stream`. */
+qj_status qj_rational_update(qj_rational* f, const qj_poly* top, const qj_poly* bottom, void* stream);
+qj_status qj_rational_counts(const qj_rational* f, int32_t* n_top, int32_t* n_bottom);
+qj_status qj_rational_destroy(qj_rational* f);
+
+/* ---- workspaces ---- */
+qj_status qj_workspace_create(qj_workspace** out);
+qj_status qj_workspace_destroy(qj_workspace* ws);
+/* Synchronizes `stream`, then copies the counters. */
+qj_status qj_workspace_stats(qj_workspace* ws, void* stream, qj_eval_stats* out);
+qj_status qj_workspace_reset_stats(qj_workspace* ws, void* stream);
+
+/* Kernel timing on a workspace (profiling hook, like the reference's TIMER,
+ * TIMER.h:37-79): when enabled, qj_field_eval / qj_points_eval record HIP
+ * events around the bulk kernel and the FP64 re-evaluation kernel on the
+ * call's stream; qj_workspace_kernel_ms synchronizes and returns the summed
+ * milliseconds and launch count since enabling. */
+qj_status qj_workspace_timing(qj_workspace* ws, int32_t enable);
+qj_status qj_workspace_kernel_ms(qj_workspace* ws, double* bulk_ms, double* band_ms, int64_t* launches);
+
+/* ---- field evaluation ---- */
+
+/* computeLogScaledPowerRationalMap (OPTIMIZE_3D.cpp:1272-1370) and its AP /
+ * cached variants through flags: for every cell, q0 = (X, Y, Z, slice) at the
+ * cell centre; while |q| < escape and it < max_iter: q = c * T(q) * B(q)^-1
+ * (q = c * T(q) with no bottom roots); field = log|q|. d_field: device,
+ * res[0]*res[1]*res[2] doubles, x fastest. band is used by QJ_FP32_BAND (the
+ * reference-facing default is 0.05). */
+qj_status qj_field_eval(qj_workspace* ws, const qj_grid* grid, const qj_rational* f,
+                        const qj_map_params* params, qj_precision precision, double band,
+                        double* d_field, void* stream);
+
+/* Same map at n arbitrary spatial points (X, Y, Z) (device, n x 3 doubles). */
+qj_status qj_points_eval(qj_workspace* ws, const double* d_points, int64_t n, const qj_rational* f,
+                         const qj_map_params* params, qj_precision precision, double band,
+                         double* d_out, void* stream);
+
+/* ---- marching cubes (TRIANGLE_MESH(const FIELD_3D&), TRIANGLE_MESH.cpp:48-57,
+ *      computeStagedMarchingCubes :989-1114) ----
+ * Flag bit k of a cell is ((float)(v_k - iso) > 0), corners ordered
+ * NNN,NNP,NPN,NPP,PNN,PNP,PPN,PPP. One vertex per edge referenced by a cell's
+ * case row, numbered by (lower endpoint linear index, axis x<y<z); vertex =
+ * p0 + d0/(d0-d1) * (p1-p0) at cell centres; triangles in cell order then
+ * table order; triangles with an edge shorter than 1e-7 are dropped.
+ * iso = 0 is the reference behaviour (it has no isovalue).
+ * qj_mc_run synchronizes `stream` and returns the counts; the mesh stays in
+ * the workspace until the next qj_mc_run on it. */
+qj_status qj_mc_run(qj_workspace* ws, const double* d_field, const qj_grid* grid, double iso,
+                    int64_t* n_vertices, int64_t* n_triangles, void* stream);
+/* Copy the last mesh: vertices n_vertices x 3 doubles, triangles n_triangles x 3
+ * int32 (0-based). Pointers may be host or device memory (hipMemcpyDefault). */
+qj_status qj_mc_copy(qj_workspace* ws, double* vertices, int32_t* triangles, void* stream);
+/* Device pointers to the last mesh (valid until the next qj_mc_run on ws). */
+qj_status qj_mc_device_ptrs(qj_workspace* ws, const double** d_vertices, const int32_t** d_triangles);
+
+/* ---- gradients ---- */
+
+/* One-iteration AP score and exponent gradients (max_iter must be 1, as the
+ * optimizer forces at toothOptimizer.cpp:402):
+ *   score        = sum_cells -tanh(alpha F) w                (weightScore, OPTIMIZE_3D.cpp:4502-4517)
+ *   grad_exp     = sum -alpha w (1 - tanh^2(alpha (F - thr)))  (computeExpGradientAP, :3589-3625)
+ *   grad_top[i]  = d score / d top power i        (computeTopGradientAP, :8905-9106)
+ *   grad_bottom[j] = reference bottom formula     (computeBottomGradientAP, :10012)
+ *   grad_top_scalar / grad_bottom_scalar           (:9250, :10360)
+ * w(cell) is the trilinear lookup of the weight field (wgrid, d_weight) at the
+ * cell centre (FIELD_3D.cpp:966-1029). d_field (device) receives the AP
+ * field. Results are written to host memory; the call synchronizes `stream`. */
+typedef struct qj_ap_grads {
+  double score;
+  double grad_exp;
+  double grad_top_scalar;
+  double grad_bottom_scalar;
+  int64_t adaptive_cells;
+} qj_ap_grads;
+qj_status qj_ap_score_gradients(qj_workspace* ws, const qj_grid* grid, const qj_rational* f,
+                                const qj_map_params* params, const qj_grid* wgrid, const double* d_weight,
+                                double tanh_alpha, double tanh_threshold, double* d_field,
+                                double* grad_top, double* grad_bottom, qj_ap_grads* out, void* stream);
+
+/* Reverse-mode adjoint of the map at points, recomputing each iteration
+ * instead of storing trajectories. Given dJ/dF per point (device, n doubles),
+ * accumulates dJ/d(root components) into d_grad_roots ((n_top + n_bottom) x 4,
+ * top first, component order w,x,y,z: the packing of
+ * computeLogRationalCenteredGradient, OPTIMIZE_3D.cpp:2872-2951) and
+ * dJ/d(powers) into d_grad_powers (n_top + n_bottom). Both are overwritten.
+ * precision QJ_FP64 or QJ_FP32_BAND (FP32 adjoint; band ignored). */
+qj_status qj_points_adjoint(qj_workspace* ws, const double* d_points, int64_t n, const qj_rational* f,
+                            const qj_map_params* params, qj_precision precision, const double* d_dJdF,
+                            double* d_grad_roots, double* d_grad_powers, void* stream);
+
+/* ---- request loop (continuous batching of field solves) ----
+ * Requests of any grid size and root count are admitted while their HBM
+ * footprint fits the budget, split into fixed-size voxel chunks, and packed
+ * with chunks of other requests into launches of a hipGraph captured once
+ * per launch shape; results stay resident in device memory. */
+typedef struct qj_server qj_server;
+
+typedef struct qj_server_config {
+  int64_t hbm_budget_bytes;   /* admission budget for fields + roots (0 = 8 GiB) */
+  int32_t chunk_voxels;       /* voxels per chunk (0 = 65536) */
+  int32_t chunks_per_launch;  /* chunks per graph launch (0 = 512) */
+  int32_t max_roots;          /* largest root count per polynomial (0 = 256) */
+  int32_t inflight_launches;  /* launches in flight (0 = 3) */
+  int32_t precision;          /* qj_precision */
+  double band;
+  int32_t device;
+} qj_server_config;
+
+typedef struct qj_request {
+  qj_grid grid;
+  qj_poly top;
+  qj_poly bottom;
+  qj_map_params params;
+  int32_t keep_field;        /* 1: keep the field resident until qj_server_release */
+  double arrival_time;       /* caller clock (seconds), reported back */
+} qj_request;
+
+typedef struct qj_completion {
+  uint64_t id;
+  qj_status status;
+  double arrival_time;
+  double admit_time;        /* server clock (qj_server_now) */
+  double done_time;
+  const double* d_field;    /* device field if keep_field, else NULL */
+  int64_t voxels;
+} qj_completion;
+
+qj_status qj_server_create(const qj_server_config* cfg, qj_server** out);
+double qj_server_now(qj_server* s); /* seconds on the server's monotonic clock */
+/* Copies the request (roots included); never blocks on the GPU. */
+qj_status qj_server_submit(qj_server* s, const qj_request* req, uint64_t* id);
+/* Up to max completions; waits up to timeout_s for at least one (0 = poll). */
+qj_status qj_server_poll(qj_server* s, qj_completion* out, int32_t max, int32_t* n_out, double timeout_s);
+qj_status qj_server_release(qj_server* s, uint64_t id); /* frees a kept field */
+qj_status qj_server_destroy(qj_server* s);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

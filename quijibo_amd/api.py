@@ -1,0 +1,279 @@
+"""Python host interface over the C ABI, shaped like the reference classes.
+
+  POLYNOMIAL_4D   roots/powers/power scalar of one polynomial (POLYNOMIAL_4D.h:31-150)
+  FIELD_3D        grid geometry + a device-resident float64 tensor, x fastest (FIELD_3D.h:58-80)
+  OPTIMIZE_3D     the map members (_maxIterations, _expScaling, _quaternionSlice) and
+                  computeLogScaledPowerRationalMap / computeRationalMapAP (OPTIMIZE_3D.h:161-169)
+  TRIANGLE_MESH   marching cubes of a FIELD_3D: vertices(), triangles(), writeOBJ (TRIANGLE_MESH.h:42-73)
+
+Device memory and streams come from torch (plumbing only); every computation
+runs in libquijibo.so's HIP kernels.
+"""
+import ctypes
+
+import numpy as np
+import torch
+
+from . import _lib
+from ._lib import FP32_BAND, FP64, MAP_ADAPTIVE, MAP_BLACK_HOLE, MAP_SKIP_ORIGIN, check
+
+DEFAULT_BAND = 0.05
+
+
+def _stream_ptr(stream=None):
+    s = stream if stream is not None else torch.cuda.current_stream()
+    return ctypes.c_void_p(s.cuda_stream)
+
+
+def _ptr(t):
+    return ctypes.c_void_p(t.data_ptr())
+
+
+class POLYNOMIAL_4D:
+    """Factored polynomial: prod_i (q - roots[i])^(power_scalar * powers[i])."""
+
+    def __init__(self, roots=None, powers=None, power_scalar=1.0):
+        r = np.zeros((0, 4)) if roots is None else np.asarray(roots, dtype=np.float64).reshape(-1, 4)
+        self.roots = np.ascontiguousarray(r)
+        self.powers = np.ascontiguousarray(np.ones(len(r)) if powers is None
+                                           else np.asarray(powers, dtype=np.float64).reshape(-1))
+        if self.powers.shape[0] != self.roots.shape[0]:
+            raise ValueError("powers must have one entry per root")
+        self.power_scalar = float(power_scalar)
+
+    def totalRoots(self):
+        return int(self.roots.shape[0])
+
+    def addRoot(self, root, power=1.0):
+        self.roots = np.ascontiguousarray(np.vstack([self.roots, np.asarray(root, dtype=np.float64).reshape(1, 4)]))
+        self.powers = np.ascontiguousarray(np.append(self.powers, float(power)))
+
+    def c_struct(self):
+        return _lib.Poly(self.totalRoots(), self.roots.ctypes.data_as(_lib.c_double_p),
+                         self.powers.ctypes.data_as(_lib.c_double_p), self.power_scalar)
+
+
+class Rational:
+    """A top/bottom POLYNOMIAL_4D pair resident in HBM (qj_rational)."""
+
+    def __init__(self, top, bottom=None):
+        _lib.load()
+        self.top = top
+        self.bottom = bottom if bottom is not None else POLYNOMIAL_4D()
+        t, b = top.c_struct(), self.bottom.c_struct()
+        h = ctypes.c_void_p()
+        check(_lib.load().qj_rational_create(ctypes.byref(t), ctypes.byref(b), ctypes.byref(h)))
+        self.handle = h
+
+    def update(self, top=None, bottom=None, stream=None):
+        if top is not None:
+            self.top = top
+        if bottom is not None:
+            self.bottom = bottom
+        t, b = self.top.c_struct(), self.bottom.c_struct()
+        check(_lib.load().qj_rational_update(self.handle, ctypes.byref(t), ctypes.byref(b), _stream_ptr(stream)))
+
+    def __del__(self):
+        try:
+            if getattr(self, "handle", None):
+                _lib.load().qj_rational_destroy(self.handle)
+                self.handle = None
+        except Exception:
+            pass
+
+
+class Workspace:
+    def __init__(self):
+        h = ctypes.c_void_p()
+        check(_lib.load().qj_workspace_create(ctypes.byref(h)))
+        self.handle = h
+
+    def stats(self, stream=None):
+        st = _lib.EvalStats()
+        check(_lib.load().qj_workspace_stats(self.handle, _stream_ptr(stream), ctypes.byref(st)))
+        return {k: getattr(st, k) for k, _ in _lib.EvalStats._fields_}
+
+    def reset_stats(self, stream=None):
+        check(_lib.load().qj_workspace_reset_stats(self.handle, _stream_ptr(stream)))
+
+    def __del__(self):
+        try:
+            if getattr(self, "handle", None):
+                _lib.load().qj_workspace_destroy(self.handle)
+                self.handle = None
+        except Exception:
+            pass
+
+
+_default_ws = {}
+
+
+def default_workspace():
+    dev = torch.cuda.current_device()
+    if dev not in _default_ws:
+        _default_ws[dev] = Workspace()
+    return _default_ws[dev]
+
+
+def make_grid(res, center=(0.0, 0.0, 0.0), lengths=(3.0, 3.0, 3.0), rotation=(1.0, 0.0, 0.0, 0.0)):
+    if isinstance(res, int):
+        res = (res, res, res)
+    g = _lib.Grid()
+    g.res[:] = [int(v) for v in res]
+    g.center[:] = [float(v) for v in center]
+    g.lengths[:] = [float(v) for v in lengths]
+    g.rotation[:] = [float(v) for v in rotation]
+    return g
+
+
+def make_params(max_iter=3, escape=2000.0, exp_scaling=1.0, slice_=0.0, flags=MAP_BLACK_HOLE):
+    return _lib.MapParams(int(max_iter), float(escape), float(exp_scaling), float(slice_), int(flags))
+
+
+class FIELD_3D:
+    """Cell-centred grid with a float64 device tensor, data[z][y][x]."""
+
+    def __init__(self, xRes, yRes, zRes, center=(0.0, 0.0, 0.0), lengths=(1.0, 1.0, 1.0), data=None,
+                 rotation=(1.0, 0.0, 0.0, 0.0), device=None):
+        self.res = (int(xRes), int(yRes), int(zRes))
+        self.center = tuple(float(v) for v in center)
+        self.lengths = tuple(float(v) for v in lengths)
+        self.rotation = tuple(float(v) for v in rotation)
+        dev = device if device is not None else torch.device("cuda", torch.cuda.current_device())
+        if data is None:
+            self.data = torch.zeros((self.res[2], self.res[1], self.res[0]), dtype=torch.float64, device=dev)
+        else:
+            if data.dtype != torch.float64 or not data.is_cuda or not data.is_contiguous():
+                raise ValueError("FIELD_3D data must be a contiguous float64 CUDA tensor")
+            self.data = data.view(self.res[2], self.res[1], self.res[0])
+
+    def xRes(self):
+        return self.res[0]
+
+    def yRes(self):
+        return self.res[1]
+
+    def zRes(self):
+        return self.res[2]
+
+    def totalCells(self):
+        return self.res[0] * self.res[1] * self.res[2]
+
+    def dx(self):
+        return self.lengths[0] / self.res[0]
+
+    def grid(self):
+        return make_grid(self.res, self.center, self.lengths, self.rotation)
+
+    def numpy(self):
+        return self.data.detach().cpu().numpy().reshape(-1)
+
+
+def field_eval(grid, rational, params, precision=FP64, band=DEFAULT_BAND, out=None, workspace=None, stream=None):
+    """qj_field_eval into a float64 CUDA tensor (allocated when out is None)."""
+    n = grid.res[0] * grid.res[1] * grid.res[2]
+    if out is None:
+        out = torch.empty(n, dtype=torch.float64, device="cuda")
+    if out.numel() != n or out.dtype != torch.float64 or not out.is_contiguous():
+        raise ValueError("out must be a contiguous float64 tensor with %d elements" % n)
+    ws = workspace or default_workspace()
+    check(_lib.load().qj_field_eval(ws.handle, ctypes.byref(grid), rational.handle, ctypes.byref(params),
+                                    int(precision), float(band), _ptr(out), _stream_ptr(stream)))
+    return out
+
+
+def points_eval(points, rational, params, precision=FP64, band=DEFAULT_BAND, out=None, workspace=None, stream=None):
+    pts = points.contiguous()
+    if pts.dtype != torch.float64 or pts.dim() != 2 or pts.shape[1] != 3:
+        raise ValueError("points must be an (n, 3) float64 CUDA tensor")
+    n = pts.shape[0]
+    if out is None:
+        out = torch.empty(n, dtype=torch.float64, device=pts.device)
+    ws = workspace or default_workspace()
+    check(_lib.load().qj_points_eval(ws.handle, _ptr(pts), n, rational.handle, ctypes.byref(params),
+                                     int(precision), float(band), _ptr(out), _stream_ptr(stream)))
+    return out
+
+
+def marching_cubes(field, grid, iso=0.0, workspace=None, stream=None, device_output=False):
+    """qj_mc_run + copy: returns (vertices (nv,3) float64, triangles (nt,3) int32)."""
+    ws = workspace or default_workspace()
+    nv = ctypes.c_int64()
+    nt = ctypes.c_int64()
+    L = _lib.load()
+    check(L.qj_mc_run(ws.handle, _ptr(field), ctypes.byref(grid), float(iso), ctypes.byref(nv), ctypes.byref(nt),
+                      _stream_ptr(stream)))
+    if device_output:
+        v = torch.empty((nv.value, 3), dtype=torch.float64, device=field.device)
+        t = torch.empty((nt.value, 3), dtype=torch.int32, device=field.device)
+        check(L.qj_mc_copy(ws.handle, _ptr(v) if nv.value else None, _ptr(t) if nt.value else None,
+                           _stream_ptr(stream)))
+        return v, t
+    v = np.empty((nv.value, 3), dtype=np.float64)
+    t = np.empty((nt.value, 3), dtype=np.int32)
+    check(L.qj_mc_copy(ws.handle, v.ctypes.data_as(ctypes.c_void_p) if nv.value else None,
+                       t.ctypes.data_as(ctypes.c_void_p) if nt.value else None, _stream_ptr(stream)))
+    return v, t
+
+
+class OPTIMIZE_3D:
+    """The map-evaluation members of OPTIMIZE_3D (OPTIMIZE_3D.h:67-81, 140-169)."""
+
+    def __init__(self, top, bottom=None, maxIterations=3, expScaling=1.0, quaternionSlice=0.0):
+        self._top = top
+        self._bottom = bottom if bottom is not None else POLYNOMIAL_4D()
+        self._maxIterations = int(maxIterations)
+        self._expScaling = float(expScaling)
+        self._quaternionSlice = float(quaternionSlice)
+        self._rational = None
+
+    def _sync_rational(self, top, bottom):
+        if (self._rational is None or self._rational.top is not top or self._rational.bottom is not bottom
+                or self._rational.top.totalRoots() != top.totalRoots()
+                or self._rational.bottom.totalRoots() != bottom.totalRoots()):
+            self._rational = Rational(top, bottom)
+        else:
+            self._rational.update(top, bottom)
+        return self._rational
+
+    def computeLogScaledPowerRationalMap(self, top=None, bottom=None, fractal=None, verbose=False,
+                                         precision=FP64, band=DEFAULT_BAND):
+        """OPTIMIZE_3D.cpp:1272-1370: escape 2000, black-hole stop, field = log|q|."""
+        top = top or self._top
+        bottom = bottom or self._bottom
+        r = self._sync_rational(top, bottom)
+        p = make_params(self._maxIterations, 2000.0, self._expScaling, self._quaternionSlice, MAP_BLACK_HOLE)
+        field_eval(fractal.grid(), r, p, precision, band, out=fractal.data.view(-1))
+        return fractal
+
+    def computeRationalMapAP(self, top=None, bottom=None, fractal=None, verbose=False):
+        """OPTIMIZE_3D.cpp:1374-1568: escape 20, origin cells skipped, overflow re-pass."""
+        top = top or self._top
+        bottom = bottom or self._bottom
+        r = self._sync_rational(top, bottom)
+        p = make_params(self._maxIterations, 20.0, self._expScaling, self._quaternionSlice,
+                        MAP_SKIP_ORIGIN | MAP_ADAPTIVE)
+        field_eval(fractal.grid(), r, p, FP64, 0.0, out=fractal.data.view(-1))
+        return fractal
+
+
+class TRIANGLE_MESH:
+    """TRIANGLE_MESH(const FIELD_3D&): staged marching cubes on the device."""
+
+    def __init__(self, field, iso=0.0):
+        self._vertices, self._triangles = marching_cubes(field.data.view(-1), field.grid(), iso)
+
+    def vertices(self):
+        return self._vertices
+
+    def triangles(self):
+        return self._triangles
+
+    def writeOBJ(self, filename):
+        """TRIANGLE_MESH::writeOBJ (TRIANGLE_MESH.cpp:2762-2800): "v %.16f", 1-based "f i j k"."""
+        with open(filename, "w") as f:
+            for v in self._vertices:
+                f.write("v %.16f %.16f %.16f\n" % (v[0], v[1], v[2]))
+            for t in self._triangles:
+                f.write("f %i %i %i\n" % (t[0] + 1, t[1] + 1, t[2] + 1))
+        return True

@@ -1,0 +1,590 @@
+// Julia field evaluation kernels for gfx950 (MI355X).
+//
+// Hot path: OPTIMIZE_3D::computeLogScaledPowerRationalMap (OPTIMIZE_3D.cpp:1272-1370)
+// and its AP/cached variants. One voxel per lane; a 256-lane block covers a
+// 16 x 4 x 4 (x,y,z) tile so a wave is a 16 x 4 x-major patch (neighbouring
+// voxels escape together, stores coalesce in 128 B runs). Root records are
+// read with uniform addresses inside the root loop, so they come through the
+// scalar cache into SGPRs; no LDS is needed. There is no dense contraction
+// here, so MFMA is not used: the work is VALU (FP64 FMA and the FP64 libm
+// sequences, or FP32 FMA plus native transcendentals).
+//
+// Kernels:
+//   k_grid64      FP64, reference expression order (the "plain FP64" method).
+//   k_grid32      FP32 bulk with exact power-of-two range scaling; marks the
+//                 band |F| < band, non-finite results and near-threshold
+//                 escape / black-hole decisions into a compacted list
+//                 (wave ballot + one atomic per block).
+//   k_list64      FP64 re-evaluation of a compacted list (band or points).
+//   k_list64x     FP64 with an explicit binary exponent (extended range) for
+//                 cells that overflowed: the reference's long double re-pass.
+#include <float.h>
+
+#include "qj_geom.h"
+#include "qj_internal.h"
+#include "qj_quat.h"
+
+namespace qj {
+
+constexpr int kTileX = 16, kTileY = 4, kTileZ = 4, kBlock = 256;
+constexpr double kBlackHole = 10.0 * (double)FLT_MIN;  // 10 * REAL_MIN, SETTINGS.h:58, OPTIMIZE_3D.cpp:1358
+
+// ---------------------------------------------------------------- geometry
+
+struct GridSource {
+  __device__ __forceinline__ static q64 start(const MapArgs& a, const double*, int64_t idx) {
+    const int64_t sx = a.res[0], sxy = (int64_t)a.res[0] * a.res[1];
+    const int z = (int)(idx / sxy);
+    const int64_t r = idx - (int64_t)z * sxy;
+    const int y = (int)(r / sx);
+    const int x = (int)(r - (int64_t)y * sx);
+    double c[3];
+    cell_center(a, x, y, z, c);
+    return {c[0], c[1], c[2], a.slice};
+  }
+};
+struct PointSource {
+  __device__ __forceinline__ static q64 start(const MapArgs& a, const double* pts, int64_t idx) {
+    return {pts[3 * idx], pts[3 * idx + 1], pts[3 * idx + 2], a.slice};
+  }
+};
+
+// ---------------------------------------------------------------- wave helpers
+
+__device__ __forceinline__ unsigned long long wave_sum_u64(unsigned long long v) {
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
+  return v;
+}
+
+// Sum v over the block and add it to counter slot `slot` (spread copies).
+__device__ __forceinline__ void block_count(unsigned long long v, unsigned long long* cnt, int slot) {
+  __shared__ unsigned long long s_part[kBlock / 64];
+  v = wave_sum_u64(v);
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  if (lane == 0) s_part[wave] = v;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    unsigned long long t = 0;
+    for (int i = 0; i < kBlock / 64; i++) t += s_part[i];
+    if (t) atomicAdd(&cnt[slot * kCounterSpread + (blockIdx.x % kCounterSpread)], t);
+  }
+  __syncthreads();
+}
+
+// Order-free compaction: flagged lanes append `value` to list; one atomic per block.
+__device__ __forceinline__ void block_append(bool flag, uint32_t value, uint32_t* list, uint32_t* count) {
+  __shared__ uint32_t s_cnt[kBlock / 64];
+  __shared__ uint32_t s_base;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const unsigned long long m = __ballot(flag);
+  const uint32_t rank = (uint32_t)__popcll(m & ((1ull << lane) - 1ull));
+  if (lane == 0) s_cnt[wave] = (uint32_t)__popcll(m);
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    uint32_t t = 0;
+    for (int i = 0; i < kBlock / 64; i++) {
+      const uint32_t c = s_cnt[i];
+      s_cnt[i] = t;
+      t += c;
+    }
+    s_base = t ? atomicAdd(count, t) : 0u;
+  }
+  __syncthreads();
+  if (flag) list[s_base + s_cnt[wave] + rank] = value;
+}
+
+// ---------------------------------------------------------------- FP64 method
+
+// POLYNOMIAL_4D::evaluateScaledPowerFactored (POLYNOMIAL_4D.cpp:912-923)
+__device__ __forceinline__ q64 poly64(const RootRec64* __restrict__ r, int n, const q64& q) {
+  q64 res = qpow(qsub(q, q64{r[0].w, r[0].x, r[0].y, r[0].z}), r[0].e);
+  for (int i = 1; i < n; i++) res = qmul(res, qpow(qsub(q, q64{r[i].w, r[i].x, r[i].y, r[i].z}), r[i].e));
+  return res;
+}
+
+__device__ __forceinline__ bool finite4(const q64& q) {
+  return isfinite(q.w) && isfinite(q.x) && isfinite(q.y) && isfinite(q.z);
+}
+
+struct Res64 {
+  double F;
+  int iters;
+  bool written;
+  bool overflow;
+};
+
+// One voxel of computeLogScaledPowerRationalMap (OPTIMIZE_3D.cpp:1317-1362) /
+// computeRationalMapAP's first pass (:1423-1475). Overflow detection replaces
+// fetestexcept(FE_OVERFLOW): any non-finite T, B, |B|^2, quotient or |q|.
+__device__ __forceinline__ Res64 eval64(const MapArgs& a, const RootRec64* __restrict__ roots, q64 q) {
+  Res64 r{0.0, 0, true, false};
+  double magnitude = qmag(q);
+  if ((a.flags & QJ_MAP_SKIP_ORIGIN) && magnitude < 1e-8) {
+    r.written = false;
+    return r;
+  }
+  const bool adaptive = (a.flags & QJ_MAP_ADAPTIVE) != 0;
+  int it = 0;
+  while (magnitude < a.escape && it < a.max_iter) {
+    const q64 T = poly64(roots, a.nt, q);
+    if (a.nb > 0) {
+      const q64 B = poly64(roots + a.nt, a.nb, q);
+      q = qdiv(T, B);
+      if (adaptive) {
+        const double m2 = B.w * B.w + B.x * B.x + B.y * B.y + B.z * B.z;
+        r.overflow |= !(finite4(T) && finite4(B) && isfinite(m2) && finite4(q));
+      }
+    } else {
+      q = T;
+      if (adaptive) r.overflow |= !finite4(T);
+    }
+    q = qscale(q, a.exp_scaling);
+    magnitude = qmag(q);
+    if (adaptive && !isfinite(magnitude)) r.overflow = true;
+    it++;
+    r.iters++;
+    if ((a.flags & QJ_MAP_BLACK_HOLE) && magnitude < kBlackHole) it = a.max_iter;
+  }
+  r.F = log(magnitude);
+  return r;
+}
+
+// ---------------------------------------------------------------- FP64 extended range
+
+// A quaternion times 2^e. Normalised so the largest |component| is in [0.5, 1).
+struct xq {
+  q64 m;
+  int e;
+};
+__device__ __forceinline__ void xnorm(xq& v) {
+  const double mx = fmax(fmax(fabs(v.m.w), fabs(v.m.x)), fmax(fabs(v.m.y), fabs(v.m.z)));
+  if (mx > 0.0 && isfinite(mx)) {
+    int k;
+    (void)frexp(mx, &k);
+    v.m.w = ldexp(v.m.w, -k);
+    v.m.x = ldexp(v.m.x, -k);
+    v.m.y = ldexp(v.m.y, -k);
+    v.m.z = ldexp(v.m.z, -k);
+    v.e += k;
+  }
+}
+// QUATERNION::pow with exps = exp(e log|q|) kept as mantissa * 2^k.
+__device__ __forceinline__ xq xpow(const q64& q, double e) {
+  const double partial = q.x * q.x + q.y * q.y + q.z * q.z;
+  const double qMagnitude = sqrt(partial + q.w * q.w);
+  const double vMagnitude = sqrt(partial);
+  const double vMagnitudeInv = (vMagnitude > 0.0) ? 1.0 / vMagnitude : 0.0;
+  const double scale = e * acos(q.w / qMagnitude) * vMagnitudeInv;
+  const double magnitude = scale * vMagnitude;
+  const double magnitudeInv = (magnitude > 0.0) ? 1.0 / magnitude : 0.0;
+  const double lnE = e * log(qMagnitude);
+  int k = 0;
+  double em;
+  if (isfinite(lnE)) {
+    k = (int)floor(lnE * 1.4426950408889634);
+    em = exp(lnE - k * 0.6931471805599453);
+  } else {
+    em = exp(lnE);
+  }
+  double s, c;
+  sincos(magnitude, &s, &c);
+  const double scale2 = scale * em * magnitudeInv * s;
+  xq r{{em * c, scale2 * q.x, scale2 * q.y, scale2 * q.z}, k};
+  xnorm(r);
+  return r;
+}
+__device__ __forceinline__ xq xpoly(const RootRec64* __restrict__ r, int n, const q64& q) {
+  xq res = xpow(qsub(q, q64{r[0].w, r[0].x, r[0].y, r[0].z}), r[0].e);
+  for (int i = 1; i < n; i++) {
+    const xq t = xpow(qsub(q, q64{r[i].w, r[i].x, r[i].y, r[i].z}), r[i].e);
+    res.m = qmul(res.m, t.m);
+    res.e += t.e;
+    xnorm(res);
+  }
+  return res;
+}
+// computeRationalMapAP's long double re-pass (OPTIMIZE_3D.cpp:1496-1562) with
+// FP64 mantissas and an integer binary exponent in place of x87 range.
+__device__ __forceinline__ Res64 eval64x(const MapArgs& a, const RootRec64* __restrict__ roots, q64 q) {
+  Res64 r{0.0, 0, true, false};
+  double magnitude = qmag(q);
+  if ((a.flags & QJ_MAP_SKIP_ORIGIN) && magnitude < 1e-8) {
+    r.written = false;
+    return r;
+  }
+  double logmag = log(magnitude);
+  int it = 0;
+  while (magnitude < a.escape && it < a.max_iter) {
+    xq T = xpoly(roots, a.nt, q);
+    xq v;
+    if (a.nb > 0) {
+      const xq B = xpoly(roots + a.nt, a.nb, q);
+      v.m = qmul(T.m, qinv(B.m));
+      v.e = T.e - B.e;
+    } else {
+      v = T;
+    }
+    v.m = qscale(v.m, a.exp_scaling);
+    xnorm(v);
+    const double mm = qmag(v.m);
+    logmag = log(mm) + v.e * 0.6931471805599453;
+    magnitude = ldexp(mm, v.e);
+    q = q64{ldexp(v.m.w, v.e), ldexp(v.m.x, v.e), ldexp(v.m.y, v.e), ldexp(v.m.z, v.e)};
+    it++;
+    r.iters++;
+    if ((a.flags & QJ_MAP_BLACK_HOLE) && magnitude < kBlackHole) it = a.max_iter;
+  }
+  r.F = isnan(logmag) ? 0.0 : logmag;  // NaN -> 0, OPTIMIZE_3D.cpp:1557-1561
+  return r;
+}
+
+// ---------------------------------------------------------------- FP32 bulk
+
+struct Consts32 {
+  float log2c;     // log2(exp_scaling)
+  float log2esc;   // log2(escape)
+  float log2bh;    // log2(10 FLT_MIN)
+  float margin;    // decision margin in log2 units
+  float band;
+};
+
+// Largest-component power-of-two renormalisation: exact, no rounding.
+__device__ __forceinline__ void renorm32(q32& t, int& E) {
+  const float mx = fmaxf(fmaxf(fabsf(t.w), fabsf(t.x)), fmaxf(fabsf(t.y), fabsf(t.z)));
+  const int k = __builtin_amdgcn_frexp_expf(mx);  // 0 for 0 / inf / nan
+  t.w = __builtin_amdgcn_ldexpf(t.w, -k);
+  t.x = __builtin_amdgcn_ldexpf(t.x, -k);
+  t.y = __builtin_amdgcn_ldexpf(t.y, -k);
+  t.z = __builtin_amdgcn_ldexpf(t.z, -k);
+  E += k;
+}
+
+// (q - r)^e in FP32. Integer kinds are exact products (b, b*b); the general
+// kind follows QUATERNION::pow's polar form with the magnitude |b|^e moved
+// into the log2 accumulator L, keeping the quaternion part unit length.
+// The reference's guards carry over: v = 0 gives a positive real result and a
+// non-positive angle drops the imaginary part.
+__device__ __forceinline__ q32 term32(const RootRec32& r, const q32& q, float& L) {
+  const q32 b = qsub(q, q32{r.w, r.x, r.y, r.z});
+  const float partial = b.x * b.x + b.y * b.y + b.z * b.z;
+  if (r.kind == 1) {
+    return q32{partial > 0.0f ? b.w : fabsf(b.w), b.x, b.y, b.z};
+  } else if (r.kind == 2) {
+    const float t = 2.0f * b.w;
+    return q32{b.w * b.w - partial, t * b.x, t * b.y, t * b.z};
+  } else {
+    const float n2 = partial + b.w * b.w;
+    const float vm = __builtin_sqrtf(partial);
+    const float th = atan2f(vm, b.w);
+    const float m = vm > 0.0f ? r.e * th : 0.0f;  // scale = e*acos(w/|q|)*0 when v = 0
+    L += r.e * 0.5f * __log2f(n2);
+    float s, c;
+    __sincosf(m, &s, &c);
+    const float f = (m > 0.0f && vm > 0.0f) ? s / vm : 0.0f;
+    return q32{c, f * b.x, f * b.y, f * b.z};
+  }
+}
+
+// Ordered product over n roots with exponent tracking (renormalise every 2
+// terms). `range` is set when the running product leaves +-lim in log2, i.e.
+// where the FP64 method would overflow or underflow and so give a different
+// (non-finite or zero) result: such voxels go to FP64.
+__device__ __forceinline__ q32 poly32(const RootRec32* __restrict__ r, int n, const q32& q, int& E, float& L,
+                                      float lim, bool& range) {
+  q32 t = term32(r[0], q, L);
+  int i = 1;
+  for (; i + 1 < n; i += 2) {
+    t = qmul(t, term32(r[i], q, L));
+    t = qmul(t, term32(r[i + 1], q, L));
+    renorm32(t, E);
+    range |= fabsf((float)E + L) > lim;
+  }
+  if (i < n) t = qmul(t, term32(r[i], q, L));
+  renorm32(t, E);
+  range |= fabsf((float)E + L) > lim;
+  return t;
+}
+
+struct Res32 {
+  float F;
+  int iters;
+  bool written;
+  bool need64;
+};
+
+__device__ __forceinline__ Res32 eval32(const MapArgs& a, const RootRec32* __restrict__ roots, const Consts32& k,
+                                        const q64& q0) {
+  Res32 res{0.0f, 0, true, false};
+  // the first escape / origin test in FP64, exactly as the reference
+  const double mag0 = qmag(q0);
+  if ((a.flags & QJ_MAP_SKIP_ORIGIN) && mag0 < 1e-8) {
+    res.written = false;
+    return res;
+  }
+  if (!(mag0 < a.escape) || a.max_iter <= 0) {
+    res.F = (float)log(mag0);
+    res.need64 = true;  // trivial case: let FP64 write it exactly
+    return res;
+  }
+  q32 q{(float)q0.w, (float)q0.x, (float)q0.y, (float)q0.z};
+  float log2mag = 0.0f;
+  bool amb = false, bad = false;
+  int it = 0;
+  while (true) {
+    int ET = 0, EB = 0;
+    float L = 0.0f;
+    const q32 T = poly32(roots, a.nt, q, ET, L, 1000.0f, bad);
+    q32 D;
+    float l2 = k.log2c;
+    if (a.nb > 0) {
+      float LB = 0.0f;
+      const q32 B = poly32(roots + a.nt, a.nb, q, EB, LB, 500.0f, bad);
+      L -= LB;
+      D = qmul(T, q32{B.w, -B.x, -B.y, -B.z});  // T conj(B) = T B^-1 |B|^2
+      const float nb2 = B.w * B.w + B.x * B.x + B.y * B.y + B.z * B.z;
+      const float nd2 = D.w * D.w + D.x * D.x + D.y * D.y + D.z * D.z;
+      l2 += 0.5f * __log2f(nd2) - __log2f(nb2) + (float)(ET - EB) + L;
+      bad |= !(nb2 > 0.0f) || !(nd2 > 0.0f);
+      if (!bad) {
+        const float sc = exp2f(k.log2c + (float)(ET - EB) + L) / nb2;
+        q = qscale(D, sc);
+      }
+    } else {
+      const float nt2 = T.w * T.w + T.x * T.x + T.y * T.y + T.z * T.z;
+      l2 += 0.5f * __log2f(nt2) + (float)ET + L;
+      bad |= !(nt2 > 0.0f);
+      if (!bad) q = qscale(T, exp2f(k.log2c + (float)ET + L));
+    }
+    log2mag = l2;
+    it++;
+    res.iters++;
+    bad |= !isfinite(l2);
+    amb |= fabsf(l2 - k.log2esc) < k.margin;
+    const bool bh = (a.flags & QJ_MAP_BLACK_HOLE) && l2 < k.log2bh;
+    if (a.flags & QJ_MAP_BLACK_HOLE) amb |= fabsf(l2 - k.log2bh) < k.margin;
+    if (bad || l2 >= k.log2esc || bh || it >= a.max_iter) break;
+  }
+  res.F = log2mag * 0.69314718f;
+  res.need64 = bad || amb || !(fabsf(res.F) >= k.band);
+  return res;
+}
+
+// ---------------------------------------------------------------- kernels
+
+__device__ __forceinline__ bool tile_voxel(const MapArgs& a, int& x, int& y, int& z) {
+  const int ntx = (a.res[0] + kTileX - 1) / kTileX, nty = (a.res[1] + kTileY - 1) / kTileY;
+  const int b = blockIdx.x;
+  const int tx = b % ntx, ty = (b / ntx) % nty, tz = b / (ntx * nty);
+  const int t = threadIdx.x;
+  x = tx * kTileX + (t & (kTileX - 1));
+  y = ty * kTileY + ((t >> 4) & (kTileY - 1));
+  z = tz * kTileZ + (t >> 6);
+  return x < a.res[0] && y < a.res[1] && z < a.res[2];
+}
+
+__global__ __launch_bounds__(kBlock) void k_grid64(MapArgs a, const RootRec64* __restrict__ roots,
+                                                   double* __restrict__ out, unsigned long long* __restrict__ cnt,
+                                                   uint32_t* __restrict__ alist, uint32_t* __restrict__ acount) {
+  int x, y, z;
+  const bool valid = tile_voxel(a, x, y, z);
+  int iters = 0;
+  bool ovf = false, nonfinite = false;
+  const int64_t idx = x + (int64_t)y * a.res[0] + (int64_t)z * a.res[0] * a.res[1];
+  if (valid) {
+    double c[3];
+    cell_center(a, x, y, z, c);
+    const Res64 r = eval64(a, roots, q64{c[0], c[1], c[2], a.slice});
+    if (r.written) {
+      out[idx] = r.F;
+      nonfinite = !isfinite(r.F);
+    }
+    iters = r.iters;
+    ovf = r.overflow;
+  }
+  block_count(valid ? 1 : 0, cnt, CNT_VOXELS);
+  block_count(iters, cnt, CNT_ITERS);
+  block_count(nonfinite ? 1 : 0, cnt, CNT_NONFINITE);
+  if (a.flags & QJ_MAP_ADAPTIVE) {
+    block_append(ovf, (uint32_t)idx, alist, acount);
+    block_count(ovf ? 1 : 0, cnt, CNT_ADAPTIVE);
+  }
+}
+
+__global__ __launch_bounds__(kBlock) void k_grid32(MapArgs a, Consts32 k, const RootRec32* __restrict__ roots,
+                                                   double* __restrict__ out, unsigned long long* __restrict__ cnt,
+                                                   uint32_t* __restrict__ blist, uint32_t* __restrict__ bcount) {
+  int x, y, z;
+  const bool valid = tile_voxel(a, x, y, z);
+  int iters = 0;
+  bool need = false;
+  const int64_t idx = x + (int64_t)y * a.res[0] + (int64_t)z * a.res[0] * a.res[1];
+  if (valid) {
+    double c[3];
+    cell_center(a, x, y, z, c);
+    const Res32 r = eval32(a, roots, k, q64{c[0], c[1], c[2], a.slice});
+    if (r.written) {
+      if (!r.need64) out[idx] = (double)r.F;
+      need = r.need64;
+    }
+    iters = r.iters;
+  }
+  block_count(valid ? 1 : 0, cnt, CNT_VOXELS);
+  block_count(iters, cnt, CNT_ITERS);
+  block_append(need, (uint32_t)idx, blist, bcount);
+}
+
+template <class Src>
+__global__ __launch_bounds__(kBlock) void k_points32(MapArgs a, Consts32 k, const RootRec32* __restrict__ roots,
+                                                     const double* __restrict__ pts, int64_t n,
+                                                     double* __restrict__ out, unsigned long long* __restrict__ cnt,
+                                                     uint32_t* __restrict__ blist, uint32_t* __restrict__ bcount) {
+  const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  const bool valid = i < n;
+  int iters = 0;
+  bool need = false;
+  if (valid) {
+    const Res32 r = eval32(a, roots, k, Src::start(a, pts, i));
+    if (r.written) {
+      if (!r.need64) out[i] = (double)r.F;
+      need = r.need64;
+    } else {
+      out[i] = __builtin_nan("");
+    }
+    iters = r.iters;
+  }
+  block_count(valid ? 1 : 0, cnt, CNT_VOXELS);
+  block_count(iters, cnt, CNT_ITERS);
+  block_append(need, (uint32_t)i, blist, bcount);
+}
+
+// FP64 over a compacted list (count read on the device), or over all n
+// indices when list == nullptr.
+template <class Src, bool kExtended>
+__global__ __launch_bounds__(kBlock) void k_list64(MapArgs a, const RootRec64* __restrict__ roots,
+                                                   const double* __restrict__ pts,
+                                                   const uint32_t* __restrict__ list,
+                                                   const uint32_t* __restrict__ count, int64_t n_all,
+                                                   double* __restrict__ out, unsigned long long* __restrict__ cnt,
+                                                   int slot_vox, int slot_iter, uint32_t* __restrict__ alist,
+                                                   uint32_t* __restrict__ acount) {
+  const int64_t n = list ? (int64_t)*count : n_all;
+  const int64_t stride = (int64_t)gridDim.x * kBlock;
+  unsigned long long vox = 0, iters = 0;
+  for (int64_t base = (int64_t)blockIdx.x * kBlock; base < n; base += stride) {
+    const int64_t i = base + threadIdx.x;
+    bool ovf = false;
+    uint32_t idx = 0;
+    if (i < n) {
+      idx = list ? list[i] : (uint32_t)i;
+      const q64 q0 = Src::start(a, pts, idx);
+      const Res64 r = kExtended ? eval64x(a, roots, q0) : eval64(a, roots, q0);
+      if (r.written) out[idx] = r.F;
+      else if (pts) out[idx] = __builtin_nan("");
+      vox++;
+      iters += r.iters;
+      ovf = r.overflow;
+    }
+    if (!kExtended && (a.flags & QJ_MAP_ADAPTIVE)) block_append(ovf, idx, alist, acount);
+  }
+  block_count(vox, cnt, slot_vox);
+  block_count(iters, cnt, slot_iter);
+}
+
+// ---------------------------------------------------------------- launchers
+
+static Consts32 consts32(const MapArgs& a, double band) {
+  Consts32 k;
+  k.log2c = (float)log2(a.exp_scaling);
+  k.log2esc = (float)log2(a.escape);
+  k.log2bh = (float)log2(kBlackHole);
+  k.margin = 0.02f;
+  k.band = (float)band;
+  return k;
+}
+
+static int list_grid(int64_t n_hint) {
+  const int64_t b = (n_hint + kBlock - 1) / kBlock;
+  return (int)(b < 1 ? 1 : (b > 8192 ? 8192 : b));
+}
+
+static qj_status run_adaptive(qj_workspace* ws, const MapArgs& a, const qj_rational* f, const double* pts,
+                              int64_t n, double* d_out, hipStream_t s) {
+  if (!(a.flags & QJ_MAP_ADAPTIVE)) return QJ_OK;
+  uint32_t* alist = ws->d_list + ws->list_cap;
+  if (pts)
+    hipLaunchKernelGGL((k_list64<PointSource, true>), dim3(list_grid(n / 64 + 1)), dim3(kBlock), 0, s, a, f->d64,
+                       pts, alist, ws->d_list_count + 1, (int64_t)0, d_out, ws->d_counters, (int)CNT_BAND_VOXELS,
+                       (int)CNT_BAND_ITERS, (uint32_t*)nullptr, (uint32_t*)nullptr);
+  else
+    hipLaunchKernelGGL((k_list64<GridSource, true>), dim3(list_grid(n / 64 + 1)), dim3(kBlock), 0, s, a, f->d64,
+                       pts, alist, ws->d_list_count + 1, (int64_t)0, d_out, ws->d_counters, (int)CNT_BAND_VOXELS,
+                       (int)CNT_BAND_ITERS, (uint32_t*)nullptr, (uint32_t*)nullptr);
+  QJ_HIP(hipGetLastError());
+  return QJ_OK;
+}
+
+qj_status launch_field(qj_workspace* ws, const MapArgs& a, const qj_rational* f, qj_precision prec, double band,
+                       double* d_out, hipStream_t s) {
+  const int64_t nvox = (int64_t)a.res[0] * a.res[1] * a.res[2];
+  if (nvox <= 0) return QJ_OK;
+  if (nvox > (int64_t)UINT32_MAX) {
+    set_error("grid of %lld cells exceeds 2^32; evaluate it in tiles", (long long)nvox);
+    return QJ_ERR_CAPACITY;
+  }
+  qj_status st = ensure_list(ws, nvox, s);
+  if (st) return st;
+  QJ_HIP(hipMemsetAsync(ws->d_list_count, 0, 2 * sizeof(uint32_t), s));
+  const int64_t ntx = (a.res[0] + kTileX - 1) / kTileX, nty = (a.res[1] + kTileY - 1) / kTileY,
+                ntz = (a.res[2] + kTileZ - 1) / kTileZ;
+  const int64_t nblocks = ntx * nty * ntz;
+  if (nblocks > INT32_MAX) return QJ_ERR_CAPACITY;
+  uint32_t* alist = ws->d_list + ws->list_cap;
+  timing_mark(ws, s);
+  if (prec == QJ_FP64) {
+    hipLaunchKernelGGL(k_grid64, dim3((unsigned)nblocks), dim3(kBlock), 0, s, a, f->d64, d_out, ws->d_counters,
+                       alist, ws->d_list_count + 1);
+    QJ_HIP(hipGetLastError());
+    timing_mark(ws, s);
+    timing_mark(ws, s);
+  } else {
+    hipLaunchKernelGGL(k_grid32, dim3((unsigned)nblocks), dim3(kBlock), 0, s, a, consts32(a, band), f->d32, d_out,
+                       ws->d_counters, ws->d_list, ws->d_list_count);
+    QJ_HIP(hipGetLastError());
+    timing_mark(ws, s);
+    timing_mark(ws, s);
+    hipLaunchKernelGGL((k_list64<GridSource, false>), dim3(list_grid(nvox / 16)), dim3(kBlock), 0, s, a, f->d64,
+                       (const double*)nullptr, ws->d_list, ws->d_list_count, (int64_t)0, d_out, ws->d_counters,
+                       (int)CNT_BAND_VOXELS, (int)CNT_BAND_ITERS, alist, ws->d_list_count + 1);
+    QJ_HIP(hipGetLastError());
+  }
+  timing_mark(ws, s);
+  return run_adaptive(ws, a, f, nullptr, nvox, d_out, s);
+}
+
+qj_status launch_points(qj_workspace* ws, const MapArgs& a, const qj_rational* f, qj_precision prec, double band,
+                        const double* d_pts, int64_t n, double* d_out, hipStream_t s) {
+  if (n <= 0) return QJ_OK;
+  if (n > (int64_t)UINT32_MAX) return QJ_ERR_CAPACITY;
+  qj_status st = ensure_list(ws, n, s);
+  if (st) return st;
+  QJ_HIP(hipMemsetAsync(ws->d_list_count, 0, 2 * sizeof(uint32_t), s));
+  uint32_t* alist = ws->d_list + ws->list_cap;
+  if (prec == QJ_FP64) {
+    hipLaunchKernelGGL((k_list64<PointSource, false>), dim3(list_grid(n)), dim3(kBlock), 0, s, a, f->d64, d_pts,
+                       (const uint32_t*)nullptr, (const uint32_t*)nullptr, n, d_out, ws->d_counters,
+                       (int)CNT_VOXELS, (int)CNT_ITERS, alist, ws->d_list_count + 1);
+    QJ_HIP(hipGetLastError());
+  } else {
+    const int64_t nb = (n + kBlock - 1) / kBlock;
+    hipLaunchKernelGGL(k_points32<PointSource>, dim3((unsigned)nb), dim3(kBlock), 0, s, a, consts32(a, band),
+                       f->d32, d_pts, n, d_out, ws->d_counters, ws->d_list, ws->d_list_count);
+    QJ_HIP(hipGetLastError());
+    hipLaunchKernelGGL((k_list64<PointSource, false>), dim3(list_grid(n / 16)), dim3(kBlock), 0, s, a, f->d64,
+                       d_pts, ws->d_list, ws->d_list_count, (int64_t)0, d_out, ws->d_counters,
+                       (int)CNT_BAND_VOXELS, (int)CNT_BAND_ITERS, alist, ws->d_list_count + 1);
+    QJ_HIP(hipGetLastError());
+  }
+  return run_adaptive(ws, a, f, d_pts, n, d_out, s);
+}
+
+}  // namespace qj

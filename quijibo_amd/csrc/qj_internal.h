@@ -1,0 +1,139 @@
+// Internal declarations shared by the host layer and the kernels.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "../../include/qj_c_api.h"
+
+namespace qj {
+
+// Per-root records in HBM. Top roots first, then bottom roots.
+struct RootRec64 {
+  double w, x, y, z;
+  double e;   // power_scalar * powers[i], rounded as the reference does (POLYNOMIAL_4D.cpp:914,919)
+  double p;   // powers[i]
+};
+struct RootRec32 {
+  float w, x, y, z;
+  float e;
+  int32_t kind;  // 1: e == 1, 2: e == 2 (exact integer powers), 0: general power
+  float pad0, pad1;
+};
+
+// Geometry and map constants passed by value to every kernel.
+struct MapArgs {
+  int32_t res[3];
+  int32_t max_iter;
+  double lo[3];      // center - 0.5 * lengths (FIELD_3D.cpp:822-825)
+  double dx[3];      // lengths / res (FIELD_3D.cpp:50-52)
+  double center[3];
+  double rot[9];     // QUATERNION::toRotationMatrix of the grid rotation (QUATERNION.cpp:395-412)
+  double slice;
+  double exp_scaling;
+  double escape;
+  uint32_t flags;
+  int32_t nt, nb;
+  double sc_top, sc_bottom;  // power scalars
+};
+
+// FIELD_3D geometry alone (marching cubes, weight-field lookups).
+struct GridGeom {
+  int32_t res[3];
+  double lo[3];
+  double dx[3];
+  double center[3];
+  double rot[9];
+  double lengths[3];
+};
+
+// Counter slots (summed over QJ_NCOUNTER_SPREAD copies to spread atomics).
+enum {
+  CNT_VOXELS = 0,
+  CNT_ITERS = 1,
+  CNT_BAND_VOXELS = 2,
+  CNT_BAND_ITERS = 3,
+  CNT_ADAPTIVE = 4,
+  CNT_NONFINITE = 5,
+  CNT_N = 8
+};
+constexpr int kCounterSpread = 64;
+
+}  // namespace qj
+
+struct qj_rational {
+  int32_t nt = 0, nb = 0;
+  double sc_top = 1, sc_bottom = 1;
+  qj::RootRec64* d64 = nullptr;
+  qj::RootRec32* d32 = nullptr;
+  std::vector<qj::RootRec64> h64;
+  std::vector<qj::RootRec32> h32;
+  int device = 0;
+};
+
+struct qj_workspace {
+  int device = 0;
+  unsigned long long* d_counters = nullptr;  // CNT_N x kCounterSpread
+  uint32_t* d_list = nullptr;                // band / adaptive voxel list
+  int64_t list_cap = 0;
+  uint32_t* d_list_count = nullptr;          // [0] band count, [1] adaptive count
+  // marching cubes
+  void* mc = nullptr;
+  // gradient partials
+  double* d_partials = nullptr;
+  int64_t partials_cap = 0;
+  // kernel timing (qj_workspace_timing)
+  bool timing = false;
+  std::vector<hipEvent_t> ev;  // pairs: bulk start/stop, band start/stop per launch
+  std::mutex mu;
+};
+
+namespace qj {
+
+void set_error(const char* fmt, ...);
+qj_status hip_fail(hipError_t e, const char* what);
+#define QJ_HIP(call)                                 \
+  do {                                               \
+    hipError_t _e = (call);                          \
+    if (_e != hipSuccess) return qj::hip_fail(_e, #call); \
+  } while (0)
+
+void rotation_matrix(const double* q, double* m);
+void make_geom(const qj_grid* g, GridGeom* out);
+// Fill MapArgs from the C structs (host arithmetic mirrors FIELD_3D).
+qj_status make_map_args(const qj_grid* g, const qj_rational* f, const qj_map_params* p, MapArgs* a);
+qj_status ensure_list(qj_workspace* ws, int64_t n, hipStream_t s);
+qj_status ensure_partials(qj_workspace* ws, int64_t n);
+// Records an event on s when timing is enabled (slot order: bulk0, bulk1, band0, band1).
+void timing_mark(qj_workspace* ws, hipStream_t s);
+
+// kernels (qj_field.hip)
+qj_status launch_field(qj_workspace* ws, const MapArgs& a, const qj_rational* f, qj_precision prec, double band,
+                       double* d_out, hipStream_t s);
+qj_status launch_points(qj_workspace* ws, const MapArgs& a, const qj_rational* f, qj_precision prec,
+                        double band, const double* d_pts, int64_t n, double* d_out, hipStream_t s);
+
+// marching cubes (qj_mc.hip)
+qj_status mc_run(qj_workspace* ws, const double* d_field, const qj_grid* g, double iso, int64_t* nv,
+                 int64_t* nt, hipStream_t s);
+qj_status mc_copy(qj_workspace* ws, double* v, int32_t* t, hipStream_t s);
+qj_status mc_ptrs(qj_workspace* ws, const double** v, const int32_t** t);
+void mc_free(qj_workspace* ws);
+
+// gradients (qj_grad.hip)
+qj_status ap_grads(qj_workspace* ws, const MapArgs& a, const qj_rational* f, const qj_grid* wg,
+                   const double* d_weight, double alpha, double thr, double* d_field, double* gt, double* gb,
+                   qj_ap_grads* out, hipStream_t s);
+qj_status points_adjoint(qj_workspace* ws, const MapArgs& a, const qj_rational* f, qj_precision prec,
+                         const double* d_pts, int64_t n, const double* d_dJdF, double* d_gr, double* d_gp,
+                         hipStream_t s);
+
+// exclusive scan of uint32 (qj_scan.hip); n elements, temp provided by ws
+qj_status scan_u32(const uint32_t* d_in, uint32_t* d_out, int64_t n, uint32_t* d_total, void* d_tmp,
+                   size_t tmp_bytes, hipStream_t s);
+size_t scan_tmp_bytes(int64_t n);
+
+}  // namespace qj

@@ -1,0 +1,403 @@
+// Marching cubes on a resident field (TRIANGLE_MESH(const FIELD_3D&) ->
+// computeStagedMarchingCubes, TRIANGLE_MESH.cpp:989-1114), bit-exact with the
+// CPU mesher: same flags, same vertex set and numbering, same vertex
+// arithmetic, same triangle order, same degenerate filter.
+//
+// Pipeline (all HBM-bound; one lane per grid point / cell):
+//   k_mc_flags      cell case from the float sign test, per-block active count (wave ballot)
+//   scan            block counts -> block offsets (order preserving)
+//   k_mc_active     ballot-compacted list of active cells, in cell order
+//   k_mc_edges      per grid point: which of its +x,+y,+z edges some adjacent
+//                   cell's case row references (gather of <= 7 neighbour flags)
+//   scan            edge counts -> vertex ids (the reference numbers vertices by
+//                   std::map<pair<int,int>> order = lower endpoint, then x<y<z)
+//   k_mc_vertices   p0 + d0/(d0-d1) (p1-p0) at cell centres, no contraction
+//   k_mc_tricount / scan / k_mc_triangles / scan / k_mc_squeeze: triangles in
+//                   cell then table order, degenerate ones (edge < 1e-7) dropped
+#include <math.h>
+
+#include "qj_geom.h"
+#include "qj_internal.h"
+#include "qj_mc_table.h"
+
+namespace qj {
+
+constexpr int kMcBlock = 256;
+
+struct McTables {
+  signed char tri[256][16];
+  unsigned char ntri[256];
+  unsigned short emask[256];
+};
+
+constexpr McTables make_tables() {
+  McTables t{QJ_MC_TRI_INIT, {}, {}};
+  for (int c = 0; c < 256; c++) {
+    int n = 0;
+    unsigned m = 0;
+    for (int k = 0; k < 16 && t.tri[c][k] >= 0; k++) {
+      n++;
+      m |= 1u << t.tri[c][k];
+    }
+    t.ntri[c] = (unsigned char)(n / 3);
+    t.emask[c] = (unsigned short)m;
+  }
+  return t;
+}
+
+__constant__ McTables kMc = make_tables();
+
+// edge id -> lower endpoint offset (dx,dy,dz) and axis
+__device__ __forceinline__ void edge_geom(int e, int& dx, int& dy, int& dz, int& axis) {
+  if (e < 4) {
+    axis = 0; dx = 0; dy = e & 1; dz = e >> 1;
+  } else if (e < 8) {
+    axis = 1; dx = e & 1; dy = 0; dz = (e >> 1) & 1;
+  } else {
+    axis = 2; dx = e & 1; dy = (e >> 1) & 1; dz = 0;
+  }
+}
+
+struct McState {
+  int64_t cap_flags = 0, cap_emask = 0, cap_cnt = 0, cap_active = 0, cap_toff = 0, cap_blk = 0;
+  int64_t cap_v = 0, cap_tp = 0, cap_keep = 0, cap_t = 0;
+  uint8_t* flags = nullptr;
+  uint8_t* emask = nullptr;
+  uint32_t* cnt = nullptr;     // per point edge count -> vertex base (scanned in place)
+  uint32_t* blk = nullptr;     // per block active count -> offset
+  uint32_t* active = nullptr;  // active cells
+  uint32_t* tri_off = nullptr; // per active cell triangle count -> offset
+  double* verts = nullptr;
+  int32_t* tris_pre = nullptr;
+  uint32_t* keep = nullptr;    // per pre-filter triangle keep -> offset
+  int32_t* tris = nullptr;
+  void* tmp = nullptr;
+  size_t tmp_bytes = 0;
+  uint32_t* totals = nullptr;  // [0] active [1] vertices [2] triangles pre-filter [3] triangles
+  int64_t nv = 0, nt = 0;
+};
+
+__device__ __forceinline__ int cell_flag(const double* __restrict__ f, int64_t i, int64_t X, int64_t XY, double iso) {
+  // CUBE corners are floats (TRIANGLE_MESH.h:283-288); test at TRIANGLE_MESH.cpp:1037-1039
+  int flag = 0;
+#pragma unroll
+  for (int k = 0; k < 8; k++) {
+    const int dx = k >> 2, dy = (k >> 1) & 1, dz = k & 1;
+    const float v = (float)(f[i + dx + dy * X + dz * XY] - iso);
+    flag |= (v > 0.0f) << k;
+  }
+  return flag;
+}
+
+__global__ __launch_bounds__(kMcBlock) void k_mc_flags(const double* __restrict__ field, GridGeom g, double iso,
+                                                       uint8_t* __restrict__ flags, uint32_t* __restrict__ blk) {
+  __shared__ uint32_t s_n[kMcBlock / 64];
+  const int64_t X = g.res[0], Y = g.res[1], Z = g.res[2], XY = X * Y, N = XY * Z;
+  const int64_t p = (int64_t)blockIdx.x * kMcBlock + threadIdx.x;
+  bool act = false;
+  if (p < N) {
+    const int64_t z = p / XY, r = p - z * XY, y = r / X, x = r - y * X;
+    int flag = 0;
+    if (x < X - 1 && y < Y - 1 && z < Z - 1) flag = cell_flag(field, p, X, XY, iso);
+    flags[p] = (uint8_t)flag;
+    act = kMc.ntri[flag] > 0;
+  }
+  const unsigned long long m = __ballot(act);
+  if ((threadIdx.x & 63) == 0) s_n[threadIdx.x >> 6] = (uint32_t)__popcll(m);
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    uint32_t t = 0;
+    for (int i = 0; i < kMcBlock / 64; i++) t += s_n[i];
+    blk[blockIdx.x] = t;
+  }
+}
+
+__global__ __launch_bounds__(kMcBlock) void k_mc_active(const uint8_t* __restrict__ flags, int64_t N,
+                                                        const uint32_t* __restrict__ blk_off,
+                                                        uint32_t* __restrict__ active) {
+  __shared__ uint32_t s_n[kMcBlock / 64];
+  const int64_t p = (int64_t)blockIdx.x * kMcBlock + threadIdx.x;
+  const bool act = p < N && kMc.ntri[flags[p]] > 0;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const unsigned long long m = __ballot(act);
+  if (lane == 0) s_n[wave] = (uint32_t)__popcll(m);
+  __syncthreads();
+  uint32_t off = blk_off[blockIdx.x];
+  for (int i = 0; i < wave; i++) off += s_n[i];
+  if (act) active[off + __popcll(m & ((1ull << lane) - 1ull))] = (uint32_t)p;
+}
+
+__global__ __launch_bounds__(kMcBlock) void k_mc_edges(const uint8_t* __restrict__ flags, GridGeom g,
+                                                       uint8_t* __restrict__ emask, uint32_t* __restrict__ cnt) {
+  const int64_t X = g.res[0], Y = g.res[1], Z = g.res[2], XY = X * Y, N = XY * Z;
+  const int64_t p = (int64_t)blockIdx.x * kMcBlock + threadIdx.x;
+  if (p >= N) return;
+  const int64_t z = p / XY, r = p - z * XY, y = r / X, x = r - y * X;
+  unsigned bits = 0;
+  // x-edge (p, p+1): cells p - (0,a,b); local id a + 2b
+  if (x < X - 1) {
+    unsigned ref = 0;
+    for (int b = 0; b < 2; b++)
+      for (int a = 0; a < 2; a++)
+        if (y >= a && z >= b) ref |= (kMc.emask[flags[p - a * X - b * XY]] >> (a + 2 * b)) & 1u;
+    bits |= ref;
+  }
+  // y-edge (p, p+X): cells p - (a,0,b); local id 4 + a + 2b
+  if (y < Y - 1) {
+    unsigned ref = 0;
+    for (int b = 0; b < 2; b++)
+      for (int a = 0; a < 2; a++)
+        if (x >= a && z >= b) ref |= (kMc.emask[flags[p - a - b * XY]] >> (4 + a + 2 * b)) & 1u;
+    bits |= ref << 1;
+  }
+  // z-edge (p, p+XY): cells p - (a,b,0); local id 8 + a + 2b
+  if (z < Z - 1) {
+    unsigned ref = 0;
+    for (int b = 0; b < 2; b++)
+      for (int a = 0; a < 2; a++)
+        if (x >= a && y >= b) ref |= (kMc.emask[flags[p - a - b * X]] >> (8 + a + 2 * b)) & 1u;
+    bits |= ref << 2;
+  }
+  emask[p] = (uint8_t)bits;
+  cnt[p] = (bits & 1u) + ((bits >> 1) & 1u) + ((bits >> 2) & 1u);
+}
+
+// computeEdgeInterpolations (TRIANGLE_MESH.cpp:2373-2425)
+__global__ __launch_bounds__(kMcBlock) void k_mc_vertices(const double* __restrict__ field, GridGeom g, double iso,
+                                                          const uint8_t* __restrict__ emask,
+                                                          const uint32_t* __restrict__ vbase,
+                                                          double* __restrict__ verts) {
+#pragma clang fp contract(off)
+  const int64_t X = g.res[0], XY = X * g.res[1], N = XY * g.res[2];
+  const int64_t p = (int64_t)blockIdx.x * kMcBlock + threadIdx.x;
+  if (p >= N) return;
+  const unsigned bits = emask[p];
+  if (!bits) return;
+  const int z = (int)(p / XY), y = (int)((p - (int64_t)z * XY) / X), x = (int)(p - (int64_t)z * XY - (int64_t)y * X);
+  double a[3];
+  cell_center(g, x, y, z, a);
+  const double d0 = field[p] - iso;
+  uint32_t k = vbase[p];
+  for (int axis = 0; axis < 3; axis++) {
+    if (!(bits & (1u << axis))) continue;
+    const int x2 = x + (axis == 0), y2 = y + (axis == 1), z2 = z + (axis == 2);
+    double b[3];
+    cell_center(g, x2, y2, z2, b);
+    const double off0 = b[0] - a[0], off1 = b[1] - a[1], off2 = b[2] - a[2];
+    const double d1 = field[p + (axis == 0 ? 1 : (axis == 1 ? X : XY))] - iso;
+    const double interpolation = d0 / (d0 - d1);
+    verts[3 * (int64_t)k + 0] = a[0] + interpolation * off0;
+    verts[3 * (int64_t)k + 1] = a[1] + interpolation * off1;
+    verts[3 * (int64_t)k + 2] = a[2] + interpolation * off2;
+    k++;
+  }
+}
+
+__global__ __launch_bounds__(kMcBlock) void k_mc_tricount(const uint32_t* __restrict__ active, int64_t na,
+                                                          const uint8_t* __restrict__ flags,
+                                                          uint32_t* __restrict__ tcount) {
+  const int64_t i = (int64_t)blockIdx.x * kMcBlock + threadIdx.x;
+  if (i < na) tcount[i] = kMc.ntri[flags[active[i]]];
+}
+
+__device__ __forceinline__ double edge_len(const double* a, const double* b) {
+#pragma clang fp contract(off)
+  const double e0 = a[0] - b[0], e1 = a[1] - b[1], e2 = a[2] - b[2];
+  return sqrt(e0 * e0 + e1 * e1 + e2 * e2);  // norm(v0 - v1), VEC3.h:215-216
+}
+
+// addVertexPairTriangle (TRIANGLE_MESH.cpp:2490-2503) + degenerate filter (:1088-1102)
+__global__ __launch_bounds__(kMcBlock) void k_mc_triangles(const uint32_t* __restrict__ active, int64_t na,
+                                                           const uint8_t* __restrict__ flags, GridGeom g,
+                                                           const uint8_t* __restrict__ emask,
+                                                           const uint32_t* __restrict__ vbase,
+                                                           const double* __restrict__ verts,
+                                                           const uint32_t* __restrict__ toff,
+                                                           int32_t* __restrict__ tris_pre,
+                                                           uint32_t* __restrict__ keep) {
+  const int64_t i = (int64_t)blockIdx.x * kMcBlock + threadIdx.x;
+  if (i >= na) return;
+  const int64_t X = g.res[0], XY = X * g.res[1];
+  const int64_t p = active[i];
+  const int flag = flags[p];
+  const int n = kMc.ntri[flag];
+  uint32_t o = toff[i];
+  for (int t = 0; t < n; t++) {
+    int32_t v[3];
+    for (int c = 0; c < 3; c++) {
+      int dx, dy, dz, axis;
+      edge_geom(kMc.tri[flag][3 * t + c], dx, dy, dz, axis);
+      const int64_t owner = p + dx + dy * X + dz * XY;
+      v[c] = (int32_t)(vbase[owner] + __popc(emask[owner] & ((1u << axis) - 1u)));
+    }
+    const double eps = 1e-7;
+    const double* p0 = verts + 3 * (int64_t)v[0];
+    const double* p1 = verts + 3 * (int64_t)v[1];
+    const double* p2 = verts + 3 * (int64_t)v[2];
+    const bool degenerate = edge_len(p0, p1) < eps || edge_len(p0, p2) < eps || edge_len(p1, p2) < eps;
+    tris_pre[3 * (int64_t)o + 0] = v[0];
+    tris_pre[3 * (int64_t)o + 1] = v[1];
+    tris_pre[3 * (int64_t)o + 2] = v[2];
+    keep[o] = degenerate ? 0u : 1u;
+    o++;
+  }
+}
+
+// koff is the exclusive scan of the keep flags: triangle i is kept when its
+// offset differs from the next one (or from the total for the last).
+__global__ __launch_bounds__(kMcBlock) void k_mc_squeeze(const int32_t* __restrict__ tris_pre,
+                                                         const uint32_t* __restrict__ koff, int64_t n,
+                                                         const uint32_t* __restrict__ total,
+                                                         int32_t* __restrict__ tris) {
+  const int64_t i = (int64_t)blockIdx.x * kMcBlock + threadIdx.x;
+  if (i >= n) return;
+  const uint32_t o = koff[i];
+  const uint32_t next = (i + 1 < n) ? koff[i + 1] : *total;
+  if (next == o) return;
+  tris[3 * (int64_t)o + 0] = tris_pre[3 * i + 0];
+  tris[3 * (int64_t)o + 1] = tris_pre[3 * i + 1];
+  tris[3 * (int64_t)o + 2] = tris_pre[3 * i + 2];
+}
+
+// ---------------------------------------------------------------- host side
+
+template <class T>
+static qj_status grow(T*& p, int64_t& cap, int64_t n, hipStream_t s) {
+  if (n <= cap) return QJ_OK;
+  if (p) {
+    QJ_HIP(hipStreamSynchronize(s));
+    QJ_HIP(hipFree(p));
+    p = nullptr;
+    cap = 0;
+  }
+  const int64_t want = n + n / 8 + 1024;
+  QJ_HIP(hipMalloc(&p, sizeof(T) * (size_t)want));
+  cap = want;
+  return QJ_OK;
+}
+
+static unsigned nblocks(int64_t n) { return (unsigned)((n + kMcBlock - 1) / kMcBlock); }
+
+qj_status mc_run(qj_workspace* ws, const double* d_field, const qj_grid* grid, double iso, int64_t* nv_out,
+                 int64_t* nt_out, hipStream_t s) {
+  if (!ws->mc) ws->mc = new McState();
+  McState& m = *(McState*)ws->mc;
+  m.nv = m.nt = 0;
+  GridGeom g;
+  make_geom(grid, &g);
+  const int64_t N = (int64_t)g.res[0] * g.res[1] * g.res[2];
+  const int64_t cells = (int64_t)(g.res[0] - 1) * (g.res[1] - 1) * (g.res[2] - 1);
+  if (N > (int64_t)UINT32_MAX || 5 * cells > (int64_t)INT32_MAX) {
+    set_error("marching cubes on %lld points exceeds 32-bit vertex/triangle ids; split the field in tiles",
+              (long long)N);
+    return QJ_ERR_CAPACITY;
+  }
+  *nv_out = *nt_out = 0;
+  if (cells <= 0) return QJ_OK;
+  qj_status st;
+  const int64_t nb = nblocks(N);
+  if ((st = grow(m.flags, m.cap_flags, N, s)) || (st = grow(m.emask, m.cap_emask, N, s)) ||
+      (st = grow(m.cnt, m.cap_cnt, N, s)) || (st = grow(m.active, m.cap_active, N, s)) ||
+      (st = grow(m.tri_off, m.cap_toff, N, s)) || (st = grow(m.blk, m.cap_blk, nb, s)))
+    return st;
+  const size_t need_tmp = scan_tmp_bytes(5 * N);  // the largest scan: pre-filter triangles <= 5 per cell
+  if (need_tmp > m.tmp_bytes) {
+    if (m.tmp) {
+      QJ_HIP(hipStreamSynchronize(s));
+      QJ_HIP(hipFree(m.tmp));
+    }
+    m.tmp = nullptr;
+    m.tmp_bytes = 0;
+    QJ_HIP(hipMalloc(&m.tmp, need_tmp));
+    m.tmp_bytes = need_tmp;
+  }
+  if (!m.totals) QJ_HIP(hipMalloc(&m.totals, 4 * sizeof(uint32_t)));
+
+  hipLaunchKernelGGL(k_mc_flags, dim3((unsigned)nb), dim3(kMcBlock), 0, s, d_field, g, iso, m.flags, m.blk);
+  QJ_HIP(hipGetLastError());
+  if ((st = scan_u32(m.blk, m.blk, nb, m.totals + 0, m.tmp, m.tmp_bytes, s))) return st;
+  hipLaunchKernelGGL(k_mc_active, dim3((unsigned)nb), dim3(kMcBlock), 0, s, m.flags, N, m.blk, m.active);
+  QJ_HIP(hipGetLastError());
+  hipLaunchKernelGGL(k_mc_edges, dim3((unsigned)nb), dim3(kMcBlock), 0, s, m.flags, g, m.emask, m.cnt);
+  QJ_HIP(hipGetLastError());
+  if ((st = scan_u32(m.cnt, m.cnt, N, m.totals + 1, m.tmp, m.tmp_bytes, s))) return st;
+  uint32_t h[4];
+  QJ_HIP(hipMemcpyAsync(h, m.totals, 2 * sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+  QJ_HIP(hipStreamSynchronize(s));
+  const int64_t na = h[0], nv = h[1];
+  if ((st = grow(m.verts, m.cap_v, 3 * nv, s))) return st;
+  if (nv > 0) {
+    hipLaunchKernelGGL(k_mc_vertices, dim3((unsigned)nb), dim3(kMcBlock), 0, s, d_field, g, iso, m.emask, m.cnt,
+                       m.verts);
+    QJ_HIP(hipGetLastError());
+  }
+  m.nv = nv;
+  if (na == 0) {
+    QJ_HIP(hipStreamSynchronize(s));
+    *nv_out = nv;
+    return QJ_OK;
+  }
+  hipLaunchKernelGGL(k_mc_tricount, dim3(nblocks(na)), dim3(kMcBlock), 0, s, m.active, na, m.flags, m.tri_off);
+  QJ_HIP(hipGetLastError());
+  if ((st = scan_u32(m.tri_off, m.tri_off, na, m.totals + 2, m.tmp, m.tmp_bytes, s))) return st;
+  QJ_HIP(hipMemcpyAsync(h + 2, m.totals + 2, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+  QJ_HIP(hipStreamSynchronize(s));
+  const int64_t ntp = h[2];
+  if ((st = grow(m.tris_pre, m.cap_tp, 3 * ntp, s)) || (st = grow(m.keep, m.cap_keep, ntp, s))) return st;
+  hipLaunchKernelGGL(k_mc_triangles, dim3(nblocks(na)), dim3(kMcBlock), 0, s, m.active, na, m.flags, g, m.emask,
+                     m.cnt, m.verts, m.tri_off, m.tris_pre, m.keep);
+  QJ_HIP(hipGetLastError());
+  if ((st = scan_u32(m.keep, m.keep, ntp, m.totals + 3, m.tmp, m.tmp_bytes, s))) return st;
+  QJ_HIP(hipMemcpyAsync(h + 3, m.totals + 3, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+  QJ_HIP(hipStreamSynchronize(s));
+  const int64_t nt = h[3];
+  if ((st = grow(m.tris, m.cap_t, 3 * nt, s))) return st;
+  if (nt > 0) {
+    hipLaunchKernelGGL(k_mc_squeeze, dim3(nblocks(ntp)), dim3(kMcBlock), 0, s, m.tris_pre, m.keep, ntp,
+                       m.totals + 3, m.tris);
+    QJ_HIP(hipGetLastError());
+  }
+  QJ_HIP(hipStreamSynchronize(s));
+  m.nt = nt;
+  *nv_out = nv;
+  *nt_out = nt;
+  return QJ_OK;
+}
+
+qj_status mc_copy(qj_workspace* ws, double* v, int32_t* t, hipStream_t s) {
+  McState* m = (McState*)ws->mc;
+  if (!m) return QJ_ERR_INVALID;
+  if (v && m->nv) QJ_HIP(hipMemcpyAsync(v, m->verts, sizeof(double) * 3 * m->nv, hipMemcpyDefault, s));
+  if (t && m->nt) QJ_HIP(hipMemcpyAsync(t, m->tris, sizeof(int32_t) * 3 * m->nt, hipMemcpyDefault, s));
+  QJ_HIP(hipStreamSynchronize(s));
+  return QJ_OK;
+}
+
+qj_status mc_ptrs(qj_workspace* ws, const double** v, const int32_t** t) {
+  McState* m = (McState*)ws->mc;
+  if (!m) return QJ_ERR_INVALID;
+  if (v) *v = m->verts;
+  if (t) *t = m->tris;
+  return QJ_OK;
+}
+
+void mc_free(qj_workspace* ws) {
+  McState* m = (McState*)ws->mc;
+  if (!m) return;
+  hipFree(m->flags);
+  hipFree(m->emask);
+  hipFree(m->cnt);
+  hipFree(m->blk);
+  hipFree(m->active);
+  hipFree(m->tri_off);
+  hipFree(m->verts);
+  hipFree(m->tris_pre);
+  hipFree(m->keep);
+  hipFree(m->tris);
+  hipFree(m->tmp);
+  hipFree(m->totals);
+  delete m;
+  ws->mc = nullptr;
+}
+
+}  // namespace qj

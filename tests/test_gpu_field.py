@@ -1,0 +1,189 @@
+"""GPU parity: the HIP field kernels (through the C ABI) against the oracle.
+
+Tolerances (stated here, used below):
+  FP64 kernel: |F_gpu - F_cpu| <= 1e-9 * max(1, |F_cpu|) for finite values, and
+  the same non-finite pattern (+inf/-inf/NaN). The kernels use the device math
+  library (OCML) and FMA contraction, the oracle glibc without contraction, so
+  results differ at the ulp level and the map amplifies that a little.
+  FP32 bulk + FP64 band: zero isosurface sign flips ((float)F > 0, the
+  marching-cubes predicate) and, inside the band, the FP64 tolerance.
+"""
+import numpy as np
+import pytest
+import torch
+
+import oracle as O
+from quijibo_amd import api, workloads as W
+from quijibo_amd._lib import FP32_BAND, FP64, MAP_ADAPTIVE, MAP_BLACK_HOLE, MAP_SKIP_ORIGIN
+
+pytestmark = pytest.mark.gpu
+
+RTOL64 = 1e-9
+
+
+def run_both(w, precision=FP64, flags=MAP_BLACK_HOLE, band=0.05, center=None, lengths=None, rotation=None,
+             escape=None, init=None):
+    center = center or w.center
+    lengths = lengths or w.lengths
+    rotation = rotation or (1.0, 0.0, 0.0, 0.0)
+    escape = w.escape if escape is None else escape
+    g = api.make_grid(w.res, center, lengths, rotation)
+    p = api.make_params(w.max_iter, escape, w.exp_scaling, w.slice, flags)
+    top = api.POLYNOMIAL_4D(w.top.roots, w.top.powers, w.top.power_scalar)
+    bot = api.POLYNOMIAL_4D(w.bottom.roots, w.bottom.powers, w.bottom.power_scalar)
+    r = api.Rational(top, bot)
+    out = torch.full((w.voxels,), -7.0, dtype=torch.float64, device="cuda") if init is None else init.clone()
+    api.field_eval(g, r, p, precision, band, out=out)
+    torch.cuda.synchronize()
+    gpu = out.cpu().numpy()
+    og = O.make_grid(w.res, center, lengths, rotation)
+    oflags = (O.BLACK_HOLE if flags & MAP_BLACK_HOLE else 0) | (O.SKIP_ORIGIN if flags & MAP_SKIP_ORIGIN else 0) | \
+             (O.AP_REPASS if flags & MAP_ADAPTIVE else 0)
+    cpu = O.field_map(og, O.poly(w.top.roots, w.top.powers, w.top.power_scalar),
+                      O.poly(w.bottom.roots, w.bottom.powers, w.bottom.power_scalar),
+                      O.make_params(w.max_iter, escape, w.exp_scaling, w.slice, oflags),
+                      init=None if init is None else init.cpu().numpy())
+    if init is None:
+        cpu_written = cpu
+    else:
+        cpu_written = cpu
+    return gpu, cpu_written, g, r
+
+
+def assert_fp64_close(gpu, cpu, rtol=RTOL64, max_outliers=0):
+    fin = np.isfinite(cpu)
+    same_class = (np.isfinite(gpu) == fin) & ((np.isnan(gpu) == np.isnan(cpu)))
+    bad_class = np.count_nonzero(~same_class)
+    err = np.abs(gpu[fin] - cpu[fin]) / np.maximum(1.0, np.abs(cpu[fin]))
+    n_out = int(np.count_nonzero(err > rtol)) + bad_class
+    print("fp64: n=%d nonfinite=%d class_mismatch=%d max_rel=%.3e outliers=%d" %
+          (gpu.size, int((~fin).sum()), bad_class, err.max() if err.size else 0.0, n_out), flush=True)
+    assert n_out <= max_outliers
+    inf = ~fin & ~np.isnan(cpu)
+    assert np.array_equal(gpu[inf], cpu[inf])
+
+
+def test_fp64_config1_64(gpu):
+    w = W.config1(64)
+    g, c, _, _ = run_both(w)
+    assert_fp64_close(g, c)
+
+
+def test_fp64_config2_48(gpu):
+    w = W.config2(48)
+    g, c, _, _ = run_both(w)
+    assert_fp64_close(g, c)
+
+
+def test_fp64_config3_roots_32(gpu):
+    w = W.config3(32)
+    g, c, _, _ = run_both(w)
+    assert_fp64_close(g, c)
+
+
+def test_fp64_rotated_offcentre_grid(gpu):
+    w = W.config2(24)
+    w.res = (24, 20, 17)
+    q = np.array([0.9, 0.2, -0.3, 0.25])
+    q /= np.linalg.norm(q)
+    g, c, _, _ = run_both(w, center=(0.1, -0.05, 0.2), lengths=(2.8, 3.1, 2.6), rotation=tuple(q))
+    assert_fp64_close(g, c)
+
+
+def test_fp64_general_exponents_and_scalars(gpu):
+    w = W.config2(32)
+    rng = np.random.default_rng(5)
+    w.top.powers = rng.uniform(0.5, 2.5, size=w.top.n)
+    w.bottom.powers = rng.uniform(0.5, 2.5, size=w.bottom.n)
+    w.top.power_scalar, w.bottom.power_scalar = 0.9, 1.1
+    w.exp_scaling = 0.8
+    w.slice = 0.05
+    g, c, _, _ = run_both(w)
+    assert_fp64_close(g, c)
+
+
+def signflips(a, b):
+    return int(np.count_nonzero((a.astype(np.float32) > 0) != (b.astype(np.float32) > 0)))
+
+
+@pytest.mark.parametrize("res", [48, 96])
+def test_fp32_band_config2(gpu, res):
+    w = W.config2(res)
+    g, c, _, _ = run_both(w, precision=FP32_BAND, band=0.05)
+    flips = signflips(g, c)
+    fin = np.isfinite(c) & np.isfinite(g)
+    err = np.abs(g[fin] - c[fin])
+    inband = np.abs(c) < 0.05
+    print("fp32band res=%d flips=%d max|dF|=%.3e band=%d" % (res, flips, err.max(), inband.sum()), flush=True)
+    assert flips == 0
+    assert np.array_equal(np.isfinite(g), np.isfinite(c))
+    e_in = np.abs(g[inband] - c[inband]) / np.maximum(1.0, np.abs(c[inband]))
+    assert e_in.size == 0 or e_in.max() <= RTOL64
+    assert err.max() < 0.05
+
+
+def test_fp32_band_config3_roots(gpu):
+    w = W.config3(32)
+    g, c, _, _ = run_both(w, precision=FP32_BAND, band=0.05)
+    assert signflips(g, c) == 0
+    assert np.array_equal(np.isnan(g), np.isnan(c))
+
+
+def test_fp32_band_general_exponents(gpu):
+    w = W.config2(40)
+    rng = np.random.default_rng(9)
+    w.top.powers = rng.uniform(0.5, 2.5, size=w.top.n)
+    w.bottom.powers = rng.uniform(0.5, 2.5, size=w.bottom.n)
+    g, c, _, _ = run_both(w, precision=FP32_BAND, band=0.05)
+    assert signflips(g, c) == 0
+
+
+def test_ap_mode_overflow_repass(gpu):
+    """computeRationalMapAP: escape 20, origin skipped, overflowed cells recomputed with extended range."""
+    top = W.RootSet(np.array([[1.9, 0.3, 0.1, 0.0], [-2.1, 0.4, -0.2, 0.0]]), np.array([250.0, 250.0]))
+    bot = W.RootSet(np.array([[0.05, -0.1, 0.2, 0.0]]), np.array([3.0]))
+    w = W.Workload("ap", (10, 10, 9), top, bot, max_iter=1, escape=20.0, exp_scaling=0.5)
+    init = torch.full((w.voxels,), 123.0, dtype=torch.float64, device="cuda")
+    g, c, _, _ = run_both(w, flags=MAP_SKIP_ORIGIN | MAP_ADAPTIVE, escape=20.0, init=init)
+    assert_fp64_close(g, c, rtol=1e-12)
+    assert np.all(np.isfinite(g))
+
+
+def test_skip_origin_leaves_cell_unwritten(gpu):
+    w = W.config1(5)  # odd resolution: the middle cell centre is the origin
+    init = torch.full((w.voxels,), 42.0, dtype=torch.float64, device="cuda")
+    g, c, _, _ = run_both(w, flags=MAP_SKIP_ORIGIN, init=init, lengths=(3.0, 3.0, 3.0))
+    mid = 2 + 5 * 2 + 25 * 2
+    assert g[mid] == 42.0 and c[mid] == 42.0
+    assert_fp64_close(g, c)
+
+
+def test_points_eval_matches_oracle(gpu):
+    w = W.config5(n_surface=3000, n_exterior=3000)
+    pts = np.concatenate([w.extra["surface"], w.extra["exterior"]])
+    top = api.POLYNOMIAL_4D(w.top.roots, w.top.powers)
+    bot = api.POLYNOMIAL_4D(w.bottom.roots, w.bottom.powers)
+    r = api.Rational(top, bot)
+    p = api.make_params(w.max_iter, 2000.0, 1.0, 0.0, MAP_BLACK_HOLE)
+    d = torch.from_numpy(pts).cuda()
+    f64 = api.points_eval(d, r, p, FP64).cpu().numpy()
+    f32 = api.points_eval(d, r, p, FP32_BAND, 0.05).cpu().numpy()
+    cpu = O.points_eval(pts, O.poly(w.top.roots, w.top.powers), O.poly(w.bottom.roots, w.bottom.powers),
+                        O.make_params(w.max_iter, 2000.0, 1.0, 0.0, O.BLACK_HOLE))
+    assert_fp64_close(f64, cpu)
+    assert signflips(f32, cpu) == 0
+
+
+def test_stats_count_iterations(gpu):
+    w = W.config1(16)
+    ws = api.Workspace()
+    g = api.make_grid(w.res, w.center, w.lengths)
+    r = api.Rational(api.POLYNOMIAL_4D(w.top.roots), api.POLYNOMIAL_4D())
+    out = torch.empty(w.voxels, dtype=torch.float64, device="cuda")
+    ws.reset_stats()
+    api.field_eval(g, r, api.make_params(3, 1e3), FP64, out=out, workspace=ws)
+    st = ws.stats()
+    _, iters = O.field_map(O.make_grid(w.res, w.center, w.lengths), O.poly(w.top.roots), O.poly(np.zeros((0, 4))),
+                           O.make_params(3, 1e3), want_iters=True)
+    assert st["voxels"] == w.voxels
+    assert st["iterations"] == int(iters.sum())

@@ -1,0 +1,80 @@
+"""GPU marching cubes (through the C ABI) against the oracle mesher on the same
+field: vertex count, triangle count, vertex coordinates and triangle indices
+must be bit-identical."""
+import numpy as np
+import pytest
+import torch
+
+import oracle as O
+from quijibo_amd import api, workloads as W
+from quijibo_amd._lib import FP64, MAP_BLACK_HOLE
+
+pytestmark = pytest.mark.gpu
+
+
+def mc_both(field_np, res, center=(0, 0, 0), lengths=(3, 3, 3), rotation=(1, 0, 0, 0), iso=0.0):
+    g = api.make_grid(res, center, lengths, rotation)
+    d = torch.from_numpy(np.ascontiguousarray(field_np)).cuda()
+    v, t = api.marching_cubes(d, g, iso)
+    og = O.make_grid(res, center, lengths, rotation)
+    f = field_np if iso == 0.0 else field_np - iso
+    ov, ot = O.marching_cubes(og, f)
+    return v, t, ov, ot
+
+
+def assert_same_mesh(v, t, ov, ot):
+    print("mc: gpu %d verts %d tris, cpu %d verts %d tris" % (len(v), len(t), len(ov), len(ot)), flush=True)
+    assert v.shape == ov.shape and t.shape == ot.shape
+    assert np.array_equal(t, ot)
+    same = (v == ov) | (np.isnan(v) & np.isnan(ov))
+    assert same.all(), np.argwhere(~same)[:5]
+
+
+def test_sphere(gpu):
+    n = 40
+    ax = -1.5 + (np.arange(n) + 0.5) * 3.0 / n
+    Z, Y, X = np.meshgrid(ax, ax, ax, indexing="ij")
+    f = 0.9 - np.sqrt(X ** 2 + (Y - 0.1) ** 2 + Z ** 2)
+    assert_same_mesh(*mc_both(f.reshape(-1), (n, n, n)))
+
+
+@pytest.mark.parametrize("flag", [1, 97, 158, 255, 0, 105, 60])
+def test_single_cells(gpu, flag):
+    f = np.empty(8)
+    for z in range(2):
+        for y in range(2):
+            for x in range(2):
+                bit = 4 * x + 2 * y + z
+                f[x + 2 * y + 4 * z] = (1.0 if (flag >> bit) & 1 else -1.0) * (1 + 0.1 * bit)
+    assert_same_mesh(*mc_both(f, (2, 2, 2), lengths=(2, 2, 2)))
+
+
+def test_random_field_all_cases(gpu):
+    rng = np.random.default_rng(2)
+    res = (23, 17, 13)
+    f = rng.normal(size=res[0] * res[1] * res[2])
+    f[rng.integers(0, f.size, 40)] = np.nan
+    f[rng.integers(0, f.size, 40)] = -np.inf
+    f[rng.integers(0, f.size, 40)] = 1e-50
+    v, t, ov, ot = mc_both(f, res, center=(0.2, 0.1, -0.3), lengths=(2.0, 1.5, 1.1),
+                           rotation=(0.8, 0.36, 0.48, 0.0))
+    assert_same_mesh(v, t, ov, ot)
+
+
+def test_iso_shift(gpu):
+    rng = np.random.default_rng(3)
+    res = (12, 12, 12)
+    f = rng.normal(size=12 ** 3)
+    assert_same_mesh(*mc_both(f, res, iso=0.3))
+
+
+@pytest.mark.parametrize("cfg", ["c1", "c3"])
+def test_julia_fields(gpu, cfg):
+    w = W.config1(64) if cfg == "c1" else W.config3(48)
+    g = api.make_grid(w.res, w.center, w.lengths)
+    r = api.Rational(api.POLYNOMIAL_4D(w.top.roots, w.top.powers), api.POLYNOMIAL_4D(w.bottom.roots, w.bottom.powers))
+    out = api.field_eval(g, r, api.make_params(w.max_iter, w.escape, 1.0, 0.0, MAP_BLACK_HOLE), FP64)
+    f = out.cpu().numpy()
+    v, t, ov, ot = mc_both(f, w.res)
+    assert len(t) > 1000
+    assert_same_mesh(v, t, ov, ot)

@@ -1,0 +1,152 @@
+"""Pins the oracle's field map (OPTIMIZE_3D.cpp:1272-1370, AP variant :1374-1568)
+against independent restatements: the 2x2 complex-matrix form of quaternions
+with principal matrix powers, and the complex plane, where the whole map is a
+complex rational map with a closed form for log|f|."""
+import numpy as np
+import pytest
+import scipy.linalg
+
+import oracle as O
+from quijibo_amd import workloads as W
+
+
+def M(q):
+    a, b, c, d = q
+    return np.array([[a + 1j * b, c + 1j * d], [-c + 1j * d, a - 1j * b]])
+
+
+def qabs(m):
+    return np.sqrt(abs(m[0, 0]) ** 2 + abs(m[0, 1]) ** 2)
+
+
+def brute_map(res, center, lengths, top, tp, bot, bp, max_iter, escape, c, slice_, black_hole):
+    out = np.zeros(res[0] * res[1] * res[2])
+    dx = [lengths[i] / res[i] for i in range(3)]
+    lo = [center[i] - 0.5 * lengths[i] for i in range(3)]
+    for z in range(res[2]):
+        for y in range(res[1]):
+            for x in range(res[0]):
+                p = [lo[0] + (x + 0.5) * dx[0], lo[1] + (y + 0.5) * dx[1], lo[2] + (z + 0.5) * dx[2]]
+                q = M([p[0], p[1], p[2], slice_])
+                mag = qabs(q)
+                it = 0
+                while mag < escape and it < max_iter:
+                    qv = np.array([q[0, 0].real, q[0, 0].imag, q[0, 1].real, q[0, 1].imag])
+                    T = np.eye(2, dtype=complex)
+                    for r, e in zip(top, tp):
+                        T = T @ scipy.linalg.fractional_matrix_power(M(qv - r), e)
+                    if len(bot):
+                        B = np.eye(2, dtype=complex)
+                        for r, e in zip(bot, bp):
+                            B = B @ scipy.linalg.fractional_matrix_power(M(qv - r), e)
+                        T = T @ np.linalg.inv(B)
+                    q = c * T
+                    mag = qabs(q)
+                    it += 1
+                    if black_hole and mag < 10 * np.finfo(np.float32).tiny:
+                        it = max_iter
+                out[x + y * res[0] + z * res[0] * res[1]] = np.log(mag)
+    return out
+
+
+@pytest.mark.parametrize("case", ["poly", "rational"])
+def test_map_matches_matrix_bruteforce(case):
+    rng = np.random.default_rng(11)
+    top = rng.normal(scale=0.6, size=(3, 4))
+    tp = np.array([1.0, 2.0, 1.5])
+    bot = rng.normal(scale=0.6, size=(2, 4)) if case == "rational" else np.zeros((0, 4))
+    bp = np.array([1.0, 0.5]) if case == "rational" else np.zeros(0)
+    res, center, lengths = (5, 4, 3), (0.1, -0.2, 0.05), (2.5, 2.0, 1.8)
+    c, slice_, mi = 0.9, 0.07, 3
+    g = O.make_grid(res, center, lengths)
+    f = O.field_map(g, O.poly(top, tp), O.poly(bot, bp), O.make_params(mi, 2000.0, c, slice_, O.BLACK_HOLE))
+    ref = brute_map(res, center, lengths, top, tp, bot, bp, mi, 2000.0, c, slice_, True)
+    fin = np.isfinite(ref)
+    assert fin.mean() > 0.9
+    assert np.array_equal(np.isfinite(f), fin)
+    assert np.allclose(f[fin], ref[fin], rtol=1e-9, atol=1e-9)
+
+
+def test_complex_plane_closed_form():
+    """Roots and voxels in span{1, i}: f is the complex map z -> c prod (z-r)^e / prod (z-b)^e."""
+    rng = np.random.default_rng(3)
+    top = np.zeros((4, 4)); top[:, :2] = rng.normal(scale=0.7, size=(4, 2))
+    bot = np.zeros((3, 4)); bot[:, :2] = rng.normal(scale=0.7, size=(3, 2))
+    tp = np.array([1.0, 2.0, 1.0, 1.5]); bp = np.array([1.0, 1.0, 2.0])
+    res = (11, 9, 3)  # z index 1 has Z = 0 exactly, so q = X + iY
+    g = O.make_grid(res, (0, 0, 0), (3.0, 3.0, 3.0))
+    c, mi, esc = 1.1, 4, 50.0
+    f = O.field_map(g, O.poly(top, tp), O.poly(bot, bp), O.make_params(mi, esc, c, 0.0, O.BLACK_HOLE))
+    dx, dy = 3.0 / 11, 3.0 / 9
+    zt = top[:, 0] + 1j * top[:, 1]
+    zb = bot[:, 0] + 1j * bot[:, 1]
+    for y in range(res[1]):
+        for x in range(res[0]):
+            z = (-1.5 + (x + 0.5) * dx) + 1j * (-1.5 + (y + 0.5) * dy)
+            it = 0
+            while abs(z) < esc and it < mi:
+                z = c * np.prod((z - zt) ** tp) / np.prod((z - zb) ** bp)
+                it += 1
+            got = f[x + y * res[0] + 1 * res[0] * res[1]]
+            assert np.isclose(got, np.log(abs(z)), rtol=1e-9, atol=1e-9), (x, y, got, np.log(abs(z)))
+
+
+def test_special_cases():
+    top = np.array([[0.5, 0.1, -0.2, 0.0]])
+    g = O.make_grid((4, 4, 4), (0, 0, 0), (2, 2, 2))
+    centers = np.array([O.cell_center(g, x, y, z) for z in range(4) for y in range(4) for x in range(4)])
+    mag0 = np.linalg.norm(centers, axis=1)
+    # maxIter 0: field = log|q0|
+    f0 = O.field_map(g, O.poly(top), O.poly(np.zeros((0, 4))), O.make_params(0, 2000.0))
+    assert np.allclose(f0, np.log(mag0), rtol=0, atol=1e-15)
+    # escape below every |q0|: no iteration
+    f1 = O.field_map(g, O.poly(top), O.poly(np.zeros((0, 4))), O.make_params(5, 0.1))
+    assert np.allclose(f1, np.log(mag0), rtol=0, atol=1e-15)
+    # one iteration of a single root with e = 1: log|c (q0 - r)|
+    f2 = O.field_map(g, O.poly(top), O.poly(np.zeros((0, 4))), O.make_params(1, 2000.0, 2.0))
+    q0 = np.concatenate([centers, np.zeros((64, 1))], axis=1)
+    assert np.allclose(f2, np.log(2.0 * np.linalg.norm(q0 - top[0], axis=1)), rtol=1e-14)
+
+
+def test_black_hole_stops_iteration():
+    g = O.make_grid((2, 2, 2), (0, 0, 0), (2, 2, 2))
+    c0 = O.cell_center(g, 0, 0, 0)
+    root = np.array([[c0[0] + 1e-5, c0[1], c0[2], 0.0]])
+    top = O.poly(root, [9.0])  # |q0 - r|^9 = 1e-45 < 10 FLT_MIN at cell 0
+    empty = O.poly(np.zeros((0, 4)))
+    fb = O.field_map(g, top, empty, O.make_params(3, 2000.0, 1.0, 0.0, O.BLACK_HOLE), want_iters=True)
+    fn = O.field_map(g, top, empty, O.make_params(3, 2000.0, 1.0, 0.0, 0), want_iters=True)
+    assert fb[1][0] == 1 and fn[1][0] == 3
+    assert np.isclose(fb[0][0], 9 * np.log(1e-5), rtol=1e-9)
+    assert fn[0][0] != fb[0][0]
+
+
+def test_ap_repass_extended_range():
+    """Exponents that overflow FP64 in one iteration: the long double re-pass
+    (OPTIMIZE_3D.cpp:1477-1567) gives the finite closed form
+    log c + sum e log|z - r| - sum e log|z - b|, while the FP64 pass does not."""
+    top = np.array([[1.9, 0.3, 0, 0], [-2.1, 0.4, 0, 0]])
+    bot = np.array([[0.05, -0.1, 0, 0]])
+    tp = np.array([250.0, 250.0]); bp = np.array([3.0])
+    g = O.make_grid((6, 6, 3), (0, 0, 0), (3, 3, 3))
+    flags = O.SKIP_ORIGIN | O.AP_REPASS
+    f, ovf = O.field_map(g, O.poly(top, tp), O.poly(bot, bp), O.make_params(1, 20.0, 0.5, 0.0, flags),
+                         want_overflow=True)
+    plain = O.field_map(g, O.poly(top, tp), O.poly(bot, bp), O.make_params(1, 20.0, 0.5, 0.0, O.SKIP_ORIGIN))
+    zt = top[:, 0] + 1j * top[:, 1]; zb = bot[:, 0] + 1j * bot[:, 1]
+    for y in range(6):
+        for x in range(6):
+            i = x + 6 * y + 36
+            z = (-1.5 + (x + 0.5) * 0.5) + 1j * (-1.5 + (y + 0.5) * 0.5)
+            want = np.log(0.5) + np.sum(tp * np.log(abs(z - zt))) - np.sum(bp * np.log(abs(z - zb)))
+            assert np.isclose(f[i], want, rtol=1e-12), (x, y, f[i], want)
+    assert ovf.sum() > 0 and not np.all(np.isfinite(plain[ovf.astype(bool)]))
+
+
+def test_config_workloads_are_deterministic():
+    a, b = W.config2(16), W.config2(16)
+    assert np.array_equal(a.top.roots, b.top.roots) and np.array_equal(a.bottom.powers, b.bottom.powers)
+    assert set(np.unique(a.top.powers)) <= {1.0, 2.0}
+    c1 = W.config1()
+    r = np.linalg.norm(c1.top.roots[:, :3], axis=1)
+    assert np.all((r >= 0.5) & (r <= 1.0)) and np.all(c1.top.roots[:, 3] == 0)
