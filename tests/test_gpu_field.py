@@ -113,7 +113,9 @@ def test_fp32_band_config2(gpu, res):
     flips = signflips(g, c)
     fin = np.isfinite(c) & np.isfinite(g)
     err = np.abs(g[fin] - c[fin])
-    inband = np.abs(c) < 0.05
+    # voxels whose FP32 value fell inside the band are FP64-exact; those are at
+    # least all voxels with |F| < band - (FP32 error), here taken as 0.01
+    inband = np.abs(c) < 0.05 - 0.01
     print("fp32band res=%d flips=%d max|dF|=%.3e band=%d" % (res, flips, err.max(), inband.sum()), flush=True)
     assert flips == 0
     assert np.array_equal(np.isfinite(g), np.isfinite(c))
