@@ -733,7 +733,11 @@ void qo_ap_score_gradients(const qo_grid* g, const qo_poly* top, const qo_poly* 
         /* gradients */
         Q R = qdiv(topEval, bottomEval);
         double Rmagnitude = qmag(R);
-        int fp64_ok = !ovf && !(Rmagnitude <= 0.0 || isinf(Rmagnitude));
+        /* a non-overflowed cell whose |R| is 0 or inf is skipped (OPTIMIZE_3D.cpp:8956);
+         * overflowed cells are redone in long double; in FP64 cells a non-finite
+         * term is redone in long double */
+        const int skip_cell = !ovf && (Rmagnitude <= 0.0 || isinf(Rmagnitude) || isnan(Rmagnitude));
+        int fp64_ok = !ovf && !skip_cell;
         double Fg = log(Rmagnitude) + log(p->exp_scaling);
         double tanhTerm = tanh(tanh_alpha * (Fg - tanh_threshold));
         double dSdF = -tanh_alpha * w * (1 - tanhTerm * tanhTerm);
@@ -756,6 +760,7 @@ void qo_ap_score_gradients(const qo_grid* g, const qo_poly* top, const qo_poly* 
               dSdFe = -(long double)tanh_alpha * w * (1 - tt * tt);                        \
             }                                                                              \
           } while (0)
+        if (skip_cell) continue;
         if (!fp64_ok) ENSURE_E();
         for (int i = 0; i < nt; i++) {
           for (int bulk = 0; bulk < 2; bulk++) {

@@ -277,3 +277,42 @@ class TRIANGLE_MESH:
             for t in self._triangles:
                 f.write("f %i %i %i\n" % (t[0] + 1, t[1] + 1, t[2] + 1))
         return True
+
+
+def ap_score_gradients(grid, rational, params, wgrid, weight, tanh_alpha=10.0, tanh_threshold=0.0, field=None,
+                       workspace=None, stream=None):
+    """One-iteration AP score and exponent gradients (qj_ap_score_gradients).
+
+    weight: float64 CUDA tensor on wgrid (x fastest). field: float64 CUDA tensor
+    that receives the AP field (cells at the origin are left as they are).
+    Returns a dict with score, grad_exp, grad_top, grad_bottom, grad_top_scalar,
+    grad_bottom_scalar, adaptive_cells and the field tensor."""
+    n = grid.res[0] * grid.res[1] * grid.res[2]
+    if field is None:
+        field = torch.zeros(n, dtype=torch.float64, device="cuda")
+    w = weight.contiguous()
+    nt, nb = rational.top.totalRoots(), rational.bottom.totalRoots()
+    gt = np.zeros(max(nt, 1))
+    gb = np.zeros(max(nb, 1))
+    out = _lib.APGrads()
+    ws = workspace or default_workspace()
+    check(_lib.load().qj_ap_score_gradients(
+        ws.handle, ctypes.byref(grid), rational.handle, ctypes.byref(params), ctypes.byref(wgrid), _ptr(w),
+        float(tanh_alpha), float(tanh_threshold), _ptr(field), gt.ctypes.data_as(ctypes.c_void_p),
+        gb.ctypes.data_as(ctypes.c_void_p), ctypes.byref(out), _stream_ptr(stream)))
+    return dict(score=out.score, grad_exp=out.grad_exp, grad_top_scalar=out.grad_top_scalar,
+                grad_bottom_scalar=out.grad_bottom_scalar, adaptive_cells=out.adaptive_cells,
+                grad_top=gt[:nt].copy(), grad_bottom=gb[:nb].copy(), field=field)
+
+
+def points_adjoint(points, rational, params, dJdF, precision=FP64, workspace=None, stream=None):
+    """qj_points_adjoint: dJ/d(roots) ((nt+nb) x 4, top first) and dJ/d(powers) (nt+nb)."""
+    pts = points.contiguous()
+    g = dJdF.contiguous().to(torch.float64)
+    nr = rational.top.totalRoots() + rational.bottom.totalRoots()
+    gr = torch.empty((nr, 4), dtype=torch.float64, device=pts.device)
+    gp = torch.empty(nr, dtype=torch.float64, device=pts.device)
+    ws = workspace or default_workspace()
+    check(_lib.load().qj_points_adjoint(ws.handle, _ptr(pts), pts.shape[0], rational.handle, ctypes.byref(params),
+                                        int(precision), _ptr(g), _ptr(gr), _ptr(gp), _stream_ptr(stream)))
+    return gr, gp
