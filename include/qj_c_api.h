@@ -244,6 +244,9 @@ double qj_server_now(qj_server* s); /* seconds on the server's monotonic clock *
 qj_status qj_server_submit(qj_server* s, const qj_request* req, uint64_t* id);
 /* Up to max completions; waits up to timeout_s for at least one (0 = poll). */
 qj_status qj_server_poll(qj_server* s, qj_completion* out, int32_t max, int32_t* n_out, double timeout_s);
+/* Copies a kept field (completion->voxels doubles, x fastest) to dst (host or
+ * device); synchronous. Call before qj_server_release of that request. */
+qj_status qj_server_fetch(qj_server* s, const qj_completion* c, double* dst, void* stream);
 qj_status qj_server_release(qj_server* s, uint64_t id); /* frees a kept field */
 qj_status qj_server_destroy(qj_server* s);
 

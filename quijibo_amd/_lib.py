@@ -105,6 +105,7 @@ SIGNATURES = [
     ("qj_server_submit", ctypes.c_int32, [_P, ctypes.POINTER(Request), ctypes.POINTER(ctypes.c_uint64)]),
     ("qj_server_poll", ctypes.c_int32, [_P, ctypes.POINTER(Completion), ctypes.c_int32,
                                         ctypes.POINTER(ctypes.c_int32), ctypes.c_double]),
+    ("qj_server_fetch", ctypes.c_int32, [_P, ctypes.POINTER(Completion), _P, _P]),
     ("qj_server_release", ctypes.c_int32, [_P, ctypes.c_uint64]),
     ("qj_server_destroy", ctypes.c_int32, [_P]),
 ]

@@ -490,6 +490,85 @@ __global__ __launch_bounds__(kBlock) void k_list64(MapArgs a, const RootRec64* _
   block_count(iters, cnt, slot_iter);
 }
 
+// ---------------------------------------------------------------- batched chunks (request loop)
+
+__global__ void k_zero_u32(uint32_t* p) { *p = 0u; }
+
+__device__ __forceinline__ q64 chunk_start(const MapArgs& a, int64_t idx) { return GridSource::start(a, nullptr, idx); }
+
+// blocks_per_chunk blocks per chunk descriptor; each lane one voxel of its chunk
+__global__ __launch_bounds__(kBlock) void k_batch32(const ChunkDesc* __restrict__ descs, int blocks_per_chunk,
+                                                    int chunk_voxels, uint32_t* __restrict__ blist,
+                                                    uint32_t* __restrict__ bcount) {
+  const int c = blockIdx.x / blocks_per_chunk;
+  const int local = (blockIdx.x - c * blocks_per_chunk) * kBlock + threadIdx.x;
+  const ChunkDesc& d = descs[c];
+  bool need = false;
+  if (local < d.count) {
+    const MapArgs& a = d.a;
+    Consts32 k;
+    k.log2c = (float)log2(a.exp_scaling);
+    k.log2esc = (float)log2(a.escape);
+    k.log2bh = (float)log2(kBlackHole);
+    k.margin = 0.02f;
+    k.band = d.band;
+    const int64_t idx = d.start + local;
+    const Res32 r = eval32(a, d.r32, k, chunk_start(a, idx));
+    if (r.written) {
+      if (!r.need64) d.out[idx] = (double)r.F;
+      need = r.need64;
+    }
+  }
+  block_append(need, (uint32_t)(c * chunk_voxels + local), blist, bcount);
+}
+
+// FP64 for every voxel of every chunk (list == nullptr) or for the band list
+__global__ __launch_bounds__(kBlock) void k_batch64(const ChunkDesc* __restrict__ descs, int blocks_per_chunk,
+                                                    int chunk_voxels, const uint32_t* __restrict__ list,
+                                                    const uint32_t* __restrict__ count) {
+  if (list) {
+    const int64_t n = *count;
+    for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += (int64_t)gridDim.x * kBlock) {
+      const uint32_t e = list[i];
+      const int c = (int)(e / (uint32_t)chunk_voxels);
+      const int local = (int)(e - (uint32_t)c * (uint32_t)chunk_voxels);
+      const ChunkDesc& d = descs[c];
+      const int64_t idx = d.start + local;
+      const Res64 r = eval64(d.a, d.r64, chunk_start(d.a, idx));
+      if (r.written) d.out[idx] = r.F;
+    }
+    return;
+  }
+  const int c = blockIdx.x / blocks_per_chunk;
+  const int local = (blockIdx.x - c * blocks_per_chunk) * kBlock + threadIdx.x;
+  const ChunkDesc& d = descs[c];
+  if (local < d.count) {
+    const int64_t idx = d.start + local;
+    const Res64 r = eval64(d.a, d.r64, chunk_start(d.a, idx));
+    if (r.written) d.out[idx] = r.F;
+  }
+}
+
+qj_status launch_batch(const ChunkDesc* d_desc, int nchunks, int chunk_voxels, qj_precision prec, uint32_t* d_band,
+                       uint32_t* d_band_count, hipStream_t s) {
+  const int bpc = (chunk_voxels + kBlock - 1) / kBlock;
+  const unsigned grid = (unsigned)(bpc * nchunks);
+  if (prec == QJ_FP64) {
+    hipLaunchKernelGGL(k_batch64, dim3(grid), dim3(kBlock), 0, s, d_desc, bpc, chunk_voxels,
+                       (const uint32_t*)nullptr, (const uint32_t*)nullptr);
+    QJ_HIP(hipGetLastError());
+    return QJ_OK;
+  }
+  hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(1), 0, s, d_band_count);
+  QJ_HIP(hipGetLastError());
+  hipLaunchKernelGGL(k_batch32, dim3(grid), dim3(kBlock), 0, s, d_desc, bpc, chunk_voxels, d_band, d_band_count);
+  QJ_HIP(hipGetLastError());
+  hipLaunchKernelGGL(k_batch64, dim3(2048), dim3(kBlock), 0, s, d_desc, bpc, chunk_voxels, (const uint32_t*)d_band,
+                     (const uint32_t*)d_band_count);
+  QJ_HIP(hipGetLastError());
+  return QJ_OK;
+}
+
 // ---------------------------------------------------------------- launchers
 
 static Consts32 consts32(const MapArgs& a, double band) {

@@ -116,6 +116,23 @@ qj_status launch_field(qj_workspace* ws, const MapArgs& a, const qj_rational* f,
 qj_status launch_points(qj_workspace* ws, const MapArgs& a, const qj_rational* f, qj_precision prec,
                         double band, const double* d_pts, int64_t n, double* d_out, hipStream_t s);
 
+// batched chunks for the request loop (qj_field.hip)
+struct ChunkDesc {
+  MapArgs a;
+  const RootRec64* r64;
+  const RootRec32* r32;
+  double* out;
+  int64_t start;  // first voxel (linear, x fastest) of this chunk in its request
+  int32_t count;  // voxels in this chunk (0 = unused slot)
+  float band;
+};
+// Enqueue one batch: FP32 bulk over every chunk then FP64 for the band (or
+// FP64 for all when prec == QJ_FP64). d_band/d_band_count: scratch of
+// nchunks * chunk_voxels entries; the count is cleared by a kernel, not a
+// memset, so the sequence can be captured in a graph and replayed.
+qj_status launch_batch(const ChunkDesc* d_desc, int nchunks, int chunk_voxels, qj_precision prec, uint32_t* d_band,
+                       uint32_t* d_band_count, hipStream_t s);
+
 // marching cubes (qj_mc.hip)
 qj_status mc_run(qj_workspace* ws, const double* d_field, const qj_grid* g, double iso, int64_t* nv,
                  int64_t* nt, hipStream_t s);

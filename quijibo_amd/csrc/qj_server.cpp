@@ -1,11 +1,530 @@
-// Request loop (filled in below).
+// Request loop: continuous batching of field solves on one GPU.
+//
+// Requests (any grid size, any root count up to max_roots) are copied on
+// submit and wait in a pending list. A scheduler thread admits them while
+// their HBM footprint (field + root records) fits a preallocated arena, splits
+// admitted requests into fixed-size voxel chunks and packs chunks of several
+// requests into one launch, shortest remaining work first. Each launch is a
+// hipGraph captured once per in-flight slot ([descriptor upload, FP32 bulk,
+// FP64 band] or [descriptor upload, FP64]) and replayed with new descriptors;
+// requests that arrive while launches are on the GPU join the next launch.
+// Fields stay resident in the arena until completion (or until released when
+// the caller asked to keep them). The reference has no serving path; its
+// analogue is one OPTIMIZE_3D::computeLogScaledPowerRationalMap call per solve
+// (OPTIMIZE_3D.cpp:1272-1370).
+#include <math.h>
+#include <string.h>
+
+#include <algorithm>
+#include <atomic>
+#include <chrono>
+#include <condition_variable>
+#include <deque>
+#include <map>
+#include <memory>
+#include <thread>
+#include <unordered_map>
+
 #include "qj_internal.h"
 
-extern "C" {
-qj_status qj_server_create(const qj_server_config*, qj_server**) { return QJ_ERR_INVALID; }
-double qj_server_now(qj_server*) { return 0; }
-qj_status qj_server_submit(qj_server*, const qj_request*, uint64_t*) { return QJ_ERR_INVALID; }
-qj_status qj_server_poll(qj_server*, qj_completion*, int32_t, int32_t*, double) { return QJ_ERR_INVALID; }
-qj_status qj_server_release(qj_server*, uint64_t) { return QJ_ERR_INVALID; }
-qj_status qj_server_destroy(qj_server*) { return QJ_ERR_INVALID; }
+namespace qj {
+
+namespace {
+
+// First-fit allocator over one device allocation (offsets, 256 B aligned).
+class Arena {
+ public:
+  bool init(size_t bytes) {
+    if (hipMalloc(&base_, bytes) != hipSuccess) return false;
+    size_ = bytes;
+    free_[0] = bytes;
+    return true;
+  }
+  void release() {
+    if (base_) (void)hipFree(base_);
+    base_ = nullptr;
+  }
+  bool alloc(size_t n, size_t* off) {
+    n = (n + 255) & ~(size_t)255;
+    for (auto it = free_.begin(); it != free_.end(); ++it) {
+      if (it->second >= n) {
+        *off = it->first;
+        const size_t rest = it->second - n;
+        const size_t o = it->first;
+        free_.erase(it);
+        if (rest) free_[o + n] = rest;
+        used_ += n;
+        return true;
+      }
+    }
+    return false;
+  }
+  void free(size_t off, size_t n) {
+    n = (n + 255) & ~(size_t)255;
+    used_ -= n;
+    auto it = free_.emplace(off, n).first;
+    auto nx = std::next(it);
+    if (nx != free_.end() && it->first + it->second == nx->first) {
+      it->second += nx->second;
+      free_.erase(nx);
+    }
+    if (it != free_.begin()) {
+      auto pv = std::prev(it);
+      if (pv->first + pv->second == it->first) {
+        pv->second += it->second;
+        free_.erase(it);
+      }
+    }
+  }
+  char* base() const { return (char*)base_; }
+  size_t size() const { return size_; }
+
+ private:
+  void* base_ = nullptr;
+  size_t size_ = 0, used_ = 0;
+  std::map<size_t, size_t> free_;
+};
+
+struct Req {
+  uint64_t id = 0;
+  qj_grid grid{};
+  qj_map_params params{};
+  int keep = 0;
+  double arrival = 0, admit = 0, done = 0;
+  std::vector<RootRec64> h64;
+  std::vector<RootRec32> h32;
+  int nt = 0, nb = 0;
+  double sct = 1, scb = 1;
+  int64_t voxels = 0, next = 0, chunks = 0, chunks_done = 0;
+  size_t off = 0, bytes = 0, off64 = 0, off32 = 0, off_field = 0;
+  bool admitted = false;
+  qj_status status = QJ_OK;
+  MapArgs args{};
+};
+
+struct Slot {
+  ChunkDesc* h_desc = nullptr;
+  ChunkDesc* d_desc = nullptr;
+  uint32_t* d_band = nullptr;
+  uint32_t* d_band_count = nullptr;
+  hipGraphExec_t exec = nullptr;
+  hipEvent_t done = nullptr;
+  bool busy = false;
+  std::vector<std::pair<uint64_t, int>> work;  // (request id, chunks in this launch)
+};
+
+}  // namespace
+
+}  // namespace qj
+
+struct qj_server {
+  qj_server_config cfg{};
+  hipStream_t stream = nullptr;
+  qj::Arena arena;
+  std::vector<qj::Slot> slots;
+  std::chrono::steady_clock::time_point t0;
+  std::thread worker;
+  std::mutex mu;
+  std::condition_variable cv_submit, cv_done;
+  std::deque<std::unique_ptr<qj::Req>> incoming;
+  std::deque<qj_completion> completions;
+  std::atomic<bool> stop{false};
+  std::atomic<uint64_t> next_id{1};
+  // owned by the worker
+  std::vector<std::unique_ptr<qj::Req>> pending;              // not admitted yet
+  std::unordered_map<uint64_t, std::unique_ptr<qj::Req>> live; // admitted, running
+  std::unordered_map<uint64_t, std::unique_ptr<qj::Req>> kept; // finished, field kept
+  std::vector<uint64_t> release_q;
+  size_t slot_next = 0;
+  qj_status fatal = QJ_OK;
+
+  double now() const {
+    return std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+  }
+};
+
+namespace qj {
+
+static void finish(qj_server* s, Req* r, qj_status st) {
+  r->done = s->now();
+  qj_completion c{};
+  c.id = r->id;
+  c.status = st;
+  c.arrival_time = r->arrival;
+  c.admit_time = r->admit;
+  c.done_time = r->done;
+  c.voxels = r->voxels;
+  c.d_field = (r->keep && st == QJ_OK && r->admitted) ? (const double*)(s->arena.base() + r->off_field) : nullptr;
+  std::lock_guard<std::mutex> lk(s->mu);
+  s->completions.push_back(c);
+  s->cv_done.notify_all();
 }
+
+static bool try_admit(qj_server* s, Req* r) {
+  size_t off;
+  if (!s->arena.alloc(r->bytes, &off)) return false;
+  r->off = off;
+  r->off_field = off;
+  r->off64 = off + (((size_t)r->voxels * sizeof(double) + 255) & ~(size_t)255);
+  r->off32 = r->off64 + ((r->h64.size() * sizeof(RootRec64) + 255) & ~(size_t)255);
+  char* b = s->arena.base();
+  if (hipMemcpyAsync(b + r->off64, r->h64.data(), r->h64.size() * sizeof(RootRec64), hipMemcpyHostToDevice,
+                     s->stream) != hipSuccess ||
+      hipMemcpyAsync(b + r->off32, r->h32.data(), r->h32.size() * sizeof(RootRec32), hipMemcpyHostToDevice,
+                     s->stream) != hipSuccess) {
+    s->arena.free(off, r->bytes);
+    return false;
+  }
+  r->admitted = true;
+  r->admit = s->now();
+  return true;
+}
+
+// Fills slot descriptors with up to chunks_per_launch chunks, shortest remaining work first.
+static int build_batch(qj_server* s, Slot& sl) {
+  std::vector<Req*> order;
+  for (auto& kv : s->live)
+    if (kv.second->next < kv.second->voxels) order.push_back(kv.second.get());
+  std::sort(order.begin(), order.end(), [](const Req* a, const Req* b) {
+    const int64_t ra = a->voxels - a->next, rb = b->voxels - b->next;
+    return ra != rb ? ra < rb : a->arrival < b->arrival;
+  });
+  const int maxc = s->cfg.chunks_per_launch, cv = s->cfg.chunk_voxels;
+  int n = 0;
+  sl.work.clear();
+  char* b = s->arena.base();
+  for (Req* r : order) {
+    int taken = 0;
+    while (n < maxc && r->next < r->voxels) {
+      ChunkDesc& d = sl.h_desc[n++];
+      d.a = r->args;
+      d.r64 = (const RootRec64*)(b + r->off64);
+      d.r32 = (const RootRec32*)(b + r->off32);
+      d.out = (double*)(b + r->off_field);
+      d.start = r->next;
+      const int64_t left = r->voxels - r->next;
+      d.count = (int32_t)(left < cv ? left : cv);
+      d.band = (float)s->cfg.band;
+      r->next += d.count;
+      taken++;
+    }
+    if (taken) sl.work.emplace_back(r->id, taken);
+    if (n == maxc) break;
+  }
+  for (int i = n; i < maxc; i++) sl.h_desc[i].count = 0;
+  return n;
+}
+
+static void complete_slot(qj_server* s, Slot& sl, qj_status st) {
+  for (auto& w : sl.work) {
+    auto it = s->live.find(w.first);
+    if (it == s->live.end()) continue;
+    Req* r = it->second.get();
+    r->chunks_done += w.second;
+    if (st != QJ_OK) r->status = st;
+    if (r->chunks_done >= r->chunks) {
+      finish(s, r, r->status);
+      if (r->keep && r->status == QJ_OK) {
+        s->kept[r->id] = std::move(it->second);
+      } else {
+        s->arena.free(r->off, r->bytes);
+      }
+      s->live.erase(it);
+    }
+  }
+  sl.work.clear();
+  sl.busy = false;
+}
+
+static void worker_main(qj_server* s) {
+  hipSetDevice(s->cfg.device);
+  while (true) {
+    // 1. take submissions and releases
+    {
+      std::unique_lock<std::mutex> lk(s->mu);
+      while (!s->incoming.empty()) {
+        s->pending.push_back(std::move(s->incoming.front()));
+        s->incoming.pop_front();
+      }
+      for (uint64_t id : s->release_q) {
+        auto it = s->kept.find(id);
+        if (it != s->kept.end()) {
+          s->arena.free(it->second->off, it->second->bytes);
+          s->kept.erase(it);
+        }
+      }
+      s->release_q.clear();
+    }
+    // 2. admit, smallest footprint first, while the arena has room
+    if (!s->pending.empty()) {
+      std::stable_sort(s->pending.begin(), s->pending.end(),
+                       [](const std::unique_ptr<Req>& a, const std::unique_ptr<Req>& b) { return a->bytes < b->bytes; });
+      std::vector<std::unique_ptr<Req>> keep;
+      for (auto& r : s->pending) {
+        if (r->bytes > s->arena.size()) {
+          finish(s, r.get(), QJ_ERR_CAPACITY);
+          continue;
+        }
+        if (try_admit(s, r.get())) {
+          const uint64_t id = r->id;
+          s->live[id] = std::move(r);
+        } else {
+          keep.push_back(std::move(r));
+        }
+      }
+      s->pending.swap(keep);
+    }
+    // 3. retire finished launches
+    for (Slot& sl : s->slots)
+      if (sl.busy && hipEventQuery(sl.done) == hipSuccess) complete_slot(s, sl, QJ_OK);
+    // 4. launch while a slot is free and there is work
+    bool launched = false;
+    bool has_work = false;
+    for (auto& kv : s->live)
+      if (kv.second->next < kv.second->voxels) has_work = true;
+    if (has_work) {
+      for (size_t k = 0; k < s->slots.size(); k++) {
+        Slot& sl = s->slots[(s->slot_next + k) % s->slots.size()];
+        if (sl.busy) continue;
+        if (build_batch(s, sl) == 0) break;
+        hipError_t e = hipGraphLaunch(sl.exec, s->stream);
+        if (e == hipSuccess) e = hipEventRecord(sl.done, s->stream);
+        sl.busy = true;
+        if (e != hipSuccess) {
+          complete_slot(s, sl, QJ_ERR_HIP);
+        }
+        s->slot_next = (s->slot_next + k + 1) % s->slots.size();
+        launched = true;
+        break;
+      }
+    }
+    if (launched) continue;
+    bool any_busy = false;
+    for (Slot& sl : s->slots) any_busy |= sl.busy;
+    if (s->stop.load() && !any_busy && s->live.empty()) {
+      std::lock_guard<std::mutex> lk(s->mu);
+      if (s->incoming.empty()) break;
+    }
+    if (any_busy) {
+      // all slots busy (or no new work): wait briefly for the oldest launch or a submission
+      std::this_thread::sleep_for(std::chrono::microseconds(20));
+    } else {
+      std::unique_lock<std::mutex> lk(s->mu);
+      s->cv_submit.wait_for(lk, std::chrono::milliseconds(1),
+                            [s] { return !s->incoming.empty() || !s->release_q.empty() || s->stop.load(); });
+    }
+  }
+  // fail whatever never got admitted
+  for (auto& r : s->pending) finish(s, r.get(), QJ_ERR_SHUTDOWN);
+  s->pending.clear();
+}
+
+static qj_status capture_slot(qj_server* s, Slot& sl) {
+  const int nch = s->cfg.chunks_per_launch;
+  QJ_HIP(hipHostMalloc((void**)&sl.h_desc, sizeof(ChunkDesc) * nch, hipHostMallocDefault));
+  memset(sl.h_desc, 0, sizeof(ChunkDesc) * nch);
+  QJ_HIP(hipMalloc(&sl.d_desc, sizeof(ChunkDesc) * nch));
+  QJ_HIP(hipMemset(sl.d_desc, 0, sizeof(ChunkDesc) * nch));
+  if (s->cfg.precision != QJ_FP64) {
+    QJ_HIP(hipMalloc(&sl.d_band, sizeof(uint32_t) * (size_t)nch * s->cfg.chunk_voxels));
+    QJ_HIP(hipMalloc(&sl.d_band_count, sizeof(uint32_t)));
+  }
+  QJ_HIP(hipEventCreateWithFlags(&sl.done, hipEventDisableTiming));
+  hipGraph_t g;
+  QJ_HIP(hipStreamBeginCapture(s->stream, hipStreamCaptureModeThreadLocal));
+  hipError_t e = hipMemcpyAsync(sl.d_desc, sl.h_desc, sizeof(ChunkDesc) * nch, hipMemcpyHostToDevice, s->stream);
+  qj_status st = QJ_OK;
+  if (e == hipSuccess)
+    st = launch_batch(sl.d_desc, nch, s->cfg.chunk_voxels, (qj_precision)s->cfg.precision, sl.d_band,
+                      sl.d_band_count, s->stream);
+  hipError_t e2 = hipStreamEndCapture(s->stream, &g);
+  if (e != hipSuccess) return hip_fail(e, "capture memcpy");
+  if (st) return st;
+  if (e2 != hipSuccess) return hip_fail(e2, "hipStreamEndCapture");
+  e = hipGraphInstantiate(&sl.exec, g, nullptr, nullptr, 0);
+  (void)hipGraphDestroy(g);
+  if (e != hipSuccess) return hip_fail(e, "hipGraphInstantiate");
+  return QJ_OK;
+}
+
+static void destroy_server(qj_server* s) {
+  for (Slot& sl : s->slots) {
+    if (sl.exec) (void)hipGraphExecDestroy(sl.exec);
+    if (sl.done) (void)hipEventDestroy(sl.done);
+    if (sl.h_desc) (void)hipHostFree(sl.h_desc);
+    if (sl.d_desc) (void)hipFree(sl.d_desc);
+    if (sl.d_band) (void)hipFree(sl.d_band);
+    if (sl.d_band_count) (void)hipFree(sl.d_band_count);
+  }
+  s->arena.release();
+  if (s->stream) (void)hipStreamDestroy(s->stream);
+  delete s;
+}
+
+}  // namespace qj
+
+using namespace qj;
+
+extern "C" {
+
+qj_status qj_server_create(const qj_server_config* cfg, qj_server** out) {
+  if (!out) return QJ_ERR_INVALID;
+  *out = nullptr;
+  qj_server* s = new qj_server();
+  if (cfg) s->cfg = *cfg;
+  qj_server_config& c = s->cfg;
+  if (c.hbm_budget_bytes <= 0) c.hbm_budget_bytes = (int64_t)8 << 30;
+  if (c.chunk_voxels <= 0) c.chunk_voxels = 65536;
+  if (c.chunks_per_launch <= 0) c.chunks_per_launch = 512;
+  if (c.max_roots <= 0) c.max_roots = 256;
+  if (c.inflight_launches <= 0) c.inflight_launches = 3;
+  if (c.band <= 0) c.band = 0.05;
+  if (c.precision != QJ_FP64 && c.precision != QJ_FP32_BAND) {
+    delete s;
+    set_error("qj_server_create: unknown precision");
+    return QJ_ERR_INVALID;
+  }
+  if ((int64_t)c.chunk_voxels * c.chunks_per_launch > (int64_t)UINT32_MAX) {
+    delete s;
+    set_error("qj_server_create: chunk_voxels * chunks_per_launch must stay below 2^32");
+    return QJ_ERR_INVALID;
+  }
+  hipError_t e = hipSetDevice(c.device);
+  if (e == hipSuccess) e = hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking);
+  if (e != hipSuccess) {
+    delete s;
+    return hip_fail(e, "qj_server_create");
+  }
+  if (!s->arena.init((size_t)c.hbm_budget_bytes)) {
+    destroy_server(s);
+    set_error("qj_server_create: cannot allocate %lld bytes of HBM", (long long)c.hbm_budget_bytes);
+    return QJ_ERR_NOMEM;
+  }
+  s->slots.resize(c.inflight_launches);
+  for (Slot& sl : s->slots) {
+    qj_status st = capture_slot(s, sl);
+    if (st) {
+      destroy_server(s);
+      return st;
+    }
+  }
+  s->t0 = std::chrono::steady_clock::now();
+  s->worker = std::thread(worker_main, s);
+  *out = s;
+  return QJ_OK;
+}
+
+double qj_server_now(qj_server* s) { return s ? s->now() : 0.0; }
+
+qj_status qj_server_submit(qj_server* s, const qj_request* req, uint64_t* id) {
+  if (!s || !req) return QJ_ERR_INVALID;
+  if (s->stop.load()) return QJ_ERR_SHUTDOWN;
+  const qj_poly& t = req->top;
+  const qj_poly& b = req->bottom;
+  if (t.n_roots < 1 || b.n_roots < 0 || t.n_roots > s->cfg.max_roots || b.n_roots > s->cfg.max_roots ||
+      !t.roots || !t.powers || (b.n_roots && (!b.roots || !b.powers))) {
+    set_error("qj_server_submit: root counts must be 1..%d (top) and 0..%d (bottom)", s->cfg.max_roots,
+              s->cfg.max_roots);
+    return QJ_ERR_INVALID;
+  }
+  if (req->params.flags & QJ_MAP_ADAPTIVE) {
+    set_error("qj_server_submit: QJ_MAP_ADAPTIVE is not served by the request loop");
+    return QJ_ERR_INVALID;
+  }
+  auto r = std::make_unique<Req>();
+  r->grid = req->grid;
+  r->params = req->params;
+  r->keep = req->keep_field;
+  r->arrival = req->arrival_time;
+  // the same root records qj_rational_create builds
+  for (int k = 0; k < 2; k++) {
+    const qj_poly& p = k ? b : t;
+    for (int i = 0; i < p.n_roots; i++) {
+      RootRec64 x;
+      x.w = p.roots[4 * i];
+      x.x = p.roots[4 * i + 1];
+      x.y = p.roots[4 * i + 2];
+      x.z = p.roots[4 * i + 3];
+      x.p = p.powers[i];
+      x.e = p.power_scalar * p.powers[i];
+      r->h64.push_back(x);
+      RootRec32 y;
+      y.w = (float)x.w;
+      y.x = (float)x.x;
+      y.y = (float)x.y;
+      y.z = (float)x.z;
+      y.e = (float)x.e;
+      y.kind = (x.e == 1.0) ? 1 : (x.e == 2.0 ? 2 : 0);
+      y.pad0 = y.pad1 = 0;
+      r->h32.push_back(y);
+    }
+  }
+  r->nt = t.n_roots;
+  r->nb = b.n_roots;
+  qj_rational f;
+  f.nt = r->nt;
+  f.nb = r->nb;
+  f.sc_top = t.power_scalar;
+  f.sc_bottom = b.power_scalar;
+  qj_status st = make_map_args(&r->grid, &f, &r->params, &r->args);
+  if (st) return st;
+  r->voxels = (int64_t)r->grid.res[0] * r->grid.res[1] * r->grid.res[2];
+  r->chunks = (r->voxels + s->cfg.chunk_voxels - 1) / s->cfg.chunk_voxels;
+  r->bytes = (((size_t)r->voxels * sizeof(double) + 255) & ~(size_t)255) +
+             ((r->h64.size() * sizeof(RootRec64) + 255) & ~(size_t)255) + r->h32.size() * sizeof(RootRec32);
+  r->id = s->next_id.fetch_add(1);
+  if (id) *id = r->id;
+  {
+    std::lock_guard<std::mutex> lk(s->mu);
+    s->incoming.push_back(std::move(r));
+  }
+  s->cv_submit.notify_one();
+  return QJ_OK;
+}
+
+qj_status qj_server_poll(qj_server* s, qj_completion* out, int32_t max, int32_t* n_out, double timeout_s) {
+  if (!s || !n_out || (max > 0 && !out)) return QJ_ERR_INVALID;
+  std::unique_lock<std::mutex> lk(s->mu);
+  if (s->completions.empty() && timeout_s > 0)
+    s->cv_done.wait_for(lk, std::chrono::duration<double>(timeout_s), [s] { return !s->completions.empty(); });
+  int n = 0;
+  while (n < max && !s->completions.empty()) {
+    out[n++] = s->completions.front();
+    s->completions.pop_front();
+  }
+  *n_out = n;
+  return QJ_OK;
+}
+
+qj_status qj_server_fetch(qj_server* s, const qj_completion* c, double* dst, void* stream) {
+  if (!s || !c || !dst || !c->d_field) {
+    set_error("qj_server_fetch: no kept field in this completion");
+    return QJ_ERR_INVALID;
+  }
+  QJ_HIP(hipMemcpyAsync(dst, c->d_field, sizeof(double) * (size_t)c->voxels, hipMemcpyDefault,
+                        (hipStream_t)stream));
+  QJ_HIP(hipStreamSynchronize((hipStream_t)stream));
+  return QJ_OK;
+}
+
+qj_status qj_server_release(qj_server* s, uint64_t id) {
+  if (!s) return QJ_ERR_INVALID;
+  {
+    std::lock_guard<std::mutex> lk(s->mu);
+    s->release_q.push_back(id);
+  }
+  s->cv_submit.notify_one();
+  return QJ_OK;
+}
+
+qj_status qj_server_destroy(qj_server* s) {
+  if (!s) return QJ_OK;
+  s->stop.store(true);
+  s->cv_submit.notify_all();
+  if (s->worker.joinable()) s->worker.join();
+  (void)hipStreamSynchronize(s->stream);
+  destroy_server(s);
+  return QJ_OK;
+}
+
+}  // extern "C"
