@@ -70,6 +70,9 @@ def main():
     ap.add_argument("--cpu-slabs", type=int, default=24)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--band", type=float, default=0.05)
+    ap.add_argument("--serve", type=int, default=2000, help="requests in the serving stream (0 = skip)")
+    ap.add_argument("--serve-rate", type=float, default=200.0)
+    ap.add_argument("--serve-budget-gib", type=float, default=64.0)
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build runs on a single MI355X and has no multi-GPU path "
@@ -154,6 +157,9 @@ def main():
                "parity_on_sample": {"sign_flips_fp64_gpu": cb["flips_fp64"],
                                     "sign_flips_fp32band_gpu": cb["flips_fp32band"],
                                     "max_rel_err_fp64_gpu": cb["max_rel_err_fp64"]}}
+    serving = None
+    if args.serve > 0:
+        serving = serve_stream(args, log)
     line = {
         "metric": "Gvoxel-iter/s at 512^3 (FP32+FP64 band vs FP64) + p50/p99 request latency",
         "value": round(value, 4),
@@ -179,8 +185,35 @@ def main():
         "parity": {"sign_flips_vs_fp64_gpu": flips_vs_fp64, "max_abs_dF_vs_fp64_gpu": maxdf},
         "roofline": roofline,
         "cpu_baseline": cpu,
+        "serving": serving,
+        "p50_ms_256": (serving or {}).get("latency", {}).get("256^3", {}).get("p50_ms") if serving else None,
+        "p99_ms_256": (serving or {}).get("latency", {}).get("256^3", {}).get("p99_ms") if serving else None,
     }
     print(json.dumps(line), flush=True)
+
+
+def serve_stream(args, log):
+    """Config 4: Poisson arrivals at serve_rate, grid edge in {64,128,256,512}, 8-256 roots
+    (half top, half bottom, Gaussian sigma 0.7), maxIter 3-5, through the request loop."""
+    from quijibo_amd import server as S, workloads as W
+    from quijibo_amd._lib import FP32_BAND
+    reqs = W.config4_requests(n=args.serve, rate=args.serve_rate)
+    srv = S.Server(hbm_budget_bytes=int(args.serve_budget_gib * (1 << 30)), precision=FP32_BAND, band=args.band)
+    # warm the graphs with one small request
+    srv.submit(reqs[0][1])
+    srv.wait_all(1, timeout=60)
+    log("serving: %d requests at %.0f req/s" % (len(reqs), args.serve_rate))
+    done, ids, start = S.replay(srv, reqs, log)
+    end = max(c.done_time for c in done)
+    srv.close()
+    vox_iters = sum(ids[c.id].voxels * ids[c.id].max_iter for c in done if c.status == 0)
+    lat = S.latency_summary(done, ids)
+    return {"requests": len(reqs), "offered_rate_req_s": args.serve_rate, "completed": sum(c.status == 0 for c in done),
+            "wall_s": round(end - start, 3), "throughput_req_s": round(len(done) / (end - start), 2),
+            "throughput_gvoxel_iter_s": round(vox_iters / (end - start) / 1e9, 3), "latency": lat,
+            "config": "hbm budget %.0f GiB, 65536-voxel chunks, launches of <= 512 chunks and ~6e9 root-term "
+                      "iterations, 2 launches in flight, fp32+fp64 band, shortest-remaining-work-first"
+                      % args.serve_budget_gib}
 
 
 def _kernel_ms(api, ws):

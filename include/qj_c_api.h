@@ -213,10 +213,12 @@ typedef struct qj_server_config {
   int32_t chunk_voxels;       /* voxels per chunk (0 = 65536) */
   int32_t chunks_per_launch;  /* chunks per graph launch (0 = 512) */
   int32_t max_roots;          /* largest root count per polynomial (0 = 256) */
-  int32_t inflight_launches;  /* launches in flight (0 = 3) */
+  int32_t inflight_launches;  /* launches in flight (0 = 2) */
   int32_t precision;          /* qj_precision */
   double band;
   int32_t device;
+  double launch_work;         /* root-term iterations per launch (voxels x max_iter x roots), so every
+                                 launch takes about the same GPU time (0 = 6e9, a few ms) */
 } qj_server_config;
 
 typedef struct qj_request {

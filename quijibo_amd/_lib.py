@@ -55,7 +55,7 @@ class ServerConfig(ctypes.Structure):
     _fields_ = [("hbm_budget_bytes", ctypes.c_int64), ("chunk_voxels", ctypes.c_int32),
                 ("chunks_per_launch", ctypes.c_int32), ("max_roots", ctypes.c_int32),
                 ("inflight_launches", ctypes.c_int32), ("precision", ctypes.c_int32), ("band", ctypes.c_double),
-                ("device", ctypes.c_int32)]
+                ("device", ctypes.c_int32), ("launch_work", ctypes.c_double)]
 
 
 class Request(ctypes.Structure):

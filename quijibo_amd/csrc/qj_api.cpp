@@ -50,6 +50,16 @@ void make_geom(const qj_grid* g, GridGeom* o) {
   rotation_matrix(g->rotation, o->rot);
 }
 
+double term_bits(const std::vector<RootRec64>& roots, double escape) {
+  double tb = 0.0;
+  for (const RootRec64& r : roots) {
+    const double rm = sqrt(r.w * r.w + r.x * r.x + r.y * r.y + r.z * r.z);
+    const double b = fabs(r.e) * log2(fmax(escape, 1.0) + rm + 1.0);
+    tb = fmax(tb, b);
+  }
+  return tb + 2.0;
+}
+
 qj_status make_map_args(const qj_grid* g, const qj_rational* f, const qj_map_params* p, MapArgs* a) {
   if (!g || !f || !p || !a) {
     set_error("null argument");
@@ -81,6 +91,7 @@ qj_status make_map_args(const qj_grid* g, const qj_rational* f, const qj_map_par
   a->nb = f->nb;
   a->sc_top = f->sc_top;
   a->sc_bottom = f->sc_bottom;
+  a->term_bits = term_bits(f->h64, p->escape);
   return QJ_OK;
 }
 

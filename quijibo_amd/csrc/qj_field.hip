@@ -96,9 +96,19 @@ __device__ __forceinline__ void block_append(bool flag, uint32_t value, uint32_t
 // ---------------------------------------------------------------- FP64 method
 
 // POLYNOMIAL_4D::evaluateScaledPowerFactored (POLYNOMIAL_4D.cpp:912-923)
+// Once a component of the running product is NaN every later product is NaN
+// in all four components (each output component reads all four inputs), and
+// so are T B^-1 and |q|: the loop can stop there with the same F.
+__device__ __forceinline__ bool has_nan(const q64& q) { return isnan(q.w) || isnan(q.x) || isnan(q.y) || isnan(q.z); }
 __device__ __forceinline__ q64 poly64(const RootRec64* __restrict__ r, int n, const q64& q) {
   q64 res = qpow(qsub(q, q64{r[0].w, r[0].x, r[0].y, r[0].z}), r[0].e);
-  for (int i = 1; i < n; i++) res = qmul(res, qpow(qsub(q, q64{r[i].w, r[i].x, r[i].y, r[i].z}), r[i].e));
+  for (int i = 1; i < n; i++) {
+    res = qmul(res, qpow(qsub(q, q64{r[i].w, r[i].x, r[i].y, r[i].z}), r[i].e));
+    if ((i & 7) == 0 && has_nan(res)) {
+      const double nan = __builtin_nan("");
+      return q64{nan, nan, nan, nan};
+    }
+  }
   return res;
 }
 
@@ -128,6 +138,15 @@ __device__ __forceinline__ Res64 eval64(const MapArgs& a, const RootRec64* __res
   while (magnitude < a.escape && it < a.max_iter) {
     const q64 T = poly64(roots, a.nt, q);
     if (a.nb > 0) {
+      if (has_nan(T) && !adaptive) {
+        // T B^-1 is all-NaN whatever B is; skip evaluating B
+        const double nan = __builtin_nan("");
+        q = q64{nan, nan, nan, nan};
+        magnitude = nan;
+        it++;
+        r.iters++;
+        break;
+      }
       const q64 B = poly64(roots + a.nt, a.nb, q);
       q = qdiv(T, B);
       if (adaptive) {
@@ -246,6 +265,7 @@ struct Consts32 {
   float log2bh;    // log2(10 FLT_MIN)
   float margin;    // decision margin in log2 units
   float band;
+  float term_bits;  // bound on log2 |term| between renormalisations
 };
 
 // Largest-component power-of-two renormalisation: exact, no rounding.
@@ -286,22 +306,25 @@ __device__ __forceinline__ q32 term32(const RootRec32& r, const q32& q, float& L
 }
 
 // Ordered product over n roots with exponent tracking (renormalise every 2
-// terms). `range` is set when the running product leaves +-lim in log2, i.e.
-// where the FP64 method would overflow or underflow and so give a different
-// (non-finite or zero) result: such voxels go to FP64.
+// terms). hi/lo receive the largest and smallest log2 magnitude of the
+// running product seen at the renormalisation points.
 __device__ __forceinline__ q32 poly32(const RootRec32* __restrict__ r, int n, const q32& q, int& E, float& L,
-                                      float lim, bool& range) {
+                                      float& hi, float& lo) {
   q32 t = term32(r[0], q, L);
   int i = 1;
   for (; i + 1 < n; i += 2) {
     t = qmul(t, term32(r[i], q, L));
     t = qmul(t, term32(r[i + 1], q, L));
     renorm32(t, E);
-    range |= fabsf((float)E + L) > lim;
+    const float l = (float)E + L;
+    hi = fmaxf(hi, l);
+    lo = fminf(lo, l);
   }
   if (i < n) t = qmul(t, term32(r[i], q, L));
   renorm32(t, E);
-  range |= fabsf((float)E + L) > lim;
+  const float l = (float)E + L;
+  hi = fmaxf(hi, l);
+  lo = fminf(lo, l);
   return t;
 }
 
@@ -312,6 +335,12 @@ struct Res32 {
   bool need64;
 };
 
+// One voxel in FP32 with the FP64 method's range behaviour reproduced where it
+// is certain: every running product of T and B must stay well inside FP64's
+// range (term_bits covers the product between two renormalisations); then
+// |B| > 2^512.5 means |B|^2 overflows in FP64, B^-1 = conj(B) * 0 and q = 0
+// exactly (F = -inf under the black-hole stop); anything closer to a range
+// edge, or outside it, goes to FP64.
 __device__ __forceinline__ Res32 eval32(const MapArgs& a, const RootRec32* __restrict__ roots, const Consts32& k,
                                         const q64& q0) {
   Res32 res{0.0f, 0, true, false};
@@ -326,27 +355,34 @@ __device__ __forceinline__ Res32 eval32(const MapArgs& a, const RootRec32* __res
     res.need64 = true;  // trivial case: let FP64 write it exactly
     return res;
   }
+  const bool black_hole = (a.flags & QJ_MAP_BLACK_HOLE) != 0;
   q32 q{(float)q0.w, (float)q0.x, (float)q0.y, (float)q0.z};
   float log2mag = 0.0f;
-  bool amb = false, bad = false;
+  bool amb = false, bad = false, zero = false;
   int it = 0;
+  const float lim = 1020.0f - k.term_bits;
   while (true) {
     int ET = 0, EB = 0;
-    float L = 0.0f;
-    const q32 T = poly32(roots, a.nt, q, ET, L, 1000.0f, bad);
-    q32 D;
+    float L = 0.0f, hiT = -1e30f, loT = 1e30f;
+    const q32 T = poly32(roots, a.nt, q, ET, L, hiT, loT);
+    bad |= !(hiT < lim && loT > -lim);
     float l2 = k.log2c;
     if (a.nb > 0) {
-      float LB = 0.0f;
-      const q32 B = poly32(roots + a.nt, a.nb, q, EB, LB, 500.0f, bad);
-      L -= LB;
-      D = qmul(T, q32{B.w, -B.x, -B.y, -B.z});  // T conj(B) = T B^-1 |B|^2
+      float LB = 0.0f, hiB = -1e30f, loB = 1e30f;
+      const q32 B = poly32(roots + a.nt, a.nb, q, EB, LB, hiB, loB);
+      bad |= !(hiB < lim && loB > -lim);
       const float nb2 = B.w * B.w + B.x * B.x + B.y * B.y + B.z * B.z;
+      const float lB = 0.5f * __log2f(nb2) + (float)EB + LB;
+      const q32 D = qmul(T, q32{B.w, -B.x, -B.y, -B.z});  // T conj(B) = T B^-1 |B|^2
       const float nd2 = D.w * D.w + D.x * D.x + D.y * D.y + D.z * D.z;
-      l2 += 0.5f * __log2f(nd2) - __log2f(nb2) + (float)(ET - EB) + L;
+      l2 += 0.5f * __log2f(nd2) - __log2f(nb2) + (float)(ET - EB) + (L - LB);
       bad |= !(nb2 > 0.0f) || !(nd2 > 0.0f);
-      if (!bad) {
-        const float sc = exp2f(k.log2c + (float)(ET - EB) + L) / nb2;
+      if (lB > 512.5f) {
+        zero = true;  // FP64: |B|^2 = inf, 1/inf = 0, q = T * 0 = 0
+      } else if (lB > 511.5f || lB < -510.0f) {
+        bad = true;   // within FP32 error of the FP64 overflow edge of |B|^2 or 1/|B|^2
+      } else if (!bad) {
+        const float sc = exp2f(k.log2c + (float)(ET - EB) + (L - LB)) / nb2;
         q = qscale(D, sc);
       }
     } else {
@@ -355,16 +391,22 @@ __device__ __forceinline__ Res32 eval32(const MapArgs& a, const RootRec32* __res
       bad |= !(nt2 > 0.0f);
       if (!bad) q = qscale(T, exp2f(k.log2c + (float)ET + L));
     }
-    log2mag = l2;
     it++;
     res.iters++;
-    bad |= !isfinite(l2);
+    if (zero) {
+      // q = 0: magnitude 0 < 10 FLT_MIN stops the loop with F = log(0)
+      bad |= !black_hole;
+      log2mag = -INFINITY;
+      break;
+    }
+    log2mag = l2;
+    bad |= !isfinite(l2) || l2 > 511.0f || l2 < -530.0f;  // |q|^2 over/underflow in FP64
     amb |= fabsf(l2 - k.log2esc) < k.margin;
-    const bool bh = (a.flags & QJ_MAP_BLACK_HOLE) && l2 < k.log2bh;
-    if (a.flags & QJ_MAP_BLACK_HOLE) amb |= fabsf(l2 - k.log2bh) < k.margin;
+    const bool bh = black_hole && l2 < k.log2bh;
+    if (black_hole) amb |= fabsf(l2 - k.log2bh) < k.margin;
     if (bad || l2 >= k.log2esc || bh || it >= a.max_iter) break;
   }
-  res.F = log2mag * 0.69314718f;
+  res.F = zero ? -INFINITY : log2mag * 0.69314718f;
   res.need64 = bad || amb || !(fabsf(res.F) >= k.band);
   return res;
 }
@@ -512,6 +554,7 @@ __global__ __launch_bounds__(kBlock) void k_batch32(const ChunkDesc* __restrict_
     k.log2bh = (float)log2(kBlackHole);
     k.margin = 0.02f;
     k.band = d.band;
+    k.term_bits = (float)a.term_bits;
     const int64_t idx = d.start + local;
     const Res32 r = eval32(a, d.r32, k, chunk_start(a, idx));
     if (r.written) {
@@ -578,6 +621,7 @@ static Consts32 consts32(const MapArgs& a, double band) {
   k.log2bh = (float)log2(kBlackHole);
   k.margin = 0.02f;
   k.band = (float)band;
+  k.term_bits = (float)a.term_bits;
   return k;
 }
 

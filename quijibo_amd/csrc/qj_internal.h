@@ -38,6 +38,7 @@ struct MapArgs {
   uint32_t flags;
   int32_t nt, nb;
   double sc_top, sc_bottom;  // power scalars
+  double term_bits;          // max over roots of |e| log2(escape + |root|) + 2: bound on one term's log2 size
 };
 
 // FIELD_3D geometry alone (marching cubes, weight-field lookups).
@@ -105,6 +106,7 @@ void rotation_matrix(const double* q, double* m);
 void make_geom(const qj_grid* g, GridGeom* out);
 // Fill MapArgs from the C structs (host arithmetic mirrors FIELD_3D).
 qj_status make_map_args(const qj_grid* g, const qj_rational* f, const qj_map_params* p, MapArgs* a);
+double term_bits(const std::vector<RootRec64>& roots, double escape);
 qj_status ensure_list(qj_workspace* ws, int64_t n, hipStream_t s);
 qj_status ensure_partials(qj_workspace* ws, int64_t n);
 // Records an event on s when timing is enabled (slot order: bulk0, bulk1, band0, band1).

@@ -180,7 +180,8 @@ static bool try_admit(qj_server* s, Req* r) {
   return true;
 }
 
-// Fills slot descriptors with up to chunks_per_launch chunks, shortest remaining work first.
+// Fills slot descriptors with up to chunks_per_launch chunks, shortest remaining
+// work first, until the launch holds about launch_work root-term iterations.
 static int build_batch(qj_server* s, Slot& sl) {
   std::vector<Req*> order;
   for (auto& kv : s->live)
@@ -191,11 +192,13 @@ static int build_batch(qj_server* s, Slot& sl) {
   });
   const int maxc = s->cfg.chunks_per_launch, cv = s->cfg.chunk_voxels;
   int n = 0;
+  double work = 0.0;
   sl.work.clear();
   char* b = s->arena.base();
   for (Req* r : order) {
     int taken = 0;
-    while (n < maxc && r->next < r->voxels) {
+    const double per_voxel = (double)r->params.max_iter * (double)(r->nt + r->nb);
+    while (n < maxc && r->next < r->voxels && (n == 0 || work < s->cfg.launch_work)) {
       ChunkDesc& d = sl.h_desc[n++];
       d.a = r->args;
       d.r64 = (const RootRec64*)(b + r->off64);
@@ -206,10 +209,11 @@ static int build_batch(qj_server* s, Slot& sl) {
       d.count = (int32_t)(left < cv ? left : cv);
       d.band = (float)s->cfg.band;
       r->next += d.count;
+      work += per_voxel * d.count;
       taken++;
     }
     if (taken) sl.work.emplace_back(r->id, taken);
-    if (n == maxc) break;
+    if (n == maxc || work >= s->cfg.launch_work) break;
   }
   for (int i = n; i < maxc; i++) sl.h_desc[i].count = 0;
   return n;
@@ -377,7 +381,8 @@ qj_status qj_server_create(const qj_server_config* cfg, qj_server** out) {
   if (c.chunk_voxels <= 0) c.chunk_voxels = 65536;
   if (c.chunks_per_launch <= 0) c.chunks_per_launch = 512;
   if (c.max_roots <= 0) c.max_roots = 256;
-  if (c.inflight_launches <= 0) c.inflight_launches = 3;
+  if (c.inflight_launches <= 0) c.inflight_launches = 2;
+  if (c.launch_work <= 0) c.launch_work = 6e9;
   if (c.band <= 0) c.band = 0.05;
   if (c.precision != QJ_FP64 && c.precision != QJ_FP32_BAND) {
     delete s;
@@ -466,6 +471,7 @@ qj_status qj_server_submit(qj_server* s, const qj_request* req, uint64_t* id) {
   f.nb = r->nb;
   f.sc_top = t.power_scalar;
   f.sc_bottom = b.power_scalar;
+  f.h64 = r->h64;
   qj_status st = make_map_args(&r->grid, &f, &r->params, &r->args);
   if (st) return st;
   r->voxels = (int64_t)r->grid.res[0] * r->grid.res[1] * r->grid.res[2];
