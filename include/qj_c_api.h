@@ -166,6 +166,23 @@ qj_status qj_mc_copy(qj_workspace* ws, double* vertices, int32_t* triangles, voi
 /* Device pointers to the last mesh (valid until the next qj_mc_run on ws). */
 qj_status qj_mc_device_ptrs(qj_workspace* ws, const double** d_vertices, const int32_t** d_triangles);
 
+/* Non-linear low-memory marching cubes (the TRIANGLE_MESH(center, lengths, res,
+ * top, bottom, expScaling, maxIterations, slice, isosurface, rotation[, cache])
+ * mesher, TRIANGLE_MESH.cpp:84-101, 681-768): the map is evaluated slab by slab
+ * (chunk_slices z-slices at a time plus a 3-slice halo; never the whole field),
+ * cases come from log|q| (TRIANGLE_MESH.cpp:379), each vertex is found by
+ * bisection on log|q| - iso re-evaluated in FP64 (at most 6 halvings, stop at
+ * |value| < 1e-8, TRIANGLE_MESH.cpp:2232-2271); vertex numbering, triangle
+ * order and the degenerate filter are the linear mesher's. params: the
+ * mesher uses escape 2000 and the black-hole stop (TRIANGLE_MESH.cpp:97).
+ * precision selects the slab evaluation (QJ_FP32_BAND or QJ_FP64); vertices are
+ * always FP64. Returns QJ_ERR_NUMERIC (mesh still available) when an edge's end
+ * values do not straddle the isosurface, where the reference exits.
+ * Results are read with qj_mc_copy / qj_mc_device_ptrs. Synchronizes `stream`. */
+qj_status qj_nlmc_run(qj_workspace* ws, const qj_grid* grid, const qj_rational* f, const qj_map_params* params,
+                      double iso, qj_precision precision, double band, int32_t chunk_slices,
+                      int64_t* n_vertices, int64_t* n_triangles, void* stream);
+
 /* ---- gradients ---- */
 
 /* One-iteration AP score and exponent gradients (max_iter must be 1, as the

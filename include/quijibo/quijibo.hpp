@@ -1,0 +1,255 @@
+// C++ host layer that keeps QUIJIBO's class API on top of the C ABI (qj_c_api.h).
+//
+// Class and member names follow the reference so a caller of the reference can
+// switch: POLYNOMIAL_4D (src/algebra/POLYNOMIAL_4D.h), FIELD_3D
+// (src/geometry/FIELD_3D.h), TRIANGLE_MESH (src/geometry/TRIANGLE_MESH.h),
+// OPTIMIZE_3D (src/solvers/OPTIMIZE_3D.h, the map / score / gradient members)
+// and the TOUPEE optimizer callback (src/TOUPEE/TAO_SOLVER.h:41-43).
+// Real is double (the reference's USING_DOUBLE_NLS build). All field work runs
+// on the GPU through qj_* calls; data that the reference keeps in host memory
+// (FIELD_3D's cells, meshes) is copied back to host memory here.
+#pragma once
+
+#include <stdio.h>
+
+#include <memory>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "../qj_c_api.h"
+
+namespace quijibo {
+
+typedef double Real;
+
+// Error raised where the reference calls exit(0).
+struct Error : std::runtime_error {
+  qj_status status;
+  Error(qj_status s, const std::string& what) : std::runtime_error(what), status(s) {}
+};
+void check(qj_status s, const char* where);
+
+struct VEC3F {
+  Real v[3];
+  VEC3F() : v{0, 0, 0} {}
+  VEC3F(Real a, Real b, Real c) : v{a, b, c} {}
+  Real& operator[](int i) { return v[i]; }
+  Real operator[](int i) const { return v[i]; }
+  void write(FILE* f) const;  // 3 doubles (VEC3.h:238-256)
+  void read(FILE* f);
+};
+
+// Storage of a quaternion, slot order (w, x, y, z) (QUATERNION.h:31-155).
+struct QUATERNION {
+  Real _w = 0, _x = 0, _y = 0, _z = 0;
+  QUATERNION() {}
+  QUATERNION(Real w, Real x, Real y, Real z) : _w(w), _x(x), _y(y), _z(z) {}
+  Real& operator[](int i) { return (&_w)[i]; }
+  Real operator[](int i) const { return (&_w)[i]; }
+  Real w() const { return _w; }
+  Real x() const { return _x; }
+  Real y() const { return _y; }
+  Real z() const { return _z; }
+  void write(FILE* f) const;  // 4 doubles (QUATERNION.cpp:286-306)
+  void read(FILE* f);
+};
+
+class POLYNOMIAL_4D {
+ public:
+  POLYNOMIAL_4D() {}
+  explicit POLYNOMIAL_4D(const std::vector<QUATERNION>& roots);
+  POLYNOMIAL_4D(const std::vector<QUATERNION>& roots, const std::vector<Real>& powers);
+
+  int totalRoots() const { return (int)_roots.size(); }
+  const std::vector<QUATERNION>& roots() const { return _roots; }
+  std::vector<QUATERNION>& rootsMutable() { return _roots; }
+  const std::vector<Real>& powers() const { return _rootPowers; }
+  std::vector<Real>& powersMutable() { return _rootPowers; }
+  Real& powerScalar() { return _powerScalar; }
+  Real powerScalar() const { return _powerScalar; }
+
+  void addRoot(const QUATERNION& r, const Real& power = 1.0);
+  void modifyRoot(int which, const QUATERNION& r) { _roots.at(which) = r; }
+  void changePower(int which, const Real& p) { _rootPowers.at(which) = p; }
+  void clear();
+
+  // POLYNOMIAL_4D::write/read layout (POLYNOMIAL_4D.cpp:650-728); the expanded
+  // coefficient arrays are written empty and skipped on read.
+  void write(FILE* file) const;
+  void read(FILE* file);
+
+  // flattened arrays for the C ABI
+  qj_poly view() const;
+
+ private:
+  std::vector<QUATERNION> _roots;
+  std::vector<Real> _rootPowers;
+  Real _powerScalar = 1.0;
+  mutable std::vector<double> _flat;
+};
+
+class FIELD_3D {
+ public:
+  FIELD_3D() {}
+  FIELD_3D(int xRes, int yRes, int zRes, const VEC3F& center = VEC3F(0, 0, 0),
+           const VEC3F& lengths = VEC3F(1, 1, 1));
+  FIELD_3D(const Real* data, int xRes, int yRes, int zRes, const VEC3F& center, const VEC3F& lengths);
+
+  Real& operator()(int x, int y, int z) { return _data[x + y * (size_t)_xRes + z * (size_t)_xRes * _yRes]; }
+  Real operator()(int x, int y, int z) const { return _data[x + y * (size_t)_xRes + z * (size_t)_xRes * _yRes]; }
+  Real& operator[](size_t i) { return _data[i]; }
+  Real operator[](size_t i) const { return _data[i]; }
+  Real operator()(const VEC3F& position) const;  // trilinear lookup (FIELD_3D.cpp:966-1029)
+
+  int xRes() const { return _xRes; }
+  int yRes() const { return _yRes; }
+  int zRes() const { return _zRes; }
+  size_t totalCells() const { return (size_t)_xRes * _yRes * _zRes; }
+  Real dx() const { return _lengths[0] / _xRes; }
+  Real dy() const { return _lengths[1] / _yRes; }
+  Real dz() const { return _lengths[2] / _zRes; }
+  const VEC3F& center() const { return _center; }
+  const VEC3F& lengths() const { return _lengths; }
+  QUATERNION& rotation() { return _rotation; }
+  const QUATERNION& rotation() const { return _rotation; }
+  Real* data() { return _data.data(); }
+  const Real* dataConst() const { return _data.data(); }
+  VEC3F cellCenter(int x, int y, int z) const;  // FIELD_3D.cpp:818-841
+
+  void write(FILE* file) const;  // FIELD_3D.cpp:4371-4395
+  void read(FILE* file);         // FIELD_3D.cpp:4400-4450
+  bool write(const std::string& filename) const;
+  bool read(const std::string& filename);
+
+  qj_grid grid() const;
+
+ private:
+  int _xRes = 0, _yRes = 0, _zRes = 0;
+  VEC3F _center, _lengths{1, 1, 1};
+  QUATERNION _rotation{1, 0, 0, 0};
+  std::vector<Real> _data;
+};
+
+struct TRIANGLE {
+  int v[3];
+};
+
+class TRIANGLE_MESH {
+ public:
+  TRIANGLE_MESH() {}
+  // marching cubes on a field (TRIANGLE_MESH.cpp:48-57 -> computeStagedMarchingCubes)
+  explicit TRIANGLE_MESH(const FIELD_3D& field, Real isosurface = 0.0);
+  // non-linear low-memory mesher (TRIANGLE_MESH.cpp:84-101): slabs of the map
+  // evaluated on the fly, vertices refined by bisection
+  TRIANGLE_MESH(const VEC3F& center, const VEC3F& lengths, const int res[3], const POLYNOMIAL_4D& top,
+                const POLYNOMIAL_4D& bottom, Real expScaling, int maxIterations, Real slice, Real isosurface,
+                const QUATERNION& rotation = QUATERNION(1, 0, 0, 0));
+
+  const std::vector<VEC3F>& vertices() const { return _vertices; }
+  const std::vector<TRIANGLE>& triangles() const { return _triangles; }
+  bool writeOBJ(const std::string& filename) const;  // TRIANGLE_MESH.cpp:2762-2800
+
+ private:
+  std::vector<VEC3F> _vertices;
+  std::vector<TRIANGLE> _triangles;
+};
+
+// The map / score / gradient members of OPTIMIZE_3D on the GPU.
+class OPTIMIZE_3D {
+ public:
+  OPTIMIZE_3D();
+  OPTIMIZE_3D(int xRes, int yRes, int zRes, const VEC3F& center, const VEC3F& lengths);
+  ~OPTIMIZE_3D();
+  OPTIMIZE_3D(const OPTIMIZE_3D&) = delete;
+  OPTIMIZE_3D& operator=(const OPTIMIZE_3D&) = delete;
+
+  POLYNOMIAL_4D& top() { return _top; }
+  POLYNOMIAL_4D& bottom() { return _bottom; }
+  FIELD_3D& fractal() { return _fractal; }
+  FIELD_3D& weightField() { return _weightField; }
+  FIELD_3D& distanceField() { return _distanceField; }
+  int& maxIterations() { return _maxIterations; }
+  Real& expScaling() { return _expScaling; }
+  Real& quaternionSlice() { return _quaternionSlice; }
+  Real& tanhAlpha() { return _tanhAlpha; }
+  Real& tanhThreshold() { return _tanhThreshold; }
+  // precision of the field maps (QJ_FP64 or QJ_FP32_BAND) and the band width
+  void setPrecision(qj_precision p, Real band = 0.05) { _precision = p; _band = band; }
+
+  // OPTIMIZE_3D.cpp:1272-1370 (escape 2000, black-hole stop)
+  void computeLogScaledPowerRationalMap(const POLYNOMIAL_4D& top, const POLYNOMIAL_4D& bottom, FIELD_3D& fractal,
+                                        bool verbose = false);
+  void computeLogScaledPowerRationalMap(bool verbose = false) {
+    computeLogScaledPowerRationalMap(_top, _bottom, _fractal, verbose);
+  }
+  // OPTIMIZE_3D.cpp:1374-1568 (escape 20, origin skipped, overflowed cells re-passed)
+  void computeRationalMapAP(const POLYNOMIAL_4D& top, const POLYNOMIAL_4D& bottom, FIELD_3D& fractal,
+                            bool verbose = false);
+
+  // AP score and gradients at one iteration (OPTIMIZE_3D.cpp:3563, 3589, 8905, 10012, 9250, 10360).
+  // One GPU pass computes all of them for the current state; the accessors
+  // reuse it until the state changes.
+  Real computeScoreAP();
+  Real computeExpGradientAP();
+  Real computeTopGradientAP(int whichRoot);
+  Real computeBottomGradientAP(int whichRoot);
+  Real computeTopBulkScalarGradientAP();
+  Real computeBottomBulkScalarGradientAP();
+  int adaptiveCells();
+
+  // .o3d layout (OPTIMIZE_3D.cpp:3254-3346)
+  void write(const std::string& filename) const;
+  void read(const std::string& filename);
+
+ private:
+  void sync_rational(const POLYNOMIAL_4D& top, const POLYNOMIAL_4D& bottom);
+  void run_ap();
+  void ensure_device(size_t cells);
+
+  int _xRes = 0, _yRes = 0, _zRes = 0;
+  VEC3F _center, _lengths{1, 1, 1};
+  POLYNOMIAL_4D _top, _bottom;
+  FIELD_3D _fractal, _distanceField, _curvatureField, _weightField;
+  int _maxIterations = 1;
+  Real _expScaling = 1.0;
+  Real _quaternionSlice = 0.0;
+  Real _tanhAlpha = 10.0, _tanhThreshold = 0.0;
+  int _historySize = 0;
+  qj_precision _precision = QJ_FP64;
+  Real _band = 0.05;
+
+  qj_rational* _rational = nullptr;
+  qj_workspace* _ws = nullptr;
+  double* _dField = nullptr;
+  double* _dWeight = nullptr;
+  size_t _dCells = 0, _dWeightCells = 0;
+  bool _apValid = false;
+  qj_ap_grads _ap{};
+  std::vector<double> _gTop, _gBottom;
+  std::vector<double> _apKey;
+};
+
+// TOUPEE callback ABI (TAO_SOLVER.h:41-43) with VECTOR = std::vector<Real>.
+typedef std::vector<Real> VECTOR;
+typedef void (*FunctionGradient)(const VECTOR& state, Real& value, VECTOR& gradient);
+typedef void (*Monitor)();
+
+// Bounded limited-memory variable metric minimisation (TAO_SOLVER::optimizeBLMVM,
+// TAO_SOLVER.cpp:72-341): projected quasi-Newton steps inside [xl, xu] with a
+// backtracking line search. Returns the number of iterations taken.
+class TAO_SOLVER {
+ public:
+  explicit TAO_SOLVER(int maxIterations = 100, int history = 5, Real gradientTolerance = 1e-8)
+      : _maxIterations(maxIterations), _history(history), _gatol(gradientTolerance) {}
+  int optimizeBLMVM(VECTOR& x, const VECTOR& xu, const VECTOR& xl, FunctionGradient functionGradient,
+                    Monitor monitor = nullptr);
+  Real value() const { return _f; }
+
+ private:
+  int _maxIterations, _history;
+  Real _gatol;
+  Real _f = 0;
+};
+
+}  // namespace quijibo

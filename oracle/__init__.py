@@ -69,6 +69,10 @@ def lib():
         L.qo_marching_cubes.argtypes = [ctypes.POINTER(Grid), dp, ctypes.POINTER(ctypes.c_int64),
                                         ctypes.POINTER(ctypes.c_int64), ctypes.c_void_p, ctypes.c_void_p]
         L.qo_marching_cubes.restype = ctypes.c_int
+        L.qo_nonlinear_mc.argtypes = [ctypes.POINTER(Grid), ctypes.POINTER(Poly), ctypes.POINTER(Poly),
+                                      ctypes.POINTER(Params), ctypes.c_double, ctypes.POINTER(ctypes.c_int64),
+                                      ctypes.POINTER(ctypes.c_int64), ctypes.c_void_p, ctypes.c_void_p]
+        L.qo_nonlinear_mc.restype = ctypes.c_int64
         L.qo_field_lookup.argtypes = [ctypes.POINTER(Grid), dp, dp]
         L.qo_field_lookup.restype = ctypes.c_double
         L.qo_ap_score_gradients.argtypes = [
@@ -157,6 +161,20 @@ def marching_cubes(grid, field):
                            verts.ctypes.data, tris.ctypes.data):
         raise MemoryError("oracle marching cubes")
     return verts, tris[: nt.value].copy()
+
+
+def nonlinear_mc(grid, top, bottom, params, iso=0.0):
+    """Returns (vertices, triangles, n_bad_edges)."""
+    nv = ctypes.c_int64()
+    nt = ctypes.c_int64()
+    L = lib()
+    L.qo_nonlinear_mc(ctypes.byref(grid), ctypes.byref(top.p), ctypes.byref(bottom.p), ctypes.byref(params),
+                      float(iso), ctypes.byref(nv), ctypes.byref(nt), None, None)
+    verts = np.empty((nv.value, 3))
+    tris = np.empty((max(nt.value, 1), 3), dtype=np.int32)
+    bad = L.qo_nonlinear_mc(ctypes.byref(grid), ctypes.byref(top.p), ctypes.byref(bottom.p), ctypes.byref(params),
+                            float(iso), ctypes.byref(nv), ctypes.byref(nt), verts.ctypes.data, tris.ctypes.data)
+    return verts, tris[: nt.value].copy(), int(bad)
 
 
 def cell_center(grid, x, y, z):

@@ -541,6 +541,150 @@ int qo_marching_cubes(const qo_grid* g, const double* field, int64_t* n_vertices
   return 0;
 }
 
+/* ---------------- non-linear marching cubes ---------------- */
+
+/* TRIANGLE_MESH::nonlinearValue (TRIANGLE_MESH.cpp:1755-1795) */
+static double nonlinear_value(const double* pos, const qo_poly* top, const qo_poly* bottom, const qo_params* p,
+                              double iso) {
+  Q q = {pos[0], pos[1], pos[2], p->slice};
+  double F = 0;
+  int of = 0;
+  iterate_point(q, top, bottom, p, &F, NULL, &of);
+  return F - iso;
+}
+
+/* TRIANGLE_MESH::midpointSearch (TRIANGLE_MESH.cpp:2232-2271); returns 1 when
+ * the end values do not straddle zero (the reference exits there) */
+static int midpoint_search(double* pp, double vp, double* pn, double vn, const qo_poly* top, const qo_poly* bottom,
+                           const qo_params* p, double iso, double* out) {
+  for (int recursion = 0;; recursion++) {
+    double m[3] = {pp[0] + pn[0], pp[1] + pn[1], pp[2] + pn[2]};
+    m[0] *= 0.5;
+    m[1] *= 0.5;
+    m[2] *= 0.5;
+    out[0] = m[0];
+    out[1] = m[1];
+    out[2] = m[2];
+    if (vp * vn >= 0.0) return 1;
+    if (recursion >= 6) return 0;
+    const double vm = nonlinear_value(m, top, bottom, p, iso);
+    if (fabs(vm) < 1e-8) return 0;
+    if (vm < 0) {
+      pn[0] = m[0]; pn[1] = m[1]; pn[2] = m[2];
+      vn = vm;
+    } else {
+      pp[0] = m[0]; pp[1] = m[1]; pp[2] = m[2];
+      vp = vm;
+    }
+  }
+}
+
+/* TRIANGLE_MESH::computeNonlinearMarchingCubesLowMemoryHuge (TRIANGLE_MESH.cpp:681-768)
+ * with computeAllLowMemorySlicesHuge (:4884-4948) and
+ * computeNonlinearEdgeInterpolationsHuge (:2034-2123), on the whole field at
+ * once (the slab streaming only bounds memory). Cases from the raw field
+ * log|q|; vertices by bisection on log|q| - iso. Returns the number of edges
+ * whose end values do not straddle zero (the reference exits on the first). */
+int64_t qo_nonlinear_mc(const qo_grid* g, const qo_poly* top, const qo_poly* bottom, const qo_params* p, double iso,
+                        int64_t* n_vertices, int64_t* n_triangles, double* vertices, int32_t* triangles) {
+  const int64_t xRes = g->res[0], yRes = g->res[1], zRes = g->res[2];
+  const int64_t slab = xRes * yRes, total = slab * zRes;
+  double* field = (double*)malloc(sizeof(double) * (size_t)total);
+  uint8_t* mark = (uint8_t*)calloc((size_t)total, 1);
+  int32_t* vid = (int32_t*)malloc(sizeof(int32_t) * (size_t)total);
+  int64_t bad = 0;
+  qo_field_map(g, top, bottom, p, field, NULL, NULL, 0);
+  for (int64_t z = 0; z < zRes - 1; z++)
+    for (int64_t y = 0; y < yRes - 1; y++)
+      for (int64_t x = 0; x < xRes - 1; x++) {
+        const int64_t index = x + y * xRes + z * slab;
+        const int flag = cell_flag(field, index, xRes, slab);
+        for (int k = 0; k < 15 && QO_MC_TRIS[flag][k]; k++) {
+          int64_t off;
+          int axis;
+          edge_of(QO_MC_TRIS[flag][k], xRes, slab, &off, &axis);
+          mark[index + off] |= (uint8_t)(1u << axis);
+        }
+      }
+  int64_t nv = 0;
+  for (int64_t i = 0; i < total; i++) {
+    vid[i] = (int32_t)nv;
+    nv += (mark[i] & 1) + ((mark[i] >> 1) & 1) + ((mark[i] >> 2) & 1);
+  }
+  if (vertices) {
+#ifdef _OPENMP
+#pragma omp parallel for schedule(dynamic, 256) reduction(+ : bad)
+#endif
+    for (int64_t i = 0; i < total; i++) {
+      if (!mark[i]) continue;
+      int64_t k = vid[i];
+      const int x = (int)(i % xRes), y = (int)((i / xRes) % yRes), z = (int)(i / slab);
+      double a[3];
+      qo_cell_center(g, x, y, z, a);
+      const double va = nonlinear_value(a, top, bottom, p, iso);
+      for (int axis = 0; axis < 3; axis++) {
+        if (!(mark[i] & (1u << axis))) continue;
+        double b[3];
+        qo_cell_center(g, x + (axis == 0), y + (axis == 1), z + (axis == 2), b);
+        const double vb = nonlinear_value(b, top, bottom, p, iso);
+        double pp[3], pn[3], vp, vn;
+        if (va * vb >= 0.0) bad++;
+        if (va < 0) {
+          memcpy(pp, b, sizeof(pp)); vp = vb; memcpy(pn, a, sizeof(pn)); vn = va;
+        } else {
+          memcpy(pp, a, sizeof(pp)); vp = va; memcpy(pn, b, sizeof(pn)); vn = vb;
+        }
+        if (midpoint_search(pp, vp, pn, vn, top, bottom, p, iso, vertices + 3 * k) && !(va * vb >= 0.0)) bad++;
+        k++;
+      }
+    }
+  }
+  int64_t nt = 0;
+  for (int64_t z = 0; z < zRes - 1; z++)
+    for (int64_t y = 0; y < yRes - 1; y++)
+      for (int64_t x = 0; x < xRes - 1; x++) {
+        const int64_t index = x + y * xRes + z * slab;
+        const int flag = cell_flag(field, index, xRes, slab);
+        for (int k = 0; k < 15 && QO_MC_TRIS[flag][k]; k += 3) {
+          int32_t v[3];
+          for (int c = 0; c < 3; c++) {
+            int64_t off;
+            int axis;
+            edge_of(QO_MC_TRIS[flag][k + c], xRes, slab, &off, &axis);
+            const int64_t lo = index + off;
+            int32_t id = vid[lo];
+            for (int a2 = 0; a2 < axis; a2++) id += (mark[lo] >> a2) & 1;
+            v[c] = id;
+          }
+          if (vertices) {
+            const double* p0 = vertices + 3 * (int64_t)v[0];
+            const double* p1 = vertices + 3 * (int64_t)v[1];
+            const double* p2 = vertices + 3 * (int64_t)v[2];
+            double e[3];
+            e[0] = p0[0] - p1[0]; e[1] = p0[1] - p1[1]; e[2] = p0[2] - p1[2];
+            const double d0 = sqrt(e[0] * e[0] + e[1] * e[1] + e[2] * e[2]);
+            e[0] = p0[0] - p2[0]; e[1] = p0[1] - p2[1]; e[2] = p0[2] - p2[2];
+            const double d1 = sqrt(e[0] * e[0] + e[1] * e[1] + e[2] * e[2]);
+            e[0] = p1[0] - p2[0]; e[1] = p1[1] - p2[1]; e[2] = p1[2] - p2[2];
+            const double d2 = sqrt(e[0] * e[0] + e[1] * e[1] + e[2] * e[2]);
+            if (d0 < 1e-7 || d1 < 1e-7 || d2 < 1e-7) continue;
+            if (triangles) {
+              triangles[3 * nt + 0] = v[0];
+              triangles[3 * nt + 1] = v[1];
+              triangles[3 * nt + 2] = v[2];
+            }
+          }
+          nt++;
+        }
+      }
+  *n_vertices = nv;
+  *n_triangles = nt;
+  free(field);
+  free(mark);
+  free(vid);
+  return bad;
+}
+
 /* ---------------- AP score and gradients at one iteration ---------------- */
 
 typedef struct {

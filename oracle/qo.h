@@ -75,6 +75,12 @@ void qo_points_eval(const double* pts, int64_t n, const qo_poly* top,
 int qo_marching_cubes(const qo_grid* g, const double* field, int64_t* n_vertices,
                       int64_t* n_triangles, double* vertices, int32_t* triangles);
 
+/* Non-linear mesher (TRIANGLE_MESH.cpp:681-768): cases from log|q|, vertices by
+ * bisection on log|q| - iso. Returns the count of edges whose end values do not
+ * straddle zero (the reference exits on the first). Two-call protocol as above. */
+int64_t qo_nonlinear_mc(const qo_grid* g, const qo_poly* top, const qo_poly* bottom, const qo_params* p, double iso,
+                        int64_t* n_vertices, int64_t* n_triangles, double* vertices, int32_t* triangles);
+
 /* FIELD_3D::operator()(VEC3F) trilinear lookup (FIELD_3D.cpp:966-1029). */
 double qo_field_lookup(const qo_grid* g, const double* data, const double* pos3);
 

@@ -93,6 +93,9 @@ SIGNATURES = [
                                         ctypes.c_double, _P, _P]),
     ("qj_mc_run", ctypes.c_int32, [_P, _P, ctypes.POINTER(Grid), ctypes.c_double, ctypes.POINTER(ctypes.c_int64),
                                    ctypes.POINTER(ctypes.c_int64), _P]),
+    ("qj_nlmc_run", ctypes.c_int32, [_P, ctypes.POINTER(Grid), _P, ctypes.POINTER(MapParams), ctypes.c_double,
+                                     ctypes.c_int32, ctypes.c_double, ctypes.c_int32, ctypes.POINTER(ctypes.c_int64),
+                                     ctypes.POINTER(ctypes.c_int64), _P]),
     ("qj_mc_copy", ctypes.c_int32, [_P, _P, _P, _P]),
     ("qj_mc_device_ptrs", ctypes.c_int32, [_P, _PP, _PP]),
     ("qj_ap_score_gradients", ctypes.c_int32, [_P, ctypes.POINTER(Grid), _P, ctypes.POINTER(MapParams),

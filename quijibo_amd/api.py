@@ -316,3 +316,22 @@ def points_adjoint(points, rational, params, dJdF, precision=FP64, workspace=Non
     check(_lib.load().qj_points_adjoint(ws.handle, _ptr(pts), pts.shape[0], rational.handle, ctypes.byref(params),
                                         int(precision), _ptr(g), _ptr(gr), _ptr(gp), _stream_ptr(stream)))
     return gr, gp
+
+
+def nonlinear_mesh(grid, rational, params, iso=0.0, precision=FP32_BAND, band=DEFAULT_BAND, chunk_slices=64,
+                   workspace=None, stream=None, allow_inconsistent=False):
+    """qj_nlmc_run + copy: the non-linear low-memory mesher. Returns (vertices, triangles)."""
+    ws = workspace or default_workspace()
+    nv = ctypes.c_int64()
+    nt = ctypes.c_int64()
+    L = _lib.load()
+    st = L.qj_nlmc_run(ws.handle, ctypes.byref(grid), rational.handle, ctypes.byref(params), float(iso),
+                       int(precision), float(band), int(chunk_slices), ctypes.byref(nv), ctypes.byref(nt),
+                       _stream_ptr(stream))
+    if st != 0 and not (allow_inconsistent and st == 5):
+        check(st)
+    v = np.empty((nv.value, 3), dtype=np.float64)
+    t = np.empty((nt.value, 3), dtype=np.int32)
+    check(L.qj_mc_copy(ws.handle, v.ctypes.data_as(ctypes.c_void_p) if nv.value else None,
+                       t.ctypes.data_as(ctypes.c_void_p) if nt.value else None, _stream_ptr(stream)))
+    return v, t

@@ -83,6 +83,8 @@ qj_status make_map_args(const qj_grid* g, const qj_rational* f, const qj_map_par
     return QJ_ERR_INVALID;
   }
   a->max_iter = p->max_iter;
+  a->zoff = 0;
+  a->pad_ = 0;
   a->slice = p->slice;
   a->exp_scaling = p->exp_scaling;
   a->escape = p->escape;
@@ -404,6 +406,27 @@ qj_status qj_mc_run(qj_workspace* ws, const double* d_field, const qj_grid* grid
     if (grid->res[i] < 1) return QJ_ERR_INVALID;
   std::lock_guard<std::mutex> lock(ws->mu);
   return mc_run(ws, d_field, grid, iso, n_vertices, n_triangles, (hipStream_t)stream);
+}
+
+qj_status qj_nlmc_run(qj_workspace* ws, const qj_grid* grid, const qj_rational* f, const qj_map_params* params,
+                      double iso, qj_precision precision, double band, int32_t chunk_slices, int64_t* n_vertices,
+                      int64_t* n_triangles, void* stream) {
+  if (!ws || !grid || !f || !params || !n_vertices || !n_triangles) {
+    set_error("qj_nlmc_run: null argument");
+    return QJ_ERR_INVALID;
+  }
+  if (precision != QJ_FP64 && precision != QJ_FP32_BAND) return QJ_ERR_INVALID;
+  if (params->flags & (QJ_MAP_SKIP_ORIGIN | QJ_MAP_ADAPTIVE)) {
+    set_error("qj_nlmc_run: the mesher's map has no origin skip or adaptive re-pass");
+    return QJ_ERR_INVALID;
+  }
+  MapArgs a;
+  qj_status st = make_map_args(grid, f, params, &a);
+  if (st) return st;
+  std::lock_guard<std::mutex> lock(ws->mu);
+  int64_t bad = 0;
+  return nlmc_run(ws, a, f, iso, precision, band, chunk_slices, n_vertices, n_triangles, &bad,
+                  (hipStream_t)stream);
 }
 
 qj_status qj_mc_copy(qj_workspace* ws, double* vertices, int32_t* triangles, void* stream) {

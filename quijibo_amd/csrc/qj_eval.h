@@ -1,0 +1,89 @@
+// FP64 evaluation of the map at one point, shared by the field kernels and
+// the non-linear mesher (OPTIMIZE_3D.cpp:1317-1362, TRIANGLE_MESH.cpp:1755-1795).
+#pragma once
+#include <float.h>
+
+#include "qj_internal.h"
+#include "qj_quat.h"
+
+namespace qj {
+
+constexpr double kBlackHole = 10.0 * (double)FLT_MIN;  // 10 * REAL_MIN, SETTINGS.h:58, OPTIMIZE_3D.cpp:1358
+
+// ---------------------------------------------------------------- FP64 method
+
+// POLYNOMIAL_4D::evaluateScaledPowerFactored (POLYNOMIAL_4D.cpp:912-923)
+// Once a component of the running product is NaN every later product is NaN
+// in all four components (each output component reads all four inputs), and
+// so are T B^-1 and |q|: the loop can stop there with the same F.
+__device__ __forceinline__ bool has_nan(const q64& q) { return isnan(q.w) || isnan(q.x) || isnan(q.y) || isnan(q.z); }
+__device__ __forceinline__ q64 poly64(const RootRec64* __restrict__ r, int n, const q64& q) {
+  q64 res = qpow(qsub(q, q64{r[0].w, r[0].x, r[0].y, r[0].z}), r[0].e);
+  for (int i = 1; i < n; i++) {
+    res = qmul(res, qpow(qsub(q, q64{r[i].w, r[i].x, r[i].y, r[i].z}), r[i].e));
+    if ((i & 7) == 0 && has_nan(res)) {
+      const double nan = __builtin_nan("");
+      return q64{nan, nan, nan, nan};
+    }
+  }
+  return res;
+}
+
+__device__ __forceinline__ bool finite4(const q64& q) {
+  return isfinite(q.w) && isfinite(q.x) && isfinite(q.y) && isfinite(q.z);
+}
+
+struct Res64 {
+  double F;
+  int iters;
+  bool written;
+  bool overflow;
+};
+
+// One voxel of computeLogScaledPowerRationalMap (OPTIMIZE_3D.cpp:1317-1362) /
+// computeRationalMapAP's first pass (:1423-1475). Overflow detection replaces
+// fetestexcept(FE_OVERFLOW): any non-finite T, B, |B|^2, quotient or |q|.
+__device__ __forceinline__ Res64 eval64(const MapArgs& a, const RootRec64* __restrict__ roots, q64 q) {
+  Res64 r{0.0, 0, true, false};
+  double magnitude = qmag(q);
+  if ((a.flags & QJ_MAP_SKIP_ORIGIN) && magnitude < 1e-8) {
+    r.written = false;
+    return r;
+  }
+  const bool adaptive = (a.flags & QJ_MAP_ADAPTIVE) != 0;
+  int it = 0;
+  while (magnitude < a.escape && it < a.max_iter) {
+    const q64 T = poly64(roots, a.nt, q);
+    if (a.nb > 0) {
+      if (has_nan(T) && !adaptive) {
+        // T B^-1 is all-NaN whatever B is; skip evaluating B
+        const double nan = __builtin_nan("");
+        q = q64{nan, nan, nan, nan};
+        magnitude = nan;
+        it++;
+        r.iters++;
+        break;
+      }
+      const q64 B = poly64(roots + a.nt, a.nb, q);
+      q = qdiv(T, B);
+      if (adaptive) {
+        const double m2 = B.w * B.w + B.x * B.x + B.y * B.y + B.z * B.z;
+        r.overflow |= !(finite4(T) && finite4(B) && isfinite(m2) && finite4(q));
+      }
+    } else {
+      q = T;
+      if (adaptive) r.overflow |= !finite4(T);
+    }
+    q = qscale(q, a.exp_scaling);
+    magnitude = qmag(q);
+    if (adaptive && !isfinite(magnitude)) r.overflow = true;
+    it++;
+    r.iters++;
+    if ((a.flags & QJ_MAP_BLACK_HOLE) && magnitude < kBlackHole) it = a.max_iter;
+  }
+  r.F = log(magnitude);
+  return r;
+}
+
+
+}  // namespace qj

@@ -23,11 +23,11 @@
 #include "qj_geom.h"
 #include "qj_internal.h"
 #include "qj_quat.h"
+#include "qj_eval.h"
 
 namespace qj {
 
 constexpr int kTileX = 16, kTileY = 4, kTileZ = 4, kBlock = 256;
-constexpr double kBlackHole = 10.0 * (double)FLT_MIN;  // 10 * REAL_MIN, SETTINGS.h:58, OPTIMIZE_3D.cpp:1358
 
 // ---------------------------------------------------------------- geometry
 
@@ -39,7 +39,7 @@ struct GridSource {
     const int y = (int)(r / sx);
     const int x = (int)(r - (int64_t)y * sx);
     double c[3];
-    cell_center(a, x, y, z, c);
+    cell_center(a, x, y, z + a.zoff, c);
     return {c[0], c[1], c[2], a.slice};
   }
 };
@@ -91,81 +91,6 @@ __device__ __forceinline__ void block_append(bool flag, uint32_t value, uint32_t
   }
   __syncthreads();
   if (flag) list[s_base + s_cnt[wave] + rank] = value;
-}
-
-// ---------------------------------------------------------------- FP64 method
-
-// POLYNOMIAL_4D::evaluateScaledPowerFactored (POLYNOMIAL_4D.cpp:912-923)
-// Once a component of the running product is NaN every later product is NaN
-// in all four components (each output component reads all four inputs), and
-// so are T B^-1 and |q|: the loop can stop there with the same F.
-__device__ __forceinline__ bool has_nan(const q64& q) { return isnan(q.w) || isnan(q.x) || isnan(q.y) || isnan(q.z); }
-__device__ __forceinline__ q64 poly64(const RootRec64* __restrict__ r, int n, const q64& q) {
-  q64 res = qpow(qsub(q, q64{r[0].w, r[0].x, r[0].y, r[0].z}), r[0].e);
-  for (int i = 1; i < n; i++) {
-    res = qmul(res, qpow(qsub(q, q64{r[i].w, r[i].x, r[i].y, r[i].z}), r[i].e));
-    if ((i & 7) == 0 && has_nan(res)) {
-      const double nan = __builtin_nan("");
-      return q64{nan, nan, nan, nan};
-    }
-  }
-  return res;
-}
-
-__device__ __forceinline__ bool finite4(const q64& q) {
-  return isfinite(q.w) && isfinite(q.x) && isfinite(q.y) && isfinite(q.z);
-}
-
-struct Res64 {
-  double F;
-  int iters;
-  bool written;
-  bool overflow;
-};
-
-// One voxel of computeLogScaledPowerRationalMap (OPTIMIZE_3D.cpp:1317-1362) /
-// computeRationalMapAP's first pass (:1423-1475). Overflow detection replaces
-// fetestexcept(FE_OVERFLOW): any non-finite T, B, |B|^2, quotient or |q|.
-__device__ __forceinline__ Res64 eval64(const MapArgs& a, const RootRec64* __restrict__ roots, q64 q) {
-  Res64 r{0.0, 0, true, false};
-  double magnitude = qmag(q);
-  if ((a.flags & QJ_MAP_SKIP_ORIGIN) && magnitude < 1e-8) {
-    r.written = false;
-    return r;
-  }
-  const bool adaptive = (a.flags & QJ_MAP_ADAPTIVE) != 0;
-  int it = 0;
-  while (magnitude < a.escape && it < a.max_iter) {
-    const q64 T = poly64(roots, a.nt, q);
-    if (a.nb > 0) {
-      if (has_nan(T) && !adaptive) {
-        // T B^-1 is all-NaN whatever B is; skip evaluating B
-        const double nan = __builtin_nan("");
-        q = q64{nan, nan, nan, nan};
-        magnitude = nan;
-        it++;
-        r.iters++;
-        break;
-      }
-      const q64 B = poly64(roots + a.nt, a.nb, q);
-      q = qdiv(T, B);
-      if (adaptive) {
-        const double m2 = B.w * B.w + B.x * B.x + B.y * B.y + B.z * B.z;
-        r.overflow |= !(finite4(T) && finite4(B) && isfinite(m2) && finite4(q));
-      }
-    } else {
-      q = T;
-      if (adaptive) r.overflow |= !finite4(T);
-    }
-    q = qscale(q, a.exp_scaling);
-    magnitude = qmag(q);
-    if (adaptive && !isfinite(magnitude)) r.overflow = true;
-    it++;
-    r.iters++;
-    if ((a.flags & QJ_MAP_BLACK_HOLE) && magnitude < kBlackHole) it = a.max_iter;
-  }
-  r.F = log(magnitude);
-  return r;
 }
 
 // ---------------------------------------------------------------- FP64 extended range
@@ -434,7 +359,7 @@ __global__ __launch_bounds__(kBlock) void k_grid64(MapArgs a, const RootRec64* _
   const int64_t idx = x + (int64_t)y * a.res[0] + (int64_t)z * a.res[0] * a.res[1];
   if (valid) {
     double c[3];
-    cell_center(a, x, y, z, c);
+    cell_center(a, x, y, z + a.zoff, c);
     const Res64 r = eval64(a, roots, q64{c[0], c[1], c[2], a.slice});
     if (r.written) {
       out[idx] = r.F;
@@ -462,7 +387,7 @@ __global__ __launch_bounds__(kBlock) void k_grid32(MapArgs a, Consts32 k, const 
   const int64_t idx = x + (int64_t)y * a.res[0] + (int64_t)z * a.res[0] * a.res[1];
   if (valid) {
     double c[3];
-    cell_center(a, x, y, z, c);
+    cell_center(a, x, y, z + a.zoff, c);
     const Res32 r = eval32(a, roots, k, q64{c[0], c[1], c[2], a.slice});
     if (r.written) {
       if (!r.need64) out[idx] = (double)r.F;

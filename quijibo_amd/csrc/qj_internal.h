@@ -28,6 +28,8 @@ struct RootRec32 {
 struct MapArgs {
   int32_t res[3];
   int32_t max_iter;
+  int32_t zoff;      // first z slice of the evaluated range within the grid (cell centres use z + zoff)
+  int32_t pad_;
   double lo[3];      // center - 0.5 * lengths (FIELD_3D.cpp:822-825)
   double dx[3];      // lengths / res (FIELD_3D.cpp:50-52)
   double center[3];
@@ -139,6 +141,8 @@ qj_status launch_batch(const ChunkDesc* d_desc, int nchunks, int chunk_voxels, q
 qj_status mc_run(qj_workspace* ws, const double* d_field, const qj_grid* g, double iso, int64_t* nv,
                  int64_t* nt, hipStream_t s);
 qj_status mc_copy(qj_workspace* ws, double* v, int32_t* t, hipStream_t s);
+qj_status nlmc_run(qj_workspace* ws, const MapArgs& a, const qj_rational* f, double iso, qj_precision prec,
+                   double band, int chunk, int64_t* nv, int64_t* nt, int64_t* n_bad, hipStream_t s);
 qj_status mc_ptrs(qj_workspace* ws, const double** v, const int32_t** t);
 void mc_free(qj_workspace* ws);
 

@@ -16,6 +16,7 @@
 //                   cell then table order, degenerate ones (edge < 1e-7) dropped
 #include <math.h>
 
+#include "qj_eval.h"
 #include "qj_geom.h"
 #include "qj_internal.h"
 #include "qj_mc_table.h"
@@ -75,6 +76,12 @@ struct McState {
   size_t tmp_bytes = 0;
   uint32_t* totals = nullptr;  // [0] active [1] vertices [2] triangles pre-filter [3] triangles
   int64_t nv = 0, nt = 0;
+  // non-linear mesher: slab of field values, slice offsets, error counter
+  double* slab = nullptr;
+  int64_t cap_slab = 0;
+  uint32_t* slice_off = nullptr;
+  int64_t cap_slice = 0;
+  unsigned int* d_bad = nullptr;
 };
 
 __device__ __forceinline__ int cell_flag(const double* __restrict__ f, int64_t i, int64_t X, int64_t XY, double iso) {
@@ -396,8 +403,370 @@ void mc_free(qj_workspace* ws) {
   hipFree(m->tris);
   hipFree(m->tmp);
   hipFree(m->totals);
+  hipFree(m->slab);
+  hipFree(m->slice_off);
+  hipFree(m->d_bad);
   delete m;
   ws->mc = nullptr;
+}
+
+
+// ---------------------------------------------------------------- non-linear low-memory mesher
+
+// TRIANGLE_MESH::computeNonlinearMarchingCubesLowMemoryHuge (TRIANGLE_MESH.cpp:681-768)
+// streamed over z-chunks: for chunk [z0, z1) the map is evaluated on slices
+// [z0-1, z1+1] (one halo slice below for the edges at z0, two above for the
+// edges of the points at z1, whose ids belong to the next chunk), cases come
+// from the raw slab values log|q| (TRIANGLE_MESH.cpp:379), and every vertex is
+// placed by bisection on re-evaluated values log|q| - iso (nonlinearValue
+// :1755-1795, midpointSearch :2232-2271, at most 6 halvings, early exit at
+// |value| < 1e-8). Vertex numbering, triangle order and the degenerate filter
+// are those of the linear mesher.
+
+__device__ __forceinline__ double nonlinear_value(const MapArgs& a, const RootRec64* __restrict__ roots,
+                                                  const double* p, double iso) {
+  const Res64 r = eval64(a, roots, q64{p[0], p[1], p[2], a.slice});
+  return r.F - iso;
+}
+
+// cases of cells in slab slices [0, ns-1) (global z = s0 + local z)
+__global__ __launch_bounds__(kMcBlock) void k_nl_flags(const double* __restrict__ slab, int64_t X, int64_t Y,
+                                                       int64_t Z, int64_t s0, int64_t ns,
+                                                       uint8_t* __restrict__ flags) {
+  const int64_t XY = X * Y, N = XY * ns;
+  const int64_t p = (int64_t)blockIdx.x * kMcBlock + threadIdx.x;
+  if (p >= N) return;
+  const int64_t lz = p / XY, r = p - lz * XY, y = r / X, x = r - y * X;
+  int flag = 0;
+  if (x < X - 1 && y < Y - 1 && s0 + lz < Z - 1 && lz + 1 < ns) flag = cell_flag(slab, p, X, XY, 0.0);
+  flags[p] = (uint8_t)flag;
+}
+
+// edge masks of points in global slices [p0, p0 + np); flags are slab-local with slab start s0
+__global__ __launch_bounds__(kMcBlock) void k_nl_edges(const uint8_t* __restrict__ flags, int64_t X, int64_t Y,
+                                                       int64_t Z, int64_t s0, int64_t p0, int64_t np,
+                                                       uint8_t* __restrict__ emask, uint32_t* __restrict__ cnt) {
+  const int64_t XY = X * Y, N = XY * np;
+  const int64_t j = (int64_t)blockIdx.x * kMcBlock + threadIdx.x;
+  if (j >= N) return;
+  const int64_t lz = j / XY, r = j - lz * XY, y = r / X, x = r - y * X;
+  const int64_t z = p0 + lz;
+  const int64_t f = (z - s0) * XY + y * X + x;  // slab-local index of the point
+  unsigned bits = 0;
+  if (x < X - 1) {
+    unsigned ref = 0;
+    for (int b = 0; b < 2; b++)
+      for (int a = 0; a < 2; a++)
+        if (y >= a && z - b >= s0) ref |= (kMc.emask[flags[f - a * X - b * XY]] >> (a + 2 * b)) & 1u;
+    bits |= ref;
+  }
+  if (y < Y - 1) {
+    unsigned ref = 0;
+    for (int b = 0; b < 2; b++)
+      for (int a = 0; a < 2; a++)
+        if (x >= a && z - b >= s0) ref |= (kMc.emask[flags[f - a - b * XY]] >> (4 + a + 2 * b)) & 1u;
+    bits |= ref << 1;
+  }
+  if (z < Z - 1) {
+    unsigned ref = 0;
+    for (int b = 0; b < 2; b++)
+      for (int a = 0; a < 2; a++)
+        if (x >= a && y >= b) ref |= (kMc.emask[flags[f - a - b * X]] >> (8 + a + 2 * b)) & 1u;
+    bits |= ref << 2;
+  }
+  emask[j] = (uint8_t)bits;
+  cnt[j] = (bits & 1u) + ((bits >> 1) & 1u) + ((bits >> 2) & 1u);
+}
+
+// vertices of the owned points [p0, p0 + np) by bisection (computeNonlinearEdgeInterpolationsHuge, :2034-2123)
+__global__ __launch_bounds__(kMcBlock) void k_nl_vertices(MapArgs a, const RootRec64* __restrict__ roots,
+                                                          double iso, int64_t p0, int64_t np,
+                                                          const uint8_t* __restrict__ emask,
+                                                          const uint32_t* __restrict__ off, uint32_t base,
+                                                          double* __restrict__ verts,
+                                                          unsigned int* __restrict__ bad) {
+#pragma clang fp contract(off)
+  const int64_t X = a.res[0], XY = X * a.res[1];
+  const int64_t j = (int64_t)blockIdx.x * kMcBlock + threadIdx.x;
+  if (j >= XY * np) return;
+  const unsigned bits = emask[j];
+  if (!bits) return;
+  const int64_t lz = j / XY, r = j - lz * XY;
+  const int y = (int)(r / X), x = (int)(r - (int64_t)y * X), z = (int)(p0 + lz);
+  double pa[3];
+  cell_center(a, x, y, z, pa);
+  const double va = nonlinear_value(a, roots, pa, iso);
+  uint32_t k = base + off[j];
+  for (int axis = 0; axis < 3; axis++) {
+    if (!(bits & (1u << axis))) continue;
+    double pb[3];
+    cell_center(a, x + (axis == 0), y + (axis == 1), z + (axis == 2), pb);
+    const double vb = nonlinear_value(a, roots, pb, iso);
+    // first = lower endpoint; the positive end is the first unless firstValue < 0
+    double pp[3], pn[3], vp, vn;
+    if (va < 0) {
+      pp[0] = pb[0]; pp[1] = pb[1]; pp[2] = pb[2]; vp = vb;
+      pn[0] = pa[0]; pn[1] = pa[1]; pn[2] = pa[2]; vn = va;
+    } else {
+      pp[0] = pa[0]; pp[1] = pa[1]; pp[2] = pa[2]; vp = va;
+      pn[0] = pb[0]; pn[1] = pb[1]; pn[2] = pb[2]; vn = vb;
+    }
+    if (va * vb >= 0.0) atomicAdd(bad, 1u);  // the reference exits here (:2082-2097)
+    double m[3];
+    for (int rec = 0;; rec++) {
+      m[0] = pp[0] + pn[0];
+      m[1] = pp[1] + pn[1];
+      m[2] = pp[2] + pn[2];
+      m[0] *= 0.5;
+      m[1] *= 0.5;
+      m[2] *= 0.5;
+      if (vp * vn >= 0.0) {
+        if (rec > 0) atomicAdd(bad, 1u);
+        break;
+      }
+      if (rec >= 6) break;
+      const double vm = nonlinear_value(a, roots, m, iso);
+      if (fabs(vm) < 1e-8) break;
+      if (vm < 0) {
+        pn[0] = m[0]; pn[1] = m[1]; pn[2] = m[2]; vn = vm;
+      } else {
+        pp[0] = m[0]; pp[1] = m[1]; pp[2] = m[2]; vp = vm;
+      }
+    }
+    verts[3 * (int64_t)k + 0] = m[0];
+    verts[3 * (int64_t)k + 1] = m[1];
+    verts[3 * (int64_t)k + 2] = m[2];
+    k++;
+  }
+}
+
+// per-block active counts of cells in global slices [c0, c0 + nc) (slab-local flags from s0)
+__global__ __launch_bounds__(kMcBlock) void k_nl_active_count(const uint8_t* __restrict__ flags, int64_t XY,
+                                                              int64_t c0, int64_t s0, int64_t nc,
+                                                              uint32_t* __restrict__ blk) {
+  __shared__ uint32_t s_n[kMcBlock / 64];
+  const int64_t j = (int64_t)blockIdx.x * kMcBlock + threadIdx.x;
+  const bool act = j < XY * nc && kMc.ntri[flags[(c0 - s0) * XY + j]] > 0;
+  const unsigned long long m = __ballot(act);
+  if ((threadIdx.x & 63) == 0) s_n[threadIdx.x >> 6] = (uint32_t)__popcll(m);
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    uint32_t t = 0;
+    for (int i = 0; i < kMcBlock / 64; i++) t += s_n[i];
+    blk[blockIdx.x] = t;
+  }
+}
+
+__global__ __launch_bounds__(kMcBlock) void k_nl_active(const uint8_t* __restrict__ flags, int64_t XY, int64_t c0,
+                                                        int64_t s0, int64_t nc, const uint32_t* __restrict__ blk_off,
+                                                        uint32_t* __restrict__ active) {
+  __shared__ uint32_t s_n[kMcBlock / 64];
+  const int64_t j = (int64_t)blockIdx.x * kMcBlock + threadIdx.x;
+  const bool act = j < XY * nc && kMc.ntri[flags[(c0 - s0) * XY + j]] > 0;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const unsigned long long m = __ballot(act);
+  if (lane == 0) s_n[wave] = (uint32_t)__popcll(m);
+  __syncthreads();
+  uint32_t o = blk_off[blockIdx.x];
+  for (int i = 0; i < wave; i++) o += s_n[i];
+  if (act) active[o + __popcll(m & ((1ull << lane) - 1ull))] = (uint32_t)j;  // chunk-local cell index
+}
+
+__global__ __launch_bounds__(kMcBlock) void k_nl_tricount(const uint32_t* __restrict__ active, int64_t na,
+                                                          const uint8_t* __restrict__ flags, int64_t fo,
+                                                          uint32_t* __restrict__ tcount) {
+  const int64_t i = (int64_t)blockIdx.x * kMcBlock + threadIdx.x;
+  if (i < na) tcount[i] = kMc.ntri[flags[fo + active[i]]];
+}
+
+// triangles of the chunk's active cells; owners at global z < z1 use base + off,
+// owners at z == z1 use base_next + slice_off
+__global__ __launch_bounds__(kMcBlock) void k_nl_triangles(
+    const uint32_t* __restrict__ active, int64_t na, const uint8_t* __restrict__ flags, int64_t fo, int64_t X,
+    int64_t XY, int64_t z0, int64_t z1, const uint8_t* __restrict__ emask, const uint32_t* __restrict__ off,
+    uint32_t base, const uint8_t* __restrict__ emask_next, const uint32_t* __restrict__ slice_off,
+    uint32_t base_next, const double* __restrict__ verts, const uint32_t* __restrict__ toff,
+    int32_t* __restrict__ tris_pre, uint32_t* __restrict__ keep) {
+  const int64_t i = (int64_t)blockIdx.x * kMcBlock + threadIdx.x;
+  if (i >= na) return;
+  const int64_t c = active[i];  // chunk-local cell index: (z - z0) * XY + y * X + x
+  const int flag = flags[fo + c];
+  const int n = kMc.ntri[flag];
+  uint32_t o = toff[i];
+  for (int t = 0; t < n; t++) {
+    int32_t v[3];
+    for (int q = 0; q < 3; q++) {
+      int dx, dy, dz, axis;
+      edge_geom(kMc.tri[flag][3 * t + q], dx, dy, dz, axis);
+      const int64_t owner = c + dx + dy * X + dz * XY;  // chunk-local point index
+      const unsigned lowbits = (1u << axis) - 1u;
+      if (z0 + owner / XY < z1) {
+        v[q] = (int32_t)(base + off[owner] + __popc(emask[owner] & lowbits));
+      } else {
+        const int64_t so = owner - (z1 - z0) * XY;
+        v[q] = (int32_t)(base_next + slice_off[so] + __popc(emask_next[so] & lowbits));
+      }
+    }
+    const double eps = 1e-7;
+    const double* p0 = verts + 3 * (int64_t)v[0];
+    const double* p1 = verts + 3 * (int64_t)v[1];
+    const double* p2 = verts + 3 * (int64_t)v[2];
+    const bool degenerate = edge_len(p0, p1) < eps || edge_len(p0, p2) < eps || edge_len(p1, p2) < eps;
+    tris_pre[3 * (int64_t)o + 0] = v[0];
+    tris_pre[3 * (int64_t)o + 1] = v[1];
+    tris_pre[3 * (int64_t)o + 2] = v[2];
+    keep[o] = degenerate ? 0u : 1u;
+    o++;
+  }
+}
+
+template <class T>
+static qj_status grow_keep(T*& p, int64_t& cap, int64_t n, int64_t used, hipStream_t s) {
+  if (n <= cap) return QJ_OK;
+  const int64_t want = n + n / 2 + 4096;
+  T* q = nullptr;
+  QJ_HIP(hipMalloc(&q, sizeof(T) * (size_t)want));
+  if (p && used > 0) QJ_HIP(hipMemcpyAsync(q, p, sizeof(T) * (size_t)used, hipMemcpyDeviceToDevice, s));
+  QJ_HIP(hipStreamSynchronize(s));
+  if (p) QJ_HIP(hipFree(p));
+  p = q;
+  cap = want;
+  return QJ_OK;
+}
+
+qj_status nlmc_run(qj_workspace* ws, const MapArgs& a0, const qj_rational* f, double iso, qj_precision prec,
+                   double band, int chunk, int64_t* nv_out, int64_t* nt_out, int64_t* n_bad, hipStream_t s) {
+  if (!ws->mc) ws->mc = new McState();
+  McState& m = *(McState*)ws->mc;
+  m.nv = m.nt = 0;
+  const int64_t X = a0.res[0], Y = a0.res[1], Z = a0.res[2], XY = X * Y;
+  *nv_out = *nt_out = 0;
+  *n_bad = 0;
+  if (X < 2 || Y < 2 || Z < 2) return QJ_OK;
+  if (chunk < 1) chunk = 64;
+  if (XY * (int64_t)(chunk + 3) > (int64_t)UINT32_MAX) {
+    set_error("non-linear mesher: slab of %lld points exceeds 32-bit indices", (long long)(XY * (chunk + 3)));
+    return QJ_ERR_CAPACITY;
+  }
+  const int64_t slab_pts = XY * (chunk + 3), own_pts = XY * (chunk + 1);
+  qj_status st;
+  if ((st = grow(m.slab, m.cap_slab, slab_pts, s)) || (st = grow(m.flags, m.cap_flags, slab_pts, s)) ||
+      (st = grow(m.emask, m.cap_emask, own_pts, s)) || (st = grow(m.cnt, m.cap_cnt, own_pts, s)) ||
+      (st = grow(m.active, m.cap_active, own_pts, s)) || (st = grow(m.tri_off, m.cap_toff, own_pts, s)) ||
+      (st = grow(m.blk, m.cap_blk, (int64_t)nblocks(own_pts), s)) ||
+      (st = grow(m.slice_off, m.cap_slice, XY, s)))
+    return st;
+  const size_t need_tmp = scan_tmp_bytes(5 * own_pts);
+  if (need_tmp > m.tmp_bytes) {
+    if (m.tmp) {
+      QJ_HIP(hipStreamSynchronize(s));
+      QJ_HIP(hipFree(m.tmp));
+    }
+    m.tmp = nullptr;
+    m.tmp_bytes = 0;
+    QJ_HIP(hipMalloc(&m.tmp, need_tmp));
+    m.tmp_bytes = need_tmp;
+  }
+  if (!m.totals) QJ_HIP(hipMalloc(&m.totals, 4 * sizeof(uint32_t)));
+  if (!m.d_bad) QJ_HIP(hipMalloc(&m.d_bad, sizeof(unsigned int)));
+  QJ_HIP(hipMemsetAsync(m.d_bad, 0, sizeof(unsigned int), s));
+  // the mesher's map: escape 2000 and the black-hole stop (TRIANGLE_MESH.cpp:97, 1789-1791)
+  MapArgs a = a0;
+  uint64_t nv = 0, nt = 0;
+  for (int64_t z0 = 0; z0 < Z; z0 += chunk) {
+    const int64_t z1 = z0 + chunk < Z ? z0 + chunk : Z;
+    const int64_t s0 = z0 > 0 ? z0 - 1 : 0;
+    const int64_t s1 = z1 + 1 < Z - 1 ? z1 + 1 : Z - 1;
+    const int64_t ns = s1 - s0 + 1;
+    // 1. slab of log|q|
+    MapArgs sa = a;
+    sa.res[2] = (int32_t)ns;
+    sa.zoff = (int32_t)s0;
+    if ((st = launch_field(ws, sa, f, prec, band, m.slab, s))) return st;
+    // 2. cases
+    hipLaunchKernelGGL(k_nl_flags, dim3(nblocks(XY * ns)), dim3(kMcBlock), 0, s, m.slab, X, Y, Z, s0, ns, m.flags);
+    QJ_HIP(hipGetLastError());
+    // 3. edge masks of points [z0, z1] (z1 only when it exists)
+    const int64_t np = (z1 < Z ? z1 + 1 : Z) - z0;
+    hipLaunchKernelGGL(k_nl_edges, dim3(nblocks(XY * np)), dim3(kMcBlock), 0, s, m.flags, X, Y, Z, s0, z0, np,
+                       m.emask, m.cnt);
+    QJ_HIP(hipGetLastError());
+    // 4. vertex ids: owned points [z0, z1), and the next chunk's first slice separately
+    const int64_t nown = XY * (z1 - z0);
+    if (z1 < Z) {
+      if ((st = scan_u32(m.cnt + nown, m.slice_off, XY, m.totals + 3, m.tmp, m.tmp_bytes, s))) return st;
+    }
+    if ((st = scan_u32(m.cnt, m.cnt, nown, m.totals + 1, m.tmp, m.tmp_bytes, s))) return st;
+    uint32_t h[4];
+    QJ_HIP(hipMemcpyAsync(h + 1, m.totals + 1, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+    QJ_HIP(hipStreamSynchronize(s));
+    const uint64_t cnt_own = h[1];
+    if (nv + cnt_own > (uint64_t)INT32_MAX) return QJ_ERR_CAPACITY;
+    if ((st = grow_keep(m.verts, m.cap_v, 3 * (int64_t)(nv + cnt_own), 3 * (int64_t)nv, s))) return st;
+    if (cnt_own > 0) {
+      hipLaunchKernelGGL(k_nl_vertices, dim3(nblocks(nown)), dim3(kMcBlock), 0, s, a, f->d64, iso, z0,
+                         z1 - z0, m.emask, m.cnt, (uint32_t)nv, m.verts, m.d_bad);
+      QJ_HIP(hipGetLastError());
+    }
+    // 5. active cells of the chunk, in cell order
+    const int64_t cz1 = z1 < Z - 1 ? z1 : Z - 1;  // cells exist for z <= Z-2
+    const int64_t nc = cz1 > z0 ? cz1 - z0 : 0;
+    int64_t na = 0;
+    if (nc > 0) {
+      const int64_t nb = nblocks(XY * nc);
+      hipLaunchKernelGGL(k_nl_active_count, dim3((unsigned)nb), dim3(kMcBlock), 0, s, m.flags, XY, z0, s0, nc,
+                         m.blk);
+      QJ_HIP(hipGetLastError());
+      if ((st = scan_u32(m.blk, m.blk, nb, m.totals + 0, m.tmp, m.tmp_bytes, s))) return st;
+      hipLaunchKernelGGL(k_nl_active, dim3((unsigned)nb), dim3(kMcBlock), 0, s, m.flags, XY, z0, s0, nc, m.blk,
+                         m.active);
+      QJ_HIP(hipGetLastError());
+      QJ_HIP(hipMemcpyAsync(h, m.totals, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+      QJ_HIP(hipStreamSynchronize(s));
+      na = h[0];
+    }
+    if (na > 0) {
+      const int64_t fo = (z0 - s0) * XY;
+      hipLaunchKernelGGL(k_nl_tricount, dim3(nblocks(na)), dim3(kMcBlock), 0, s, m.active, na, m.flags, fo,
+                         m.tri_off);
+      QJ_HIP(hipGetLastError());
+      if ((st = scan_u32(m.tri_off, m.tri_off, na, m.totals + 2, m.tmp, m.tmp_bytes, s))) return st;
+      QJ_HIP(hipMemcpyAsync(h + 2, m.totals + 2, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+      QJ_HIP(hipStreamSynchronize(s));
+      const int64_t ntp = h[2];
+      if ((st = grow(m.tris_pre, m.cap_tp, 3 * ntp, s)) || (st = grow(m.keep, m.cap_keep, ntp, s))) return st;
+      hipLaunchKernelGGL(k_nl_triangles, dim3(nblocks(na)), dim3(kMcBlock), 0, s, m.active, na, m.flags, fo, X, XY,
+                         z0, z1, m.emask, m.cnt, (uint32_t)nv, m.emask + nown, m.slice_off,
+                         (uint32_t)(nv + cnt_own), m.verts, m.tri_off, m.tris_pre, m.keep);
+      QJ_HIP(hipGetLastError());
+      if ((st = scan_u32(m.keep, m.keep, ntp, m.totals + 3, m.tmp, m.tmp_bytes, s))) return st;
+      QJ_HIP(hipMemcpyAsync(h + 3, m.totals + 3, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+      QJ_HIP(hipStreamSynchronize(s));
+      const int64_t ntk = h[3];
+      if ((st = grow_keep(m.tris, m.cap_t, 3 * (int64_t)(nt + ntk), 3 * (int64_t)nt, s))) return st;
+      if (ntk > 0) {
+        hipLaunchKernelGGL(k_mc_squeeze, dim3(nblocks(ntp)), dim3(kMcBlock), 0, s, m.tris_pre, m.keep, ntp,
+                           m.totals + 3, m.tris + 3 * nt);
+        QJ_HIP(hipGetLastError());
+      }
+      nt += ntk;
+    }
+    nv += cnt_own;
+  }
+  unsigned int bad = 0;
+  QJ_HIP(hipMemcpyAsync(&bad, m.d_bad, sizeof(bad), hipMemcpyDeviceToHost, s));
+  QJ_HIP(hipStreamSynchronize(s));
+  m.nv = (int64_t)nv;
+  m.nt = (int64_t)nt;
+  *nv_out = m.nv;
+  *nt_out = m.nt;
+  *n_bad = bad;
+  if (bad) {
+    set_error("non-linear mesher: %u edges whose end values do not straddle the isosurface "
+              "(the reference exits at TRIANGLE_MESH.cpp:2082-2097)", bad);
+    return QJ_ERR_NUMERIC;
+  }
+  return QJ_OK;
 }
 
 }  // namespace qj

@@ -182,3 +182,25 @@ def test_float_sign_predicate():
     f[0] = 0.5
     v, t = O.marching_cubes(g, f)
     assert len(v) == 3 and len(t) == 1
+
+
+def test_nonlinear_mesher_sphere_closed_form():
+    """One root, e = 1, one iteration: F = log(c |q - r|), so the surface is the
+    sphere |q - r| = 1/c. Bisection vertices lie within edge/2^7 of it; the
+    topology equals the linear mesher's on the same field."""
+    n = 24
+    g = O.make_grid((n, n, n), (0, 0, 0), (3, 3, 3))
+    r = np.array([[0.11, -0.07, 0.05, 0.0]])
+    c = 1.0 / 0.9
+    top = O.poly(r)
+    empty = O.poly(np.zeros((0, 4)))
+    p = O.make_params(1, 2000.0, c, 0.0, O.BLACK_HOLE)
+    v, t, bad = O.nonlinear_mc(g, top, empty, p)
+    assert bad == 0 and len(t) > 500
+    rad = np.linalg.norm(v - r[0, :3], axis=1)
+    dx = 3.0 / n
+    assert np.all(np.abs(rad - 0.9) <= dx / 2 ** 7 * 1.0001)
+    f = O.field_map(g, top, empty, p)
+    lv, lt = O.marching_cubes(g, f)
+    assert np.array_equal(lt, t) and len(lv) == len(v)
+    assert np.max(np.abs(np.linalg.norm(lv - r[0, :3], axis=1) - 0.9)) > dx / 2 ** 7  # linear is worse
