@@ -693,19 +693,31 @@ qj_status nlmc_run(qj_workspace* ws, const MapArgs& a0, const qj_rational* f, do
     QJ_HIP(hipGetLastError());
     // 4. vertex ids: owned points [z0, z1), and the next chunk's first slice separately
     const int64_t nown = XY * (z1 - z0);
+    uint32_t h[4] = {0, 0, 0, 0};
     if (z1 < Z) {
       if ((st = scan_u32(m.cnt + nown, m.slice_off, XY, m.totals + 3, m.tmp, m.tmp_bytes, s))) return st;
+      QJ_HIP(hipMemcpyAsync(h + 3, m.totals + 3, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
     }
     if ((st = scan_u32(m.cnt, m.cnt, nown, m.totals + 1, m.tmp, m.tmp_bytes, s))) return st;
-    uint32_t h[4];
     QJ_HIP(hipMemcpyAsync(h + 1, m.totals + 1, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
     QJ_HIP(hipStreamSynchronize(s));
-    const uint64_t cnt_own = h[1];
-    if (nv + cnt_own > (uint64_t)INT32_MAX) return QJ_ERR_CAPACITY;
-    if ((st = grow_keep(m.verts, m.cap_v, 3 * (int64_t)(nv + cnt_own), 3 * (int64_t)nv, s))) return st;
-    if (cnt_own > 0) {
-      hipLaunchKernelGGL(k_nl_vertices, dim3(nblocks(nown)), dim3(kMcBlock), 0, s, a, f->d64, iso, z0,
-                         z1 - z0, m.emask, m.cnt, (uint32_t)nv, m.verts, m.d_bad);
+    const uint64_t cnt_own = h[1], cnt_next = h[3];
+    if (nv + cnt_own + cnt_next > (uint64_t)INT32_MAX) return QJ_ERR_CAPACITY;
+    // The chunk's triangles reach the first slice of the next chunk, whose
+    // vertices the degenerate filter reads: place them now (ids base_next +
+    // slice_off, the ids the next chunk's numbering gives them) and skip the
+    // chunk's own first slice, placed by the previous chunk.
+    if ((st = grow_keep(m.verts, m.cap_v, 3 * (int64_t)(nv + cnt_own + cnt_next), 3 * (int64_t)nv, s))) return st;
+    const int64_t skip = z0 > 0 ? 1 : 0;
+    if (cnt_own > 0 && z1 - z0 > skip) {
+      hipLaunchKernelGGL(k_nl_vertices, dim3(nblocks(XY * (z1 - z0 - skip))), dim3(kMcBlock), 0, s, a, f->d64, iso,
+                         z0 + skip, z1 - z0 - skip, m.emask + skip * XY, m.cnt + skip * XY, (uint32_t)nv, m.verts,
+                         m.d_bad);
+      QJ_HIP(hipGetLastError());
+    }
+    if (cnt_next > 0) {
+      hipLaunchKernelGGL(k_nl_vertices, dim3(nblocks(XY)), dim3(kMcBlock), 0, s, a, f->d64, iso, z1, (int64_t)1,
+                         m.emask + nown, m.slice_off, (uint32_t)(nv + cnt_own), m.verts, m.d_bad);
       QJ_HIP(hipGetLastError());
     }
     // 5. active cells of the chunk, in cell order
