@@ -3,8 +3,8 @@
 // Class and member names follow the reference so a caller of the reference can
 // switch: POLYNOMIAL_4D (src/algebra/POLYNOMIAL_4D.h), FIELD_3D
 // (src/geometry/FIELD_3D.h), TRIANGLE_MESH (src/geometry/TRIANGLE_MESH.h),
-// OPTIMIZE_3D (src/solvers/OPTIMIZE_3D.h, the map / score / gradient members)
-// and the TOUPEE optimizer callback (src/TOUPEE/TAO_SOLVER.h:41-43).
+// and OPTIMIZE_3D (src/solvers/OPTIMIZE_3D.h, the map / score / gradient
+// members and the .o3d file).
 // Real is double (the reference's USING_DOUBLE_NLS build). All field work runs
 // on the GPU through qj_* calls; data that the reference keeps in host memory
 // (FIELD_3D's cells, meshes) is copied back to host memory here.
@@ -95,6 +95,7 @@ class FIELD_3D {
   FIELD_3D(int xRes, int yRes, int zRes, const VEC3F& center = VEC3F(0, 0, 0),
            const VEC3F& lengths = VEC3F(1, 1, 1));
   FIELD_3D(const Real* data, int xRes, int yRes, int zRes, const VEC3F& center, const VEC3F& lengths);
+  FIELD_3D& operator=(Real value);  // fill
 
   Real& operator()(int x, int y, int z) { return _data[x + y * (size_t)_xRes + z * (size_t)_xRes * _yRes]; }
   Real operator()(int x, int y, int z) const { return _data[x + y * (size_t)_xRes + z * (size_t)_xRes * _yRes]; }
@@ -117,10 +118,15 @@ class FIELD_3D {
   const Real* dataConst() const { return _data.data(); }
   VEC3F cellCenter(int x, int y, int z) const;  // FIELD_3D.cpp:818-841
 
-  void write(FILE* file) const;  // FIELD_3D.cpp:4371-4395
-  void read(FILE* file);         // FIELD_3D.cpp:4400-4450
-  bool write(const std::string& filename) const;
-  bool read(const std::string& filename);
+  // FIELD_3D.cpp:4371-4395 / 4400-4450: int xRes, yRes, zRes; center and
+  // lengths as 3 doubles each; then the cells as doubles, x fastest. The
+  // rotation is not stored (the reference's format has no field for it).
+  void write(FILE* file) const;
+  void read(FILE* file);
+  // FIELD_3D.cpp:280-305, 378-390: a name ending in "gz" is read and written
+  // through zlib with the same layout
+  void write(const std::string& filename) const;
+  void read(const std::string& filename);
 
   qj_grid grid() const;
 
@@ -142,11 +148,16 @@ class TRIANGLE_MESH {
   explicit TRIANGLE_MESH(const FIELD_3D& field, Real isosurface = 0.0);
   // non-linear low-memory mesher (TRIANGLE_MESH.cpp:84-101): slabs of the map
   // evaluated on the fly, vertices refined by bisection
+  // precision of the slab evaluation (vertices are always FP64 bisection);
+  // chunkSlices = z-slices evaluated per slab
   TRIANGLE_MESH(const VEC3F& center, const VEC3F& lengths, const int res[3], const POLYNOMIAL_4D& top,
                 const POLYNOMIAL_4D& bottom, Real expScaling, int maxIterations, Real slice, Real isosurface,
-                const QUATERNION& rotation = QUATERNION(1, 0, 0, 0));
+                const QUATERNION& rotation = QUATERNION(1, 0, 0, 0), qj_precision precision = QJ_FP32_BAND,
+                int chunkSlices = 64);
 
+  std::vector<VEC3F>& vertices() { return _vertices; }
   const std::vector<VEC3F>& vertices() const { return _vertices; }
+  std::vector<TRIANGLE>& triangles() { return _triangles; }
   const std::vector<TRIANGLE>& triangles() const { return _triangles; }
   bool writeOBJ(const std::string& filename) const;  // TRIANGLE_MESH.cpp:2762-2800
 
@@ -198,9 +209,20 @@ class OPTIMIZE_3D {
   Real computeBottomBulkScalarGradientAP();
   int adaptiveCells();
 
-  // .o3d layout (OPTIMIZE_3D.cpp:3254-3346)
+  // .o3d layout (OPTIMIZE_3D.cpp:3254-3346): int xRes, yRes, zRes; center;
+  // lengths; top and bottom POLYNOMIAL_4D; double expScaling; int history
+  // size (0, the history is never written); the fractal, distance and
+  // curvature FIELD_3Ds. read() restores these members; the weight field is
+  // the caller's to set (the reference derives it from the distance field,
+  // FIELD_3D.cpp:5316, outside the map/score path).
   void write(const std::string& filename) const;
   void read(const std::string& filename);
+  FIELD_3D& curvatureField() { return _curvatureField; }
+  const VEC3F& center() const { return _center; }
+  const VEC3F& lengths() const { return _lengths; }
+  int xRes() const { return _xRes; }
+  int yRes() const { return _yRes; }
+  int zRes() const { return _zRes; }
 
  private:
   void sync_rational(const POLYNOMIAL_4D& top, const POLYNOMIAL_4D& bottom);
@@ -228,28 +250,6 @@ class OPTIMIZE_3D {
   qj_ap_grads _ap{};
   std::vector<double> _gTop, _gBottom;
   std::vector<double> _apKey;
-};
-
-// TOUPEE callback ABI (TAO_SOLVER.h:41-43) with VECTOR = std::vector<Real>.
-typedef std::vector<Real> VECTOR;
-typedef void (*FunctionGradient)(const VECTOR& state, Real& value, VECTOR& gradient);
-typedef void (*Monitor)();
-
-// Bounded limited-memory variable metric minimisation (TAO_SOLVER::optimizeBLMVM,
-// TAO_SOLVER.cpp:72-341): projected quasi-Newton steps inside [xl, xu] with a
-// backtracking line search. Returns the number of iterations taken.
-class TAO_SOLVER {
- public:
-  explicit TAO_SOLVER(int maxIterations = 100, int history = 5, Real gradientTolerance = 1e-8)
-      : _maxIterations(maxIterations), _history(history), _gatol(gradientTolerance) {}
-  int optimizeBLMVM(VECTOR& x, const VECTOR& xu, const VECTOR& xl, FunctionGradient functionGradient,
-                    Monitor monitor = nullptr);
-  Real value() const { return _f; }
-
- private:
-  int _maxIterations, _history;
-  Real _gatol;
-  Real _f = 0;
 };
 
 }  // namespace quijibo

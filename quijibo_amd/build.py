@@ -14,7 +14,8 @@ LIB = os.path.join(HERE, "libquijibo.so")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("QJ_OFFLOAD_ARCH", "gfx950")
 
-SOURCES = ["qj_api.cpp", "qj_field.hip", "qj_mc.hip", "qj_scan.hip", "qj_grad.hip", "qj_server.cpp"]
+SOURCES = ["csrc/qj_api.cpp", "csrc/qj_field.hip", "csrc/qj_mc.hip", "csrc/qj_scan.hip", "csrc/qj_grad.hip",
+           "csrc/qj_server.cpp", "cpp/quijibo.cpp"]
 
 # Packed FP32 (v_pk_*_f32) is disabled for all device code.
 DEVICE_FLAGS = ["-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"]
@@ -23,8 +24,8 @@ COMMON = ["-O3", "-fPIC", "-std=c++17", "--offload-arch=" + ARCH, "-Wall", "-Wno
 
 
 def _compile(src):
-    obj = os.path.join(BUILD, src + ".o")
-    cmd = [HIPCC, "-x", "hip"] + COMMON + DEVICE_FLAGS + ["-c", os.path.join(CSRC, src), "-o", obj]
+    obj = os.path.join(BUILD, os.path.basename(src) + ".o")
+    cmd = [HIPCC, "-x", "hip"] + COMMON + DEVICE_FLAGS + ["-c", os.path.join(HERE, src), "-o", obj]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError("hipcc failed for %s:\n%s\n%s" % (src, " ".join(cmd), r.stderr[-6000:]))
@@ -40,12 +41,31 @@ def build(verbose=False):
         if verbose and err:
             print(err, file=sys.stderr)
     tmp = LIB + ".tmp"
-    cmd = [HIPCC, "--offload-arch=" + ARCH, "-shared", "-fPIC", "-o", tmp] + [o for o, _ in results] + ["-lpthread"]
+    cmd = [HIPCC, "--offload-arch=" + ARCH, "-shared", "-fPIC", "-o", tmp] + [o for o, _ in results] + ["-lpthread", "-lz"]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError("link failed:\n%s" % r.stderr[-4000:])
     os.replace(tmp, LIB)
+    build_driver()
     return LIB
+
+
+DRIVER_SRC = os.path.join(HERE, "..", "tests", "cpp", "host_layer_driver.cpp")
+DRIVER = os.path.join(HERE, "..", "tests", "cpp", "_build", "host_layer_driver")
+
+
+def build_driver():
+    """The C++ host-layer test program (a plain host executable linked to libquijibo.so)."""
+    os.makedirs(os.path.dirname(DRIVER), exist_ok=True)
+    cmd = ["g++", "-O2", "-std=c++17", "-Wall", "-I" + os.path.join(HERE, "..", "include"),
+           "-I/opt/rocm/include", "-D__HIP_PLATFORM_AMD__", DRIVER_SRC, "-o", DRIVER + ".tmp", "-L" + HERE,
+           "-lquijibo", "-Wl,-rpath," + os.path.relpath(HERE, os.path.dirname(DRIVER)).join(["$ORIGIN/", ""]),
+           "-L/opt/rocm/lib", "-lamdhip64", "-Wl,-rpath,/opt/rocm/lib"]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError("driver build failed:\n%s\n%s" % (" ".join(cmd), r.stderr[-4000:]))
+    os.replace(DRIVER + ".tmp", DRIVER)
+    return DRIVER
 
 
 if __name__ == "__main__":

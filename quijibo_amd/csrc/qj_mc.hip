@@ -672,7 +672,7 @@ qj_status nlmc_run(qj_workspace* ws, const MapArgs& a0, const qj_rational* f, do
   QJ_HIP(hipMemsetAsync(m.d_bad, 0, sizeof(unsigned int), s));
   // the mesher's map: escape 2000 and the black-hole stop (TRIANGLE_MESH.cpp:97, 1789-1791)
   MapArgs a = a0;
-  uint64_t nv = 0, nt = 0;
+  uint64_t nv = 0, nt = 0, placed_next = 0;
   for (int64_t z0 = 0; z0 < Z; z0 += chunk) {
     const int64_t z1 = z0 + chunk < Z ? z0 + chunk : Z;
     const int64_t s0 = z0 > 0 ? z0 - 1 : 0;
@@ -707,7 +707,10 @@ qj_status nlmc_run(qj_workspace* ws, const MapArgs& a0, const qj_rational* f, do
     // vertices the degenerate filter reads: place them now (ids base_next +
     // slice_off, the ids the next chunk's numbering gives them) and skip the
     // chunk's own first slice, placed by the previous chunk.
-    if ((st = grow_keep(m.verts, m.cap_v, 3 * (int64_t)(nv + cnt_own + cnt_next), 3 * (int64_t)nv, s))) return st;
+    // keep what is already placed: ids < nv plus this chunk's first slice (placed by the previous chunk)
+    if ((st = grow_keep(m.verts, m.cap_v, 3 * (int64_t)(nv + cnt_own + cnt_next), 3 * (int64_t)(nv + placed_next),
+                        s)))
+      return st;
     const int64_t skip = z0 > 0 ? 1 : 0;
     if (cnt_own > 0 && z1 - z0 > skip) {
       hipLaunchKernelGGL(k_nl_vertices, dim3(nblocks(XY * (z1 - z0 - skip))), dim3(kMcBlock), 0, s, a, f->d64, iso,
@@ -764,6 +767,7 @@ qj_status nlmc_run(qj_workspace* ws, const MapArgs& a0, const qj_rational* f, do
       nt += ntk;
     }
     nv += cnt_own;
+    placed_next = cnt_next;
   }
   unsigned int bad = 0;
   QJ_HIP(hipMemcpyAsync(&bad, m.d_bad, sizeof(bad), hipMemcpyDeviceToHost, s));

@@ -1,0 +1,168 @@
+// Test program for the C++ host layer (include/quijibo/quijibo.hpp), run by
+// tests/test_formats.py (io and copy modes, no GPU) and
+// tests/test_gpu_cpp_host_layer.py (map / ap modes, GPU).
+//
+//   host_layer_driver io  <dir>
+//       writes a POLYNOMIAL_4D, FIELD_3D (plain and .gz) and .o3d file into
+//       <dir>, reads each back and checks the round trip field by field.
+//   host_layer_driver copy <in.o3d> <out.o3d>
+//       OPTIMIZE_3D::read then OPTIMIZE_3D::write.
+//   host_layer_driver map <in.o3d> <maxIterations> <precision 0|1> <outdir>
+//       OPTIMIZE_3D::read, computeLogScaledPowerRationalMap into the fractal
+//       -> <outdir>/field.bin; TRIANGLE_MESH(fractal) -> linear.obj;
+//       TRIANGLE_MESH(center, lengths, res, top, bottom, ...) -> nonlinear.obj.
+//   host_layer_driver ap <in.o3d> <weight.field> <alpha> <threshold>
+//       one-iteration AP score and gradients printed as JSON.
+#include <math.h>
+#include <stdio.h>
+#include <string.h>
+
+#include <string>
+
+#include "quijibo/quijibo.hpp"
+
+using namespace quijibo;
+
+static int fail(const char* what) {
+  fprintf(stderr, "FAIL: %s\n", what);
+  return 1;
+}
+
+static bool same_field(const FIELD_3D& a, const FIELD_3D& b) {
+  if (a.xRes() != b.xRes() || a.yRes() != b.yRes() || a.zRes() != b.zRes()) return false;
+  for (int i = 0; i < 3; i++)
+    if (a.center()[i] != b.center()[i] || a.lengths()[i] != b.lengths()[i]) return false;
+  return memcmp(a.dataConst(), b.dataConst(), sizeof(double) * a.totalCells()) == 0;
+}
+
+static bool same_poly(const POLYNOMIAL_4D& a, const POLYNOMIAL_4D& b) {
+  if (a.totalRoots() != b.totalRoots() || a.powerScalar() != b.powerScalar()) return false;
+  for (int i = 0; i < a.totalRoots(); i++) {
+    if (a.powers()[i] != b.powers()[i]) return false;
+    for (int k = 0; k < 4; k++)
+      if (a.roots()[i][k] != b.roots()[i][k]) return false;
+  }
+  return true;
+}
+
+static int run_io(const std::string& dir) {
+  POLYNOMIAL_4D top;
+  top.addRoot(QUATERNION(0.5, -0.25, 0.125, 0.0), 1.0);
+  top.addRoot(QUATERNION(-0.75, 0.5, 0.3, 0.01), 2.0);
+  top.powerScalar() = 1.25;
+  POLYNOMIAL_4D bottom(std::vector<QUATERNION>{QUATERNION(0.1, 0.2, 0.3, 0.4)});
+  FILE* f = fopen((dir + "/poly.bin").c_str(), "wb");
+  if (!f) return fail("open poly.bin");
+  top.write(f);
+  fclose(f);
+  f = fopen((dir + "/poly.bin").c_str(), "rb");
+  POLYNOMIAL_4D back;
+  back.read(f);
+  fclose(f);
+  if (!same_poly(top, back)) return fail("POLYNOMIAL_4D round trip");
+
+  FIELD_3D field(5, 4, 3, VEC3F(0.1, -0.2, 0.3), VEC3F(3.0, 2.0, 1.5));
+  for (size_t i = 0; i < field.totalCells(); i++) field[i] = 0.5 * (double)i - 7.0 + 1e-3 * (double)(i * i);
+  field.write(dir + "/field.bin");
+  field.write(dir + "/field.bin.gz");
+  FIELD_3D a, b;
+  a.read(dir + "/field.bin");
+  b.read(dir + "/field.bin.gz");
+  if (!same_field(field, a)) return fail("FIELD_3D round trip");
+  if (!same_field(field, b)) return fail("FIELD_3D gz round trip");
+
+  OPTIMIZE_3D opt(5, 4, 3, VEC3F(0.1, -0.2, 0.3), VEC3F(3.0, 2.0, 1.5));
+  opt.top() = top;
+  opt.bottom() = bottom;
+  opt.expScaling() = 0.75;
+  opt.fractal() = field;
+  opt.distanceField() = field;
+  opt.distanceField()[0] = 42.0;
+  opt.write(dir + "/state.o3d");
+  OPTIMIZE_3D in;
+  in.read(dir + "/state.o3d");
+  if (in.xRes() != 5 || in.yRes() != 4 || in.zRes() != 3) return fail(".o3d resolution");
+  if (!same_poly(in.top(), top) || !same_poly(in.bottom(), bottom)) return fail(".o3d polynomials");
+  if (in.expScaling() != 0.75) return fail(".o3d expScaling");
+  if (!same_field(in.fractal(), field) || !same_field(in.distanceField(), opt.distanceField()))
+    return fail(".o3d fields");
+  if (in.curvatureField().totalCells() != 0) return fail(".o3d curvature field");
+
+  // an invalid file is an error, not an exit
+  FILE* bad = fopen((dir + "/short.o3d").c_str(), "wb");
+  int three[3] = {4, 4, 4};
+  fwrite(three, sizeof(int), 3, bad);
+  fclose(bad);
+  bool threw = false;
+  try {
+    OPTIMIZE_3D s;
+    s.read(dir + "/short.o3d");
+  } catch (const Error& e) {
+    threw = e.status == QJ_ERR_INVALID;
+  }
+  if (!threw) return fail("short .o3d not rejected");
+  printf("io ok\n");
+  return 0;
+}
+
+static int run_map(const std::string& in, int maxIter, int precision, const std::string& out) {
+  OPTIMIZE_3D opt;
+  opt.read(in);
+  opt.maxIterations() = maxIter;
+  opt.setPrecision(precision ? QJ_FP32_BAND : QJ_FP64);
+  FIELD_3D& fractal = opt.fractal();
+  opt.computeLogScaledPowerRationalMap();
+  fractal.write(out + "/field.bin");
+  TRIANGLE_MESH linear(fractal);
+  if (!linear.writeOBJ(out + "/linear.obj")) return fail("writeOBJ linear");
+  const int res[3] = {opt.xRes(), opt.yRes(), opt.zRes()};
+  TRIANGLE_MESH nonlinear(opt.center(), opt.lengths(), res, opt.top(), opt.bottom(), opt.expScaling(), maxIter,
+                          opt.quaternionSlice(), 0.0, QUATERNION(1, 0, 0, 0),
+                          precision ? QJ_FP32_BAND : QJ_FP64, 7);
+  if (!nonlinear.writeOBJ(out + "/nonlinear.obj")) return fail("writeOBJ nonlinear");
+  printf("map ok %zu %zu %zu %zu\n", linear.vertices().size(), linear.triangles().size(),
+         nonlinear.vertices().size(), nonlinear.triangles().size());
+  return 0;
+}
+
+static int run_ap(const std::string& in, const std::string& weight, double alpha, double thr) {
+  OPTIMIZE_3D opt;
+  opt.read(in);
+  opt.maxIterations() = 1;
+  opt.tanhAlpha() = alpha;
+  opt.tanhThreshold() = thr;
+  opt.weightField().read(weight);
+  const double score = opt.computeScoreAP();
+  printf("{\"score\": %.17g, \"grad_exp\": %.17g, \"grad_top_scalar\": %.17g, \"grad_bottom_scalar\": %.17g, ",
+         score, opt.computeExpGradientAP(), opt.computeTopBulkScalarGradientAP(),
+         opt.computeBottomBulkScalarGradientAP());
+  printf("\"adaptive_cells\": %d, \"grad_top\": [", opt.adaptiveCells());
+  for (int i = 0; i < opt.top().totalRoots(); i++) printf("%s%.17g", i ? ", " : "", opt.computeTopGradientAP(i));
+  printf("], \"grad_bottom\": [");
+  for (int i = 0; i < opt.bottom().totalRoots(); i++)
+    printf("%s%.17g", i ? ", " : "", opt.computeBottomGradientAP(i));
+  printf("]}\n");
+  return 0;
+}
+
+int main(int argc, char** argv) {
+  try {
+    const std::string mode = argc > 1 ? argv[1] : "";
+    if (mode == "io" && argc == 3) return run_io(argv[2]);
+    if (mode == "copy" && argc == 4) {
+      OPTIMIZE_3D o;
+      o.read(argv[2]);
+      o.write(argv[3]);
+      printf("copy ok\n");
+      return 0;
+    }
+    if (mode == "map" && argc == 6) return run_map(argv[2], atoi(argv[3]), atoi(argv[4]), argv[5]);
+    if (mode == "ap" && argc == 6) return run_ap(argv[2], argv[3], atof(argv[4]), atof(argv[5]));
+  } catch (const Error& e) {
+    fprintf(stderr, "quijibo::Error %d: %s\n", e.status, e.what());
+    return 2;
+  }
+  fprintf(stderr, "usage: host_layer_driver io <dir> | map <o3d> <maxIter> <prec> <outdir> | ap <o3d> <weight> "
+                  "<alpha> <thr>\n");
+  return 64;
+}

@@ -1,0 +1,102 @@
+"""The C++ host layer (POLYNOMIAL_4D, FIELD_3D, TRIANGLE_MESH, OPTIMIZE_3D over
+the C ABI) driven from a C++ program: inputs written as a .o3d file by the
+Python format writer, outputs read back from the FIELD_3D and OBJ files the
+C++ classes wrote, compared with the oracle."""
+import json
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+import oracle as O
+from quijibo_amd import formats as FM, workloads as W
+
+pytestmark = pytest.mark.gpu
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+DRIVER = os.path.join(HERE, "cpp", "_build", "host_layer_driver")
+
+
+def _o3d(tmp_path, w, res):
+    s = FM.O3D(res, w.center, w.lengths, FM.Polynomial4D(w.top.roots, w.top.powers, w.top.power_scalar),
+               FM.Polynomial4D(w.bottom.roots, w.bottom.powers, w.bottom.power_scalar), w.exp_scaling, 0,
+               FM.Field3D(res, w.center, w.lengths))
+    p = tmp_path / "in.o3d"
+    FM.write_o3d(p, s)
+    return p
+
+
+def _run(args):
+    print("driver:", " ".join(args[1:]), flush=True)
+    r = subprocess.run([DRIVER] + args, capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    return r.stdout
+
+
+def _oracle_map(w, res):
+    g = O.make_grid(res, w.center, w.lengths)
+    t, b = O.poly(w.top.roots, w.top.powers), O.poly(w.bottom.roots, w.bottom.powers)
+    p = O.make_params(w.max_iter, 2000.0, w.exp_scaling, w.slice, O.BLACK_HOLE)
+    return g, t, b, p
+
+
+@pytest.mark.parametrize("cfg", ["c1", "c2"])
+def test_map_and_meshes(gpu, tmp_path, cfg):
+    w = W.config1(8) if cfg == "c1" else W.config2(8)
+    res = (40, 36, 32)
+    _run(["map", str(_o3d(tmp_path, w, res)), str(w.max_iter), "0", str(tmp_path)])
+    g, t, b, p = _oracle_map(w, res)
+    field = FM.read_field(tmp_path / "field.bin")
+    assert field.res == res and field.center == tuple(w.center)
+    ref = O.field_map(g, t, b, p)
+    fin = np.isfinite(ref)
+    assert np.array_equal(np.isfinite(field.data), fin)
+    err = np.abs(field.data[fin] - ref[fin]) / np.maximum(1.0, np.abs(ref[fin]))
+    assert err.max() <= 1e-9
+    # linear mesher on the field the C++ layer produced: bit-exact vs the oracle's MC
+    lv, lt = FM.read_obj(tmp_path / "linear.obj")
+    ov, ot = O.marching_cubes(g, field.data)
+    assert np.array_equal(lt, ot) and np.max(np.abs(lv - ov)) <= 1e-15
+    # non-linear mesher (slabs of 7 slices): same triangles as the oracle
+    nv, nt = FM.read_obj(tmp_path / "nonlinear.obj")
+    pv, pt, bad = O.nonlinear_mc(g, t, b, p)
+    assert bad == 0 and np.array_equal(nt, pt)
+    d = np.max(np.abs(nv - pv), axis=1)
+    assert (d > 1e-15).mean() <= 0.005 and np.all(d <= 3.0 / min(res) / 2 ** 6 * 1.0001)
+
+
+def test_fp32_band_map_keeps_the_topology(gpu, tmp_path):
+    w = W.config2(8)
+    res = (40, 36, 32)
+    (tmp_path / "a").mkdir()
+    (tmp_path / "b").mkdir()
+    p = _o3d(tmp_path, w, res)
+    _run(["map", str(p), str(w.max_iter), "0", str(tmp_path / "a")])
+    _run(["map", str(p), str(w.max_iter), "1", str(tmp_path / "b")])
+    _, t64 = FM.read_obj(tmp_path / "a" / "linear.obj")
+    _, t32 = FM.read_obj(tmp_path / "b" / "linear.obj")
+    assert np.array_equal(t64, t32)
+    v64, n64 = FM.read_obj(tmp_path / "a" / "nonlinear.obj")
+    v32, n32 = FM.read_obj(tmp_path / "b" / "nonlinear.obj")
+    assert np.array_equal(n64, n32) and np.array_equal(v64, v32)
+
+
+def test_ap_score_and_gradients(gpu, tmp_path):
+    w = W.config2(8)
+    res = (24, 22, 20)
+    ax = [-1.5 + (np.arange(res[i]) + 0.5) * 3.0 / res[i] for i in range(3)]
+    Z, Y, X = np.meshgrid(ax[2], ax[1], ax[0], indexing="ij")
+    weight = (1.0 / (np.abs(np.sqrt(X ** 2 + Y ** 2 + Z ** 2) - 0.8) + 0.05)).reshape(-1)
+    FM.write_field(tmp_path / "w.field", FM.Field3D(res, w.center, w.lengths, weight))
+    out = json.loads(_run(["ap", str(_o3d(tmp_path, w, res)), str(tmp_path / "w.field"), "10.0", "0.0"]))
+    g = O.make_grid(res, w.center, w.lengths)
+    cpu = O.ap_score_gradients(g, O.poly(w.top.roots, w.top.powers, w.top.power_scalar),
+                               O.poly(w.bottom.roots, w.bottom.powers, w.bottom.power_scalar),
+                               O.make_params(1, 20.0, w.exp_scaling, w.slice, O.SKIP_ORIGIN | O.AP_REPASS), g,
+                               weight)
+    for k in ("score", "grad_exp", "grad_top_scalar", "grad_bottom_scalar", "grad_top", "grad_bottom"):
+        a = np.atleast_1d(np.asarray(out[k], dtype=np.float64))
+        b = np.atleast_1d(np.asarray(cpu[k], dtype=np.float64))
+        assert np.max(np.abs(a - b)) <= 1e-9 * np.max(np.abs(b)), k
+    assert out["adaptive_cells"] == cpu["overflowed"]

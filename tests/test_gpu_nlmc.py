@@ -20,7 +20,8 @@ def _run(w, res, precision=FP64, chunk=64, iso=0.0):
     g = api.make_grid(res, w.center, w.lengths)
     r = api.Rational(api.POLYNOMIAL_4D(w.top.roots, w.top.powers), api.POLYNOMIAL_4D(w.bottom.roots, w.bottom.powers))
     p = api.make_params(w.max_iter, 2000.0, w.exp_scaling, w.slice, MAP_BLACK_HOLE)
-    return api.nonlinear_mesh(g, r, p, iso, precision, 0.05, chunk)
+    # a fresh workspace each time, so the buffers grow from empty across the chunks
+    return api.nonlinear_mesh(g, r, p, iso, precision, 0.05, chunk, workspace=api.Workspace())
 
 
 def _oracle(w, res, iso=0.0):
