@@ -75,8 +75,8 @@ def main():
     ap.add_argument("--serve-budget-gib", type=float, default=64.0)
     args = ap.parse_args()
     if args.gpus != 1:
-        print("bench.py measures one GPU: this build runs on a single MI355X and has no multi-GPU path "
-              "(the sharding design is in DESIGN.md); run with --gpus 1", file=sys.stderr)
+        print("bench.py measures one GPU: the field, mesher and gradient paths run on a single MI355X "
+              "and have no multi-GPU launch; run with --gpus 1", file=sys.stderr)
         sys.exit(2)
 
     import torch
