@@ -73,6 +73,7 @@ def main():
     ap.add_argument("--serve", type=int, default=2000, help="requests in the serving stream (0 = skip)")
     ap.add_argument("--serve-rate", type=float, default=200.0)
     ap.add_argument("--serve-budget-gib", type=float, default=64.0)
+    ap.add_argument("--probe", action="store_true", help="counter probe: two field launches and a 1 GiB copy")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: the field, mesher and gradient paths run on a single MI355X "
@@ -94,6 +95,23 @@ def main():
     out64 = torch.empty(w.voxels, dtype=torch.float64, device="cuda")
     nominal = w.voxels * w.max_iter
     stream = torch.cuda.current_stream()
+
+    if args.probe:
+        # Counter probe for rocprofv3 --pmc: one FP32+band field after a warm-up, then a
+        # plain 1 GiB device-to-device copy whose FETCH_SIZE / WRITE_SIZE readings give
+        # the scale factor for the field kernel's readings.
+        for i in range(2):
+            log("probe: fp32+band field %d" % i)
+            api.field_eval(grid, r, params, FP32_BAND, args.band, out=out32, workspace=ws)
+            torch.cuda.synchronize()
+        src = torch.ones(1 << 27, dtype=torch.float64, device="cuda")
+        dst = torch.empty_like(src)
+        torch.cuda.synchronize()
+        log("probe: copy 1 GiB")
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"probe": "k_grid32 512^3 x2, then copy of %d bytes" % (8 << 27)}))
+        return
 
     # plain FP64 kernel
     log("fp64: warmup")

@@ -84,7 +84,8 @@ qj_status make_map_args(const qj_grid* g, const qj_rational* f, const qj_map_par
   }
   a->max_iter = p->max_iter;
   a->zoff = 0;
-  a->pad_ = 0;
+  a->gen = 0;
+  for (const RootRec64& r : f->h64) a->gen |= (r.e == 1.0 || r.e == 2.0) ? 0 : 1;
   a->slice = p->slice;
   a->exp_scaling = p->exp_scaling;
   a->escape = p->escape;

@@ -194,64 +194,115 @@ struct Consts32 {
 };
 
 // Largest-component power-of-two renormalisation: exact, no rounding.
-__device__ __forceinline__ void renorm32(q32& t, int& E) {
+// Returns the applied exponent (0 for 0 / inf / nan).
+__device__ __forceinline__ int renorm32(q32& t, int& E) {
   const float mx = fmaxf(fmaxf(fabsf(t.w), fabsf(t.x)), fmaxf(fabsf(t.y), fabsf(t.z)));
-  const int k = __builtin_amdgcn_frexp_expf(mx);  // 0 for 0 / inf / nan
+  const int k = __builtin_amdgcn_frexp_expf(mx);
   t.w = __builtin_amdgcn_ldexpf(t.w, -k);
   t.x = __builtin_amdgcn_ldexpf(t.x, -k);
   t.y = __builtin_amdgcn_ldexpf(t.y, -k);
   t.z = __builtin_amdgcn_ldexpf(t.z, -k);
   E += k;
+  return k;
 }
 
-// (q - r)^e in FP32. Integer kinds are exact products (b, b*b); the general
-// kind follows QUATERNION::pow's polar form with the magnitude |b|^e moved
-// into the log2 accumulator L, keeping the quaternion part unit length.
-// The reference's guards carry over: v = 0 gives a positive real result and a
-// non-positive angle drops the imaginary part.
+// General-kind term (q - r)^e in FP32, following QUATERNION::pow's polar form
+// with the magnitude |b|^e moved into the log2 accumulator L, so the
+// quaternion part is unit length. The reference's guards carry over: v = 0
+// gives a positive real result and a non-positive angle drops the imaginary
+// part.
+__device__ __forceinline__ q32 general32(const RootRec32& r, const q32& b, float& L) {
+  const float partial = b.x * b.x + b.y * b.y + b.z * b.z;
+  const float n2 = partial + b.w * b.w;
+  const float vm = __builtin_sqrtf(partial);
+  const float th = atan2f(vm, b.w);
+  const float m = vm > 0.0f ? r.e * th : 0.0f;  // scale = e*acos(w/|q|)*0 when v = 0
+  L += r.e * 0.5f * __log2f(n2);
+  float s, c;
+  __sincosf(m, &s, &c);
+  const float f = (m > 0.0f && vm > 0.0f) ? s / vm : 0.0f;
+  return q32{c, f * b.x, f * b.y, f * b.z};
+}
+
+// e = 1: pow(b, 1) is b itself except for v = 0, where the polar form gives +|w|.
+__device__ __forceinline__ q32 kind1(const q32& b) {
+  const float vmax = fmaxf(fmaxf(fabsf(b.x), fabsf(b.y)), fabsf(b.z));
+  return q32{vmax > 0.0f ? b.w : fabsf(b.w), b.x, b.y, b.z};
+}
+
+// e = 2: b * b (for v = 0 this is (w^2, 0, 0, 0), the polar form's value too).
+__device__ __forceinline__ q32 kind2(const q32& b) {
+  const float partial = b.x * b.x + b.y * b.y + b.z * b.z;
+  const float t = 2.0f * b.w;
+  return q32{b.w * b.w - partial, t * b.x, t * b.y, t * b.z};
+}
+
 __device__ __forceinline__ q32 term32(const RootRec32& r, const q32& q, float& L) {
   const q32 b = qsub(q, q32{r.w, r.x, r.y, r.z});
-  const float partial = b.x * b.x + b.y * b.y + b.z * b.z;
+  if (r.kind == 1) return kind1(b);
+  if (r.kind == 2) return kind2(b);
+  return general32(r, b, L);
+}
+
+// t <- t * (q - r)^e, with the product taken inside each kind's branch so the
+// branches share no term registers.
+template <bool kGeneral>
+__device__ __forceinline__ void apply32(q32& t, const RootRec32& r, const q32& q, float& L) {
+  const q32 b = qsub(q, q32{r.w, r.x, r.y, r.z});
   if (r.kind == 1) {
-    return q32{partial > 0.0f ? b.w : fabsf(b.w), b.x, b.y, b.z};
-  } else if (r.kind == 2) {
-    const float t = 2.0f * b.w;
-    return q32{b.w * b.w - partial, t * b.x, t * b.y, t * b.z};
+    t = qmul(t, kind1(b));
+  } else if (!kGeneral || r.kind == 2) {
+    t = qmul(t, kind2(b));
   } else {
-    const float n2 = partial + b.w * b.w;
-    const float vm = __builtin_sqrtf(partial);
-    const float th = atan2f(vm, b.w);
-    const float m = vm > 0.0f ? r.e * th : 0.0f;  // scale = e*acos(w/|q|)*0 when v = 0
-    L += r.e * 0.5f * __log2f(n2);
-    float s, c;
-    __sincosf(m, &s, &c);
-    const float f = (m > 0.0f && vm > 0.0f) ? s / vm : 0.0f;
-    return q32{c, f * b.x, f * b.y, f * b.z};
+    t = qmul(t, general32(r, b, L));
   }
 }
 
-// Ordered product over n roots with exponent tracking (renormalise every 2
-// terms). hi/lo receive the largest and smallest log2 magnitude of the
-// running product seen at the renormalisation points.
+// Ordered product over n roots with exponent tracking, renormalised every kRN
+// terms. hi/lo receive the largest and smallest log2 magnitude of the running
+// product at the renormalisation points (E, plus L when general kinds exist);
+// `bad` is set when a group of kRN terms fell below 2^-60 (kRN = 4 only: a
+// group of four can leave FP32's normal range, two cannot).
+template <int kRN, bool kGeneral>
 __device__ __forceinline__ q32 poly32(const RootRec32* __restrict__ r, int n, const q32& q, int& E, float& L,
-                                      float& hi, float& lo) {
+                                      float& hi, float& lo, bool& bad) {
   q32 t = term32(r[0], q, L);
   int i = 1;
-  for (; i + 1 < n; i += 2) {
-    t = qmul(t, term32(r[i], q, L));
-    t = qmul(t, term32(r[i + 1], q, L));
-    renorm32(t, E);
+  int mn = 0;
+  int ehi = -(1 << 30), elo = 1 << 30;
+  for (; i + kRN - 1 < n; i += kRN) {
+#pragma unroll
+    for (int j = 0; j < kRN; j++) apply32<kGeneral>(t, r[i + j], q, L);
+    const int k = renorm32(t, E);
+    if (kRN > 2) mn = min(mn, k);
+    if (kGeneral) {
+      const float l = (float)E + L;
+      hi = fmaxf(hi, l);
+      lo = fminf(lo, l);
+    } else {
+      ehi = max(ehi, E);
+      elo = min(elo, E);
+    }
+  }
+  for (; i < n; i++) apply32<kGeneral>(t, r[i], q, L);
+  const int k = renorm32(t, E);
+  if (kRN > 2) bad |= min(mn, k) < -60;
+  if (kGeneral) {
     const float l = (float)E + L;
     hi = fmaxf(hi, l);
     lo = fminf(lo, l);
+  } else {
+    ehi = max(ehi, E);
+    elo = min(elo, E);
+    hi = fmaxf(hi, (float)ehi);
+    lo = fminf(lo, (float)elo);
   }
-  if (i < n) t = qmul(t, term32(r[i], q, L));
-  renorm32(t, E);
-  const float l = (float)E + L;
-  hi = fmaxf(hi, l);
-  lo = fminf(lo, l);
   return t;
 }
+
+// Renormalising every 4 terms needs 4 terms of up to 2^term_bits to stay in
+// FP32 range: 4 * 30 + 1 < 127.
+constexpr double kRN4MaxTermBits = 30.0;
 
 struct Res32 {
   float F;
@@ -266,6 +317,7 @@ struct Res32 {
 // |B| > 2^512.5 means |B|^2 overflows in FP64, B^-1 = conj(B) * 0 and q = 0
 // exactly (F = -inf under the black-hole stop); anything closer to a range
 // edge, or outside it, goes to FP64.
+template <int kRN, bool kGeneral>
 __device__ __forceinline__ Res32 eval32(const MapArgs& a, const RootRec32* __restrict__ roots, const Consts32& k,
                                         const q64& q0) {
   Res32 res{0.0f, 0, true, false};
@@ -285,16 +337,17 @@ __device__ __forceinline__ Res32 eval32(const MapArgs& a, const RootRec32* __res
   float log2mag = 0.0f;
   bool amb = false, bad = false, zero = false;
   int it = 0;
-  const float lim = 1020.0f - k.term_bits;
+  // FP64 running products between two checkpoints grow by at most (kRN - 1) terms
+  const float lim = 1020.0f - (float)(kRN - 1) * k.term_bits;
   while (true) {
     int ET = 0, EB = 0;
     float L = 0.0f, hiT = -1e30f, loT = 1e30f;
-    const q32 T = poly32(roots, a.nt, q, ET, L, hiT, loT);
+    const q32 T = poly32<kRN, kGeneral>(roots, a.nt, q, ET, L, hiT, loT, bad);
     bad |= !(hiT < lim && loT > -lim);
     float l2 = k.log2c;
     if (a.nb > 0) {
       float LB = 0.0f, hiB = -1e30f, loB = 1e30f;
-      const q32 B = poly32(roots + a.nt, a.nb, q, EB, LB, hiB, loB);
+      const q32 B = poly32<kRN, kGeneral>(roots + a.nt, a.nb, q, EB, LB, hiB, loB, bad);
       bad |= !(hiB < lim && loB > -lim);
       const float nb2 = B.w * B.w + B.x * B.x + B.y * B.y + B.z * B.z;
       const float lB = 0.5f * __log2f(nb2) + (float)EB + LB;
@@ -377,6 +430,7 @@ __global__ __launch_bounds__(kBlock) void k_grid64(MapArgs a, const RootRec64* _
   }
 }
 
+template <int kRN, bool kGeneral>
 __global__ __launch_bounds__(kBlock) void k_grid32(MapArgs a, Consts32 k, const RootRec32* __restrict__ roots,
                                                    double* __restrict__ out, unsigned long long* __restrict__ cnt,
                                                    uint32_t* __restrict__ blist, uint32_t* __restrict__ bcount) {
@@ -388,7 +442,7 @@ __global__ __launch_bounds__(kBlock) void k_grid32(MapArgs a, Consts32 k, const 
   if (valid) {
     double c[3];
     cell_center(a, x, y, z + a.zoff, c);
-    const Res32 r = eval32(a, roots, k, q64{c[0], c[1], c[2], a.slice});
+    const Res32 r = eval32<kRN, kGeneral>(a, roots, k, q64{c[0], c[1], c[2], a.slice});
     if (r.written) {
       if (!r.need64) out[idx] = (double)r.F;
       need = r.need64;
@@ -400,7 +454,7 @@ __global__ __launch_bounds__(kBlock) void k_grid32(MapArgs a, Consts32 k, const 
   block_append(need, (uint32_t)idx, blist, bcount);
 }
 
-template <class Src>
+template <class Src, int kRN, bool kGeneral>
 __global__ __launch_bounds__(kBlock) void k_points32(MapArgs a, Consts32 k, const RootRec32* __restrict__ roots,
                                                      const double* __restrict__ pts, int64_t n,
                                                      double* __restrict__ out, unsigned long long* __restrict__ cnt,
@@ -410,7 +464,7 @@ __global__ __launch_bounds__(kBlock) void k_points32(MapArgs a, Consts32 k, cons
   int iters = 0;
   bool need = false;
   if (valid) {
-    const Res32 r = eval32(a, roots, k, Src::start(a, pts, i));
+    const Res32 r = eval32<kRN, kGeneral>(a, roots, k, Src::start(a, pts, i));
     if (r.written) {
       if (!r.need64) out[i] = (double)r.F;
       need = r.need64;
@@ -481,7 +535,10 @@ __global__ __launch_bounds__(kBlock) void k_batch32(const ChunkDesc* __restrict_
     k.band = d.band;
     k.term_bits = (float)a.term_bits;
     const int64_t idx = d.start + local;
-    const Res32 r = eval32(a, d.r32, k, chunk_start(a, idx));
+    // per-chunk variant (uniform across the block: a block holds voxels of one chunk)
+    const Res32 r = a.gen ? eval32<2, true>(a, d.r32, k, chunk_start(a, idx))
+                          : (a.term_bits <= kRN4MaxTermBits ? eval32<4, false>(a, d.r32, k, chunk_start(a, idx))
+                                                            : eval32<2, false>(a, d.r32, k, chunk_start(a, idx)));
     if (r.written) {
       if (!r.need64) d.out[idx] = (double)r.F;
       need = r.need64;
@@ -595,8 +652,17 @@ qj_status launch_field(qj_workspace* ws, const MapArgs& a, const qj_rational* f,
     timing_mark(ws, s);
     timing_mark(ws, s);
   } else {
-    hipLaunchKernelGGL(k_grid32, dim3((unsigned)nblocks), dim3(kBlock), 0, s, a, consts32(a, band), f->d32, d_out,
-                       ws->d_counters, ws->d_list, ws->d_list_count);
+    const Consts32 k = consts32(a, band);
+    const dim3 g((unsigned)nblocks), b(kBlock);
+    if (a.gen)
+      hipLaunchKernelGGL((k_grid32<2, true>), g, b, 0, s, a, k, f->d32, d_out, ws->d_counters, ws->d_list,
+                         ws->d_list_count);
+    else if (a.term_bits <= kRN4MaxTermBits)
+      hipLaunchKernelGGL((k_grid32<4, false>), g, b, 0, s, a, k, f->d32, d_out, ws->d_counters, ws->d_list,
+                         ws->d_list_count);
+    else
+      hipLaunchKernelGGL((k_grid32<2, false>), g, b, 0, s, a, k, f->d32, d_out, ws->d_counters, ws->d_list,
+                         ws->d_list_count);
     QJ_HIP(hipGetLastError());
     timing_mark(ws, s);
     timing_mark(ws, s);
@@ -624,8 +690,17 @@ qj_status launch_points(qj_workspace* ws, const MapArgs& a, const qj_rational* f
     QJ_HIP(hipGetLastError());
   } else {
     const int64_t nb = (n + kBlock - 1) / kBlock;
-    hipLaunchKernelGGL(k_points32<PointSource>, dim3((unsigned)nb), dim3(kBlock), 0, s, a, consts32(a, band),
-                       f->d32, d_pts, n, d_out, ws->d_counters, ws->d_list, ws->d_list_count);
+    const Consts32 k = consts32(a, band);
+    const dim3 g((unsigned)nb), b(kBlock);
+    if (a.gen)
+      hipLaunchKernelGGL((k_points32<PointSource, 2, true>), g, b, 0, s, a, k, f->d32, d_pts, n, d_out,
+                         ws->d_counters, ws->d_list, ws->d_list_count);
+    else if (a.term_bits <= kRN4MaxTermBits)
+      hipLaunchKernelGGL((k_points32<PointSource, 4, false>), g, b, 0, s, a, k, f->d32, d_pts, n, d_out,
+                         ws->d_counters, ws->d_list, ws->d_list_count);
+    else
+      hipLaunchKernelGGL((k_points32<PointSource, 2, false>), g, b, 0, s, a, k, f->d32, d_pts, n, d_out,
+                         ws->d_counters, ws->d_list, ws->d_list_count);
     QJ_HIP(hipGetLastError());
     hipLaunchKernelGGL((k_list64<PointSource, false>), dim3(list_grid(n / 16)), dim3(kBlock), 0, s, a, f->d64,
                        d_pts, ws->d_list, ws->d_list_count, (int64_t)0, d_out, ws->d_counters,

@@ -29,7 +29,7 @@ struct MapArgs {
   int32_t res[3];
   int32_t max_iter;
   int32_t zoff;      // first z slice of the evaluated range within the grid (cell centres use z + zoff)
-  int32_t pad_;
+  int32_t gen;               // 1 when some root's exponent is not exactly 1 or 2 (FP32 general kind)
   double lo[3];      // center - 0.5 * lengths (FIELD_3D.cpp:822-825)
   double dx[3];      // lengths / res (FIELD_3D.cpp:50-52)
   double center[3];
