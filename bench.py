@@ -74,6 +74,7 @@ def main():
     ap.add_argument("--serve-rate", type=float, default=200.0)
     ap.add_argument("--serve-budget-gib", type=float, default=64.0)
     ap.add_argument("--probe", action="store_true", help="counter probe: two field launches and a 1 GiB copy")
+    ap.add_argument("--serve-only", action="store_true", help="run only the serving stream")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: the field, mesher and gradient paths run on a single MI355X "
@@ -95,6 +96,10 @@ def main():
     out64 = torch.empty(w.voxels, dtype=torch.float64, device="cuda")
     nominal = w.voxels * w.max_iter
     stream = torch.cuda.current_stream()
+
+    if args.serve_only:
+        print(json.dumps({"serving": serve_stream(args, log)}))
+        return
 
     if args.probe:
         # Counter probe for rocprofv3 --pmc: one FP32+band field after a warm-up, then a

@@ -17,10 +17,36 @@ constexpr double kBlackHole = 10.0 * (double)FLT_MIN;  // 10 * REAL_MIN, SETTING
 // in all four components (each output component reads all four inputs), and
 // so are T B^-1 and |q|: the loop can stop there with the same F.
 __device__ __forceinline__ bool has_nan(const q64& q) { return isnan(q.w) || isnan(q.x) || isnan(q.y) || isnan(q.z); }
+
+// (q - r)^e. With kExactInt, exponents exactly 1 and 2 are taken as b and b*b
+// instead of QUATERNION::pow's polar form (QUATERNION.cpp:375-393); the two
+// agree to a few ulps, and the polar form's special cases are kept: b = 0
+// gives NaN, v = 0 gives (|w|^e, +0 * v).
+template <bool kExactInt>
+__device__ __forceinline__ q64 term64(const RootRec64& r, const q64& q) {
+  const q64 b = qsub(q, q64{r.w, r.x, r.y, r.z});
+  if (kExactInt && (r.e == 1.0 || r.e == 2.0)) {
+    const double partial = b.x * b.x + b.y * b.y + b.z * b.z;
+    if (!(partial + b.w * b.w > 0.0)) {
+      const double nan = __builtin_nan("");
+      return q64{nan, nan, nan, nan};
+    }
+    if (!(partial > 0.0)) {
+      const double m = r.e == 1.0 ? fabs(b.w) : b.w * b.w;
+      return q64{m, 0.0 * b.x, 0.0 * b.y, 0.0 * b.z};
+    }
+    if (r.e == 1.0) return b;
+    const double t = 2.0 * b.w;
+    return q64{b.w * b.w - partial, t * b.x, t * b.y, t * b.z};
+  }
+  return qpow(b, r.e);
+}
+
+template <bool kExactInt = false>
 __device__ __forceinline__ q64 poly64(const RootRec64* __restrict__ r, int n, const q64& q) {
-  q64 res = qpow(qsub(q, q64{r[0].w, r[0].x, r[0].y, r[0].z}), r[0].e);
+  q64 res = term64<kExactInt>(r[0], q);
   for (int i = 1; i < n; i++) {
-    res = qmul(res, qpow(qsub(q, q64{r[i].w, r[i].x, r[i].y, r[i].z}), r[i].e));
+    res = qmul(res, term64<kExactInt>(r[i], q));
     if ((i & 7) == 0 && has_nan(res)) {
       const double nan = __builtin_nan("");
       return q64{nan, nan, nan, nan};
@@ -43,6 +69,7 @@ struct Res64 {
 // One voxel of computeLogScaledPowerRationalMap (OPTIMIZE_3D.cpp:1317-1362) /
 // computeRationalMapAP's first pass (:1423-1475). Overflow detection replaces
 // fetestexcept(FE_OVERFLOW): any non-finite T, B, |B|^2, quotient or |q|.
+template <bool kExactInt = false>
 __device__ __forceinline__ Res64 eval64(const MapArgs& a, const RootRec64* __restrict__ roots, q64 q) {
   Res64 r{0.0, 0, true, false};
   double magnitude = qmag(q);
@@ -53,7 +80,7 @@ __device__ __forceinline__ Res64 eval64(const MapArgs& a, const RootRec64* __res
   const bool adaptive = (a.flags & QJ_MAP_ADAPTIVE) != 0;
   int it = 0;
   while (magnitude < a.escape && it < a.max_iter) {
-    const q64 T = poly64(roots, a.nt, q);
+    const q64 T = poly64<kExactInt>(roots, a.nt, q);
     if (a.nb > 0) {
       if (has_nan(T) && !adaptive) {
         // T B^-1 is all-NaN whatever B is; skip evaluating B
@@ -64,7 +91,7 @@ __device__ __forceinline__ Res64 eval64(const MapArgs& a, const RootRec64* __res
         r.iters++;
         break;
       }
-      const q64 B = poly64(roots + a.nt, a.nb, q);
+      const q64 B = poly64<kExactInt>(roots + a.nt, a.nb, q);
       q = qdiv(T, B);
       if (adaptive) {
         const double m2 = B.w * B.w + B.x * B.x + B.y * B.y + B.z * B.z;

@@ -479,8 +479,9 @@ __global__ __launch_bounds__(kBlock) void k_points32(MapArgs a, Consts32 k, cons
 }
 
 // FP64 over a compacted list (count read on the device), or over all n
-// indices when list == nullptr.
-template <class Src, bool kExtended>
+// indices when list == nullptr. kExactInt: integer exponents as exact
+// products (the FP32 path's band and range re-evaluation).
+template <class Src, bool kExtended, bool kExactInt = false>
 __global__ __launch_bounds__(kBlock) void k_list64(MapArgs a, const RootRec64* __restrict__ roots,
                                                    const double* __restrict__ pts,
                                                    const uint32_t* __restrict__ list,
@@ -498,7 +499,7 @@ __global__ __launch_bounds__(kBlock) void k_list64(MapArgs a, const RootRec64* _
     if (i < n) {
       idx = list ? list[i] : (uint32_t)i;
       const q64 q0 = Src::start(a, pts, idx);
-      const Res64 r = kExtended ? eval64x(a, roots, q0) : eval64(a, roots, q0);
+      const Res64 r = kExtended ? eval64x(a, roots, q0) : eval64<kExactInt>(a, roots, q0);
       if (r.written) out[idx] = r.F;
       else if (pts) out[idx] = __builtin_nan("");
       vox++;
@@ -559,7 +560,7 @@ __global__ __launch_bounds__(kBlock) void k_batch64(const ChunkDesc* __restrict_
       const int local = (int)(e - (uint32_t)c * (uint32_t)chunk_voxels);
       const ChunkDesc& d = descs[c];
       const int64_t idx = d.start + local;
-      const Res64 r = eval64(d.a, d.r64, chunk_start(d.a, idx));
+      const Res64 r = eval64<true>(d.a, d.r64, chunk_start(d.a, idx));
       if (r.written) d.out[idx] = r.F;
     }
     return;
@@ -666,7 +667,7 @@ qj_status launch_field(qj_workspace* ws, const MapArgs& a, const qj_rational* f,
     QJ_HIP(hipGetLastError());
     timing_mark(ws, s);
     timing_mark(ws, s);
-    hipLaunchKernelGGL((k_list64<GridSource, false>), dim3(list_grid(nvox / 16)), dim3(kBlock), 0, s, a, f->d64,
+    hipLaunchKernelGGL((k_list64<GridSource, false, true>), dim3(list_grid(nvox / 16)), dim3(kBlock), 0, s, a, f->d64,
                        (const double*)nullptr, ws->d_list, ws->d_list_count, (int64_t)0, d_out, ws->d_counters,
                        (int)CNT_BAND_VOXELS, (int)CNT_BAND_ITERS, alist, ws->d_list_count + 1);
     QJ_HIP(hipGetLastError());
@@ -702,7 +703,7 @@ qj_status launch_points(qj_workspace* ws, const MapArgs& a, const qj_rational* f
       hipLaunchKernelGGL((k_points32<PointSource, 2, false>), g, b, 0, s, a, k, f->d32, d_pts, n, d_out,
                          ws->d_counters, ws->d_list, ws->d_list_count);
     QJ_HIP(hipGetLastError());
-    hipLaunchKernelGGL((k_list64<PointSource, false>), dim3(list_grid(n / 16)), dim3(kBlock), 0, s, a, f->d64,
+    hipLaunchKernelGGL((k_list64<PointSource, false, true>), dim3(list_grid(n / 16)), dim3(kBlock), 0, s, a, f->d64,
                        d_pts, ws->d_list, ws->d_list_count, (int64_t)0, d_out, ws->d_counters,
                        (int)CNT_BAND_VOXELS, (int)CNT_BAND_ITERS, alist, ws->d_list_count + 1);
     QJ_HIP(hipGetLastError());
