@@ -75,6 +75,9 @@ def main():
     ap.add_argument("--serve-budget-gib", type=float, default=64.0)
     ap.add_argument("--probe", action="store_true", help="counter probe: two field launches and a 1 GiB copy")
     ap.add_argument("--serve-only", action="store_true", help="run only the serving stream")
+    ap.add_argument("--serve-launch-work", type=float, default=0.0,
+                    help="root-term iterations per request-loop launch (0 = library default)")
+    ap.add_argument("--serve-inflight", type=int, default=0, help="launches in flight (0 = library default)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: the field, mesher and gradient paths run on a single MI355X "
@@ -221,7 +224,8 @@ def serve_stream(args, log):
     from quijibo_amd import server as S, workloads as W
     from quijibo_amd._lib import FP32_BAND
     reqs = W.config4_requests(n=args.serve, rate=args.serve_rate)
-    srv = S.Server(hbm_budget_bytes=int(args.serve_budget_gib * (1 << 30)), precision=FP32_BAND, band=args.band)
+    srv = S.Server(hbm_budget_bytes=int(args.serve_budget_gib * (1 << 30)), precision=FP32_BAND, band=args.band,
+                   launch_work=args.serve_launch_work, inflight_launches=args.serve_inflight)
     # warm the graphs with one small request
     srv.submit(reqs[0][1])
     srv.wait_all(1, timeout=60)
@@ -234,9 +238,10 @@ def serve_stream(args, log):
     return {"requests": len(reqs), "offered_rate_req_s": args.serve_rate, "completed": sum(c.status == 0 for c in done),
             "wall_s": round(end - start, 3), "throughput_req_s": round(len(done) / (end - start), 2),
             "throughput_gvoxel_iter_s": round(vox_iters / (end - start) / 1e9, 3), "latency": lat,
-            "config": "hbm budget %.0f GiB, 65536-voxel chunks, launches of <= 512 chunks and ~6e9 root-term "
-                      "iterations, 2 launches in flight, fp32+fp64 band, shortest-remaining-work-first"
-                      % args.serve_budget_gib}
+            "config": "hbm budget %.0f GiB, 65536-voxel chunks, launches of <= 512 chunks and ~%s root-term "
+                      "iterations, %s launches in flight, fp32+fp64 band, shortest-remaining-work-first"
+                      % (args.serve_budget_gib, ("%.1e" % args.serve_launch_work) if args.serve_launch_work else "3e9",
+                         args.serve_inflight or 2)}
 
 
 def _kernel_ms(api, ws):

@@ -235,7 +235,7 @@ typedef struct qj_server_config {
   double band;
   int32_t device;
   double launch_work;         /* root-term iterations per launch (voxels x max_iter x roots), so every
-                                 launch takes about the same GPU time (0 = 6e9, a few ms) */
+                                 launch takes about the same GPU time (0 = 3e9, about 1.5 ms) */
 } qj_server_config;
 
 typedef struct qj_request {

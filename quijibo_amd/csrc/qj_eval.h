@@ -18,14 +18,18 @@ constexpr double kBlackHole = 10.0 * (double)FLT_MIN;  // 10 * REAL_MIN, SETTING
 // so are T B^-1 and |q|: the loop can stop there with the same F.
 __device__ __forceinline__ bool has_nan(const q64& q) { return isnan(q.w) || isnan(q.x) || isnan(q.y) || isnan(q.z); }
 
-// (q - r)^e. With kExactInt, exponents exactly 1 and 2 are taken as b and b*b
-// instead of QUATERNION::pow's polar form (QUATERNION.cpp:375-393); the two
-// agree to a few ulps, and the polar form's special cases are kept: b = 0
-// gives NaN, v = 0 gives (|w|^e, +0 * v).
-template <bool kExactInt>
+// How term64 takes (q - r)^e:
+//   kPolar      QUATERNION::pow's polar form for every exponent (QUATERNION.cpp:375-393)
+//   kExactInt   exponents exactly 1 and 2 as b and b*b, others polar
+//   kExactOnly  every exponent is 1 or 2 (the caller checked), no polar code
+// The exact forms agree with the polar form to a few ulps and keep its special
+// cases: b = 0 gives NaN, v = 0 gives (|w|^e, +0 * v).
+enum PowMode { kPolar = 0, kExactInt = 1, kExactOnly = 2 };
+
+template <int kPow>
 __device__ __forceinline__ q64 term64(const RootRec64& r, const q64& q) {
   const q64 b = qsub(q, q64{r.w, r.x, r.y, r.z});
-  if (kExactInt && (r.e == 1.0 || r.e == 2.0)) {
+  if (kPow == kExactOnly || (kPow == kExactInt && (r.e == 1.0 || r.e == 2.0))) {
     const double partial = b.x * b.x + b.y * b.y + b.z * b.z;
     if (!(partial + b.w * b.w > 0.0)) {
       const double nan = __builtin_nan("");
@@ -39,14 +43,15 @@ __device__ __forceinline__ q64 term64(const RootRec64& r, const q64& q) {
     const double t = 2.0 * b.w;
     return q64{b.w * b.w - partial, t * b.x, t * b.y, t * b.z};
   }
+  if (kPow == kExactOnly) return b;  // not reached: kExactOnly callers have only e = 1, 2
   return qpow(b, r.e);
 }
 
-template <bool kExactInt = false>
+template <int kPow = kPolar>
 __device__ __forceinline__ q64 poly64(const RootRec64* __restrict__ r, int n, const q64& q) {
-  q64 res = term64<kExactInt>(r[0], q);
+  q64 res = term64<kPow>(r[0], q);
   for (int i = 1; i < n; i++) {
-    res = qmul(res, term64<kExactInt>(r[i], q));
+    res = qmul(res, term64<kPow>(r[i], q));
     if ((i & 7) == 0 && has_nan(res)) {
       const double nan = __builtin_nan("");
       return q64{nan, nan, nan, nan};
@@ -69,7 +74,7 @@ struct Res64 {
 // One voxel of computeLogScaledPowerRationalMap (OPTIMIZE_3D.cpp:1317-1362) /
 // computeRationalMapAP's first pass (:1423-1475). Overflow detection replaces
 // fetestexcept(FE_OVERFLOW): any non-finite T, B, |B|^2, quotient or |q|.
-template <bool kExactInt = false>
+template <int kPow = kPolar>
 __device__ __forceinline__ Res64 eval64(const MapArgs& a, const RootRec64* __restrict__ roots, q64 q) {
   Res64 r{0.0, 0, true, false};
   double magnitude = qmag(q);
@@ -80,7 +85,7 @@ __device__ __forceinline__ Res64 eval64(const MapArgs& a, const RootRec64* __res
   const bool adaptive = (a.flags & QJ_MAP_ADAPTIVE) != 0;
   int it = 0;
   while (magnitude < a.escape && it < a.max_iter) {
-    const q64 T = poly64<kExactInt>(roots, a.nt, q);
+    const q64 T = poly64<kPow>(roots, a.nt, q);
     if (a.nb > 0) {
       if (has_nan(T) && !adaptive) {
         // T B^-1 is all-NaN whatever B is; skip evaluating B
@@ -91,7 +96,7 @@ __device__ __forceinline__ Res64 eval64(const MapArgs& a, const RootRec64* __res
         r.iters++;
         break;
       }
-      const q64 B = poly64<kExactInt>(roots + a.nt, a.nb, q);
+      const q64 B = poly64<kPow>(roots + a.nt, a.nb, q);
       q = qdiv(T, B);
       if (adaptive) {
         const double m2 = B.w * B.w + B.x * B.x + B.y * B.y + B.z * B.z;

@@ -479,9 +479,9 @@ __global__ __launch_bounds__(kBlock) void k_points32(MapArgs a, Consts32 k, cons
 }
 
 // FP64 over a compacted list (count read on the device), or over all n
-// indices when list == nullptr. kExactInt: integer exponents as exact
-// products (the FP32 path's band and range re-evaluation).
-template <class Src, bool kExtended, bool kExactInt = false>
+// indices when list == nullptr. kPow: the PowMode (kExactInt / kExactOnly for
+// the FP32 path's band and range re-evaluation).
+template <class Src, bool kExtended, int kPow = kPolar>
 __global__ __launch_bounds__(kBlock) void k_list64(MapArgs a, const RootRec64* __restrict__ roots,
                                                    const double* __restrict__ pts,
                                                    const uint32_t* __restrict__ list,
@@ -499,7 +499,7 @@ __global__ __launch_bounds__(kBlock) void k_list64(MapArgs a, const RootRec64* _
     if (i < n) {
       idx = list ? list[i] : (uint32_t)i;
       const q64 q0 = Src::start(a, pts, idx);
-      const Res64 r = kExtended ? eval64x(a, roots, q0) : eval64<kExactInt>(a, roots, q0);
+      const Res64 r = kExtended ? eval64x(a, roots, q0) : eval64<kPow>(a, roots, q0);
       if (r.written) out[idx] = r.F;
       else if (pts) out[idx] = __builtin_nan("");
       vox++;
@@ -514,83 +514,164 @@ __global__ __launch_bounds__(kBlock) void k_list64(MapArgs a, const RootRec64* _
 
 // ---------------------------------------------------------------- batched chunks (request loop)
 
-__global__ void k_zero_u32(uint32_t* p) { *p = 0u; }
+__global__ void k_zero_u32(uint32_t* p, int n) {
+  if ((int)threadIdx.x < n) p[threadIdx.x] = 0u;
+}
 
 __device__ __forceinline__ q64 chunk_start(const MapArgs& a, int64_t idx) { return GridSource::start(a, nullptr, idx); }
 
-// blocks_per_chunk blocks per chunk descriptor; each lane one voxel of its chunk
-__global__ __launch_bounds__(kBlock) void k_batch32(const ChunkDesc* __restrict__ descs, int blocks_per_chunk,
-                                                    int chunk_voxels, uint32_t* __restrict__ blist,
-                                                    uint32_t* __restrict__ bcount) {
-  const int c = blockIdx.x / blocks_per_chunk;
-  const int local = (blockIdx.x - c * blocks_per_chunk) * kBlock + threadIdx.x;
-  const ChunkDesc& d = descs[c];
-  bool need = false;
-  if (local < d.count) {
-    const MapArgs& a = d.a;
-    Consts32 k;
-    k.log2c = (float)log2(a.exp_scaling);
-    k.log2esc = (float)log2(a.escape);
-    k.log2bh = (float)log2(kBlackHole);
-    k.margin = 0.02f;
-    k.band = d.band;
-    k.term_bits = (float)a.term_bits;
-    const int64_t idx = d.start + local;
-    // per-chunk variant (uniform across the block: a block holds voxels of one chunk)
-    const Res32 r = a.gen ? eval32<2, true>(a, d.r32, k, chunk_start(a, idx))
-                          : (a.term_bits <= kRN4MaxTermBits ? eval32<4, false>(a, d.r32, k, chunk_start(a, idx))
-                                                            : eval32<2, false>(a, d.r32, k, chunk_start(a, idx)));
-    if (r.written) {
-      if (!r.need64) d.out[idx] = (double)r.F;
-      need = r.need64;
-    }
-  }
-  block_append(need, (uint32_t)(c * chunk_voxels + local), blist, bcount);
+// Work queue over (chunk, block-of-chunk) tiles: a resident grid takes tiles
+// with one atomic per tile until every tile of its chunk range is taken, so a
+// launch costs what its used chunks cost (no empty blocks) and blocks that
+// draw cheap tiles take more of them. Every block leaves when the queue passes
+// the tile count.
+__device__ __forceinline__ int next_tile(uint32_t* queue) {
+  __shared__ int s_tile;
+  __syncthreads();  // the previous tile's readers are done with s_tile
+  if (threadIdx.x == 0) s_tile = (int)atomicAdd(queue, 1u);
+  __syncthreads();
+  return s_tile;
 }
 
-// FP64 for every voxel of every chunk (list == nullptr) or for the band list
-__global__ __launch_bounds__(kBlock) void k_batch64(const ChunkDesc* __restrict__ descs, int blocks_per_chunk,
-                                                    int chunk_voxels, const uint32_t* __restrict__ list,
-                                                    const uint32_t* __restrict__ count) {
-  if (list) {
-    const int64_t n = *count;
-    for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += (int64_t)gridDim.x * kBlock) {
-      const uint32_t e = list[i];
-      const int c = (int)(e / (uint32_t)chunk_voxels);
-      const int local = (int)(e - (uint32_t)c * (uint32_t)chunk_voxels);
-      const ChunkDesc& d = descs[c];
+// Batch counters (uint32): [0] band count of integer-exponent chunks (list from
+// the front), [1] band count of general chunks (list from the back), [2..4]
+// FP32 queues per variant, [5] FP64 queue.
+enum { BC_BAND_INT = 0, BC_BAND_GEN = 1, BC_Q32 = 2, BC_Q64 = 5, BC_N = 8 };
+
+// Descriptors are grouped by FP32 variant: chunks [0, g1) have general
+// exponents, [g1, g2) integer exponents renormalised every 4 terms, [g2,
+// total) integer exponents renormalised every 2.
+template <int kVar>
+__global__ __launch_bounds__(kBlock) void k_batch32(const ChunkDesc* __restrict__ descs, int blocks_per_chunk,
+                                                    int chunk_voxels, uint32_t* __restrict__ blist, int64_t list_cap,
+                                                    uint32_t* __restrict__ counters) {
+  constexpr int kRN = kVar == 1 ? 4 : 2;
+  constexpr bool kGen = kVar == 0;
+  const ChunkDesc& d0 = descs[0];
+  const int c0 = kVar == 0 ? 0 : (kVar == 1 ? d0.g1 : d0.g2);
+  const int c1 = kVar == 0 ? d0.g1 : (kVar == 1 ? d0.g2 : d0.total);
+  const int tiles = (c1 - c0) * blocks_per_chunk;
+  for (int tile = next_tile(counters + BC_Q32 + kVar); tile < tiles; tile = next_tile(counters + BC_Q32 + kVar)) {
+    const int c = c0 + tile / blocks_per_chunk;
+    const int local = (tile % blocks_per_chunk) * kBlock + threadIdx.x;
+    const ChunkDesc& d = descs[c];
+    bool need = false;
+    if (local < d.count) {
+      const MapArgs& a = d.a;
+      Consts32 k;
+      k.log2c = (float)log2(a.exp_scaling);
+      k.log2esc = (float)log2(a.escape);
+      k.log2bh = (float)log2(kBlackHole);
+      k.margin = 0.02f;
+      k.band = d.band;
+      k.term_bits = (float)a.term_bits;
       const int64_t idx = d.start + local;
-      const Res64 r = eval64<true>(d.a, d.r64, chunk_start(d.a, idx));
-      if (r.written) d.out[idx] = r.F;
+      const Res32 r = eval32<kRN, kGen>(a, d.r32, k, chunk_start(a, idx));
+      if (r.written) {
+        if (!r.need64) d.out[idx] = (double)r.F;
+        need = r.need64;
+      }
     }
-    return;
+    const uint32_t e = (uint32_t)(c * chunk_voxels + local);
+    if (kGen) {
+      // general chunks fill the list from the back: entry k at list_cap - 1 - k
+      __shared__ uint32_t s_gbase;
+      const unsigned long long m = __ballot(need);
+      __shared__ uint32_t s_gcnt[kBlock / 64];
+      const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+      if (lane == 0) s_gcnt[wave] = (uint32_t)__popcll(m);
+      __syncthreads();
+      if (threadIdx.x == 0) {
+        uint32_t t = 0;
+        for (int w = 0; w < kBlock / 64; w++) {
+          const uint32_t cw = s_gcnt[w];
+          s_gcnt[w] = t;
+          t += cw;
+        }
+        s_gbase = t ? atomicAdd(counters + BC_BAND_GEN, t) : 0u;
+      }
+      __syncthreads();
+      if (need) {
+        const uint32_t k = s_gbase + s_gcnt[wave] + (uint32_t)__popcll(m & ((1ull << lane) - 1ull));
+        blist[list_cap - 1 - (int64_t)k] = e;
+      }
+    } else {
+      block_append(need, e, blist, counters + BC_BAND_INT);
+    }
   }
-  const int c = blockIdx.x / blocks_per_chunk;
-  const int local = (blockIdx.x - c * blocks_per_chunk) * kBlock + threadIdx.x;
-  const ChunkDesc& d = descs[c];
-  if (local < d.count) {
+}
+
+// FP64 for the band list of one exponent class (integer exponents as exact
+// products); kFromEnd reads the general chunks' list from the back.
+template <int kPow, bool kFromEnd>
+__global__ __launch_bounds__(kBlock) void k_batch64_list(const ChunkDesc* __restrict__ descs, int chunk_voxels,
+                                                         const uint32_t* __restrict__ list, int64_t list_cap,
+                                                         const uint32_t* __restrict__ count) {
+  const int64_t n = *count;
+  for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += (int64_t)gridDim.x * kBlock) {
+    const uint32_t e = kFromEnd ? list[list_cap - 1 - i] : list[i];
+    const int c = (int)(e / (uint32_t)chunk_voxels);
+    const int local = (int)(e - (uint32_t)c * (uint32_t)chunk_voxels);
+    const ChunkDesc& d = descs[c];
     const int64_t idx = d.start + local;
-    const Res64 r = eval64(d.a, d.r64, chunk_start(d.a, idx));
+    const Res64 r = eval64<kPow>(d.a, d.r64, chunk_start(d.a, idx));
     if (r.written) d.out[idx] = r.F;
   }
 }
 
+// FP64 (the reference method) for every voxel of every chunk, work-queue tiles
+__global__ __launch_bounds__(kBlock) void k_batch64(const ChunkDesc* __restrict__ descs, int blocks_per_chunk,
+                                                    uint32_t* __restrict__ counters) {
+  const int tiles = descs[0].total * blocks_per_chunk;
+  for (int tile = next_tile(counters + BC_Q64); tile < tiles; tile = next_tile(counters + BC_Q64)) {
+    const int c = tile / blocks_per_chunk;
+    const int local = (tile - c * blocks_per_chunk) * kBlock + threadIdx.x;
+    const ChunkDesc& d = descs[c];
+    if (local < d.count) {
+      const int64_t idx = d.start + local;
+      const Res64 r = eval64(d.a, d.r64, chunk_start(d.a, idx));
+      if (r.written) d.out[idx] = r.F;
+    }
+  }
+}
+
+static int resident_blocks(const void* kernel) {
+  int dev = 0, cus = 0, per_cu = 0;
+  if (hipGetDevice(&dev) != hipSuccess) return 2048;
+  if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus <= 0) return 2048;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, kBlock, 0) != hipSuccess || per_cu <= 0)
+    return 2 * cus;
+  return per_cu * cus;
+}
+
+int batch_variant(const MapArgs& a) { return a.gen ? 0 : (a.term_bits <= kRN4MaxTermBits ? 1 : 2); }
+
 qj_status launch_batch(const ChunkDesc* d_desc, int nchunks, int chunk_voxels, qj_precision prec, uint32_t* d_band,
-                       uint32_t* d_band_count, hipStream_t s) {
+                       uint32_t* d_counters, hipStream_t s) {
   const int bpc = (chunk_voxels + kBlock - 1) / kBlock;
-  const unsigned grid = (unsigned)(bpc * nchunks);
+  const int64_t cap = (int64_t)nchunks * chunk_voxels;
+  hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, s, d_counters, (int)BC_N);
+  QJ_HIP(hipGetLastError());
   if (prec == QJ_FP64) {
-    hipLaunchKernelGGL(k_batch64, dim3(grid), dim3(kBlock), 0, s, d_desc, bpc, chunk_voxels,
-                       (const uint32_t*)nullptr, (const uint32_t*)nullptr);
+    static const int g64 = resident_blocks((const void*)k_batch64);
+    hipLaunchKernelGGL(k_batch64, dim3(g64), dim3(kBlock), 0, s, d_desc, bpc, d_counters);
     QJ_HIP(hipGetLastError());
     return QJ_OK;
   }
-  hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(1), 0, s, d_band_count);
+  static const int g0 = resident_blocks((const void*)k_batch32<0>);
+  static const int g1 = resident_blocks((const void*)k_batch32<1>);
+  static const int g2 = resident_blocks((const void*)k_batch32<2>);
+  hipLaunchKernelGGL(k_batch32<1>, dim3(g1), dim3(kBlock), 0, s, d_desc, bpc, chunk_voxels, d_band, cap, d_counters);
   QJ_HIP(hipGetLastError());
-  hipLaunchKernelGGL(k_batch32, dim3(grid), dim3(kBlock), 0, s, d_desc, bpc, chunk_voxels, d_band, d_band_count);
+  hipLaunchKernelGGL(k_batch32<2>, dim3(g2), dim3(kBlock), 0, s, d_desc, bpc, chunk_voxels, d_band, cap, d_counters);
   QJ_HIP(hipGetLastError());
-  hipLaunchKernelGGL(k_batch64, dim3(2048), dim3(kBlock), 0, s, d_desc, bpc, chunk_voxels, (const uint32_t*)d_band,
-                     (const uint32_t*)d_band_count);
+  hipLaunchKernelGGL(k_batch32<0>, dim3(g0), dim3(kBlock), 0, s, d_desc, bpc, chunk_voxels, d_band, cap, d_counters);
+  QJ_HIP(hipGetLastError());
+  hipLaunchKernelGGL((k_batch64_list<kExactOnly, false>), dim3(2048), dim3(kBlock), 0, s, d_desc, chunk_voxels,
+                     (const uint32_t*)d_band, cap, (const uint32_t*)(d_counters + BC_BAND_INT));
+  QJ_HIP(hipGetLastError());
+  hipLaunchKernelGGL((k_batch64_list<kExactInt, true>), dim3(1024), dim3(kBlock), 0, s, d_desc, chunk_voxels,
+                     (const uint32_t*)d_band, cap, (const uint32_t*)(d_counters + BC_BAND_GEN));
   QJ_HIP(hipGetLastError());
   return QJ_OK;
 }
@@ -667,9 +748,14 @@ qj_status launch_field(qj_workspace* ws, const MapArgs& a, const qj_rational* f,
     QJ_HIP(hipGetLastError());
     timing_mark(ws, s);
     timing_mark(ws, s);
-    hipLaunchKernelGGL((k_list64<GridSource, false, true>), dim3(list_grid(nvox / 16)), dim3(kBlock), 0, s, a, f->d64,
-                       (const double*)nullptr, ws->d_list, ws->d_list_count, (int64_t)0, d_out, ws->d_counters,
-                       (int)CNT_BAND_VOXELS, (int)CNT_BAND_ITERS, alist, ws->d_list_count + 1);
+    if (a.gen)
+      hipLaunchKernelGGL((k_list64<GridSource, false, kExactInt>), dim3(list_grid(nvox / 16)), dim3(kBlock), 0, s, a,
+                         f->d64, (const double*)nullptr, ws->d_list, ws->d_list_count, (int64_t)0, d_out,
+                         ws->d_counters, (int)CNT_BAND_VOXELS, (int)CNT_BAND_ITERS, alist, ws->d_list_count + 1);
+    else
+      hipLaunchKernelGGL((k_list64<GridSource, false, kExactOnly>), dim3(list_grid(nvox / 16)), dim3(kBlock), 0, s, a,
+                         f->d64, (const double*)nullptr, ws->d_list, ws->d_list_count, (int64_t)0, d_out,
+                         ws->d_counters, (int)CNT_BAND_VOXELS, (int)CNT_BAND_ITERS, alist, ws->d_list_count + 1);
     QJ_HIP(hipGetLastError());
   }
   timing_mark(ws, s);
@@ -703,9 +789,14 @@ qj_status launch_points(qj_workspace* ws, const MapArgs& a, const qj_rational* f
       hipLaunchKernelGGL((k_points32<PointSource, 2, false>), g, b, 0, s, a, k, f->d32, d_pts, n, d_out,
                          ws->d_counters, ws->d_list, ws->d_list_count);
     QJ_HIP(hipGetLastError());
-    hipLaunchKernelGGL((k_list64<PointSource, false, true>), dim3(list_grid(n / 16)), dim3(kBlock), 0, s, a, f->d64,
-                       d_pts, ws->d_list, ws->d_list_count, (int64_t)0, d_out, ws->d_counters,
-                       (int)CNT_BAND_VOXELS, (int)CNT_BAND_ITERS, alist, ws->d_list_count + 1);
+    if (a.gen)
+      hipLaunchKernelGGL((k_list64<PointSource, false, kExactInt>), dim3(list_grid(n / 16)), dim3(kBlock), 0, s, a,
+                         f->d64, d_pts, ws->d_list, ws->d_list_count, (int64_t)0, d_out, ws->d_counters,
+                         (int)CNT_BAND_VOXELS, (int)CNT_BAND_ITERS, alist, ws->d_list_count + 1);
+    else
+      hipLaunchKernelGGL((k_list64<PointSource, false, kExactOnly>), dim3(list_grid(n / 16)), dim3(kBlock), 0, s, a,
+                         f->d64, d_pts, ws->d_list, ws->d_list_count, (int64_t)0, d_out, ws->d_counters,
+                         (int)CNT_BAND_VOXELS, (int)CNT_BAND_ITERS, alist, ws->d_list_count + 1);
     QJ_HIP(hipGetLastError());
   }
   return run_adaptive(ws, a, f, d_pts, n, d_out, s);

@@ -129,13 +129,20 @@ struct ChunkDesc {
   int64_t start;  // first voxel (linear, x fastest) of this chunk in its request
   int32_t count;  // voxels in this chunk (0 = unused slot)
   float band;
+  int32_t total;  // chunks in use in this launch (read from the first descriptor)
+  int32_t g1, g2;  // variant groups (first descriptor): [0,g1) general, [g1,g2) RN4 integer, [g2,total) RN2 integer
 };
+// FP32 variant of a chunk's root set: 0 general exponents, 1 integer exponents
+// with a small term bound, 2 other integer exponents.
+int batch_variant(const MapArgs& a);
 // Enqueue one batch: FP32 bulk over every chunk then FP64 for the band (or
 // FP64 for all when prec == QJ_FP64). d_band/d_band_count: scratch of
 // nchunks * chunk_voxels entries; the count is cleared by a kernel, not a
 // memset, so the sequence can be captured in a graph and replayed.
+// d_counters: 8 uint32 (band counts and work queues), cleared at the start of
+// every batch; d_band: nchunks * chunk_voxels entries.
 qj_status launch_batch(const ChunkDesc* d_desc, int nchunks, int chunk_voxels, qj_precision prec, uint32_t* d_band,
-                       uint32_t* d_band_count, hipStream_t s);
+                       uint32_t* d_counters, hipStream_t s);
 
 // marching cubes (qj_mc.hip)
 qj_status mc_run(qj_workspace* ws, const double* d_field, const qj_grid* g, double iso, int64_t* nv,

@@ -106,9 +106,10 @@ struct Slot {
   ChunkDesc* h_desc = nullptr;
   ChunkDesc* d_desc = nullptr;
   uint32_t* d_band = nullptr;
-  uint32_t* d_band_count = nullptr;
+  uint32_t* d_band_count = nullptr;  // [band count, FP32 queue, FP64 queue, spare]
   hipGraphExec_t exec = nullptr;
   hipEvent_t done = nullptr;
+  hipStream_t stream = nullptr;  // one stream per slot: one launch's FP64 band pass overlaps the next's FP32 bulk
   bool busy = false;
   std::vector<std::pair<uint64_t, int>> work;  // (request id, chunks in this launch)
 };
@@ -122,6 +123,7 @@ struct qj_server {
   hipStream_t stream = nullptr;
   qj::Arena arena;
   std::vector<qj::Slot> slots;
+  std::vector<qj::ChunkDesc> batch_tmp;  // build_batch scratch
   std::chrono::steady_clock::time_point t0;
   std::thread worker;
   std::mutex mu;
@@ -168,10 +170,11 @@ static bool try_admit(qj_server* s, Req* r) {
   r->off64 = off + (((size_t)r->voxels * sizeof(double) + 255) & ~(size_t)255);
   r->off32 = r->off64 + ((r->h64.size() * sizeof(RootRec64) + 255) & ~(size_t)255);
   char* b = s->arena.base();
-  if (hipMemcpyAsync(b + r->off64, r->h64.data(), r->h64.size() * sizeof(RootRec64), hipMemcpyHostToDevice,
-                     s->stream) != hipSuccess ||
-      hipMemcpyAsync(b + r->off32, r->h32.data(), r->h32.size() * sizeof(RootRec32), hipMemcpyHostToDevice,
-                     s->stream) != hipSuccess) {
+  // synchronous (a few KB): the launches run on the slots' own streams
+  if (hipMemcpy(b + r->off64, r->h64.data(), r->h64.size() * sizeof(RootRec64), hipMemcpyHostToDevice) !=
+          hipSuccess ||
+      hipMemcpy(b + r->off32, r->h32.data(), r->h32.size() * sizeof(RootRec32), hipMemcpyHostToDevice) !=
+          hipSuccess) {
     s->arena.free(off, r->bytes);
     return false;
   }
@@ -195,11 +198,13 @@ static int build_batch(qj_server* s, Slot& sl) {
   double work = 0.0;
   sl.work.clear();
   char* b = s->arena.base();
+  std::vector<ChunkDesc>& tmp = s->batch_tmp;
+  tmp.resize(maxc);
   for (Req* r : order) {
     int taken = 0;
     const double per_voxel = (double)r->params.max_iter * (double)(r->nt + r->nb);
     while (n < maxc && r->next < r->voxels && (n == 0 || work < s->cfg.launch_work)) {
-      ChunkDesc& d = sl.h_desc[n++];
+      ChunkDesc& d = tmp[n++];
       d.a = r->args;
       d.r64 = (const RootRec64*)(b + r->off64);
       d.r32 = (const RootRec32*)(b + r->off32);
@@ -215,7 +220,16 @@ static int build_batch(qj_server* s, Slot& sl) {
     if (taken) sl.work.emplace_back(r->id, taken);
     if (n == maxc || work >= s->cfg.launch_work) break;
   }
+  // group by FP32 variant (the kernels take one group each); order inside a
+  // group keeps the shortest-remaining-work order
+  int g[4] = {0, 0, 0, 0};
+  for (int i = 0; i < n; i++) g[batch_variant(tmp[i].a) + 1]++;
+  int pos[3] = {0, g[1], g[1] + g[2]};
+  for (int i = 0; i < n; i++) sl.h_desc[pos[batch_variant(tmp[i].a)]++] = tmp[i];
   for (int i = n; i < maxc; i++) sl.h_desc[i].count = 0;
+  sl.h_desc[0].total = n;
+  sl.h_desc[0].g1 = g[1];
+  sl.h_desc[0].g2 = g[1] + g[2];
   return n;
 }
 
@@ -291,8 +305,8 @@ static void worker_main(qj_server* s) {
         Slot& sl = s->slots[(s->slot_next + k) % s->slots.size()];
         if (sl.busy) continue;
         if (build_batch(s, sl) == 0) break;
-        hipError_t e = hipGraphLaunch(sl.exec, s->stream);
-        if (e == hipSuccess) e = hipEventRecord(sl.done, s->stream);
+        hipError_t e = hipGraphLaunch(sl.exec, sl.stream);
+        if (e == hipSuccess) e = hipEventRecord(sl.done, sl.stream);
         sl.busy = true;
         if (e != hipSuccess) {
           complete_slot(s, sl, QJ_ERR_HIP);
@@ -329,19 +343,22 @@ static qj_status capture_slot(qj_server* s, Slot& sl) {
   memset(sl.h_desc, 0, sizeof(ChunkDesc) * nch);
   QJ_HIP(hipMalloc(&sl.d_desc, sizeof(ChunkDesc) * nch));
   QJ_HIP(hipMemset(sl.d_desc, 0, sizeof(ChunkDesc) * nch));
-  if (s->cfg.precision != QJ_FP64) {
+  if (s->cfg.precision == QJ_FP64) {
+    QJ_HIP(hipMalloc(&sl.d_band_count, 8 * sizeof(uint32_t)));  // work-queue counters
+  } else {
     QJ_HIP(hipMalloc(&sl.d_band, sizeof(uint32_t) * (size_t)nch * s->cfg.chunk_voxels));
-    QJ_HIP(hipMalloc(&sl.d_band_count, sizeof(uint32_t)));
+    QJ_HIP(hipMalloc(&sl.d_band_count, 8 * sizeof(uint32_t)));
   }
   QJ_HIP(hipEventCreateWithFlags(&sl.done, hipEventDisableTiming));
+  QJ_HIP(hipStreamCreateWithFlags(&sl.stream, hipStreamNonBlocking));
   hipGraph_t g;
-  QJ_HIP(hipStreamBeginCapture(s->stream, hipStreamCaptureModeThreadLocal));
-  hipError_t e = hipMemcpyAsync(sl.d_desc, sl.h_desc, sizeof(ChunkDesc) * nch, hipMemcpyHostToDevice, s->stream);
+  QJ_HIP(hipStreamBeginCapture(sl.stream, hipStreamCaptureModeThreadLocal));
+  hipError_t e = hipMemcpyAsync(sl.d_desc, sl.h_desc, sizeof(ChunkDesc) * nch, hipMemcpyHostToDevice, sl.stream);
   qj_status st = QJ_OK;
   if (e == hipSuccess)
     st = launch_batch(sl.d_desc, nch, s->cfg.chunk_voxels, (qj_precision)s->cfg.precision, sl.d_band,
-                      sl.d_band_count, s->stream);
-  hipError_t e2 = hipStreamEndCapture(s->stream, &g);
+                      sl.d_band_count, sl.stream);
+  hipError_t e2 = hipStreamEndCapture(sl.stream, &g);
   if (e != hipSuccess) return hip_fail(e, "capture memcpy");
   if (st) return st;
   if (e2 != hipSuccess) return hip_fail(e2, "hipStreamEndCapture");
@@ -353,12 +370,14 @@ static qj_status capture_slot(qj_server* s, Slot& sl) {
 
 static void destroy_server(qj_server* s) {
   for (Slot& sl : s->slots) {
+    if (sl.stream) (void)hipStreamSynchronize(sl.stream);
     if (sl.exec) (void)hipGraphExecDestroy(sl.exec);
     if (sl.done) (void)hipEventDestroy(sl.done);
     if (sl.h_desc) (void)hipHostFree(sl.h_desc);
     if (sl.d_desc) (void)hipFree(sl.d_desc);
     if (sl.d_band) (void)hipFree(sl.d_band);
     if (sl.d_band_count) (void)hipFree(sl.d_band_count);
+    if (sl.stream) (void)hipStreamDestroy(sl.stream);
   }
   s->arena.release();
   if (s->stream) (void)hipStreamDestroy(s->stream);
@@ -382,7 +401,7 @@ qj_status qj_server_create(const qj_server_config* cfg, qj_server** out) {
   if (c.chunks_per_launch <= 0) c.chunks_per_launch = 512;
   if (c.max_roots <= 0) c.max_roots = 256;
   if (c.inflight_launches <= 0) c.inflight_launches = 2;
-  if (c.launch_work <= 0) c.launch_work = 6e9;
+  if (c.launch_work <= 0) c.launch_work = 3e9;
   if (c.band <= 0) c.band = 0.05;
   if (c.precision != QJ_FP64 && c.precision != QJ_FP32_BAND) {
     delete s;
