@@ -20,6 +20,10 @@ sys.path.insert(0, ROOT)
 
 PEAK_FP32_TFLOPS = 157.3   # MI355X published FP32 vector peak
 PEAK_FP64_TFLOPS = 78.6    # MI355X published FP64 vector peak
+# HBM bytes per k_grid32 launch of the 512^3 workload, measured with rocprofv3
+# FETCH_SIZE and WRITE_SIZE (each scaled by the same counter on a 1 GiB copy):
+# 0.227 MB fetched + 1113.36 MB written.
+K_GRID32_TRAFFIC_BYTES = 1113589699
 
 
 def log(*a):
@@ -166,8 +170,12 @@ def main():
     band_ms = k32["band_ms"] / max(1, k32["launches"])
     flops = iters_per_launch * basic_ops_per_voxel_iter(nt, nb)
     achieved = flops / (bulk_ms * 1e-3) / 1e12
-    roofline = {"bound": "valu", "kernel": "k_grid32", "achieved": round(achieved, 3), "peak": PEAK_FP32_TFLOPS,
-                "unit": "TFLOP/s", "frac": round(achieved / PEAK_FP32_TFLOPS, 4), "traffic": None,
+    traffic = K_GRID32_TRAFFIC_BYTES if args.res == 512 else None
+    roofline = {"bound": "mfma", "compute_unit": "FP32 vector ALU (no matrix op applies; published FP32 peak)",
+                "kernel": "k_grid32", "achieved": round(achieved, 3), "peak": PEAK_FP32_TFLOPS,
+                "unit": "TFLOP/s", "frac": round(achieved / PEAK_FP32_TFLOPS, 4), "traffic": traffic,
+                "traffic_note": "HBM bytes per launch from rocprofv3 FETCH_SIZE/WRITE_SIZE passes scaled by a 1 GiB "
+                                "copy (profiles/k_grid32_counters_c2_512.md); algorithmic bytes %d" % (w.voxels * 8),
                 "algorithmic_flops_per_launch": flops, "kernel_ms": round(bulk_ms, 4),
                 "executed_voxel_iters_per_launch": iters_per_launch}
     fp64_ms = k64["bulk_ms"] / max(1, k64["launches"])
