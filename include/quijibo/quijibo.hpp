@@ -252,4 +252,61 @@ class OPTIMIZE_3D {
   std::vector<double> _apKey;
 };
 
+// TOUPEE callback ABI (TAO_SOLVER.h:41-43) with VECTOR = std::vector<Real>.
+typedef std::vector<Real> VECTOR;
+typedef void (*FunctionGradient)(const VECTOR& state, Real& value, VECTOR& gradient);
+typedef void (*Monitor)();
+
+// Bound-constrained limited-memory variable metric minimisation
+// (TAO_SOLVER::optimizeBLMVM, TAO_SOLVER.cpp:72-341): projected quasi-Newton
+// steps inside [lowerBound, upperBound] with a bounded More-Thuente line
+// search, TAO's default convergence test and the reference's defaults
+// (tolerances 1e-8, 2000 iterations).
+class TAO_SOLVER {
+ public:
+  enum SOLVER_STATE {  // TAO_SOLVER.h:62-85
+    CONVERGED_FATOL = 1,
+    CONVERGED_FRTOL = 2,
+    CONVERGED_GATOL = 3,
+    CONVERGED_GRTOL = 4,
+    CONVERGED_GTTOL = 5,
+    CONVERGED_STEPTOL = 6,
+    CONVERGED_MINF = 7,
+    CONVERGED_USER = 8,
+    DIVERGED_MAXITS = -2,
+    DIVERGED_NAN = -4,
+    DIVERGED_MAXFCN = -5,
+    DIVERGED_LS_FAILURE = -6,
+    CONTINUE_ITERATING = 0
+  };
+
+  TAO_SOLVER() {}
+  void SetTolerances(Real fatol, Real frtol, Real gatol, Real grtol, Real gttol);
+  int& maxIterations() { return _maxIterations; }
+  Real& minimumValue() { return _fmin; }
+  Real& maxScore() { return _maxScore; }  // progress printing scale in the reference; kept for callers
+
+  // Minimises from `solution` (overwritten with the result); returns the
+  // number of iterations. functionGradient is called once per trial point.
+  int optimizeBLMVM(VECTOR& solution, const VECTOR& upperBound, const VECTOR& lowerBound,
+                    FunctionGradient functionGradient, Monitor monitor = nullptr);
+
+  SOLVER_STATE reason() const { return _reason; }
+  int iterations() const { return _niter; }
+  Real value() const { return _f; }
+  // objective after the start and after each iteration
+  const std::vector<Real>& history() const { return _history; }
+
+ private:
+  void converged(Real f, Real gnorm);
+
+  Real _fatol = 1e-8, _frtol = 1e-8, _gatol = 1e-8, _grtol = 1e-8, _gttol = 0.0;
+  Real _fmin = -1e100, _maxScore = 1.0;
+  int _maxIterations = 2000;
+  int _niter = 0;
+  Real _gnorm0 = 0, _f = 0;
+  SOLVER_STATE _reason = CONTINUE_ITERATING;
+  std::vector<Real> _history;
+};
+
 }  // namespace quijibo

@@ -13,6 +13,12 @@
 //       TRIANGLE_MESH(center, lengths, res, top, bottom, ...) -> nonlinear.obj.
 //   host_layer_driver ap <in.o3d> <weight.field> <alpha> <threshold>
 //       one-iteration AP score and gradients printed as JSON.
+//   host_layer_driver blmvm rosenbrock|boxquad
+//       TAO_SOLVER::optimizeBLMVM on a test problem (no GPU); JSON result.
+//   host_layer_driver optimize <in.o3d> <weight.field> <alpha> <iterations>
+//       the shape optimizer's exponent fit (toothOptimizer.cpp:65-110,
+//       160-218): BLMVM over [log c, bottom powers, top powers, top scalar,
+//       bottom scalar] with the GPU AP score and gradients as the callback.
 #include <math.h>
 #include <stdio.h>
 #include <string.h>
@@ -145,6 +151,120 @@ static int run_ap(const std::string& in, const std::string& weight, double alpha
   return 0;
 }
 
+// ---- BLMVM test problems (the same functions as tests/test_blmvm.py) ----
+static void rosenbrock(const VECTOR& x, Real& f, VECTOR& g) {
+  const size_t n = x.size();
+  f = 0;
+  g.assign(n, 0.0);
+  for (size_t i = 0; i + 1 < n; i++) {
+    const Real a = x[i + 1] - x[i] * x[i], b = 1.0 - x[i];
+    f += 100.0 * a * a + b * b;
+    g[i] += -400.0 * a * x[i] - 2.0 * b;
+    g[i + 1] += 200.0 * a;
+  }
+}
+
+static void boxquad(const VECTOR& x, Real& f, VECTOR& g) {
+  // 0.5 x'Ax - b'x with A = tridiag(-1, 2.5, -1)
+  static const Real b[6] = {1.0, -2.0, 3.0, 0.5, -1.0, 2.0};
+  const size_t n = x.size();
+  f = 0;
+  g.assign(n, 0.0);
+  for (size_t i = 0; i < n; i++) {
+    Real ax = 2.5 * x[i];
+    if (i > 0) ax -= x[i - 1];
+    if (i + 1 < n) ax -= x[i + 1];
+    g[i] = ax - b[i];
+    f += 0.5 * x[i] * ax - b[i] * x[i];
+  }
+}
+
+static void print_vec(const char* name, const VECTOR& v) {
+  printf("\"%s\": [", name);
+  for (size_t i = 0; i < v.size(); i++) printf("%s%.17g", i ? ", " : "", v[i]);
+  printf("]");
+}
+
+static int run_blmvm(const std::string& problem) {
+  TAO_SOLVER tao;
+  VECTOR x, lo, hi;
+  FunctionGradient fg;
+  if (problem == "rosenbrock") {
+    x = {-1.2, 1.0, -1.2, 1.0};
+    lo.assign(4, -3.4028234663852886e38);
+    hi.assign(4, 3.4028234663852886e38);
+    fg = rosenbrock;
+  } else if (problem == "boxquad") {
+    x.assign(6, 0.0);
+    lo.assign(6, -0.5);
+    hi.assign(6, 0.8);
+    fg = boxquad;
+  } else {
+    return fail("unknown problem");
+  }
+  tao.optimizeBLMVM(x, hi, lo, fg);
+  printf("{\"iterations\": %d, \"reason\": %d, \"f\": %.17g, ", tao.iterations(), (int)tao.reason(), tao.value());
+  print_vec("x", x);
+  printf(", ");
+  print_vec("history", tao.history());
+  printf("}\n");
+  return 0;
+}
+
+// ---- the exponent fit with the GPU callback ----
+static OPTIMIZE_3D* g_opt = nullptr;
+static int g_evals = 0;
+
+static void scaled_function_gradient(const VECTOR& state, Real& function, VECTOR& gradient) {
+  OPTIMIZE_3D& o = *g_opt;
+  const int nt = o.top().totalRoots(), nb = o.bottom().totalRoots();
+  const int scalingStart = 1 + nt + nb;
+  o.expScaling() = exp(state[0]);
+  int k = 1;
+  for (int i = 0; i < nb; i++, k++) o.bottom().powersMutable()[i] = state[k];
+  for (int i = 0; i < nt; i++, k++) o.top().powersMutable()[i] = state[k];
+  o.top().powerScalar() = state[scalingStart];
+  o.bottom().powerScalar() = state[scalingStart + 1];
+  function = o.computeScoreAP();
+  gradient.resize(scalingStart + 2);
+  gradient[0] = o.computeExpGradientAP();
+  for (int i = 0; i < nb; i++) gradient[i + 1] = o.computeBottomGradientAP(i);
+  for (int i = 0; i < nt; i++) gradient[i + 1 + nb] = o.computeTopGradientAP(i);
+  gradient[scalingStart] = o.computeTopBulkScalarGradientAP();
+  gradient[scalingStart + 1] = o.computeBottomBulkScalarGradientAP();
+  g_evals++;
+}
+
+static int run_optimize(const std::string& in, const std::string& weight, double alpha, int iterations) {
+  OPTIMIZE_3D opt;
+  opt.read(in);
+  opt.maxIterations() = 1;
+  opt.tanhAlpha() = alpha;
+  opt.weightField().read(weight);
+  g_opt = &opt;
+  const int nt = opt.top().totalRoots(), nb = opt.bottom().totalRoots();
+  const int n = 1 + nt + nb + 2;
+  VECTOR x(n);
+  x[0] = log(opt.expScaling());
+  int k = 1;
+  for (int i = 0; i < nb; i++, k++) x[k] = opt.bottom().powers()[i];
+  for (int i = 0; i < nt; i++, k++) x[k] = opt.top().powers()[i];
+  x[n - 2] = opt.top().powerScalar();
+  x[n - 1] = opt.bottom().powerScalar();
+  VECTOR hi(n, 3.4028234663852886e38), lo(n, 0.0);  // toothOptimizer.cpp:189-191
+  lo[0] = -3.4028234663852886e38;
+  TAO_SOLVER tao;
+  tao.maxIterations() = iterations;
+  tao.optimizeBLMVM(x, hi, lo, scaled_function_gradient);
+  printf("{\"iterations\": %d, \"reason\": %d, \"evaluations\": %d, \"f\": %.17g, ", tao.iterations(),
+         (int)tao.reason(), g_evals, tao.value());
+  print_vec("x", x);
+  printf(", ");
+  print_vec("history", tao.history());
+  printf("}\n");
+  return 0;
+}
+
 int main(int argc, char** argv) {
   try {
     const std::string mode = argc > 1 ? argv[1] : "";
@@ -158,6 +278,8 @@ int main(int argc, char** argv) {
     }
     if (mode == "map" && argc == 6) return run_map(argv[2], atoi(argv[3]), atoi(argv[4]), argv[5]);
     if (mode == "ap" && argc == 6) return run_ap(argv[2], argv[3], atof(argv[4]), atof(argv[5]));
+    if (mode == "blmvm" && argc == 3) return run_blmvm(argv[2]);
+    if (mode == "optimize" && argc == 6) return run_optimize(argv[2], argv[3], atof(argv[4]), atoi(argv[5]));
   } catch (const Error& e) {
     fprintf(stderr, "quijibo::Error %d: %s\n", e.status, e.what());
     return 2;
