@@ -65,6 +65,10 @@ typedef struct qj_poly {
 #define QJ_MAP_SKIP_ORIGIN 2u /* leave cells with |q0| < 1e-8 unwritten (OPTIMIZE_3D.cpp:1428, 1619) */
 #define QJ_MAP_ADAPTIVE 4u    /* flag overflowed cells and recompute them with extended range
                                  (computeRationalMapAP's long double re-pass, OPTIMIZE_3D.cpp:1477-1567) */
+#define QJ_MAP_LAGRANGE 8u    /* RATIONAL_4D::evaluateLagrange order (RATIONAL_4D.cpp:27-88): top and bottom
+                                 factors interleaved, r = (r t_i) / b_i, then the leftovers; the map of the
+                                 Lagrange mesher (TRIANGLE_MESH.cpp:106-123, 281-324). QJ_FP64 only; not
+                                 with QJ_MAP_SKIP_ORIGIN or QJ_MAP_ADAPTIVE */
 
 /* Map parameters: the OPTIMIZE_3D members the map reads (OPTIMIZE_3D.h:67,73,81)
  * plus the escape radius each reference variant hard-codes (2000 at
@@ -182,6 +186,26 @@ qj_status qj_mc_device_ptrs(qj_workspace* ws, const double** d_vertices, const i
 qj_status qj_nlmc_run(qj_workspace* ws, const qj_grid* grid, const qj_rational* f, const qj_map_params* params,
                       double iso, qj_precision precision, double band, int32_t chunk_slices,
                       int64_t* n_vertices, int64_t* n_triangles, void* stream);
+
+/* The edge cache of the last qj_nlmc_run / qj_nlmc_resume on ws, the data of
+ * the mesher's cache file (writeEdgeCacheHuge / readEdgeCacheHuge,
+ * TRIANGLE_MESH.cpp:431-586): one edge per vertex, in vertex order, as the
+ * lower endpoint's (x, y, z) and the axis (0 x, 1 y, 2 z), and the active
+ * cells in cell order as (x, y, z, flag). Arrays are int32 x 4 per entry, host
+ * or device (hipMemcpyDefault); pass null arrays to read the counts only. */
+qj_status qj_nlmc_cache(qj_workspace* ws, int64_t* n_edges, int32_t* edges, int64_t* n_cells, int32_t* cells,
+                        void* stream);
+
+/* The mesher's resume path (TRIANGLE_MESH.cpp:703-711 with a cache present):
+ * rebuild the mesh from an edge cache (host arrays in the qj_nlmc_cache
+ * layout) without evaluating slabs: each listed edge's vertex by the same
+ * bisection as qj_nlmc_run, triangles from the listed cells. Edges must be in
+ * vertex order and inside the grid; a triangle whose edge is not listed is an
+ * error. The mesh is identical to the run that wrote the cache. Results via
+ * qj_mc_copy. Synchronizes `stream`. */
+qj_status qj_nlmc_resume(qj_workspace* ws, const qj_grid* grid, const qj_rational* f, const qj_map_params* params,
+                         double iso, int64_t n_edges, const int32_t* edges, int64_t n_cells, const int32_t* cells,
+                         int64_t* n_vertices, int64_t* n_triangles, void* stream);
 
 /* ---- gradients ---- */
 

@@ -40,6 +40,14 @@ struct VEC3F {
   void read(FILE* f);
 };
 
+struct VEC3I {  // integer triple (VEC3.h TVEC3<int>)
+  int v[3];
+  VEC3I() : v{0, 0, 0} {}
+  VEC3I(int a, int b, int c) : v{a, b, c} {}
+  int& operator[](int i) { return v[i]; }
+  int operator[](int i) const { return v[i]; }
+};
+
 // Storage of a quaternion, slot order (w, x, y, z) (QUATERNION.h:31-155).
 struct QUATERNION {
   Real _w = 0, _x = 0, _y = 0, _z = 0;
@@ -146,14 +154,23 @@ class TRIANGLE_MESH {
   TRIANGLE_MESH() {}
   // marching cubes on a field (TRIANGLE_MESH.cpp:48-57 -> computeStagedMarchingCubes)
   explicit TRIANGLE_MESH(const FIELD_3D& field, Real isosurface = 0.0);
-  // non-linear low-memory mesher (TRIANGLE_MESH.cpp:84-101): slabs of the map
-  // evaluated on the fly, vertices refined by bisection
-  // precision of the slab evaluation (vertices are always FP64 bisection);
-  // chunkSlices = z-slices evaluated per slab
-  TRIANGLE_MESH(const VEC3F& center, const VEC3F& lengths, const int res[3], const POLYNOMIAL_4D& top,
+  // Non-linear low-memory mesher with an edge cache (TRIANGLE_MESH.cpp:84-101,
+  // 681-768): slabs of the map evaluated on the fly, vertices refined by
+  // bisection. When cacheFilename names an existing cache (readEdgeCacheHuge
+  // layout) the mesh is rebuilt from it without evaluating slabs; otherwise
+  // the cache is written after the run (an empty name skips the cache).
+  // precision: slab evaluation (vertices are always FP64); chunkSlices:
+  // z-slices evaluated per slab.
+  TRIANGLE_MESH(const VEC3F& center, const VEC3F& lengths, const VEC3I& res, const POLYNOMIAL_4D& top,
                 const POLYNOMIAL_4D& bottom, Real expScaling, int maxIterations, Real slice, Real isosurface,
-                const QUATERNION& rotation = QUATERNION(1, 0, 0, 0), qj_precision precision = QJ_FP32_BAND,
+                const QUATERNION& rotation, const std::string& cacheFilename, qj_precision precision = QJ_FP32_BAND,
                 int chunkSlices = 64);
+  // The same mesher with the Lagrange evaluation order (TRIANGLE_MESH.cpp:
+  // 106-123, computeLagrangeNonlinearMarchingCubesLowMemoryHuge :773-857),
+  // FP64 slabs.
+  TRIANGLE_MESH(const VEC3F& center, const VEC3F& lengths, const VEC3I& res, const POLYNOMIAL_4D& top,
+                const POLYNOMIAL_4D& bottom, Real expScaling, int maxIterations, Real slice, Real isosurface,
+                const QUATERNION& rotation);
 
   std::vector<VEC3F>& vertices() { return _vertices; }
   const std::vector<VEC3F>& vertices() const { return _vertices; }

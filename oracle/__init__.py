@@ -16,6 +16,7 @@ _lib = None
 BLACK_HOLE = 1
 SKIP_ORIGIN = 2
 AP_REPASS = 4
+LAGRANGE = 8
 
 
 class Grid(ctypes.Structure):

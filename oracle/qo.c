@@ -286,6 +286,30 @@ double qo_field_lookup(const qo_grid* g, const double* data, const double* posit
 
 /* ---------------- the map ---------------- */
 
+/* RATIONAL_4D::evaluateLagrange (RATIONAL_4D.cpp:27-88): with bottom roots,
+ * r = 1, then r = (r * t_i) / b_i for the first min(nt, nb) root pairs, then the
+ * remaining top terms multiplied or the remaining bottom terms divided, in
+ * order; t_i / b_i are evaluateRoot (POLYNOMIAL_4D.cpp:928-940). Without
+ * bottom roots it is the factored top product. */
+static Q term_of(const qo_poly* p, int i, Q q) {
+  const Q b = {q.w - p->roots[4 * i], q.x - p->roots[4 * i + 1], q.y - p->roots[4 * i + 2],
+               q.z - p->roots[4 * i + 3]};
+  return qpow(b, p->power_scalar * p->powers[i]);
+}
+
+static Q lagrange_eval(const qo_poly* top, const qo_poly* bottom, Q q) {
+  if (bottom->n == 0) return poly_eval(top, q);
+  Q r = {1.0, 0.0, 0.0, 0.0};
+  const int same = top->n > bottom->n ? bottom->n : top->n;
+  for (int i = 0; i < same; i++) {
+    r = qmul(r, term_of(top, i, q));
+    r = qdiv(r, term_of(bottom, i, q));
+  }
+  for (int i = same; i < top->n; i++) r = qmul(r, term_of(top, i, q));
+  for (int i = same; i < bottom->n; i++) r = qdiv(r, term_of(bottom, i, q));
+  return r;
+}
+
 /* One voxel of OPTIMIZE_3D::computeLogScaledPowerRationalMap
  * (OPTIMIZE_3D.cpp:1317-1362) / computeRationalMapAP first pass (:1423-1475).
  * Returns 1 when the value was written, 0 for a skipped origin cell. */
@@ -297,12 +321,16 @@ static int iterate_point(Q q, const qo_poly* top, const qo_poly* bottom, const q
   const int ap = (p->flags & QO_AP_REPASS) != 0;
   while (magnitude < p->escape && it < p->max_iter) {
     if (ap) feclearexcept(FE_OVERFLOW);
-    Q topEval = poly_eval(top, q);
-    if (bottom->n > 0) {
-      Q bottomEval = poly_eval(bottom, q);
-      q = qdiv(topEval, bottomEval);
+    if (p->flags & QO_LAGRANGE) {
+      q = lagrange_eval(top, bottom, q);  /* computeLagrangeNonlinearSlice, TRIANGLE_MESH.cpp:281-324 */
     } else {
-      q = topEval;
+      Q topEval = poly_eval(top, q);
+      if (bottom->n > 0) {
+        Q bottomEval = poly_eval(bottom, q);
+        q = qdiv(topEval, bottomEval);
+      } else {
+        q = topEval;
+      }
     }
     if (ap && fetestexcept(FE_OVERFLOW)) *overflowed = 1;
     q = qscale(q, p->exp_scaling);
@@ -547,6 +575,26 @@ int qo_marching_cubes(const qo_grid* g, const double* field, int64_t* n_vertices
 static double nonlinear_value(const double* pos, const qo_poly* top, const qo_poly* bottom, const qo_params* p,
                               double iso) {
   Q q = {pos[0], pos[1], pos[2], p->slice};
+  if (p->flags & QO_LAGRANGE) {
+    /* TRIANGLE_MESH::lagrangeNonlinearValue (TRIANGLE_MESH.cpp:1684-1730): an
+     * infinite |q| restores the previous iterate and stops. The reference's
+     * previous magnitude is uninitialised before the first iteration; here it
+     * starts as |q0|. */
+    double magnitude = qmag(q), prev = magnitude;
+    int it = 0;
+    while (magnitude < p->escape && it < p->max_iter) {
+      q = qscale(lagrange_eval(top, bottom, q), p->exp_scaling);
+      magnitude = qmag(q);
+      if (isinf(magnitude)) {
+        magnitude = prev;
+        break;
+      }
+      prev = magnitude;
+      it++;
+      if (magnitude < 10.0 * FLT_MIN) it = p->max_iter;
+    }
+    return log(magnitude) - iso;
+  }
   double F = 0;
   int of = 0;
   iterate_point(q, top, bottom, p, &F, NULL, &of);

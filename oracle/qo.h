@@ -34,7 +34,8 @@ typedef struct qo_poly {
 enum {
   QO_BLACK_HOLE = 1u,  /* stop when |q| < 10*FLT_MIN (OPTIMIZE_3D.cpp:1358-1359) */
   QO_SKIP_ORIGIN = 2u, /* leave the cell unwritten when |q0| < 1e-8 (OPTIMIZE_3D.cpp:1428) */
-  QO_AP_REPASS = 4u    /* overflow flags + long double re-pass (OPTIMIZE_3D.cpp:1440-1567) */
+  QO_AP_REPASS = 4u,   /* overflow flags + long double re-pass (OPTIMIZE_3D.cpp:1440-1567) */
+  QO_LAGRANGE = 8u     /* RATIONAL_4D::evaluateLagrange order (RATIONAL_4D.cpp:27-88) */
 };
 
 typedef struct qo_params {

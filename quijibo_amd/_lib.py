@@ -15,6 +15,7 @@ FP32_BAND = 1
 MAP_BLACK_HOLE = 1
 MAP_SKIP_ORIGIN = 2
 MAP_ADAPTIVE = 4
+MAP_LAGRANGE = 8
 
 c_double_p = ctypes.POINTER(ctypes.c_double)
 
@@ -96,6 +97,11 @@ SIGNATURES = [
     ("qj_nlmc_run", ctypes.c_int32, [_P, ctypes.POINTER(Grid), _P, ctypes.POINTER(MapParams), ctypes.c_double,
                                      ctypes.c_int32, ctypes.c_double, ctypes.c_int32, ctypes.POINTER(ctypes.c_int64),
                                      ctypes.POINTER(ctypes.c_int64), _P]),
+    ("qj_nlmc_cache", ctypes.c_int32, [_P, ctypes.POINTER(ctypes.c_int64), _P, ctypes.POINTER(ctypes.c_int64), _P,
+                                       _P]),
+    ("qj_nlmc_resume", ctypes.c_int32, [_P, ctypes.POINTER(Grid), _P, ctypes.POINTER(MapParams), ctypes.c_double,
+                                        ctypes.c_int64, _P, ctypes.c_int64, _P, ctypes.POINTER(ctypes.c_int64),
+                                        ctypes.POINTER(ctypes.c_int64), _P]),
     ("qj_mc_copy", ctypes.c_int32, [_P, _P, _P, _P]),
     ("qj_mc_device_ptrs", ctypes.c_int32, [_P, _PP, _PP]),
     ("qj_ap_score_gradients", ctypes.c_int32, [_P, ctypes.POINTER(Grid), _P, ctypes.POINTER(MapParams),

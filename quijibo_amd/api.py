@@ -335,3 +335,35 @@ def nonlinear_mesh(grid, rational, params, iso=0.0, precision=FP32_BAND, band=DE
     check(L.qj_mc_copy(ws.handle, v.ctypes.data_as(ctypes.c_void_p) if nv.value else None,
                        t.ctypes.data_as(ctypes.c_void_p) if nt.value else None, _stream_ptr(stream)))
     return v, t
+
+
+def nonlinear_mesh_cache(workspace=None, stream=None):
+    """qj_nlmc_cache: (edges n x 4 int32 [x, y, z, axis], cells m x 4 int32 [x, y, z, flag]) of the last
+    non-linear mesh on the workspace."""
+    ws = workspace or default_workspace()
+    L = _lib.load()
+    ne, nc = ctypes.c_int64(), ctypes.c_int64()
+    check(L.qj_nlmc_cache(ws.handle, ctypes.byref(ne), None, ctypes.byref(nc), None, _stream_ptr(stream)))
+    e = np.zeros((ne.value, 4), dtype=np.int32)
+    c = np.zeros((nc.value, 4), dtype=np.int32)
+    check(L.qj_nlmc_cache(ws.handle, None, e.ctypes.data_as(ctypes.c_void_p) if ne.value else None, None,
+                          c.ctypes.data_as(ctypes.c_void_p) if nc.value else None, _stream_ptr(stream)))
+    return e, c
+
+
+def nonlinear_mesh_resume(grid, rational, params, edges, cells, iso=0.0, workspace=None, stream=None):
+    """qj_nlmc_resume: the mesh rebuilt from an edge cache; returns (vertices, triangles)."""
+    ws = workspace or default_workspace()
+    L = _lib.load()
+    e = np.ascontiguousarray(edges, dtype=np.int32).reshape(-1, 4)
+    c = np.ascontiguousarray(cells, dtype=np.int32).reshape(-1, 4)
+    nv, nt = ctypes.c_int64(), ctypes.c_int64()
+    check(L.qj_nlmc_resume(ws.handle, ctypes.byref(grid), rational.handle, ctypes.byref(params), float(iso),
+                           e.shape[0], e.ctypes.data_as(ctypes.c_void_p) if e.shape[0] else None, c.shape[0],
+                           c.ctypes.data_as(ctypes.c_void_p) if c.shape[0] else None, ctypes.byref(nv),
+                           ctypes.byref(nt), _stream_ptr(stream)))
+    v = np.empty((nv.value, 3), dtype=np.float64)
+    t = np.empty((nt.value, 3), dtype=np.int32)
+    check(L.qj_mc_copy(ws.handle, v.ctypes.data_as(ctypes.c_void_p) if nv.value else None,
+                       t.ctypes.data_as(ctypes.c_void_p) if nt.value else None, _stream_ptr(stream)))
+    return v, t

@@ -385,10 +385,9 @@ TRIANGLE_MESH::TRIANGLE_MESH(const FIELD_3D& field, Real isosurface) {
   fetch_mesh(ws, nv, nt, _vertices, _triangles);
 }
 
-TRIANGLE_MESH::TRIANGLE_MESH(const VEC3F& center, const VEC3F& lengths, const int res[3], const POLYNOMIAL_4D& top,
-                             const POLYNOMIAL_4D& bottom, Real expScaling, int maxIterations, Real slice,
-                             Real isosurface, const QUATERNION& rotation, qj_precision precision, int chunkSlices) {
-  qj_workspace* ws = shared_workspace();
+namespace {
+
+qj_grid mesher_grid(const VEC3F& center, const VEC3F& lengths, const VEC3I& res, const QUATERNION& rotation) {
   qj_grid g;
   for (int i = 0; i < 3; i++) {
     g.res[i] = res[i];
@@ -396,15 +395,133 @@ TRIANGLE_MESH::TRIANGLE_MESH(const VEC3F& center, const VEC3F& lengths, const in
     g.lengths[i] = lengths[i];
   }
   for (int i = 0; i < 4; i++) g.rotation[i] = rotation[i];
+  return g;
+}
+
+// readEdgeCacheHuge (TRIANGLE_MESH.cpp:431-481): int count; per vertex the
+// two grid points of its edge (3 + 3 ints); int count; per active cell the
+// flag and the cell (1 + 3 ints). Returns false when the file does not exist.
+bool read_edge_cache(const std::string& name, std::vector<int32_t>& edges, std::vector<int32_t>& cells) {
+  FILE* f = fopen(name.c_str(), "rb");
+  if (!f) return false;
+  Reader r;
+  r.f = f;
+  try {
+    const int n = r.i32();
+    if (n < 0) throw Error(QJ_ERR_INVALID, "edge cache: negative triplet count");
+    edges.resize(4 * (size_t)n);
+    for (int i = 0; i < n; i++) {
+      int a[6];
+      r.get(a, sizeof a);
+      const int dx = a[3] - a[0], dy = a[4] - a[1], dz = a[5] - a[2];
+      int axis = -1;
+      if (dx == 1 && dy == 0 && dz == 0) axis = 0;
+      if (dx == 0 && dy == 1 && dz == 0) axis = 1;
+      if (dx == 0 && dy == 0 && dz == 1) axis = 2;
+      if (axis < 0) throw Error(QJ_ERR_INVALID, "edge cache: a triplet is not a +x/+y/+z grid edge");
+      edges[4 * i] = a[0];
+      edges[4 * i + 1] = a[1];
+      edges[4 * i + 2] = a[2];
+      edges[4 * i + 3] = axis;
+    }
+    const int m = r.i32();
+    if (m < 0) throw Error(QJ_ERR_INVALID, "edge cache: negative flag count");
+    cells.resize(4 * (size_t)m);
+    for (int i = 0; i < m; i++) {
+      int a[4];
+      r.get(a, sizeof a);
+      cells[4 * i] = a[1];
+      cells[4 * i + 1] = a[2];
+      cells[4 * i + 2] = a[3];
+      cells[4 * i + 3] = a[0];
+    }
+  } catch (...) {
+    fclose(f);
+    throw;
+  }
+  fclose(f);
+  return true;
+}
+
+// writeEdgeCacheHuge (TRIANGLE_MESH.cpp:531-583)
+void write_edge_cache(const std::string& name, const std::vector<int32_t>& edges, const std::vector<int32_t>& cells) {
+  Writer w;
+  w.f = fopen(name.c_str(), "wb");
+  if (!w.f) throw Error(QJ_ERR_INVALID, "edge cache: cannot open " + name);
+  try {
+    const int n = (int)(edges.size() / 4), m = (int)(cells.size() / 4);
+    w.i32(n);
+    for (int i = 0; i < n; i++) {
+      const int32_t* e = &edges[4 * (size_t)i];
+      const int a[6] = {e[0], e[1], e[2], e[0] + (e[3] == 0), e[1] + (e[3] == 1), e[2] + (e[3] == 2)};
+      w.put(a, sizeof a);
+    }
+    w.i32(m);
+    for (int i = 0; i < m; i++) {
+      const int32_t* c = &cells[4 * (size_t)i];
+      const int a[4] = {c[3], c[0], c[1], c[2]};
+      w.put(a, sizeof a);
+    }
+  } catch (...) {
+    fclose(w.f);
+    throw;
+  }
+  fclose(w.f);
+}
+
+}  // namespace
+
+TRIANGLE_MESH::TRIANGLE_MESH(const VEC3F& center, const VEC3F& lengths, const VEC3I& res, const POLYNOMIAL_4D& top,
+                             const POLYNOMIAL_4D& bottom, Real expScaling, int maxIterations, Real slice,
+                             Real isosurface, const QUATERNION& rotation, const std::string& cacheFilename,
+                             qj_precision precision, int chunkSlices) {
+  qj_workspace* ws = shared_workspace();
+  const qj_grid g = mesher_grid(center, lengths, res, rotation);
   const qj_poly t = top.view(), b = bottom.view();
   qj_rational* f = nullptr;
   check(qj_rational_create(&t, &b, &f), "qj_rational_create");
   // the mesher's map: escape 2000 and the black-hole stop (TRIANGLE_MESH.cpp:97, 1789-1791)
   const qj_map_params p{maxIterations, 2000.0, expScaling, slice, QJ_MAP_BLACK_HOLE};
   int64_t nv = 0, nt = 0;
-  const qj_status st = qj_nlmc_run(ws, &g, f, &p, isosurface, precision, 0.05, chunkSlices, &nv, &nt, nullptr);
+  std::vector<int32_t> edges, cells;
+  qj_status st;
+  try {
+    if (!cacheFilename.empty() && read_edge_cache(cacheFilename, edges, cells)) {
+      st = qj_nlmc_resume(ws, &g, f, &p, isosurface, (int64_t)(edges.size() / 4), edges.data(),
+                          (int64_t)(cells.size() / 4), cells.data(), &nv, &nt, nullptr);
+    } else {
+      st = qj_nlmc_run(ws, &g, f, &p, isosurface, precision, 0.05, chunkSlices, &nv, &nt, nullptr);
+      if (st == QJ_OK && !cacheFilename.empty()) {
+        int64_t ne = 0, nc = 0;
+        check(qj_nlmc_cache(ws, &ne, nullptr, &nc, nullptr, nullptr), "qj_nlmc_cache");
+        edges.resize(4 * (size_t)ne);
+        cells.resize(4 * (size_t)nc);
+        check(qj_nlmc_cache(ws, nullptr, edges.data(), nullptr, cells.data(), nullptr), "qj_nlmc_cache");
+        write_edge_cache(cacheFilename, edges, cells);
+      }
+    }
+  } catch (...) {
+    qj_rational_destroy(f);
+    throw;
+  }
   qj_rational_destroy(f);
-  check(st, "qj_nlmc_run");
+  check(st, "non-linear mesher");
+  fetch_mesh(ws, nv, nt, _vertices, _triangles);
+}
+
+TRIANGLE_MESH::TRIANGLE_MESH(const VEC3F& center, const VEC3F& lengths, const VEC3I& res, const POLYNOMIAL_4D& top,
+                             const POLYNOMIAL_4D& bottom, Real expScaling, int maxIterations, Real slice,
+                             Real isosurface, const QUATERNION& rotation) {
+  qj_workspace* ws = shared_workspace();
+  const qj_grid g = mesher_grid(center, lengths, res, rotation);
+  const qj_poly t = top.view(), b = bottom.view();
+  qj_rational* f = nullptr;
+  check(qj_rational_create(&t, &b, &f), "qj_rational_create");
+  const qj_map_params p{maxIterations, 2000.0, expScaling, slice, QJ_MAP_BLACK_HOLE | QJ_MAP_LAGRANGE};
+  int64_t nv = 0, nt = 0;
+  const qj_status st = qj_nlmc_run(ws, &g, f, &p, isosurface, QJ_FP64, 0.05, 64, &nv, &nt, nullptr);
+  qj_rational_destroy(f);
+  check(st, "Lagrange mesher");
   fetch_mesh(ws, nv, nt, _vertices, _triangles);
 }
 

@@ -82,6 +82,10 @@ qj_status make_map_args(const qj_grid* g, const qj_rational* f, const qj_map_par
     set_error("max_iter must be >= 0");
     return QJ_ERR_INVALID;
   }
+  if ((p->flags & QJ_MAP_LAGRANGE) && (p->flags & (QJ_MAP_SKIP_ORIGIN | QJ_MAP_ADAPTIVE))) {
+    set_error("QJ_MAP_LAGRANGE does not combine with QJ_MAP_SKIP_ORIGIN or QJ_MAP_ADAPTIVE");
+    return QJ_ERR_INVALID;
+  }
   a->max_iter = p->max_iter;
   a->zoff = 0;
   a->gen = 0;
@@ -374,6 +378,10 @@ qj_status qj_field_eval(qj_workspace* ws, const qj_grid* grid, const qj_rational
     set_error("qj_field_eval: unknown precision %d", (int)precision);
     return QJ_ERR_INVALID;
   }
+  if (params && (params->flags & QJ_MAP_LAGRANGE) && precision != QJ_FP64) {
+    set_error("qj_field_eval: QJ_MAP_LAGRANGE needs QJ_FP64");
+    return QJ_ERR_INVALID;
+  }
   MapArgs a;
   qj_status st = make_map_args(grid, f, params, &a);
   if (st) return st;
@@ -389,6 +397,10 @@ qj_status qj_points_eval(qj_workspace* ws, const double* d_points, int64_t n, co
     return QJ_ERR_INVALID;
   }
   if (precision != QJ_FP64 && precision != QJ_FP32_BAND) return QJ_ERR_INVALID;
+  if (params && (params->flags & QJ_MAP_LAGRANGE) && precision != QJ_FP64) {
+    set_error("qj_points_eval: QJ_MAP_LAGRANGE needs QJ_FP64");
+    return QJ_ERR_INVALID;
+  }
   qj_grid g{{1, 1, 1}, {0, 0, 0}, {1, 1, 1}, {1, 0, 0, 0}};
   MapArgs a;
   qj_status st = make_map_args(&g, f, params, &a);
@@ -441,6 +453,34 @@ qj_status qj_mc_device_ptrs(qj_workspace* ws, const double** d_vertices, const i
   return mc_ptrs(ws, d_vertices, d_triangles);
 }
 
+qj_status qj_nlmc_cache(qj_workspace* ws, int64_t* n_edges, int32_t* edges, int64_t* n_cells, int32_t* cells,
+                        void* stream) {
+  if (!ws) {
+    set_error("qj_nlmc_cache: null workspace");
+    return QJ_ERR_INVALID;
+  }
+  std::lock_guard<std::mutex> lock(ws->mu);
+  return nlmc_cache(ws, n_edges, edges, n_cells, cells, (hipStream_t)stream);
+}
+
+qj_status qj_nlmc_resume(qj_workspace* ws, const qj_grid* grid, const qj_rational* f, const qj_map_params* params,
+                         double iso, int64_t n_edges, const int32_t* edges, int64_t n_cells, const int32_t* cells,
+                         int64_t* n_vertices, int64_t* n_triangles, void* stream) {
+  if (!ws || !grid || !f || !params || !n_vertices || !n_triangles) {
+    set_error("qj_nlmc_resume: null argument");
+    return QJ_ERR_INVALID;
+  }
+  if (params->flags & (QJ_MAP_SKIP_ORIGIN | QJ_MAP_ADAPTIVE)) {
+    set_error("qj_nlmc_resume: the mesher's map has no origin skip or adaptive re-pass");
+    return QJ_ERR_INVALID;
+  }
+  MapArgs a;
+  qj_status st = make_map_args(grid, f, params, &a);
+  if (st) return st;
+  std::lock_guard<std::mutex> lock(ws->mu);
+  return nlmc_resume(ws, a, f, iso, n_edges, edges, n_cells, cells, n_vertices, n_triangles, (hipStream_t)stream);
+}
+
 qj_status qj_ap_score_gradients(qj_workspace* ws, const qj_grid* grid, const qj_rational* f,
                                 const qj_map_params* params, const qj_grid* wgrid, const double* d_weight,
                                 double tanh_alpha, double tanh_threshold, double* d_field, double* grad_top,
@@ -451,6 +491,10 @@ qj_status qj_ap_score_gradients(qj_workspace* ws, const qj_grid* grid, const qj_
   }
   if (!params || params->max_iter != 1) {
     set_error("qj_ap_score_gradients: the AP gradients are defined for max_iter == 1 (toothOptimizer.cpp:402)");
+    return QJ_ERR_INVALID;
+  }
+  if (params->flags & QJ_MAP_LAGRANGE) {
+    set_error("qj_ap_score_gradients: the AP gradients are of the factored order; QJ_MAP_LAGRANGE is not supported");
     return QJ_ERR_INVALID;
   }
   MapArgs a;
@@ -471,6 +515,10 @@ qj_status qj_points_adjoint(qj_workspace* ws, const double* d_points, int64_t n,
     return QJ_ERR_INVALID;
   }
   if (precision != QJ_FP64 && precision != QJ_FP32_BAND) return QJ_ERR_INVALID;
+  if (params && (params->flags & QJ_MAP_LAGRANGE)) {
+    set_error("qj_points_adjoint: the adjoint is of the factored order; QJ_MAP_LAGRANGE is not supported");
+    return QJ_ERR_INVALID;
+  }
   qj_grid g{{1, 1, 1}, {0, 0, 0}, {1, 1, 1}, {1, 0, 0, 0}};
   MapArgs a;
   qj_status st = make_map_args(&g, f, params, &a);

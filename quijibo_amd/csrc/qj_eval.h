@@ -71,6 +71,24 @@ struct Res64 {
   bool overflow;
 };
 
+// RATIONAL_4D::evaluateLagrange (RATIONAL_4D.cpp:27-88): with bottom roots,
+// r = 1, then r = (r t_i) / b_i for the paired roots, then the leftover top
+// terms multiplied or bottom terms divided; the factored top product otherwise.
+template <int kPow = kPolar>
+__device__ __forceinline__ q64 lagrange64(const RootRec64* __restrict__ roots, int nt, int nb, const q64& q) {
+  if (nb == 0) return poly64<kPow>(roots, nt, q);
+  const RootRec64* bot = roots + nt;
+  q64 r{1.0, 0.0, 0.0, 0.0};
+  const int same = nt < nb ? nt : nb;
+  for (int i = 0; i < same; i++) {
+    r = qmul(r, term64<kPow>(roots[i], q));
+    r = qdiv(r, term64<kPow>(bot[i], q));
+  }
+  for (int i = same; i < nt; i++) r = qmul(r, term64<kPow>(roots[i], q));
+  for (int i = same; i < nb; i++) r = qdiv(r, term64<kPow>(bot[i], q));
+  return r;
+}
+
 // One voxel of computeLogScaledPowerRationalMap (OPTIMIZE_3D.cpp:1317-1362) /
 // computeRationalMapAP's first pass (:1423-1475). Overflow detection replaces
 // fetestexcept(FE_OVERFLOW): any non-finite T, B, |B|^2, quotient or |q|.
@@ -85,6 +103,14 @@ __device__ __forceinline__ Res64 eval64(const MapArgs& a, const RootRec64* __res
   const bool adaptive = (a.flags & QJ_MAP_ADAPTIVE) != 0;
   int it = 0;
   while (magnitude < a.escape && it < a.max_iter) {
+    if (a.flags & QJ_MAP_LAGRANGE) {
+      q = qscale(lagrange64<kPow>(roots, a.nt, a.nb, q), a.exp_scaling);
+      magnitude = qmag(q);
+      it++;
+      r.iters++;
+      if ((a.flags & QJ_MAP_BLACK_HOLE) && magnitude < kBlackHole) it = a.max_iter;
+      continue;
+    }
     const q64 T = poly64<kPow>(roots, a.nt, q);
     if (a.nb > 0) {
       if (has_nan(T) && !adaptive) {

@@ -148,6 +148,11 @@ qj_status launch_batch(const ChunkDesc* d_desc, int nchunks, int chunk_voxels, q
 qj_status mc_run(qj_workspace* ws, const double* d_field, const qj_grid* g, double iso, int64_t* nv,
                  int64_t* nt, hipStream_t s);
 qj_status mc_copy(qj_workspace* ws, double* v, int32_t* t, hipStream_t s);
+qj_status nlmc_cache(qj_workspace* ws, int64_t* n_edges, int32_t* edges, int64_t* n_cells, int32_t* cells,
+                     hipStream_t s);
+qj_status nlmc_resume(qj_workspace* ws, const MapArgs& a, const qj_rational* f, double iso, int64_t ne,
+                      const int32_t* h_edges, int64_t nc, const int32_t* h_cells, int64_t* nv_out, int64_t* nt_out,
+                      hipStream_t s);
 qj_status nlmc_run(qj_workspace* ws, const MapArgs& a, const qj_rational* f, double iso, qj_precision prec,
                    double band, int chunk, int64_t* nv, int64_t* nt, int64_t* n_bad, hipStream_t s);
 qj_status mc_ptrs(qj_workspace* ws, const double** v, const int32_t** t);

@@ -16,6 +16,8 @@
 //                   cell then table order, degenerate ones (edge < 1e-7) dropped
 #include <math.h>
 
+#include <vector>
+
 #include "qj_eval.h"
 #include "qj_geom.h"
 #include "qj_internal.h"
@@ -82,6 +84,12 @@ struct McState {
   uint32_t* slice_off = nullptr;
   int64_t cap_slice = 0;
   unsigned int* d_bad = nullptr;
+  // edge cache of the last non-linear mesh (TRIANGLE_MESH.cpp:431-586): per
+  // vertex its edge (lower endpoint x, y, z, axis), per active cell (x, y, z, flag)
+  int32_t* c_edges = nullptr;
+  int32_t* c_cells = nullptr;
+  int64_t cap_ce = 0, cap_cc = 0, ncells = 0;
+  bool cache_valid = false;
 };
 
 __device__ __forceinline__ int cell_flag(const double* __restrict__ f, int64_t i, int64_t X, int64_t XY, double iso) {
@@ -290,6 +298,7 @@ qj_status mc_run(qj_workspace* ws, const double* d_field, const qj_grid* grid, d
   if (!ws->mc) ws->mc = new McState();
   McState& m = *(McState*)ws->mc;
   m.nv = m.nt = 0;
+  m.cache_valid = false;
   GridGeom g;
   make_geom(grid, &g);
   const int64_t N = (int64_t)g.res[0] * g.res[1] * g.res[2];
@@ -406,6 +415,8 @@ void mc_free(qj_workspace* ws) {
   hipFree(m->slab);
   hipFree(m->slice_off);
   hipFree(m->d_bad);
+  hipFree(m->c_edges);
+  hipFree(m->c_cells);
   delete m;
   ws->mc = nullptr;
 }
@@ -425,6 +436,26 @@ void mc_free(qj_workspace* ws) {
 
 __device__ __forceinline__ double nonlinear_value(const MapArgs& a, const RootRec64* __restrict__ roots,
                                                   const double* p, double iso) {
+  if (a.flags & QJ_MAP_LAGRANGE) {
+    // lagrangeNonlinearValue (TRIANGLE_MESH.cpp:1684-1730): an infinite |q|
+    // restores the previous iterate and stops. Before the first iteration the
+    // reference's previous magnitude is uninitialised; here it is |q0|.
+    q64 q{p[0], p[1], p[2], a.slice};
+    double magnitude = qmag(q), prev = magnitude;
+    int it = 0;
+    while (magnitude < a.escape && it < a.max_iter) {
+      q = qscale(lagrange64(roots, a.nt, a.nb, q), a.exp_scaling);
+      magnitude = qmag(q);
+      if (isinf(magnitude)) {
+        magnitude = prev;
+        break;
+      }
+      prev = magnitude;
+      it++;
+      if (magnitude < kBlackHole) it = a.max_iter;
+    }
+    return log(magnitude) - iso;
+  }
   const Res64 r = eval64(a, roots, q64{p[0], p[1], p[2], a.slice});
   return r.F - iso;
 }
@@ -478,14 +509,60 @@ __global__ __launch_bounds__(kMcBlock) void k_nl_edges(const uint8_t* __restrict
   cnt[j] = (bits & 1u) + ((bits >> 1) & 1u) + ((bits >> 2) & 1u);
 }
 
-// vertices of the owned points [p0, p0 + np) by bisection (computeNonlinearEdgeInterpolationsHuge, :2034-2123)
+// One vertex by bisection between grid points (x, y, z) and its +axis
+// neighbour (computeNonlinearEdgeInterpolationsHuge :2034-2123 with
+// midpointSearch :2232-2271). va: value at (x, y, z). Counts edges whose end
+// values do not straddle the surface in *bad (the reference exits there).
+__device__ __forceinline__ void place_vertex(const MapArgs& a, const RootRec64* __restrict__ roots, double iso,
+                                             int x, int y, int z, int axis, const double* pa, double va,
+                                             double* __restrict__ out, unsigned int* __restrict__ bad) {
+#pragma clang fp contract(off)
+  double pb[3];
+  cell_center(a, x + (axis == 0), y + (axis == 1), z + (axis == 2), pb);
+  const double vb = nonlinear_value(a, roots, pb, iso);
+  // first = lower endpoint; the positive end is the first unless firstValue < 0
+  double pp[3], pn[3], vp, vn;
+  if (va < 0) {
+    pp[0] = pb[0]; pp[1] = pb[1]; pp[2] = pb[2]; vp = vb;
+    pn[0] = pa[0]; pn[1] = pa[1]; pn[2] = pa[2]; vn = va;
+  } else {
+    pp[0] = pa[0]; pp[1] = pa[1]; pp[2] = pa[2]; vp = va;
+    pn[0] = pb[0]; pn[1] = pb[1]; pn[2] = pb[2]; vn = vb;
+  }
+  if (va * vb >= 0.0) atomicAdd(bad, 1u);  // the reference exits here (:2082-2097)
+  double m[3];
+  for (int rec = 0;; rec++) {
+    m[0] = pp[0] + pn[0];
+    m[1] = pp[1] + pn[1];
+    m[2] = pp[2] + pn[2];
+    m[0] *= 0.5;
+    m[1] *= 0.5;
+    m[2] *= 0.5;
+    if (vp * vn >= 0.0) {
+      if (rec > 0) atomicAdd(bad, 1u);
+      break;
+    }
+    if (rec >= 6) break;
+    const double vm = nonlinear_value(a, roots, m, iso);
+    if (fabs(vm) < 1e-8) break;
+    if (vm < 0) {
+      pn[0] = m[0]; pn[1] = m[1]; pn[2] = m[2]; vn = vm;
+    } else {
+      pp[0] = m[0]; pp[1] = m[1]; pp[2] = m[2]; vp = vm;
+    }
+  }
+  out[0] = m[0];
+  out[1] = m[1];
+  out[2] = m[2];
+}
+
+// vertices of the owned points [p0, p0 + np) by bisection, and their edges
 __global__ __launch_bounds__(kMcBlock) void k_nl_vertices(MapArgs a, const RootRec64* __restrict__ roots,
                                                           double iso, int64_t p0, int64_t np,
                                                           const uint8_t* __restrict__ emask,
                                                           const uint32_t* __restrict__ off, uint32_t base,
-                                                          double* __restrict__ verts,
+                                                          double* __restrict__ verts, int32_t* __restrict__ edges,
                                                           unsigned int* __restrict__ bad) {
-#pragma clang fp contract(off)
   const int64_t X = a.res[0], XY = X * a.res[1];
   const int64_t j = (int64_t)blockIdx.x * kMcBlock + threadIdx.x;
   if (j >= XY * np) return;
@@ -499,45 +576,27 @@ __global__ __launch_bounds__(kMcBlock) void k_nl_vertices(MapArgs a, const RootR
   uint32_t k = base + off[j];
   for (int axis = 0; axis < 3; axis++) {
     if (!(bits & (1u << axis))) continue;
-    double pb[3];
-    cell_center(a, x + (axis == 0), y + (axis == 1), z + (axis == 2), pb);
-    const double vb = nonlinear_value(a, roots, pb, iso);
-    // first = lower endpoint; the positive end is the first unless firstValue < 0
-    double pp[3], pn[3], vp, vn;
-    if (va < 0) {
-      pp[0] = pb[0]; pp[1] = pb[1]; pp[2] = pb[2]; vp = vb;
-      pn[0] = pa[0]; pn[1] = pa[1]; pn[2] = pa[2]; vn = va;
-    } else {
-      pp[0] = pa[0]; pp[1] = pa[1]; pp[2] = pa[2]; vp = va;
-      pn[0] = pb[0]; pn[1] = pb[1]; pn[2] = pb[2]; vn = vb;
-    }
-    if (va * vb >= 0.0) atomicAdd(bad, 1u);  // the reference exits here (:2082-2097)
-    double m[3];
-    for (int rec = 0;; rec++) {
-      m[0] = pp[0] + pn[0];
-      m[1] = pp[1] + pn[1];
-      m[2] = pp[2] + pn[2];
-      m[0] *= 0.5;
-      m[1] *= 0.5;
-      m[2] *= 0.5;
-      if (vp * vn >= 0.0) {
-        if (rec > 0) atomicAdd(bad, 1u);
-        break;
-      }
-      if (rec >= 6) break;
-      const double vm = nonlinear_value(a, roots, m, iso);
-      if (fabs(vm) < 1e-8) break;
-      if (vm < 0) {
-        pn[0] = m[0]; pn[1] = m[1]; pn[2] = m[2]; vn = vm;
-      } else {
-        pp[0] = m[0]; pp[1] = m[1]; pp[2] = m[2]; vp = vm;
-      }
-    }
-    verts[3 * (int64_t)k + 0] = m[0];
-    verts[3 * (int64_t)k + 1] = m[1];
-    verts[3 * (int64_t)k + 2] = m[2];
+    place_vertex(a, roots, iso, x, y, z, axis, pa, va, verts + 3 * (int64_t)k, bad);
+    edges[4 * (int64_t)k + 0] = x;
+    edges[4 * (int64_t)k + 1] = y;
+    edges[4 * (int64_t)k + 2] = z;
+    edges[4 * (int64_t)k + 3] = axis;
     k++;
   }
+}
+
+// (x, y, z, flag) of the chunk's active cells, appended in cell order
+__global__ __launch_bounds__(kMcBlock) void k_nl_cells(const uint32_t* __restrict__ active, int64_t na,
+                                                       const uint8_t* __restrict__ flags, int64_t fo, int64_t X,
+                                                       int64_t XY, int64_t z0, int32_t* __restrict__ cells) {
+  const int64_t i = (int64_t)blockIdx.x * kMcBlock + threadIdx.x;
+  if (i >= na) return;
+  const int64_t c = active[i];
+  const int64_t lz = c / XY, r = c - lz * XY, y = r / X, x = r - y * X;
+  cells[4 * i + 0] = (int32_t)x;
+  cells[4 * i + 1] = (int32_t)y;
+  cells[4 * i + 2] = (int32_t)(z0 + lz);
+  cells[4 * i + 3] = flags[fo + c];
 }
 
 // per-block active counts of cells in global slices [c0, c0 + nc) (slab-local flags from s0)
@@ -642,6 +701,8 @@ qj_status nlmc_run(qj_workspace* ws, const MapArgs& a0, const qj_rational* f, do
   const int64_t X = a0.res[0], Y = a0.res[1], Z = a0.res[2], XY = X * Y;
   *nv_out = *nt_out = 0;
   *n_bad = 0;
+  m.ncells = 0;
+  m.cache_valid = false;
   if (X < 2 || Y < 2 || Z < 2) return QJ_OK;
   if (chunk < 1) chunk = 64;
   if (XY * (int64_t)(chunk + 3) > (int64_t)UINT32_MAX) {
@@ -682,7 +743,8 @@ qj_status nlmc_run(qj_workspace* ws, const MapArgs& a0, const qj_rational* f, do
     MapArgs sa = a;
     sa.res[2] = (int32_t)ns;
     sa.zoff = (int32_t)s0;
-    if ((st = launch_field(ws, sa, f, prec, band, m.slab, s))) return st;
+    // the Lagrange order has no FP32 path: its slabs are FP64
+    if ((st = launch_field(ws, sa, f, (a.flags & QJ_MAP_LAGRANGE) ? QJ_FP64 : prec, band, m.slab, s))) return st;
     // 2. cases
     hipLaunchKernelGGL(k_nl_flags, dim3(nblocks(XY * ns)), dim3(kMcBlock), 0, s, m.slab, X, Y, Z, s0, ns, m.flags);
     QJ_HIP(hipGetLastError());
@@ -711,16 +773,19 @@ qj_status nlmc_run(qj_workspace* ws, const MapArgs& a0, const qj_rational* f, do
     if ((st = grow_keep(m.verts, m.cap_v, 3 * (int64_t)(nv + cnt_own + cnt_next), 3 * (int64_t)(nv + placed_next),
                         s)))
       return st;
+    if ((st = grow_keep(m.c_edges, m.cap_ce, 4 * (int64_t)(nv + cnt_own + cnt_next),
+                        4 * (int64_t)(nv + placed_next), s)))
+      return st;
     const int64_t skip = z0 > 0 ? 1 : 0;
     if (cnt_own > 0 && z1 - z0 > skip) {
       hipLaunchKernelGGL(k_nl_vertices, dim3(nblocks(XY * (z1 - z0 - skip))), dim3(kMcBlock), 0, s, a, f->d64, iso,
                          z0 + skip, z1 - z0 - skip, m.emask + skip * XY, m.cnt + skip * XY, (uint32_t)nv, m.verts,
-                         m.d_bad);
+                         m.c_edges, m.d_bad);
       QJ_HIP(hipGetLastError());
     }
     if (cnt_next > 0) {
       hipLaunchKernelGGL(k_nl_vertices, dim3(nblocks(XY)), dim3(kMcBlock), 0, s, a, f->d64, iso, z1, (int64_t)1,
-                         m.emask + nown, m.slice_off, (uint32_t)(nv + cnt_own), m.verts, m.d_bad);
+                         m.emask + nown, m.slice_off, (uint32_t)(nv + cnt_own), m.verts, m.c_edges, m.d_bad);
       QJ_HIP(hipGetLastError());
     }
     // 5. active cells of the chunk, in cell order
@@ -742,6 +807,11 @@ qj_status nlmc_run(qj_workspace* ws, const MapArgs& a0, const qj_rational* f, do
     }
     if (na > 0) {
       const int64_t fo = (z0 - s0) * XY;
+      if ((st = grow_keep(m.c_cells, m.cap_cc, 4 * (m.ncells + na), 4 * m.ncells, s))) return st;
+      hipLaunchKernelGGL(k_nl_cells, dim3(nblocks(na)), dim3(kMcBlock), 0, s, m.active, na, m.flags, fo, X, XY, z0,
+                         m.c_cells + 4 * m.ncells);
+      QJ_HIP(hipGetLastError());
+      m.ncells += na;
       hipLaunchKernelGGL(k_nl_tricount, dim3(nblocks(na)), dim3(kMcBlock), 0, s, m.active, na, m.flags, fo,
                          m.tri_off);
       QJ_HIP(hipGetLastError());
@@ -774,9 +844,226 @@ qj_status nlmc_run(qj_workspace* ws, const MapArgs& a0, const qj_rational* f, do
   QJ_HIP(hipStreamSynchronize(s));
   m.nv = (int64_t)nv;
   m.nt = (int64_t)nt;
+  m.cache_valid = true;
   *nv_out = m.nv;
   *nt_out = m.nt;
   *n_bad = bad;
+  if (bad) {
+    set_error("non-linear mesher: %u edges whose end values do not straddle the isosurface "
+              "(the reference exits at TRIANGLE_MESH.cpp:2082-2097)", bad);
+    return QJ_ERR_NUMERIC;
+  }
+  return QJ_OK;
+}
+
+// ---------------------------------------------------------------- edge cache
+
+qj_status nlmc_cache(qj_workspace* ws, int64_t* n_edges, int32_t* edges, int64_t* n_cells, int32_t* cells,
+                     hipStream_t s) {
+  McState* m = (McState*)ws->mc;
+  if (!m || !m->cache_valid) {
+    set_error("qj_nlmc_cache: no non-linear mesh in this workspace");
+    return QJ_ERR_INVALID;
+  }
+  if (n_edges) *n_edges = m->nv;
+  if (n_cells) *n_cells = m->ncells;
+  if (edges && m->nv) QJ_HIP(hipMemcpyAsync(edges, m->c_edges, sizeof(int32_t) * 4 * m->nv, hipMemcpyDefault, s));
+  if (cells && m->ncells)
+    QJ_HIP(hipMemcpyAsync(cells, m->c_cells, sizeof(int32_t) * 4 * m->ncells, hipMemcpyDefault, s));
+  QJ_HIP(hipStreamSynchronize(s));
+  return QJ_OK;
+}
+
+__global__ __launch_bounds__(kMcBlock) void k_cache_vertices(MapArgs a, const RootRec64* __restrict__ roots,
+                                                             double iso, const int32_t* __restrict__ edges, int64_t ne,
+                                                             double* __restrict__ verts,
+                                                             unsigned int* __restrict__ bad) {
+  const int64_t i = (int64_t)blockIdx.x * kMcBlock + threadIdx.x;
+  if (i >= ne) return;
+  const int x = edges[4 * i], y = edges[4 * i + 1], z = edges[4 * i + 2], axis = edges[4 * i + 3];
+  double pa[3];
+  cell_center(a, x, y, z, pa);
+  const double va = nonlinear_value(a, roots, pa, iso);
+  place_vertex(a, roots, iso, x, y, z, axis, pa, va, verts + 3 * i, bad);
+}
+
+__global__ __launch_bounds__(kMcBlock) void k_cache_tricount(const int32_t* __restrict__ cells, int64_t nc,
+                                                             uint32_t* __restrict__ tcount) {
+  const int64_t i = (int64_t)blockIdx.x * kMcBlock + threadIdx.x;
+  if (i < nc) tcount[i] = kMc.ntri[cells[4 * i + 3]];
+}
+
+// position of key in the ascending keys, or -1
+__device__ __forceinline__ int64_t find_key(const int64_t* __restrict__ keys, int64_t n, int64_t key) {
+  int64_t lo = 0, hi = n;
+  while (lo < hi) {
+    const int64_t mid = lo + ((hi - lo) >> 1);
+    if (keys[mid] < key) lo = mid + 1;
+    else hi = mid;
+  }
+  return (lo < n && keys[lo] == key) ? lo : -1;
+}
+
+__global__ __launch_bounds__(kMcBlock) void k_cache_triangles(
+    const int32_t* __restrict__ cells, int64_t nc, int64_t X, int64_t XY, const int64_t* __restrict__ keys,
+    int64_t ne, const double* __restrict__ verts, const uint32_t* __restrict__ toff, int32_t* __restrict__ tris_pre,
+    uint32_t* __restrict__ keep, unsigned int* __restrict__ missing) {
+  const int64_t i = (int64_t)blockIdx.x * kMcBlock + threadIdx.x;
+  if (i >= nc) return;
+  const int64_t x = cells[4 * i], y = cells[4 * i + 1], z = cells[4 * i + 2];
+  const int flag = cells[4 * i + 3];
+  const int n = kMc.ntri[flag];
+  uint32_t o = toff[i];
+  for (int t = 0; t < n; t++) {
+    int32_t v[3];
+    bool ok = true;
+    for (int q = 0; q < 3; q++) {
+      int dx, dy, dz, axis;
+      edge_geom(kMc.tri[flag][3 * t + q], dx, dy, dz, axis);
+      const int64_t key = ((z + dz) * XY + (y + dy) * X + (x + dx)) * 3 + axis;
+      const int64_t id = find_key(keys, ne, key);
+      if (id < 0) ok = false;
+      v[q] = id < 0 ? 0 : (int32_t)id;
+    }
+    if (!ok) atomicAdd(missing, 1u);
+    const double eps = 1e-7;
+    const double* p0 = verts + 3 * (int64_t)v[0];
+    const double* p1 = verts + 3 * (int64_t)v[1];
+    const double* p2 = verts + 3 * (int64_t)v[2];
+    const bool degenerate = edge_len(p0, p1) < eps || edge_len(p0, p2) < eps || edge_len(p1, p2) < eps;
+    tris_pre[3 * (int64_t)o + 0] = v[0];
+    tris_pre[3 * (int64_t)o + 1] = v[1];
+    tris_pre[3 * (int64_t)o + 2] = v[2];
+    keep[o] = (degenerate || !ok) ? 0u : 1u;
+    o++;
+  }
+}
+
+qj_status nlmc_resume(qj_workspace* ws, const MapArgs& a, const qj_rational* f, double iso, int64_t ne,
+                      const int32_t* h_edges, int64_t nc, const int32_t* h_cells, int64_t* nv_out, int64_t* nt_out,
+                      hipStream_t s) {
+  if (!ws->mc) ws->mc = new McState();
+  McState& m = *(McState*)ws->mc;
+  m.nv = m.nt = 0;
+  m.cache_valid = false;
+  *nv_out = *nt_out = 0;
+  const int64_t X = a.res[0], Y = a.res[1], Z = a.res[2], XY = X * Y;
+  if (ne < 0 || nc < 0 || (ne && !h_edges) || (nc && !h_cells) || ne > (int64_t)INT32_MAX ||
+      nc > (int64_t)UINT32_MAX / 5) {
+    set_error("qj_nlmc_resume: bad edge or cell arrays");
+    return QJ_ERR_INVALID;
+  }
+  // host checks: every edge inside the grid with its far end, keys strictly
+  // ascending (the cache's vertex order); every cell inside the grid
+  std::vector<int64_t> keys((size_t)ne);
+  for (int64_t i = 0; i < ne; i++) {
+    const int32_t x = h_edges[4 * i], y = h_edges[4 * i + 1], z = h_edges[4 * i + 2], ax = h_edges[4 * i + 3];
+    if (x < 0 || y < 0 || z < 0 || ax < 0 || ax > 2 || x + (ax == 0) >= X || y + (ax == 1) >= Y ||
+        z + (ax == 2) >= Z) {
+      set_error("qj_nlmc_resume: edge %lld lies outside the grid", (long long)i);
+      return QJ_ERR_INVALID;
+    }
+    keys[i] = ((int64_t)z * XY + (int64_t)y * X + x) * 3 + ax;
+    if (i && keys[i] <= keys[i - 1]) {
+      set_error("qj_nlmc_resume: edges must be in vertex order (lower endpoint, then axis)");
+      return QJ_ERR_INVALID;
+    }
+  }
+  for (int64_t i = 0; i < nc; i++) {
+    const int32_t x = h_cells[4 * i], y = h_cells[4 * i + 1], z = h_cells[4 * i + 2], fl = h_cells[4 * i + 3];
+    if (x < 0 || y < 0 || z < 0 || x >= X - 1 || y >= Y - 1 || z >= Z - 1 || fl < 1 || fl > 254) {
+      set_error("qj_nlmc_resume: cell %lld lies outside the grid or has flag %d", (long long)i, (int)fl);
+      return QJ_ERR_INVALID;
+    }
+  }
+  qj_status st;
+  if ((st = grow_keep(m.verts, m.cap_v, 3 * ne, 0, s)) || (st = grow_keep(m.c_edges, m.cap_ce, 4 * ne, 0, s)) ||
+      (st = grow_keep(m.c_cells, m.cap_cc, 4 * nc, 0, s)) || (st = grow(m.tri_off, m.cap_toff, nc, s)))
+    return st;
+  if (!m.totals) QJ_HIP(hipMalloc(&m.totals, 4 * sizeof(uint32_t)));
+  if (!m.d_bad) QJ_HIP(hipMalloc(&m.d_bad, sizeof(unsigned int)));
+  int64_t* d_keys = nullptr;
+  unsigned int* d_missing = nullptr;
+  QJ_HIP(hipMalloc(&d_keys, sizeof(int64_t) * (size_t)(ne > 0 ? ne : 1)));
+  QJ_HIP(hipMalloc(&d_missing, sizeof(unsigned int)));
+  struct Free {
+    void* a;
+    void* b;
+    ~Free() {
+      (void)hipFree(a);
+      (void)hipFree(b);
+    }
+  } guard{d_keys, d_missing};
+  QJ_HIP(hipMemsetAsync(m.d_bad, 0, sizeof(unsigned int), s));
+  QJ_HIP(hipMemsetAsync(d_missing, 0, sizeof(unsigned int), s));
+  if (ne) {
+    QJ_HIP(hipMemcpyAsync(d_keys, keys.data(), sizeof(int64_t) * ne, hipMemcpyHostToDevice, s));
+    QJ_HIP(hipMemcpyAsync(m.c_edges, h_edges, sizeof(int32_t) * 4 * ne, hipMemcpyHostToDevice, s));
+    hipLaunchKernelGGL(k_cache_vertices, dim3(nblocks(ne)), dim3(kMcBlock), 0, s, a, f->d64, iso, m.c_edges, ne,
+                       m.verts, m.d_bad);
+    QJ_HIP(hipGetLastError());
+  }
+  int64_t nt = 0;
+  if (nc) {
+    QJ_HIP(hipMemcpyAsync(m.c_cells, h_cells, sizeof(int32_t) * 4 * nc, hipMemcpyHostToDevice, s));
+    hipLaunchKernelGGL(k_cache_tricount, dim3(nblocks(nc)), dim3(kMcBlock), 0, s, m.c_cells, nc, m.tri_off);
+    QJ_HIP(hipGetLastError());
+    const size_t need_tmp = scan_tmp_bytes(nc > 4096 ? nc : 4096);
+    if (need_tmp > m.tmp_bytes) {
+      if (m.tmp) {
+        QJ_HIP(hipStreamSynchronize(s));
+        QJ_HIP(hipFree(m.tmp));
+      }
+      m.tmp = nullptr;
+      m.tmp_bytes = 0;
+      QJ_HIP(hipMalloc(&m.tmp, need_tmp));
+      m.tmp_bytes = need_tmp;
+    }
+    if ((st = scan_u32(m.tri_off, m.tri_off, nc, m.totals + 2, m.tmp, m.tmp_bytes, s))) return st;
+    uint32_t h = 0;
+    QJ_HIP(hipMemcpyAsync(&h, m.totals + 2, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+    QJ_HIP(hipStreamSynchronize(s));
+    const int64_t ntp = h;
+    if ((st = grow(m.tris_pre, m.cap_tp, 3 * ntp, s)) || (st = grow(m.keep, m.cap_keep, ntp, s)) ||
+        (st = grow(m.tris, m.cap_t, 3 * ntp, s)))
+      return st;
+    if (scan_tmp_bytes(ntp) > m.tmp_bytes) {
+      QJ_HIP(hipStreamSynchronize(s));
+      QJ_HIP(hipFree(m.tmp));
+      m.tmp = nullptr;
+      m.tmp_bytes = 0;
+      QJ_HIP(hipMalloc(&m.tmp, scan_tmp_bytes(ntp)));
+      m.tmp_bytes = scan_tmp_bytes(ntp);
+    }
+    if (ntp > 0) {
+      hipLaunchKernelGGL(k_cache_triangles, dim3(nblocks(nc)), dim3(kMcBlock), 0, s, m.c_cells, nc, X, XY, d_keys,
+                         ne, m.verts, m.tri_off, m.tris_pre, m.keep, d_missing);
+      QJ_HIP(hipGetLastError());
+      if ((st = scan_u32(m.keep, m.keep, ntp, m.totals + 3, m.tmp, m.tmp_bytes, s))) return st;
+      QJ_HIP(hipMemcpyAsync(&h, m.totals + 3, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+      QJ_HIP(hipStreamSynchronize(s));
+      nt = h;
+      if (nt > 0) {
+        hipLaunchKernelGGL(k_mc_squeeze, dim3(nblocks(ntp)), dim3(kMcBlock), 0, s, m.tris_pre, m.keep, ntp,
+                           m.totals + 3, m.tris);
+        QJ_HIP(hipGetLastError());
+      }
+    }
+  }
+  unsigned int bad = 0, missing = 0;
+  QJ_HIP(hipMemcpyAsync(&bad, m.d_bad, sizeof(bad), hipMemcpyDeviceToHost, s));
+  QJ_HIP(hipMemcpyAsync(&missing, d_missing, sizeof(missing), hipMemcpyDeviceToHost, s));
+  QJ_HIP(hipStreamSynchronize(s));
+  m.nv = ne;
+  m.nt = nt;
+  m.ncells = nc;
+  m.cache_valid = true;
+  *nv_out = m.nv;
+  *nt_out = m.nt;
+  if (missing) {
+    set_error("qj_nlmc_resume: %u triangles reference edges the cache does not list", missing);
+    return QJ_ERR_INVALID;
+  }
   if (bad) {
     set_error("non-linear mesher: %u edges whose end values do not straddle the isosurface "
               "(the reference exits at TRIANGLE_MESH.cpp:2082-2097)", bad);

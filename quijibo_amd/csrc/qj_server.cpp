@@ -455,6 +455,10 @@ qj_status qj_server_submit(qj_server* s, const qj_request* req, uint64_t* id) {
     set_error("qj_server_submit: QJ_MAP_ADAPTIVE is not served by the request loop");
     return QJ_ERR_INVALID;
   }
+  if ((req->params.flags & QJ_MAP_LAGRANGE) && s->cfg.precision != QJ_FP64) {
+    set_error("qj_server_submit: QJ_MAP_LAGRANGE needs a QJ_FP64 server");
+    return QJ_ERR_INVALID;
+  }
   auto r = std::make_unique<Req>();
   r->grid = req->grid;
   r->params = req->params;

@@ -209,3 +209,42 @@ def read_obj(path):
             elif line.startswith("f "):
                 t.append([int(x.split("/")[0]) - 1 for x in line.split()[1:4]])
     return np.array(v, dtype=np.float64).reshape(-1, 3), np.array(t, dtype=np.int32).reshape(-1, 3)
+
+
+def edges_to_triplets(edges):
+    """(x, y, z, axis) rows -> the cache's (first xyz, second xyz) grid-point pairs."""
+    e = np.asarray(edges, dtype=np.int32).reshape(-1, 4)
+    first = e[:, :3]
+    second = first + np.eye(3, dtype=np.int32)[e[:, 3]] if len(e) else first.copy()
+    return first, second
+
+
+def write_edge_cache(path, edges, cells):
+    """TRIANGLE_MESH::writeEdgeCacheHuge (TRIANGLE_MESH.cpp:531-583): int count, then per
+    vertex the two grid points of its edge as 3 + 3 ints (map order); int count, then per
+    active cell int flag and the cell's 3 ints."""
+    first, second = edges_to_triplets(edges)
+    c = np.asarray(cells, dtype=np.int32).reshape(-1, 4)
+    out = struct.pack("<i", len(first)) + np.ascontiguousarray(np.c_[first, second], dtype="<i4").tobytes()
+    out += struct.pack("<i", len(c)) + np.ascontiguousarray(c[:, [3, 0, 1, 2]], dtype="<i4").tobytes()
+    with open(path, "wb") as fh:
+        fh.write(out)
+
+
+def read_edge_cache(path):
+    """readEdgeCacheHuge (TRIANGLE_MESH.cpp:431-481) -> (edges n x 4 [x, y, z, axis], cells m x 4 [x, y, z, flag])."""
+    with open(path, "rb") as fh:
+        r = _In(fh.read())
+    n = r.i32()
+    if n < 0:
+        raise ValueError("negative triplet count")
+    t = np.frombuffer(r.take(24 * n), dtype="<i4").reshape(-1, 6).astype(np.int32)
+    d = t[:, 3:] - t[:, :3]
+    if len(d) and not np.all((d.sum(axis=1) == 1) & (d.min(axis=1) == 0)):
+        raise ValueError("a triplet's points are not grid neighbours along +x, +y or +z")
+    edges = np.c_[t[:, :3], np.argmax(d, axis=1) if len(d) else np.zeros(0, np.int32)].astype(np.int32)
+    m = r.i32()
+    if m < 0:
+        raise ValueError("negative flag count")
+    f = np.frombuffer(r.take(16 * m), dtype="<i4").reshape(-1, 4).astype(np.int32)
+    return edges.reshape(-1, 4), f[:, [1, 2, 3, 0]].reshape(-1, 4)

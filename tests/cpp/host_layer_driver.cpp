@@ -10,7 +10,9 @@
 //   host_layer_driver map <in.o3d> <maxIterations> <precision 0|1> <outdir>
 //       OPTIMIZE_3D::read, computeLogScaledPowerRationalMap into the fractal
 //       -> <outdir>/field.bin; TRIANGLE_MESH(fractal) -> linear.obj;
-//       TRIANGLE_MESH(center, lengths, res, top, bottom, ...) -> nonlinear.obj.
+//       TRIANGLE_MESH(center, lengths, res, top, bottom, ..., cache) -> nonlinear.obj
+//       (writes <outdir>/mesh.cache), again from the cache -> resumed.obj, and
+//       the Lagrange mesher -> lagrange.obj.
 //   host_layer_driver ap <in.o3d> <weight.field> <alpha> <threshold>
 //       one-iteration AP score and gradients printed as JSON.
 //   host_layer_driver blmvm rosenbrock|boxquad
@@ -121,13 +123,21 @@ static int run_map(const std::string& in, int maxIter, int precision, const std:
   fractal.write(out + "/field.bin");
   TRIANGLE_MESH linear(fractal);
   if (!linear.writeOBJ(out + "/linear.obj")) return fail("writeOBJ linear");
-  const int res[3] = {opt.xRes(), opt.yRes(), opt.zRes()};
+  const VEC3I res(opt.xRes(), opt.yRes(), opt.zRes());
+  const std::string cache = out + "/mesh.cache";
+  // first run writes the edge cache, the second rebuilds the mesh from it
   TRIANGLE_MESH nonlinear(opt.center(), opt.lengths(), res, opt.top(), opt.bottom(), opt.expScaling(), maxIter,
-                          opt.quaternionSlice(), 0.0, QUATERNION(1, 0, 0, 0),
+                          opt.quaternionSlice(), 0.0, QUATERNION(1, 0, 0, 0), cache,
                           precision ? QJ_FP32_BAND : QJ_FP64, 7);
   if (!nonlinear.writeOBJ(out + "/nonlinear.obj")) return fail("writeOBJ nonlinear");
-  printf("map ok %zu %zu %zu %zu\n", linear.vertices().size(), linear.triangles().size(),
-         nonlinear.vertices().size(), nonlinear.triangles().size());
+  TRIANGLE_MESH resumed(opt.center(), opt.lengths(), res, opt.top(), opt.bottom(), opt.expScaling(), maxIter,
+                        opt.quaternionSlice(), 0.0, QUATERNION(1, 0, 0, 0), cache);
+  if (!resumed.writeOBJ(out + "/resumed.obj")) return fail("writeOBJ resumed");
+  TRIANGLE_MESH lagrange(opt.center(), opt.lengths(), res, opt.top(), opt.bottom(), opt.expScaling(), maxIter,
+                         opt.quaternionSlice(), 0.0, QUATERNION(1, 0, 0, 0));
+  if (!lagrange.writeOBJ(out + "/lagrange.obj")) return fail("writeOBJ lagrange");
+  printf("map ok %zu %zu %zu %zu %zu\n", linear.vertices().size(), linear.triangles().size(),
+         nonlinear.vertices().size(), nonlinear.triangles().size(), lagrange.triangles().size());
   return 0;
 }
 

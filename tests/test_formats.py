@@ -115,3 +115,21 @@ def test_cpp_layer_reads_and_rewrites_python_o3d(tmp_path):
     r = subprocess.run([DRIVER, "copy", str(tmp_path / "c.o3d"), str(tmp_path / "d.o3d")], capture_output=True,
                        text=True, timeout=60)
     assert r.returncode == 2 and "too short" in r.stderr
+
+
+def test_edge_cache_layout(tmp_path):
+    edges = np.array([[1, 2, 3, 0], [1, 2, 3, 2], [4, 0, 3, 1]], dtype=np.int32)
+    cells = np.array([[1, 2, 3, 17], [4, 0, 2, 254]], dtype=np.int32)
+    FM.write_edge_cache(tmp_path / "c.cache", edges, cells)
+    # writeEdgeCacheHuge (TRIANGLE_MESH.cpp:531-583): count, (first xyz, second xyz)...,
+    # count, (flag, xyz)...
+    want = struct.pack("<i", 3) + struct.pack("<6i", 1, 2, 3, 2, 2, 3) + struct.pack("<6i", 1, 2, 3, 1, 2, 4)
+    want += struct.pack("<6i", 4, 0, 3, 4, 1, 3)
+    want += struct.pack("<i", 2) + struct.pack("<4i", 17, 1, 2, 3) + struct.pack("<4i", 254, 4, 0, 2)
+    assert (tmp_path / "c.cache").read_bytes() == want
+    e, c = FM.read_edge_cache(tmp_path / "c.cache")
+    assert np.array_equal(e, edges) and np.array_equal(c, cells)
+    bad = struct.pack("<i", 1) + struct.pack("<6i", 0, 0, 0, 1, 1, 0) + struct.pack("<i", 0)
+    (tmp_path / "b.cache").write_bytes(bad)
+    with pytest.raises(ValueError):
+        FM.read_edge_cache(tmp_path / "b.cache")

@@ -64,6 +64,17 @@ def test_map_and_meshes(gpu, tmp_path, cfg):
     assert bad == 0 and np.array_equal(nt, pt)
     d = np.max(np.abs(nv - pv), axis=1)
     assert (d > 1e-15).mean() <= 0.005 and np.all(d <= 3.0 / min(res) / 2 ** 6 * 1.0001)
+    # the edge cache the constructor wrote, and the mesh rebuilt from it
+    edges, cells = FM.read_edge_cache(tmp_path / "mesh.cache")
+    assert len(edges) == len(nv) and len(cells) > 0
+    assert (tmp_path / "resumed.obj").read_bytes() == (tmp_path / "nonlinear.obj").read_bytes()
+    # the Lagrange constructor
+    lv, lt = FM.read_obj(tmp_path / "lagrange.obj")
+    lp = O.make_params(w.max_iter, 2000.0, w.exp_scaling, w.slice, O.BLACK_HOLE | O.LAGRANGE)
+    qv, qt, bad = O.nonlinear_mc(g, t, b, lp)
+    assert bad == 0 and np.array_equal(lt, qt)
+    d = np.max(np.abs(lv - qv), axis=1)
+    assert (d > 1e-15).mean() <= 0.005 and np.all(d <= 3.0 / min(res) / 2 ** 6 * 1.0001)
 
 
 def test_fp32_band_map_keeps_the_topology(gpu, tmp_path):

@@ -11,7 +11,7 @@ import torch
 
 import oracle as O
 from quijibo_amd import api, workloads as W
-from quijibo_amd._lib import FP32_BAND, FP64, MAP_BLACK_HOLE
+from quijibo_amd._lib import FP32_BAND, FP64, MAP_BLACK_HOLE, MAP_LAGRANGE
 
 pytestmark = pytest.mark.gpu
 
@@ -51,3 +51,70 @@ def test_chunking_and_precision_do_not_change_the_mesh(gpu):
     for prec, chunk in ((FP64, 1), (FP64, 5), (FP32_BAND, 7), (FP32_BAND, 64)):
         v, t = _run(w, res, prec, chunk)
         assert np.array_equal(t, ref[1]) and np.array_equal(v, ref[0]), (prec, chunk)
+
+
+def _setup(w, res, flags=MAP_BLACK_HOLE):
+    g = api.make_grid(res, w.center, w.lengths)
+    r = api.Rational(api.POLYNOMIAL_4D(w.top.roots, w.top.powers), api.POLYNOMIAL_4D(w.bottom.roots, w.bottom.powers))
+    return g, r, api.make_params(w.max_iter, 2000.0, w.exp_scaling, w.slice, flags)
+
+
+def test_edge_cache_resume_rebuilds_the_same_mesh(gpu):
+    w = W.config2(8)
+    res = (40, 36, 32)
+    g, r, p = _setup(w, res)
+    ws = api.Workspace()
+    v, t = api.nonlinear_mesh(g, r, p, 0.0, FP32_BAND, 0.05, 6, workspace=ws)
+    edges, cells = api.nonlinear_mesh_cache(workspace=ws)
+    assert len(edges) == len(v)
+    # the cache is the reference's: edges straddle the sign change of (float)F in the
+    # oracle's field, in vertex order, and the cells are every cell whose flag is not 0 or 255
+    f = O.field_map(O.make_grid(res, w.center, w.lengths), O.poly(w.top.roots, w.top.powers),
+                    O.poly(w.bottom.roots, w.bottom.powers),
+                    O.make_params(w.max_iter, 2000.0, w.exp_scaling, w.slice, O.BLACK_HOLE))
+    pos = (f.astype(np.float32) > 0).reshape(res[2], res[1], res[0])
+    x, y, z, ax = edges.T
+    a = pos[z, y, x]
+    b = pos[z + (ax == 2), y + (ax == 1), x + (ax == 0)]
+    assert np.all(a != b)
+    keys = ((z.astype(np.int64) * res[1] + y) * res[0] + x) * 3 + ax
+    assert np.all(np.diff(keys) > 0)
+    flag = np.zeros((res[2] - 1, res[1] - 1, res[0] - 1), dtype=np.int32)
+    for k in range(8):
+        dx, dy, dz = k >> 2, (k >> 1) & 1, k & 1
+        flag += (1 << k) * pos[dz:res[2] - 1 + dz, dy:res[1] - 1 + dy, dx:res[0] - 1 + dx]
+    zz, yy, xx = np.nonzero((flag != 0) & (flag != 255))
+    assert np.array_equal(cells, np.c_[xx, yy, zz, flag[zz, yy, xx]].astype(np.int32))
+    # rebuild from the cache in a fresh workspace: bit-identical
+    v2, t2 = api.nonlinear_mesh_resume(g, r, p, edges, cells, workspace=api.Workspace())
+    assert np.array_equal(v2, v) and np.array_equal(t2, t)
+    # a cache missing an edge is rejected, not meshed
+    with pytest.raises(RuntimeError):
+        api.nonlinear_mesh_resume(g, r, p, np.delete(edges, len(edges) // 2, axis=0), cells,
+                                  workspace=api.Workspace())
+
+
+@pytest.mark.parametrize("cfg", ["c1", "c2"])
+def test_lagrange_mesher_vs_oracle(gpu, cfg):
+    w = W.config1(8) if cfg == "c1" else W.config2(8)
+    res = (36, 34, 32)
+    g, r, p = _setup(w, res, MAP_BLACK_HOLE | MAP_LAGRANGE)
+    v, t = api.nonlinear_mesh(g, r, p, 0.0, FP64, 0.05, 5, workspace=api.Workspace())
+    og = O.make_grid(res, w.center, w.lengths)
+    top, bot = O.poly(w.top.roots, w.top.powers), O.poly(w.bottom.roots, w.bottom.powers)
+    op = O.make_params(w.max_iter, 2000.0, w.exp_scaling, w.slice, O.BLACK_HOLE | O.LAGRANGE)
+    ov, ot, bad = O.nonlinear_mc(og, top, bot, op)
+    assert bad == 0 and np.array_equal(t, ot)
+    d = np.max(np.abs(v - ov), axis=1)
+    assert (d > 1e-12).mean() <= 0.005 and np.all(d <= 3.0 / min(res) / 2 ** 6 * 1.0001)
+    # the field in the Lagrange order (FP64 only)
+    F = api.field_eval(g, r, p, FP64).cpu().numpy()
+    ref = O.field_map(og, top, bot, op)
+    fin = np.isfinite(ref)
+    assert np.array_equal(np.isfinite(F), fin)
+    assert np.max(np.abs(F[fin] - ref[fin]) / np.maximum(1.0, np.abs(ref[fin]))) <= 1e-9
+    if cfg == "c2":
+        std = O.field_map(og, top, bot, O.make_params(w.max_iter, 2000.0, w.exp_scaling, w.slice, O.BLACK_HOLE))
+        assert np.max(np.abs(std[fin] - ref[fin])) > 1e-3  # a different map from the factored order
+    with pytest.raises(RuntimeError):
+        api.field_eval(g, r, p, FP32_BAND)

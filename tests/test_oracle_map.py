@@ -150,3 +150,66 @@ def test_config_workloads_are_deterministic():
     c1 = W.config1()
     r = np.linalg.norm(c1.top.roots[:, :3], axis=1)
     assert np.all((r >= 0.5) & (r <= 1.0)) and np.all(c1.top.roots[:, 3] == 0)
+
+
+def _lagrange_brute(p, slice_, top, tp, bot, bp, c, max_iter, escape=2000.0):
+    """RATIONAL_4D::evaluateLagrange order (RATIONAL_4D.cpp:27-88) with 2x2
+    complex matrices: r = (r t_i) b_i^-1 for the paired roots, then the rest."""
+    q = M([p[0], p[1], p[2], slice_])
+    mag = qabs(q)
+    it = 0
+    while mag < escape and it < max_iter:
+        qv = np.array([q[0, 0].real, q[0, 0].imag, q[0, 1].real, q[0, 1].imag])
+        term = lambda r, e: scipy.linalg.fractional_matrix_power(M(qv - r), e)
+        if len(bot) == 0:
+            R = np.eye(2, dtype=complex)
+            for r, e in zip(top, tp):
+                R = R @ term(r, e)
+        else:
+            R = np.eye(2, dtype=complex)
+            same = min(len(top), len(bot))
+            for i in range(same):
+                R = R @ term(top[i], tp[i])
+                R = R @ np.linalg.inv(term(bot[i], bp[i]))
+            for i in range(same, len(top)):
+                R = R @ term(top[i], tp[i])
+            for i in range(same, len(bot)):
+                R = R @ np.linalg.inv(term(bot[i], bp[i]))
+        q = c * R
+        mag = qabs(q)
+        it += 1
+        if mag < 10 * np.finfo(np.float32).tiny:
+            it = max_iter
+    return np.log(mag)
+
+
+@pytest.mark.parametrize("nt,nb", [(3, 2), (2, 4), (3, 3), (4, 0)])
+def test_lagrange_order_matches_matrix_products(nt, nb):
+    rng = np.random.default_rng(nt * 10 + nb)
+    top = rng.normal(scale=0.6, size=(nt, 4))
+    bot = rng.normal(scale=0.6, size=(nb, 4))
+    tp = rng.choice([1.0, 2.0, 1.5], nt)
+    bp = rng.choice([1.0, 2.0, 0.75], nb)
+    pts = rng.uniform(-1.2, 1.2, size=(12, 3))
+    p = O.make_params(2, 2000.0, 0.8, 0.05, O.BLACK_HOLE | O.LAGRANGE)
+    got = O.points_eval(pts, O.poly(top, tp), O.poly(bot, bp), p)
+    want = np.array([_lagrange_brute(x, 0.05, top, tp, bot, bp, 0.8, 2) for x in pts])
+    assert np.allclose(got, want, rtol=1e-9, atol=1e-9)
+    if nb:
+        # the interleaved order is a different map when the factors do not commute
+        std = O.points_eval(pts, O.poly(top, tp), O.poly(bot, bp), O.make_params(2, 2000.0, 0.8, 0.05, O.BLACK_HOLE))
+        assert np.max(np.abs(std - got)) > 1e-3
+    else:
+        std = O.points_eval(pts, O.poly(top, tp), O.poly(bot, bp), O.make_params(2, 2000.0, 0.8, 0.05, O.BLACK_HOLE))
+        assert np.array_equal(std, got)
+
+
+def test_lagrange_equals_factored_order_in_a_complex_plane():
+    # points on the X axis, slice 0 and roots in the (w, x) plane: every factor commutes
+    rng = np.random.default_rng(3)
+    top = np.c_[rng.normal(size=(3, 2)), np.zeros((3, 2))]
+    bot = np.c_[rng.normal(size=(2, 2)), np.zeros((2, 2))]
+    pts = np.c_[np.linspace(-1.3, 1.3, 40), np.zeros((40, 2))]
+    a = O.points_eval(pts, O.poly(top), O.poly(bot), O.make_params(3, 2000.0, 1.1, 0.0, O.BLACK_HOLE | O.LAGRANGE))
+    b = O.points_eval(pts, O.poly(top), O.poly(bot), O.make_params(3, 2000.0, 1.1, 0.0, O.BLACK_HOLE))
+    assert np.allclose(a, b, rtol=1e-12, atol=1e-12)
