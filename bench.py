@@ -71,8 +71,9 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--res", type=int, default=512)
     ap.add_argument("--fp64-steps", type=int, default=2)
-    ap.add_argument("--cpu-slabs", type=int, default=24)
+    ap.add_argument("--cpu-slabs", type=int, default=64)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-c3", action="store_true", help="skip the 1024^3 tiled section")
     ap.add_argument("--band", type=float, default=0.05)
     ap.add_argument("--serve", type=int, default=2000, help="requests in the serving stream (0 = skip)")
     ap.add_argument("--serve-rate", type=float, default=200.0)
@@ -191,6 +192,7 @@ def main():
                "parity_on_sample": {"sign_flips_fp64_gpu": cb["flips_fp64"],
                                     "sign_flips_fp32band_gpu": cb["flips_fp32band"],
                                     "max_rel_err_fp64_gpu": cb["max_rel_err_fp64"]}}
+    c3 = None if args.no_c3 else c3_tiles(args, log)
     serving = None
     if args.serve > 0:
         serving = serve_stream(args, log)
@@ -219,6 +221,7 @@ def main():
         "parity": {"sign_flips_vs_fp64_gpu": flips_vs_fp64, "max_abs_dF_vs_fp64_gpu": maxdf},
         "roofline": roofline,
         "cpu_baseline": cpu,
+        "c3_1024_tiles": c3,
         "serving": serving,
         "p50_ms_256": (serving or {}).get("latency", {}).get("256^3", {}).get("p50_ms") if serving else None,
         "p99_ms_256": (serving or {}).get("latency", {}).get("256^3", {}).get("p99_ms") if serving else None,
@@ -250,6 +253,76 @@ def serve_stream(args, log):
                       "iterations, %s launches in flight, fp32+fp64 band, shortest-remaining-work-first"
                       % (args.serve_budget_gib, ("%.1e" % args.serve_launch_work) if args.serve_launch_work else "3e9",
                          args.serve_inflight or 2)}
+
+
+def c3_tiles(args, log):
+    """1024^3 in 64 tiles of 256^3 (128+128 torus roots, maxIter 3), FP32+band, then marching
+    cubes per tile on the GPU. Two tiles are re-evaluated with the FP64 kernel and their
+    isosurface sign flips counted (the per-tile mesh parity against the oracle is a test:
+    tests/test_gpu_mc.py::test_c3_full_tile_mesh_is_bit_exact)."""
+    import ctypes
+    import torch
+    from quijibo_amd import api, workloads as W
+    from quijibo_amd._lib import FP32_BAND, FP64, MAP_BLACK_HOLE
+    w = W.config3(1024, 256)
+    tile = 256
+    per = tile ** 3
+    nt_axis = w.res[0] // tile
+    tiles = [(tz, ty, tx) for tz in range(nt_axis) for ty in range(nt_axis) for tx in range(nt_axis)]
+    r = api.Rational(api.POLYNOMIAL_4D(w.top.roots, w.top.powers), api.POLYNOMIAL_4D(w.bottom.roots, w.bottom.powers))
+    params = api.make_params(w.max_iter, w.escape, w.exp_scaling, w.slice, MAP_BLACK_HOLE)
+    ws = api.Workspace()
+    out = torch.empty(len(tiles) * per, dtype=torch.float64, device="cuda")
+    grids = []
+    for (tz, ty, tx) in tiles:
+        c, tl = W.tile_grid(w, tz, ty, tx, tile)
+        grids.append(api.make_grid((tile, tile, tile), c, tl))
+    log("c3: %d tiles of %d^3, %d+%d roots" % (len(tiles), tile, w.top.n, w.bottom.n))
+    api.field_eval(grids[0], r, params, FP32_BAND, args.band, out=out[:per], workspace=ws)  # warm-up
+    torch.cuda.synchronize()
+    ws.reset_stats()
+    t0 = time.perf_counter()
+    for k, g in enumerate(grids):
+        api.field_eval(g, r, params, FP32_BAND, args.band, out=out[k * per:(k + 1) * per], workspace=ws)
+    torch.cuda.synchronize()
+    t_eval = time.perf_counter() - t0
+    st = ws.stats()
+    log("c3: field %.1f ms" % (t_eval * 1e3))
+    L = api._lib.load()
+    nv, nt = ctypes.c_int64(), ctypes.c_int64()
+    tot_v = tot_t = 0
+    t0 = time.perf_counter()
+    for k, g in enumerate(grids):
+        api.check(L.qj_mc_run(ws.handle, api._ptr(out[k * per:(k + 1) * per]), ctypes.byref(g), 0.0,
+                              ctypes.byref(nv), ctypes.byref(nt), api._stream_ptr(None)))
+        tot_v += nv.value
+        tot_t += nt.value
+    torch.cuda.synchronize()
+    t_mc = time.perf_counter() - t0
+    log("c3: marching cubes %.1f ms, %d vertices, %d triangles" % (t_mc * 1e3, tot_v, tot_t))
+    # parity on two tiles: one holding the torus surface, one elsewhere
+    check = [tiles.index((1, 1, 0)), tiles.index((2, 2, 2))]
+    parity = []
+    f64 = torch.empty(per, dtype=torch.float64, device="cuda")
+    for k in check:
+        g = grids[k]
+        fld = out[k * per:(k + 1) * per]
+        host = fld.cpu().numpy()
+        api.field_eval(g, r, params, FP64, out=f64, workspace=ws)
+        h64 = f64.cpu().numpy()
+        flips = int(np.count_nonzero((h64.astype(np.float32) > 0) != (host.astype(np.float32) > 0)))
+        fin = np.isfinite(h64) & np.isfinite(host)
+        parity.append({"tile": list(tiles[k]), "sign_flips_vs_fp64_gpu": flips,
+                       "max_abs_dF_vs_fp64_gpu": float(np.max(np.abs(h64[fin] - host[fin]))) if fin.any() else 0.0})
+        log("c3: tile %s parity %s" % (tiles[k], parity[-1]))
+    del out, f64
+    torch.cuda.empty_cache()
+    vox_iters = len(tiles) * per * w.max_iter
+    return {"workload": "1024^3 on [-1.5,1.5]^3 as 64 tiles of 256^3, 128+128 torus roots (R=1, r=0.35), powers 1, "
+                        "maxIter 3, FP32+FP64 band",
+            "gvoxel_iter_s": round(vox_iters / t_eval / 1e9, 4), "field_ms": round(t_eval * 1e3, 2),
+            "band_voxels": st["band_voxels"], "mc_ms_all_tiles": round(t_mc * 1e3, 2),
+            "vertices": tot_v, "triangles": tot_t, "parity": parity}
 
 
 def _kernel_ms(api, ws):

@@ -78,3 +78,20 @@ def test_julia_fields(gpu, cfg):
     v, t, ov, ot = mc_both(f, w.res)
     assert len(t) > 1000
     assert_same_mesh(v, t, ov, ot)
+
+
+def test_c3_full_tile_mesh_is_bit_exact(gpu):
+    """One 256^3 tile of the 1024^3 torus workload (128+128 roots, maxIter 3), FP32+band
+    field: the GPU mesh equals the oracle's marching cubes on the same field."""
+    from quijibo_amd._lib import FP32_BAND
+    w = W.config3(1024, 256)
+    c, tl = W.tile_grid(w, 1, 1, 0, 256)
+    g = api.make_grid((256, 256, 256), c, tl)
+    r = api.Rational(api.POLYNOMIAL_4D(w.top.roots, w.top.powers), api.POLYNOMIAL_4D(w.bottom.roots, w.bottom.powers))
+    out = api.field_eval(g, r, api.make_params(w.max_iter, w.escape, 1.0, 0.0, MAP_BLACK_HOLE), FP32_BAND, 0.05)
+    f = out.cpu().numpy()
+    v, t = api.marching_cubes(out, g)
+    ov, ot = O.marching_cubes(O.make_grid((256, 256, 256), c, tl), f)
+    print("c3 tile: %d vertices %d triangles" % (len(v), len(t)), flush=True)
+    assert len(t) > 10000
+    assert_same_mesh(v, t, ov, ot)  # vertices on edges with an infinite end are NaN on both sides
