@@ -143,6 +143,10 @@ qj_status qj_workspace_kernel_ms(qj_workspace* ws, double* bulk_ms, double* band
  * (q = c * T(q) with no bottom roots); field = log|q|. d_field: device,
  * res[0]*res[1]*res[2] doubles, x fastest. band is used by QJ_FP32_BAND (the
  * reference-facing default is 0.05). */
+/* qj_field_eval and qj_points_eval enqueue kernels only (no memset nodes, no
+ * host synchronization), so they may be captured in a hipGraph once a first
+ * call of at least the same size has sized the workspace's lists (a larger
+ * call reallocates). */
 qj_status qj_field_eval(qj_workspace* ws, const qj_grid* grid, const qj_rational* f,
                         const qj_map_params* params, qj_precision precision, double band,
                         double* d_field, void* stream);

@@ -720,7 +720,9 @@ qj_status launch_field(qj_workspace* ws, const MapArgs& a, const qj_rational* f,
   }
   qj_status st = ensure_list(ws, nvox, s);
   if (st) return st;
-  QJ_HIP(hipMemsetAsync(ws->d_list_count, 0, 2 * sizeof(uint32_t), s));
+  // list counters cleared by a kernel, not a memset node, so the call can be captured in a graph
+  hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, s, ws->d_list_count, 2);
+  QJ_HIP(hipGetLastError());
   const int64_t ntx = (a.res[0] + kTileX - 1) / kTileX, nty = (a.res[1] + kTileY - 1) / kTileY,
                 ntz = (a.res[2] + kTileZ - 1) / kTileZ;
   const int64_t nblocks = ntx * nty * ntz;
@@ -768,7 +770,9 @@ qj_status launch_points(qj_workspace* ws, const MapArgs& a, const qj_rational* f
   if (n > (int64_t)UINT32_MAX) return QJ_ERR_CAPACITY;
   qj_status st = ensure_list(ws, n, s);
   if (st) return st;
-  QJ_HIP(hipMemsetAsync(ws->d_list_count, 0, 2 * sizeof(uint32_t), s));
+  // list counters cleared by a kernel, not a memset node, so the call can be captured in a graph
+  hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, s, ws->d_list_count, 2);
+  QJ_HIP(hipGetLastError());
   uint32_t* alist = ws->d_list + ws->list_cap;
   if (prec == QJ_FP64) {
     hipLaunchKernelGGL((k_list64<PointSource, false>), dim3(list_grid(n)), dim3(kBlock), 0, s, a, f->d64, d_pts,

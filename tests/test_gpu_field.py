@@ -209,3 +209,30 @@ def test_serving_workloads_fp64_and_band(gpu, k):
     inband = np.isfinite(c2) & (np.abs(c2) < 0.05 - 0.01)
     if inband.any():
         assert np.max(np.abs(g32[inband] - c2[inband]) / np.maximum(1.0, np.abs(c2[inband]))) <= 1e-8
+
+
+@pytest.mark.parametrize("precision", [FP32_BAND, FP64])
+def test_field_eval_replays_in_a_captured_graph(gpu, precision):
+    """qj_field_eval enqueues kernels only: captured once in a graph it replays the
+    eager result exactly, replay after replay, with other work in between."""
+    w = W.config2(40)
+    g = api.make_grid(w.res, w.center, w.lengths)
+    r = api.Rational(api.POLYNOMIAL_4D(w.top.roots, w.top.powers), api.POLYNOMIAL_4D(w.bottom.roots, w.bottom.powers))
+    p = api.make_params(w.max_iter, w.escape, w.exp_scaling, w.slice, MAP_BLACK_HOLE)
+    ws = api.Workspace()
+    eager = api.field_eval(g, r, p, precision, 0.05, workspace=ws).clone()  # also sizes the workspace lists
+    out = torch.empty_like(eager)
+    s = torch.cuda.Stream()
+    s.wait_stream(torch.cuda.current_stream())
+    graph = torch.cuda.CUDAGraph()
+    with torch.cuda.graph(graph, stream=s):
+        api.field_eval(g, r, p, precision, 0.05, out=out, workspace=ws, stream=s)
+    torch.cuda.synchronize()
+    for k in range(3):
+        out.fill_(float(k) - 7.0)
+        noise = torch.randn(1 << 16, device="cuda").sum()  # unrelated eager work between replays
+        graph.replay()
+        torch.cuda.synchronize()
+        assert torch.equal(torch.nan_to_num(out, 1e300, 1e301, -1e301), torch.nan_to_num(eager, 1e300, 1e301, -1e301))
+        assert torch.isnan(out).sum() == torch.isnan(eager).sum()
+        del noise
