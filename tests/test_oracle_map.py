@@ -213,3 +213,30 @@ def test_lagrange_equals_factored_order_in_a_complex_plane():
     a = O.points_eval(pts, O.poly(top), O.poly(bot), O.make_params(3, 2000.0, 1.1, 0.0, O.BLACK_HOLE | O.LAGRANGE))
     b = O.points_eval(pts, O.poly(top), O.poly(bot), O.make_params(3, 2000.0, 1.1, 0.0, O.BLACK_HOLE))
     assert np.allclose(a, b, rtol=1e-12, atol=1e-12)
+
+
+def test_field_lookup_reproduces_linear_fields_and_clamps():
+    """FIELD_3D::operator()(VEC3F) (FIELD_3D.cpp:966-1029): trilinear interpolation
+    between cell centres is exact for a linear field (up to the float weights), and
+    positions outside the box take the nearest cell's values."""
+    res = (7, 5, 6)
+    center, lengths = (0.3, -0.2, 0.1), (2.0, 1.5, 1.8)
+    g = O.make_grid(res, center, lengths)
+    a, b, c, d = 1.7, -0.6, 2.3, 0.25
+    dx = [lengths[i] / res[i] for i in range(3)]
+    lo = [center[i] - 0.5 * lengths[i] for i in range(3)]
+    zz, yy, xx = np.meshgrid(*[lo[i] + (np.arange(res[i]) + 0.5) * dx[i] for i in (2, 1, 0)], indexing="ij")
+    data = (a * xx + b * yy + c * zz + d).reshape(-1)
+    rng = np.random.default_rng(11)
+    inner_lo = [lo[i] + 0.5 * dx[i] for i in range(3)]
+    inner_hi = [lo[i] + lengths[i] - 0.5 * dx[i] for i in range(3)]
+    for _ in range(200):
+        p = [rng.uniform(inner_lo[i], inner_hi[i]) for i in range(3)]
+        want = a * p[0] + b * p[1] + c * p[2] + d
+        assert abs(O.field_lookup(g, data, p) - want) <= 1e-6 * (abs(want) + 5)
+    # at a cell centre the lookup returns the cell value
+    assert abs(O.field_lookup(g, data, [xx[2, 3, 4], yy[2, 3, 4], zz[2, 3, 4]]) - data[4 + 3 * 7 + 2 * 35]) < 1e-12
+    # far outside both corners of an axis clamp to the same cell; the weights are not
+    # clamped (s0 + s1 = 1 still), so the corner value comes back up to rounding
+    assert abs(O.field_lookup(g, data, [-50.0, -50.0, -50.0]) - data[0]) <= 1e-9 * abs(data[0])
+    assert abs(O.field_lookup(g, data, [50.0, 50.0, 50.0]) - data[-1]) <= 1e-9 * abs(data[-1])
