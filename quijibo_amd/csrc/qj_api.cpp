@@ -116,10 +116,10 @@ qj_status ensure_list(qj_workspace* ws, int64_t n, hipStream_t s) {
   return QJ_OK;
 }
 
-qj_status ensure_partials(qj_workspace* ws, int64_t n) {
+qj_status ensure_partials(qj_workspace* ws, int64_t n, hipStream_t s) {
   if (n <= ws->partials_cap) return QJ_OK;
   if (ws->d_partials) {
-    QJ_HIP(hipDeviceSynchronize());
+    QJ_HIP(hipStreamSynchronize(s));
     QJ_HIP(hipFree(ws->d_partials));
     ws->d_partials = nullptr;
     ws->partials_cap = 0;

@@ -386,7 +386,7 @@ qj_status ap_grads(qj_workspace* ws, const MapArgs& a, const qj_rational* f, con
     set_error("too many roots for the gradient kernel (%d)", a.nt + a.nb);
     return QJ_ERR_CAPACITY;
   }
-  qj_status st = ensure_partials(ws, nblocks * K + K + 1);
+  qj_status st = ensure_partials(ws, nblocks * K + K + 1, s);
   if (st) return st;
   APArgs g;
   g.a = a;
@@ -667,7 +667,7 @@ qj_status points_adjoint(qj_workspace* ws, const MapArgs& a, const qj_rational* 
     return QJ_ERR_CAPACITY;
   }
   const int64_t nblocks = n > 0 ? (n + kAdjBlock - 1) / kAdjBlock : 1;
-  qj_status st = ensure_partials(ws, nblocks * K);
+  qj_status st = ensure_partials(ws, nblocks * K, s);
   if (st) return st;
   if (n > 0) {
     if (prec == QJ_FP64)

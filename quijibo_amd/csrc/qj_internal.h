@@ -110,7 +110,7 @@ void make_geom(const qj_grid* g, GridGeom* out);
 qj_status make_map_args(const qj_grid* g, const qj_rational* f, const qj_map_params* p, MapArgs* a);
 double term_bits(const std::vector<RootRec64>& roots, double escape);
 qj_status ensure_list(qj_workspace* ws, int64_t n, hipStream_t s);
-qj_status ensure_partials(qj_workspace* ws, int64_t n);
+qj_status ensure_partials(qj_workspace* ws, int64_t n, hipStream_t s);
 // Records an event on s when timing is enabled (slot order: bulk0, bulk1, band0, band1).
 void timing_mark(qj_workspace* ws, hipStream_t s);
 

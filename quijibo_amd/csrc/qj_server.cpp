@@ -106,7 +106,7 @@ struct Slot {
   ChunkDesc* h_desc = nullptr;
   ChunkDesc* d_desc = nullptr;
   uint32_t* d_band = nullptr;
-  uint32_t* d_band_count = nullptr;  // [band count, FP32 queue, FP64 queue, spare]
+  uint32_t* d_band_count = nullptr;  // batch counters: band counts and work queues (8 x uint32)
   hipGraphExec_t exec = nullptr;
   hipEvent_t done = nullptr;
   hipStream_t stream = nullptr;  // one stream per slot: one launch's FP64 band pass overlaps the next's FP32 bulk

@@ -236,3 +236,49 @@ def test_field_eval_replays_in_a_captured_graph(gpu, precision):
         assert torch.equal(torch.nan_to_num(out, 1e300, 1e301, -1e301), torch.nan_to_num(eager, 1e300, 1e301, -1e301))
         assert torch.isnan(out).sum() == torch.isnan(eager).sum()
         del noise
+
+
+@pytest.mark.parametrize("precision", [FP64, FP32_BAND])
+def test_edge_cases(gpu, precision):
+    """maxIter 0 and cells already outside the escape radius give log|q0|
+    (OPTIMIZE_3D.cpp:1319-1328); zero and negative exponents follow pow's
+    guards; degenerate grids and empty point sets."""
+    def check(g, c):
+        if precision == FP64:
+            assert_fp64_close(g, c, rtol=1e-8)
+            return
+        # FP32 bulk: no sign flips, the same non-finite classes, FP64 values in the band
+        assert np.count_nonzero((g.astype(np.float32) > 0) != (c.astype(np.float32) > 0)) == 0
+        assert np.array_equal(np.isnan(g), np.isnan(c)) and np.array_equal(np.isinf(g), np.isinf(c))
+        inband = np.isfinite(c) & (np.abs(c) < 0.04)
+        if inband.any():
+            assert np.max(np.abs(g[inband] - c[inband])) <= 1e-8
+        # outside the band the FP32 values carry FP32 error, amplified where the map is chaotic
+
+    w = W.config2(12)
+    w.max_iter = 0
+    g, c, _, _ = run_both(w, precision)
+    assert_fp64_close(g, c)  # trivial cells are written by the FP64 pass in both modes
+    w = W.config2(12)
+    g, c, _, _ = run_both(w, precision, escape=1.2)  # the box corners start outside
+    check(g, c)
+    outside = np.array([np.sqrt(x * x + y * y + z * z) >= 1.2 for z in (-1.375, 1.375) for y in (-1.375, 1.375)
+                        for x in (-1.375, 1.375)])
+    assert outside.all()
+    w = W.config2(10)
+    rng = np.random.default_rng(4)
+    w.top.powers = rng.choice([0.0, -1.0, 1.0, 2.0, -0.5], w.top.n)
+    w.bottom.powers = rng.choice([0.0, 1.0, -2.0, 1.5], w.bottom.n)
+    g, c, _, _ = run_both(w, precision)
+    check(g, c)
+    for res in ((1, 1, 1), (2, 1, 3), (1, 7, 1)):
+        w = W.config1(4)
+        w.res = res
+        g, c, gr, _ = run_both(w, precision)
+        check(g, c)
+        v, t = api.marching_cubes(torch.from_numpy(c.copy()).cuda(), gr)
+        assert len(v) == 0 and len(t) == 0
+    w = W.config1(4)
+    r = api.Rational(api.POLYNOMIAL_4D(w.top.roots, w.top.powers))
+    empty = torch.empty((0, 3), dtype=torch.float64, device="cuda")
+    assert api.points_eval(empty, r, api.make_params(3), precision).numel() == 0
