@@ -647,7 +647,7 @@ static int resident_blocks(const void* kernel) {
 int batch_variant(const MapArgs& a) { return a.gen ? 0 : (a.term_bits <= kRN4MaxTermBits ? 1 : 2); }
 
 qj_status launch_batch(const ChunkDesc* d_desc, int nchunks, int chunk_voxels, qj_precision prec, uint32_t* d_band,
-                       uint32_t* d_counters, hipStream_t s) {
+                       uint32_t* d_counters, unsigned variants, hipStream_t s) {
   const int bpc = (chunk_voxels + kBlock - 1) / kBlock;
   const int64_t cap = (int64_t)nchunks * chunk_voxels;
   hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, s, d_counters, (int)BC_N);
@@ -661,18 +661,30 @@ qj_status launch_batch(const ChunkDesc* d_desc, int nchunks, int chunk_voxels, q
   static const int g0 = resident_blocks((const void*)k_batch32<0>);
   static const int g1 = resident_blocks((const void*)k_batch32<1>);
   static const int g2 = resident_blocks((const void*)k_batch32<2>);
-  hipLaunchKernelGGL(k_batch32<1>, dim3(g1), dim3(kBlock), 0, s, d_desc, bpc, chunk_voxels, d_band, cap, d_counters);
-  QJ_HIP(hipGetLastError());
-  hipLaunchKernelGGL(k_batch32<2>, dim3(g2), dim3(kBlock), 0, s, d_desc, bpc, chunk_voxels, d_band, cap, d_counters);
-  QJ_HIP(hipGetLastError());
-  hipLaunchKernelGGL(k_batch32<0>, dim3(g0), dim3(kBlock), 0, s, d_desc, bpc, chunk_voxels, d_band, cap, d_counters);
-  QJ_HIP(hipGetLastError());
-  hipLaunchKernelGGL((k_batch64_list<kExactOnly, false>), dim3(2048), dim3(kBlock), 0, s, d_desc, chunk_voxels,
-                     (const uint32_t*)d_band, cap, (const uint32_t*)(d_counters + BC_BAND_INT));
-  QJ_HIP(hipGetLastError());
-  hipLaunchKernelGGL((k_batch64_list<kExactInt, true>), dim3(1024), dim3(kBlock), 0, s, d_desc, chunk_voxels,
-                     (const uint32_t*)d_band, cap, (const uint32_t*)(d_counters + BC_BAND_GEN));
-  QJ_HIP(hipGetLastError());
+  // a variant with no chunks is not launched: an empty persistent grid still waits for free
+  // compute units behind the other in-flight launch
+  if (variants & 2u) {
+    hipLaunchKernelGGL(k_batch32<1>, dim3(g1), dim3(kBlock), 0, s, d_desc, bpc, chunk_voxels, d_band, cap, d_counters);
+    QJ_HIP(hipGetLastError());
+  }
+  if (variants & 4u) {
+    hipLaunchKernelGGL(k_batch32<2>, dim3(g2), dim3(kBlock), 0, s, d_desc, bpc, chunk_voxels, d_band, cap, d_counters);
+    QJ_HIP(hipGetLastError());
+  }
+  if (variants & 1u) {
+    hipLaunchKernelGGL(k_batch32<0>, dim3(g0), dim3(kBlock), 0, s, d_desc, bpc, chunk_voxels, d_band, cap, d_counters);
+    QJ_HIP(hipGetLastError());
+  }
+  if (variants & 6u) {  // band voxels of the integer-exponent variants
+    hipLaunchKernelGGL((k_batch64_list<kExactOnly, false>), dim3(2048), dim3(kBlock), 0, s, d_desc, chunk_voxels,
+                       (const uint32_t*)d_band, cap, (const uint32_t*)(d_counters + BC_BAND_INT));
+    QJ_HIP(hipGetLastError());
+  }
+  if (variants & 1u) {  // band voxels of the general variant
+    hipLaunchKernelGGL((k_batch64_list<kExactInt, true>), dim3(1024), dim3(kBlock), 0, s, d_desc, chunk_voxels,
+                       (const uint32_t*)d_band, cap, (const uint32_t*)(d_counters + BC_BAND_GEN));
+    QJ_HIP(hipGetLastError());
+  }
   return QJ_OK;
 }
 

@@ -140,9 +140,11 @@ int batch_variant(const MapArgs& a);
 // nchunks * chunk_voxels entries; the count is cleared by a kernel, not a
 // memset, so the sequence can be captured in a graph and replayed.
 // d_counters: 8 uint32 (band counts and work queues), cleared at the start of
-// every batch; d_band: nchunks * chunk_voxels entries.
+// every batch; d_band: nchunks * chunk_voxels entries. variants: bit v set when
+// the batch holds chunks of FP32 variant v (kernels of absent variants are not
+// enqueued; ignored for QJ_FP64).
 qj_status launch_batch(const ChunkDesc* d_desc, int nchunks, int chunk_voxels, qj_precision prec, uint32_t* d_band,
-                       uint32_t* d_counters, hipStream_t s);
+                       uint32_t* d_counters, unsigned variants, hipStream_t s);
 
 // marching cubes (qj_mc.hip)
 qj_status mc_run(qj_workspace* ws, const double* d_field, const qj_grid* g, double iso, int64_t* nv,

@@ -5,8 +5,9 @@
 // their HBM footprint (field + root records) fits a preallocated arena, splits
 // admitted requests into fixed-size voxel chunks and packs chunks of several
 // requests into one launch, shortest remaining work first. Each launch is a
-// hipGraph captured once per in-flight slot ([descriptor upload, FP32 bulk,
-// FP64 band] or [descriptor upload, FP64]) and replayed with new descriptors;
+// hipGraph captured once per in-flight slot and set of FP32 variants present
+// ([descriptor upload, FP32 bulk per variant, FP64 band] or [descriptor
+// upload, FP64]) and replayed with new descriptors;
 // requests that arrive while launches are on the GPU join the next launch.
 // Fields stay resident in the arena until completion (or until released when
 // the caller asked to keep them). The reference has no serving path; its
@@ -107,7 +108,10 @@ struct Slot {
   ChunkDesc* d_desc = nullptr;
   uint32_t* d_band = nullptr;
   uint32_t* d_band_count = nullptr;  // batch counters: band counts and work queues (8 x uint32)
-  hipGraphExec_t exec = nullptr;
+  // one graph per set of FP32 variants present in a batch (bit v = variant v);
+  // an FP64 server uses exec[0] only
+  hipGraphExec_t exec[8] = {};
+  unsigned variants = 0;
   hipEvent_t done = nullptr;
   hipStream_t stream = nullptr;  // one stream per slot: one launch's FP64 band pass overlaps the next's FP32 bulk
   bool busy = false;
@@ -230,6 +234,7 @@ static int build_batch(qj_server* s, Slot& sl) {
   sl.h_desc[0].total = n;
   sl.h_desc[0].g1 = g[1];
   sl.h_desc[0].g2 = g[1] + g[2];
+  sl.variants = s->cfg.precision == QJ_FP64 ? 0u : (g[1] ? 1u : 0u) | (g[2] ? 2u : 0u) | (g[3] ? 4u : 0u);
   return n;
 }
 
@@ -305,7 +310,7 @@ static void worker_main(qj_server* s) {
         Slot& sl = s->slots[(s->slot_next + k) % s->slots.size()];
         if (sl.busy) continue;
         if (build_batch(s, sl) == 0) break;
-        hipError_t e = hipGraphLaunch(sl.exec, sl.stream);
+        hipError_t e = hipGraphLaunch(sl.exec[sl.variants], sl.stream);
         if (e == hipSuccess) e = hipEventRecord(sl.done, sl.stream);
         sl.busy = true;
         if (e != hipSuccess) {
@@ -351,27 +356,31 @@ static qj_status capture_slot(qj_server* s, Slot& sl) {
   }
   QJ_HIP(hipEventCreateWithFlags(&sl.done, hipEventDisableTiming));
   QJ_HIP(hipStreamCreateWithFlags(&sl.stream, hipStreamNonBlocking));
-  hipGraph_t g;
-  QJ_HIP(hipStreamBeginCapture(sl.stream, hipStreamCaptureModeThreadLocal));
-  hipError_t e = hipMemcpyAsync(sl.d_desc, sl.h_desc, sizeof(ChunkDesc) * nch, hipMemcpyHostToDevice, sl.stream);
-  qj_status st = QJ_OK;
-  if (e == hipSuccess)
-    st = launch_batch(sl.d_desc, nch, s->cfg.chunk_voxels, (qj_precision)s->cfg.precision, sl.d_band,
-                      sl.d_band_count, sl.stream);
-  hipError_t e2 = hipStreamEndCapture(sl.stream, &g);
-  if (e != hipSuccess) return hip_fail(e, "capture memcpy");
-  if (st) return st;
-  if (e2 != hipSuccess) return hip_fail(e2, "hipStreamEndCapture");
-  e = hipGraphInstantiate(&sl.exec, g, nullptr, nullptr, 0);
-  (void)hipGraphDestroy(g);
-  if (e != hipSuccess) return hip_fail(e, "hipGraphInstantiate");
+  const unsigned first = s->cfg.precision == QJ_FP64 ? 0u : 1u, last = s->cfg.precision == QJ_FP64 ? 0u : 7u;
+  for (unsigned v = first; v <= last; v++) {
+    hipGraph_t g;
+    QJ_HIP(hipStreamBeginCapture(sl.stream, hipStreamCaptureModeThreadLocal));
+    hipError_t e = hipMemcpyAsync(sl.d_desc, sl.h_desc, sizeof(ChunkDesc) * nch, hipMemcpyHostToDevice, sl.stream);
+    qj_status st = QJ_OK;
+    if (e == hipSuccess)
+      st = launch_batch(sl.d_desc, nch, s->cfg.chunk_voxels, (qj_precision)s->cfg.precision, sl.d_band,
+                        sl.d_band_count, v, sl.stream);
+    hipError_t e2 = hipStreamEndCapture(sl.stream, &g);
+    if (e != hipSuccess) return hip_fail(e, "capture memcpy");
+    if (st) return st;
+    if (e2 != hipSuccess) return hip_fail(e2, "hipStreamEndCapture");
+    e = hipGraphInstantiate(&sl.exec[v], g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    if (e != hipSuccess) return hip_fail(e, "hipGraphInstantiate");
+  }
   return QJ_OK;
 }
 
 static void destroy_server(qj_server* s) {
   for (Slot& sl : s->slots) {
     if (sl.stream) (void)hipStreamSynchronize(sl.stream);
-    if (sl.exec) (void)hipGraphExecDestroy(sl.exec);
+    for (hipGraphExec_t x : sl.exec)
+      if (x) (void)hipGraphExecDestroy(x);
     if (sl.done) (void)hipEventDestroy(sl.done);
     if (sl.h_desc) (void)hipHostFree(sl.h_desc);
     if (sl.d_desc) (void)hipFree(sl.d_desc);
