@@ -23,9 +23,16 @@ COMMON = ["-O3", "-fPIC", "-std=c++17", "--offload-arch=" + ARCH, "-Wall", "-Wno
           "-Wno-unused-variable", "-Wno-unused-but-set-variable", "-Wno-unused-value", "-Wno-unused-result", "-I" + os.path.join(HERE, "..", "include")]
 
 
+# The FP32 field kernels branch per root on the exponent kind with the
+# quaternion product inside each branch; SimplifyCFG's common-code sinking
+# would merge those products behind register copies.
+FILE_FLAGS = {"csrc/qj_field.hip": ["-mllvm", "-simplifycfg-sink-common=false", "-mllvm", "-structurizecfg-skip-uniform-regions"]}
+
+
 def _compile(src):
     obj = os.path.join(BUILD, os.path.basename(src) + ".o")
-    cmd = [HIPCC, "-x", "hip"] + COMMON + DEVICE_FLAGS + ["-c", os.path.join(HERE, src), "-o", obj]
+    cmd = ([HIPCC, "-x", "hip"] + COMMON + DEVICE_FLAGS + FILE_FLAGS.get(src, []) +
+           ["-c", os.path.join(HERE, src), "-o", obj])
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError("hipcc failed for %s:\n%s\n%s" % (src, " ".join(cmd), r.stderr[-6000:]))

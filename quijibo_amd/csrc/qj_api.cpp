@@ -170,8 +170,27 @@ static void fill_records(const qj_poly* p, std::vector<RootRec64>& h64, std::vec
     t.z = (float)r.z;
     t.e = (float)r.e;
     t.kind = (r.e == 1.0) ? 1 : (r.e == 2.0 ? 2 : 0);
-    t.pad0 = t.pad1 = 0;
     h32.push_back(t);
+  }
+}
+
+void pack_roots32(const RootRec32* rec, int nt, int nb, std::vector<uint8_t>& blk) {
+  const int n = nt + nb;
+  blk.assign(roots32_bytes(n), 0);
+  F4* q = (F4*)blk.data();
+  int32_t* pat = (int32_t*)(q + n);
+  float* e = (float*)(pat + n);
+  for (int i = 0; i < n; i++) {
+    q[i] = F4{rec[i].w, rec[i].x, rec[i].y, rec[i].z};
+    e[i] = rec[i].e;
+    // kinds of roots i..i+3 of the same polynomial
+    const int end = i < nt ? nt : n;
+    int32_t p = 0;
+    for (int j = 0; j < 4 && i + j < end; j++) {
+      const int k = rec[i + j].kind;
+      p |= ((k == 2 ? kPatSquare : 0) | (k == 0 ? kPatGeneral : 0)) << (2 * j);
+    }
+    pat[i] = p;
   }
 }
 
@@ -225,12 +244,13 @@ qj_status qj_rational_create(const qj_poly* top, const qj_poly* bottom, qj_ratio
   f->sc_bottom = bottom->power_scalar;
   fill_records(top, f->h64, f->h32);
   fill_records(bottom, f->h64, f->h32);
+  pack_roots32(f->h32.data(), f->nt, f->nb, f->blk32);
   hipGetDevice(&f->device);
   const size_t n = f->h64.size();
   hipError_t e = hipMalloc(&f->d64, sizeof(RootRec64) * n);
-  if (e == hipSuccess) e = hipMalloc(&f->d32, sizeof(RootRec32) * n);
+  if (e == hipSuccess) e = hipMalloc(&f->d32, f->blk32.size());
   if (e == hipSuccess) e = hipMemcpy(f->d64, f->h64.data(), sizeof(RootRec64) * n, hipMemcpyHostToDevice);
-  if (e == hipSuccess) e = hipMemcpy(f->d32, f->h32.data(), sizeof(RootRec32) * n, hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = hipMemcpy(f->d32, f->blk32.data(), f->blk32.size(), hipMemcpyHostToDevice);
   if (e != hipSuccess) {
     hipFree(f->d64);
     hipFree(f->d32);
@@ -263,9 +283,10 @@ qj_status qj_rational_update(qj_rational* f, const qj_poly* top, const qj_poly* 
   f->sc_bottom = bottom->power_scalar;
   fill_records(top, f->h64, f->h32);
   fill_records(bottom, f->h64, f->h32);
+  pack_roots32(f->h32.data(), f->nt, f->nb, f->blk32);
   const size_t n = f->h64.size();
   QJ_HIP(hipMemcpyAsync(f->d64, f->h64.data(), sizeof(RootRec64) * n, hipMemcpyHostToDevice, s));
-  QJ_HIP(hipMemcpyAsync(f->d32, f->h32.data(), sizeof(RootRec32) * n, hipMemcpyHostToDevice, s));
+  QJ_HIP(hipMemcpyAsync(f->d32, f->blk32.data(), f->blk32.size(), hipMemcpyHostToDevice, s));
   return QJ_OK;
 }
 

@@ -211,23 +211,28 @@ __device__ __forceinline__ int renorm32(q32& t, int& E) {
 // quaternion part is unit length. The reference's guards carry over: v = 0
 // gives a positive real result and a non-positive angle drops the imaginary
 // part.
-__device__ __forceinline__ q32 general32(const RootRec32& r, const q32& b, float& L) {
+__device__ __forceinline__ q32 general32(float e, const q32& b, float& L) {
   const float partial = b.x * b.x + b.y * b.y + b.z * b.z;
   const float n2 = partial + b.w * b.w;
   const float vm = __builtin_sqrtf(partial);
   const float th = atan2f(vm, b.w);
-  const float m = vm > 0.0f ? r.e * th : 0.0f;  // scale = e*acos(w/|q|)*0 when v = 0
-  L += r.e * 0.5f * __log2f(n2);
+  const float m = vm > 0.0f ? e * th : 0.0f;  // scale = e*acos(w/|q|)*0 when v = 0
+  L += e * 0.5f * __log2f(n2);
   float s, c;
   __sincosf(m, &s, &c);
   const float f = (m > 0.0f && vm > 0.0f) ? s / vm : 0.0f;
   return q32{c, f * b.x, f * b.y, f * b.z};
 }
 
-// e = 1: pow(b, 1) is b itself except for v = 0, where the polar form gives +|w|.
-__device__ __forceinline__ q32 kind1(const q32& b) {
-  const float vmax = fmaxf(fmaxf(fabsf(b.x), fabsf(b.y)), fabsf(b.z));
-  return q32{vmax > 0.0f ? b.w : fabsf(b.w), b.x, b.y, b.z};
+// e = 1: pow(b, 1) is b itself except for v = 0, where the polar form gives
+// +|w|. The FP32 path uses b and keeps the running minimum of |b.x| instead: a
+// voxel that met b.x = 0 is sent to FP64, which applies the guard exactly. The
+// minimum is one v_min_f32 with an |.| source modifier; fminf would add two
+// canonicalising maxes per term. A NaN b.x leaves vx unchanged (the voxel is
+// flagged through its magnitude).
+__device__ __forceinline__ q32 kind1(const q32& b, float& vx) {
+  asm("v_min_f32_e64 %0, %1, |%2|" : "=v"(vx) : "v"(vx), "v"(b.x));
+  return b;
 }
 
 // e = 2: b * b (for v = 0 this is (w^2, 0, 0, 0), the polar form's value too).
@@ -237,24 +242,30 @@ __device__ __forceinline__ q32 kind2(const q32& b) {
   return q32{b.w * b.w - partial, t * b.x, t * b.y, t * b.z};
 }
 
-__device__ __forceinline__ q32 term32(const RootRec32& r, const q32& q, float& L) {
+// First term of a product; kb: the root's pattern bits (kPatSquare,
+// kPatGeneral).
+__device__ __forceinline__ q32 term32(const F4& r, int kb, const float* __restrict__ e, const q32& q, float& L,
+                                      float& vx) {
   const q32 b = qsub(q, q32{r.w, r.x, r.y, r.z});
-  if (r.kind == 1) return kind1(b);
-  if (r.kind == 2) return kind2(b);
-  return general32(r, b, L);
+  if (kb & kPatGeneral) return general32(*e, b, L);
+  if (kb & kPatSquare) return kind2(b);
+  return kind1(b, vx);
 }
 
 // t <- t * (q - r)^e, with the product taken inside each kind's branch so the
-// branches share no term registers.
+// branches share no term registers (this file is compiled without
+// SimplifyCFG's common-code sinking, which would merge the products behind
+// register copies).
 template <bool kGeneral>
-__device__ __forceinline__ void apply32(q32& t, const RootRec32& r, const q32& q, float& L) {
+__device__ __forceinline__ void apply32(q32& t, const F4& r, int kb, const float* __restrict__ e, const q32& q,
+                                        float& L, float& vx) {
   const q32 b = qsub(q, q32{r.w, r.x, r.y, r.z});
-  if (r.kind == 1) {
-    t = qmul(t, kind1(b));
-  } else if (!kGeneral || r.kind == 2) {
+  if (kGeneral && (kb & kPatGeneral)) {
+    t = qmul(t, general32(*e, b, L));
+  } else if (kb & kPatSquare) {
     t = qmul(t, kind2(b));
   } else {
-    t = qmul(t, general32(r, b, L));
+    t = qmul(t, kind1(b, vx));
   }
 }
 
@@ -262,17 +273,24 @@ __device__ __forceinline__ void apply32(q32& t, const RootRec32& r, const q32& q
 // terms. hi/lo receive the largest and smallest log2 magnitude of the running
 // product at the renormalisation points (E, plus L when general kinds exist);
 // `bad` is set when a group of kRN terms fell below 2^-60 (kRN = 4 only: a
-// group of four can leave FP32's normal range, two cannot).
+// group of four can leave FP32's normal range, two cannot). vx: minimum |x| over
+// the e = 1 terms (see kind1). The roots of a group are read before its first
+// term: one scalar load of kRN * 16 bytes and one pattern word.
 template <int kRN, bool kGeneral>
-__device__ __forceinline__ q32 poly32(const RootRec32* __restrict__ r, int n, const q32& q, int& E, float& L,
-                                      float& hi, float& lo, bool& bad) {
-  q32 t = term32(r[0], q, L);
+__device__ __forceinline__ q32 poly32(const F4* __restrict__ rq, const int32_t* __restrict__ rp,
+                                      const float* __restrict__ re, int n, const q32& q, int& E, float& L, float& hi,
+                                      float& lo, bool& bad, float& vx) {
+  q32 t = term32(rq[0], rp[0], re, q, L, vx);
   int i = 1;
   int mn = 0;
   int ehi = -(1 << 30), elo = 1 << 30;
   for (; i + kRN - 1 < n; i += kRN) {
+    F4 g[kRN];
 #pragma unroll
-    for (int j = 0; j < kRN; j++) apply32<kGeneral>(t, r[i + j], q, L);
+    for (int j = 0; j < kRN; j++) g[j] = rq[i + j];
+    const int pat = rp[i];
+#pragma unroll
+    for (int j = 0; j < kRN; j++) apply32<kGeneral>(t, g[j], pat >> (2 * j), re + i + j, q, L, vx);
     const int k = renorm32(t, E);
     if (kRN > 2) mn = min(mn, k);
     if (kGeneral) {
@@ -284,7 +302,7 @@ __device__ __forceinline__ q32 poly32(const RootRec32* __restrict__ r, int n, co
       elo = min(elo, E);
     }
   }
-  for (; i < n; i++) apply32<kGeneral>(t, r[i], q, L);
+  for (; i < n; i++) apply32<kGeneral>(t, rq[i], rp[i], re + i, q, L, vx);
   const int k = renorm32(t, E);
   if (kRN > 2) bad |= min(mn, k) < -60;
   if (kGeneral) {
@@ -318,7 +336,7 @@ struct Res32 {
 // exactly (F = -inf under the black-hole stop); anything closer to a range
 // edge, or outside it, goes to FP64.
 template <int kRN, bool kGeneral>
-__device__ __forceinline__ Res32 eval32(const MapArgs& a, const RootRec32* __restrict__ roots, const Consts32& k,
+__device__ __forceinline__ Res32 eval32(const MapArgs& a, const Roots32& roots, const Consts32& k,
                                         const q64& q0) {
   Res32 res{0.0f, 0, true, false};
   // the first escape / origin test in FP64, exactly as the reference
@@ -335,6 +353,7 @@ __device__ __forceinline__ Res32 eval32(const MapArgs& a, const RootRec32* __res
   const bool black_hole = (a.flags & QJ_MAP_BLACK_HOLE) != 0;
   q32 q{(float)q0.w, (float)q0.x, (float)q0.y, (float)q0.z};
   float log2mag = 0.0f;
+  float vx = INFINITY;
   bool amb = false, bad = false, zero = false;
   int it = 0;
   // FP64 running products between two checkpoints grow by at most (kRN - 1) terms
@@ -342,12 +361,13 @@ __device__ __forceinline__ Res32 eval32(const MapArgs& a, const RootRec32* __res
   while (true) {
     int ET = 0, EB = 0;
     float L = 0.0f, hiT = -1e30f, loT = 1e30f;
-    const q32 T = poly32<kRN, kGeneral>(roots, a.nt, q, ET, L, hiT, loT, bad);
+    const q32 T = poly32<kRN, kGeneral>(roots.q, roots.pat, roots.e, a.nt, q, ET, L, hiT, loT, bad, vx);
     bad |= !(hiT < lim && loT > -lim);
     float l2 = k.log2c;
     if (a.nb > 0) {
       float LB = 0.0f, hiB = -1e30f, loB = 1e30f;
-      const q32 B = poly32<kRN, kGeneral>(roots + a.nt, a.nb, q, EB, LB, hiB, loB, bad);
+      const q32 B = poly32<kRN, kGeneral>(roots.q + a.nt, roots.pat + a.nt, roots.e + a.nt, a.nb, q, EB, LB, hiB,
+                                          loB, bad, vx);
       bad |= !(hiB < lim && loB > -lim);
       const float nb2 = B.w * B.w + B.x * B.x + B.y * B.y + B.z * B.z;
       const float lB = 0.5f * __log2f(nb2) + (float)EB + LB;
@@ -385,7 +405,7 @@ __device__ __forceinline__ Res32 eval32(const MapArgs& a, const RootRec32* __res
     if (bad || l2 >= k.log2esc || bh || it >= a.max_iter) break;
   }
   res.F = zero ? -INFINITY : log2mag * 0.69314718f;
-  res.need64 = bad || amb || !(fabsf(res.F) >= k.band);
+  res.need64 = bad || amb || !(vx > 0.0f) || !(fabsf(res.F) >= k.band);
   return res;
 }
 
@@ -431,9 +451,10 @@ __global__ __launch_bounds__(kBlock) void k_grid64(MapArgs a, const RootRec64* _
 }
 
 template <int kRN, bool kGeneral>
-__global__ __launch_bounds__(kBlock) void k_grid32(MapArgs a, Consts32 k, const RootRec32* __restrict__ roots,
+__global__ __launch_bounds__(kBlock) void k_grid32(MapArgs a, Consts32 k, const F4* __restrict__ roots,
                                                    double* __restrict__ out, unsigned long long* __restrict__ cnt,
                                                    uint32_t* __restrict__ blist, uint32_t* __restrict__ bcount) {
+  const Roots32 R = roots32_view(roots, a.nt + a.nb);
   int x, y, z;
   const bool valid = tile_voxel(a, x, y, z);
   int iters = 0;
@@ -442,7 +463,7 @@ __global__ __launch_bounds__(kBlock) void k_grid32(MapArgs a, Consts32 k, const 
   if (valid) {
     double c[3];
     cell_center(a, x, y, z + a.zoff, c);
-    const Res32 r = eval32<kRN, kGeneral>(a, roots, k, q64{c[0], c[1], c[2], a.slice});
+    const Res32 r = eval32<kRN, kGeneral>(a, R, k, q64{c[0], c[1], c[2], a.slice});
     if (r.written) {
       if (!r.need64) out[idx] = (double)r.F;
       need = r.need64;
@@ -455,16 +476,17 @@ __global__ __launch_bounds__(kBlock) void k_grid32(MapArgs a, Consts32 k, const 
 }
 
 template <class Src, int kRN, bool kGeneral>
-__global__ __launch_bounds__(kBlock) void k_points32(MapArgs a, Consts32 k, const RootRec32* __restrict__ roots,
+__global__ __launch_bounds__(kBlock) void k_points32(MapArgs a, Consts32 k, const F4* __restrict__ roots,
                                                      const double* __restrict__ pts, int64_t n,
                                                      double* __restrict__ out, unsigned long long* __restrict__ cnt,
                                                      uint32_t* __restrict__ blist, uint32_t* __restrict__ bcount) {
+  const Roots32 R = roots32_view(roots, a.nt + a.nb);
   const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
   const bool valid = i < n;
   int iters = 0;
   bool need = false;
   if (valid) {
-    const Res32 r = eval32<kRN, kGeneral>(a, roots, k, Src::start(a, pts, i));
+    const Res32 r = eval32<kRN, kGeneral>(a, R, k, Src::start(a, pts, i));
     if (r.written) {
       if (!r.need64) out[i] = (double)r.F;
       need = r.need64;
@@ -566,7 +588,7 @@ __global__ __launch_bounds__(kBlock) void k_batch32(const ChunkDesc* __restrict_
       k.band = d.band;
       k.term_bits = (float)a.term_bits;
       const int64_t idx = d.start + local;
-      const Res32 r = eval32<kRN, kGen>(a, d.r32, k, chunk_start(a, idx));
+      const Res32 r = eval32<kRN, kGen>(a, roots32_view(d.r32, a.nt + a.nb), k, chunk_start(a, idx));
       if (r.written) {
         if (!r.need64) d.out[idx] = (double)r.F;
         need = r.need64;
@@ -751,13 +773,13 @@ qj_status launch_field(qj_workspace* ws, const MapArgs& a, const qj_rational* f,
     const Consts32 k = consts32(a, band);
     const dim3 g((unsigned)nblocks), b(kBlock);
     if (a.gen)
-      hipLaunchKernelGGL((k_grid32<2, true>), g, b, 0, s, a, k, f->d32, d_out, ws->d_counters, ws->d_list,
+      hipLaunchKernelGGL((k_grid32<2, true>), g, b, 0, s, a, k, (const F4*)f->d32, d_out, ws->d_counters, ws->d_list,
                          ws->d_list_count);
     else if (a.term_bits <= kRN4MaxTermBits)
-      hipLaunchKernelGGL((k_grid32<4, false>), g, b, 0, s, a, k, f->d32, d_out, ws->d_counters, ws->d_list,
+      hipLaunchKernelGGL((k_grid32<4, false>), g, b, 0, s, a, k, (const F4*)f->d32, d_out, ws->d_counters, ws->d_list,
                          ws->d_list_count);
     else
-      hipLaunchKernelGGL((k_grid32<2, false>), g, b, 0, s, a, k, f->d32, d_out, ws->d_counters, ws->d_list,
+      hipLaunchKernelGGL((k_grid32<2, false>), g, b, 0, s, a, k, (const F4*)f->d32, d_out, ws->d_counters, ws->d_list,
                          ws->d_list_count);
     QJ_HIP(hipGetLastError());
     timing_mark(ws, s);
@@ -796,13 +818,13 @@ qj_status launch_points(qj_workspace* ws, const MapArgs& a, const qj_rational* f
     const Consts32 k = consts32(a, band);
     const dim3 g((unsigned)nb), b(kBlock);
     if (a.gen)
-      hipLaunchKernelGGL((k_points32<PointSource, 2, true>), g, b, 0, s, a, k, f->d32, d_pts, n, d_out,
+      hipLaunchKernelGGL((k_points32<PointSource, 2, true>), g, b, 0, s, a, k, (const F4*)f->d32, d_pts, n, d_out,
                          ws->d_counters, ws->d_list, ws->d_list_count);
     else if (a.term_bits <= kRN4MaxTermBits)
-      hipLaunchKernelGGL((k_points32<PointSource, 4, false>), g, b, 0, s, a, k, f->d32, d_pts, n, d_out,
+      hipLaunchKernelGGL((k_points32<PointSource, 4, false>), g, b, 0, s, a, k, (const F4*)f->d32, d_pts, n, d_out,
                          ws->d_counters, ws->d_list, ws->d_list_count);
     else
-      hipLaunchKernelGGL((k_points32<PointSource, 2, false>), g, b, 0, s, a, k, f->d32, d_pts, n, d_out,
+      hipLaunchKernelGGL((k_points32<PointSource, 2, false>), g, b, 0, s, a, k, (const F4*)f->d32, d_pts, n, d_out,
                          ws->d_counters, ws->d_list, ws->d_list_count);
     QJ_HIP(hipGetLastError());
     if (a.gen)

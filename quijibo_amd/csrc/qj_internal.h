@@ -17,12 +17,36 @@ struct RootRec64 {
   double e;   // power_scalar * powers[i], rounded as the reference does (POLYNOMIAL_4D.cpp:914,919)
   double p;   // powers[i]
 };
+// FP32 view of one root (host side; kernels read the packed block below).
 struct RootRec32 {
   float w, x, y, z;
   float e;
   int32_t kind;  // 1: e == 1, 2: e == 2 (exact integer powers), 0: general power
-  float pad0, pad1;
 };
+
+struct alignas(16) F4 {
+  float w, x, y, z;
+};
+
+// Device block of the FP32 roots of one rational function, top roots then
+// bottom roots (n = nt + nb): F4 q[n] | int32 pat[n] | float e[n]. pat[i]
+// holds the exponent kinds of roots i..i+3 of the same polynomial, two bits
+// per root (bit 2j: root i+j has e = 2; bit 2j+1: its exponent is neither 1
+// nor 2), so a group of four roots is one 64-byte scalar load and one word.
+constexpr int kPatSquare = 1, kPatGeneral = 2;
+struct Roots32 {
+  const F4* q;
+  const int32_t* pat;
+  const float* e;
+};
+__host__ __device__ inline Roots32 roots32_view(const void* blk, int n) {
+  const F4* q = (const F4*)blk;
+  const int32_t* pat = (const int32_t*)(q + n);
+  return Roots32{q, pat, (const float*)(pat + n)};
+}
+inline size_t roots32_bytes(int n) { return (size_t)n * (sizeof(F4) + 2 * sizeof(int32_t)); }
+// Packs nt top records followed by nb bottom records into the block layout.
+void pack_roots32(const RootRec32* rec, int nt, int nb, std::vector<uint8_t>& blk);
 
 // Geometry and map constants passed by value to every kernel.
 struct MapArgs {
@@ -71,9 +95,10 @@ struct qj_rational {
   int32_t nt = 0, nb = 0;
   double sc_top = 1, sc_bottom = 1;
   qj::RootRec64* d64 = nullptr;
-  qj::RootRec32* d32 = nullptr;
+  void* d32 = nullptr;  // FP32 root block (roots32_view)
   std::vector<qj::RootRec64> h64;
   std::vector<qj::RootRec32> h32;
+  std::vector<uint8_t> blk32;
   int device = 0;
 };
 
@@ -124,7 +149,7 @@ qj_status launch_points(qj_workspace* ws, const MapArgs& a, const qj_rational* f
 struct ChunkDesc {
   MapArgs a;
   const RootRec64* r64;
-  const RootRec32* r32;
+  const F4* r32;  // FP32 root block of the chunk's request (roots32_view)
   double* out;
   int64_t start;  // first voxel (linear, x fastest) of this chunk in its request
   int32_t count;  // voxels in this chunk (0 = unused slot)

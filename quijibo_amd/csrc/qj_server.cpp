@@ -94,6 +94,7 @@ struct Req {
   double arrival = 0, admit = 0, done = 0;
   std::vector<RootRec64> h64;
   std::vector<RootRec32> h32;
+  std::vector<uint8_t> blk32;  // FP32 root block (roots32_view)
   int nt = 0, nb = 0;
   double sct = 1, scb = 1;
   int64_t voxels = 0, next = 0, chunks = 0, chunks_done = 0;
@@ -177,7 +178,7 @@ static bool try_admit(qj_server* s, Req* r) {
   // synchronous (a few KB): the launches run on the slots' own streams
   if (hipMemcpy(b + r->off64, r->h64.data(), r->h64.size() * sizeof(RootRec64), hipMemcpyHostToDevice) !=
           hipSuccess ||
-      hipMemcpy(b + r->off32, r->h32.data(), r->h32.size() * sizeof(RootRec32), hipMemcpyHostToDevice) !=
+      hipMemcpy(b + r->off32, r->blk32.data(), r->blk32.size(), hipMemcpyHostToDevice) !=
           hipSuccess) {
     s->arena.free(off, r->bytes);
     return false;
@@ -211,7 +212,7 @@ static int build_batch(qj_server* s, Slot& sl) {
       ChunkDesc& d = tmp[n++];
       d.a = r->args;
       d.r64 = (const RootRec64*)(b + r->off64);
-      d.r32 = (const RootRec32*)(b + r->off32);
+      d.r32 = (const F4*)(b + r->off32);
       d.out = (double*)(b + r->off_field);
       d.start = r->next;
       const int64_t left = r->voxels - r->next;
@@ -492,12 +493,12 @@ qj_status qj_server_submit(qj_server* s, const qj_request* req, uint64_t* id) {
       y.z = (float)x.z;
       y.e = (float)x.e;
       y.kind = (x.e == 1.0) ? 1 : (x.e == 2.0 ? 2 : 0);
-      y.pad0 = y.pad1 = 0;
       r->h32.push_back(y);
     }
   }
   r->nt = t.n_roots;
   r->nb = b.n_roots;
+  pack_roots32(r->h32.data(), r->nt, r->nb, r->blk32);
   qj_rational f;
   f.nt = r->nt;
   f.nb = r->nb;
@@ -509,7 +510,7 @@ qj_status qj_server_submit(qj_server* s, const qj_request* req, uint64_t* id) {
   r->voxels = (int64_t)r->grid.res[0] * r->grid.res[1] * r->grid.res[2];
   r->chunks = (r->voxels + s->cfg.chunk_voxels - 1) / s->cfg.chunk_voxels;
   r->bytes = (((size_t)r->voxels * sizeof(double) + 255) & ~(size_t)255) +
-             ((r->h64.size() * sizeof(RootRec64) + 255) & ~(size_t)255) + r->h32.size() * sizeof(RootRec32);
+             ((r->h64.size() * sizeof(RootRec64) + 255) & ~(size_t)255) + r->blk32.size();
   r->id = s->next_id.fetch_add(1);
   if (id) *id = r->id;
   {
