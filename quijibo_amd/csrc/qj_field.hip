@@ -196,7 +196,13 @@ struct Consts32 {
 // Largest-component power-of-two renormalisation: exact, no rounding.
 // Returns the applied exponent (0 for 0 / inf / nan).
 __device__ __forceinline__ int renorm32(q32& t, int& E) {
-  const float mx = fmaxf(fmaxf(fabsf(t.w), fabsf(t.x)), fmaxf(fabsf(t.y), fabsf(t.z)));
+  // max |component| in two instructions (fmaxf would canonicalise each input
+  // first); a NaN component is dropped, and a NaN product is flagged by the
+  // caller through its magnitude
+  float mx;
+  asm("v_max3_f32 %0, |%1|, |%2|, |%3|\n\tv_max_f32_e64 %0, %0, |%4|"
+      : "=&v"(mx)
+      : "v"(t.w), "v"(t.x), "v"(t.y), "v"(t.z));
   const int k = __builtin_amdgcn_frexp_expf(mx);
   t.w = __builtin_amdgcn_ldexpf(t.w, -k);
   t.x = __builtin_amdgcn_ldexpf(t.x, -k);
