@@ -184,14 +184,6 @@ __device__ __forceinline__ Res64 eval64x(const MapArgs& a, const RootRec64* __re
 
 // ---------------------------------------------------------------- FP32 bulk
 
-struct Consts32 {
-  float log2c;     // log2(exp_scaling)
-  float log2esc;   // log2(escape)
-  float log2bh;    // log2(10 FLT_MIN)
-  float margin;    // decision margin in log2 units
-  float band;
-  float term_bits;  // bound on log2 |term| between renormalisations
-};
 
 // Largest-component power-of-two renormalisation: exact, no rounding.
 // Returns the applied exponent (0 for 0 / inf / nan).
@@ -250,7 +242,8 @@ __device__ __forceinline__ q32 kind2(const q32& b) {
 
 // First term of a product; kb: the root's pattern bits (kPatSquare,
 // kPatGeneral).
-__device__ __forceinline__ q32 term32(const F4& r, int kb, const float* __restrict__ e, const q32& q, float& L,
+template <class EP>
+__device__ __forceinline__ q32 term32(const F4& r, int kb, EP e, const q32& q, float& L,
                                       float& vx) {
   const q32 b = qsub(q, q32{r.w, r.x, r.y, r.z});
   if (kb & kPatGeneral) return general32(*e, b, L);
@@ -262,8 +255,8 @@ __device__ __forceinline__ q32 term32(const F4& r, int kb, const float* __restri
 // branches share no term registers (this file is compiled without
 // SimplifyCFG's common-code sinking, which would merge the products behind
 // register copies).
-template <bool kGeneral>
-__device__ __forceinline__ void apply32(q32& t, const F4& r, int kb, const float* __restrict__ e, const q32& q,
+template <bool kGeneral, class EP>
+__device__ __forceinline__ void apply32(q32& t, const F4& r, int kb, EP e, const q32& q,
                                         float& L, float& vx) {
   const q32 b = qsub(q, q32{r.w, r.x, r.y, r.z});
   if (kGeneral && (kb & kPatGeneral)) {
@@ -282,18 +275,22 @@ __device__ __forceinline__ void apply32(q32& t, const F4& r, int kb, const float
 // group of four can leave FP32's normal range, two cannot). vx: minimum |x| over
 // the e = 1 terms (see kind1). The roots of a group are read before its first
 // term: one scalar load of kRN * 16 bytes and one pattern word.
-template <int kRN, bool kGeneral>
-__device__ __forceinline__ q32 poly32(const F4* __restrict__ rq, const int32_t* __restrict__ rp,
-                                      const float* __restrict__ re, int n, const q32& q, int& E, float& L, float& hi,
+template <class QP>
+__device__ __forceinline__ F4 load_root(QP p) {
+  return F4{p->w, p->x, p->y, p->z};
+}
+
+template <int kRN, bool kGeneral, class QP, class PP, class EP>
+__device__ __forceinline__ q32 poly32(QP rq, PP rp, EP re, int n, const q32& q, int& E, float& L, float& hi,
                                       float& lo, bool& bad, float& vx) {
-  q32 t = term32(rq[0], rp[0], re, q, L, vx);
+  q32 t = term32(load_root(rq), rp[0], re, q, L, vx);
   int i = 1;
   int mn = 0;
   int ehi = -(1 << 30), elo = 1 << 30;
   for (; i + kRN - 1 < n; i += kRN) {
     F4 g[kRN];
 #pragma unroll
-    for (int j = 0; j < kRN; j++) g[j] = rq[i + j];
+    for (int j = 0; j < kRN; j++) g[j] = load_root(rq + i + j);
     const int pat = rp[i];
 #pragma unroll
     for (int j = 0; j < kRN; j++) apply32<kGeneral>(t, g[j], pat >> (2 * j), re + i + j, q, L, vx);
@@ -308,7 +305,7 @@ __device__ __forceinline__ q32 poly32(const F4* __restrict__ rq, const int32_t* 
       elo = min(elo, E);
     }
   }
-  for (; i < n; i++) apply32<kGeneral>(t, rq[i], rp[i], re + i, q, L, vx);
+  for (; i < n; i++) apply32<kGeneral>(t, load_root(rq + i), rp[i], re + i, q, L, vx);
   const int k = renorm32(t, E);
   if (kRN > 2) bad |= min(mn, k) < -60;
   if (kGeneral) {
@@ -341,8 +338,8 @@ struct Res32 {
 // |B| > 2^512.5 means |B|^2 overflows in FP64, B^-1 = conj(B) * 0 and q = 0
 // exactly (F = -inf under the black-hole stop); anything closer to a range
 // edge, or outside it, goes to FP64.
-template <int kRN, bool kGeneral>
-__device__ __forceinline__ Res32 eval32(const MapArgs& a, const Roots32& roots, const Consts32& k,
+template <int kRN, bool kGeneral, class RootsT>
+__device__ __forceinline__ Res32 eval32(const MapArgs& a, const RootsT& roots, const Consts32& k,
                                         const q64& q0) {
   Res32 res{0.0f, 0, true, false};
   // the first escape / origin test in FP64, exactly as the reference
@@ -558,7 +555,22 @@ __device__ __forceinline__ int next_tile(uint32_t* queue) {
   __syncthreads();  // the previous tile's readers are done with s_tile
   if (threadIdx.x == 0) s_tile = (int)atomicAdd(queue, 1u);
   __syncthreads();
-  return s_tile;
+  return __builtin_amdgcn_readfirstlane(s_tile);  // uniform: descriptor reads become scalar loads
+}
+
+// A root block read through the constant address space: its loads are known
+// not to be clobbered by the kernel's stores, so a block-uniform address gives
+// scalar loads even when the pointer itself was read from memory.
+struct Roots32K {
+  const __attribute__((address_space(4))) F4* q;
+  const __attribute__((address_space(4))) int32_t* pat;
+  const __attribute__((address_space(4))) float* e;
+};
+__device__ __forceinline__ Roots32K roots32_const(const F4* blk, int n) {
+  const Roots32 r = roots32_view(blk, n);
+  return Roots32K{(const __attribute__((address_space(4))) F4*)r.q,
+                  (const __attribute__((address_space(4))) int32_t*)r.pat,
+                  (const __attribute__((address_space(4))) float*)r.e};
 }
 
 // Batch counters (uint32): [0] band count of integer-exponent chunks (list from
@@ -586,15 +598,9 @@ __global__ __launch_bounds__(kBlock) void k_batch32(const ChunkDesc* __restrict_
     bool need = false;
     if (local < d.count) {
       const MapArgs& a = d.a;
-      Consts32 k;
-      k.log2c = (float)log2(a.exp_scaling);
-      k.log2esc = (float)log2(a.escape);
-      k.log2bh = (float)log2(kBlackHole);
-      k.margin = 0.02f;
-      k.band = d.band;
-      k.term_bits = (float)a.term_bits;
+      const Consts32 k = d.k;
       const int64_t idx = d.start + local;
-      const Res32 r = eval32<kRN, kGen>(a, roots32_view(d.r32, a.nt + a.nb), k, chunk_start(a, idx));
+      const Res32 r = eval32<kRN, kGen>(a, roots32_const(d.r32, a.nt + a.nb), k, chunk_start(a, idx));
       if (r.written) {
         if (!r.need64) d.out[idx] = (double)r.F;
         need = r.need64;
@@ -718,7 +724,7 @@ qj_status launch_batch(const ChunkDesc* d_desc, int nchunks, int chunk_voxels, q
 
 // ---------------------------------------------------------------- launchers
 
-static Consts32 consts32(const MapArgs& a, double band) {
+Consts32 consts32(const MapArgs& a, double band) {
   Consts32 k;
   k.log2c = (float)log2(a.exp_scaling);
   k.log2esc = (float)log2(a.escape);

@@ -145,6 +145,17 @@ qj_status launch_field(qj_workspace* ws, const MapArgs& a, const qj_rational* f,
 qj_status launch_points(qj_workspace* ws, const MapArgs& a, const qj_rational* f, qj_precision prec,
                         double band, const double* d_pts, int64_t n, double* d_out, hipStream_t s);
 
+// Per-map constants of the FP32 kernels (log2 units).
+struct Consts32 {
+  float log2c;      // log2(exp_scaling)
+  float log2esc;    // log2(escape)
+  float log2bh;     // log2(10 FLT_MIN)
+  float margin;     // decision margin in log2 units
+  float band;
+  float term_bits;  // bound on log2 |term| between renormalisations
+};
+Consts32 consts32(const MapArgs& a, double band);
+
 // batched chunks for the request loop (qj_field.hip)
 struct ChunkDesc {
   MapArgs a;
@@ -153,7 +164,7 @@ struct ChunkDesc {
   double* out;
   int64_t start;  // first voxel (linear, x fastest) of this chunk in its request
   int32_t count;  // voxels in this chunk (0 = unused slot)
-  float band;
+  Consts32 k;     // FP32 constants of the chunk's request (consts32)
   int32_t total;  // chunks in use in this launch (read from the first descriptor)
   int32_t g1, g2;  // variant groups (first descriptor): [0,g1) general, [g1,g2) RN4 integer, [g2,total) RN2 integer
 };

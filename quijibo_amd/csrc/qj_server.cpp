@@ -102,6 +102,7 @@ struct Req {
   bool admitted = false;
   qj_status status = QJ_OK;
   MapArgs args{};
+  Consts32 k32{};
 };
 
 struct Slot {
@@ -217,7 +218,7 @@ static int build_batch(qj_server* s, Slot& sl) {
       d.start = r->next;
       const int64_t left = r->voxels - r->next;
       d.count = (int32_t)(left < cv ? left : cv);
-      d.band = (float)s->cfg.band;
+      d.k = r->k32;
       r->next += d.count;
       work += per_voxel * d.count;
       taken++;
@@ -507,6 +508,7 @@ qj_status qj_server_submit(qj_server* s, const qj_request* req, uint64_t* id) {
   f.h64 = r->h64;
   qj_status st = make_map_args(&r->grid, &f, &r->params, &r->args);
   if (st) return st;
+  r->k32 = consts32(r->args, s->cfg.band);
   r->voxels = (int64_t)r->grid.res[0] * r->grid.res[1] * r->grid.res[2];
   r->chunks = (r->voxels + s->cfg.chunk_voxels - 1) / s->cfg.chunk_voxels;
   r->bytes = (((size_t)r->voxels * sizeof(double) + 255) & ~(size_t)255) +
