@@ -74,6 +74,7 @@ def main():
     ap.add_argument("--cpu-slabs", type=int, default=64)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-c3", action="store_true", help="skip the 1024^3 tiled section")
+    ap.add_argument("--no-grad", action="store_true", help="skip the gradient section")
     ap.add_argument("--band", type=float, default=0.05)
     ap.add_argument("--serve", type=int, default=2000, help="requests in the serving stream (0 = skip)")
     ap.add_argument("--serve-rate", type=float, default=200.0)
@@ -193,6 +194,7 @@ def main():
                                     "sign_flips_fp32band_gpu": cb["flips_fp32band"],
                                     "max_rel_err_fp64_gpu": cb["max_rel_err_fp64"]}}
     c3 = None if args.no_c3 else c3_tiles(args, log)
+    grads = None if args.no_grad else gradients(args, log)
     serving = None
     if args.serve > 0:
         serving = serve_stream(args, log)
@@ -222,6 +224,7 @@ def main():
         "roofline": roofline,
         "cpu_baseline": cpu,
         "c3_1024_tiles": c3,
+        "gradients": grads,
         "serving": serving,
         "p50_ms_256": (serving or {}).get("latency", {}).get("256^3", {}).get("p50_ms") if serving else None,
         "p99_ms_256": (serving or {}).get("latency", {}).get("256^3", {}).get("p99_ms") if serving else None,
@@ -253,6 +256,60 @@ def serve_stream(args, log):
                       "iterations, %s launches in flight, fp32+fp64 band, shortest-remaining-work-first"
                       % (args.serve_budget_gib, ("%.1e" % args.serve_launch_work) if args.serve_launch_work else "3e9",
                          args.serve_inflight or 2)}
+
+
+def gradients(args, log):
+    """Config 5 (40k points: 20k on a superquadric, 20k at distance 0.2 outside; 64+64 roots,
+    maxIter 3): forward field and the reverse-mode gradient over all 512 root components and
+    128 powers, FP64 and FP32. The AP pass (one iteration, escape 20, skip-origin and extended
+    range) with its score and exponent gradients on a 97^3 grid of the 32+32 headline roots."""
+    import torch
+    from quijibo_amd import api, grad as G, workloads as W
+    from quijibo_amd._lib import FP32_BAND, FP64, MAP_BLACK_HOLE, MAP_ADAPTIVE, MAP_SKIP_ORIGIN
+    n = 20000
+    w = W.config5(n_surface=n, n_exterior=n, max_iter=3)
+    pts = torch.from_numpy(np.concatenate([w.extra["surface"], w.extra["exterior"]])).cuda()
+    p = api.make_params(w.max_iter, 2000.0, 1.0, 0.0, MAP_BLACK_HOLE)
+    r = api.Rational(api.POLYNOMIAL_4D(w.top.roots, w.top.powers), api.POLYNOMIAL_4D(w.bottom.roots, w.bottom.powers))
+    ws = api.Workspace()
+    reps = 10
+    out = {}
+    for name, prec in (("fp64", FP64), ("fp32", FP32_BAND)):
+        F = api.points_eval(pts, r, p, prec, workspace=ws)
+        seed = G.superquadric_objective_seed(F, n)
+        api.points_adjoint(pts, r, p, seed, prec, workspace=ws)  # warm-up
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            F = api.points_eval(pts, r, p, prec, workspace=ws)
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        for _ in range(reps):
+            api.points_adjoint(pts, r, p, seed, prec, workspace=ws)
+        torch.cuda.synchronize()
+        t2 = time.perf_counter()
+        out["c5_" + name] = {"forward_ms": round((t1 - t0) / reps * 1e3, 3),
+                             "adjoint_ms": round((t2 - t1) / reps * 1e3, 3)}
+    log("gradients: c5 %s" % out)
+    h = W.headline(97)
+    res = (97, 97, 97)
+    g = api.make_grid(res, h.center, h.lengths)
+    ax = [-h.lengths[i] / 2 + (np.arange(res[i]) + 0.5) * h.lengths[i] / res[i] for i in range(3)]
+    Zc, Yc, Xc = np.meshgrid(ax[2], ax[1], ax[0], indexing="ij")
+    weight = torch.from_numpy((1.0 / (np.abs(np.sqrt(Xc ** 2 + Yc ** 2 + Zc ** 2) - 0.8) + 0.05)).reshape(-1)).cuda()
+    pa = api.make_params(1, 20.0, h.exp_scaling, 0.0, MAP_SKIP_ORIGIN | MAP_ADAPTIVE)
+    ra = api.Rational(api.POLYNOMIAL_4D(h.top.roots, h.top.powers), api.POLYNOMIAL_4D(h.bottom.roots, h.bottom.powers))
+    api.ap_score_gradients(g, ra, pa, g, weight, workspace=ws)  # warm-up
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        res_ap = api.ap_score_gradients(g, ra, pa, g, weight, workspace=ws)
+    torch.cuda.synchronize()
+    out["ap_97"] = {"ms": round((time.perf_counter() - t0) / reps * 1e3, 3), "score": res_ap["score"],
+                    "adaptive_cells": res_ap["adaptive_cells"]}
+    out["workload"] = ("c5: 40000 points, 64+64 roots, maxIter 3, gradient over 512 root components + 128 powers; "
+                       "ap_97: 97^3 grid, 32+32 roots, one iteration, escape 20, score + exponent gradients")
+    return out
 
 
 def c3_tiles(args, log):
