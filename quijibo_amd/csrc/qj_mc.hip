@@ -50,6 +50,28 @@ constexpr McTables make_tables() {
 
 __constant__ McTables kMc = make_tables();
 
+// Case tables staged in LDS for the kernels that look them up per lane: with
+// divergent indices, reads of __constant__ memory are scattered vector loads
+// (12 per grid point in the edge kernels); from LDS they are ds_reads.
+static_assert(kMcBlock == 256, "one table entry per thread");
+
+__device__ __forceinline__ void stage_emask(unsigned short* s_em) {
+  s_em[threadIdx.x] = kMc.emask[threadIdx.x];
+  __syncthreads();
+}
+
+// s_tri: the 256 x 16 triangle table as 1024 words; s_ntri: triangles per case
+__device__ __forceinline__ void stage_tri(uint32_t* s_tri, uint8_t* s_ntri) {
+  const uint32_t* src = (const uint32_t*)&kMc.tri[0][0];
+  for (int i = threadIdx.x; i < 1024; i += kMcBlock) s_tri[i] = src[i];
+  s_ntri[threadIdx.x] = kMc.ntri[threadIdx.x];
+  __syncthreads();
+}
+
+__device__ __forceinline__ int tri_edge(const uint32_t* s_tri, int flag, int k) {
+  return ((const signed char*)s_tri)[flag * 16 + k];
+}
+
 // edge id -> lower endpoint offset (dx,dy,dz) and axis
 __device__ __forceinline__ void edge_geom(int e, int& dx, int& dy, int& dz, int& axis) {
   if (e < 4) {
@@ -144,6 +166,8 @@ __global__ __launch_bounds__(kMcBlock) void k_mc_active(const uint8_t* __restric
 
 __global__ __launch_bounds__(kMcBlock) void k_mc_edges(const uint8_t* __restrict__ flags, GridGeom g,
                                                        uint8_t* __restrict__ emask, uint32_t* __restrict__ cnt) {
+  __shared__ unsigned short s_em[256];
+  stage_emask(s_em);
   const int64_t X = g.res[0], Y = g.res[1], Z = g.res[2], XY = X * Y, N = XY * Z;
   const int64_t p = (int64_t)blockIdx.x * kMcBlock + threadIdx.x;
   if (p >= N) return;
@@ -154,7 +178,7 @@ __global__ __launch_bounds__(kMcBlock) void k_mc_edges(const uint8_t* __restrict
     unsigned ref = 0;
     for (int b = 0; b < 2; b++)
       for (int a = 0; a < 2; a++)
-        if (y >= a && z >= b) ref |= (kMc.emask[flags[p - a * X - b * XY]] >> (a + 2 * b)) & 1u;
+        if (y >= a && z >= b) ref |= (s_em[flags[p - a * X - b * XY]] >> (a + 2 * b)) & 1u;
     bits |= ref;
   }
   // y-edge (p, p+X): cells p - (a,0,b); local id 4 + a + 2b
@@ -162,7 +186,7 @@ __global__ __launch_bounds__(kMcBlock) void k_mc_edges(const uint8_t* __restrict
     unsigned ref = 0;
     for (int b = 0; b < 2; b++)
       for (int a = 0; a < 2; a++)
-        if (x >= a && z >= b) ref |= (kMc.emask[flags[p - a - b * XY]] >> (4 + a + 2 * b)) & 1u;
+        if (x >= a && z >= b) ref |= (s_em[flags[p - a - b * XY]] >> (4 + a + 2 * b)) & 1u;
     bits |= ref << 1;
   }
   // z-edge (p, p+XY): cells p - (a,b,0); local id 8 + a + 2b
@@ -170,7 +194,7 @@ __global__ __launch_bounds__(kMcBlock) void k_mc_edges(const uint8_t* __restrict
     unsigned ref = 0;
     for (int b = 0; b < 2; b++)
       for (int a = 0; a < 2; a++)
-        if (x >= a && y >= b) ref |= (kMc.emask[flags[p - a - b * X]] >> (8 + a + 2 * b)) & 1u;
+        if (x >= a && y >= b) ref |= (s_em[flags[p - a - b * X]] >> (8 + a + 2 * b)) & 1u;
     bits |= ref << 2;
   }
   emask[p] = (uint8_t)bits;
@@ -230,18 +254,21 @@ __global__ __launch_bounds__(kMcBlock) void k_mc_triangles(const uint32_t* __res
                                                            const uint32_t* __restrict__ toff,
                                                            int32_t* __restrict__ tris_pre,
                                                            uint32_t* __restrict__ keep) {
+  __shared__ uint32_t s_tri[1024];
+  __shared__ uint8_t s_ntri[256];
+  stage_tri(s_tri, s_ntri);
   const int64_t i = (int64_t)blockIdx.x * kMcBlock + threadIdx.x;
   if (i >= na) return;
   const int64_t X = g.res[0], XY = X * g.res[1];
   const int64_t p = active[i];
   const int flag = flags[p];
-  const int n = kMc.ntri[flag];
+  const int n = s_ntri[flag];
   uint32_t o = toff[i];
   for (int t = 0; t < n; t++) {
     int32_t v[3];
     for (int c = 0; c < 3; c++) {
       int dx, dy, dz, axis;
-      edge_geom(kMc.tri[flag][3 * t + c], dx, dy, dz, axis);
+      edge_geom(tri_edge(s_tri, flag, 3 * t + c), dx, dy, dz, axis);
       const int64_t owner = p + dx + dy * X + dz * XY;
       v[c] = (int32_t)(vbase[owner] + __popc(emask[owner] & ((1u << axis) - 1u)));
     }
@@ -477,6 +504,8 @@ __global__ __launch_bounds__(kMcBlock) void k_nl_flags(const double* __restrict_
 __global__ __launch_bounds__(kMcBlock) void k_nl_edges(const uint8_t* __restrict__ flags, int64_t X, int64_t Y,
                                                        int64_t Z, int64_t s0, int64_t p0, int64_t np,
                                                        uint8_t* __restrict__ emask, uint32_t* __restrict__ cnt) {
+  __shared__ unsigned short s_em[256];
+  stage_emask(s_em);
   const int64_t XY = X * Y, N = XY * np;
   const int64_t j = (int64_t)blockIdx.x * kMcBlock + threadIdx.x;
   if (j >= N) return;
@@ -488,21 +517,21 @@ __global__ __launch_bounds__(kMcBlock) void k_nl_edges(const uint8_t* __restrict
     unsigned ref = 0;
     for (int b = 0; b < 2; b++)
       for (int a = 0; a < 2; a++)
-        if (y >= a && z - b >= s0) ref |= (kMc.emask[flags[f - a * X - b * XY]] >> (a + 2 * b)) & 1u;
+        if (y >= a && z - b >= s0) ref |= (s_em[flags[f - a * X - b * XY]] >> (a + 2 * b)) & 1u;
     bits |= ref;
   }
   if (y < Y - 1) {
     unsigned ref = 0;
     for (int b = 0; b < 2; b++)
       for (int a = 0; a < 2; a++)
-        if (x >= a && z - b >= s0) ref |= (kMc.emask[flags[f - a - b * XY]] >> (4 + a + 2 * b)) & 1u;
+        if (x >= a && z - b >= s0) ref |= (s_em[flags[f - a - b * XY]] >> (4 + a + 2 * b)) & 1u;
     bits |= ref << 1;
   }
   if (z < Z - 1) {
     unsigned ref = 0;
     for (int b = 0; b < 2; b++)
       for (int a = 0; a < 2; a++)
-        if (x >= a && y >= b) ref |= (kMc.emask[flags[f - a - b * X]] >> (8 + a + 2 * b)) & 1u;
+        if (x >= a && y >= b) ref |= (s_em[flags[f - a - b * X]] >> (8 + a + 2 * b)) & 1u;
     bits |= ref << 2;
   }
   emask[j] = (uint8_t)bits;
@@ -646,17 +675,20 @@ __global__ __launch_bounds__(kMcBlock) void k_nl_triangles(
     uint32_t base, const uint8_t* __restrict__ emask_next, const uint32_t* __restrict__ slice_off,
     uint32_t base_next, const double* __restrict__ verts, const uint32_t* __restrict__ toff,
     int32_t* __restrict__ tris_pre, uint32_t* __restrict__ keep) {
+  __shared__ uint32_t s_tri[1024];
+  __shared__ uint8_t s_ntri[256];
+  stage_tri(s_tri, s_ntri);
   const int64_t i = (int64_t)blockIdx.x * kMcBlock + threadIdx.x;
   if (i >= na) return;
   const int64_t c = active[i];  // chunk-local cell index: (z - z0) * XY + y * X + x
   const int flag = flags[fo + c];
-  const int n = kMc.ntri[flag];
+  const int n = s_ntri[flag];
   uint32_t o = toff[i];
   for (int t = 0; t < n; t++) {
     int32_t v[3];
     for (int q = 0; q < 3; q++) {
       int dx, dy, dz, axis;
-      edge_geom(kMc.tri[flag][3 * t + q], dx, dy, dz, axis);
+      edge_geom(tri_edge(s_tri, flag, 3 * t + q), dx, dy, dz, axis);
       const int64_t owner = c + dx + dy * X + dz * XY;  // chunk-local point index
       const unsigned lowbits = (1u << axis) - 1u;
       if (z0 + owner / XY < z1) {
