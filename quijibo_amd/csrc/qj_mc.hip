@@ -6,14 +6,19 @@
 // Pipeline (all HBM-bound; one lane per grid point / cell):
 //   k_mc_flags      cell case from the float sign test, per-block active count (wave ballot)
 //   scan            block counts -> block offsets (order preserving)
-//   k_mc_active     ballot-compacted list of active cells, in cell order
+//   k_mc_active     ballot-compacted list of active cells, in cell order, with
+//                   each cell's triangle count
 //   k_mc_edges      per grid point: which of its +x,+y,+z edges some adjacent
-//                   cell's case row references (gather of <= 7 neighbour flags)
-//   scan            edge counts -> vertex ids (the reference numbers vertices by
-//                   std::map<pair<int,int>> order = lower endpoint, then x<y<z)
-//   k_mc_vertices   p0 + d0/(d0-d1) (p1-p0) at cell centres, no contraction
-//   k_mc_tricount / scan / k_mc_triangles / scan / k_mc_squeeze: triangles in
-//                   cell then table order, degenerate ones (edge < 1e-7) dropped
+//                   cell's case row references (gather of <= 7 neighbour flags);
+//                   per block of points: vertex count
+//   scan            block vertex counts -> first vertex id per block (the
+//                   reference numbers vertices by std::map<pair<int,int>> order
+//                   = lower endpoint, then x<y<z); triangle counts -> offsets
+//   (one read-back of the three totals)
+//   k_mc_vertices   ids within the block, p0 + d0/(d0-d1) (p1-p0) at cell
+//                   centres, no contraction
+//   k_mc_triangles / scan / k_mc_squeeze: triangles in cell then table order,
+//                   degenerate ones (edge < 1e-7) dropped
 #include <math.h>
 
 #include <vector>
@@ -88,7 +93,10 @@ struct McState {
   int64_t cap_v = 0, cap_tp = 0, cap_keep = 0, cap_t = 0;
   uint8_t* flags = nullptr;
   uint8_t* emask = nullptr;
-  uint32_t* cnt = nullptr;     // per point edge count -> vertex base (scanned in place)
+  uint32_t* cnt = nullptr;     // per point edge count -> vertex base (scanned in place; non-linear mesher)
+  uint16_t* vb16 = nullptr;    // per point first vertex id within its block (linear mesher)
+  uint32_t* blkv = nullptr;    // per block vertex count -> first vertex id of the block
+  int64_t cap_vb16 = 0, cap_blkv = 0;
   uint32_t* blk = nullptr;     // per block active count -> offset
   uint32_t* active = nullptr;  // active cells
   uint32_t* tri_off = nullptr; // per active cell triangle count -> offset
@@ -149,74 +157,115 @@ __global__ __launch_bounds__(kMcBlock) void k_mc_flags(const double* __restrict_
   }
 }
 
+// tcount (optional): triangles of each active cell, at the cell's list position
 __global__ __launch_bounds__(kMcBlock) void k_mc_active(const uint8_t* __restrict__ flags, int64_t N,
                                                         const uint32_t* __restrict__ blk_off,
-                                                        uint32_t* __restrict__ active) {
+                                                        uint32_t* __restrict__ active, uint32_t* __restrict__ tcount) {
   __shared__ uint32_t s_n[kMcBlock / 64];
   const int64_t p = (int64_t)blockIdx.x * kMcBlock + threadIdx.x;
-  const bool act = p < N && kMc.ntri[flags[p]] > 0;
+  const int nt = p < N ? kMc.ntri[flags[p]] : 0;
+  const bool act = nt > 0;
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const unsigned long long m = __ballot(act);
   if (lane == 0) s_n[wave] = (uint32_t)__popcll(m);
   __syncthreads();
   uint32_t off = blk_off[blockIdx.x];
   for (int i = 0; i < wave; i++) off += s_n[i];
-  if (act) active[off + __popcll(m & ((1ull << lane) - 1ull))] = (uint32_t)p;
+  if (act) {
+    const uint32_t k = off + __popcll(m & ((1ull << lane) - 1ull));
+    active[k] = (uint32_t)p;
+    if (tcount) tcount[k] = (uint32_t)nt;
+  }
 }
 
+// Sum over the block of one value per thread (every thread gets it).
+__device__ __forceinline__ uint32_t mc_block_sum(uint32_t v) {
+  __shared__ uint32_t s_w[kMcBlock / 64];
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
+  if ((threadIdx.x & 63) == 0) s_w[threadIdx.x >> 6] = v;
+  __syncthreads();
+  uint32_t t = 0;
+  for (int i = 0; i < kMcBlock / 64; i++) t += s_w[i];
+  return t;
+}
+
+// Exclusive prefix over the block of one value per thread.
+__device__ __forceinline__ uint32_t mc_block_exclusive(uint32_t v) {
+  __shared__ uint32_t s_w[kMcBlock / 64];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  uint32_t x = v;
+  for (int o = 1; o < 64; o <<= 1) {
+    const uint32_t y = __shfl_up(x, o);
+    if (lane >= o) x += y;
+  }
+  if (lane == 63) s_w[wave] = x;
+  __syncthreads();
+  uint32_t before = 0;
+  for (int i = 0; i < wave; i++) before += s_w[i];
+  return before + x - v;
+}
+
+// Edge masks (bit axis set when the +axis edge from the point carries a vertex)
+// and the vertex count of each block of points.
 __global__ __launch_bounds__(kMcBlock) void k_mc_edges(const uint8_t* __restrict__ flags, GridGeom g,
-                                                       uint8_t* __restrict__ emask, uint32_t* __restrict__ cnt) {
+                                                       uint8_t* __restrict__ emask, uint32_t* __restrict__ blkv) {
   __shared__ unsigned short s_em[256];
   stage_emask(s_em);
   const int64_t X = g.res[0], Y = g.res[1], Z = g.res[2], XY = X * Y, N = XY * Z;
   const int64_t p = (int64_t)blockIdx.x * kMcBlock + threadIdx.x;
-  if (p >= N) return;
-  const int64_t z = p / XY, r = p - z * XY, y = r / X, x = r - y * X;
   unsigned bits = 0;
-  // x-edge (p, p+1): cells p - (0,a,b); local id a + 2b
-  if (x < X - 1) {
-    unsigned ref = 0;
-    for (int b = 0; b < 2; b++)
-      for (int a = 0; a < 2; a++)
-        if (y >= a && z >= b) ref |= (s_em[flags[p - a * X - b * XY]] >> (a + 2 * b)) & 1u;
-    bits |= ref;
+  if (p < N) {
+    const int64_t z = p / XY, r = p - z * XY, y = r / X, x = r - y * X;
+    // x-edge (p, p+1): cells p - (0,a,b); local id a + 2b
+    if (x < X - 1) {
+      unsigned ref = 0;
+      for (int b = 0; b < 2; b++)
+        for (int a = 0; a < 2; a++)
+          if (y >= a && z >= b) ref |= (s_em[flags[p - a * X - b * XY]] >> (a + 2 * b)) & 1u;
+      bits |= ref;
+    }
+    // y-edge (p, p+X): cells p - (a,0,b); local id 4 + a + 2b
+    if (y < Y - 1) {
+      unsigned ref = 0;
+      for (int b = 0; b < 2; b++)
+        for (int a = 0; a < 2; a++)
+          if (x >= a && z >= b) ref |= (s_em[flags[p - a - b * XY]] >> (4 + a + 2 * b)) & 1u;
+      bits |= ref << 1;
+    }
+    // z-edge (p, p+XY): cells p - (a,b,0); local id 8 + a + 2b
+    if (z < Z - 1) {
+      unsigned ref = 0;
+      for (int b = 0; b < 2; b++)
+        for (int a = 0; a < 2; a++)
+          if (x >= a && y >= b) ref |= (s_em[flags[p - a - b * X]] >> (8 + a + 2 * b)) & 1u;
+      bits |= ref << 2;
+    }
+    emask[p] = (uint8_t)bits;
   }
-  // y-edge (p, p+X): cells p - (a,0,b); local id 4 + a + 2b
-  if (y < Y - 1) {
-    unsigned ref = 0;
-    for (int b = 0; b < 2; b++)
-      for (int a = 0; a < 2; a++)
-        if (x >= a && z >= b) ref |= (s_em[flags[p - a - b * XY]] >> (4 + a + 2 * b)) & 1u;
-    bits |= ref << 1;
-  }
-  // z-edge (p, p+XY): cells p - (a,b,0); local id 8 + a + 2b
-  if (z < Z - 1) {
-    unsigned ref = 0;
-    for (int b = 0; b < 2; b++)
-      for (int a = 0; a < 2; a++)
-        if (x >= a && y >= b) ref |= (s_em[flags[p - a - b * X]] >> (8 + a + 2 * b)) & 1u;
-    bits |= ref << 2;
-  }
-  emask[p] = (uint8_t)bits;
-  cnt[p] = (bits & 1u) + ((bits >> 1) & 1u) + ((bits >> 2) & 1u);
+  const uint32_t t = mc_block_sum((uint32_t)__popc(bits));
+  if (threadIdx.x == 0) blkv[blockIdx.x] = t;
 }
 
-// computeEdgeInterpolations (TRIANGLE_MESH.cpp:2373-2425)
+// computeEdgeInterpolations (TRIANGLE_MESH.cpp:2373-2425). Vertex ids follow
+// the point order (the block's first id from blk_off, then the prefix within
+// the block, which is also kept in vb16 for the triangle kernel).
 __global__ __launch_bounds__(kMcBlock) void k_mc_vertices(const double* __restrict__ field, GridGeom g, double iso,
                                                           const uint8_t* __restrict__ emask,
-                                                          const uint32_t* __restrict__ vbase,
-                                                          double* __restrict__ verts) {
+                                                          const uint32_t* __restrict__ blk_off,
+                                                          uint16_t* __restrict__ vb16, double* __restrict__ verts) {
 #pragma clang fp contract(off)
   const int64_t X = g.res[0], XY = X * g.res[1], N = XY * g.res[2];
   const int64_t p = (int64_t)blockIdx.x * kMcBlock + threadIdx.x;
+  const unsigned bits = p < N ? emask[p] : 0u;
+  const uint32_t local = mc_block_exclusive((uint32_t)__popc(bits));
   if (p >= N) return;
-  const unsigned bits = emask[p];
+  vb16[p] = (uint16_t)local;
   if (!bits) return;
   const int z = (int)(p / XY), y = (int)((p - (int64_t)z * XY) / X), x = (int)(p - (int64_t)z * XY - (int64_t)y * X);
   double a[3];
   cell_center(g, x, y, z, a);
   const double d0 = field[p] - iso;
-  uint32_t k = vbase[p];
+  uint32_t k = blk_off[blockIdx.x] + local;
   for (int axis = 0; axis < 3; axis++) {
     if (!(bits & (1u << axis))) continue;
     const int x2 = x + (axis == 0), y2 = y + (axis == 1), z2 = z + (axis == 2);
@@ -232,13 +281,6 @@ __global__ __launch_bounds__(kMcBlock) void k_mc_vertices(const double* __restri
   }
 }
 
-__global__ __launch_bounds__(kMcBlock) void k_mc_tricount(const uint32_t* __restrict__ active, int64_t na,
-                                                          const uint8_t* __restrict__ flags,
-                                                          uint32_t* __restrict__ tcount) {
-  const int64_t i = (int64_t)blockIdx.x * kMcBlock + threadIdx.x;
-  if (i < na) tcount[i] = kMc.ntri[flags[active[i]]];
-}
-
 __device__ __forceinline__ double edge_len(const double* a, const double* b) {
 #pragma clang fp contract(off)
   const double e0 = a[0] - b[0], e1 = a[1] - b[1], e2 = a[2] - b[2];
@@ -249,7 +291,8 @@ __device__ __forceinline__ double edge_len(const double* a, const double* b) {
 __global__ __launch_bounds__(kMcBlock) void k_mc_triangles(const uint32_t* __restrict__ active, int64_t na,
                                                            const uint8_t* __restrict__ flags, GridGeom g,
                                                            const uint8_t* __restrict__ emask,
-                                                           const uint32_t* __restrict__ vbase,
+                                                           const uint32_t* __restrict__ blk_off,
+                                                           const uint16_t* __restrict__ vb16,
                                                            const double* __restrict__ verts,
                                                            const uint32_t* __restrict__ toff,
                                                            int32_t* __restrict__ tris_pre,
@@ -270,7 +313,7 @@ __global__ __launch_bounds__(kMcBlock) void k_mc_triangles(const uint32_t* __res
       int dx, dy, dz, axis;
       edge_geom(tri_edge(s_tri, flag, 3 * t + c), dx, dy, dz, axis);
       const int64_t owner = p + dx + dy * X + dz * XY;
-      v[c] = (int32_t)(vbase[owner] + __popc(emask[owner] & ((1u << axis) - 1u)));
+      v[c] = (int32_t)(blk_off[owner / kMcBlock] + vb16[owner] + __popc(emask[owner] & ((1u << axis) - 1u)));
     }
     const double eps = 1e-7;
     const double* p0 = verts + 3 * (int64_t)v[0];
@@ -340,8 +383,9 @@ qj_status mc_run(qj_workspace* ws, const double* d_field, const qj_grid* grid, d
   qj_status st;
   const int64_t nb = nblocks(N);
   if ((st = grow(m.flags, m.cap_flags, N, s)) || (st = grow(m.emask, m.cap_emask, N, s)) ||
-      (st = grow(m.cnt, m.cap_cnt, N, s)) || (st = grow(m.active, m.cap_active, N, s)) ||
-      (st = grow(m.tri_off, m.cap_toff, N, s)) || (st = grow(m.blk, m.cap_blk, nb, s)))
+      (st = grow(m.vb16, m.cap_vb16, N, s)) || (st = grow(m.active, m.cap_active, N, s)) ||
+      (st = grow(m.tri_off, m.cap_toff, N, s)) || (st = grow(m.blk, m.cap_blk, nb, s)) ||
+      (st = grow(m.blkv, m.cap_blkv, nb, s)))
     return st;
   const size_t need_tmp = scan_tmp_bytes(5 * N);  // the largest scan: pre-filter triangles <= 5 per cell
   if (need_tmp > m.tmp_bytes) {
@@ -356,51 +400,46 @@ qj_status mc_run(qj_workspace* ws, const double* d_field, const qj_grid* grid, d
   }
   if (!m.totals) QJ_HIP(hipMalloc(&m.totals, 4 * sizeof(uint32_t)));
 
+  // counts: active cells (totals[0]), vertices (totals[1]) and pre-filter
+  // triangles (totals[2]), read back together
   hipLaunchKernelGGL(k_mc_flags, dim3((unsigned)nb), dim3(kMcBlock), 0, s, d_field, g, iso, m.flags, m.blk);
   QJ_HIP(hipGetLastError());
   if ((st = scan_u32(m.blk, m.blk, nb, m.totals + 0, m.tmp, m.tmp_bytes, s))) return st;
-  hipLaunchKernelGGL(k_mc_active, dim3((unsigned)nb), dim3(kMcBlock), 0, s, m.flags, N, m.blk, m.active);
+  hipLaunchKernelGGL(k_mc_active, dim3((unsigned)nb), dim3(kMcBlock), 0, s, m.flags, N, m.blk, m.active, m.tri_off);
   QJ_HIP(hipGetLastError());
-  hipLaunchKernelGGL(k_mc_edges, dim3((unsigned)nb), dim3(kMcBlock), 0, s, m.flags, g, m.emask, m.cnt);
+  hipLaunchKernelGGL(k_mc_edges, dim3((unsigned)nb), dim3(kMcBlock), 0, s, m.flags, g, m.emask, m.blkv);
   QJ_HIP(hipGetLastError());
-  if ((st = scan_u32(m.cnt, m.cnt, N, m.totals + 1, m.tmp, m.tmp_bytes, s))) return st;
+  if ((st = scan_u32(m.blkv, m.blkv, nb, m.totals + 1, m.tmp, m.tmp_bytes, s))) return st;
+  if ((st = scan_u32(m.tri_off, m.tri_off, N, m.totals + 2, m.tmp, m.tmp_bytes, s, m.totals + 0))) return st;
   uint32_t h[4];
-  QJ_HIP(hipMemcpyAsync(h, m.totals, 2 * sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+  QJ_HIP(hipMemcpyAsync(h, m.totals, 3 * sizeof(uint32_t), hipMemcpyDeviceToHost, s));
   QJ_HIP(hipStreamSynchronize(s));
-  const int64_t na = h[0], nv = h[1];
-  if ((st = grow(m.verts, m.cap_v, 3 * nv, s))) return st;
+  const int64_t na = h[0], nv = h[1], ntp = h[2];
+  if ((st = grow(m.verts, m.cap_v, 3 * nv, s)) || (st = grow(m.tris_pre, m.cap_tp, 3 * ntp, s)) ||
+      (st = grow(m.keep, m.cap_keep, ntp, s)) || (st = grow(m.tris, m.cap_t, 3 * ntp, s)))
+    return st;
   if (nv > 0) {
-    hipLaunchKernelGGL(k_mc_vertices, dim3((unsigned)nb), dim3(kMcBlock), 0, s, d_field, g, iso, m.emask, m.cnt,
-                       m.verts);
+    hipLaunchKernelGGL(k_mc_vertices, dim3((unsigned)nb), dim3(kMcBlock), 0, s, d_field, g, iso, m.emask, m.blkv,
+                       m.vb16, m.verts);
     QJ_HIP(hipGetLastError());
   }
   m.nv = nv;
-  if (na == 0) {
+  if (na == 0 || ntp == 0) {
     QJ_HIP(hipStreamSynchronize(s));
     *nv_out = nv;
     return QJ_OK;
   }
-  hipLaunchKernelGGL(k_mc_tricount, dim3(nblocks(na)), dim3(kMcBlock), 0, s, m.active, na, m.flags, m.tri_off);
-  QJ_HIP(hipGetLastError());
-  if ((st = scan_u32(m.tri_off, m.tri_off, na, m.totals + 2, m.tmp, m.tmp_bytes, s))) return st;
-  QJ_HIP(hipMemcpyAsync(h + 2, m.totals + 2, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
-  QJ_HIP(hipStreamSynchronize(s));
-  const int64_t ntp = h[2];
-  if ((st = grow(m.tris_pre, m.cap_tp, 3 * ntp, s)) || (st = grow(m.keep, m.cap_keep, ntp, s))) return st;
   hipLaunchKernelGGL(k_mc_triangles, dim3(nblocks(na)), dim3(kMcBlock), 0, s, m.active, na, m.flags, g, m.emask,
-                     m.cnt, m.verts, m.tri_off, m.tris_pre, m.keep);
+                     m.blkv, m.vb16, m.verts, m.tri_off, m.tris_pre, m.keep);
   QJ_HIP(hipGetLastError());
   if ((st = scan_u32(m.keep, m.keep, ntp, m.totals + 3, m.tmp, m.tmp_bytes, s))) return st;
+  // tris holds up to ntp triangles, so the squeeze needs no count from the host
+  hipLaunchKernelGGL(k_mc_squeeze, dim3(nblocks(ntp)), dim3(kMcBlock), 0, s, m.tris_pre, m.keep, ntp, m.totals + 3,
+                     m.tris);
+  QJ_HIP(hipGetLastError());
   QJ_HIP(hipMemcpyAsync(h + 3, m.totals + 3, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
   QJ_HIP(hipStreamSynchronize(s));
   const int64_t nt = h[3];
-  if ((st = grow(m.tris, m.cap_t, 3 * nt, s))) return st;
-  if (nt > 0) {
-    hipLaunchKernelGGL(k_mc_squeeze, dim3(nblocks(ntp)), dim3(kMcBlock), 0, s, m.tris_pre, m.keep, ntp,
-                       m.totals + 3, m.tris);
-    QJ_HIP(hipGetLastError());
-  }
-  QJ_HIP(hipStreamSynchronize(s));
   m.nt = nt;
   *nv_out = nv;
   *nt_out = nt;
@@ -430,6 +469,8 @@ void mc_free(qj_workspace* ws) {
   hipFree(m->flags);
   hipFree(m->emask);
   hipFree(m->cnt);
+  hipFree(m->vb16);
+  hipFree(m->blkv);
   hipFree(m->blk);
   hipFree(m->active);
   hipFree(m->tri_off);
