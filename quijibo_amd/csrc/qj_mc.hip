@@ -88,6 +88,11 @@ __device__ __forceinline__ void edge_geom(int e, int& dx, int& dy, int& dz, int&
   }
 }
 
+__device__ __forceinline__ double edge_len3(double e0, double e1, double e2) {
+#pragma clang fp contract(off)
+  return sqrt(e0 * e0 + e1 * e1 + e2 * e2);
+}
+
 struct McState {
   int64_t cap_flags = 0, cap_emask = 0, cap_cnt = 0, cap_active = 0, cap_toff = 0, cap_blk = 0;
   int64_t cap_v = 0, cap_tp = 0, cap_keep = 0, cap_t = 0;
@@ -95,6 +100,8 @@ struct McState {
   uint8_t* emask = nullptr;
   uint32_t* cnt = nullptr;     // per point edge count -> vertex base (scanned in place; non-linear mesher)
   uint16_t* vb16 = nullptr;    // per point first vertex id within its block (linear mesher)
+  uint8_t* vnear = nullptr;    // per vertex: within kNearEnd of an edge end (or off the edge)
+  int64_t cap_vnear = 0;
   uint32_t* blkv = nullptr;    // per block vertex count -> first vertex id of the block
   int64_t cap_vb16 = 0, cap_blkv = 0;
   uint32_t* blk = nullptr;     // per block active count -> offset
@@ -249,10 +256,23 @@ __global__ __launch_bounds__(kMcBlock) void k_mc_edges(const uint8_t* __restrict
 // computeEdgeInterpolations (TRIANGLE_MESH.cpp:2373-2425). Vertex ids follow
 // the point order (the block's first id from blk_off, then the prefix within
 // the block, which is also kept in vb16 for the triangle kernel).
+//
+// vnear marks vertices closer than kNearEnd to an end of their edge (or not
+// between its ends). Two vertices of one triangle lie on edges of one cell:
+// parallel or skew edges are at least one grid spacing h apart, and on two
+// edges meeting at a corner c the vertices satisfy |v1 - v2| >= max |vi - c|.
+// With h >= kPrefilterSpacing, a triangle with fewer than two marked vertices
+// therefore has every side longer than the 1e-7 of the degenerate test, which
+// the triangle kernel then skips (it reads three bytes instead of nine
+// coordinates); triangles with two or three marked vertices take the test.
+constexpr double kNearEnd = 2e-7;
+constexpr double kPrefilterSpacing = 1e-6;
+
 __global__ __launch_bounds__(kMcBlock) void k_mc_vertices(const double* __restrict__ field, GridGeom g, double iso,
                                                           const uint8_t* __restrict__ emask,
                                                           const uint32_t* __restrict__ blk_off,
-                                                          uint16_t* __restrict__ vb16, double* __restrict__ verts) {
+                                                          uint16_t* __restrict__ vb16, double* __restrict__ verts,
+                                                          uint8_t* __restrict__ vnear) {
 #pragma clang fp contract(off)
   const int64_t X = g.res[0], XY = X * g.res[1], N = XY * g.res[2];
   const int64_t p = (int64_t)blockIdx.x * kMcBlock + threadIdx.x;
@@ -274,9 +294,12 @@ __global__ __launch_bounds__(kMcBlock) void k_mc_vertices(const double* __restri
     const double off0 = b[0] - a[0], off1 = b[1] - a[1], off2 = b[2] - a[2];
     const double d1 = field[p + (axis == 0 ? 1 : (axis == 1 ? X : XY))] - iso;
     const double interpolation = d0 / (d0 - d1);
-    verts[3 * (int64_t)k + 0] = a[0] + interpolation * off0;
-    verts[3 * (int64_t)k + 1] = a[1] + interpolation * off1;
-    verts[3 * (int64_t)k + 2] = a[2] + interpolation * off2;
+    const double v0 = a[0] + interpolation * off0, v1 = a[1] + interpolation * off1, v2 = a[2] + interpolation * off2;
+    verts[3 * (int64_t)k + 0] = v0;
+    verts[3 * (int64_t)k + 1] = v1;
+    verts[3 * (int64_t)k + 2] = v2;
+    const double da = edge_len3(v0 - a[0], v1 - a[1], v2 - a[2]), db = edge_len3(v0 - b[0], v1 - b[1], v2 - b[2]);
+    vnear[k] = (da >= kNearEnd && db >= kNearEnd) ? 0 : 1;
     k++;
   }
 }
@@ -294,6 +317,7 @@ __global__ __launch_bounds__(kMcBlock) void k_mc_triangles(const uint32_t* __res
                                                            const uint32_t* __restrict__ blk_off,
                                                            const uint16_t* __restrict__ vb16,
                                                            const double* __restrict__ verts,
+                                                           const uint8_t* __restrict__ vnear,
                                                            const uint32_t* __restrict__ toff,
                                                            int32_t* __restrict__ tris_pre,
                                                            uint32_t* __restrict__ keep) {
@@ -315,11 +339,14 @@ __global__ __launch_bounds__(kMcBlock) void k_mc_triangles(const uint32_t* __res
       const int64_t owner = p + dx + dy * X + dz * XY;
       v[c] = (int32_t)(blk_off[owner / kMcBlock] + vb16[owner] + __popc(emask[owner] & ((1u << axis) - 1u)));
     }
-    const double eps = 1e-7;
-    const double* p0 = verts + 3 * (int64_t)v[0];
-    const double* p1 = verts + 3 * (int64_t)v[1];
-    const double* p2 = verts + 3 * (int64_t)v[2];
-    const bool degenerate = edge_len(p0, p1) < eps || edge_len(p0, p2) < eps || edge_len(p1, p2) < eps;
+    bool degenerate = false;
+    if (!vnear || (int)vnear[v[0]] + (int)vnear[v[1]] + (int)vnear[v[2]] >= 2) {
+      const double eps = 1e-7;
+      const double* p0 = verts + 3 * (int64_t)v[0];
+      const double* p1 = verts + 3 * (int64_t)v[1];
+      const double* p2 = verts + 3 * (int64_t)v[2];
+      degenerate = edge_len(p0, p1) < eps || edge_len(p0, p2) < eps || edge_len(p1, p2) < eps;
+    }
     tris_pre[3 * (int64_t)o + 0] = v[0];
     tris_pre[3 * (int64_t)o + 1] = v[1];
     tris_pre[3 * (int64_t)o + 2] = v[2];
@@ -415,13 +442,25 @@ qj_status mc_run(qj_workspace* ws, const double* d_field, const qj_grid* grid, d
   QJ_HIP(hipMemcpyAsync(h, m.totals, 3 * sizeof(uint32_t), hipMemcpyDeviceToHost, s));
   QJ_HIP(hipStreamSynchronize(s));
   const int64_t na = h[0], nv = h[1], ntp = h[2];
-  if ((st = grow(m.verts, m.cap_v, 3 * nv, s)) || (st = grow(m.tris_pre, m.cap_tp, 3 * ntp, s)) ||
-      (st = grow(m.keep, m.cap_keep, ntp, s)) || (st = grow(m.tris, m.cap_t, 3 * ntp, s)))
+  if ((st = grow(m.verts, m.cap_v, 3 * nv, s)) || (st = grow(m.vnear, m.cap_vnear, nv, s)) ||
+      (st = grow(m.tris_pre, m.cap_tp, 3 * ntp, s)) || (st = grow(m.keep, m.cap_keep, ntp, s)) ||
+      (st = grow(m.tris, m.cap_t, 3 * ntp, s)))
     return st;
   if (nv > 0) {
     hipLaunchKernelGGL(k_mc_vertices, dim3((unsigned)nb), dim3(kMcBlock), 0, s, d_field, g, iso, m.emask, m.blkv,
-                       m.vb16, m.verts);
+                       m.vb16, m.verts, m.vnear);
     QJ_HIP(hipGetLastError());
+  }
+  // the degenerate-test prefilter holds when the cell-centre map is a rotation
+  // (edges meeting at a corner stay perpendicular) and every grid spacing is at
+  // least kPrefilterSpacing
+  bool prefilter = true;
+  for (int i = 0; i < 3; i++) {
+    prefilter &= fabs(g.dx[i]) >= kPrefilterSpacing;
+    for (int j = 0; j < 3; j++) {
+      const double d = g.rot[i] * g.rot[j] + g.rot[3 + i] * g.rot[3 + j] + g.rot[6 + i] * g.rot[6 + j];
+      prefilter &= fabs(d - (i == j ? 1.0 : 0.0)) < 1e-9;
+    }
   }
   m.nv = nv;
   if (na == 0 || ntp == 0) {
@@ -430,7 +469,8 @@ qj_status mc_run(qj_workspace* ws, const double* d_field, const qj_grid* grid, d
     return QJ_OK;
   }
   hipLaunchKernelGGL(k_mc_triangles, dim3(nblocks(na)), dim3(kMcBlock), 0, s, m.active, na, m.flags, g, m.emask,
-                     m.blkv, m.vb16, m.verts, m.tri_off, m.tris_pre, m.keep);
+                     m.blkv, m.vb16, m.verts, prefilter ? (const uint8_t*)m.vnear : nullptr, m.tri_off, m.tris_pre,
+                     m.keep);
   QJ_HIP(hipGetLastError());
   if ((st = scan_u32(m.keep, m.keep, ntp, m.totals + 3, m.tmp, m.tmp_bytes, s))) return st;
   // tris holds up to ntp triangles, so the squeeze needs no count from the host
@@ -470,6 +510,7 @@ void mc_free(qj_workspace* ws) {
   hipFree(m->emask);
   hipFree(m->cnt);
   hipFree(m->vb16);
+  hipFree(m->vnear);
   hipFree(m->blkv);
   hipFree(m->blk);
   hipFree(m->active);

@@ -61,6 +61,22 @@ def test_random_field_all_cases(gpu):
     assert_same_mesh(v, t, ov, ot)
 
 
+def test_exact_iso_values_and_degenerate_triangles(gpu):
+    """Values exactly at the isovalue put vertices on grid points, so triangles
+    with coincident corners occur and the 1e-7 degenerate filter drops them
+    (TRIANGLE_MESH.cpp:1088-1102). Once with a rotated grid (the kernel skips
+    the distance test for triangles with fewer than two vertices near an edge
+    end) and once with a spacing below 1e-6 (every triangle takes the test)."""
+    rng = np.random.default_rng(5)
+    res = (19, 16, 14)
+    f = rng.integers(-1, 2, size=res[0] * res[1] * res[2]).astype(np.float64)
+    v, t, ov, ot = mc_both(f, res, center=(0.1, -0.2, 0.3), lengths=(2.0, 1.7, 1.5), rotation=(0.8, 0.36, 0.48, 0.0))
+    assert_same_mesh(v, t, ov, ot)
+    assert len(np.unique(v, axis=0)) < len(v)  # coincident vertices: the filter had cases to decide
+    v, t, ov, ot = mc_both(f, res, lengths=(1e-5, 1e-5, 1e-5))
+    assert_same_mesh(v, t, ov, ot)
+
+
 def test_iso_shift(gpu):
     rng = np.random.default_rng(3)
     res = (12, 12, 12)
