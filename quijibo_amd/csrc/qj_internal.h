@@ -204,11 +204,9 @@ qj_status points_adjoint(qj_workspace* ws, const MapArgs& a, const qj_rational* 
                          const double* d_pts, int64_t n, const double* d_dJdF, double* d_gr, double* d_gp,
                          hipStream_t s);
 
-// exclusive scan of uint32 (qj_scan.hip); n elements, temp provided by ws. With
-// d_n the element count is read on the device (at most n), so a scan can follow
-// a compaction whose length the host has not read back.
+// exclusive scan of uint32 (qj_scan.hip); n elements, temp provided by ws
 qj_status scan_u32(const uint32_t* d_in, uint32_t* d_out, int64_t n, uint32_t* d_total, void* d_tmp,
-                   size_t tmp_bytes, hipStream_t s, const uint32_t* d_n = nullptr);
+                   size_t tmp_bytes, hipStream_t s);
 size_t scan_tmp_bytes(int64_t n);
 
 }  // namespace qj

@@ -6,14 +6,16 @@
 // Pipeline (all HBM-bound; one lane per grid point / cell):
 //   k_mc_flags      cell case from the float sign test, per-block active count (wave ballot)
 //   scan            block counts -> block offsets (order preserving)
-//   k_mc_active     ballot-compacted list of active cells, in cell order, with
-//                   each cell's triangle count
+//   k_mc_active     ballot-compacted list of active cells, in cell order; per
+//                   block of cells its triangle count, and each active cell's
+//                   first triangle within the block
 //   k_mc_edges      per grid point: which of its +x,+y,+z edges some adjacent
 //                   cell's case row references (gather of <= 7 neighbour flags);
 //                   per block of points: vertex count
 //   scan            block vertex counts -> first vertex id per block (the
 //                   reference numbers vertices by std::map<pair<int,int>> order
-//                   = lower endpoint, then x<y<z); triangle counts -> offsets
+//                   = lower endpoint, then x<y<z); block triangle counts ->
+//                   first triangle per block
 //   (one read-back of the three totals)
 //   k_mc_vertices   ids within the block, p0 + d0/(d0-d1) (p1-p0) at cell
 //                   centres, no contraction
@@ -101,7 +103,9 @@ struct McState {
   uint32_t* cnt = nullptr;     // per point edge count -> vertex base (scanned in place; non-linear mesher)
   uint16_t* vb16 = nullptr;    // per point first vertex id within its block (linear mesher)
   uint8_t* vnear = nullptr;    // per vertex: within kNearEnd of an edge end (or off the edge)
-  int64_t cap_vnear = 0;
+  uint16_t* tloc16 = nullptr;  // per active cell first triangle within its block of cells
+  uint32_t* blkt = nullptr;    // per block of cells triangle count -> first triangle of the block
+  int64_t cap_vnear = 0, cap_tloc16 = 0, cap_blkt = 0;
   uint32_t* blkv = nullptr;    // per block vertex count -> first vertex id of the block
   int64_t cap_vb16 = 0, cap_blkv = 0;
   uint32_t* blk = nullptr;     // per block active count -> offset
@@ -164,10 +168,40 @@ __global__ __launch_bounds__(kMcBlock) void k_mc_flags(const double* __restrict_
   }
 }
 
-// tcount (optional): triangles of each active cell, at the cell's list position
+
+// Block-wide exclusive prefix and total of a small per-thread count given as
+// three bit flags of weights W0, W1, W2 (edge bits 1,1,1; a triangle count
+// 0..5 in binary 1,2,4): three ballots and masked popcounts, no shuffles.
+struct BlockPrefix {
+  uint32_t excl;
+  uint32_t total;
+};
+template <int W0, int W1, int W2>
+__device__ __forceinline__ BlockPrefix mc_block_prefix3(unsigned v) {
+  __shared__ uint32_t s_w[kMcBlock / 64];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const unsigned long long lt = (1ull << lane) - 1ull;
+  const unsigned long long b0 = __ballot(v & 1u), b1 = __ballot(v & 2u), b2 = __ballot(v & 4u);
+  const uint32_t within = W0 * __popcll(b0 & lt) + W1 * __popcll(b1 & lt) + W2 * __popcll(b2 & lt);
+  if (lane == 0) s_w[wave] = W0 * __popcll(b0) + W1 * __popcll(b1) + W2 * __popcll(b2);
+  __syncthreads();
+  uint32_t before = 0, total = 0;
+  for (int i = 0; i < kMcBlock / 64; i++) {
+    const uint32_t c = s_w[i];
+    if (i < wave) before += c;
+    total += c;
+  }
+  return BlockPrefix{before + within, total};
+}
+
+// Active cells in cell order (ballot compaction; the block's first position
+// comes from blk_off), and per block of cells its triangle count (blkt) with
+// each active cell's first triangle within the block (tloc16, at the cell's
+// list position).
 __global__ __launch_bounds__(kMcBlock) void k_mc_active(const uint8_t* __restrict__ flags, int64_t N,
                                                         const uint32_t* __restrict__ blk_off,
-                                                        uint32_t* __restrict__ active, uint32_t* __restrict__ tcount) {
+                                                        uint32_t* __restrict__ active, uint16_t* __restrict__ tloc16,
+                                                        uint32_t* __restrict__ blkt) {
   __shared__ uint32_t s_n[kMcBlock / 64];
   const int64_t p = (int64_t)blockIdx.x * kMcBlock + threadIdx.x;
   const int nt = p < N ? kMc.ntri[flags[p]] : 0;
@@ -175,41 +209,15 @@ __global__ __launch_bounds__(kMcBlock) void k_mc_active(const uint8_t* __restric
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const unsigned long long m = __ballot(act);
   if (lane == 0) s_n[wave] = (uint32_t)__popcll(m);
-  __syncthreads();
+  const BlockPrefix tp = mc_block_prefix3<1, 2, 4>((unsigned)nt);  // synchronises the block
   uint32_t off = blk_off[blockIdx.x];
   for (int i = 0; i < wave; i++) off += s_n[i];
   if (act) {
     const uint32_t k = off + __popcll(m & ((1ull << lane) - 1ull));
     active[k] = (uint32_t)p;
-    if (tcount) tcount[k] = (uint32_t)nt;
+    tloc16[k] = (uint16_t)tp.excl;
   }
-}
-
-// Sum over the block of one value per thread (every thread gets it).
-__device__ __forceinline__ uint32_t mc_block_sum(uint32_t v) {
-  __shared__ uint32_t s_w[kMcBlock / 64];
-  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
-  if ((threadIdx.x & 63) == 0) s_w[threadIdx.x >> 6] = v;
-  __syncthreads();
-  uint32_t t = 0;
-  for (int i = 0; i < kMcBlock / 64; i++) t += s_w[i];
-  return t;
-}
-
-// Exclusive prefix over the block of one value per thread.
-__device__ __forceinline__ uint32_t mc_block_exclusive(uint32_t v) {
-  __shared__ uint32_t s_w[kMcBlock / 64];
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  uint32_t x = v;
-  for (int o = 1; o < 64; o <<= 1) {
-    const uint32_t y = __shfl_up(x, o);
-    if (lane >= o) x += y;
-  }
-  if (lane == 63) s_w[wave] = x;
-  __syncthreads();
-  uint32_t before = 0;
-  for (int i = 0; i < wave; i++) before += s_w[i];
-  return before + x - v;
+  if (threadIdx.x == 0) blkt[blockIdx.x] = tp.total;
 }
 
 // Edge masks (bit axis set when the +axis edge from the point carries a vertex)
@@ -249,8 +257,8 @@ __global__ __launch_bounds__(kMcBlock) void k_mc_edges(const uint8_t* __restrict
     }
     emask[p] = (uint8_t)bits;
   }
-  const uint32_t t = mc_block_sum((uint32_t)__popc(bits));
-  if (threadIdx.x == 0) blkv[blockIdx.x] = t;
+  const BlockPrefix vp = mc_block_prefix3<1, 1, 1>(bits);
+  if (threadIdx.x == 0) blkv[blockIdx.x] = vp.total;
 }
 
 // computeEdgeInterpolations (TRIANGLE_MESH.cpp:2373-2425). Vertex ids follow
@@ -277,7 +285,7 @@ __global__ __launch_bounds__(kMcBlock) void k_mc_vertices(const double* __restri
   const int64_t X = g.res[0], XY = X * g.res[1], N = XY * g.res[2];
   const int64_t p = (int64_t)blockIdx.x * kMcBlock + threadIdx.x;
   const unsigned bits = p < N ? emask[p] : 0u;
-  const uint32_t local = mc_block_exclusive((uint32_t)__popc(bits));
+  const uint32_t local = mc_block_prefix3<1, 1, 1>(bits).excl;
   if (p >= N) return;
   vb16[p] = (uint16_t)local;
   if (!bits) return;
@@ -318,7 +326,8 @@ __global__ __launch_bounds__(kMcBlock) void k_mc_triangles(const uint32_t* __res
                                                            const uint16_t* __restrict__ vb16,
                                                            const double* __restrict__ verts,
                                                            const uint8_t* __restrict__ vnear,
-                                                           const uint32_t* __restrict__ toff,
+                                                           const uint32_t* __restrict__ blk_toff,
+                                                           const uint16_t* __restrict__ tloc16,
                                                            int32_t* __restrict__ tris_pre,
                                                            uint32_t* __restrict__ keep) {
   __shared__ uint32_t s_tri[1024];
@@ -330,7 +339,7 @@ __global__ __launch_bounds__(kMcBlock) void k_mc_triangles(const uint32_t* __res
   const int64_t p = active[i];
   const int flag = flags[p];
   const int n = s_ntri[flag];
-  uint32_t o = toff[i];
+  uint32_t o = blk_toff[p / kMcBlock] + tloc16[i];
   for (int t = 0; t < n; t++) {
     int32_t v[3];
     for (int c = 0; c < 3; c++) {
@@ -411,8 +420,8 @@ qj_status mc_run(qj_workspace* ws, const double* d_field, const qj_grid* grid, d
   const int64_t nb = nblocks(N);
   if ((st = grow(m.flags, m.cap_flags, N, s)) || (st = grow(m.emask, m.cap_emask, N, s)) ||
       (st = grow(m.vb16, m.cap_vb16, N, s)) || (st = grow(m.active, m.cap_active, N, s)) ||
-      (st = grow(m.tri_off, m.cap_toff, N, s)) || (st = grow(m.blk, m.cap_blk, nb, s)) ||
-      (st = grow(m.blkv, m.cap_blkv, nb, s)))
+      (st = grow(m.tloc16, m.cap_tloc16, N, s)) || (st = grow(m.blk, m.cap_blk, nb, s)) ||
+      (st = grow(m.blkv, m.cap_blkv, nb, s)) || (st = grow(m.blkt, m.cap_blkt, nb, s)))
     return st;
   const size_t need_tmp = scan_tmp_bytes(5 * N);  // the largest scan: pre-filter triangles <= 5 per cell
   if (need_tmp > m.tmp_bytes) {
@@ -432,12 +441,13 @@ qj_status mc_run(qj_workspace* ws, const double* d_field, const qj_grid* grid, d
   hipLaunchKernelGGL(k_mc_flags, dim3((unsigned)nb), dim3(kMcBlock), 0, s, d_field, g, iso, m.flags, m.blk);
   QJ_HIP(hipGetLastError());
   if ((st = scan_u32(m.blk, m.blk, nb, m.totals + 0, m.tmp, m.tmp_bytes, s))) return st;
-  hipLaunchKernelGGL(k_mc_active, dim3((unsigned)nb), dim3(kMcBlock), 0, s, m.flags, N, m.blk, m.active, m.tri_off);
+  hipLaunchKernelGGL(k_mc_active, dim3((unsigned)nb), dim3(kMcBlock), 0, s, m.flags, N, m.blk, m.active, m.tloc16,
+                     m.blkt);
   QJ_HIP(hipGetLastError());
   hipLaunchKernelGGL(k_mc_edges, dim3((unsigned)nb), dim3(kMcBlock), 0, s, m.flags, g, m.emask, m.blkv);
   QJ_HIP(hipGetLastError());
   if ((st = scan_u32(m.blkv, m.blkv, nb, m.totals + 1, m.tmp, m.tmp_bytes, s))) return st;
-  if ((st = scan_u32(m.tri_off, m.tri_off, N, m.totals + 2, m.tmp, m.tmp_bytes, s, m.totals + 0))) return st;
+  if ((st = scan_u32(m.blkt, m.blkt, nb, m.totals + 2, m.tmp, m.tmp_bytes, s))) return st;
   uint32_t h[4];
   QJ_HIP(hipMemcpyAsync(h, m.totals, 3 * sizeof(uint32_t), hipMemcpyDeviceToHost, s));
   QJ_HIP(hipStreamSynchronize(s));
@@ -469,8 +479,8 @@ qj_status mc_run(qj_workspace* ws, const double* d_field, const qj_grid* grid, d
     return QJ_OK;
   }
   hipLaunchKernelGGL(k_mc_triangles, dim3(nblocks(na)), dim3(kMcBlock), 0, s, m.active, na, m.flags, g, m.emask,
-                     m.blkv, m.vb16, m.verts, prefilter ? (const uint8_t*)m.vnear : nullptr, m.tri_off, m.tris_pre,
-                     m.keep);
+                     m.blkv, m.vb16, m.verts, prefilter ? (const uint8_t*)m.vnear : nullptr, m.blkt, m.tloc16,
+                     m.tris_pre, m.keep);
   QJ_HIP(hipGetLastError());
   if ((st = scan_u32(m.keep, m.keep, ntp, m.totals + 3, m.tmp, m.tmp_bytes, s))) return st;
   // tris holds up to ntp triangles, so the squeeze needs no count from the host
@@ -511,6 +521,8 @@ void mc_free(qj_workspace* ws) {
   hipFree(m->cnt);
   hipFree(m->vb16);
   hipFree(m->vnear);
+  hipFree(m->tloc16);
+  hipFree(m->blkt);
   hipFree(m->blkv);
   hipFree(m->blk);
   hipFree(m->active);

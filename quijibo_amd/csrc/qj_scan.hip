@@ -34,12 +34,8 @@ __device__ __forceinline__ uint32_t block_exclusive(uint32_t v, uint32_t* total)
   return excl;
 }
 
-// n_dev (optional): the element count on the device, at most n (the grid is
-// sized for n; blocks past *n_dev contribute nothing).
 __global__ __launch_bounds__(kScanThreads) void k_scan_reduce(const uint32_t* __restrict__ in, int64_t n,
-                                                              const uint32_t* __restrict__ n_dev,
                                                               uint32_t* __restrict__ partial) {
-  if (n_dev && (int64_t)*n_dev < n) n = (int64_t)*n_dev;
   const int64_t base = (int64_t)blockIdx.x * kScanChunk + (int64_t)threadIdx.x * kScanItems;
   uint32_t s = 0;
   for (int k = 0; k < kScanItems; k++)
@@ -68,10 +64,7 @@ __global__ __launch_bounds__(kScanThreads) void k_scan_partials(uint32_t* __rest
 
 // in and out may alias (in-place scan): each thread reads its items before writing them.
 __global__ __launch_bounds__(kScanThreads) void k_scan_apply(const uint32_t* in, int64_t n,
-                                                             const uint32_t* __restrict__ n_dev,
                                                              const uint32_t* __restrict__ partial, uint32_t* out) {
-  if (n_dev && (int64_t)*n_dev < n) n = (int64_t)*n_dev;
-  if ((int64_t)blockIdx.x * kScanChunk >= n) return;
   const int64_t base = (int64_t)blockIdx.x * kScanChunk + (int64_t)threadIdx.x * kScanItems;
   uint32_t v[kScanItems];
   uint32_t s = 0;
@@ -93,7 +86,7 @@ size_t scan_tmp_bytes(int64_t n) {
 }
 
 qj_status scan_u32(const uint32_t* d_in, uint32_t* d_out, int64_t n, uint32_t* d_total, void* d_tmp,
-                   size_t tmp_bytes, hipStream_t s, const uint32_t* d_n) {
+                   size_t tmp_bytes, hipStream_t s) {
   if (n <= 0) {
     if (d_total) QJ_HIP(hipMemsetAsync(d_total, 0, sizeof(uint32_t), s));
     return QJ_OK;
@@ -101,11 +94,11 @@ qj_status scan_u32(const uint32_t* d_in, uint32_t* d_out, int64_t n, uint32_t* d
   const int64_t nb = (n + kScanChunk - 1) / kScanChunk;
   if (tmp_bytes < scan_tmp_bytes(n) || nb > INT32_MAX) return QJ_ERR_CAPACITY;
   uint32_t* partial = (uint32_t*)d_tmp;
-  hipLaunchKernelGGL(k_scan_reduce, dim3((unsigned)nb), dim3(kScanThreads), 0, s, d_in, n, d_n, partial);
+  hipLaunchKernelGGL(k_scan_reduce, dim3((unsigned)nb), dim3(kScanThreads), 0, s, d_in, n, partial);
   QJ_HIP(hipGetLastError());
   hipLaunchKernelGGL(k_scan_partials, dim3(1), dim3(kScanThreads), 0, s, partial, nb, d_total);
   QJ_HIP(hipGetLastError());
-  hipLaunchKernelGGL(k_scan_apply, dim3((unsigned)nb), dim3(kScanThreads), 0, s, d_in, n, d_n, partial, d_out);
+  hipLaunchKernelGGL(k_scan_apply, dim3((unsigned)nb), dim3(kScanThreads), 0, s, d_in, n, partial, d_out);
   QJ_HIP(hipGetLastError());
   return QJ_OK;
 }
