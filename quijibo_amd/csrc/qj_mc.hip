@@ -90,9 +90,9 @@ __device__ __forceinline__ void edge_geom(int e, int& dx, int& dy, int& dz, int&
   }
 }
 
-__device__ __forceinline__ double edge_len3(double e0, double e1, double e2) {
+__device__ __forceinline__ double sq3(double e0, double e1, double e2) {
 #pragma clang fp contract(off)
-  return sqrt(e0 * e0 + e1 * e1 + e2 * e2);
+  return e0 * e0 + e1 * e1 + e2 * e2;
 }
 
 struct McState {
@@ -282,33 +282,45 @@ __global__ __launch_bounds__(kMcBlock) void k_mc_vertices(const double* __restri
                                                           uint16_t* __restrict__ vb16, double* __restrict__ verts,
                                                           uint8_t* __restrict__ vnear) {
 #pragma clang fp contract(off)
+  // the block's vertices (at most 3 per point) are listed in LDS in id order
+  // and shared out one per thread, so the FP64 work is not spread over
+  // mostly idle lanes
+  __shared__ uint8_t s_pt[3 * kMcBlock];
+  __shared__ uint8_t s_ax[3 * kMcBlock];
   const int64_t X = g.res[0], XY = X * g.res[1], N = XY * g.res[2];
-  const int64_t p = (int64_t)blockIdx.x * kMcBlock + threadIdx.x;
+  const int64_t p0 = (int64_t)blockIdx.x * kMcBlock;
+  const int64_t p = p0 + threadIdx.x;
   const unsigned bits = p < N ? emask[p] : 0u;
-  const uint32_t local = mc_block_prefix3<1, 1, 1>(bits).excl;
-  if (p >= N) return;
-  vb16[p] = (uint16_t)local;
-  if (!bits) return;
-  const int z = (int)(p / XY), y = (int)((p - (int64_t)z * XY) / X), x = (int)(p - (int64_t)z * XY - (int64_t)y * X);
-  double a[3];
-  cell_center(g, x, y, z, a);
-  const double d0 = field[p] - iso;
-  uint32_t k = blk_off[blockIdx.x] + local;
+  const BlockPrefix vp = mc_block_prefix3<1, 1, 1>(bits);
+  if (p < N) vb16[p] = (uint16_t)vp.excl;
+  uint32_t k = vp.excl;
   for (int axis = 0; axis < 3; axis++) {
-    if (!(bits & (1u << axis))) continue;
-    const int x2 = x + (axis == 0), y2 = y + (axis == 1), z2 = z + (axis == 2);
-    double b[3];
-    cell_center(g, x2, y2, z2, b);
+    if (bits & (1u << axis)) {
+      s_pt[k] = (uint8_t)threadIdx.x;
+      s_ax[k] = (uint8_t)axis;
+      k++;
+    }
+  }
+  __syncthreads();
+  const uint32_t base = blk_off[blockIdx.x];
+  for (uint32_t j = threadIdx.x; j < vp.total; j += kMcBlock) {
+    const int64_t q = p0 + s_pt[j];
+    const int axis = s_ax[j];
+    const int z = (int)(q / XY), y = (int)((q - (int64_t)z * XY) / X), x = (int)(q - (int64_t)z * XY - (int64_t)y * X);
+    double a[3], b[3];
+    cell_center(g, x, y, z, a);
+    cell_center(g, x + (axis == 0), y + (axis == 1), z + (axis == 2), b);
+    const double d0 = field[q] - iso;
     const double off0 = b[0] - a[0], off1 = b[1] - a[1], off2 = b[2] - a[2];
-    const double d1 = field[p + (axis == 0 ? 1 : (axis == 1 ? X : XY))] - iso;
+    const double d1 = field[q + (axis == 0 ? 1 : (axis == 1 ? X : XY))] - iso;
     const double interpolation = d0 / (d0 - d1);
     const double v0 = a[0] + interpolation * off0, v1 = a[1] + interpolation * off1, v2 = a[2] + interpolation * off2;
-    verts[3 * (int64_t)k + 0] = v0;
-    verts[3 * (int64_t)k + 1] = v1;
-    verts[3 * (int64_t)k + 2] = v2;
-    const double da = edge_len3(v0 - a[0], v1 - a[1], v2 - a[2]), db = edge_len3(v0 - b[0], v1 - b[1], v2 - b[2]);
-    vnear[k] = (da >= kNearEnd && db >= kNearEnd) ? 0 : 1;
-    k++;
+    const int64_t id = (int64_t)base + j;
+    verts[3 * id + 0] = v0;
+    verts[3 * id + 1] = v1;
+    verts[3 * id + 2] = v2;
+    const double da = sq3(v0 - a[0], v1 - a[1], v2 - a[2]), db = sq3(v0 - b[0], v1 - b[1], v2 - b[2]);
+    vnear[id] = (da >= kNearEnd * kNearEnd && db >= kNearEnd * kNearEnd) ? 0 : 1;
   }
 }
 
