@@ -11,16 +11,24 @@ from quijibo_amd._lib import FP32_BAND, FP64, MAP_BLACK_HOLE
 pytestmark = pytest.mark.gpu
 
 
-def _mixed(n=10, seed=5):
+def _mixed(n=12, seed=5):
+    """Requests of all three FP32 variants in one stream: integer exponents with
+    escape 2000 (renormalise every 4 terms), integer exponents with escape 1e6
+    (term bound over 30 bits: every 2 terms) and general exponents."""
     rng = np.random.default_rng(seed)
     out = []
     for i in range(n):
         e = int(rng.choice([24, 40, 64, 80]))
         nr = int(rng.integers(8, 64))
         nt = nr // 2
-        top = W.RootSet(rng.normal(scale=0.7, size=(nt, 4)), rng.integers(1, 3, size=nt).astype(float))
+        kind = i % 3
+        tp = rng.integers(1, 3, size=nt).astype(float)
+        if kind == 2:
+            tp = rng.choice([1.0, 1.5, 2.0, 0.75], size=nt)
+        top = W.RootSet(rng.normal(scale=0.7, size=(nt, 4)), tp)
         bot = W.RootSet(rng.normal(scale=0.7, size=(nr - nt, 4)), np.ones(nr - nt))
-        w = W.Workload("m%d" % i, (e, e + 3, e - 5), top, bot, max_iter=int(rng.integers(3, 6)))
+        w = W.Workload("m%d" % i, (e, e + 3, e - 5), top, bot, max_iter=int(rng.integers(3, 6)),
+                       escape=1e6 if kind == 1 else 2000.0)
         out.append(w)
     return out
 
