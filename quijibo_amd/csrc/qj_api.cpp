@@ -48,6 +48,9 @@ void make_geom(const qj_grid* g, GridGeom* o) {
     o->lengths[i] = g->lengths[i];
   }
   rotation_matrix(g->rotation, o->rot);
+  const uint64_t xy = (uint64_t)g->res[0] * (uint64_t)g->res[1];
+  o->div_x = make_fastdiv((uint32_t)(g->res[0] > 0 ? g->res[0] : 1));
+  o->div_xy = make_fastdiv(xy > 0 && xy <= 0xffffffffull ? (uint32_t)xy : 1u);
 }
 
 double term_bits(const std::vector<RootRec64>& roots, double escape) {

@@ -67,7 +67,27 @@ struct MapArgs {
   double term_bits;          // max over roots of |e| log2(escape + |root|) + 2: bound on one term's log2 size
 };
 
-// FIELD_3D geometry alone (marching cubes, weight-field lookups).
+// Division of a 32-bit unsigned integer by an invariant divisor d >= 1 with a
+// multiply-high and two shifts (Granlund and Montgomery, "Division by
+// invariant integers using multiplication", PLDI 1994, section 4):
+// q = (t + ((n - t) >> s1)) >> s2, t = mulhi(m, n), with l = ceil(log2 d),
+// m = floor(2^32 (2^l - d) / d) + 1, s1 = min(l, 1), s2 = max(l - 1, 0).
+struct FastDiv {
+  uint32_t m, s1, s2;
+};
+inline FastDiv make_fastdiv(uint32_t d) {
+  uint32_t l = 0;
+  while (l < 32 && (1ull << l) < d) l++;
+  const unsigned __int128 num = ((unsigned __int128)1 << 32) * (((unsigned __int128)1 << l) - d);
+  return FastDiv{(uint32_t)(num / d + 1), l < 1 ? l : 1u, l > 1 ? l - 1 : 0u};
+}
+__host__ __device__ inline uint32_t fast_div(uint32_t n, const FastDiv& f) {
+  const uint32_t t = (uint32_t)(((uint64_t)f.m * n) >> 32);
+  return (t + ((n - t) >> f.s1)) >> f.s2;
+}
+
+// FIELD_3D geometry alone (marching cubes, weight-field lookups). div_x and
+// div_xy divide point indices by X and X*Y (valid while X*Y < 2^32).
 struct GridGeom {
   int32_t res[3];
   double lo[3];
@@ -75,6 +95,7 @@ struct GridGeom {
   double center[3];
   double rot[9];
   double lengths[3];
+  FastDiv div_x, div_xy;
 };
 
 // Counter slots (summed over QJ_NCOUNTER_SPREAD copies to spread atomics).

@@ -145,6 +145,17 @@ __device__ __forceinline__ int cell_flag(const double* __restrict__ f, int64_t i
   return flag;
 }
 
+// grid coordinates of a point index (the mesher runs on fields of < 2^32 points)
+__device__ __forceinline__ void point_xyz(const GridGeom& g, int64_t p, int64_t& x, int64_t& y, int64_t& z) {
+  const uint32_t pu = (uint32_t)p;
+  const uint32_t zu = fast_div(pu, g.div_xy);
+  const uint32_t r = pu - zu * (uint32_t)g.res[0] * (uint32_t)g.res[1];
+  const uint32_t yu = fast_div(r, g.div_x);
+  z = zu;
+  y = yu;
+  x = r - yu * (uint32_t)g.res[0];
+}
+
 __global__ __launch_bounds__(kMcBlock) void k_mc_flags(const double* __restrict__ field, GridGeom g, double iso,
                                                        uint8_t* __restrict__ flags, uint32_t* __restrict__ blk) {
   __shared__ uint32_t s_n[kMcBlock / 64];
@@ -152,7 +163,8 @@ __global__ __launch_bounds__(kMcBlock) void k_mc_flags(const double* __restrict_
   const int64_t p = (int64_t)blockIdx.x * kMcBlock + threadIdx.x;
   bool act = false;
   if (p < N) {
-    const int64_t z = p / XY, r = p - z * XY, y = r / X, x = r - y * X;
+    int64_t x, y, z;
+    point_xyz(g, p, x, y, z);
     int flag = 0;
     if (x < X - 1 && y < Y - 1 && z < Z - 1) flag = cell_flag(field, p, X, XY, iso);
     flags[p] = (uint8_t)flag;
@@ -230,7 +242,8 @@ __global__ __launch_bounds__(kMcBlock) void k_mc_edges(const uint8_t* __restrict
   const int64_t p = (int64_t)blockIdx.x * kMcBlock + threadIdx.x;
   unsigned bits = 0;
   if (p < N) {
-    const int64_t z = p / XY, r = p - z * XY, y = r / X, x = r - y * X;
+    int64_t x, y, z;
+    point_xyz(g, p, x, y, z);
     // x-edge (p, p+1): cells p - (0,a,b); local id a + 2b
     if (x < X - 1) {
       unsigned ref = 0;
@@ -306,7 +319,9 @@ __global__ __launch_bounds__(kMcBlock) void k_mc_vertices(const double* __restri
   for (uint32_t j = threadIdx.x; j < vp.total; j += kMcBlock) {
     const int64_t q = p0 + s_pt[j];
     const int axis = s_ax[j];
-    const int z = (int)(q / XY), y = (int)((q - (int64_t)z * XY) / X), x = (int)(q - (int64_t)z * XY - (int64_t)y * X);
+    int64_t x64, y64, z64;
+    point_xyz(g, q, x64, y64, z64);
+    const int x = (int)x64, y = (int)y64, z = (int)z64;
     double a[3], b[3];
     cell_center(g, x, y, z, a);
     cell_center(g, x + (axis == 0), y + (axis == 1), z + (axis == 2), b);
