@@ -75,6 +75,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-c3", action="store_true", help="skip the 1024^3 tiled section")
     ap.add_argument("--no-grad", action="store_true", help="skip the gradient section")
+    ap.add_argument("--no-nlmc", action="store_true", help="skip the non-linear mesher section")
     ap.add_argument("--band", type=float, default=0.05)
     ap.add_argument("--serve", type=int, default=2000, help="requests in the serving stream (0 = skip)")
     ap.add_argument("--serve-rate", type=float, default=200.0)
@@ -195,6 +196,7 @@ def main():
                                     "max_rel_err_fp64_gpu": cb["max_rel_err_fp64"]}}
     c3 = None if args.no_c3 else c3_tiles(args, log)
     grads = None if args.no_grad else gradients(args, log)
+    nlmc = None if args.no_nlmc else nonlinear_mesher(args, log)
     serving = None
     if args.serve > 0:
         serving = serve_stream(args, log)
@@ -225,6 +227,7 @@ def main():
         "cpu_baseline": cpu,
         "c3_1024_tiles": c3,
         "gradients": grads,
+        "nonlinear_mesher": nlmc,
         "serving": serving,
         "p50_ms_256": (serving or {}).get("latency", {}).get("256^3", {}).get("p50_ms") if serving else None,
         "p99_ms_256": (serving or {}).get("latency", {}).get("256^3", {}).get("p99_ms") if serving else None,
@@ -256,6 +259,31 @@ def serve_stream(args, log):
                       "iterations, %s launches in flight, fp32+fp64 band, shortest-remaining-work-first"
                       % (args.serve_budget_gib, ("%.1e" % args.serve_launch_work) if args.serve_launch_work else "3e9",
                          args.serve_inflight or 2)}
+
+
+def nonlinear_mesher(args, log):
+    """The low-memory non-linear mesher on the headline roots at 512^3: slabs of 64 z-slices
+    evaluated FP32+band, bisection vertices (up to 6 steps, FP64 map evaluations), triangles;
+    device memory O(512^2 x 64) instead of the 1 GiB field."""
+    import torch
+    from quijibo_amd import api, workloads as W
+    from quijibo_amd._lib import FP32_BAND, MAP_BLACK_HOLE
+    w = W.headline(512)
+    g = api.make_grid(w.res, w.center, w.lengths)
+    p = api.make_params(w.max_iter, w.escape, w.exp_scaling, w.slice, MAP_BLACK_HOLE)
+    r = api.Rational(api.POLYNOMIAL_4D(w.top.roots, w.top.powers), api.POLYNOMIAL_4D(w.bottom.roots, w.bottom.powers))
+    ws = api.Workspace()
+    small = api.make_grid((64, 64, 64), w.center, w.lengths)
+    api.nonlinear_mesh(small, r, p, 0.0, FP32_BAND, args.band, 64, workspace=ws)  # warm-up
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    v, t = api.nonlinear_mesh(g, r, p, 0.0, FP32_BAND, args.band, 64, workspace=ws)
+    torch.cuda.synchronize()
+    ms = (time.perf_counter() - t0) * 1e3
+    log("nonlinear mesher 512^3: %.1f ms, %d vertices, %d triangles" % (ms, len(v), len(t)))
+    return {"workload": "512^3 headline roots (32+32, powers U{1,2}, maxIter 4), slabs of 64 z-slices, FP32+band "
+                        "slab values, bisection vertices; time includes the copy of the mesh to the host",
+            "ms": round(ms, 1), "vertices": int(len(v)), "triangles": int(len(t))}
 
 
 def gradients(args, log):

@@ -706,32 +706,62 @@ __device__ __forceinline__ void place_vertex(const MapArgs& a, const RootRec64* 
   out[2] = m[2];
 }
 
-// vertices of the owned points [p0, p0 + np) by bisection, and their edges
+// vertices of the owned points [p0, p0 + np) by bisection, and their edges.
+// About one point in seven owns an edge, so the block first lists its points
+// with edges and their vertices in LDS, then evaluates one listed point per
+// thread (the lower end value, shared by the point's edges) and bisects one
+// vertex per thread: the FP64 map evaluations run on full waves.
 __global__ __launch_bounds__(kMcBlock) void k_nl_vertices(MapArgs a, const RootRec64* __restrict__ roots,
                                                           double iso, int64_t p0, int64_t np,
                                                           const uint8_t* __restrict__ emask,
                                                           const uint32_t* __restrict__ off, uint32_t base,
                                                           double* __restrict__ verts, int32_t* __restrict__ edges,
                                                           unsigned int* __restrict__ bad) {
-  const int64_t X = a.res[0], XY = X * a.res[1];
-  const int64_t j = (int64_t)blockIdx.x * kMcBlock + threadIdx.x;
-  if (j >= XY * np) return;
-  const unsigned bits = emask[j];
-  if (!bits) return;
-  const int64_t lz = j / XY, r = j - lz * XY;
-  const int y = (int)(r / X), x = (int)(r - (int64_t)y * X), z = (int)(p0 + lz);
-  double pa[3];
-  cell_center(a, x, y, z, pa);
-  const double va = nonlinear_value(a, roots, pa, iso);
-  uint32_t k = base + off[j];
-  for (int axis = 0; axis < 3; axis++) {
-    if (!(bits & (1u << axis))) continue;
-    place_vertex(a, roots, iso, x, y, z, axis, pa, va, verts + 3 * (int64_t)k, bad);
-    edges[4 * (int64_t)k + 0] = x;
-    edges[4 * (int64_t)k + 1] = y;
-    edges[4 * (int64_t)k + 2] = z;
-    edges[4 * (int64_t)k + 3] = axis;
-    k++;
+  __shared__ uint8_t s_lp[kMcBlock];        // listed point -> point within the block
+  __shared__ double s_va[kMcBlock];         // map value at the listed point
+  __shared__ uint8_t s_slp[3 * kMcBlock];   // vertex slot -> listed point
+  __shared__ uint8_t s_sax[3 * kMcBlock];   // vertex slot -> axis
+  const int64_t X = a.res[0], XY = X * a.res[1], N = XY * np;
+  const int64_t j0 = (int64_t)blockIdx.x * kMcBlock;
+  const int64_t j = j0 + threadIdx.x;
+  const unsigned bits = j < N ? emask[j] : 0u;
+  const BlockPrefix lp = mc_block_prefix3<1, 0, 0>(bits != 0u ? 1u : 0u);
+  const BlockPrefix vp = mc_block_prefix3<1, 1, 1>(bits);
+  if (bits) {
+    s_lp[lp.excl] = (uint8_t)threadIdx.x;
+    uint32_t k = vp.excl;
+    for (int axis = 0; axis < 3; axis++) {
+      if (bits & (1u << axis)) {
+        s_slp[k] = (uint8_t)lp.excl;
+        s_sax[k] = (uint8_t)axis;
+        k++;
+      }
+    }
+  }
+  __syncthreads();
+  for (uint32_t i = threadIdx.x; i < lp.total; i += kMcBlock) {
+    const int64_t q = j0 + s_lp[i];
+    const int64_t lz = q / XY, r = q - lz * XY;
+    const int y = (int)(r / X), x = (int)(r - (int64_t)y * X), z = (int)(p0 + lz);
+    double pa[3];
+    cell_center(a, x, y, z, pa);
+    s_va[i] = nonlinear_value(a, roots, pa, iso);
+  }
+  __syncthreads();
+  const uint32_t first = base + off[j0];  // vertex id of the block's first listed edge
+  for (uint32_t sl = threadIdx.x; sl < vp.total; sl += kMcBlock) {
+    const int l = s_slp[sl], axis = s_sax[sl];
+    const int64_t q = j0 + s_lp[l];
+    const int64_t lz = q / XY, r = q - lz * XY;
+    const int y = (int)(r / X), x = (int)(r - (int64_t)y * X), z = (int)(p0 + lz);
+    double pa[3];
+    cell_center(a, x, y, z, pa);
+    const int64_t k = (int64_t)first + sl;
+    place_vertex(a, roots, iso, x, y, z, axis, pa, s_va[l], verts + 3 * k, bad);
+    edges[4 * k + 0] = x;
+    edges[4 * k + 1] = y;
+    edges[4 * k + 2] = z;
+    edges[4 * k + 3] = axis;
   }
 }
 
