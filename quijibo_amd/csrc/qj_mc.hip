@@ -580,7 +580,10 @@ void mc_free(qj_workspace* ws) {
 // placed by bisection on re-evaluated values log|q| - iso (nonlinearValue
 // :1755-1795, midpointSearch :2232-2271, at most 6 halvings, early exit at
 // |value| < 1e-8). Vertex numbering, triangle order and the degenerate filter
-// are those of the linear mesher.
+// are those of the linear mesher. The re-evaluations take exponents 1 and 2
+// as exact products (b and b b) and others in the polar form, as the FP64
+// band does: the values differ from the polar form's in the last few ulps,
+// the same order as the device math library against the host's.
 
 __device__ __forceinline__ double nonlinear_value(const MapArgs& a, const RootRec64* __restrict__ roots,
                                                   const double* p, double iso) {
@@ -592,7 +595,7 @@ __device__ __forceinline__ double nonlinear_value(const MapArgs& a, const RootRe
     double magnitude = qmag(q), prev = magnitude;
     int it = 0;
     while (magnitude < a.escape && it < a.max_iter) {
-      q = qscale(lagrange64(roots, a.nt, a.nb, q), a.exp_scaling);
+      q = qscale(lagrange64<kExactInt>(roots, a.nt, a.nb, q), a.exp_scaling);
       magnitude = qmag(q);
       if (isinf(magnitude)) {
         magnitude = prev;
@@ -604,7 +607,7 @@ __device__ __forceinline__ double nonlinear_value(const MapArgs& a, const RootRe
     }
     return log(magnitude) - iso;
   }
-  const Res64 r = eval64(a, roots, q64{p[0], p[1], p[2], a.slice});
+  const Res64 r = eval64<kExactInt>(a, roots, q64{p[0], p[1], p[2], a.slice});
   return r.F - iso;
 }
 
