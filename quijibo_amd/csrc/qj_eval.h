@@ -26,6 +26,14 @@ __device__ __forceinline__ bool has_nan(const q64& q) { return isnan(q.w) || isn
 // cases: b = 0 gives NaN, v = 0 gives (|w|^e, +0 * v).
 enum PowMode { kPolar = 0, kExactInt = 1, kExactOnly = 2 };
 
+// One root record by value through any pointer type (generic, or the constant
+// address space where a kernel wants scalar loads of a pointer it read from
+// memory).
+template <class RP>
+__device__ __forceinline__ RootRec64 load_root64(RP p) {
+  return RootRec64{p->w, p->x, p->y, p->z, p->e, p->p};
+}
+
 template <int kPow>
 __device__ __forceinline__ q64 term64(const RootRec64& r, const q64& q) {
   const q64 b = qsub(q, q64{r.w, r.x, r.y, r.z});
@@ -47,11 +55,11 @@ __device__ __forceinline__ q64 term64(const RootRec64& r, const q64& q) {
   return qpow(b, r.e);
 }
 
-template <int kPow = kPolar>
-__device__ __forceinline__ q64 poly64(const RootRec64* __restrict__ r, int n, const q64& q) {
-  q64 res = term64<kPow>(r[0], q);
+template <int kPow = kPolar, class RP = const RootRec64*>
+__device__ __forceinline__ q64 poly64(RP r, int n, const q64& q) {
+  q64 res = term64<kPow>(load_root64(r), q);
   for (int i = 1; i < n; i++) {
-    res = qmul(res, term64<kPow>(r[i], q));
+    res = qmul(res, term64<kPow>(load_root64(r + i), q));
     if ((i & 7) == 0 && has_nan(res)) {
       const double nan = __builtin_nan("");
       return q64{nan, nan, nan, nan};
@@ -74,26 +82,26 @@ struct Res64 {
 // RATIONAL_4D::evaluateLagrange (RATIONAL_4D.cpp:27-88): with bottom roots,
 // r = 1, then r = (r t_i) / b_i for the paired roots, then the leftover top
 // terms multiplied or bottom terms divided; the factored top product otherwise.
-template <int kPow = kPolar>
-__device__ __forceinline__ q64 lagrange64(const RootRec64* __restrict__ roots, int nt, int nb, const q64& q) {
+template <int kPow = kPolar, class RP = const RootRec64*>
+__device__ __forceinline__ q64 lagrange64(RP roots, int nt, int nb, const q64& q) {
   if (nb == 0) return poly64<kPow>(roots, nt, q);
-  const RootRec64* bot = roots + nt;
+  const RP bot = roots + nt;
   q64 r{1.0, 0.0, 0.0, 0.0};
   const int same = nt < nb ? nt : nb;
   for (int i = 0; i < same; i++) {
-    r = qmul(r, term64<kPow>(roots[i], q));
-    r = qdiv(r, term64<kPow>(bot[i], q));
+    r = qmul(r, term64<kPow>(load_root64(roots + i), q));
+    r = qdiv(r, term64<kPow>(load_root64(bot + i), q));
   }
-  for (int i = same; i < nt; i++) r = qmul(r, term64<kPow>(roots[i], q));
-  for (int i = same; i < nb; i++) r = qdiv(r, term64<kPow>(bot[i], q));
+  for (int i = same; i < nt; i++) r = qmul(r, term64<kPow>(load_root64(roots + i), q));
+  for (int i = same; i < nb; i++) r = qdiv(r, term64<kPow>(load_root64(bot + i), q));
   return r;
 }
 
 // One voxel of computeLogScaledPowerRationalMap (OPTIMIZE_3D.cpp:1317-1362) /
 // computeRationalMapAP's first pass (:1423-1475). Overflow detection replaces
 // fetestexcept(FE_OVERFLOW): any non-finite T, B, |B|^2, quotient or |q|.
-template <int kPow = kPolar>
-__device__ __forceinline__ Res64 eval64(const MapArgs& a, const RootRec64* __restrict__ roots, q64 q) {
+template <int kPow = kPolar, class RP = const RootRec64*>
+__device__ __forceinline__ Res64 eval64(const MapArgs& a, RP roots, q64 q) {
   Res64 r{0.0, 0, true, false};
   double magnitude = qmag(q);
   if ((a.flags & QJ_MAP_SKIP_ORIGIN) && magnitude < 1e-8) {

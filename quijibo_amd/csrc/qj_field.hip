@@ -646,10 +646,21 @@ __global__ __launch_bounds__(kBlock) void k_batch64_list(const ChunkDesc* __rest
     const uint32_t e = kFromEnd ? list[list_cap - 1 - i] : list[i];
     const int c = (int)(e / (uint32_t)chunk_voxels);
     const int local = (int)(e - (uint32_t)c * (uint32_t)chunk_voxels);
-    const ChunkDesc& d = descs[c];
-    const int64_t idx = d.start + local;
-    const Res64 r = eval64<kPow>(d.a, d.r64, chunk_start(d.a, idx));
-    if (r.written) d.out[idx] = r.F;
+    // consecutive entries nearly always come from one chunk: then the chunk
+    // index is wave-uniform and its root records come in scalar loads
+    const int cw = __builtin_amdgcn_readfirstlane(c);
+    if (__ballot(c != cw) == 0ull) {
+      const ChunkDesc& d = descs[cw];
+      const int64_t idx = d.start + local;
+      typedef const __attribute__((address_space(4))) RootRec64 cR64;
+      const Res64 r = eval64<kPow>(d.a, (cR64*)d.r64, chunk_start(d.a, idx));
+      if (r.written) d.out[idx] = r.F;
+    } else {
+      const ChunkDesc& d = descs[c];
+      const int64_t idx = d.start + local;
+      const Res64 r = eval64<kPow>(d.a, d.r64, chunk_start(d.a, idx));
+      if (r.written) d.out[idx] = r.F;
+    }
   }
 }
 
