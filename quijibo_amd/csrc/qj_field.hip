@@ -24,10 +24,10 @@
 #include "qj_internal.h"
 #include "qj_quat.h"
 #include "qj_eval.h"
+#include "qj_fp32.h"
 
 namespace qj {
 
-constexpr int kTileX = 16, kTileY = 4, kTileZ = 4, kBlock = 256;
 
 // ---------------------------------------------------------------- geometry
 
@@ -48,50 +48,6 @@ struct PointSource {
     return {pts[3 * idx], pts[3 * idx + 1], pts[3 * idx + 2], a.slice};
   }
 };
-
-// ---------------------------------------------------------------- wave helpers
-
-__device__ __forceinline__ unsigned long long wave_sum_u64(unsigned long long v) {
-  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
-  return v;
-}
-
-// Sum v over the block and add it to counter slot `slot` (spread copies).
-__device__ __forceinline__ void block_count(unsigned long long v, unsigned long long* cnt, int slot) {
-  __shared__ unsigned long long s_part[kBlock / 64];
-  v = wave_sum_u64(v);
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  if (lane == 0) s_part[wave] = v;
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    unsigned long long t = 0;
-    for (int i = 0; i < kBlock / 64; i++) t += s_part[i];
-    if (t) atomicAdd(&cnt[slot * kCounterSpread + (blockIdx.x % kCounterSpread)], t);
-  }
-  __syncthreads();
-}
-
-// Order-free compaction: flagged lanes append `value` to list; one atomic per block.
-__device__ __forceinline__ void block_append(bool flag, uint32_t value, uint32_t* list, uint32_t* count) {
-  __shared__ uint32_t s_cnt[kBlock / 64];
-  __shared__ uint32_t s_base;
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  const unsigned long long m = __ballot(flag);
-  const uint32_t rank = (uint32_t)__popcll(m & ((1ull << lane) - 1ull));
-  if (lane == 0) s_cnt[wave] = (uint32_t)__popcll(m);
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    uint32_t t = 0;
-    for (int i = 0; i < kBlock / 64; i++) {
-      const uint32_t c = s_cnt[i];
-      s_cnt[i] = t;
-      t += c;
-    }
-    s_base = t ? atomicAdd(count, t) : 0u;
-  }
-  __syncthreads();
-  if (flag) list[s_base + s_cnt[wave] + rank] = value;
-}
 
 // ---------------------------------------------------------------- FP64 extended range
 
@@ -182,248 +138,8 @@ __device__ __forceinline__ Res64 eval64x(const MapArgs& a, const RootRec64* __re
   return r;
 }
 
-// ---------------------------------------------------------------- FP32 bulk
-
-
-// Largest-component power-of-two renormalisation: exact, no rounding.
-// Returns the applied exponent (0 for 0 / inf / nan).
-__device__ __forceinline__ int renorm32(q32& t, int& E) {
-  // max |component| in two instructions (fmaxf would canonicalise each input
-  // first); a NaN component is dropped, and a NaN product is flagged by the
-  // caller through its magnitude
-  float mx;
-  asm("v_max3_f32 %0, |%1|, |%2|, |%3|\n\tv_max_f32_e64 %0, %0, |%4|"
-      : "=&v"(mx)
-      : "v"(t.w), "v"(t.x), "v"(t.y), "v"(t.z));
-  const int k = __builtin_amdgcn_frexp_expf(mx);
-  t.w = __builtin_amdgcn_ldexpf(t.w, -k);
-  t.x = __builtin_amdgcn_ldexpf(t.x, -k);
-  t.y = __builtin_amdgcn_ldexpf(t.y, -k);
-  t.z = __builtin_amdgcn_ldexpf(t.z, -k);
-  E += k;
-  return k;
-}
-
-// General-kind term (q - r)^e in FP32, following QUATERNION::pow's polar form
-// with the magnitude |b|^e moved into the log2 accumulator L, so the
-// quaternion part is unit length. The reference's guards carry over: v = 0
-// gives a positive real result and a non-positive angle drops the imaginary
-// part.
-__device__ __forceinline__ q32 general32(float e, const q32& b, float& L) {
-  const float partial = b.x * b.x + b.y * b.y + b.z * b.z;
-  const float n2 = partial + b.w * b.w;
-  const float vm = __builtin_sqrtf(partial);
-  const float th = atan2f(vm, b.w);
-  const float m = vm > 0.0f ? e * th : 0.0f;  // scale = e*acos(w/|q|)*0 when v = 0
-  L += e * 0.5f * __log2f(n2);
-  float s, c;
-  __sincosf(m, &s, &c);
-  const float f = (m > 0.0f && vm > 0.0f) ? s / vm : 0.0f;
-  return q32{c, f * b.x, f * b.y, f * b.z};
-}
-
-// e = 1: pow(b, 1) is b itself except for v = 0, where the polar form gives
-// +|w|. The FP32 path uses b and keeps the running minimum of |b.x| instead: a
-// voxel that met b.x = 0 is sent to FP64, which applies the guard exactly. The
-// minimum is one v_min_f32 with an |.| source modifier; fminf would add two
-// canonicalising maxes per term. A NaN b.x leaves vx unchanged (the voxel is
-// flagged through its magnitude).
-__device__ __forceinline__ q32 kind1(const q32& b, float& vx) {
-  asm("v_min_f32_e64 %0, %1, |%2|" : "=v"(vx) : "v"(vx), "v"(b.x));
-  return b;
-}
-
-// e = 2: b * b (for v = 0 this is (w^2, 0, 0, 0), the polar form's value too).
-__device__ __forceinline__ q32 kind2(const q32& b) {
-  const float partial = b.x * b.x + b.y * b.y + b.z * b.z;
-  const float t = 2.0f * b.w;
-  return q32{b.w * b.w - partial, t * b.x, t * b.y, t * b.z};
-}
-
-// First term of a product; kb: the root's pattern bits (kPatSquare,
-// kPatGeneral).
-template <class EP>
-__device__ __forceinline__ q32 term32(const F4& r, int kb, EP e, const q32& q, float& L,
-                                      float& vx) {
-  const q32 b = qsub(q, q32{r.w, r.x, r.y, r.z});
-  if (kb & kPatGeneral) return general32(*e, b, L);
-  if (kb & kPatSquare) return kind2(b);
-  return kind1(b, vx);
-}
-
-// t <- t * (q - r)^e, with the product taken inside each kind's branch so the
-// branches share no term registers (this file is compiled without
-// SimplifyCFG's common-code sinking, which would merge the products behind
-// register copies).
-template <bool kGeneral, class EP>
-__device__ __forceinline__ void apply32(q32& t, const F4& r, int kb, EP e, const q32& q,
-                                        float& L, float& vx) {
-  const q32 b = qsub(q, q32{r.w, r.x, r.y, r.z});
-  if (kGeneral && (kb & kPatGeneral)) {
-    t = qmul(t, general32(*e, b, L));
-  } else if (kb & kPatSquare) {
-    t = qmul(t, kind2(b));
-  } else {
-    t = qmul(t, kind1(b, vx));
-  }
-}
-
-// Ordered product over n roots with exponent tracking, renormalised every kRN
-// terms. hi/lo receive the largest and smallest log2 magnitude of the running
-// product at the renormalisation points (E, plus L when general kinds exist);
-// `bad` is set when a group of kRN terms fell below 2^-60 (kRN = 4 only: a
-// group of four can leave FP32's normal range, two cannot). vx: minimum |x| over
-// the e = 1 terms (see kind1). The roots of a group are read before its first
-// term: one scalar load of kRN * 16 bytes and one pattern word.
-template <class QP>
-__device__ __forceinline__ F4 load_root(QP p) {
-  return F4{p->w, p->x, p->y, p->z};
-}
-
-template <int kRN, bool kGeneral, class QP, class PP, class EP>
-__device__ __forceinline__ q32 poly32(QP rq, PP rp, EP re, int n, const q32& q, int& E, float& L, float& hi,
-                                      float& lo, bool& bad, float& vx) {
-  q32 t = term32(load_root(rq), rp[0], re, q, L, vx);
-  int i = 1;
-  int mn = 0;
-  int ehi = -(1 << 30), elo = 1 << 30;
-  for (; i + kRN - 1 < n; i += kRN) {
-    F4 g[kRN];
-#pragma unroll
-    for (int j = 0; j < kRN; j++) g[j] = load_root(rq + i + j);
-    const int pat = rp[i];
-#pragma unroll
-    for (int j = 0; j < kRN; j++) apply32<kGeneral>(t, g[j], pat >> (2 * j), re + i + j, q, L, vx);
-    const int k = renorm32(t, E);
-    if (kRN > 2) mn = min(mn, k);
-    if (kGeneral) {
-      const float l = (float)E + L;
-      hi = fmaxf(hi, l);
-      lo = fminf(lo, l);
-    } else {
-      ehi = max(ehi, E);
-      elo = min(elo, E);
-    }
-  }
-  for (; i < n; i++) apply32<kGeneral>(t, load_root(rq + i), rp[i], re + i, q, L, vx);
-  const int k = renorm32(t, E);
-  if (kRN > 2) bad |= min(mn, k) < -60;
-  if (kGeneral) {
-    const float l = (float)E + L;
-    hi = fmaxf(hi, l);
-    lo = fminf(lo, l);
-  } else {
-    ehi = max(ehi, E);
-    elo = min(elo, E);
-    hi = fmaxf(hi, (float)ehi);
-    lo = fminf(lo, (float)elo);
-  }
-  return t;
-}
-
-// Renormalising every 4 terms needs 4 terms of up to 2^term_bits to stay in
-// FP32 range: 4 * 30 + 1 < 127.
-constexpr double kRN4MaxTermBits = 30.0;
-
-struct Res32 {
-  float F;
-  int iters;
-  bool written;
-  bool need64;
-};
-
-// One voxel in FP32 with the FP64 method's range behaviour reproduced where it
-// is certain: every running product of T and B must stay well inside FP64's
-// range (term_bits covers the product between two renormalisations); then
-// |B| > 2^512.5 means |B|^2 overflows in FP64, B^-1 = conj(B) * 0 and q = 0
-// exactly (F = -inf under the black-hole stop); anything closer to a range
-// edge, or outside it, goes to FP64.
-template <int kRN, bool kGeneral, class RootsT>
-__device__ __forceinline__ Res32 eval32(const MapArgs& a, const RootsT& roots, const Consts32& k,
-                                        const q64& q0) {
-  Res32 res{0.0f, 0, true, false};
-  // the first escape / origin test in FP64, exactly as the reference
-  const double mag0 = qmag(q0);
-  if ((a.flags & QJ_MAP_SKIP_ORIGIN) && mag0 < 1e-8) {
-    res.written = false;
-    return res;
-  }
-  if (!(mag0 < a.escape) || a.max_iter <= 0) {
-    res.F = (float)log(mag0);
-    res.need64 = true;  // trivial case: let FP64 write it exactly
-    return res;
-  }
-  const bool black_hole = (a.flags & QJ_MAP_BLACK_HOLE) != 0;
-  q32 q{(float)q0.w, (float)q0.x, (float)q0.y, (float)q0.z};
-  float log2mag = 0.0f;
-  float vx = INFINITY;
-  bool amb = false, bad = false, zero = false;
-  int it = 0;
-  // FP64 running products between two checkpoints grow by at most (kRN - 1) terms
-  const float lim = 1020.0f - (float)(kRN - 1) * k.term_bits;
-  while (true) {
-    int ET = 0, EB = 0;
-    float L = 0.0f, hiT = -1e30f, loT = 1e30f;
-    const q32 T = poly32<kRN, kGeneral>(roots.q, roots.pat, roots.e, a.nt, q, ET, L, hiT, loT, bad, vx);
-    bad |= !(hiT < lim && loT > -lim);
-    float l2 = k.log2c;
-    if (a.nb > 0) {
-      float LB = 0.0f, hiB = -1e30f, loB = 1e30f;
-      const q32 B = poly32<kRN, kGeneral>(roots.q + a.nt, roots.pat + a.nt, roots.e + a.nt, a.nb, q, EB, LB, hiB,
-                                          loB, bad, vx);
-      bad |= !(hiB < lim && loB > -lim);
-      const float nb2 = B.w * B.w + B.x * B.x + B.y * B.y + B.z * B.z;
-      const float lB = 0.5f * __log2f(nb2) + (float)EB + LB;
-      const q32 D = qmul(T, q32{B.w, -B.x, -B.y, -B.z});  // T conj(B) = T B^-1 |B|^2
-      const float nd2 = D.w * D.w + D.x * D.x + D.y * D.y + D.z * D.z;
-      l2 += 0.5f * __log2f(nd2) - __log2f(nb2) + (float)(ET - EB) + (L - LB);
-      bad |= !(nb2 > 0.0f) || !(nd2 > 0.0f);
-      if (lB > 512.5f) {
-        zero = true;  // FP64: |B|^2 = inf, 1/inf = 0, q = T * 0 = 0
-      } else if (lB > 511.5f || lB < -510.0f) {
-        bad = true;   // within FP32 error of the FP64 overflow edge of |B|^2 or 1/|B|^2
-      } else if (!bad) {
-        const float sc = exp2f(k.log2c + (float)(ET - EB) + (L - LB)) / nb2;
-        q = qscale(D, sc);
-      }
-    } else {
-      const float nt2 = T.w * T.w + T.x * T.x + T.y * T.y + T.z * T.z;
-      l2 += 0.5f * __log2f(nt2) + (float)ET + L;
-      bad |= !(nt2 > 0.0f);
-      if (!bad) q = qscale(T, exp2f(k.log2c + (float)ET + L));
-    }
-    it++;
-    res.iters++;
-    if (zero) {
-      // q = 0: magnitude 0 < 10 FLT_MIN stops the loop with F = log(0)
-      bad |= !black_hole;
-      log2mag = -INFINITY;
-      break;
-    }
-    log2mag = l2;
-    bad |= !isfinite(l2) || l2 > 511.0f || l2 < -530.0f;  // |q|^2 over/underflow in FP64
-    amb |= fabsf(l2 - k.log2esc) < k.margin;
-    const bool bh = black_hole && l2 < k.log2bh;
-    if (black_hole) amb |= fabsf(l2 - k.log2bh) < k.margin;
-    if (bad || l2 >= k.log2esc || bh || it >= a.max_iter) break;
-  }
-  res.F = zero ? -INFINITY : log2mag * 0.69314718f;
-  res.need64 = bad || amb || !(vx > 0.0f) || !(fabsf(res.F) >= k.band);
-  return res;
-}
-
 // ---------------------------------------------------------------- kernels
 
-__device__ __forceinline__ bool tile_voxel(const MapArgs& a, int& x, int& y, int& z) {
-  const int ntx = (a.res[0] + kTileX - 1) / kTileX, nty = (a.res[1] + kTileY - 1) / kTileY;
-  const int b = blockIdx.x;
-  const int tx = b % ntx, ty = (b / ntx) % nty, tz = b / (ntx * nty);
-  const int t = threadIdx.x;
-  x = tx * kTileX + (t & (kTileX - 1));
-  y = ty * kTileY + ((t >> 4) & (kTileY - 1));
-  z = tz * kTileZ + (t >> 6);
-  return x < a.res[0] && y < a.res[1] && z < a.res[2];
-}
 
 __global__ __launch_bounds__(kBlock) void k_grid64(MapArgs a, const RootRec64* __restrict__ roots,
                                                    double* __restrict__ out, unsigned long long* __restrict__ cnt,
@@ -458,24 +174,8 @@ __global__ __launch_bounds__(kBlock) void k_grid32(MapArgs a, Consts32 k, const 
                                                    double* __restrict__ out, unsigned long long* __restrict__ cnt,
                                                    uint32_t* __restrict__ blist, uint32_t* __restrict__ bcount) {
   const Roots32 R = roots32_view(roots, a.nt + a.nb);
-  int x, y, z;
-  const bool valid = tile_voxel(a, x, y, z);
-  int iters = 0;
-  bool need = false;
-  const int64_t idx = x + (int64_t)y * a.res[0] + (int64_t)z * a.res[0] * a.res[1];
-  if (valid) {
-    double c[3];
-    cell_center(a, x, y, z + a.zoff, c);
-    const Res32 r = eval32<kRN, kGeneral>(a, R, k, q64{c[0], c[1], c[2], a.slice});
-    if (r.written) {
-      if (!r.need64) out[idx] = (double)r.F;
-      need = r.need64;
-    }
-    iters = r.iters;
-  }
-  block_count(valid ? 1 : 0, cnt, CNT_VOXELS);
-  block_count(iters, cnt, CNT_ITERS);
-  block_append(need, (uint32_t)idx, blist, bcount);
+  grid32_body<kRN, kGeneral>(a, k, root_acc(R.q, R.pat, R.e, 0), root_acc(R.q, R.pat, R.e, a.nt), out, cnt,
+                                    blist, bcount);
 }
 
 template <class Src, int kRN, bool kGeneral>
@@ -489,7 +189,8 @@ __global__ __launch_bounds__(kBlock) void k_points32(MapArgs a, Consts32 k, cons
   int iters = 0;
   bool need = false;
   if (valid) {
-    const Res32 r = eval32<kRN, kGeneral>(a, R, k, Src::start(a, pts, i));
+    const Res32 r = eval32<kRN, kGeneral>(a, root_acc(R.q, R.pat, R.e, 0), root_acc(R.q, R.pat, R.e, a.nt), k,
+                                                Src::start(a, pts, i));
     if (r.written) {
       if (!r.need64) out[i] = (double)r.F;
       need = r.need64;
@@ -600,7 +301,9 @@ __global__ __launch_bounds__(kBlock) void k_batch32(const ChunkDesc* __restrict_
       const MapArgs& a = d.a;
       const Consts32 k = d.k;
       const int64_t idx = d.start + local;
-      const Res32 r = eval32<kRN, kGen>(a, roots32_const(d.r32, a.nt + a.nb), k, chunk_start(a, idx));
+      const Roots32K R = roots32_const(d.r32, a.nt + a.nb);
+      const Res32 r = eval32<kRN, kGen>(a, root_acc(R.q, R.pat, R.e, 0), root_acc(R.q, R.pat, R.e, a.nt), k,
+                                             chunk_start(a, idx));
       if (r.written) {
         if (!r.need64) d.out[idx] = (double)r.F;
         need = r.need64;
