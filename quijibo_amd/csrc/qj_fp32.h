@@ -136,10 +136,10 @@ __device__ __forceinline__ q32 kind2(const q32& b) {
 
 // First term of a product; kb: the root's pattern bits (kPatSquare,
 // kPatGeneral); r.ex(i) is read only for a general exponent.
-template <class Acc>
+template <bool kGeneral, class Acc>
 __device__ __forceinline__ q32 term32(const F4& rt, int kb, const Acc& r, int i, const q32& q, float& L, float& vx) {
   const q32 b = qsub32(q, rt);
-  if (kb & kPatGeneral) return general32(r.ex(i), b, L);
+  if (kGeneral && (kb & kPatGeneral)) return general32(r.ex(i), b, L);
   if (kb & kPatSquare) return kind2(b);
   return kind1(b, vx);
 }
@@ -185,7 +185,7 @@ __device__ __forceinline__ RootAcc<QP, PP, EP> root_acc(QP q, PP p, EP e, int of
 template <int kRN, bool kGeneral, class Acc>
 __device__ __forceinline__ q32 poly32(const Acc& r, int n, const q32& q, int& E, float& L, float& hi, float& lo,
                                       bool& bad, float& vx) {
-  q32 t = term32(r.root(0), r.pat(0), r, 0, q, L, vx);
+  q32 t = term32<kGeneral>(r.root(0), r.pat(0), r, 0, q, L, vx);
   int mn = 0;
   int ehi = -(1 << 30), elo = 1 << 30;
   auto group = [&](int i) {
