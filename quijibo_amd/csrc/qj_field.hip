@@ -349,15 +349,19 @@ __global__ __launch_bounds__(kBlock) void k_batch64_list(const ChunkDesc* __rest
     const uint32_t e = kFromEnd ? list[list_cap - 1 - i] : list[i];
     const int c = (int)(e / (uint32_t)chunk_voxels);
     const int local = (int)(e - (uint32_t)c * (uint32_t)chunk_voxels);
-    // consecutive entries nearly always come from one chunk: then the chunk
-    // index is wave-uniform and its root records come in scalar loads
+    // the list interleaves the chunks that were in flight together, but a
+    // wave's entries nearly always come from one request: chunks of a request
+    // share its map arguments, roots and output, so those are then
+    // wave-uniform and the root records come in scalar loads; only the
+    // chunk's first voxel is per lane
     const int cw = __builtin_amdgcn_readfirstlane(c);
-    if (__ballot(c != cw) == 0ull) {
-      const ChunkDesc& d = descs[cw];
-      const int64_t idx = d.start + local;
+    const ChunkDesc& dw = descs[cw];
+    const ChunkDesc& dl = descs[c];
+    if (__ballot(dl.r64 != dw.r64) == 0ull) {
+      const int64_t idx = dl.start + local;
       typedef const __attribute__((address_space(4))) RootRec64 cR64;
-      const Res64 r = eval64<kPow>(d.a, (cR64*)d.r64, chunk_start(d.a, idx));
-      if (r.written) d.out[idx] = r.F;
+      const Res64 r = eval64<kPow>(dw.a, (cR64*)dw.r64, chunk_start(dw.a, idx));
+      if (r.written) dw.out[idx] = r.F;
     } else {
       const ChunkDesc& d = descs[c];
       const int64_t idx = d.start + local;
