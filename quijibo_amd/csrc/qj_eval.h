@@ -34,36 +34,123 @@ __device__ __forceinline__ RootRec64 load_root64(RP p) {
   return RootRec64{p->w, p->x, p->y, p->z, p->e, p->p};
 }
 
+// The exact forms' arithmetic with every rounding fixed by the source
+// (explicit fused multiply-adds, contraction off), so that the fast and the
+// guarded product below, and every kernel that inlines them, round alike.
+__device__ __forceinline__ q64 qsub64x(const q64& a, const RootRec64& r) {
+#pragma clang fp contract(off)
+  return q64{a.w - r.w, a.x - r.x, a.y - r.y, a.z - r.z};
+}
+__device__ __forceinline__ double partial64(const q64& b) {
+#pragma clang fp contract(off)
+  return __builtin_fma(b.z, b.z, __builtin_fma(b.y, b.y, b.x * b.x));
+}
+__device__ __forceinline__ q64 square64(const q64& b, double partial) {
+#pragma clang fp contract(off)
+  const double t = 2.0 * b.w;
+  return q64{__builtin_fma(b.w, b.w, -partial), t * b.x, t * b.y, t * b.z};
+}
+// operator*(QUATERNION, QUATERNION) (QUATERNION.cpp:154-162), same term order
+__device__ __forceinline__ q64 qmul64x(const q64& l, const q64& r) {
+#pragma clang fp contract(off)
+  q64 f;
+  f.x = __builtin_fma(l.w, r.x, __builtin_fma(r.w, l.x, __builtin_fma(-l.z, r.y, l.y * r.z)));
+  f.y = __builtin_fma(l.w, r.y, __builtin_fma(r.w, l.y, __builtin_fma(-l.x, r.z, l.z * r.x)));
+  f.z = __builtin_fma(l.w, r.z, __builtin_fma(r.w, l.z, __builtin_fma(-l.y, r.x, l.x * r.y)));
+  f.w = __builtin_fma(-l.z, r.z, __builtin_fma(-r.y, l.y, __builtin_fma(-l.x, r.x, l.w * r.w)));
+  return f;
+}
+
+__device__ __forceinline__ bool is_one(double e) { return __builtin_bit_cast(unsigned long long, e) == 0x3FF0000000000000ull; }
+__device__ __forceinline__ bool is_two(double e) { return __builtin_bit_cast(unsigned long long, e) == 0x4000000000000000ull; }
+
+// (q - r)^e with the special cases (guarded form).
 template <int kPow>
 __device__ __forceinline__ q64 term64(const RootRec64& r, const q64& q) {
-  const q64 b = qsub(q, q64{r.w, r.x, r.y, r.z});
-  if (kPow == kExactOnly || (kPow == kExactInt && (r.e == 1.0 || r.e == 2.0))) {
-    const double partial = b.x * b.x + b.y * b.y + b.z * b.z;
+  if (kPow == kPolar) return qpow(qsub(q, q64{r.w, r.x, r.y, r.z}), r.e);
+  const q64 b = qsub64x(q, r);
+  if (kPow == kExactOnly || is_one(r.e) || is_two(r.e)) {
+    const double partial = partial64(b);
     if (!(partial + b.w * b.w > 0.0)) {
       const double nan = __builtin_nan("");
       return q64{nan, nan, nan, nan};
     }
     if (!(partial > 0.0)) {
-      const double m = r.e == 1.0 ? fabs(b.w) : b.w * b.w;
+      const double m = is_one(r.e) ? fabs(b.w) : b.w * b.w;
       return q64{m, 0.0 * b.x, 0.0 * b.y, 0.0 * b.z};
     }
-    if (r.e == 1.0) return b;
-    const double t = 2.0 * b.w;
-    return q64{b.w * b.w - partial, t * b.x, t * b.y, t * b.z};
+    if (is_one(r.e)) return b;
+    return square64(b, partial);
   }
-  if (kPow == kExactOnly) return b;  // not reached: kExactOnly callers have only e = 1, 2
   return qpow(b, r.e);
 }
 
-template <int kPow = kPolar, class RP = const RootRec64*>
-__device__ __forceinline__ q64 poly64(RP r, int n, const q64& q) {
+// (q - r)^e without the special cases; vx keeps the smallest |b.x|. With
+// |b.x| >= kExactMinX for every root, partial >= 1e-300 > 0 and term64 takes
+// exactly these branches.
+constexpr double kExactMinX = 1e-150;
+// vx = min(vx, |x|) in one instruction (fmin would canonicalise both inputs
+// first); a NaN x leaves vx unchanged, and a NaN product is caught by has_nan.
+__device__ __forceinline__ void min_abs64(double& vx, double x) {
+  asm("v_min_f64 %0, %1, |%2|" : "=v"(vx) : "v"(vx), "v"(x));
+}
+template <int kPow>
+__device__ __forceinline__ q64 term64_plain(const RootRec64& r, const q64& q, double& vx) {
+  const q64 b = qsub64x(q, r);
+  min_abs64(vx, b.x);
+  if (is_one(r.e)) return b;
+  if (kPow == kExactOnly || is_two(r.e)) return square64(b, partial64(b));
+  return qpow(b, r.e);
+}
+// t <- t * (q - r)^e, the product inside each kind's branch (no copies at the merge)
+template <int kPow>
+__device__ __forceinline__ void apply64_plain(q64& t, const RootRec64& r, const q64& q, double& vx) {
+  const q64 b = qsub64x(q, r);
+  min_abs64(vx, b.x);
+  if (is_one(r.e)) {
+    t = qmul64x(t, b);
+  } else if (kPow == kExactOnly || is_two(r.e)) {
+    t = qmul64x(t, square64(b, partial64(b)));
+  } else {
+    t = qmul64x(t, qpow(b, r.e));
+  }
+}
+
+template <int kPow, class RP>
+__device__ __forceinline__ q64 poly64_guarded(RP r, int n, const q64& q) {
   q64 res = term64<kPow>(load_root64(r), q);
   for (int i = 1; i < n; i++) {
-    res = qmul(res, term64<kPow>(load_root64(r + i), q));
+    res = kPow == kPolar ? qmul(res, term64<kPow>(load_root64(r + i), q))
+                         : qmul64x(res, term64<kPow>(load_root64(r + i), q));
     if ((i & 7) == 0 && has_nan(res)) {
       const double nan = __builtin_nan("");
       return q64{nan, nan, nan, nan};
     }
+  }
+  return res;
+}
+
+// The factored product. The exact modes first run it without the special
+// cases (no per-term tests, no NaN stand-ins), and only a point where some
+// |b.x| < kExactMinX (a special case may apply) runs the guarded form again;
+// otherwise the two are the same arithmetic.
+template <int kPow = kPolar, class RP = const RootRec64*>
+__device__ __forceinline__ q64 poly64(RP r, int n, const q64& q) {
+  if (kPow == kPolar) return poly64_guarded<kPow>(r, n, q);
+  double vx = __builtin_inf();
+  q64 res = term64_plain<kPow>(load_root64(r), q, vx);
+  bool nan = false;
+  for (int i = 1; i < n; i++) {
+    apply64_plain<kPow>(res, load_root64(r + i), q, vx);
+    if ((i & 7) == 0 && has_nan(res)) {
+      nan = true;
+      break;
+    }
+  }
+  if (!(vx >= kExactMinX)) return poly64_guarded<kPow>(r, n, q);
+  if (nan) {
+    const double n4 = __builtin_nan("");
+    return q64{n4, n4, n4, n4};
   }
   return res;
 }
