@@ -22,7 +22,7 @@ PEAK_FP32_TFLOPS = 157.3   # MI355X published FP32 vector peak
 PEAK_FP64_TFLOPS = 78.6    # MI355X published FP64 vector peak
 # HBM bytes per k_grid32 launch of the 512^3 workload, measured with rocprofv3
 # FETCH_SIZE and WRITE_SIZE (each scaled by the same counter on a 1 GiB copy):
-# 0.226 MB fetched + 1113.24 MB written.
+# 0.160 MB fetched + 1113.24 MB written.
 K_GRID32_TRAFFIC_BYTES = 1113401623
 
 
