@@ -282,3 +282,45 @@ def test_edge_cases(gpu, precision):
     r = api.Rational(api.POLYNOMIAL_4D(w.top.roots, w.top.powers))
     empty = torch.empty((0, 3), dtype=torch.float64, device="cuda")
     assert api.points_eval(empty, r, api.make_params(3), precision).numel() == 0
+
+
+def _special_case_workload():
+    """Roots placed on the grid so that the first iteration meets the exact
+    forms' special cases: q - r = 0 (a root at a cell centre), v = 0 with
+    w < 0 (x, y, z of q - r all zero), and q - r with only its x part zero.
+    Voxel q0 = (X, Y, Z, slice) in (w, x, y, z); cell centres of a 4^3 grid on
+    [-1.5, 1.5]^3 are +-0.375 and +-1.125."""
+    rng = np.random.default_rng(11)
+    special_top = np.array([[0.375, -0.375, 1.125, 0.0],    # b = 0 at (0.375, -0.375, 1.125)
+                            [0.9, 0.375, 0.375, 0.0],       # v = 0, w = X - 0.9 for Y = Z = 0.375
+                            [-0.2, -1.125, 0.3, 0.1]])      # b.x = 0 where Y = -1.125
+    special_bot = np.array([[1.4, -0.375, -0.375, 0.0],     # v = 0 (squared) for Y = Z = -0.375
+                            [-1.125, 0.375, -1.125, 0.0]])  # b = 0 (squared)
+    top = np.vstack([special_top, rng.normal(scale=0.6, size=(5, 4))])
+    bot = np.vstack([special_bot, rng.normal(scale=0.6, size=(4, 4))])
+    tp = np.array([1.0, 1.0, 2.0, 1.0, 2.0, 1.0, 1.0, 2.0])
+    bp = np.array([2.0, 2.0, 1.0, 1.0, 2.0, 1.0])
+    return W.Workload("special_cases", (4, 4, 4), W.RootSet(top, tp), W.RootSet(bot, bp), max_iter=3)
+
+
+def test_exact_form_special_cases_in_the_band(gpu):
+    """The FP64 band re-evaluation runs the exact forms without per-term tests
+    and repeats a point's product in the guarded form when some |(q - r).x| is
+    below 1e-150; roots on cell centres make that happen, and the results keep
+    the polar form's special cases (b = 0 gives NaN, v = 0 gives (|w|^e, 0))."""
+    w = _special_case_workload()
+    g64, c, _, _ = run_both(w, FP64)
+    assert_fp64_close(g64, c)
+    g32, c32, _, _ = run_both(w, FP32_BAND)
+    assert np.array_equal(np.isnan(g32), np.isnan(c)) and np.array_equal(np.isinf(g32), np.isinf(c))
+    fin = np.isfinite(c)
+    assert np.count_nonzero((g32[fin].astype(np.float32) > 0) != (c[fin].astype(np.float32) > 0)) == 0
+    # cells with Y = -0.375 or 0.375 meet (q - r).x = 0 for an exponent-1 root (top roots 0
+    # and 1), so the FP32 pass hands them to FP64, whose fast product sees min |b.x| = 0 and
+    # runs the guarded form; the FP64 tolerance holds there (data is [z][y][x])
+    sp = np.zeros(w.res[::-1], dtype=bool)
+    sp[:, 1, :] = True
+    sp[:, 2, :] = True
+    sp = sp.reshape(-1) & fin
+    assert np.max(np.abs(g32[sp] - c[sp]) / np.maximum(1.0, np.abs(c[sp]))) <= 1e-8
+    assert np.isnan(c).any()  # the b = 0 cells

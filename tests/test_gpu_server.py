@@ -96,3 +96,21 @@ def test_kept_field_blocks_until_released(gpu):
     second = srv.wait_all(1, timeout=30)
     assert second[0].id == b and second[0].status == 0
     srv.close()
+
+
+def test_special_cases_match_the_direct_evaluation(gpu):
+    """The request loop's band pass takes the same fast and guarded products
+    as qj_field_eval: identical fields for roots on cell centres."""
+    from test_gpu_field import _special_case_workload
+    w = _special_case_workload()
+    srv = S.Server(hbm_budget_bytes=64 << 20, chunk_voxels=256, chunks_per_launch=4, precision=FP32_BAND)
+    rid = srv.submit(w, keep_field=True)
+    c = srv.wait_all(1, timeout=60)
+    assert len(c) == 1 and c[0].status == 0
+    got = srv.fetch(c[0], device=False)
+    srv.release(rid)
+    srv.close()
+    ref = _direct(w, FP32_BAND)
+    assert np.array_equal(np.isnan(got), np.isnan(ref))
+    fin = np.isfinite(ref)
+    assert np.array_equal(got[fin], ref[fin]) and np.array_equal(np.isinf(got), np.isinf(ref))
