@@ -10,14 +10,20 @@
 // sequences, or FP32 FMA plus native transcendentals).
 //
 // Kernels:
-//   k_grid64      FP64, reference expression order (the "plain FP64" method).
-//   k_grid32      FP32 bulk with exact power-of-two range scaling; marks the
-//                 band |F| < band, non-finite results and near-threshold
-//                 escape / black-hole decisions into a compacted list
-//                 (wave ballot + one atomic per block).
-//   k_list64      FP64 re-evaluation of a compacted list (band or points).
-//   k_list64x     FP64 with an explicit binary exponent (extended range) for
-//                 cells that overflowed: the reference's long double re-pass.
+//   k_grid64        FP64, reference expression order (the "plain FP64" method).
+//   k_grid32        FP32 bulk with exact power-of-two range scaling (device
+//                   code in qj_fp32.h); marks the band |F| < band, non-finite
+//                   results and near-threshold escape / black-hole decisions
+//                   into a compacted list (wave ballot + one atomic per block).
+//   k_points32      the same for a list of points.
+//   k_list64        FP64 re-evaluation of a compacted list (band or points);
+//                   k_list64<.., true> runs eval64x, FP64 with an explicit
+//                   binary exponent (extended range) for cells that
+//                   overflowed: the reference's long double re-pass.
+//   k_batch32, k_batch64_list, k_batch64
+//                   the request loop's work-queue kernels over chunks of
+//                   many requests (FP32 bulk per exponent variant, FP64 band
+//                   lists, plain FP64).
 #include <float.h>
 
 #include "qj_geom.h"
