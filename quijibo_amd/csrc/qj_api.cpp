@@ -53,6 +53,15 @@ void make_geom(const qj_grid* g, GridGeom* o) {
   o->div_xy = make_fastdiv(xy > 0 && xy <= 0xffffffffull ? (uint32_t)xy : 1u);
 }
 
+double near_bits(const std::vector<RootRec64>& roots) {
+  double rmax = 0.0, emax = 0.0;
+  for (const RootRec64& r : roots) {
+    rmax = fmax(rmax, sqrt(r.w * r.w + r.x * r.x + r.y * r.y + r.z * r.z));
+    emax = fmax(emax, fabs(r.e));
+  }
+  return emax * log2(2.0 * rmax + 1.0) + 1.0;
+}
+
 double term_bits(const std::vector<RootRec64>& roots, double escape) {
   double tb = 0.0;
   for (const RootRec64& r : roots) {
@@ -102,6 +111,7 @@ qj_status make_map_args(const qj_grid* g, const qj_rational* f, const qj_map_par
   a->sc_top = f->sc_top;
   a->sc_bottom = f->sc_bottom;
   a->term_bits = term_bits(f->h64, p->escape);
+  a->near_bits = near_bits(f->h64);
   return QJ_OK;
 }
 

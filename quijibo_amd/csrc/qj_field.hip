@@ -281,22 +281,22 @@ __device__ __forceinline__ Roots32K roots32_const(const F4* blk, int n) {
 }
 
 // Batch counters (uint32): [0] band count of integer-exponent chunks (list from
-// the front), [1] band count of general chunks (list from the back), [2..4]
-// FP32 queues per variant, [5] FP64 queue.
-enum { BC_BAND_INT = 0, BC_BAND_GEN = 1, BC_Q32 = 2, BC_Q64 = 5, BC_N = 8 };
+// the front), [1] band count of general chunks (list from the back), [2..5]
+// FP32 queues per variant, [6] FP64 queue.
+enum { BC_BAND_INT = 0, BC_BAND_GEN = 1, BC_Q32 = 2, BC_Q64 = 6, BC_N = 8 };
 
 // Descriptors are grouped by FP32 variant: chunks [0, g1) have general
-// exponents, [g1, g2) integer exponents renormalised every 4 terms, [g2,
-// total) integer exponents renormalised every 2.
+// exponents, [g1, g2) integer exponents renormalised every 4 terms, [g2, g3)
+// every 2 and [g3, total) every 8.
 template <int kVar>
 __global__ __launch_bounds__(kBlock) void k_batch32(const ChunkDesc* __restrict__ descs, int blocks_per_chunk,
                                                     int chunk_voxels, uint32_t* __restrict__ blist, int64_t list_cap,
                                                     uint32_t* __restrict__ counters) {
-  constexpr int kRN = kVar == 1 ? 4 : 2;
+  constexpr int kRN = kVar == 1 ? 4 : (kVar == 3 ? 8 : 2);
   constexpr bool kGen = kVar == 0;
   const ChunkDesc& d0 = descs[0];
-  const int c0 = kVar == 0 ? 0 : (kVar == 1 ? d0.g1 : d0.g2);
-  const int c1 = kVar == 0 ? d0.g1 : (kVar == 1 ? d0.g2 : d0.total);
+  const int c0 = kVar == 0 ? 0 : (kVar == 1 ? d0.g1 : (kVar == 2 ? d0.g2 : d0.g3));
+  const int c1 = kVar == 0 ? d0.g1 : (kVar == 1 ? d0.g2 : (kVar == 2 ? d0.g3 : d0.total));
   const int tiles = (c1 - c0) * blocks_per_chunk;
   for (int tile = next_tile(counters + BC_Q32 + kVar); tile < tiles; tile = next_tile(counters + BC_Q32 + kVar)) {
     const int c = c0 + tile / blocks_per_chunk;
@@ -402,7 +402,11 @@ static int resident_blocks(const void* kernel) {
   return per_cu * cus;
 }
 
-int batch_variant(const MapArgs& a) { return a.gen ? 0 : (a.term_bits <= kRN4MaxTermBits ? 1 : 2); }
+int batch_variant(const MapArgs& a) {
+  if (a.gen) return 0;
+  const int rn = rn_of(a);
+  return rn == 8 ? 3 : (rn == 4 ? 1 : 2);
+}
 
 qj_status launch_batch(const ChunkDesc* d_desc, int nchunks, int chunk_voxels, qj_precision prec, uint32_t* d_band,
                        uint32_t* d_counters, unsigned variants, hipStream_t s) {
@@ -419,6 +423,7 @@ qj_status launch_batch(const ChunkDesc* d_desc, int nchunks, int chunk_voxels, q
   static const int g0 = resident_blocks((const void*)k_batch32<0>);
   static const int g1 = resident_blocks((const void*)k_batch32<1>);
   static const int g2 = resident_blocks((const void*)k_batch32<2>);
+  static const int g3 = resident_blocks((const void*)k_batch32<3>);
   // a variant with no chunks is not launched: an empty persistent grid still waits for free
   // compute units behind the other in-flight launch
   if (variants & 2u) {
@@ -429,11 +434,15 @@ qj_status launch_batch(const ChunkDesc* d_desc, int nchunks, int chunk_voxels, q
     hipLaunchKernelGGL(k_batch32<2>, dim3(g2), dim3(kBlock), 0, s, d_desc, bpc, chunk_voxels, d_band, cap, d_counters);
     QJ_HIP(hipGetLastError());
   }
+  if (variants & 8u) {
+    hipLaunchKernelGGL(k_batch32<3>, dim3(g3), dim3(kBlock), 0, s, d_desc, bpc, chunk_voxels, d_band, cap, d_counters);
+    QJ_HIP(hipGetLastError());
+  }
   if (variants & 1u) {
     hipLaunchKernelGGL(k_batch32<0>, dim3(g0), dim3(kBlock), 0, s, d_desc, bpc, chunk_voxels, d_band, cap, d_counters);
     QJ_HIP(hipGetLastError());
   }
-  if (variants & 6u) {  // band voxels of the integer-exponent variants
+  if (variants & 14u) {  // band voxels of the integer-exponent variants
     hipLaunchKernelGGL((k_batch64_list<kExactOnly, false>), dim3(2048), dim3(kBlock), 0, s, d_desc, chunk_voxels,
                        (const uint32_t*)d_band, cap, (const uint32_t*)(d_counters + BC_BAND_INT));
     QJ_HIP(hipGetLastError());
@@ -511,7 +520,10 @@ qj_status launch_field(qj_workspace* ws, const MapArgs& a, const qj_rational* f,
     if (a.gen)
       hipLaunchKernelGGL((k_grid32<2, true>), g, b, 0, s, a, k, (const F4*)f->d32, d_out, ws->d_counters, ws->d_list,
                          ws->d_list_count);
-    else if (a.term_bits <= kRN4MaxTermBits)
+    else if (rn_of(a) == 8)
+      hipLaunchKernelGGL((k_grid32<8, false>), g, b, 0, s, a, k, (const F4*)f->d32, d_out, ws->d_counters, ws->d_list,
+                         ws->d_list_count);
+    else if (rn_of(a) == 4)
       hipLaunchKernelGGL((k_grid32<4, false>), g, b, 0, s, a, k, (const F4*)f->d32, d_out, ws->d_counters, ws->d_list,
                          ws->d_list_count);
     else
@@ -556,7 +568,10 @@ qj_status launch_points(qj_workspace* ws, const MapArgs& a, const qj_rational* f
     if (a.gen)
       hipLaunchKernelGGL((k_points32<PointSource, 2, true>), g, b, 0, s, a, k, (const F4*)f->d32, d_pts, n, d_out,
                          ws->d_counters, ws->d_list, ws->d_list_count);
-    else if (a.term_bits <= kRN4MaxTermBits)
+    else if (rn_of(a) == 8)
+      hipLaunchKernelGGL((k_points32<PointSource, 8, false>), g, b, 0, s, a, k, (const F4*)f->d32, d_pts, n, d_out,
+                         ws->d_counters, ws->d_list, ws->d_list_count);
+    else if (rn_of(a) == 4)
       hipLaunchKernelGGL((k_points32<PointSource, 4, false>), g, b, 0, s, a, k, (const F4*)f->d32, d_pts, n, d_out,
                          ws->d_counters, ws->d_list, ws->d_list_count);
     else

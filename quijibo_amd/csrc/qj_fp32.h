@@ -188,13 +188,19 @@ __device__ __forceinline__ q32 poly32(const Acc& r, int n, const q32& q, int& E,
   q32 t = term32<kGeneral>(r.root(0), r.pat(0), r, 0, q, L, vx);
   int mn = 0;
   int ehi = -(1 << 30), elo = 1 << 30;
+  // roots come in sub-groups of up to 4: one scalar load of four roots and
+  // one pattern word (the kinds of roots i..i+3) each
+  constexpr int kSub = kRN < 4 ? kRN : 4;
   auto group = [&](int i) {
-    F4 g[kRN];
 #pragma unroll
-    for (int j = 0; j < kRN; j++) g[j] = r.root(i + j);
-    const int pat = r.pat(i);
+    for (int s0 = 0; s0 < kRN; s0 += kSub) {
+      F4 g[kSub];
 #pragma unroll
-    for (int j = 0; j < kRN; j++) apply32<kGeneral>(t, g[j], pat >> (2 * j), r, i + j, q, L, vx);
+      for (int j = 0; j < kSub; j++) g[j] = r.root(i + s0 + j);
+      const int pat = r.pat(i + s0);
+#pragma unroll
+      for (int j = 0; j < kSub; j++) apply32<kGeneral>(t, g[j], pat >> (2 * j), r, i + s0 + j, q, L, vx);
+    }
     const int k = renorm32(t, E);
     if (kRN > 2) mn = min(mn, k);
     if (kGeneral) {
@@ -227,6 +233,22 @@ __device__ __forceinline__ q32 poly32(const Acc& r, int n, const q32& q, int& E,
 // Renormalising every 4 terms needs 4 terms of up to 2^term_bits to stay in
 // FP32 range: 4 * 30 + 1 < 127.
 constexpr double kRN4MaxTermBits = 30.0;
+// Every 8 terms (integer exponents only): the first group holds the first
+// term and 8 more, 9 * (15 - 2) + 1 < 127 (term_bits carries a 2-bit margin).
+// A running product that underflowed inside a group must end the group below
+// 2^-60 (the check that sends it to FP64): a product of at most 9 terms
+// starting at >= 2^-1 falls under 2^-126 only if some term is below 2^-14,
+// that is q is within 1 of a root, and then every term is at most 2^near_bits,
+// so the group ends below 2^(-126 + 8 near_bits) <= 2^-62 when near_bits <= 8.
+constexpr double kRN8MaxTermBits = 15.0;
+constexpr double kRN8MaxNearBits = 8.0;
+// FP32 variant for an integer-exponent map: terms between renormalisations.
+// 8-term groups only pay when a polynomial has at least one full group after
+// its first term.
+__host__ __device__ inline int rn_of(const MapArgs& a) {
+  if (a.term_bits <= kRN8MaxTermBits && a.near_bits <= kRN8MaxNearBits && (a.nt > 8 || a.nb > 8)) return 8;
+  return a.term_bits <= kRN4MaxTermBits ? 4 : 2;
+}
 
 struct Res32 {
   float F;
@@ -263,18 +285,22 @@ __device__ __forceinline__ Res32 eval32(const MapArgs& a, const AccT& top, const
   float vx = __builtin_inff();
   bool amb = false, bad = false, zero = false;
   int it = 0;
-  // FP64 running products between two checkpoints grow by at most (kRN - 1) terms
+  // FP64 running products between two checkpoints grow by at most (kRN - 1)
+  // terms; with 8-term groups a product may also dip up to 60 + 7 near_bits
+  // (<= 116) bits below a checkpoint before the group ends, so the low side
+  // keeps a further 64 bits from FP64's range
   const float lim = 1020.0f - (float)(kRN - 1) * k.term_bits;
+  const float lim_lo = lim - (kRN > 4 ? 64.0f : 0.0f);
   while (true) {
     int ET = 0, EB = 0;
     float L = 0.0f, hiT = -1e30f, loT = 1e30f;
     const q32 T = poly32<kRN, kGeneral>(top, a.nt, q, ET, L, hiT, loT, bad, vx);
-    bad |= !(hiT < lim && loT > -lim);
+    bad |= !(hiT < lim && loT > -lim_lo);
     float l2 = k.log2c;
     if (a.nb > 0) {
       float LB = 0.0f, hiB = -1e30f, loB = 1e30f;
       const q32 B = poly32<kRN, kGeneral>(bot, a.nb, q, EB, LB, hiB, loB, bad, vx);
-      bad |= !(hiB < lim && loB > -lim);
+      bad |= !(hiB < lim && loB > -lim_lo);
       const float nb2 = B.w * B.w + B.x * B.x + B.y * B.y + B.z * B.z;
       const float lB = 0.5f * __log2f(nb2) + (float)EB + LB;
       const q32 D = qmul32(T, q32{B.w, -B.x, -B.y, -B.z});  // T conj(B) = T B^-1 |B|^2

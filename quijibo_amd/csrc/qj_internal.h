@@ -92,6 +92,7 @@ void make_geom(const qj_grid* g, GridGeom* out);
 // Fill MapArgs from the C structs (host arithmetic mirrors FIELD_3D).
 qj_status make_map_args(const qj_grid* g, const qj_rational* f, const qj_map_params* p, MapArgs* a);
 double term_bits(const std::vector<RootRec64>& roots, double escape);
+double near_bits(const std::vector<RootRec64>& roots);
 qj_status ensure_list(qj_workspace* ws, int64_t n, hipStream_t s);
 qj_status ensure_partials(qj_workspace* ws, int64_t n, hipStream_t s);
 // Records an event on s when timing is enabled (slot order: bulk0, bulk1, band0, band1).
@@ -115,10 +116,11 @@ struct ChunkDesc {
   int32_t count;  // voxels in this chunk (0 = unused slot)
   Consts32 k;     // FP32 constants of the chunk's request (consts32)
   int32_t total;  // chunks in use in this launch (read from the first descriptor)
-  int32_t g1, g2;  // variant groups (first descriptor): [0,g1) general, [g1,g2) RN4 integer, [g2,total) RN2 integer
+  int32_t g1, g2, g3;  // variant groups (first descriptor): [0,g1) general, [g1,g2) RN4 integer, [g2,g3) RN2
+                       // integer, [g3,total) RN8 integer
 };
-// FP32 variant of a chunk's root set: 0 general exponents, 1 integer exponents
-// with a small term bound, 2 other integer exponents.
+// FP32 variant of a chunk's root set: 0 general exponents; integer exponents
+// renormalised every 4 terms (1), every 2 (2) or every 8 (3) (rn_of).
 int batch_variant(const MapArgs& a);
 // Enqueue one batch: FP32 bulk over every chunk then FP64 for the band (or
 // FP64 for all when prec == QJ_FP64). d_band/d_band_count: scratch of

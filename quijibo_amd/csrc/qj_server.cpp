@@ -112,7 +112,7 @@ struct Slot {
   uint32_t* d_band_count = nullptr;  // batch counters: band counts and work queues (8 x uint32)
   // one graph per set of FP32 variants present in a batch (bit v = variant v);
   // an FP64 server uses exec[0] only
-  hipGraphExec_t exec[8] = {};
+  hipGraphExec_t exec[16] = {};
   unsigned variants = 0;
   hipEvent_t done = nullptr;
   hipStream_t stream = nullptr;  // one stream per slot: one launch's FP64 band pass overlaps the next's FP32 bulk
@@ -228,15 +228,18 @@ static int build_batch(qj_server* s, Slot& sl) {
   }
   // group by FP32 variant (the kernels take one group each); order inside a
   // group keeps the shortest-remaining-work order
-  int g[4] = {0, 0, 0, 0};
+  int g[5] = {0, 0, 0, 0, 0};
   for (int i = 0; i < n; i++) g[batch_variant(tmp[i].a) + 1]++;
-  int pos[3] = {0, g[1], g[1] + g[2]};
+  int pos[4] = {0, g[1], g[1] + g[2], g[1] + g[2] + g[3]};
   for (int i = 0; i < n; i++) sl.h_desc[pos[batch_variant(tmp[i].a)]++] = tmp[i];
   for (int i = n; i < maxc; i++) sl.h_desc[i].count = 0;
   sl.h_desc[0].total = n;
   sl.h_desc[0].g1 = g[1];
   sl.h_desc[0].g2 = g[1] + g[2];
-  sl.variants = s->cfg.precision == QJ_FP64 ? 0u : (g[1] ? 1u : 0u) | (g[2] ? 2u : 0u) | (g[3] ? 4u : 0u);
+  sl.h_desc[0].g3 = g[1] + g[2] + g[3];
+  sl.variants = s->cfg.precision == QJ_FP64
+                    ? 0u
+                    : (g[1] ? 1u : 0u) | (g[2] ? 2u : 0u) | (g[3] ? 4u : 0u) | (g[4] ? 8u : 0u);
   return n;
 }
 
@@ -358,7 +361,7 @@ static qj_status capture_slot(qj_server* s, Slot& sl) {
   }
   QJ_HIP(hipEventCreateWithFlags(&sl.done, hipEventDisableTiming));
   QJ_HIP(hipStreamCreateWithFlags(&sl.stream, hipStreamNonBlocking));
-  const unsigned first = s->cfg.precision == QJ_FP64 ? 0u : 1u, last = s->cfg.precision == QJ_FP64 ? 0u : 7u;
+  const unsigned first = s->cfg.precision == QJ_FP64 ? 0u : 1u, last = s->cfg.precision == QJ_FP64 ? 0u : 15u;
   for (unsigned v = first; v <= last; v++) {
     hipGraph_t g;
     QJ_HIP(hipStreamBeginCapture(sl.stream, hipStreamCaptureModeThreadLocal));

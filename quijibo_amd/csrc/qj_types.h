@@ -57,6 +57,7 @@ struct MapArgs {
   int32_t nt, nb;
   double sc_top, sc_bottom;  // power scalars
   double term_bits;          // max over roots of |e| log2(escape + |root|) + 2: bound on one term's log2 size
+  double near_bits;          // max |e| log2(2 max|root| + 1) + 1: bound on a term's log2 size when q is within 1 of a root
 };
 
 // Per-map constants of the FP32 kernels (log2 units).

@@ -12,19 +12,22 @@ pytestmark = pytest.mark.gpu
 
 
 def _mixed(n=12, seed=5):
-    """Requests of all three FP32 variants in one stream: integer exponents with
+    """Requests of all four FP32 variants in one stream: exponents 1 and 2 with
     escape 2000 (renormalise every 4 terms), integer exponents with escape 1e6
-    (term bound over 30 bits: every 2 terms) and general exponents."""
+    (term bound over 30 bits: every 2 terms), general exponents, and exponents
+    all 1 with escape 2000 (every 8 terms)."""
     rng = np.random.default_rng(seed)
     out = []
     for i in range(n):
         e = int(rng.choice([24, 40, 64, 80]))
         nr = int(rng.integers(8, 64))
         nt = nr // 2
-        kind = i % 3
+        kind = i % 4
         tp = rng.integers(1, 3, size=nt).astype(float)
         if kind == 2:
             tp = rng.choice([1.0, 1.5, 2.0, 0.75], size=nt)
+        if kind == 3:
+            tp = np.ones(nt)
         top = W.RootSet(rng.normal(scale=0.7, size=(nt, 4)), tp)
         bot = W.RootSet(rng.normal(scale=0.7, size=(nr - nt, 4)), np.ones(nr - nt))
         w = W.Workload("m%d" % i, (e, e + 3, e - 5), top, bot, max_iter=int(rng.integers(3, 6)),
