@@ -81,6 +81,8 @@ def main():
     ap.add_argument("--serve-rate", type=float, default=200.0)
     ap.add_argument("--serve-budget-gib", type=float, default=64.0)
     ap.add_argument("--probe", action="store_true", help="counter probe: two field launches and a 1 GiB copy")
+    ap.add_argument("--mc-probe", action="store_true",
+                    help="counter probe: marching cubes of one 256^3 torus tile (twice), then a 1 GiB copy")
     ap.add_argument("--serve-only", action="store_true", help="run only the serving stream")
     ap.add_argument("--serve-launch-work", type=float, default=0.0,
                     help="root-term iterations per request-loop launch (0 = library default)")
@@ -109,6 +111,10 @@ def main():
 
     if args.serve_only:
         print(json.dumps({"serving": serve_stream(args, log)}))
+        return
+
+    if args.mc_probe:
+        print(json.dumps({"mc_probe": mc_probe(args, log)}))
         return
 
     if args.probe:
@@ -338,6 +344,42 @@ def gradients(args, log):
     out["workload"] = ("c5: 40000 points, 64+64 roots, maxIter 3, gradient over 512 root components + 128 powers; "
                        "ap_97: 97^3 grid, 32+32 roots, one iteration, escape 20, score + exponent gradients")
     return out
+
+
+def mc_probe(args, log):
+    """Counter probe for marching cubes: the 256^3 tile (1, 1, 0) of the 1024^3 torus workload
+    (it holds part of the surface) evaluated FP32+band, meshed twice (a warm-up and the run the
+    counters are read from), then a plain 1 GiB device-to-device copy whose FETCH_SIZE /
+    WRITE_SIZE readings scale the others."""
+    import ctypes
+    import torch
+    from quijibo_amd import api, workloads as W
+    from quijibo_amd._lib import FP32_BAND, MAP_BLACK_HOLE
+    w = W.config3(1024, 256)
+    c, tl = W.tile_grid(w, 1, 1, 0, 256)
+    g = api.make_grid((256, 256, 256), c, tl)
+    r = api.Rational(api.POLYNOMIAL_4D(w.top.roots, w.top.powers), api.POLYNOMIAL_4D(w.bottom.roots, w.bottom.powers))
+    params = api.make_params(w.max_iter, w.escape, w.exp_scaling, w.slice, MAP_BLACK_HOLE)
+    ws = api.Workspace()
+    fld = api.field_eval(g, r, params, FP32_BAND, args.band, workspace=ws)
+    torch.cuda.synchronize()
+    L = api._lib.load()
+    nv, nt = ctypes.c_int64(), ctypes.c_int64()
+    runs = []
+    for i in range(2):
+        log("mc probe: marching cubes %d" % i)
+        t0 = time.perf_counter()
+        api.check(L.qj_mc_run(ws.handle, api._ptr(fld), ctypes.byref(g), 0.0, ctypes.byref(nv), ctypes.byref(nt),
+                              api._stream_ptr(None)))
+        torch.cuda.synchronize()
+        runs.append({"ms": round((time.perf_counter() - t0) * 1e3, 3), "vertices": nv.value, "triangles": nt.value})
+    src = torch.ones(1 << 27, dtype=torch.float64, device="cuda")
+    dst = torch.empty_like(src)
+    torch.cuda.synchronize()
+    log("mc probe: copy 1 GiB")
+    dst.copy_(src)
+    torch.cuda.synchronize()
+    return {"tile": [1, 1, 0], "points": 256 ** 3, "runs": runs, "copy_bytes": 8 << 27}
 
 
 def c3_tiles(args, log):
