@@ -1,7 +1,8 @@
 """Oracle (test infrastructure only): ctypes bindings to the C restatement in qo.c.
 
 Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg import this
-package, as the checker and the CPU baseline. The shipped library never uses it.
+package, as the reference results the GPU path is compared with and as the CPU
+baseline. The shipped library never uses it.
 """
 import ctypes
 import os

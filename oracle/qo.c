@@ -3,7 +3,7 @@
  * (FP64; x87 long double only where the reference itself switches to it).
  *
  * TEST INFRASTRUCTURE ONLY: used by tests/, __graft_entry__.smoke() and the
- * cpu_baseline leg of bench.py as the checker and the CPU baseline. The
+ * cpu_baseline leg of bench.py, to compare against and as the CPU baseline. The
  * shipped library never links or calls it.
  *
  * Built with -ffp-contract=off so every expression rounds in the reference's

@@ -4,7 +4,7 @@
  *
  * TEST INFRASTRUCTURE ONLY. Nothing in the shipped library links, loads or
  * calls this code. Only tests/, __graft_entry__.smoke() and bench.py's
- * cpu_baseline leg use it, and only as the checker / CPU baseline.
+ * cpu_baseline leg use it, and only to compare against / as the CPU baseline.
  *
  * Every function cites the reference file:line it restates. Quaternion slot
  * order is the reference's QUATERNION(w, x, y, z) (QUATERNION.h:36,150-154).
