@@ -585,9 +585,13 @@ void mc_free(qj_workspace* ws) {
 // band does: the values differ from the polar form's in the last few ulps,
 // the same order as the device math library against the host's.
 
+// kPow: kExactOnly when every exponent is 1 or 2 (no polar code, far fewer
+// registers), kExactInt otherwise; both give the same values for exponents 1
+// and 2. kLag: the map in Lagrange order (QJ_MAP_LAGRANGE).
+template <int kPow, bool kLag>
 __device__ __forceinline__ double nonlinear_value(const MapArgs& a, const RootRec64* __restrict__ roots,
                                                   const double* p, double iso) {
-  if (a.flags & QJ_MAP_LAGRANGE) {
+  if (kLag) {
     // lagrangeNonlinearValue (TRIANGLE_MESH.cpp:1684-1730): an infinite |q|
     // restores the previous iterate and stops. Before the first iteration the
     // reference's previous magnitude is uninitialised; here it is |q0|.
@@ -595,7 +599,7 @@ __device__ __forceinline__ double nonlinear_value(const MapArgs& a, const RootRe
     double magnitude = qmag(q), prev = magnitude;
     int it = 0;
     while (magnitude < a.escape && it < a.max_iter) {
-      q = qscale(lagrange64<kExactInt>(roots, a.nt, a.nb, q), a.exp_scaling);
+      q = qscale(lagrange64<kPow>(roots, a.nt, a.nb, q), a.exp_scaling);
       magnitude = qmag(q);
       if (isinf(magnitude)) {
         magnitude = prev;
@@ -607,7 +611,7 @@ __device__ __forceinline__ double nonlinear_value(const MapArgs& a, const RootRe
     }
     return log(magnitude) - iso;
   }
-  const Res64 r = eval64<kExactInt>(a, roots, q64{p[0], p[1], p[2], a.slice});
+  const Res64 r = eval64<kPow>(a, roots, q64{p[0], p[1], p[2], a.slice});
   return r.F - iso;
 }
 
@@ -666,13 +670,14 @@ __global__ __launch_bounds__(kMcBlock) void k_nl_edges(const uint8_t* __restrict
 // neighbour (computeNonlinearEdgeInterpolationsHuge :2034-2123 with
 // midpointSearch :2232-2271). va: value at (x, y, z). Counts edges whose end
 // values do not straddle the surface in *bad (the reference exits there).
+template <int kPow, bool kLag>
 __device__ __forceinline__ void place_vertex(const MapArgs& a, const RootRec64* __restrict__ roots, double iso,
                                              int x, int y, int z, int axis, const double* pa, double va,
                                              double* __restrict__ out, unsigned int* __restrict__ bad) {
 #pragma clang fp contract(off)
   double pb[3];
   cell_center(a, x + (axis == 0), y + (axis == 1), z + (axis == 2), pb);
-  const double vb = nonlinear_value(a, roots, pb, iso);
+  const double vb = nonlinear_value<kPow, kLag>(a, roots, pb, iso);
   // first = lower endpoint; the positive end is the first unless firstValue < 0
   double pp[3], pn[3], vp, vn;
   if (va < 0) {
@@ -696,7 +701,7 @@ __device__ __forceinline__ void place_vertex(const MapArgs& a, const RootRec64* 
       break;
     }
     if (rec >= 6) break;
-    const double vm = nonlinear_value(a, roots, m, iso);
+    const double vm = nonlinear_value<kPow, kLag>(a, roots, m, iso);
     if (fabs(vm) < 1e-8) break;
     if (vm < 0) {
       pn[0] = m[0]; pn[1] = m[1]; pn[2] = m[2]; vn = vm;
@@ -714,6 +719,7 @@ __device__ __forceinline__ void place_vertex(const MapArgs& a, const RootRec64* 
 // with edges and their vertices in LDS, then evaluates one listed point per
 // thread (the lower end value, shared by the point's edges) and bisects one
 // vertex per thread: the FP64 map evaluations run on full waves.
+template <int kPow, bool kLag>
 __global__ __launch_bounds__(kMcBlock) void k_nl_vertices(MapArgs a, const RootRec64* __restrict__ roots,
                                                           double iso, int64_t p0, int64_t np,
                                                           const uint8_t* __restrict__ emask,
@@ -748,7 +754,7 @@ __global__ __launch_bounds__(kMcBlock) void k_nl_vertices(MapArgs a, const RootR
     const int y = (int)(r / X), x = (int)(r - (int64_t)y * X), z = (int)(p0 + lz);
     double pa[3];
     cell_center(a, x, y, z, pa);
-    s_va[i] = nonlinear_value(a, roots, pa, iso);
+    s_va[i] = nonlinear_value<kPow, kLag>(a, roots, pa, iso);
   }
   __syncthreads();
   const uint32_t first = base + off[j0];  // vertex id of the block's first listed edge
@@ -760,7 +766,7 @@ __global__ __launch_bounds__(kMcBlock) void k_nl_vertices(MapArgs a, const RootR
     double pa[3];
     cell_center(a, x, y, z, pa);
     const int64_t k = (int64_t)first + sl;
-    place_vertex(a, roots, iso, x, y, z, axis, pa, s_va[l], verts + 3 * k, bad);
+    place_vertex<kPow, kLag>(a, roots, iso, x, y, z, axis, pa, s_va[l], verts + 3 * k, bad);
     edges[4 * k + 0] = x;
     edges[4 * k + 1] = y;
     edges[4 * k + 2] = z;
@@ -768,6 +774,16 @@ __global__ __launch_bounds__(kMcBlock) void k_nl_vertices(MapArgs a, const RootR
   }
 }
 
+// Launch a mesher vertex kernel instantiated for the map's power mode and
+// order (see nonlinear_value).
+template <class... Args>
+static void launch_nl_vertices(const MapArgs& a, dim3 grid, hipStream_t s, Args... args) {
+  const bool lag = (a.flags & QJ_MAP_LAGRANGE) != 0;
+  if (a.gen && lag) hipLaunchKernelGGL((k_nl_vertices<kExactInt, true>), grid, dim3(kMcBlock), 0, s, a, args...);
+  else if (a.gen) hipLaunchKernelGGL((k_nl_vertices<kExactInt, false>), grid, dim3(kMcBlock), 0, s, a, args...);
+  else if (lag) hipLaunchKernelGGL((k_nl_vertices<kExactOnly, true>), grid, dim3(kMcBlock), 0, s, a, args...);
+  else hipLaunchKernelGGL((k_nl_vertices<kExactOnly, false>), grid, dim3(kMcBlock), 0, s, a, args...);
+}
 // (x, y, z, flag) of the chunk's active cells, appended in cell order
 __global__ __launch_bounds__(kMcBlock) void k_nl_cells(const uint32_t* __restrict__ active, int64_t na,
                                                        const uint8_t* __restrict__ flags, int64_t fo, int64_t X,
@@ -964,14 +980,13 @@ qj_status nlmc_run(qj_workspace* ws, const MapArgs& a0, const qj_rational* f, do
       return st;
     const int64_t skip = z0 > 0 ? 1 : 0;
     if (cnt_own > 0 && z1 - z0 > skip) {
-      hipLaunchKernelGGL(k_nl_vertices, dim3(nblocks(XY * (z1 - z0 - skip))), dim3(kMcBlock), 0, s, a, f->d64, iso,
-                         z0 + skip, z1 - z0 - skip, m.emask + skip * XY, m.cnt + skip * XY, (uint32_t)nv, m.verts,
-                         m.c_edges, m.d_bad);
+      launch_nl_vertices(a, dim3(nblocks(XY * (z1 - z0 - skip))), s, f->d64, iso, z0 + skip, z1 - z0 - skip,
+                         m.emask + skip * XY, m.cnt + skip * XY, (uint32_t)nv, m.verts, m.c_edges, m.d_bad);
       QJ_HIP(hipGetLastError());
     }
     if (cnt_next > 0) {
-      hipLaunchKernelGGL(k_nl_vertices, dim3(nblocks(XY)), dim3(kMcBlock), 0, s, a, f->d64, iso, z1, (int64_t)1,
-                         m.emask + nown, m.slice_off, (uint32_t)(nv + cnt_own), m.verts, m.c_edges, m.d_bad);
+      launch_nl_vertices(a, dim3(nblocks(XY)), s, f->d64, iso, z1, (int64_t)1, m.emask + nown, m.slice_off,
+                         (uint32_t)(nv + cnt_own), m.verts, m.c_edges, m.d_bad);
       QJ_HIP(hipGetLastError());
     }
     // 5. active cells of the chunk, in cell order
@@ -1060,6 +1075,7 @@ qj_status nlmc_cache(qj_workspace* ws, int64_t* n_edges, int32_t* edges, int64_t
   return QJ_OK;
 }
 
+template <int kPow, bool kLag>
 __global__ __launch_bounds__(kMcBlock) void k_cache_vertices(MapArgs a, const RootRec64* __restrict__ roots,
                                                              double iso, const int32_t* __restrict__ edges, int64_t ne,
                                                              double* __restrict__ verts,
@@ -1069,8 +1085,17 @@ __global__ __launch_bounds__(kMcBlock) void k_cache_vertices(MapArgs a, const Ro
   const int x = edges[4 * i], y = edges[4 * i + 1], z = edges[4 * i + 2], axis = edges[4 * i + 3];
   double pa[3];
   cell_center(a, x, y, z, pa);
-  const double va = nonlinear_value(a, roots, pa, iso);
-  place_vertex(a, roots, iso, x, y, z, axis, pa, va, verts + 3 * i, bad);
+  const double va = nonlinear_value<kPow, kLag>(a, roots, pa, iso);
+  place_vertex<kPow, kLag>(a, roots, iso, x, y, z, axis, pa, va, verts + 3 * i, bad);
+}
+
+template <class... Args>
+static void launch_cache_vertices(const MapArgs& a, dim3 grid, hipStream_t s, Args... args) {
+  const bool lag = (a.flags & QJ_MAP_LAGRANGE) != 0;
+  if (a.gen && lag) hipLaunchKernelGGL((k_cache_vertices<kExactInt, true>), grid, dim3(kMcBlock), 0, s, a, args...);
+  else if (a.gen) hipLaunchKernelGGL((k_cache_vertices<kExactInt, false>), grid, dim3(kMcBlock), 0, s, a, args...);
+  else if (lag) hipLaunchKernelGGL((k_cache_vertices<kExactOnly, true>), grid, dim3(kMcBlock), 0, s, a, args...);
+  else hipLaunchKernelGGL((k_cache_vertices<kExactOnly, false>), grid, dim3(kMcBlock), 0, s, a, args...);
 }
 
 __global__ __launch_bounds__(kMcBlock) void k_cache_tricount(const int32_t* __restrict__ cells, int64_t nc,
@@ -1185,8 +1210,7 @@ qj_status nlmc_resume(qj_workspace* ws, const MapArgs& a, const qj_rational* f, 
   if (ne) {
     QJ_HIP(hipMemcpyAsync(d_keys, keys.data(), sizeof(int64_t) * ne, hipMemcpyHostToDevice, s));
     QJ_HIP(hipMemcpyAsync(m.c_edges, h_edges, sizeof(int32_t) * 4 * ne, hipMemcpyHostToDevice, s));
-    hipLaunchKernelGGL(k_cache_vertices, dim3(nblocks(ne)), dim3(kMcBlock), 0, s, a, f->d64, iso, m.c_edges, ne,
-                       m.verts, m.d_bad);
+    launch_cache_vertices(a, dim3(nblocks(ne)), s, f->d64, iso, m.c_edges, ne, m.verts, m.d_bad);
     QJ_HIP(hipGetLastError());
   }
   int64_t nt = 0;
