@@ -198,7 +198,10 @@ __device__ __forceinline__ Res64 eval64(const MapArgs& a, RP roots, q64 q) {
   const bool adaptive = (a.flags & QJ_MAP_ADAPTIVE) != 0;
   int it = 0;
   while (magnitude < a.escape && it < a.max_iter) {
-    if (a.flags & QJ_MAP_LAGRANGE) {
+    // Lagrange order reaches eval64 only through the FP64 method (the API
+    // rejects it for the FP32 band, and the mesher evaluates it itself), so
+    // the exact-power instances carry no Lagrange code
+    if (kPow == kPolar && (a.flags & QJ_MAP_LAGRANGE)) {
       q = qscale(lagrange64<kPow>(roots, a.nt, a.nb, q), a.exp_scaling);
       magnitude = qmag(q);
       it++;
