@@ -187,7 +187,7 @@ __device__ __forceinline__ q64 lagrange64(RP roots, int nt, int nb, const q64& q
 // One voxel of computeLogScaledPowerRationalMap (OPTIMIZE_3D.cpp:1317-1362) /
 // computeRationalMapAP's first pass (:1423-1475). Overflow detection replaces
 // fetestexcept(FE_OVERFLOW): any non-finite T, B, |B|^2, quotient or |q|.
-template <int kPow = kPolar, class RP = const RootRec64*>
+template <int kPow = kPolar, class RP = const RootRec64*, bool kLagrange = true>
 __device__ __forceinline__ Res64 eval64(const MapArgs& a, RP roots, q64 q) {
   Res64 r{0.0, 0, true, false};
   double magnitude = qmag(q);
@@ -201,7 +201,7 @@ __device__ __forceinline__ Res64 eval64(const MapArgs& a, RP roots, q64 q) {
     // Lagrange order reaches eval64 only through the FP64 method (the API
     // rejects it for the FP32 band, and the mesher evaluates it itself), so
     // the exact-power instances carry no Lagrange code
-    if (kPow == kPolar && (a.flags & QJ_MAP_LAGRANGE)) {
+    if (kPow == kPolar && kLagrange && (a.flags & QJ_MAP_LAGRANGE)) {
       q = qscale(lagrange64<kPow>(roots, a.nt, a.nb, q), a.exp_scaling);
       magnitude = qmag(q);
       it++;

@@ -147,6 +147,9 @@ __device__ __forceinline__ Res64 eval64x(const MapArgs& a, const RootRec64* __re
 // ---------------------------------------------------------------- kernels
 
 
+// kLagrange: the map may be in Lagrange order (QJ_MAP_LAGRANGE); the
+// factored-order instance carries no Lagrange code and needs fewer registers.
+template <bool kLagrange>
 __global__ __launch_bounds__(kBlock) void k_grid64(MapArgs a, const RootRec64* __restrict__ roots,
                                                    double* __restrict__ out, unsigned long long* __restrict__ cnt,
                                                    uint32_t* __restrict__ alist, uint32_t* __restrict__ acount) {
@@ -158,7 +161,7 @@ __global__ __launch_bounds__(kBlock) void k_grid64(MapArgs a, const RootRec64* _
   if (valid) {
     double c[3];
     cell_center(a, x, y, z + a.zoff, c);
-    const Res64 r = eval64(a, roots, q64{c[0], c[1], c[2], a.slice});
+    const Res64 r = eval64<kPolar, const RootRec64*, kLagrange>(a, roots, q64{c[0], c[1], c[2], a.slice});
     if (r.written) {
       out[idx] = r.F;
       nonfinite = !isfinite(r.F);
@@ -509,8 +512,12 @@ qj_status launch_field(qj_workspace* ws, const MapArgs& a, const qj_rational* f,
   uint32_t* alist = ws->d_list + ws->list_cap;
   timing_mark(ws, s);
   if (prec == QJ_FP64) {
-    hipLaunchKernelGGL(k_grid64, dim3((unsigned)nblocks), dim3(kBlock), 0, s, a, f->d64, d_out, ws->d_counters,
-                       alist, ws->d_list_count + 1);
+    if (a.flags & QJ_MAP_LAGRANGE)
+      hipLaunchKernelGGL(k_grid64<true>, dim3((unsigned)nblocks), dim3(kBlock), 0, s, a, f->d64, d_out,
+                         ws->d_counters, alist, ws->d_list_count + 1);
+    else
+      hipLaunchKernelGGL(k_grid64<false>, dim3((unsigned)nblocks), dim3(kBlock), 0, s, a, f->d64, d_out,
+                         ws->d_counters, alist, ws->d_list_count + 1);
     QJ_HIP(hipGetLastError());
     timing_mark(ws, s);
     timing_mark(ws, s);
