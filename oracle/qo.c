@@ -593,7 +593,9 @@ static double nonlinear_value(const double* pos, const qo_poly* top, const qo_po
       it++;
       if (magnitude < 10.0 * FLT_MIN) it = p->max_iter;
     }
-    return log(magnitude) - iso;
+    /* an infinite final value is returned as FLT_MAX (TRIANGLE_MESH.cpp:1733-1742) */
+    const double v = log(magnitude) - iso;
+    return isinf(v) ? (double)FLT_MAX : v;
   }
   double F = 0;
   int of = 0;
@@ -633,8 +635,13 @@ static int midpoint_search(double* pp, double vp, double* pn, double vn, const q
  * once (the slab streaming only bounds memory). Cases from the raw field
  * log|q|; vertices by bisection on log|q| - iso. Returns the number of edges
  * whose end values do not straddle zero (the reference exits on the first). */
-int64_t qo_nonlinear_mc(const qo_grid* g, const qo_poly* top, const qo_poly* bottom, const qo_params* p, double iso,
-                        int64_t* n_vertices, int64_t* n_triangles, double* vertices, int32_t* triangles) {
+int64_t qo_nonlinear_mc(const qo_grid* g, const qo_poly* top, const qo_poly* bottom, const qo_params* p_in,
+                        double iso, int64_t* n_vertices, int64_t* n_triangles, double* vertices, int32_t* triangles) {
+  /* the mesher's slabs and values always stop at the black hole
+   * (TRIANGLE_MESH.cpp:374-376, 1789-1791) */
+  qo_params pp = *p_in;
+  pp.flags |= QO_BLACK_HOLE;
+  const qo_params* p = &pp;
   const int64_t xRes = g->res[0], yRes = g->res[1], zRes = g->res[2];
   const int64_t slab = xRes * yRes, total = slab * zRes;
   double* field = (double*)malloc(sizeof(double) * (size_t)total);
@@ -731,6 +738,51 @@ int64_t qo_nonlinear_mc(const qo_grid* g, const qo_poly* top, const qo_poly* bot
   free(mark);
   free(vid);
   return bad;
+}
+
+/* ---------------- FIELD_3D clean-up passes ---------------- */
+
+/* FIELD_3D::clampInfsToMean (FIELD_3D.cpp:5182-5207) and clampNansToMean
+ * (:5212-5234): serial sum in index order of the cells that are not infinite
+ * (not NaN), mean = sum / count, then the infinite (NaN) cells take the mean.
+ * The reference counts in an int; the count here is 64-bit (the same for
+ * fields under 2^31 cells). Returns the mean. */
+static double clamp_to_mean(double* data, int64_t n, int nans) {
+  double sum = 0;
+  int64_t seen = 0;
+  for (int64_t i = 0; i < n; i++)
+    if (nans ? !isnan(data[i]) : !isinf(data[i])) {
+      sum += data[i];
+      seen++;
+    }
+  const double mean = sum / (double)seen;
+  for (int64_t i = 0; i < n; i++)
+    if (nans ? isnan(data[i]) : isinf(data[i])) data[i] = mean;
+  return mean;
+}
+double qo_clamp_infs_to_mean(double* data, int64_t n) { return clamp_to_mean(data, n, 0); }
+double qo_clamp_nans_to_mean(double* data, int64_t n) { return clamp_to_mean(data, n, 1); }
+
+/* FIELD_3D::setBorders (FIELD_3D.cpp:5357-5382) */
+void qo_set_borders(const qo_grid* g, double* data, double value) {
+  const int64_t X = g->res[0], Y = g->res[1], Z = g->res[2];
+#define AT(x, y, z) data[(x) + (y) * X + (z) * X * Y]
+  for (int64_t x = 0; x < X; x++)
+    for (int64_t y = 0; y < Y; y++) {
+      AT(x, y, 0) = value;
+      AT(x, y, Z - 1) = value;
+    }
+  for (int64_t x = 0; x < X; x++)
+    for (int64_t z = 0; z < Z; z++) {
+      AT(x, 0, z) = value;
+      AT(x, Y - 1, z) = value;
+    }
+  for (int64_t y = 0; y < Y; y++)
+    for (int64_t z = 0; z < Z; z++) {
+      AT(0, y, z) = value;
+      AT(X - 1, y, z) = value;
+    }
+#undef AT
 }
 
 /* ---------------- AP score and gradients at one iteration ---------------- */

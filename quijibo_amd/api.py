@@ -29,6 +29,27 @@ def _ptr(t):
     return ctypes.c_void_p(t.data_ptr())
 
 
+def _device_tensor(t, name, numel=None, dtype=torch.float64):
+    """Checks a tensor handed to the C ABI as a device pointer: a contiguous CUDA
+    tensor of the kernel's element type and, when numel is given, that size.
+    The C ABI receives plain pointers, so these checks happen here."""
+    if not isinstance(t, torch.Tensor):
+        raise ValueError("%s must be a torch tensor" % name)
+    if not t.is_cuda:
+        raise ValueError("%s must be a CUDA tensor (got device %s)" % (name, t.device))
+    if t.dtype != dtype:
+        raise ValueError("%s must be %s (got %s)" % (name, dtype, t.dtype))
+    if not t.is_contiguous():
+        raise ValueError("%s must be contiguous" % name)
+    if numel is not None and t.numel() != numel:
+        raise ValueError("%s must have %d elements (got %d)" % (name, numel, t.numel()))
+    return t
+
+
+def _cells(grid):
+    return int(grid.res[0]) * int(grid.res[1]) * int(grid.res[2])
+
+
 class POLYNOMIAL_4D:
     """Factored polynomial: prod_i (q - roots[i])^(power_scalar * powers[i])."""
 
@@ -168,14 +189,35 @@ class FIELD_3D:
     def numpy(self):
         return self.data.detach().cpu().numpy().reshape(-1)
 
+    def clampInfsToMean(self, workspace=None, stream=None):
+        """FIELD_3D::clampInfsToMean (FIELD_3D.cpp:5182-5207) on the device; returns the mean used."""
+        return _clamp(self, "qj_field_clamp_infs_to_mean", workspace, stream)
+
+    def clampNansToMean(self, workspace=None, stream=None):
+        """FIELD_3D::clampNansToMean (FIELD_3D.cpp:5212-5234) on the device; returns the mean used."""
+        return _clamp(self, "qj_field_clamp_nans_to_mean", workspace, stream)
+
+    def setBorders(self, value, stream=None):
+        """FIELD_3D::setBorders (FIELD_3D.cpp:5357-5382): the six faces take value."""
+        g = self.grid()
+        check(_lib.load().qj_field_set_borders(ctypes.byref(g), float(value), _ptr(self.data), _stream_ptr(stream)))
+
+
+def _clamp(field, fn, workspace, stream):
+    _device_tensor(field.data, "field", field.totalCells())
+    ws = workspace or default_workspace()
+    mean = ctypes.c_double()
+    check(getattr(_lib.load(), fn)(ws.handle, _ptr(field.data), field.totalCells(), ctypes.byref(mean),
+                                   _stream_ptr(stream)))
+    return mean.value
+
 
 def field_eval(grid, rational, params, precision=FP64, band=DEFAULT_BAND, out=None, workspace=None, stream=None):
     """qj_field_eval into a float64 CUDA tensor (allocated when out is None)."""
-    n = grid.res[0] * grid.res[1] * grid.res[2]
+    n = _cells(grid)
     if out is None:
         out = torch.empty(n, dtype=torch.float64, device="cuda")
-    if out.numel() != n or out.dtype != torch.float64 or not out.is_contiguous():
-        raise ValueError("out must be a contiguous float64 tensor with %d elements" % n)
+    _device_tensor(out, "out", n)
     ws = workspace or default_workspace()
     check(_lib.load().qj_field_eval(ws.handle, ctypes.byref(grid), rational.handle, ctypes.byref(params),
                                     int(precision), float(band), _ptr(out), _stream_ptr(stream)))
@@ -183,12 +225,13 @@ def field_eval(grid, rational, params, precision=FP64, band=DEFAULT_BAND, out=No
 
 
 def points_eval(points, rational, params, precision=FP64, band=DEFAULT_BAND, out=None, workspace=None, stream=None):
-    pts = points.contiguous()
-    if pts.dtype != torch.float64 or pts.dim() != 2 or pts.shape[1] != 3:
+    if not isinstance(points, torch.Tensor) or points.dim() != 2 or points.shape[1] != 3:
         raise ValueError("points must be an (n, 3) float64 CUDA tensor")
+    pts = _device_tensor(points.contiguous(), "points")
     n = pts.shape[0]
     if out is None:
         out = torch.empty(n, dtype=torch.float64, device=pts.device)
+    _device_tensor(out, "out", n)
     ws = workspace or default_workspace()
     check(_lib.load().qj_points_eval(ws.handle, _ptr(pts), n, rational.handle, ctypes.byref(params),
                                      int(precision), float(band), _ptr(out), _stream_ptr(stream)))
@@ -197,6 +240,7 @@ def points_eval(points, rational, params, precision=FP64, band=DEFAULT_BAND, out
 
 def marching_cubes(field, grid, iso=0.0, workspace=None, stream=None, device_output=False):
     """qj_mc_run + copy: returns (vertices (nv,3) float64, triangles (nt,3) int32)."""
+    _device_tensor(field, "field", _cells(grid))
     ws = workspace or default_workspace()
     nv = ctypes.c_int64()
     nt = ctypes.c_int64()
@@ -287,10 +331,11 @@ def ap_score_gradients(grid, rational, params, wgrid, weight, tanh_alpha=10.0, t
     that receives the AP field (cells at the origin are left as they are).
     Returns a dict with score, grad_exp, grad_top, grad_bottom, grad_top_scalar,
     grad_bottom_scalar, adaptive_cells and the field tensor."""
-    n = grid.res[0] * grid.res[1] * grid.res[2]
+    n = _cells(grid)
     if field is None:
         field = torch.zeros(n, dtype=torch.float64, device="cuda")
-    w = weight.contiguous()
+    _device_tensor(field, "field", n)
+    w = _device_tensor(weight.contiguous() if isinstance(weight, torch.Tensor) else weight, "weight", _cells(wgrid))
     nt, nb = rational.top.totalRoots(), rational.bottom.totalRoots()
     gt = np.zeros(max(nt, 1))
     gb = np.zeros(max(nb, 1))
@@ -307,8 +352,12 @@ def ap_score_gradients(grid, rational, params, wgrid, weight, tanh_alpha=10.0, t
 
 def points_adjoint(points, rational, params, dJdF, precision=FP64, workspace=None, stream=None):
     """qj_points_adjoint: dJ/d(roots) ((nt+nb) x 4, top first) and dJ/d(powers) (nt+nb)."""
-    pts = points.contiguous()
-    g = dJdF.contiguous().to(torch.float64)
+    if not isinstance(points, torch.Tensor) or points.dim() != 2 or points.shape[1] != 3:
+        raise ValueError("points must be an (n, 3) float64 CUDA tensor")
+    pts = _device_tensor(points.contiguous(), "points")
+    if not isinstance(dJdF, torch.Tensor):
+        raise ValueError("dJdF must be a torch tensor")
+    g = _device_tensor(dJdF.contiguous().to(torch.float64), "dJdF", pts.shape[0])
     nr = rational.top.totalRoots() + rational.bottom.totalRoots()
     gr = torch.empty((nr, 4), dtype=torch.float64, device=pts.device)
     gp = torch.empty(nr, dtype=torch.float64, device=pts.device)

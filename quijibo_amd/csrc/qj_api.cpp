@@ -470,6 +470,9 @@ qj_status qj_nlmc_run(qj_workspace* ws, const qj_grid* grid, const qj_rational* 
   MapArgs a;
   qj_status st = make_map_args(grid, f, params, &a);
   if (st) return st;
+  // the mesher's slabs and vertex values always stop at the black hole
+  // (TRIANGLE_MESH.cpp:374-376, 1789-1791)
+  a.flags |= QJ_MAP_BLACK_HOLE;
   std::lock_guard<std::mutex> lock(ws->mu);
   int64_t bad = 0;
   return nlmc_run(ws, a, f, iso, precision, band, chunk_slices, n_vertices, n_triangles, &bad,
@@ -511,8 +514,43 @@ qj_status qj_nlmc_resume(qj_workspace* ws, const qj_grid* grid, const qj_rationa
   MapArgs a;
   qj_status st = make_map_args(grid, f, params, &a);
   if (st) return st;
+  // the mesher's value always stops at the black hole (TRIANGLE_MESH.cpp:1789-1791)
+  a.flags |= QJ_MAP_BLACK_HOLE;
   std::lock_guard<std::mutex> lock(ws->mu);
   return nlmc_resume(ws, a, f, iso, n_edges, edges, n_cells, cells, n_vertices, n_triangles, (hipStream_t)stream);
+}
+
+static qj_status clamp_entry(qj_workspace* ws, double* d_field, int64_t n, bool nans, double* mean_out,
+                             void* stream, const char* name) {
+  if (!ws || (n > 0 && !d_field) || n < 0) {
+    set_error("%s: null workspace or field, or negative size", name);
+    return QJ_ERR_INVALID;
+  }
+  std::lock_guard<std::mutex> lock(ws->mu);
+  return field_clamp_to_mean(ws, d_field, n, nans, mean_out, (hipStream_t)stream);
+}
+
+qj_status qj_field_clamp_infs_to_mean(qj_workspace* ws, double* d_field, int64_t n, double* mean_out,
+                                      void* stream) {
+  return clamp_entry(ws, d_field, n, false, mean_out, stream, "qj_field_clamp_infs_to_mean");
+}
+
+qj_status qj_field_clamp_nans_to_mean(qj_workspace* ws, double* d_field, int64_t n, double* mean_out,
+                                      void* stream) {
+  return clamp_entry(ws, d_field, n, true, mean_out, stream, "qj_field_clamp_nans_to_mean");
+}
+
+qj_status qj_field_set_borders(const qj_grid* grid, double value, double* d_field, void* stream) {
+  if (!grid || !d_field) {
+    set_error("qj_field_set_borders: null argument");
+    return QJ_ERR_INVALID;
+  }
+  for (int i = 0; i < 3; i++)
+    if (grid->res[i] < 1) {
+      set_error("qj_field_set_borders: grid resolution must be >= 1 (got %d)", grid->res[i]);
+      return QJ_ERR_INVALID;
+    }
+  return field_set_borders(grid->res, value, d_field, (hipStream_t)stream);
 }
 
 qj_status qj_ap_score_gradients(qj_workspace* ws, const qj_grid* grid, const qj_rational* f,

@@ -21,6 +21,7 @@
 //                   centres, no contraction
 //   k_mc_triangles / scan / k_mc_squeeze: triangles in cell then table order,
 //                   degenerate ones (edge < 1e-7) dropped
+#include <float.h>
 #include <math.h>
 
 #include <vector>
@@ -609,7 +610,9 @@ __device__ __forceinline__ double nonlinear_value(const MapArgs& a, const RootRe
       it++;
       if (magnitude < kBlackHole) it = a.max_iter;
     }
-    return log(magnitude) - iso;
+    // an infinite value (|q| = 0) is reported as FLT_MAX (TRIANGLE_MESH.cpp:1733-1742)
+    const double v = log(magnitude) - iso;
+    return isinf(v) ? (double)FLT_MAX : v;
   }
   const Res64 r = eval64<kPow>(a, roots, q64{p[0], p[1], p[2], a.slice});
   return r.F - iso;
@@ -1136,12 +1139,17 @@ __global__ __launch_bounds__(kMcBlock) void k_cache_triangles(
       if (id < 0) ok = false;
       v[q] = id < 0 ? 0 : (int32_t)id;
     }
-    if (!ok) atomicAdd(missing, 1u);
-    const double eps = 1e-7;
-    const double* p0 = verts + 3 * (int64_t)v[0];
-    const double* p1 = verts + 3 * (int64_t)v[1];
-    const double* p2 = verts + 3 * (int64_t)v[2];
-    const bool degenerate = edge_len(p0, p1) < eps || edge_len(p0, p2) < eps || edge_len(p1, p2) < eps;
+    bool degenerate = false;
+    if (!ok) {
+      atomicAdd(missing, 1u);
+    } else {
+      // vertex coordinates are read only for triangles whose three edges are listed
+      const double eps = 1e-7;
+      const double* p0 = verts + 3 * (int64_t)v[0];
+      const double* p1 = verts + 3 * (int64_t)v[1];
+      const double* p2 = verts + 3 * (int64_t)v[2];
+      degenerate = edge_len(p0, p1) < eps || edge_len(p0, p2) < eps || edge_len(p1, p2) < eps;
+    }
     tris_pre[3 * (int64_t)o + 0] = v[0];
     tris_pre[3 * (int64_t)o + 1] = v[1];
     tris_pre[3 * (int64_t)o + 2] = v[2];
@@ -1162,6 +1170,11 @@ qj_status nlmc_resume(qj_workspace* ws, const MapArgs& a, const qj_rational* f, 
   if (ne < 0 || nc < 0 || (ne && !h_edges) || (nc && !h_cells) || ne > (int64_t)INT32_MAX ||
       nc > (int64_t)UINT32_MAX / 5) {
     set_error("qj_nlmc_resume: bad edge or cell arrays");
+    return QJ_ERR_INVALID;
+  }
+  // every flag in 1..254 has at least one triangle, so active cells need edges
+  if (nc > 0 && ne == 0) {
+    set_error("qj_nlmc_resume: %lld active cells but no edges", (long long)nc);
     return QJ_ERR_INVALID;
   }
   // host checks: every edge inside the grid with its far end, keys strictly

@@ -107,6 +107,58 @@ def test_ap_top_gradient_is_the_score_derivative(gpu):
     assert abs(fd - base["grad_exp"]) <= 1e-5 * max(1.0, abs(fd))
 
 
+def _gpu_ap(case, tp=None, bp=None, st=None, sb=None):
+    """The GPU AP pass on a test_oracle_grad case (same grid for the weights)."""
+    g = api.make_grid(case["res"], case["center"], case["lengths"])
+    r = api.Rational(api.POLYNOMIAL_4D(case["top"], case["tp"] if tp is None else tp, case["st"] if st is None else st),
+                     api.POLYNOMIAL_4D(case["bot"], case["bp"] if bp is None else bp, case["sb"] if sb is None else sb))
+    p = api.make_params(1, 20.0, case["c"], 0.0, MAP_SKIP_ORIGIN | MAP_ADAPTIVE)
+    return api.ap_score_gradients(g, r, p, g, torch.from_numpy(case["weight"]).cuda(), 10.0, 0.0)
+
+
+def test_ap_gradients_match_closed_form_on_commuting_plane(gpu):
+    """All six outputs against the complex closed form (no oracle involved)."""
+    from test_oracle_grad import plane_case, plane_closed_form
+    case = plane_case()
+    cf = plane_closed_form(case)
+    r = _gpu_ap(case)
+    assert r["adaptive_cells"] == 0
+    for k in ("score", "grad_exp", "grad_top", "grad_bottom", "grad_top_scalar", "grad_bottom_scalar"):
+        e = _rel(r[k], cf[k])
+        print("%s rel err vs closed form %.2e" % (k, e), flush=True)
+        assert e < 1e-10, k
+
+
+def test_ap_scalar_and_bottom_gradients_are_score_derivatives(gpu):
+    """grad_bottom, grad_top_scalar and grad_bottom_scalar against central
+    differences of the GPU score on a general 3D grid (at one iteration every
+    reference gradient formula is the exact derivative)."""
+    rng = np.random.default_rng(12)
+    case = dict(top=rng.normal(scale=0.6, size=(7, 4)), bot=rng.normal(scale=0.6, size=(6, 4)),
+                tp=rng.uniform(0.8, 2.0, 7), bp=rng.uniform(0.8, 2.0, 6), st=0.95, sb=1.2, c=1.3,
+                res=(20, 18, 16), center=(0.0, 0.05, 0.1), lengths=(2.6, 2.6, 2.6))
+    case["weight"] = rng.uniform(0.5, 2.0, 20 * 18 * 16)
+    base = _gpu_ap(case)
+    assert base["adaptive_cells"] == 0
+    h = 1e-6
+    fdb = []
+    for j in range(6):
+        s = []
+        for sg in (1, -1):
+            bp = case["bp"].copy()
+            bp[j] += sg * h
+            s.append(_gpu_ap(case, bp=bp)["score"])
+        fdb.append((s[0] - s[1]) / (2 * h))
+    e = _rel(base["grad_bottom"], fdb)
+    print("grad_bottom vs FD rel %.2e" % e, flush=True)
+    assert e < 1e-6
+    for key, kw in (("grad_top_scalar", "st"), ("grad_bottom_scalar", "sb")):
+        s = [_gpu_ap(case, **{kw: case[kw] + sg * h})["score"] for sg in (1, -1)]
+        fd = (s[0] - s[1]) / (2 * h)
+        print("%s %.10e FD %.10e" % (key, base[key], fd), flush=True)
+        assert abs(base[key] - fd) < 1e-6 * abs(fd), key
+
+
 def _c5(n=20000, max_iter=3):
     w = W.config5(n_surface=n, n_exterior=n, max_iter=max_iter)
     pts = torch.from_numpy(np.concatenate([w.extra["surface"], w.extra["exterior"]])).cuda()

@@ -92,6 +92,11 @@ def test_edge_cache_resume_rebuilds_the_same_mesh(gpu):
     with pytest.raises(RuntimeError):
         api.nonlinear_mesh_resume(g, r, p, np.delete(edges, len(edges) // 2, axis=0), cells,
                                   workspace=api.Workspace())
+    # active cells with no edges at all: rejected on the host before any launch
+    from quijibo_amd._lib import QJError
+    with pytest.raises(QJError) as ei:
+        api.nonlinear_mesh_resume(g, r, p, np.zeros((0, 4), np.int32), cells[:3], workspace=api.Workspace())
+    assert ei.value.status == 1 and "no edges" in str(ei.value)
 
 
 @pytest.mark.parametrize("cfg", ["c1", "c2"])
@@ -118,3 +123,48 @@ def test_lagrange_mesher_vs_oracle(gpu, cfg):
         assert np.max(np.abs(std[fin] - ref[fin])) > 1e-3  # a different map from the factored order
     with pytest.raises(RuntimeError):
         api.field_eval(g, r, p, FP32_BAND)
+
+
+def test_iterate_on_zero_and_forced_black_hole(gpu):
+    """An iterate that underflows to exactly 0 at a bisection midpoint: the
+    Lagrange mesher's value is FLT_MAX, the factored mesher's -inf, as in the
+    oracle; and the mesher applies the black-hole stop whatever the flags say."""
+    from test_oracle_mc import edge_vertex, zero_iterate_case
+    og, p0, p1, m, top, bot, c = zero_iterate_case()
+    g = api.make_grid((6, 6, 6), (0.0, 0.0, 0.0), (0.6, 0.6, 0.6))
+    r = api.Rational(api.POLYNOMIAL_4D(top, [3.0]), api.POLYNOMIAL_4D(bot, [1.0]))
+    for flags, oflags in ((MAP_BLACK_HOLE | MAP_LAGRANGE, O.BLACK_HOLE | O.LAGRANGE), (MAP_BLACK_HOLE, O.BLACK_HOLE),
+                          (MAP_LAGRANGE, O.BLACK_HOLE | O.LAGRANGE), (0, O.BLACK_HOLE)):
+        v, t = api.nonlinear_mesh(g, r, api.make_params(1, 2000.0, c, 0.0, flags), 0.0, FP64, 0.05, 2,
+                                  workspace=api.Workspace())
+        ov, ot, bad = O.nonlinear_mc(og, O.poly(top, [3.0]), O.poly(bot, [1.0]),
+                                     O.make_params(1, 2000.0, c, 0.0, oflags))
+        assert bad == 0 and np.array_equal(t, ot), flags
+        assert np.max(np.abs(v - ov)) <= 1e-12, flags
+        x = edge_vertex(v, p0, p1)[0]
+        if flags & MAP_LAGRANGE:
+            assert p0[0] < x <= m[0]
+        else:
+            assert m[0] <= x < p1[0]
+
+
+@pytest.mark.parametrize("cfg", ["c1", "c2"])
+def test_lagrange_points_eval_matches_oracle(gpu, cfg):
+    """qj_points_eval in FP64 with the Lagrange order (the polar-form point kernel)."""
+    w = W.config1(8) if cfg == "c1" else W.config2(8)
+    rng = np.random.default_rng(17)
+    pts = rng.uniform(-1.4, 1.4, size=(5000, 3))
+    r = api.Rational(api.POLYNOMIAL_4D(w.top.roots, w.top.powers * 1.1),
+                     api.POLYNOMIAL_4D(w.bottom.roots, w.bottom.powers * 1.1))
+    p = api.make_params(w.max_iter, 2000.0, w.exp_scaling, w.slice, MAP_BLACK_HOLE | MAP_LAGRANGE)
+    F = api.points_eval(torch.from_numpy(pts).cuda(), r, p, FP64).cpu().numpy()
+    ref = O.points_eval(pts, O.poly(w.top.roots, w.top.powers * 1.1), O.poly(w.bottom.roots, w.bottom.powers * 1.1),
+                        O.make_params(w.max_iter, 2000.0, w.exp_scaling, w.slice, O.BLACK_HOLE | O.LAGRANGE))
+    fin = np.isfinite(ref)
+    assert np.array_equal(np.isfinite(F), fin)
+    assert np.max(np.abs(F[fin] - ref[fin]) / np.maximum(1.0, np.abs(ref[fin]))) <= 1e-9
+    if cfg == "c2":
+        std = O.points_eval(pts, O.poly(w.top.roots, w.top.powers * 1.1),
+                            O.poly(w.bottom.roots, w.bottom.powers * 1.1),
+                            O.make_params(w.max_iter, 2000.0, w.exp_scaling, w.slice, O.BLACK_HOLE))
+        assert np.max(np.abs(std[fin] - ref[fin])) > 1e-3

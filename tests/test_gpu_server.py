@@ -117,3 +117,26 @@ def test_special_cases_match_the_direct_evaluation(gpu):
     assert np.array_equal(np.isnan(got), np.isnan(ref))
     fin = np.isfinite(ref)
     assert np.array_equal(got[fin], ref[fin]) and np.array_equal(np.isinf(got), np.isinf(ref))
+
+
+def test_lagrange_request_matches_the_direct_evaluation(gpu):
+    """An FP64 request in the Lagrange order (QJ_MAP_LAGRANGE) through the
+    request loop's FP64 kernel, against qj_field_eval with the same flags."""
+    from quijibo_amd._lib import MAP_LAGRANGE
+    w = W.config2(40)
+    w.top.powers = w.top.powers * 1.1
+    srv = S.Server(hbm_budget_bytes=64 << 20, chunk_voxels=4096, chunks_per_launch=8, precision=FP64)
+    rid = srv.submit(w, keep_field=True, flags=MAP_BLACK_HOLE | MAP_LAGRANGE)
+    c = srv.wait_all(1, timeout=60)
+    assert len(c) == 1 and c[0].status == 0
+    got = srv.fetch(c[0], device=False)
+    srv.release(rid)
+    srv.close()
+    g = api.make_grid(w.res, w.center, w.lengths)
+    r = api.Rational(api.POLYNOMIAL_4D(w.top.roots, w.top.powers), api.POLYNOMIAL_4D(w.bottom.roots, w.bottom.powers))
+    ref = api.field_eval(g, r, api.make_params(w.max_iter, w.escape, w.exp_scaling, w.slice,
+                                               MAP_BLACK_HOLE | MAP_LAGRANGE), FP64).cpu().numpy()
+    assert np.array_equal(got, ref, equal_nan=True)
+    std = _direct(w, FP64)
+    fin = np.isfinite(ref) & np.isfinite(std)
+    assert np.max(np.abs(std[fin] - ref[fin])) > 1e-3  # the Lagrange order is a different map

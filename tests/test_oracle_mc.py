@@ -204,3 +204,36 @@ def test_nonlinear_mesher_sphere_closed_form():
     lv, lt = O.marching_cubes(g, f)
     assert np.array_equal(lt, t) and len(lv) == len(v)
     assert np.max(np.abs(np.linalg.norm(lv - r[0, :3], axis=1) - 0.9)) > dx / 2 ** 7  # linear is worse
+
+
+def zero_iterate_case():
+    """A top root (power 3) 1e-150 from the midpoint of one x-edge, in the slice
+    component: at that midpoint the first iterate underflows to exactly 0. The
+    Lagrange mesher's value is then FLT_MAX (TRIANGLE_MESH.cpp:1733-1742) and
+    bisection keeps the negative half; the factored mesher's value is -inf
+    (:1793-1794) and it keeps the positive half."""
+    g = O.make_grid((6, 6, 6), (0.0, 0.0, 0.0), (0.6, 0.6, 0.6))
+    p0 = O.cell_center(g, 2, 2, 2)
+    p1 = O.cell_center(g, 3, 2, 2)
+    m = (p0 + p1) * 0.5
+    top = np.array([[m[0], m[1], m[2], -1e-150]])
+    bot = np.array([[p1[0] + 0.01, p1[1], p1[2], 0.0]])
+    return g, p0, p1, m, top, bot, float(np.exp(5.5))
+
+
+def edge_vertex(v, p0, p1):
+    sel = v[(v[:, 1] == p0[1]) & (v[:, 2] == p0[2]) & (v[:, 0] > p0[0]) & (v[:, 0] < p1[0])]
+    assert len(sel) == 1
+    return sel[0]
+
+
+def test_nonlinear_mesher_iterate_on_zero():
+    g, p0, p1, m, top, bot, c = zero_iterate_case()
+    F = O.field_map(g, O.poly(top, [3.0]), O.poly(bot, [1.0]), O.make_params(1, 2000.0, c, 0.0)).reshape(6, 6, 6)
+    assert F[2, 2, 2] < 0 < F[2, 2, 3]  # the edge straddles the surface
+    lag = O.nonlinear_mc(g, O.poly(top, [3.0]), O.poly(bot, [1.0]),
+                         O.make_params(1, 2000.0, c, 0.0, O.BLACK_HOLE | O.LAGRANGE))
+    fac = O.nonlinear_mc(g, O.poly(top, [3.0]), O.poly(bot, [1.0]), O.make_params(1, 2000.0, c, 0.0, O.BLACK_HOLE))
+    assert lag[2] == 0 and fac[2] == 0
+    assert p0[0] < edge_vertex(lag[0], p0, p1)[0] <= m[0]
+    assert m[0] <= edge_vertex(fac[0], p0, p1)[0] < p1[0]
