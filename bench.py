@@ -24,6 +24,12 @@ PEAK_FP64_TFLOPS = 78.6    # MI355X published FP64 vector peak
 # FETCH_SIZE and WRITE_SIZE (each scaled by the same counter on a 1 GiB copy):
 # 0.160 MB fetched + 1113.24 MB written.
 K_GRID32_TRAFFIC_BYTES = 1113401623
+# Issued-instruction side of k_grid32 at 512^3 from rocprofv3 counters
+# (SQ_INSTS_VALU, SQ_WAVES, GRBM_GUI_ACTIVE; profiles/k_grid32_counters_c2_512.md):
+# VALU instructions per SIMD-cycle against the 0.48-0.5 a stream of independent
+# FP32 FMAs issues on gfx950.
+K_GRID32_VALU_BUSY = {"valu_insts_per_simd_cycle": 0.455, "fma_stream_insts_per_simd_cycle": 0.478,
+                      "busy_frac": 0.95, "source": "profiles/k_grid32_counters_c2_512.md (three boxes: 0.45-0.46)"}
 
 
 def log(*a):
@@ -149,6 +155,27 @@ def main():
     k64 = _kernel_ms(api, ws)
     log("fp64: %.2f ms/step" % (t64 * 1e3))
 
+    # FP64 with exponents 1 and 2 as exact products: the same precision as the
+    # plain FP64 kernel, the algorithm of the FP32 path, so the two ratios
+    # below separate the algorithm's share of the speedup from precision's
+    from quijibo_amd._lib import FP64_EXACT
+    out64x = torch.empty(w.voxels, dtype=torch.float64, device="cuda")
+    log("fp64 exact: warmup")
+    api.field_eval(grid, r, params, FP64_EXACT, out=out64x, workspace=ws)
+    torch.cuda.synchronize()
+    ws.reset_stats()
+    api._lib.load().qj_workspace_timing(ws.handle, 1)
+    t0 = time.perf_counter()
+    for _ in range(args.fp64_steps):
+        api.field_eval(grid, r, params, FP64_EXACT, out=out64x, workspace=ws)
+    torch.cuda.synchronize()
+    t64x = (time.perf_counter() - t0) / args.fp64_steps
+    k64x = _kernel_ms(api, ws)
+    api._lib.load().qj_workspace_timing(ws.handle, 0)
+    log("fp64 exact: %.2f ms/step" % (t64x * 1e3))
+    f64x = out64x.cpu().numpy()
+    del out64x
+
     # FP32 bulk + FP64 band
     for _ in range(args.warmup):
         api.field_eval(grid, r, params, FP32_BAND, args.band, out=out32, workspace=ws)
@@ -180,9 +207,13 @@ def main():
     flops = iters_per_launch * basic_ops_per_voxel_iter(nt, nb)
     achieved = flops / (bulk_ms * 1e-3) / 1e12
     traffic = K_GRID32_TRAFFIC_BYTES if args.res == 512 else None
-    roofline = {"bound": "mfma", "compute_unit": "FP32 vector ALU (no matrix op applies; published FP32 peak)",
+    roofline = {"bound": "valu", "compute_unit": "FP32 vector ALU (no matrix op applies; published FP32 peak)",
                 "kernel": "k_grid32", "achieved": round(achieved, 3), "peak": PEAK_FP32_TFLOPS,
                 "unit": "TFLOP/s", "frac": round(achieved / PEAK_FP32_TFLOPS, 4), "traffic": traffic,
+                "frac_note": "delivered reference-method work: executed voxel-iterations x the reference's %d basic "
+                             "ops each (the polar pow's 23 per term), over the published peak; k_grid32 takes "
+                             "exponents 1 and 2 as exact products and issues fewer" % basic_ops_per_voxel_iter(nt, nb),
+                "valu_busy": K_GRID32_VALU_BUSY,
                 "traffic_note": "HBM bytes per launch from rocprofv3 FETCH_SIZE/WRITE_SIZE passes scaled by a 1 GiB "
                                 "copy (profiles/k_grid32_counters_c2_512.md); algorithmic bytes %d" % (w.voxels * 8),
                 "algorithmic_flops_per_launch": flops, "kernel_ms": round(bulk_ms, 4),
@@ -226,6 +257,12 @@ def main():
                        "achieved_tflops": round(fp64_flops / (fp64_ms * 1e-3) / 1e12, 3),
                        "peak": PEAK_FP64_TFLOPS},
         "speedup_vs_fp64": round(value / value64, 3),
+        "fp64_exact": {"value": round(nominal / t64x / 1e9, 4), "ms_per_step": round(t64x * 1e3, 3),
+                       "kernel_ms": round(k64x["bulk_ms"] / max(1, k64x["launches"]), 3),
+                       "what": "all voxels in FP64, exponents 1 and 2 as exact products (QJ_FP64_EXACT)",
+                       "sign_flips_vs_fp64_gpu": int(np.count_nonzero((f64.astype(np.float32) > 0) !=
+                                                                      (f64x.astype(np.float32) > 0)))},
+        "speedup_vs_fp64_exact": round(value / (nominal / t64x / 1e9), 3),
         "band": {"voxels": st32["band_voxels"] // args.steps, "fraction": st32["band_voxels"] / args.steps / w.voxels,
                  "band_kernel_ms": round(band_ms, 4)},
         "parity": {"sign_flips_vs_fp64_gpu": flips_vs_fp64, "max_abs_dF_vs_fp64_gpu": maxdf},

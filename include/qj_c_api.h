@@ -18,6 +18,7 @@
  *   OPTIMIZE_3D::computeScoreAP, computeExpGradientAP, compute{Top,Bottom}GradientAP,
  *     compute{Top,Bottom}BulkScalarGradientAP       src/solvers/OPTIMIZE_3D.cpp:3563, 3589, 8905, 10012, 9250, 10360
  *   OPTIMIZE_3D::computeLogRationalCenteredGradient (root-position gradient) src/solvers/OPTIMIZE_3D.cpp:2872-2951
+ *   FIELD_3D::clampInfsToMean / clampNansToMean / setBorders  src/geometry/FIELD_3D.cpp:5182-5234, 5357-5382
  */
 #ifndef QJ_C_API_H
 #define QJ_C_API_H
@@ -86,10 +87,15 @@ typedef struct qj_map_params {
 #define QJ_PRESET_AP_ESCAPE 20.0             /* computeRationalMapAP, flags = SKIP_ORIGIN|ADAPTIVE */
 
 typedef enum qj_precision {
-  QJ_FP64 = 0,      /* every voxel in FP64, reference expression order */
-  QJ_FP32_BAND = 1  /* FP32 bulk; FP64 re-evaluation of the band |F| < band, of
-                       non-finite results and of voxels whose escape or black-hole
-                       test was within FP32's error of its threshold */
+  QJ_FP64 = 0,       /* every voxel in FP64, reference expression order */
+  QJ_FP32_BAND = 1,  /* FP32 bulk; FP64 re-evaluation of the band |F| < band, of
+                        non-finite results and of voxels whose escape or black-hole
+                        test was within FP32's error of its threshold */
+  QJ_FP64_EXACT = 2  /* every voxel in FP64, exponents exactly 1 and 2 as the exact
+                        products b and b*b (a few ulps from the polar form, with its
+                        special cases), other exponents polar; the FP32 path's band
+                        arithmetic. qj_field_eval, qj_field_eval_noalloc and
+                        qj_points_eval; not with QJ_MAP_LAGRANGE */
 } qj_precision;
 
 /* Counters accumulated by the evaluation calls on a workspace since the last
@@ -101,6 +107,12 @@ typedef struct qj_eval_stats {
   int64_t band_iterations;   /* executed voxel-iterations of the FP64 re-evaluation */
   int64_t adaptive_voxels;   /* cells flagged as overflowed (QJ_MAP_ADAPTIVE) */
   int64_t nonfinite;         /* non-finite results written */
+  /* the sampled FP64 cross-check (qj_workspace_check), summed over fields */
+  int64_t checked;           /* cells re-evaluated with the reference method in FP64 */
+  int64_t check_sign_flips;  /* of those, cells where (float)F > 0 differs from FP64's */
+  int64_t check_class_mismatch; /* cells whose class (finite, NaN, +inf, -inf) differs */
+  double check_max_abs_dF;   /* largest |F - F_fp64| over checked cells finite in both */
+  double check_max_rel_dF;   /* largest |F - F_fp64| / max(1, |F_fp64|) */
 } qj_eval_stats;
 
 typedef struct qj_rational qj_rational;   /* top/bottom root sets resident in HBM */
@@ -126,6 +138,17 @@ qj_status qj_workspace_destroy(qj_workspace* ws);
 /* Synchronizes `stream`, then copies the counters. */
 qj_status qj_workspace_stats(qj_workspace* ws, void* stream, qj_eval_stats* out);
 qj_status qj_workspace_reset_stats(qj_workspace* ws, void* stream);
+
+/* The error statement of every field (no counterpart in the reference, which
+ * has one precision): with samples > 0, each qj_field_eval on ws re-evaluates
+ * `samples` evenly spaced cells (cell (2i+1) N / (2 samples), i < samples)
+ * with the reference method in FP64 (polar powers, reference expression
+ * order; the extended-range re-pass for cells QJ_MAP_ADAPTIVE flags) after
+ * the field is written, and adds the comparison to the workspace statistics
+ * (qj_eval_stats.checked ...). One extra kernel per field, about the latency
+ * of one FP64 evaluation per cell (0.5 ms for 32+32 roots, maxIter 4).
+ * 0 (the default) turns it off; at most 2^24. */
+qj_status qj_workspace_check(qj_workspace* ws, int64_t samples);
 
 /* Kernel timing on a workspace (profiling hook, like the reference's TIMER,
  * TIMER.h:37-79): when enabled, qj_field_eval / qj_points_eval record HIP
@@ -210,6 +233,28 @@ qj_status qj_nlmc_cache(qj_workspace* ws, int64_t* n_edges, int32_t* edges, int6
 qj_status qj_nlmc_resume(qj_workspace* ws, const qj_grid* grid, const qj_rational* f, const qj_map_params* params,
                          double iso, int64_t n_edges, const int32_t* edges, int64_t n_cells, const int32_t* cells,
                          int64_t* n_vertices, int64_t* n_triangles, void* stream);
+
+/* ---- FIELD_3D clean-up before meshing (the reference runs
+ *      clampInfsToMean then setBorders(totalCells()) on a fractal before
+ *      TRIANGLE_MESH(field), toothOptimizer.cpp:250-254) ---- */
+
+/* FIELD_3D::clampInfsToMean (FIELD_3D.cpp:5182-5207): every infinite cell of
+ * d_field (device, n doubles) takes the mean of the non-infinite cells. NaN
+ * cells are summed like the reference's serial sum does, so one NaN makes the
+ * mean NaN; with no summed cell the mean is 0/0 = NaN. The finite cells are
+ * summed in double-double arithmetic in a fixed order, so the mean is the
+ * correctly rounded sum over the count (the reference's serial sum differs
+ * from it by at most (n-1) 2^-53 sum|x|). Enqueued on `stream`; when mean_out
+ * (host) is not NULL the call synchronizes and stores the mean there. */
+qj_status qj_field_clamp_infs_to_mean(qj_workspace* ws, double* d_field, int64_t n, double* mean_out,
+                                      void* stream);
+/* FIELD_3D::clampNansToMean (FIELD_3D.cpp:5212-5234): the same for NaN cells,
+ * with the mean of the non-NaN cells (an infinite cell makes it infinite). */
+qj_status qj_field_clamp_nans_to_mean(qj_workspace* ws, double* d_field, int64_t n, double* mean_out,
+                                      void* stream);
+/* FIELD_3D::setBorders (FIELD_3D.cpp:5357-5382): the cells on the six faces of
+ * the grid take `value`. Enqueued on `stream`. */
+qj_status qj_field_set_borders(const qj_grid* grid, double value, double* d_field, void* stream);
 
 /* ---- gradients ---- */
 

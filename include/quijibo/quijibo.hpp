@@ -138,6 +138,13 @@ class FIELD_3D {
 
   qj_grid grid() const;
 
+  // Clean-up before meshing, run on the GPU (FIELD_3D.cpp:5182-5234,
+  // 5357-5382): infinite (NaN) cells take the mean of the other cells;
+  // setBorders sets the six faces to a constant.
+  void clampInfsToMean();
+  void clampNansToMean();
+  void setBorders(const Real& value);
+
  private:
   int _xRes = 0, _yRes = 0, _zRes = 0;
   VEC3F _center, _lengths{1, 1, 1};

@@ -81,6 +81,11 @@ def lib():
             ctypes.POINTER(Grid), ctypes.POINTER(Poly), ctypes.POINTER(Poly), ctypes.POINTER(Params),
             ctypes.POINTER(Grid), dp, ctypes.c_double, ctypes.c_double, dp, dp, dp,
             ctypes.POINTER(APResult), ctypes.c_int]
+        L.qo_clamp_infs_to_mean.argtypes = [dp, ctypes.c_int64]
+        L.qo_clamp_infs_to_mean.restype = ctypes.c_double
+        L.qo_clamp_nans_to_mean.argtypes = [dp, ctypes.c_int64]
+        L.qo_clamp_nans_to_mean.restype = ctypes.c_double
+        L.qo_set_borders.argtypes = [ctypes.POINTER(Grid), dp, ctypes.c_double]
         _lib = L
     return _lib
 
@@ -236,3 +241,24 @@ def qmag(a):
 def poly_eval(p, q):
     q = np.ascontiguousarray(q, dtype=np.float64)
     out = np.empty(4); lib().qo_poly_eval(ctypes.byref(p.p), _dp(q), _dp(out)); return out
+
+
+def clamp_infs_to_mean(field):
+    """FIELD_3D::clampInfsToMean on a copy; returns (field, mean)."""
+    f = np.array(field, dtype=np.float64).reshape(-1).copy()
+    m = lib().qo_clamp_infs_to_mean(_dp(f), f.size)
+    return f, m
+
+
+def clamp_nans_to_mean(field):
+    """FIELD_3D::clampNansToMean on a copy; returns (field, mean)."""
+    f = np.array(field, dtype=np.float64).reshape(-1).copy()
+    m = lib().qo_clamp_nans_to_mean(_dp(f), f.size)
+    return f, m
+
+
+def set_borders(grid, field, value):
+    """FIELD_3D::setBorders on a copy."""
+    f = np.array(field, dtype=np.float64).reshape(-1).copy()
+    lib().qo_set_borders(ctypes.byref(grid), _dp(f), float(value))
+    return f

@@ -85,6 +85,13 @@ int64_t qo_nonlinear_mc(const qo_grid* g, const qo_poly* top, const qo_poly* bot
 /* FIELD_3D::operator()(VEC3F) trilinear lookup (FIELD_3D.cpp:966-1029). */
 double qo_field_lookup(const qo_grid* g, const double* data, const double* pos3);
 
+/* FIELD_3D::clampInfsToMean / clampNansToMean (FIELD_3D.cpp:5182-5234):
+ * in place, serial sum; return the mean used. */
+double qo_clamp_infs_to_mean(double* data, int64_t n);
+double qo_clamp_nans_to_mean(double* data, int64_t n);
+/* FIELD_3D::setBorders (FIELD_3D.cpp:5357-5382) */
+void qo_set_borders(const qo_grid* g, double* data, double value);
+
 /* Score and analytic gradients at one iteration (AP semantics).
  * weight grid/data: the weight field sampled at each fractal cell centre. */
 typedef struct qo_ap_result {

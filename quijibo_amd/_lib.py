@@ -12,6 +12,7 @@ LIB_PATH = os.path.join(_HERE, "libquijibo.so")
 QJ_OK = 0
 FP64 = 0
 FP32_BAND = 1
+FP64_EXACT = 2
 MAP_BLACK_HOLE = 1
 MAP_SKIP_ORIGIN = 2
 MAP_ADAPTIVE = 4
@@ -44,7 +45,9 @@ class MapParams(ctypes.Structure):
 class EvalStats(ctypes.Structure):
     _fields_ = [("voxels", ctypes.c_int64), ("iterations", ctypes.c_int64), ("band_voxels", ctypes.c_int64),
                 ("band_iterations", ctypes.c_int64), ("adaptive_voxels", ctypes.c_int64),
-                ("nonfinite", ctypes.c_int64)]
+                ("nonfinite", ctypes.c_int64), ("checked", ctypes.c_int64), ("check_sign_flips", ctypes.c_int64),
+                ("check_class_mismatch", ctypes.c_int64), ("check_max_abs_dF", ctypes.c_double),
+                ("check_max_rel_dF", ctypes.c_double)]
 
 
 class APGrads(ctypes.Structure):
@@ -86,6 +89,7 @@ SIGNATURES = [
     ("qj_workspace_destroy", ctypes.c_int32, [_P]),
     ("qj_workspace_stats", ctypes.c_int32, [_P, _P, ctypes.POINTER(EvalStats)]),
     ("qj_workspace_reset_stats", ctypes.c_int32, [_P, _P]),
+    ("qj_workspace_check", ctypes.c_int32, [_P, ctypes.c_int64]),
     ("qj_workspace_timing", ctypes.c_int32, [_P, ctypes.c_int32]),
     ("qj_workspace_kernel_ms", ctypes.c_int32, [_P, c_double_p, c_double_p, ctypes.POINTER(ctypes.c_int64)]),
     ("qj_field_eval", ctypes.c_int32, [_P, ctypes.POINTER(Grid), _P, ctypes.POINTER(MapParams), ctypes.c_int32,
@@ -104,6 +108,9 @@ SIGNATURES = [
                                         ctypes.POINTER(ctypes.c_int64), _P]),
     ("qj_mc_copy", ctypes.c_int32, [_P, _P, _P, _P]),
     ("qj_mc_device_ptrs", ctypes.c_int32, [_P, _PP, _PP]),
+    ("qj_field_clamp_infs_to_mean", ctypes.c_int32, [_P, _P, ctypes.c_int64, c_double_p, _P]),
+    ("qj_field_clamp_nans_to_mean", ctypes.c_int32, [_P, _P, ctypes.c_int64, c_double_p, _P]),
+    ("qj_field_set_borders", ctypes.c_int32, [ctypes.POINTER(Grid), ctypes.c_double, _P, _P]),
     ("qj_ap_score_gradients", ctypes.c_int32, [_P, ctypes.POINTER(Grid), _P, ctypes.POINTER(MapParams),
                                                ctypes.POINTER(Grid), _P, ctypes.c_double, ctypes.c_double, _P, _P,
                                                _P, ctypes.POINTER(APGrads), _P]),

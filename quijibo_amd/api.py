@@ -117,6 +117,11 @@ class Workspace:
     def reset_stats(self, stream=None):
         check(_lib.load().qj_workspace_reset_stats(self.handle, _stream_ptr(stream)))
 
+    def check(self, samples):
+        """Sampled FP64 cross-check of every field evaluated on this workspace
+        (qj_workspace_check); the results appear in stats()."""
+        check(_lib.load().qj_workspace_check(self.handle, int(samples)))
+
     def __del__(self):
         try:
             if getattr(self, "handle", None):

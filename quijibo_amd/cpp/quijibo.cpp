@@ -373,6 +373,40 @@ void fetch_mesh(qj_workspace* ws, int64_t nv, int64_t nt, std::vector<VEC3F>& v,
 }
 }  // namespace
 
+namespace {
+
+// one FIELD_3D pass on the device: the cells go up, `op` runs, they come back
+template <class Op>
+void device_pass(FIELD_3D& field, Op op) {
+  const size_t n = field.totalCells();
+  if (!n) return;
+  DeviceBuffer d(sizeof(double) * n);
+  hip_check(hipMemcpy(d.p, field.data(), sizeof(double) * n, hipMemcpyHostToDevice), "hipMemcpy field");
+  op((double*)d.p);
+  hip_check(hipMemcpy(field.data(), d.p, sizeof(double) * n, hipMemcpyDeviceToHost), "hipMemcpy field");
+}
+
+}  // namespace
+
+void FIELD_3D::clampInfsToMean() {
+  device_pass(*this, [&](double* d) {
+    check(qj_field_clamp_infs_to_mean(shared_workspace(), d, (int64_t)totalCells(), nullptr, nullptr),
+          "qj_field_clamp_infs_to_mean");
+  });
+}
+
+void FIELD_3D::clampNansToMean() {
+  device_pass(*this, [&](double* d) {
+    check(qj_field_clamp_nans_to_mean(shared_workspace(), d, (int64_t)totalCells(), nullptr, nullptr),
+          "qj_field_clamp_nans_to_mean");
+  });
+}
+
+void FIELD_3D::setBorders(const Real& value) {
+  const qj_grid g = grid();
+  device_pass(*this, [&](double* d) { check(qj_field_set_borders(&g, value, d, nullptr), "qj_field_set_borders"); });
+}
+
 TRIANGLE_MESH::TRIANGLE_MESH(const FIELD_3D& field, Real isosurface) {
   qj_workspace* ws = shared_workspace();
   DeviceBuffer d(sizeof(double) * field.totalCells());

@@ -364,6 +364,26 @@ qj_status qj_workspace_stats(qj_workspace* ws, void* stream, qj_eval_stats* out)
   out->band_iterations = t[CNT_BAND_ITERS];
   out->adaptive_voxels = t[CNT_ADAPTIVE];
   out->nonfinite = t[CNT_NONFINITE];
+  out->checked = t[CNT_CHECKED];
+  out->check_sign_flips = t[CNT_CHECK_FLIPS];
+  out->check_class_mismatch = t[CNT_CHECK_CLASS];
+  unsigned long long mabs = 0, mrel = 0;
+  for (int i = 0; i < kCounterSpread; i++) {
+    mabs = h[CNT_CHECK_MAXABS * kCounterSpread + i] > mabs ? h[CNT_CHECK_MAXABS * kCounterSpread + i] : mabs;
+    mrel = h[CNT_CHECK_MAXREL * kCounterSpread + i] > mrel ? h[CNT_CHECK_MAXREL * kCounterSpread + i] : mrel;
+  }
+  memcpy(&out->check_max_abs_dF, &mabs, sizeof(double));
+  memcpy(&out->check_max_rel_dF, &mrel, sizeof(double));
+  return QJ_OK;
+}
+
+qj_status qj_workspace_check(qj_workspace* ws, int64_t samples) {
+  if (!ws || samples < 0 || samples > (int64_t(1) << 24)) {
+    set_error("qj_workspace_check: null workspace or samples outside [0, 2^24]");
+    return QJ_ERR_INVALID;
+  }
+  std::lock_guard<std::mutex> lock(ws->mu);
+  ws->check_samples = samples;
   return QJ_OK;
 }
 
@@ -408,7 +428,7 @@ qj_status qj_field_eval(qj_workspace* ws, const qj_grid* grid, const qj_rational
     set_error("qj_field_eval: null workspace or output");
     return QJ_ERR_INVALID;
   }
-  if (precision != QJ_FP64 && precision != QJ_FP32_BAND) {
+  if (precision != QJ_FP64 && precision != QJ_FP32_BAND && precision != QJ_FP64_EXACT) {
     set_error("qj_field_eval: unknown precision %d", (int)precision);
     return QJ_ERR_INVALID;
   }
@@ -430,7 +450,7 @@ qj_status qj_points_eval(qj_workspace* ws, const double* d_points, int64_t n, co
     set_error("qj_points_eval: bad arguments");
     return QJ_ERR_INVALID;
   }
-  if (precision != QJ_FP64 && precision != QJ_FP32_BAND) return QJ_ERR_INVALID;
+  if (precision != QJ_FP64 && precision != QJ_FP32_BAND && precision != QJ_FP64_EXACT) return QJ_ERR_INVALID;
   if (params && (params->flags & QJ_MAP_LAGRANGE) && precision != QJ_FP64) {
     set_error("qj_points_eval: QJ_MAP_LAGRANGE needs QJ_FP64");
     return QJ_ERR_INVALID;

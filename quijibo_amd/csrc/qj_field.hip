@@ -149,7 +149,9 @@ __device__ __forceinline__ Res64 eval64x(const MapArgs& a, const RootRec64* __re
 
 // kLagrange: the map may be in Lagrange order (QJ_MAP_LAGRANGE); the
 // factored-order instance carries no Lagrange code and needs fewer registers.
-template <bool kLagrange>
+// kPow: kPolar is the reference method (QJ_FP64); kExactOnly / kExactInt take
+// exponents 1 and 2 as exact products (QJ_FP64_EXACT).
+template <bool kLagrange, int kPow = kPolar>
 __global__ __launch_bounds__(kBlock) void k_grid64(MapArgs a, const RootRec64* __restrict__ roots,
                                                    double* __restrict__ out, unsigned long long* __restrict__ cnt,
                                                    uint32_t* __restrict__ alist, uint32_t* __restrict__ acount) {
@@ -161,7 +163,7 @@ __global__ __launch_bounds__(kBlock) void k_grid64(MapArgs a, const RootRec64* _
   if (valid) {
     double c[3];
     cell_center(a, x, y, z + a.zoff, c);
-    const Res64 r = eval64<kPolar, const RootRec64*, kLagrange>(a, roots, q64{c[0], c[1], c[2], a.slice});
+    const Res64 r = eval64<kPow, const RootRec64*, kLagrange>(a, roots, q64{c[0], c[1], c[2], a.slice});
     if (r.written) {
       out[idx] = r.F;
       nonfinite = !isfinite(r.F);
@@ -245,6 +247,66 @@ __global__ __launch_bounds__(kBlock) void k_list64(MapArgs a, const RootRec64* _
   }
   block_count(vox, cnt, slot_vox);
   block_count(iters, cnt, slot_iter);
+}
+
+// ---------------------------------------------------------------- sampled FP64 cross-check
+
+// Block maximum of a non-negative double into counter slot `slot` (the bits
+// of non-negative doubles order like unsigned integers).
+__device__ __forceinline__ void block_max(double v, unsigned long long* cnt, int slot) {
+  __shared__ unsigned long long s_part[kBlock / 64];
+  unsigned long long b = __builtin_bit_cast(unsigned long long, v > 0.0 ? v : 0.0);
+  for (int o = 32; o > 0; o >>= 1) {
+    const unsigned long long t = __shfl_xor(b, o);
+    b = t > b ? t : b;
+  }
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  if (lane == 0) s_part[wave] = b;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    unsigned long long m = 0;
+    for (int i = 0; i < kBlock / 64; i++) m = s_part[i] > m ? s_part[i] : m;
+    if (m) atomicMax(&cnt[slot * kCounterSpread + (blockIdx.x % kCounterSpread)], m);
+  }
+  __syncthreads();
+}
+
+// The error statement of a finished field: at `samples` evenly spaced cells
+// the reference method is evaluated again in FP64 (polar powers, reference
+// expression order; the extended-range re-pass where QJ_MAP_ADAPTIVE flags an
+// overflow) and compared with the value written: counts of checked cells,
+// sign disagreements of (float)F > 0 (the mesher's predicate) and class
+// disagreements (finite / NaN / +-inf), and the largest absolute and relative
+// (to max(1, |F64|)) differences over cells finite in both.
+__global__ __launch_bounds__(kBlock) void k_check64(MapArgs a, const RootRec64* __restrict__ roots,
+                                                    const double* __restrict__ field, int64_t nvox, int64_t samples,
+                                                    unsigned long long* __restrict__ cnt) {
+  const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  unsigned long long checked = 0, flips = 0, cls = 0;
+  double dabs = 0.0, drel = 0.0;
+  if (i < samples) {
+    const int64_t idx = ((2 * i + 1) * nvox) / (2 * samples);
+    const q64 q0 = GridSource::start(a, nullptr, idx);
+    Res64 r = eval64<kPolar, const RootRec64*, true>(a, roots, q0);
+    if ((a.flags & QJ_MAP_ADAPTIVE) && r.overflow) r = eval64x(a, roots, q0);
+    if (r.written) {
+      const double F = field[idx];
+      checked = 1;
+      const bool fin = isfinite(F), fin64 = isfinite(r.F);
+      if (fin != fin64 || isnan(F) != isnan(r.F) || (!fin && !isnan(F) && F != r.F)) {
+        cls = 1;
+      } else if (fin) {
+        dabs = fabs(F - r.F);
+        drel = dabs / fmax(1.0, fabs(r.F));
+      }
+      flips = (((float)F > 0.0f) != ((float)r.F > 0.0f)) ? 1 : 0;
+    }
+  }
+  block_count(checked, cnt, CNT_CHECKED);
+  block_count(flips, cnt, CNT_CHECK_FLIPS);
+  block_count(cls, cnt, CNT_CHECK_CLASS);
+  block_max(dabs, cnt, CNT_CHECK_MAXABS);
+  block_max(drel, cnt, CNT_CHECK_MAXREL);
 }
 
 // ---------------------------------------------------------------- batched chunks (request loop)
@@ -476,20 +538,105 @@ static int list_grid(int64_t n_hint) {
   return (int)(b < 1 ? 1 : (b > 8192 ? 8192 : b));
 }
 
-static qj_status run_adaptive(qj_workspace* ws, const MapArgs& a, const qj_rational* f, const double* pts,
+// Device buffers of one field evaluation: the band list and the adaptive
+// list (list_cap entries each), their two counts, and the statistics
+// counters. ws (may be null) only carries the optional kernel timing.
+struct FieldBufs {
+  uint32_t* list;
+  uint32_t* alist;
+  int64_t list_cap;
+  uint32_t* list_count;
+  unsigned long long* counters;
+  qj_workspace* ws;
+};
+
+static FieldBufs ws_bufs(qj_workspace* ws) {
+  return FieldBufs{ws->d_list, ws->d_list + ws->list_cap, ws->list_cap, ws->d_list_count, ws->d_counters, ws};
+}
+
+static void mark(const FieldBufs& b, hipStream_t s) {
+  if (b.ws) timing_mark(b.ws, s);
+}
+
+static qj_status run_adaptive(const FieldBufs& fb, const MapArgs& a, const qj_rational* f, const double* pts,
                               int64_t n, double* d_out, hipStream_t s) {
   if (!(a.flags & QJ_MAP_ADAPTIVE)) return QJ_OK;
-  uint32_t* alist = ws->d_list + ws->list_cap;
   if (pts)
     hipLaunchKernelGGL((k_list64<PointSource, true>), dim3(list_grid(n / 64 + 1)), dim3(kBlock), 0, s, a, f->d64,
-                       pts, alist, ws->d_list_count + 1, (int64_t)0, d_out, ws->d_counters, (int)CNT_BAND_VOXELS,
+                       pts, fb.alist, fb.list_count + 1, (int64_t)0, d_out, fb.counters, (int)CNT_BAND_VOXELS,
                        (int)CNT_BAND_ITERS, (uint32_t*)nullptr, (uint32_t*)nullptr);
   else
     hipLaunchKernelGGL((k_list64<GridSource, true>), dim3(list_grid(n / 64 + 1)), dim3(kBlock), 0, s, a, f->d64,
-                       pts, alist, ws->d_list_count + 1, (int64_t)0, d_out, ws->d_counters, (int)CNT_BAND_VOXELS,
+                       pts, fb.alist, fb.list_count + 1, (int64_t)0, d_out, fb.counters, (int)CNT_BAND_VOXELS,
                        (int)CNT_BAND_ITERS, (uint32_t*)nullptr, (uint32_t*)nullptr);
   QJ_HIP(hipGetLastError());
   return QJ_OK;
+}
+
+// One field (or one z-slab of it: a.res[2] slices from global slice a.zoff,
+// written from d_out on) with lists of at least a.res[0] a.res[1] a.res[2]
+// entries.
+static qj_status launch_field_bufs(const FieldBufs& fb, const MapArgs& a, const qj_rational* f, qj_precision prec,
+                                   double band, double* d_out, hipStream_t s) {
+  const int64_t nvox = (int64_t)a.res[0] * a.res[1] * a.res[2];
+  // list counters cleared by a kernel, not a memset node, so the call can be captured in a graph
+  hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, s, fb.list_count, 2);
+  QJ_HIP(hipGetLastError());
+  const int64_t ntx = (a.res[0] + kTileX - 1) / kTileX, nty = (a.res[1] + kTileY - 1) / kTileY,
+                ntz = (a.res[2] + kTileZ - 1) / kTileZ;
+  const int64_t nblocks = ntx * nty * ntz;
+  if (nblocks > INT32_MAX) return QJ_ERR_CAPACITY;
+  mark(fb, s);
+  if (prec == QJ_FP64_EXACT) {
+    if (a.gen)
+      hipLaunchKernelGGL((k_grid64<false, kExactInt>), dim3((unsigned)nblocks), dim3(kBlock), 0, s, a, f->d64, d_out,
+                         fb.counters, fb.alist, fb.list_count + 1);
+    else
+      hipLaunchKernelGGL((k_grid64<false, kExactOnly>), dim3((unsigned)nblocks), dim3(kBlock), 0, s, a, f->d64,
+                         d_out, fb.counters, fb.alist, fb.list_count + 1);
+    QJ_HIP(hipGetLastError());
+    mark(fb, s);
+    mark(fb, s);
+  } else if (prec == QJ_FP64) {
+    if (a.flags & QJ_MAP_LAGRANGE)
+      hipLaunchKernelGGL(k_grid64<true>, dim3((unsigned)nblocks), dim3(kBlock), 0, s, a, f->d64, d_out,
+                         fb.counters, fb.alist, fb.list_count + 1);
+    else
+      hipLaunchKernelGGL(k_grid64<false>, dim3((unsigned)nblocks), dim3(kBlock), 0, s, a, f->d64, d_out,
+                         fb.counters, fb.alist, fb.list_count + 1);
+    QJ_HIP(hipGetLastError());
+    mark(fb, s);
+    mark(fb, s);
+  } else {
+    const Consts32 k = consts32(a, band);
+    const dim3 g((unsigned)nblocks), b(kBlock);
+    if (a.gen)
+      hipLaunchKernelGGL((k_grid32<2, true>), g, b, 0, s, a, k, (const F4*)f->d32, d_out, fb.counters, fb.list,
+                         fb.list_count);
+    else if (rn_of(a) == 8)
+      hipLaunchKernelGGL((k_grid32<8, false>), g, b, 0, s, a, k, (const F4*)f->d32, d_out, fb.counters, fb.list,
+                         fb.list_count);
+    else if (rn_of(a) == 4)
+      hipLaunchKernelGGL((k_grid32<4, false>), g, b, 0, s, a, k, (const F4*)f->d32, d_out, fb.counters, fb.list,
+                         fb.list_count);
+    else
+      hipLaunchKernelGGL((k_grid32<2, false>), g, b, 0, s, a, k, (const F4*)f->d32, d_out, fb.counters, fb.list,
+                         fb.list_count);
+    QJ_HIP(hipGetLastError());
+    mark(fb, s);
+    mark(fb, s);
+    if (a.gen)
+      hipLaunchKernelGGL((k_list64<GridSource, false, kExactInt>), dim3(list_grid(nvox / 16)), dim3(kBlock), 0, s, a,
+                         f->d64, (const double*)nullptr, fb.list, fb.list_count, (int64_t)0, d_out, fb.counters,
+                         (int)CNT_BAND_VOXELS, (int)CNT_BAND_ITERS, fb.alist, fb.list_count + 1);
+    else
+      hipLaunchKernelGGL((k_list64<GridSource, false, kExactOnly>), dim3(list_grid(nvox / 16)), dim3(kBlock), 0, s,
+                         a, f->d64, (const double*)nullptr, fb.list, fb.list_count, (int64_t)0, d_out, fb.counters,
+                         (int)CNT_BAND_VOXELS, (int)CNT_BAND_ITERS, fb.alist, fb.list_count + 1);
+    QJ_HIP(hipGetLastError());
+  }
+  mark(fb, s);
+  return run_adaptive(fb, a, f, nullptr, nvox, d_out, s);
 }
 
 qj_status launch_field(qj_workspace* ws, const MapArgs& a, const qj_rational* f, qj_precision prec, double band,
@@ -502,55 +649,14 @@ qj_status launch_field(qj_workspace* ws, const MapArgs& a, const qj_rational* f,
   }
   qj_status st = ensure_list(ws, nvox, s);
   if (st) return st;
-  // list counters cleared by a kernel, not a memset node, so the call can be captured in a graph
-  hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, s, ws->d_list_count, 2);
-  QJ_HIP(hipGetLastError());
-  const int64_t ntx = (a.res[0] + kTileX - 1) / kTileX, nty = (a.res[1] + kTileY - 1) / kTileY,
-                ntz = (a.res[2] + kTileZ - 1) / kTileZ;
-  const int64_t nblocks = ntx * nty * ntz;
-  if (nblocks > INT32_MAX) return QJ_ERR_CAPACITY;
-  uint32_t* alist = ws->d_list + ws->list_cap;
-  timing_mark(ws, s);
-  if (prec == QJ_FP64) {
-    if (a.flags & QJ_MAP_LAGRANGE)
-      hipLaunchKernelGGL(k_grid64<true>, dim3((unsigned)nblocks), dim3(kBlock), 0, s, a, f->d64, d_out,
-                         ws->d_counters, alist, ws->d_list_count + 1);
-    else
-      hipLaunchKernelGGL(k_grid64<false>, dim3((unsigned)nblocks), dim3(kBlock), 0, s, a, f->d64, d_out,
-                         ws->d_counters, alist, ws->d_list_count + 1);
-    QJ_HIP(hipGetLastError());
-    timing_mark(ws, s);
-    timing_mark(ws, s);
-  } else {
-    const Consts32 k = consts32(a, band);
-    const dim3 g((unsigned)nblocks), b(kBlock);
-    if (a.gen)
-      hipLaunchKernelGGL((k_grid32<2, true>), g, b, 0, s, a, k, (const F4*)f->d32, d_out, ws->d_counters, ws->d_list,
-                         ws->d_list_count);
-    else if (rn_of(a) == 8)
-      hipLaunchKernelGGL((k_grid32<8, false>), g, b, 0, s, a, k, (const F4*)f->d32, d_out, ws->d_counters, ws->d_list,
-                         ws->d_list_count);
-    else if (rn_of(a) == 4)
-      hipLaunchKernelGGL((k_grid32<4, false>), g, b, 0, s, a, k, (const F4*)f->d32, d_out, ws->d_counters, ws->d_list,
-                         ws->d_list_count);
-    else
-      hipLaunchKernelGGL((k_grid32<2, false>), g, b, 0, s, a, k, (const F4*)f->d32, d_out, ws->d_counters, ws->d_list,
-                         ws->d_list_count);
-    QJ_HIP(hipGetLastError());
-    timing_mark(ws, s);
-    timing_mark(ws, s);
-    if (a.gen)
-      hipLaunchKernelGGL((k_list64<GridSource, false, kExactInt>), dim3(list_grid(nvox / 16)), dim3(kBlock), 0, s, a,
-                         f->d64, (const double*)nullptr, ws->d_list, ws->d_list_count, (int64_t)0, d_out,
-                         ws->d_counters, (int)CNT_BAND_VOXELS, (int)CNT_BAND_ITERS, alist, ws->d_list_count + 1);
-    else
-      hipLaunchKernelGGL((k_list64<GridSource, false, kExactOnly>), dim3(list_grid(nvox / 16)), dim3(kBlock), 0, s, a,
-                         f->d64, (const double*)nullptr, ws->d_list, ws->d_list_count, (int64_t)0, d_out,
-                         ws->d_counters, (int)CNT_BAND_VOXELS, (int)CNT_BAND_ITERS, alist, ws->d_list_count + 1);
+  if ((st = launch_field_bufs(ws_bufs(ws), a, f, prec, band, d_out, s))) return st;
+  if (ws->check_samples > 0) {
+    const int64_t m = ws->check_samples < nvox ? ws->check_samples : nvox;
+    hipLaunchKernelGGL(k_check64, dim3((unsigned)((m + kBlock - 1) / kBlock)), dim3(kBlock), 0, s, a, f->d64,
+                       (const double*)d_out, nvox, m, ws->d_counters);
     QJ_HIP(hipGetLastError());
   }
-  timing_mark(ws, s);
-  return run_adaptive(ws, a, f, nullptr, nvox, d_out, s);
+  return QJ_OK;
 }
 
 qj_status launch_points(qj_workspace* ws, const MapArgs& a, const qj_rational* f, qj_precision prec, double band,
@@ -559,14 +665,24 @@ qj_status launch_points(qj_workspace* ws, const MapArgs& a, const qj_rational* f
   if (n > (int64_t)UINT32_MAX) return QJ_ERR_CAPACITY;
   qj_status st = ensure_list(ws, n, s);
   if (st) return st;
+  const FieldBufs fb = ws_bufs(ws);
   // list counters cleared by a kernel, not a memset node, so the call can be captured in a graph
-  hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, s, ws->d_list_count, 2);
+  hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, s, fb.list_count, 2);
   QJ_HIP(hipGetLastError());
-  uint32_t* alist = ws->d_list + ws->list_cap;
   if (prec == QJ_FP64) {
     hipLaunchKernelGGL((k_list64<PointSource, false>), dim3(list_grid(n)), dim3(kBlock), 0, s, a, f->d64, d_pts,
-                       (const uint32_t*)nullptr, (const uint32_t*)nullptr, n, d_out, ws->d_counters,
-                       (int)CNT_VOXELS, (int)CNT_ITERS, alist, ws->d_list_count + 1);
+                       (const uint32_t*)nullptr, (const uint32_t*)nullptr, n, d_out, fb.counters,
+                       (int)CNT_VOXELS, (int)CNT_ITERS, fb.alist, fb.list_count + 1);
+    QJ_HIP(hipGetLastError());
+  } else if (prec == QJ_FP64_EXACT) {
+    if (a.gen)
+      hipLaunchKernelGGL((k_list64<PointSource, false, kExactInt>), dim3(list_grid(n)), dim3(kBlock), 0, s, a,
+                         f->d64, d_pts, (const uint32_t*)nullptr, (const uint32_t*)nullptr, n, d_out, fb.counters,
+                         (int)CNT_VOXELS, (int)CNT_ITERS, fb.alist, fb.list_count + 1);
+    else
+      hipLaunchKernelGGL((k_list64<PointSource, false, kExactOnly>), dim3(list_grid(n)), dim3(kBlock), 0, s, a,
+                         f->d64, d_pts, (const uint32_t*)nullptr, (const uint32_t*)nullptr, n, d_out, fb.counters,
+                         (int)CNT_VOXELS, (int)CNT_ITERS, fb.alist, fb.list_count + 1);
     QJ_HIP(hipGetLastError());
   } else {
     const int64_t nb = (n + kBlock - 1) / kBlock;
@@ -574,28 +690,29 @@ qj_status launch_points(qj_workspace* ws, const MapArgs& a, const qj_rational* f
     const dim3 g((unsigned)nb), b(kBlock);
     if (a.gen)
       hipLaunchKernelGGL((k_points32<PointSource, 2, true>), g, b, 0, s, a, k, (const F4*)f->d32, d_pts, n, d_out,
-                         ws->d_counters, ws->d_list, ws->d_list_count);
+                         fb.counters, fb.list, fb.list_count);
     else if (rn_of(a) == 8)
       hipLaunchKernelGGL((k_points32<PointSource, 8, false>), g, b, 0, s, a, k, (const F4*)f->d32, d_pts, n, d_out,
-                         ws->d_counters, ws->d_list, ws->d_list_count);
+                         fb.counters, fb.list, fb.list_count);
     else if (rn_of(a) == 4)
       hipLaunchKernelGGL((k_points32<PointSource, 4, false>), g, b, 0, s, a, k, (const F4*)f->d32, d_pts, n, d_out,
-                         ws->d_counters, ws->d_list, ws->d_list_count);
+                         fb.counters, fb.list, fb.list_count);
     else
       hipLaunchKernelGGL((k_points32<PointSource, 2, false>), g, b, 0, s, a, k, (const F4*)f->d32, d_pts, n, d_out,
-                         ws->d_counters, ws->d_list, ws->d_list_count);
+                         fb.counters, fb.list, fb.list_count);
     QJ_HIP(hipGetLastError());
     if (a.gen)
       hipLaunchKernelGGL((k_list64<PointSource, false, kExactInt>), dim3(list_grid(n / 16)), dim3(kBlock), 0, s, a,
-                         f->d64, d_pts, ws->d_list, ws->d_list_count, (int64_t)0, d_out, ws->d_counters,
-                         (int)CNT_BAND_VOXELS, (int)CNT_BAND_ITERS, alist, ws->d_list_count + 1);
+                         f->d64, d_pts, fb.list, fb.list_count, (int64_t)0, d_out, fb.counters,
+                         (int)CNT_BAND_VOXELS, (int)CNT_BAND_ITERS, fb.alist, fb.list_count + 1);
     else
       hipLaunchKernelGGL((k_list64<PointSource, false, kExactOnly>), dim3(list_grid(n / 16)), dim3(kBlock), 0, s, a,
-                         f->d64, d_pts, ws->d_list, ws->d_list_count, (int64_t)0, d_out, ws->d_counters,
-                         (int)CNT_BAND_VOXELS, (int)CNT_BAND_ITERS, alist, ws->d_list_count + 1);
+                         f->d64, d_pts, fb.list, fb.list_count, (int64_t)0, d_out, fb.counters,
+                         (int)CNT_BAND_VOXELS, (int)CNT_BAND_ITERS, fb.alist, fb.list_count + 1);
     QJ_HIP(hipGetLastError());
   }
-  return run_adaptive(ws, a, f, d_pts, n, d_out, s);
+  return run_adaptive(fb, a, f, d_pts, n, d_out, s);
 }
 
 }  // namespace qj
+

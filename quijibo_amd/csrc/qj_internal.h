@@ -71,6 +71,8 @@ struct qj_workspace {
   // gradient partials
   double* d_partials = nullptr;
   int64_t partials_cap = 0;
+  // sampled FP64 cross-check of every field (qj_workspace_check); 0 = off
+  int64_t check_samples = 0;
   // kernel timing (qj_workspace_timing)
   bool timing = false;
   std::vector<hipEvent_t> ev;  // pairs: bulk start/stop, band start/stop per launch
@@ -154,6 +156,11 @@ qj_status ap_grads(qj_workspace* ws, const MapArgs& a, const qj_rational* f, con
 qj_status points_adjoint(qj_workspace* ws, const MapArgs& a, const qj_rational* f, qj_precision prec,
                          const double* d_pts, int64_t n, const double* d_dJdF, double* d_gr, double* d_gp,
                          hipStream_t s);
+
+// FIELD_3D clamp / border passes (qj_fieldops.hip)
+qj_status field_clamp_to_mean(qj_workspace* ws, double* d_field, int64_t n, bool nans, double* mean_out,
+                              hipStream_t s);
+qj_status field_set_borders(const int32_t* res, double value, double* d_field, hipStream_t s);
 
 // exclusive scan of uint32 (qj_scan.hip); n elements, temp provided by ws
 qj_status scan_u32(const uint32_t* d_in, uint32_t* d_out, int64_t n, uint32_t* d_total, void* d_tmp,

@@ -77,7 +77,12 @@ enum {
   CNT_BAND_ITERS = 3,
   CNT_ADAPTIVE = 4,
   CNT_NONFINITE = 5,
-  CNT_N = 8
+  CNT_CHECKED = 6,        // cells re-evaluated by the sampled FP64 cross-check
+  CNT_CHECK_FLIPS = 7,    // of those, sign disagreements of (float)F > 0
+  CNT_CHECK_CLASS = 8,    // finite / NaN / inf class disagreements
+  CNT_CHECK_MAXABS = 9,   // max |dF| (bits of a non-negative double, atomic max)
+  CNT_CHECK_MAXREL = 10,  // max |dF| / max(1, |F64|)
+  CNT_N = 12
 };
 constexpr int kCounterSpread = 64;
 

@@ -9,7 +9,9 @@
 //       OPTIMIZE_3D::read then OPTIMIZE_3D::write.
 //   host_layer_driver map <in.o3d> <maxIterations> <precision 0|1> <outdir>
 //       OPTIMIZE_3D::read, computeLogScaledPowerRationalMap into the fractal
-//       -> <outdir>/field.bin; TRIANGLE_MESH(fractal) -> linear.obj;
+//       -> <outdir>/field.bin; TRIANGLE_MESH(fractal) -> linear.obj; the
+//       fractal with clampInfsToMean and setBorders(totalCells) -> clamped.bin
+//       and its mesh -> clamped.obj;
 //       TRIANGLE_MESH(center, lengths, res, top, bottom, ..., cache) -> nonlinear.obj
 //       (writes <outdir>/mesh.cache), again from the cache -> resumed.obj, and
 //       the Lagrange mesher -> lagrange.obj.
@@ -123,6 +125,13 @@ static int run_map(const std::string& in, int maxIter, int precision, const std:
   fractal.write(out + "/field.bin");
   TRIANGLE_MESH linear(fractal);
   if (!linear.writeOBJ(out + "/linear.obj")) return fail("writeOBJ linear");
+  // the reference's clean-up before meshing a fractal (toothOptimizer.cpp:250-254)
+  FIELD_3D fieldCopy(fractal);
+  fieldCopy.clampInfsToMean();
+  fieldCopy.setBorders((Real)fieldCopy.totalCells());
+  fieldCopy.write(out + "/clamped.bin");
+  TRIANGLE_MESH clamped(fieldCopy);
+  if (!clamped.writeOBJ(out + "/clamped.obj")) return fail("writeOBJ clamped");
   const VEC3I res(opt.xRes(), opt.yRes(), opt.zRes());
   const std::string cache = out + "/mesh.cache";
   // first run writes the edge cache, the second rebuilds the mesh from it

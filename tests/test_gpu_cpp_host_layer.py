@@ -58,6 +58,16 @@ def test_map_and_meshes(gpu, tmp_path, cfg):
     lv, lt = FM.read_obj(tmp_path / "linear.obj")
     ov, ot = O.marching_cubes(g, field.data)
     assert np.array_equal(lt, ot) and np.max(np.abs(lv - ov)) <= 1e-15
+    # clampInfsToMean + setBorders(totalCells) through the C++ FIELD_3D, then its mesh
+    cf = FM.read_field(tmp_path / "clamped.bin")
+    rf, _ = O.clamp_infs_to_mean(field.data)
+    rf = O.set_borders(g, rf, float(np.prod(res)))
+    same = np.isfinite(field.data)
+    assert np.array_equal(cf.data[same], rf[same])
+    assert np.allclose(cf.data[~same], rf[~same], rtol=1e-12, atol=0, equal_nan=True)
+    cv, ct = FM.read_obj(tmp_path / "clamped.obj")
+    ov, ot = O.marching_cubes(g, cf.data)
+    assert np.array_equal(ct, ot) and np.max(np.abs(cv - ov), initial=0.0) <= 1e-15
     # non-linear mesher (slabs of 7 slices): same triangles as the oracle
     nv, nt = FM.read_obj(tmp_path / "nonlinear.obj")
     pv, pt, bad = O.nonlinear_mc(g, t, b, p)

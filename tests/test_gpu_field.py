@@ -324,3 +324,63 @@ def test_exact_form_special_cases_in_the_band(gpu):
     sp = sp.reshape(-1) & fin
     assert np.max(np.abs(g32[sp] - c[sp]) / np.maximum(1.0, np.abs(c[sp]))) <= 1e-8
     assert np.isnan(c).any()  # the b = 0 cells
+
+
+@pytest.mark.parametrize("general", [False, True])
+def test_fp64_exact_matches_the_oracle_and_the_band(gpu, general):
+    """QJ_FP64_EXACT (exponents 1 and 2 as exact products): the FP64 tolerance
+    against the oracle's polar form, the same non-finite cells, no sign flips;
+    and inside the band the FP32 + FP64 band field is this kernel's value bit
+    for bit (the band pass runs the same arithmetic)."""
+    from quijibo_amd._lib import FP64_EXACT
+    w = W.config2(64)
+    if general:
+        rng = np.random.default_rng(13)
+        w.top.powers = np.where(rng.uniform(size=w.top.n) < 0.3, 1.5, w.top.powers)
+    g, c, grid, r = run_both(w, precision=FP64_EXACT)
+    assert_fp64_close(g, c)
+    assert signflips(g, c) == 0
+    p = api.make_params(w.max_iter, w.escape, w.exp_scaling, w.slice, MAP_BLACK_HOLE)
+    f32 = api.field_eval(grid, r, p, FP32_BAND, 0.05).cpu().numpy()
+    inband = np.abs(f32) < 0.04
+    assert inband.sum() > 100 and np.array_equal(f32[inband], g[inband])
+    # the polar FP64 kernel differs from the exact one in the last bits only
+    polar = api.field_eval(grid, r, p, FP64).cpu().numpy()
+    fin = np.isfinite(polar)
+    assert np.array_equal(np.isfinite(g), fin)
+    assert np.max(np.abs(polar[fin] - g[fin]) / np.maximum(1.0, np.abs(polar[fin]))) <= RTOL64
+    # points: the same values at the cell centres
+    pts = torch.from_numpy(np.stack(np.meshgrid(*[-1.5 + (np.arange(64) + 0.5) * 3.0 / 64] * 3, indexing="ij"),
+                                    axis=-1)[:, :, :, ::-1].reshape(-1, 3).copy()).cuda()
+    pe = api.points_eval(pts, r, p, FP64_EXACT).cpu().numpy()
+    assert np.array_equal(pe, g, equal_nan=True)
+
+
+def test_sampled_fp64_cross_check_in_the_stats(gpu):
+    """qj_workspace_check: the FP32 + band field's error statement, against the
+    FP64 kernel at the same evenly spaced cells computed here from its field."""
+    w = W.config2(64)
+    g = api.make_grid(w.res, w.center, w.lengths)
+    r = api.Rational(api.POLYNOMIAL_4D(w.top.roots, w.top.powers), api.POLYNOMIAL_4D(w.bottom.roots, w.bottom.powers))
+    p = api.make_params(w.max_iter, w.escape, w.exp_scaling, w.slice, MAP_BLACK_HOLE)
+    ws = api.Workspace()
+    f64 = api.field_eval(g, r, p, FP64, workspace=ws).cpu().numpy()
+    samples = 5000
+    ws.check(samples)
+    ws.reset_stats()
+    f32 = api.field_eval(g, r, p, FP32_BAND, 0.05, workspace=ws).cpu().numpy()
+    st = ws.stats()
+    n = w.voxels
+    idx = ((2 * np.arange(samples) + 1) * n) // (2 * samples)
+    a, b = f32[idx], f64[idx]
+    fin = np.isfinite(a) & np.isfinite(b)
+    print("check:", {k: st[k] for k in st if k.startswith("check") or k == "checked"}, flush=True)
+    assert st["checked"] == samples
+    assert st["check_sign_flips"] == signflips(a, b) == 0
+    assert st["check_class_mismatch"] == int(np.count_nonzero(np.isfinite(a) != np.isfinite(b)))
+    assert abs(st["check_max_abs_dF"] - np.max(np.abs(a[fin] - b[fin]))) <= 1e-12
+    assert 0 < st["check_max_abs_dF"] < 0.05
+    # off again: no more checked cells
+    ws.check(0)
+    api.field_eval(g, r, p, FP32_BAND, 0.05, workspace=ws)
+    assert ws.stats()["checked"] == samples

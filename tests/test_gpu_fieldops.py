@@ -1,0 +1,127 @@
+"""FIELD_3D clean-up passes on the GPU (qj_field_clamp_infs_to_mean,
+qj_field_clamp_nans_to_mean, qj_field_set_borders) against the oracle's
+serial restatement, and the reference's clamp-then-mesh step
+(toothOptimizer.cpp:250-254) on a torus tile.
+
+Tolerances: the device mean is the correctly rounded sum of the summed cells
+over their count (double-double sum), checked bit for bit against
+math.fsum / count; against the oracle's serial sum it is within that sum's
+error bound (n - 1) 2^-53 sum|x| / count. Cells that are not replaced are
+bit-identical; the faces are exact."""
+import math
+
+import numpy as np
+import pytest
+import torch
+
+import oracle as O
+from quijibo_amd import api, workloads as W
+from quijibo_amd._lib import FP32_BAND, MAP_BLACK_HOLE
+
+pytestmark = pytest.mark.gpu
+
+U = 2.0 ** -53
+
+
+def _field(res, data):
+    return api.FIELD_3D(*res, data=torch.from_numpy(np.ascontiguousarray(data, dtype=np.float64)).cuda())
+
+
+def _random_field(n, seed, n_inf=0, n_nan=0):
+    rng = np.random.default_rng(seed)
+    f = rng.normal(size=n) * np.exp(rng.uniform(-8, 8, size=n))
+    at = rng.choice(n, n_inf + n_nan, replace=False)
+    f[at[:n_inf]] = np.where(rng.uniform(size=n_inf) < 0.5, np.inf, -np.inf)
+    f[at[n_inf:]] = np.nan
+    return f
+
+
+@pytest.mark.parametrize("n", [1, 255, 256, 1000, 70001, 3 << 20])
+def test_clamp_infs_matches_the_serial_mean(gpu, n):
+    res = (n, 1, 1)
+    f = _random_field(n, n, n_inf=max(1, n // 50) if n > 1 else 0)
+    F = _field(res, f)
+    mean = F.clampInfsToMean()
+    got = F.numpy()
+    keep = np.isfinite(f)
+    exact = math.fsum(f[keep]) / keep.sum()
+    assert mean == exact, (mean, exact)
+    _, serial = O.clamp_infs_to_mean(f)
+    assert abs(mean - serial) <= (keep.sum() - 1) * U * np.sum(np.abs(f[keep])) / keep.sum()
+    assert np.array_equal(got[keep], f[keep])
+    assert np.all(got[~keep] == mean)
+
+
+def test_clamp_nans_and_the_serial_sum_quirks(gpu):
+    f = _random_field(50000, 3, n_nan=400)
+    F = _field((100, 50, 10), f)
+    mean = F.clampNansToMean()
+    keep = ~np.isnan(f)
+    assert mean == math.fsum(f[keep]) / keep.sum()
+    assert not np.isnan(F.numpy()).any()
+    # a NaN cell makes clampInfsToMean's mean NaN; an infinite cell makes
+    # clampNansToMean's mean infinite (both signs: NaN)
+    g = np.array([1.0, np.nan, np.inf, 2.0] * 1000)
+    for fn, want in (("clampInfsToMean", "nan"), ("clampNansToMean", "inf")):
+        G = _field((4000, 1, 1), g)
+        m = getattr(G, fn)()
+        ref_f, ref_m = (O.clamp_infs_to_mean if fn == "clampInfsToMean" else O.clamp_nans_to_mean)(g)
+        assert (math.isnan(m) and math.isnan(ref_m)) if want == "nan" else (m == ref_m == np.inf)
+        assert np.array_equal(G.numpy(), ref_f, equal_nan=True)
+    h = np.array([1.0, np.nan, np.inf, -np.inf] * 100)
+    H = _field((400, 1, 1), h)
+    assert math.isnan(H.clampNansToMean())
+    Z = _field((6, 1, 1), np.full(6, np.inf))
+    assert math.isnan(Z.clampInfsToMean()) and np.isnan(Z.numpy()).all()
+
+
+@pytest.mark.parametrize("res", [(5, 4, 3), (1, 6, 7), (2, 2, 2), (9, 1, 1), (130, 67, 33)])
+def test_set_borders_matches_the_oracle(gpu, res):
+    f = np.arange(np.prod(res), dtype=np.float64)
+    F = _field(res, f)
+    F.setBorders(float(np.prod(res)))
+    assert np.array_equal(F.numpy(), O.set_borders(O.make_grid(res), f, float(np.prod(res))))
+
+
+def test_clamp_then_mesh_on_a_torus_tile(gpu):
+    """The reference's step before meshing a fractal: clampInfsToMean, then
+    setBorders(totalCells) (toothOptimizer.cpp:250-254). On a 128^3 window of
+    a 1024^3 torus tile (128 + 128 roots) in FP32 + FP64 band; the mesh of the
+    clamped field equals the oracle's mesh of the same field bit for bit."""
+    w = W.config3()
+    c, L = W.tile_grid(w, 2, 2, 2, 128)
+    res = (128, 128, 128)
+    g = api.make_grid(res, c, L)
+    r = api.Rational(api.POLYNOMIAL_4D(w.top.roots, w.top.powers), api.POLYNOMIAL_4D(w.bottom.roots, w.bottom.powers))
+    p = api.make_params(w.max_iter, 2000.0, 1.0, 0.0, MAP_BLACK_HOLE)
+    raw = api.field_eval(g, r, p, FP32_BAND, 0.05)
+    f0 = raw.cpu().numpy()
+    n_nan, n_inf = int(np.isnan(f0).sum()), int(np.isinf(f0).sum())
+    print("tile window: %d NaN, %d infinite cells" % (n_nan, n_inf), flush=True)
+    assert n_inf > 0
+    F = api.FIELD_3D(*res, center=c, lengths=L, data=raw.clone())
+    if n_nan:
+        F.clampNansToMean()
+    F.clampInfsToMean()
+    F.setBorders(F.totalCells())
+    clamped = F.numpy()
+    assert np.isfinite(clamped).all()
+    ref = f0.copy()
+    if n_nan:
+        ref, _ = O.clamp_nans_to_mean(ref)
+    ref, _ = O.clamp_infs_to_mean(ref)
+    ref = O.set_borders(O.make_grid(res, c, L), ref, float(F.totalCells()))
+    changed = ~np.isfinite(f0)
+    assert np.array_equal(clamped[~changed], ref[~changed])
+    assert np.allclose(clamped[changed], ref[changed], rtol=1e-12, atol=0)
+    v, t = api.marching_cubes(F.data.view(-1), F.grid())
+    ov, ot = O.marching_cubes(O.make_grid(res, c, L), clamped)
+    assert np.array_equal(v, ov) and np.array_equal(t, ot)
+    assert len(t) > 0 and np.isfinite(v).all()
+    # the raw field's mesh: vertices on edges whose lower end is -inf are NaN
+    # (d0 / (d0 - d1) = -inf / -inf), on both sides
+    rv, rt = api.marching_cubes(raw, g)
+    orv, ort = O.marching_cubes(O.make_grid(res, c, L), f0)
+    print("raw mesh: %d of %d vertices non-finite" % (int((~np.isfinite(rv)).any(axis=1).sum()), len(rv)),
+          flush=True)
+    assert np.array_equal(rv, orv, equal_nan=True) and np.array_equal(rt, ort)
