@@ -174,6 +174,20 @@ qj_status qj_field_eval(qj_workspace* ws, const qj_grid* grid, const qj_rational
                         const qj_map_params* params, qj_precision precision, double band,
                         double* d_field, void* stream);
 
+/* qj_field_eval without a workspace and without any device allocation, for
+ * callers that cannot allocate (code under hipGraph capture, tight serving
+ * loops, harnesses that forbid hipMalloc). No counterpart in the reference.
+ * Same arguments and the same field, bit for bit, as qj_field_eval. The band
+ * and overflow lists live in a file-scope device array of 32 MiB per GPU
+ * (held until the process ends) and the grid is evaluated in z-slabs of at
+ * most 4194304 cells, so any grid whose x*y slice has at most 4194304 cells is
+ * accepted (QJ_ERR_CAPACITY otherwise). Enqueues kernels only; no statistics
+ * or timing. Every call on one GPU shares that array: calls that overlap on
+ * one GPU (other streams, other threads) race and must be serialized by the
+ * caller. qj_init sets the array up; otherwise the first call on a GPU does. */
+qj_status qj_field_eval_noalloc(const qj_grid* grid, const qj_rational* f, const qj_map_params* params,
+                                qj_precision precision, double band, double* d_field, void* stream);
+
 /* Same map at n arbitrary spatial points (X, Y, Z) (device, n x 3 doubles). */
 qj_status qj_points_eval(qj_workspace* ws, const double* d_points, int64_t n, const qj_rational* f,
                          const qj_map_params* params, qj_precision precision, double band,

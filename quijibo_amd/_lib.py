@@ -94,6 +94,8 @@ SIGNATURES = [
     ("qj_workspace_kernel_ms", ctypes.c_int32, [_P, c_double_p, c_double_p, ctypes.POINTER(ctypes.c_int64)]),
     ("qj_field_eval", ctypes.c_int32, [_P, ctypes.POINTER(Grid), _P, ctypes.POINTER(MapParams), ctypes.c_int32,
                                        ctypes.c_double, _P, _P]),
+    ("qj_field_eval_noalloc", ctypes.c_int32, [ctypes.POINTER(Grid), _P, ctypes.POINTER(MapParams), ctypes.c_int32,
+                                               ctypes.c_double, _P, _P]),
     ("qj_points_eval", ctypes.c_int32, [_P, _P, ctypes.c_int64, _P, ctypes.POINTER(MapParams), ctypes.c_int32,
                                         ctypes.c_double, _P, _P]),
     ("qj_mc_run", ctypes.c_int32, [_P, _P, ctypes.POINTER(Grid), ctypes.c_double, ctypes.POINTER(ctypes.c_int64),

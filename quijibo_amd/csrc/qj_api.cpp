@@ -237,7 +237,8 @@ qj_status qj_init(int32_t device) {
     return QJ_ERR_INVALID;
   }
   QJ_HIP(hipSetDevice(device));
-  return QJ_OK;
+  // the allocation-free entry point's scratch address on this GPU
+  return scratch_setup(device, nullptr);
 }
 
 qj_status qj_rational_create(const qj_poly* top, const qj_poly* bottom, qj_rational** out) {

@@ -716,3 +716,82 @@ qj_status launch_points(qj_workspace* ws, const MapArgs& a, const qj_rational* f
 
 }  // namespace qj
 
+// ---------------------------------------------------------------- allocation-free entry point
+
+namespace qj {
+
+// Scratch of qj_field_eval_noalloc, a file-scope device array (each GPU holds
+// its own copy until the process ends): for one z-slab of at most
+// kStaticSlabVoxels cells, the band list and the adaptive list, their two
+// counts, and the statistics counters the kernels add to. 32 MiB + 4.3 KB.
+constexpr int64_t kStaticSlabVoxels = int64_t(1) << 22;
+constexpr size_t kStaticListBytes = sizeof(uint32_t) * (size_t)kStaticSlabVoxels;
+constexpr size_t kStaticCountOff = 2 * kStaticListBytes;
+constexpr size_t kStaticCounterOff = kStaticCountOff + 256;
+constexpr size_t kStaticBytes = kStaticCounterOff + sizeof(unsigned long long) * CNT_N * kCounterSpread;
+static __device__ __align__(256) unsigned char scratch[kStaticBytes];
+
+constexpr int kMaxDevices = 64;
+static std::mutex g_scratch_mu;
+static unsigned char* g_scratch[kMaxDevices];  // device address of `scratch` per GPU
+
+// Looks up the address of this GPU's copy of `scratch`; `device` must be the
+// current device. Host code uses only that address, never the array's name.
+qj_status scratch_setup(int device, unsigned char** out) {
+  if (device < 0 || device >= kMaxDevices) {
+    set_error("device %d beyond the %d the allocation-free entry point supports", device, kMaxDevices);
+    return QJ_ERR_INVALID;
+  }
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch[device]) {
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess) return hip_fail(e, "hipGetSymbolAddress(scratch)");
+    g_scratch[device] = (unsigned char*)p;
+  }
+  if (out) *out = g_scratch[device];
+  return QJ_OK;
+}
+
+}  // namespace qj
+
+extern "C" qj_status qj_field_eval_noalloc(const qj_grid* grid, const qj_rational* f, const qj_map_params* params,
+                                           qj_precision precision, double band, double* d_field, void* stream) {
+  using namespace qj;
+  if (!d_field) {
+    set_error("qj_field_eval_noalloc: null output");
+    return QJ_ERR_INVALID;
+  }
+  if (precision != QJ_FP64 && precision != QJ_FP32_BAND && precision != QJ_FP64_EXACT) {
+    set_error("qj_field_eval_noalloc: unknown precision %d", (int)precision);
+    return QJ_ERR_INVALID;
+  }
+  if (params && (params->flags & QJ_MAP_LAGRANGE) && precision != QJ_FP64) {
+    set_error("qj_field_eval_noalloc: QJ_MAP_LAGRANGE needs QJ_FP64");
+    return QJ_ERR_INVALID;
+  }
+  MapArgs a;
+  qj_status st = make_map_args(grid, f, params, &a);
+  if (st) return st;
+  const int64_t XY = (int64_t)a.res[0] * a.res[1];
+  if (XY > kStaticSlabVoxels) {
+    set_error("qj_field_eval_noalloc: a z-slice of %lld cells exceeds the maximum of %lld", (long long)XY,
+              (long long)kStaticSlabVoxels);
+    return QJ_ERR_CAPACITY;
+  }
+  int dev = 0;
+  QJ_HIP(hipGetDevice(&dev));
+  unsigned char* base = nullptr;
+  if ((st = scratch_setup(dev, &base))) return st;
+  const FieldBufs fb{(uint32_t*)base, (uint32_t*)(base + kStaticListBytes), kStaticSlabVoxels,
+                     (uint32_t*)(base + kStaticCountOff), (unsigned long long*)(base + kStaticCounterOff), nullptr};
+  const hipStream_t s = (hipStream_t)stream;
+  const int64_t Z = a.res[2], nz = kStaticSlabVoxels / XY;
+  for (int64_t z0 = 0; z0 < Z; z0 += nz) {
+    MapArgs sa = a;
+    sa.res[2] = (int32_t)(Z - z0 < nz ? Z - z0 : nz);
+    sa.zoff = (int32_t)z0;
+    if ((st = launch_field_bufs(fb, sa, f, precision, band, d_field + z0 * XY, s))) return st;
+  }
+  return QJ_OK;
+}

@@ -157,6 +157,10 @@ qj_status points_adjoint(qj_workspace* ws, const MapArgs& a, const qj_rational* 
                          const double* d_pts, int64_t n, const double* d_dJdF, double* d_gr, double* d_gp,
                          hipStream_t s);
 
+// address of this GPU's copy of the allocation-free entry point's scratch
+// array (qj_field.hip); `device` must be current
+qj_status scratch_setup(int device, unsigned char** out);
+
 // FIELD_3D clamp / border passes (qj_fieldops.hip)
 qj_status field_clamp_to_mean(qj_workspace* ws, double* d_field, int64_t n, bool nans, double* mean_out,
                               hipStream_t s);
