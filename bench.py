@@ -93,6 +93,10 @@ def main():
     ap.add_argument("--serve-launch-work", type=float, default=0.0,
                     help="root-term iterations per request-loop launch (0 = library default)")
     ap.add_argument("--serve-inflight", type=int, default=0, help="launches in flight (0 = library default)")
+    ap.add_argument("--serve-pending-s", type=float, default=2.0,
+                    help="refuse submissions beyond this many seconds of backlog (0 = never refuse)")
+    ap.add_argument("--serve-aging-s", type=float, default=0.5, help="requests waiting longer go first")
+    ap.add_argument("--serve-check", type=int, default=512, help="cross-check samples per served result")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: the field, mesher and gradient paths run on a single MI355X "
@@ -274,34 +278,56 @@ def main():
         "serving": serving,
         "p50_ms_256": (serving or {}).get("latency", {}).get("256^3", {}).get("p50_ms") if serving else None,
         "p99_ms_256": (serving or {}).get("latency", {}).get("256^3", {}).get("p99_ms") if serving else None,
+        "p99_ms_512": (serving or {}).get("latency", {}).get("512^3", {}).get("p99_ms") if serving else None,
+        "serving_refused": (serving or {}).get("refused") if serving else None,
     }
     print(json.dumps(line), flush=True)
 
 
 def serve_stream(args, log):
     """Config 4: Poisson arrivals at serve_rate, grid edge in {64,128,256,512}, 8-256 roots
-    (half top, half bottom, Gaussian sigma 0.7), maxIter 3-5, through the request loop."""
+    (half top, half bottom, Gaussian sigma 0.7), maxIter 3-5, through the request loop, in the
+    reference's default-build range (QJ_MAP_EXTENDED_RANGE, Real = long double). Submissions
+    beyond serve_pending_s of backlog (in seconds of the loop's measured root-term rate) are
+    refused; requests waiting longer than serve_aging_s go first; every result carries its
+    sampled FP64 cross-check."""
     from quijibo_amd import server as S, workloads as W
-    from quijibo_amd._lib import FP32_BAND
+    from quijibo_amd._lib import FP32_BAND, MAP_BLACK_HOLE, MAP_EXTENDED_RANGE
     reqs = W.config4_requests(n=args.serve, rate=args.serve_rate)
+    rate = 2.5e12  # root-term iterations per second the loop sustains on this stream
     srv = S.Server(hbm_budget_bytes=int(args.serve_budget_gib * (1 << 30)), precision=FP32_BAND, band=args.band,
-                   launch_work=args.serve_launch_work, inflight_launches=args.serve_inflight)
+                   launch_work=args.serve_launch_work, inflight_launches=args.serve_inflight,
+                   max_pending_work=args.serve_pending_s * rate, aging_s=args.serve_aging_s,
+                   check_samples=args.serve_check)
+    flags = MAP_BLACK_HOLE | MAP_EXTENDED_RANGE
     # warm the graphs with one small request
-    srv.submit(reqs[0][1])
+    srv.submit(reqs[0][1], flags=flags)
     srv.wait_all(1, timeout=60)
     log("serving: %d requests at %.0f req/s" % (len(reqs), args.serve_rate))
-    done, ids, start = S.replay(srv, reqs, log)
-    end = max(c.done_time for c in done)
+    done, ids, start, refused = S.replay(srv, reqs, log, flags=flags)
+    end = max(c.done_time for c in done) if done else start
     srv.close()
     vox_iters = sum(ids[c.id].voxels * ids[c.id].max_iter for c in done if c.status == 0)
     lat = S.latency_summary(done, ids)
-    return {"requests": len(reqs), "offered_rate_req_s": args.serve_rate, "completed": sum(c.status == 0 for c in done),
-            "wall_s": round(end - start, 3), "throughput_req_s": round(len(done) / (end - start), 2),
-            "throughput_gvoxel_iter_s": round(vox_iters / (end - start) / 1e9, 3), "latency": lat,
+    ok = [c for c in done if c.status == 0]
+    check = {"checked_cells": int(sum(c.checked for c in ok)),
+             "results_checked": int(sum(1 for c in ok if c.checked > 0)),
+             "sign_flips": int(sum(c.check_sign_flips for c in ok)),
+             "class_mismatch": int(sum(c.check_class_mismatch for c in ok)),
+             "max_abs_dF": max((c.check_max_abs_dF for c in ok), default=0.0),
+             "max_rel_dF": max((c.check_max_rel_dF for c in ok), default=0.0),
+             "samples_per_result": args.serve_check}
+    return {"requests": len(reqs), "offered_rate_req_s": args.serve_rate, "completed": len(ok),
+            "refused": refused, "wall_s": round(end - start, 3),
+            "throughput_req_s": round(len(ok) / max(1e-9, end - start), 2),
+            "throughput_gvoxel_iter_s": round(vox_iters / max(1e-9, end - start) / 1e9, 3), "latency": lat,
+            "error_statement": check,
             "config": "hbm budget %.0f GiB, 65536-voxel chunks, launches of <= 512 chunks and ~%s root-term "
-                      "iterations, %s launches in flight, fp32+fp64 band, shortest-remaining-work-first"
+                      "iterations, %s launches in flight, fp32+fp64 band, extended range, shortest-remaining-work "
+                      "first, aging %.2f s, refusal beyond %.1f s of backlog (%.2g root-term iterations)"
                       % (args.serve_budget_gib, ("%.1e" % args.serve_launch_work) if args.serve_launch_work else "3e9",
-                         args.serve_inflight or 2)}
+                         args.serve_inflight or 2, args.serve_aging_s, args.serve_pending_s,
+                         args.serve_pending_s * rate)}
 
 
 def nonlinear_mesher(args, log):
@@ -420,27 +446,30 @@ def mc_probe(args, log):
 
 
 def c3_tiles(args, log):
-    """1024^3 in 64 tiles of 256^3 (128+128 torus roots, maxIter 3), FP32+band, then marching
-    cubes per tile on the GPU. Two tiles are re-evaluated with the FP64 kernel and their
-    isosurface sign flips counted (the per-tile mesh parity against the oracle is a test:
-    tests/test_gpu_mc.py::test_c3_full_tile_mesh_is_bit_exact)."""
-    import ctypes
+    """1024^3 in 64 tiles of 256^3 (128+128 torus roots, maxIter 3), FP32+band in the reference's
+    default-build range (QJ_MAP_EXTENDED_RANGE, Real = long double: products of 256 roots stay
+    finite), then the reference's clean-up before meshing (clampInfsToMean, setBorders(totalCells),
+    toothOptimizer.cpp:250-254) and marching cubes of all 64 tiles in one call. Every tile's error
+    statement comes from a sampled FP64 cross-check (4096 cells per tile, untimed pass); two tiles
+    are re-evaluated in full with the FP64 kernel."""
     import torch
     from quijibo_amd import api, workloads as W
-    from quijibo_amd._lib import FP32_BAND, FP64, MAP_BLACK_HOLE
+    from quijibo_amd._lib import FP32_BAND, FP64, MAP_BLACK_HOLE, MAP_EXTENDED_RANGE
     w = W.config3(1024, 256)
     tile = 256
     per = tile ** 3
     nt_axis = w.res[0] // tile
     tiles = [(tz, ty, tx) for tz in range(nt_axis) for ty in range(nt_axis) for tx in range(nt_axis)]
     r = api.Rational(api.POLYNOMIAL_4D(w.top.roots, w.top.powers), api.POLYNOMIAL_4D(w.bottom.roots, w.bottom.powers))
-    params = api.make_params(w.max_iter, w.escape, w.exp_scaling, w.slice, MAP_BLACK_HOLE)
+    params = api.make_params(w.max_iter, w.escape, w.exp_scaling, w.slice, MAP_BLACK_HOLE | MAP_EXTENDED_RANGE)
     ws = api.Workspace()
     out = torch.empty(len(tiles) * per, dtype=torch.float64, device="cuda")
-    grids = []
+    grids, centers, lengths = [], [], []
     for (tz, ty, tx) in tiles:
         c, tl = W.tile_grid(w, tz, ty, tx, tile)
         grids.append(api.make_grid((tile, tile, tile), c, tl))
+        centers.append(c)
+        lengths.append(tl)
     log("c3: %d tiles of %d^3, %d+%d roots" % (len(tiles), tile, w.top.n, w.bottom.n))
     api.field_eval(grids[0], r, params, FP32_BAND, args.band, out=out[:per], workspace=ws)  # warm-up
     torch.cuda.synchronize()
@@ -452,26 +481,51 @@ def c3_tiles(args, log):
     t_eval = time.perf_counter() - t0
     st = ws.stats()
     log("c3: field %.1f ms" % (t_eval * 1e3))
-    L = api._lib.load()
-    nv, nt = ctypes.c_int64(), ctypes.c_int64()
-    tot_v = tot_t = 0
-    t0 = time.perf_counter()
+    # error statement of every tile (untimed): sampled FP64 cross-check
+    chk = api.Workspace()
+    chk.check(4096)
+    scratch = torch.empty(per, dtype=torch.float64, device="cuda")
+    worst = {"max_abs_dF": 0.0, "max_rel_dF": 0.0, "sign_flips": 0, "class_mismatch": 0, "checked": 0}
     for k, g in enumerate(grids):
-        api.check(L.qj_mc_run(ws.handle, api._ptr(out[k * per:(k + 1) * per]), ctypes.byref(g), 0.0,
-                              ctypes.byref(nv), ctypes.byref(nt), api._stream_ptr(None)))
-        tot_v += nv.value
-        tot_t += nt.value
+        chk.reset_stats()
+        api.field_eval(g, r, params, FP32_BAND, args.band, out=scratch, workspace=chk)
+        s_k = chk.stats()
+        worst["max_abs_dF"] = max(worst["max_abs_dF"], s_k["check_max_abs_dF"])
+        worst["max_rel_dF"] = max(worst["max_rel_dF"], s_k["check_max_rel_dF"])
+        worst["sign_flips"] += s_k["check_sign_flips"]
+        worst["class_mismatch"] += s_k["check_class_mismatch"]
+        worst["checked"] += s_k["checked"]
+    nonfinite = int((~torch.isfinite(out)).sum().item())
+    log("c3: error statement over %d tiles %s, %d non-finite cells" % (len(tiles), worst, nonfinite))
+    # clean-up before meshing, then all tiles in one marching-cubes call
+    t0 = time.perf_counter()
+    for k in range(len(tiles)):
+        F = api.FIELD_3D(tile, tile, tile, center=centers[k], lengths=lengths[k], data=out[k * per:(k + 1) * per])
+        F.clampInfsToMean(workspace=ws)
+        F.setBorders(F.totalCells())
+    torch.cuda.synchronize()
+    t_clamp = time.perf_counter() - t0
+    fields = [out[k * per:(k + 1) * per] for k in range(len(tiles))]
+    api.marching_cubes_tiles(fields[:2], grids[:2], workspace=ws)  # warm-up (buffer sizes)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    v, t, counts = api.marching_cubes_tiles(fields, grids, workspace=ws)
     torch.cuda.synchronize()
     t_mc = time.perf_counter() - t0
-    log("c3: marching cubes %.1f ms, %d vertices, %d triangles" % (t_mc * 1e3, tot_v, tot_t))
-    # parity on two tiles: one holding the torus surface, one elsewhere
+    tot_v, tot_t = int(v.shape[0]), int(t.shape[0])
+    nan_vertices = int((~torch.isfinite(v)).any(dim=1).sum().item())
+    log("c3: marching cubes %.1f ms, %d vertices, %d triangles, %d non-finite vertices" % (t_mc * 1e3, tot_v,
+                                                                                          tot_t, nan_vertices))
+    del v, t
+    # full FP64 parity on two tiles: one holding the torus surface, one elsewhere (fields re-evaluated,
+    # since the ones above were clamped)
     check = [tiles.index((1, 1, 0)), tiles.index((2, 2, 2))]
     parity = []
     f64 = torch.empty(per, dtype=torch.float64, device="cuda")
     for k in check:
         g = grids[k]
-        fld = out[k * per:(k + 1) * per]
-        host = fld.cpu().numpy()
+        api.field_eval(g, r, params, FP32_BAND, args.band, out=scratch, workspace=ws)
+        host = scratch.cpu().numpy()
         api.field_eval(g, r, params, FP64, out=f64, workspace=ws)
         h64 = f64.cpu().numpy()
         flips = int(np.count_nonzero((h64.astype(np.float32) > 0) != (host.astype(np.float32) > 0)))
@@ -479,14 +533,16 @@ def c3_tiles(args, log):
         parity.append({"tile": list(tiles[k]), "sign_flips_vs_fp64_gpu": flips,
                        "max_abs_dF_vs_fp64_gpu": float(np.max(np.abs(h64[fin] - host[fin]))) if fin.any() else 0.0})
         log("c3: tile %s parity %s" % (tiles[k], parity[-1]))
-    del out, f64
+    del out, f64, scratch, fields
     torch.cuda.empty_cache()
     vox_iters = len(tiles) * per * w.max_iter
     return {"workload": "1024^3 on [-1.5,1.5]^3 as 64 tiles of 256^3, 128+128 torus roots (R=1, r=0.35), powers 1, "
-                        "maxIter 3, FP32+FP64 band",
+                        "maxIter 3, FP32+FP64 band, extended range",
             "gvoxel_iter_s": round(vox_iters / t_eval / 1e9, 4), "field_ms": round(t_eval * 1e3, 2),
-            "band_voxels": st["band_voxels"], "mc_ms_all_tiles": round(t_mc * 1e3, 2),
-            "vertices": tot_v, "triangles": tot_t, "parity": parity}
+            "band_voxels": st["band_voxels"], "nonfinite_cells": nonfinite, "error_statement": worst,
+            "clamp_borders_ms": round(t_clamp * 1e3, 2), "mc_ms_all_tiles": round(t_mc * 1e3, 2),
+            "mc_host_syncs": 1, "vertices": tot_v, "triangles": tot_t, "nonfinite_vertices": nan_vertices,
+            "parity": parity}
 
 
 def _kernel_ms(api, ws):

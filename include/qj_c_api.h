@@ -15,6 +15,7 @@
  *   OPTIMIZE_3D::computeLogScaledPowerRationalMapCached  src/solvers/OPTIMIZE_3D.cpp:1572-1718
  *   the dormant GPU seam setTopRoots/setConsts/runMap  src/solvers/OPTIMIZE_3D.cpp:3842-3858
  *   TRIANGLE_MESH(const FIELD_3D&) / computeStagedMarchingCubes  src/geometry/TRIANGLE_MESH.h:42, .cpp:989-1114
+ *   TRIANGLE_MESH(const FIELD_3D&, top, bottom, ...) / computeNonlinearMarchingCubes  TRIANGLE_MESH.h:45, .cpp:62-79,152
  *   OPTIMIZE_3D::computeScoreAP, computeExpGradientAP, compute{Top,Bottom}GradientAP,
  *     compute{Top,Bottom}BulkScalarGradientAP       src/solvers/OPTIMIZE_3D.cpp:3563, 3589, 8905, 10012, 9250, 10360
  *   OPTIMIZE_3D::computeLogRationalCenteredGradient (root-position gradient) src/solvers/OPTIMIZE_3D.cpp:2872-2951
@@ -38,7 +39,8 @@ enum {
   QJ_ERR_NOMEM = 3,       /* device allocation failed */
   QJ_ERR_CAPACITY = 4,    /* request larger than the configured capacity */
   QJ_ERR_NUMERIC = 5,     /* inconsistent numeric state (the reference would exit(0)) */
-  QJ_ERR_SHUTDOWN = 6     /* server is stopping */
+  QJ_ERR_SHUTDOWN = 6,    /* server is stopping */
+  QJ_ERR_BUSY = 7         /* request refused: the request loop's pending work is over its limit */
 };
 
 /* A FIELD_3D geometry (FIELD_3D.h:58,73-80; FIELD_3D.cpp:28-57): cell-centred,
@@ -70,6 +72,12 @@ typedef struct qj_poly {
                                  factors interleaved, r = (r t_i) / b_i, then the leftovers; the map of the
                                  Lagrange mesher (TRIANGLE_MESH.cpp:106-123, 281-324). QJ_FP64 only; not
                                  with QJ_MAP_SKIP_ORIGIN or QJ_MAP_ADAPTIVE */
+#define QJ_MAP_EXTENDED_RANGE 16u /* the range of the reference's default build (Real = long double,
+                                 SETTINGS.h:10): factored products, quotient and |q| carry a separate binary
+                                 exponent, so products of many roots stay finite where FP64 gives inf, NaN or
+                                 -inf; FP64 mantissas, so values agree with a long double evaluation to the
+                                 FP64 tolerance. Every precision, the mesher and the request loop; not with
+                                 QJ_MAP_ADAPTIVE (which has its own re-pass) or QJ_MAP_LAGRANGE */
 
 /* Map parameters: the OPTIMIZE_3D members the map reads (OPTIMIZE_3D.h:67,73,81)
  * plus the escape radius each reference variant hard-codes (2000 at
@@ -205,6 +213,16 @@ qj_status qj_points_eval(qj_workspace* ws, const double* d_points, int64_t n, co
  * the workspace until the next qj_mc_run on it. */
 qj_status qj_mc_run(qj_workspace* ws, const double* d_field, const qj_grid* grid, double iso,
                     int64_t* n_vertices, int64_t* n_triangles, void* stream);
+/* Marching cubes of n fields (tiles) in one call, with every count kept on
+ * the device and read back once at the end (no host synchronisation per
+ * field). Each field is meshed as qj_mc_run would mesh it alone; the meshes
+ * are stored one after another in the workspace: field k's vertices follow
+ * those of fields 0..k-1, and so do its triangles, whose vertex indices are
+ * local to field k. n_vertices / n_triangles (host, n entries each) receive
+ * the counts. qj_mc_copy / qj_mc_device_ptrs give the concatenated mesh.
+ * Synchronizes `stream`. */
+qj_status qj_mc_run_tiles(qj_workspace* ws, int32_t n, const double* const* d_fields, const qj_grid* grids,
+                          double iso, int64_t* n_vertices, int64_t* n_triangles, void* stream);
 /* Copy the last mesh: vertices n_vertices x 3 doubles, triangles n_triangles x 3
  * int32 (0-based). Pointers may be host or device memory (hipMemcpyDefault). */
 qj_status qj_mc_copy(qj_workspace* ws, double* vertices, int32_t* triangles, void* stream);
@@ -227,6 +245,22 @@ qj_status qj_mc_device_ptrs(qj_workspace* ws, const double** d_vertices, const i
 qj_status qj_nlmc_run(qj_workspace* ws, const qj_grid* grid, const qj_rational* f, const qj_map_params* params,
                       double iso, qj_precision precision, double band, int32_t chunk_slices,
                       int64_t* n_vertices, int64_t* n_triangles, void* stream);
+
+/* The non-linear mesher on a given field: TRIANGLE_MESH(const FIELD_3D& field,
+ * top, bottom, expScaling, maxIterations, slice, isosurface)
+ * (TRIANGLE_MESH.h:45, TRIANGLE_MESH.cpp:62-79 -> computeNonlinearMarchingCubes
+ * :152-215, computeNonlinearEdgeInterpolations :1824-1910). Cases come from
+ * d_field (device, x fastest) with the float test (float)v > 0 (no
+ * isovalue); each vertex is placed by the bisection of qj_nlmc_run on the map
+ * re-evaluated in FP64 at both edge ends; vertex numbering, triangle order
+ * and the degenerate filter are the linear mesher's. params as for
+ * qj_nlmc_run (the reference's escape is 2000; the black-hole stop always
+ * applies). Results via qj_mc_copy; the edge cache via qj_nlmc_cache.
+ * Synchronizes `stream`. Returns QJ_ERR_NUMERIC (mesh still available) when
+ * an edge's re-evaluated end values do not straddle the isosurface. */
+qj_status qj_nlmc_field(qj_workspace* ws, const double* d_field, const qj_grid* grid, const qj_rational* f,
+                        const qj_map_params* params, double iso, int64_t* n_vertices, int64_t* n_triangles,
+                        void* stream);
 
 /* The edge cache of the last qj_nlmc_run / qj_nlmc_resume on ws, the data of
  * the mesher's cache file (writeEdgeCacheHuge / readEdgeCacheHuge,
@@ -323,6 +357,13 @@ typedef struct qj_server_config {
   int32_t device;
   double launch_work;         /* root-term iterations per launch (voxels x max_iter x roots), so every
                                  launch takes about the same GPU time (0 = 3e9, about 1.5 ms) */
+  double max_pending_work;    /* root-term iterations admitted or waiting beyond which qj_server_submit
+                                 refuses a request with QJ_ERR_BUSY (0 = no limit; a request arriving at an
+                                 idle server is always taken) */
+  double aging_s;             /* a request waiting longer than this many seconds is admitted and packed
+                                 before the others, oldest first (0 = 0.25) */
+  int32_t check_samples;      /* per-result error statement: cells of every finished field re-evaluated with
+                                 the reference method in FP64 and compared (qj_completion.checked ...; 0 = off) */
 } qj_server_config;
 
 typedef struct qj_request {
@@ -342,11 +383,20 @@ typedef struct qj_completion {
   double done_time;
   const double* d_field;    /* device field if keep_field, else NULL */
   int64_t voxels;
+  /* the per-result cross-check (qj_server_config.check_samples; zeros when off),
+     as in qj_eval_stats: cells compared, sign disagreements of (float)F > 0,
+     class disagreements, largest absolute and relative |F - F_fp64| */
+  int64_t checked;
+  int64_t check_sign_flips;
+  int64_t check_class_mismatch;
+  double check_max_abs_dF;
+  double check_max_rel_dF;
 } qj_completion;
 
 qj_status qj_server_create(const qj_server_config* cfg, qj_server** out);
 double qj_server_now(qj_server* s); /* seconds on the server's monotonic clock */
-/* Copies the request (roots included); never blocks on the GPU. */
+/* Copies the request (roots included); never blocks on the GPU. Returns
+ * QJ_ERR_BUSY (no id, nothing queued) when max_pending_work would be exceeded. */
 qj_status qj_server_submit(qj_server* s, const qj_request* req, uint64_t* id);
 /* Up to max completions; waits up to timeout_s for at least one (0 = poll). */
 qj_status qj_server_poll(qj_server* s, qj_completion* out, int32_t max, int32_t* n_out, double timeout_s);

@@ -161,6 +161,11 @@ class TRIANGLE_MESH {
   TRIANGLE_MESH() {}
   // marching cubes on a field (TRIANGLE_MESH.cpp:48-57 -> computeStagedMarchingCubes)
   explicit TRIANGLE_MESH(const FIELD_3D& field, Real isosurface = 0.0);
+  // Non-linear marching cubes on a field (TRIANGLE_MESH.cpp:62-79 ->
+  // computeNonlinearMarchingCubes :152): cases from the field's signs,
+  // vertices by bisection on the map re-evaluated on the GPU.
+  TRIANGLE_MESH(const FIELD_3D& field, const POLYNOMIAL_4D& top, const POLYNOMIAL_4D& bottom, Real expScaling,
+                int maxIterations, Real slice, Real isosurface);
   // Non-linear low-memory mesher with an edge cache (TRIANGLE_MESH.cpp:84-101,
   // 681-768): slabs of the map evaluated on the fly, vertices refined by
   // bisection. When cacheFilename names an existing cache (readEdgeCacheHuge

@@ -18,6 +18,7 @@ BLACK_HOLE = 1
 SKIP_ORIGIN = 2
 AP_REPASS = 4
 LAGRANGE = 8
+EXTENDED = 16
 
 
 class Grid(ctypes.Structure):
@@ -75,6 +76,11 @@ def lib():
                                       ctypes.POINTER(Params), ctypes.c_double, ctypes.POINTER(ctypes.c_int64),
                                       ctypes.POINTER(ctypes.c_int64), ctypes.c_void_p, ctypes.c_void_p]
         L.qo_nonlinear_mc.restype = ctypes.c_int64
+        L.qo_nonlinear_mc_field.argtypes = [ctypes.POINTER(Grid), ctypes.c_void_p, ctypes.POINTER(Poly),
+                                            ctypes.POINTER(Poly), ctypes.POINTER(Params), ctypes.c_double,
+                                            ctypes.POINTER(ctypes.c_int64), ctypes.POINTER(ctypes.c_int64),
+                                            ctypes.c_void_p, ctypes.c_void_p]
+        L.qo_nonlinear_mc_field.restype = ctypes.c_int64
         L.qo_field_lookup.argtypes = [ctypes.POINTER(Grid), dp, dp]
         L.qo_field_lookup.restype = ctypes.c_double
         L.qo_ap_score_gradients.argtypes = [
@@ -181,6 +187,22 @@ def nonlinear_mc(grid, top, bottom, params, iso=0.0):
     tris = np.empty((max(nt.value, 1), 3), dtype=np.int32)
     bad = L.qo_nonlinear_mc(ctypes.byref(grid), ctypes.byref(top.p), ctypes.byref(bottom.p), ctypes.byref(params),
                             float(iso), ctypes.byref(nv), ctypes.byref(nt), verts.ctypes.data, tris.ctypes.data)
+    return verts, tris[: nt.value].copy(), int(bad)
+
+
+def nonlinear_mc_field(grid, field, top, bottom, params, iso=0.0):
+    """The non-linear mesher on a given field; returns (vertices, triangles, n_bad_edges)."""
+    f = np.ascontiguousarray(np.asarray(field, dtype=np.float64).reshape(-1))
+    nv = ctypes.c_int64()
+    nt = ctypes.c_int64()
+    L = lib()
+    L.qo_nonlinear_mc_field(ctypes.byref(grid), f.ctypes.data, ctypes.byref(top.p), ctypes.byref(bottom.p),
+                            ctypes.byref(params), float(iso), ctypes.byref(nv), ctypes.byref(nt), None, None)
+    verts = np.empty((nv.value, 3))
+    tris = np.empty((max(nt.value, 1), 3), dtype=np.int32)
+    bad = L.qo_nonlinear_mc_field(ctypes.byref(grid), f.ctypes.data, ctypes.byref(top.p), ctypes.byref(bottom.p),
+                                  ctypes.byref(params), float(iso), ctypes.byref(nv), ctypes.byref(nt),
+                                  verts.ctypes.data, tris.ctypes.data)
     return verts, tris[: nt.value].copy(), int(bad)
 
 

@@ -310,11 +310,56 @@ static Q lagrange_eval(const qo_poly* top, const qo_poly* bottom, Q q) {
   return r;
 }
 
+/* POLYNOMIAL_4D::evaluateScaledPowerFactored (POLYNOMIAL_4D.cpp:912-923) with
+ * Real = long double: QUATERNION_T<long double>::pow and operator* */
+static QE poly_eval_ld(const qo_poly* p, QE q) {
+  const long double s = (long double)p->power_scalar;
+  QE r0 = {p->roots[0], p->roots[1], p->roots[2], p->roots[3]};
+  QE result = epow(esub(q, r0), s * (long double)p->powers[0]);
+  for (int i = 1; i < p->n; i++) {
+    QE ri = {p->roots[4 * i], p->roots[4 * i + 1], p->roots[4 * i + 2], p->roots[4 * i + 3]};
+    result = emul(result, epow(esub(q, ri), s * (long double)p->powers[i]));
+  }
+  return result;
+}
+
+/* One voxel of computeLogScaledPowerRationalMap (OPTIMIZE_3D.cpp:1317-1362) in
+ * the reference's default build, Real = long double (SETTINGS.h:10): every
+ * quaternion operation, |q| and the escape / black-hole tests (REAL_MIN stays
+ * FLT_MIN, SETTINGS.h:58) in x87 long double; the field value is stored as a
+ * double (the FIELD_3D file format, FIELD_3D.cpp:4383-4394). The cell centre
+ * is the double one. */
+static int iterate_point_ld(Q q0, const qo_poly* top, const qo_poly* bottom, const qo_params* p, double* out,
+                            int32_t* iters) {
+  QE q = {q0.w, q0.x, q0.y, q0.z};
+  long double magnitude = emag(q);
+  if ((p->flags & QO_SKIP_ORIGIN) && magnitude < 1e-8L) return 0;
+  int it = 0, executed = 0;
+  while (magnitude < (long double)p->escape && it < p->max_iter) {
+    QE topEval = poly_eval_ld(top, q);
+    if (bottom->n > 0) {
+      QE bottomEval = poly_eval_ld(bottom, q);
+      q = ediv(topEval, bottomEval);
+    } else {
+      q = topEval;
+    }
+    q = escale(q, (long double)p->exp_scaling);
+    magnitude = emag(q);
+    it++;
+    executed++;
+    if ((p->flags & QO_BLACK_HOLE) && magnitude < 10.0L * (long double)FLT_MIN) it = p->max_iter;
+  }
+  *out = (double)logl(magnitude);
+  if (iters) *iters = executed;
+  return 1;
+}
+
 /* One voxel of OPTIMIZE_3D::computeLogScaledPowerRationalMap
  * (OPTIMIZE_3D.cpp:1317-1362) / computeRationalMapAP first pass (:1423-1475).
  * Returns 1 when the value was written, 0 for a skipped origin cell. */
 static int iterate_point(Q q, const qo_poly* top, const qo_poly* bottom, const qo_params* p,
                          double* out, int32_t* iters, int* overflowed) {
+  if (p->flags & QO_EXTENDED) return iterate_point_ld(q, top, bottom, p, out, iters);
   double magnitude = qmag(q);
   if ((p->flags & QO_SKIP_ORIGIN) && magnitude < 1e-8) return 0;
   int it = 0, executed = 0;
@@ -637,6 +682,16 @@ static int midpoint_search(double* pp, double vp, double* pn, double vn, const q
  * whose end values do not straddle zero (the reference exits on the first). */
 int64_t qo_nonlinear_mc(const qo_grid* g, const qo_poly* top, const qo_poly* bottom, const qo_params* p_in,
                         double iso, int64_t* n_vertices, int64_t* n_triangles, double* vertices, int32_t* triangles) {
+  return qo_nonlinear_mc_field(g, NULL, top, bottom, p_in, iso, n_vertices, n_triangles, vertices, triangles);
+}
+
+/* The same mesher with the cases taken from a given field (the
+ * TRIANGLE_MESH(const FIELD_3D&, top, bottom, ...) constructor,
+ * TRIANGLE_MESH.cpp:62-79 -> computeNonlinearMarchingCubes :152-215); with
+ * given == NULL the field is the map's own (the slab mesher). */
+int64_t qo_nonlinear_mc_field(const qo_grid* g, const double* given, const qo_poly* top, const qo_poly* bottom,
+                              const qo_params* p_in, double iso, int64_t* n_vertices, int64_t* n_triangles,
+                              double* vertices, int32_t* triangles) {
   /* the mesher's slabs and values always stop at the black hole
    * (TRIANGLE_MESH.cpp:374-376, 1789-1791) */
   qo_params pp = *p_in;
@@ -648,7 +703,8 @@ int64_t qo_nonlinear_mc(const qo_grid* g, const qo_poly* top, const qo_poly* bot
   uint8_t* mark = (uint8_t*)calloc((size_t)total, 1);
   int32_t* vid = (int32_t*)malloc(sizeof(int32_t) * (size_t)total);
   int64_t bad = 0;
-  qo_field_map(g, top, bottom, p, field, NULL, NULL, 0);
+  if (given) memcpy(field, given, sizeof(double) * (size_t)total);
+  else qo_field_map(g, top, bottom, p, field, NULL, NULL, 0);
   for (int64_t z = 0; z < zRes - 1; z++)
     for (int64_t y = 0; y < yRes - 1; y++)
       for (int64_t x = 0; x < xRes - 1; x++) {
@@ -783,6 +839,132 @@ void qo_set_borders(const qo_grid* g, double* data, double value) {
       AT(X - 1, y, z) = value;
     }
 #undef AT
+}
+
+/* ---------------- scoring fields of the shape optimizer ---------------- */
+
+/* FIELD_3D::outerShell + innerShell (FIELD_3D.cpp:5070-5145): 1 where a cell
+ * of one sign has a 26-neighbour of the other sign (zero cells have none). */
+static void shells(const qo_grid* g, const double* d, double* out) {
+  const int64_t X = g->res[0], Y = g->res[1], Z = g->res[2], XY = X * Y;
+  for (int64_t z = 0; z < Z; z++)
+    for (int64_t y = 0; y < Y; y++)
+      for (int64_t x = 0; x < X; x++) {
+        const double c = d[x + y * X + z * XY];
+        int flip = 0;
+        for (int i = -1; i <= 1; i++)
+          for (int j = -1; j <= 1; j++)
+            for (int k = -1; k <= 1; k++) {
+              if (i == 0 && j == 0 && k == 0) continue;
+              const int64_t xn = x + i, yn = y + j, zn = z + k;
+              if (xn < 0 || xn > X - 1 || yn < 0 || yn > Y - 1 || zn < 0 || zn > Z - 1) continue;
+              if (d[xn + yn * X + zn * XY] * c < 0) flip = 1;
+            }
+        /* outer shell for c >= 0, inner shell for c <= 0: one of them, or
+         * neither at c == 0 (no product with 0 is negative) */
+        out[x + y * X + z * XY] = flip ? 1.0 : 0.0;
+      }
+}
+
+/* FIELD_3D::invertedDistance (FIELD_3D.cpp:5241-5297): the shell cells get
+ * 1, then sweep after sweep (each reading the previous sweep) every cell
+ * still at or below 1e-5 takes 1 + the smallest of its 6-neighbours above
+ * 1e-5; then 1 / value, negated inside (distance < 0) and quartered outside.
+ * The reference sweeps forever when a cell can never be reached; here that
+ * returns -1. */
+int qo_inverted_distance(const qo_grid* g, const double* dist, double* out) {
+  const int64_t X = g->res[0], Y = g->res[1], Z = g->res[2], XY = X * Y, n = XY * Z;
+  double* cur = (double*)malloc(sizeof(double) * (size_t)n);
+  shells(g, dist, out);
+  const double eps = 1e-5;
+  int status = 0;
+  for (;;) {
+    int64_t zeros = 0, updated = 0;
+    memcpy(cur, out, sizeof(double) * (size_t)n);
+    for (int64_t z = 0; z < Z; z++)
+      for (int64_t y = 0; y < Y; y++)
+        for (int64_t x = 0; x < X; x++) {
+          const int64_t i = x + y * X + z * XY;
+          if (out[i] > eps) continue;
+          zeros++;
+          double nb[6];
+          nb[0] = (x != 0) ? out[i - 1] : 0;
+          nb[1] = (x != X - 1) ? out[i + 1] : 0;
+          nb[2] = (y != 0) ? out[i - X] : 0;
+          nb[3] = (y != Y - 1) ? out[i + X] : 0;
+          nb[4] = (z != 0) ? out[i - XY] : 0;
+          nb[5] = (z != Z - 1) ? out[i + XY] : 0;
+          double biggest = (double)n + 1;
+          for (int k = 0; k < 6; k++)
+            if (nb[k] > 1e-5 && nb[k] < biggest) biggest = nb[k];
+          if (biggest < (double)n) {
+            cur[i] = biggest + 1;
+            updated++;
+          }
+        }
+    memcpy(out, cur, sizeof(double) * (size_t)n);
+    if (zeros == 0) break;
+    if (updated == 0) {
+      status = -1;
+      break;
+    }
+  }
+  free(cur);
+  for (int64_t i = 0; i < n; i++) {
+    out[i] = 1.0 / out[i];
+    if (dist[i] < 0) out[i] *= -1;
+    else out[i] *= 0.25;
+  }
+  return status;
+}
+
+/* FIELD_3D::invertedDistanceNormalized (FIELD_3D.cpp:5316-5352): the field
+ * over the sum of its absolute values, then the inside and the outside each
+ * scaled to sum to 0.5 in absolute value; *abs_sum receives the absolute sum
+ * of the result (OPTIMIZE_3D::read's _inverseFieldSum, OPTIMIZE_3D.cpp:3375-3378).
+ * Serial sums in index order. */
+int qo_inverted_distance_normalized(const qo_grid* g, const double* dist, double* out, double* abs_sum) {
+  const int64_t n = (int64_t)g->res[0] * g->res[1] * g->res[2];
+  const int status = qo_inverted_distance(g, dist, out);
+  double t = 0;
+  for (int64_t i = 0; i < n; i++) t += fabs(out[i]);
+  const double scale = 1.0 / t;
+  for (int64_t i = 0; i < n; i++) out[i] *= scale;
+  double inside = 0, outside = 0;
+  for (int64_t i = 0; i < n; i++) {
+    if (dist[i] < 0) inside += out[i];
+    else outside += out[i];
+  }
+  const double in_n = 0.5 / fabs(inside), out_n = 0.5 / fabs(outside);
+  for (int64_t i = 0; i < n; i++) out[i] *= (dist[i] < 0) ? in_n : out_n;
+  if (abs_sum) {
+    double a = 0;
+    for (int64_t i = 0; i < n; i++) a += fabs(out[i]);
+    *abs_sum = a;
+  }
+  return status;
+}
+
+/* OPTIMIZE_3D::weightScore (OPTIMIZE_3D.cpp:4502-4517) with the threshold of
+ * inverseDistanceScore (:4522-4563; 0 in weightScore):
+ * sum_cells -tanh(alpha (F - threshold)) w(cell centre), and
+ * computeMaximumScaledPowerRationalScore (:4484-4498): sum |w(cell centre)|.
+ * w is the trilinear lookup of the weight field (FIELD_3D.cpp:966-1029). */
+void qo_weight_score(const qo_grid* g, const double* field, const qo_grid* wg, const double* weight, double alpha,
+                     double threshold, double* score, double* max_score) {
+  const int64_t X = g->res[0], Y = g->res[1], Z = g->res[2];
+  double s = 0, m = 0;
+  for (int64_t z = 0; z < Z; z++)
+    for (int64_t y = 0; y < Y; y++)
+      for (int64_t x = 0; x < X; x++) {
+        double c[3];
+        qo_cell_center(g, (int)x, (int)y, (int)z, c);
+        const double w = qo_field_lookup(wg, weight, c);
+        s += -tanh(alpha * (field[x + y * X + z * X * Y] - threshold)) * w;
+        m += fabs(w);
+      }
+  if (score) *score = s;
+  if (max_score) *max_score = m;
 }
 
 /* ---------------- AP score and gradients at one iteration ---------------- */

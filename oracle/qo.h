@@ -35,7 +35,8 @@ enum {
   QO_BLACK_HOLE = 1u,  /* stop when |q| < 10*FLT_MIN (OPTIMIZE_3D.cpp:1358-1359) */
   QO_SKIP_ORIGIN = 2u, /* leave the cell unwritten when |q0| < 1e-8 (OPTIMIZE_3D.cpp:1428) */
   QO_AP_REPASS = 4u,   /* overflow flags + long double re-pass (OPTIMIZE_3D.cpp:1440-1567) */
-  QO_LAGRANGE = 8u     /* RATIONAL_4D::evaluateLagrange order (RATIONAL_4D.cpp:27-88) */
+  QO_LAGRANGE = 8u,    /* RATIONAL_4D::evaluateLagrange order (RATIONAL_4D.cpp:27-88) */
+  QO_EXTENDED = 16u    /* the default build's Real = long double (SETTINGS.h:10) for the whole map */
 };
 
 typedef struct qo_params {
@@ -82,6 +83,12 @@ int qo_marching_cubes(const qo_grid* g, const double* field, int64_t* n_vertices
 int64_t qo_nonlinear_mc(const qo_grid* g, const qo_poly* top, const qo_poly* bottom, const qo_params* p, double iso,
                         int64_t* n_vertices, int64_t* n_triangles, double* vertices, int32_t* triangles);
 
+/* The same with the cases from a given field (TRIANGLE_MESH.cpp:62-79, 152-215);
+ * given == NULL evaluates the map's own field. */
+int64_t qo_nonlinear_mc_field(const qo_grid* g, const double* given, const qo_poly* top, const qo_poly* bottom,
+                              const qo_params* p, double iso, int64_t* n_vertices, int64_t* n_triangles,
+                              double* vertices, int32_t* triangles);
+
 /* FIELD_3D::operator()(VEC3F) trilinear lookup (FIELD_3D.cpp:966-1029). */
 double qo_field_lookup(const qo_grid* g, const double* data, const double* pos3);
 
@@ -91,6 +98,15 @@ double qo_clamp_infs_to_mean(double* data, int64_t n);
 double qo_clamp_nans_to_mean(double* data, int64_t n);
 /* FIELD_3D::setBorders (FIELD_3D.cpp:5357-5382) */
 void qo_set_borders(const qo_grid* g, double* data, double value);
+
+/* FIELD_3D::invertedDistance / invertedDistanceNormalized (FIELD_3D.cpp:5241-5352)
+ * of a signed distance field; 0, or -1 when a cell can never be reached. */
+int qo_inverted_distance(const qo_grid* g, const double* dist, double* out);
+int qo_inverted_distance_normalized(const qo_grid* g, const double* dist, double* out, double* abs_sum);
+/* weightScore / inverseDistanceScore (threshold) and the maximum score
+ * (OPTIMIZE_3D.cpp:4484-4563) */
+void qo_weight_score(const qo_grid* g, const double* field, const qo_grid* wg, const double* weight, double alpha,
+                     double threshold, double* score, double* max_score);
 
 /* Score and analytic gradients at one iteration (AP semantics).
  * weight grid/data: the weight field sampled at each fractal cell centre. */

@@ -17,6 +17,7 @@ MAP_BLACK_HOLE = 1
 MAP_SKIP_ORIGIN = 2
 MAP_ADAPTIVE = 4
 MAP_LAGRANGE = 8
+MAP_EXTENDED_RANGE = 16
 
 c_double_p = ctypes.POINTER(ctypes.c_double)
 
@@ -59,7 +60,9 @@ class ServerConfig(ctypes.Structure):
     _fields_ = [("hbm_budget_bytes", ctypes.c_int64), ("chunk_voxels", ctypes.c_int32),
                 ("chunks_per_launch", ctypes.c_int32), ("max_roots", ctypes.c_int32),
                 ("inflight_launches", ctypes.c_int32), ("precision", ctypes.c_int32), ("band", ctypes.c_double),
-                ("device", ctypes.c_int32), ("launch_work", ctypes.c_double)]
+                ("device", ctypes.c_int32), ("launch_work", ctypes.c_double),
+                ("max_pending_work", ctypes.c_double), ("aging_s", ctypes.c_double),
+                ("check_samples", ctypes.c_int32)]
 
 
 class Request(ctypes.Structure):
@@ -70,7 +73,9 @@ class Request(ctypes.Structure):
 class Completion(ctypes.Structure):
     _fields_ = [("id", ctypes.c_uint64), ("status", ctypes.c_int32), ("arrival_time", ctypes.c_double),
                 ("admit_time", ctypes.c_double), ("done_time", ctypes.c_double), ("d_field", ctypes.c_void_p),
-                ("voxels", ctypes.c_int64)]
+                ("voxels", ctypes.c_int64), ("checked", ctypes.c_int64), ("check_sign_flips", ctypes.c_int64),
+                ("check_class_mismatch", ctypes.c_int64), ("check_max_abs_dF", ctypes.c_double),
+                ("check_max_rel_dF", ctypes.c_double)]
 
 
 # every symbol declared in include/qj_c_api.h: (name, restype, argtypes)
@@ -100,9 +105,13 @@ SIGNATURES = [
                                         ctypes.c_double, _P, _P]),
     ("qj_mc_run", ctypes.c_int32, [_P, _P, ctypes.POINTER(Grid), ctypes.c_double, ctypes.POINTER(ctypes.c_int64),
                                    ctypes.POINTER(ctypes.c_int64), _P]),
+    ("qj_mc_run_tiles", ctypes.c_int32, [_P, ctypes.c_int32, _P, ctypes.POINTER(Grid), ctypes.c_double,
+                                         ctypes.POINTER(ctypes.c_int64), ctypes.POINTER(ctypes.c_int64), _P]),
     ("qj_nlmc_run", ctypes.c_int32, [_P, ctypes.POINTER(Grid), _P, ctypes.POINTER(MapParams), ctypes.c_double,
                                      ctypes.c_int32, ctypes.c_double, ctypes.c_int32, ctypes.POINTER(ctypes.c_int64),
                                      ctypes.POINTER(ctypes.c_int64), _P]),
+    ("qj_nlmc_field", ctypes.c_int32, [_P, _P, ctypes.POINTER(Grid), _P, ctypes.POINTER(MapParams), ctypes.c_double,
+                                       ctypes.POINTER(ctypes.c_int64), ctypes.POINTER(ctypes.c_int64), _P]),
     ("qj_nlmc_cache", ctypes.c_int32, [_P, ctypes.POINTER(ctypes.c_int64), _P, ctypes.POINTER(ctypes.c_int64), _P,
                                        _P]),
     ("qj_nlmc_resume", ctypes.c_int32, [_P, ctypes.POINTER(Grid), _P, ctypes.POINTER(MapParams), ctypes.c_double,

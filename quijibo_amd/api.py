@@ -265,6 +265,33 @@ def marching_cubes(field, grid, iso=0.0, workspace=None, stream=None, device_out
     return v, t
 
 
+def marching_cubes_tiles(fields, grids, iso=0.0, workspace=None, stream=None):
+    """qj_mc_run_tiles: marching cubes of several fields with one read-back of
+    the counts. Returns (vertices, triangles, counts) with the meshes one after
+    another (triangle indices local to their field) as device tensors and the
+    per-field (vertices, triangles) counts."""
+    n = len(fields)
+    if len(grids) != n:
+        raise ValueError("one grid per field")
+    for k, (f, g) in enumerate(zip(fields, grids)):
+        _device_tensor(f, "fields[%d]" % k, _cells(g))
+    ws = workspace or default_workspace()
+    ptrs = (ctypes.c_void_p * max(n, 1))(*[f.data_ptr() for f in fields])
+    garr = (_lib.Grid * max(n, 1))(*grids)
+    nv = (ctypes.c_int64 * max(n, 1))()
+    nt = (ctypes.c_int64 * max(n, 1))()
+    L = _lib.load()
+    check(L.qj_mc_run_tiles(ws.handle, n, ctypes.cast(ptrs, ctypes.c_void_p), garr, float(iso), nv, nt,
+                            _stream_ptr(stream)))
+    counts = [(nv[k], nt[k]) for k in range(n)]
+    tv, tt = sum(c[0] for c in counts), sum(c[1] for c in counts)
+    dev = fields[0].device if n else torch.device("cuda")
+    v = torch.empty((tv, 3), dtype=torch.float64, device=dev)
+    t = torch.empty((tt, 3), dtype=torch.int32, device=dev)
+    check(L.qj_mc_copy(ws.handle, _ptr(v) if tv else None, _ptr(t) if tt else None, _stream_ptr(stream)))
+    return v, t, counts
+
+
 class OPTIMIZE_3D:
     """The map-evaluation members of OPTIMIZE_3D (OPTIMIZE_3D.h:67-81, 140-169)."""
 
@@ -382,6 +409,25 @@ def nonlinear_mesh(grid, rational, params, iso=0.0, precision=FP32_BAND, band=DE
     st = L.qj_nlmc_run(ws.handle, ctypes.byref(grid), rational.handle, ctypes.byref(params), float(iso),
                        int(precision), float(band), int(chunk_slices), ctypes.byref(nv), ctypes.byref(nt),
                        _stream_ptr(stream))
+    if st != 0 and not (allow_inconsistent and st == 5):
+        check(st)
+    v = np.empty((nv.value, 3), dtype=np.float64)
+    t = np.empty((nt.value, 3), dtype=np.int32)
+    check(L.qj_mc_copy(ws.handle, v.ctypes.data_as(ctypes.c_void_p) if nv.value else None,
+                       t.ctypes.data_as(ctypes.c_void_p) if nt.value else None, _stream_ptr(stream)))
+    return v, t
+
+
+def nonlinear_mesh_field(field, grid, rational, params, iso=0.0, workspace=None, stream=None, allow_inconsistent=False):
+    """qj_nlmc_field: the non-linear mesher on a given field (cases from its
+    signs, vertices by bisection on the map). Returns (vertices, triangles)."""
+    _device_tensor(field, "field", _cells(grid))
+    ws = workspace or default_workspace()
+    nv = ctypes.c_int64()
+    nt = ctypes.c_int64()
+    L = _lib.load()
+    st = L.qj_nlmc_field(ws.handle, _ptr(field), ctypes.byref(grid), rational.handle, ctypes.byref(params),
+                         float(iso), ctypes.byref(nv), ctypes.byref(nt), _stream_ptr(stream))
     if st != 0 and not (allow_inconsistent and st == 5):
         check(st)
     v = np.empty((nv.value, 3), dtype=np.float64)

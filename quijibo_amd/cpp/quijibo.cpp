@@ -543,6 +543,26 @@ TRIANGLE_MESH::TRIANGLE_MESH(const VEC3F& center, const VEC3F& lengths, const VE
   fetch_mesh(ws, nv, nt, _vertices, _triangles);
 }
 
+TRIANGLE_MESH::TRIANGLE_MESH(const FIELD_3D& field, const POLYNOMIAL_4D& top, const POLYNOMIAL_4D& bottom,
+                             Real expScaling, int maxIterations, Real slice, Real isosurface) {
+  qj_workspace* ws = shared_workspace();
+  DeviceBuffer d(sizeof(double) * field.totalCells());
+  if (field.totalCells())
+    hip_check(hipMemcpy(d.p, field.dataConst(), sizeof(double) * field.totalCells(), hipMemcpyHostToDevice),
+              "hipMemcpy field");
+  const qj_grid g = field.grid();
+  const qj_poly t = top.view(), b = bottom.view();
+  qj_rational* f = nullptr;
+  check(qj_rational_create(&t, &b, &f), "qj_rational_create");
+  // _escapeRadius = 2000 (TRIANGLE_MESH.cpp:76); the black-hole stop of nonlinearValue
+  const qj_map_params p{maxIterations, 2000.0, expScaling, slice, QJ_MAP_BLACK_HOLE};
+  int64_t nv = 0, nt = 0;
+  const qj_status st = qj_nlmc_field(ws, (const double*)d.p, &g, f, &p, isosurface, &nv, &nt, nullptr);
+  qj_rational_destroy(f);
+  check(st, "non-linear mesher on a field");
+  fetch_mesh(ws, nv, nt, _vertices, _triangles);
+}
+
 TRIANGLE_MESH::TRIANGLE_MESH(const VEC3F& center, const VEC3F& lengths, const VEC3I& res, const POLYNOMIAL_4D& top,
                              const POLYNOMIAL_4D& bottom, Real expScaling, int maxIterations, Real slice,
                              Real isosurface, const QUATERNION& rotation) {

@@ -98,6 +98,10 @@ qj_status make_map_args(const qj_grid* g, const qj_rational* f, const qj_map_par
     set_error("QJ_MAP_LAGRANGE does not combine with QJ_MAP_SKIP_ORIGIN or QJ_MAP_ADAPTIVE");
     return QJ_ERR_INVALID;
   }
+  if ((p->flags & QJ_MAP_EXTENDED_RANGE) && (p->flags & (QJ_MAP_LAGRANGE | QJ_MAP_ADAPTIVE))) {
+    set_error("QJ_MAP_EXTENDED_RANGE does not combine with QJ_MAP_LAGRANGE or QJ_MAP_ADAPTIVE");
+    return QJ_ERR_INVALID;
+  }
   a->max_iter = p->max_iter;
   a->zoff = 0;
   a->gen = 0;
@@ -225,6 +229,7 @@ const char* qj_status_string(qj_status s) {
     case QJ_ERR_CAPACITY: return "capacity exceeded";
     case QJ_ERR_NUMERIC: return "inconsistent numeric state";
     case QJ_ERR_SHUTDOWN: return "server shutting down";
+    case QJ_ERR_BUSY: return "refused: pending work over the limit";
     default: return "unknown status";
   }
 }
@@ -476,6 +481,21 @@ qj_status qj_mc_run(qj_workspace* ws, const double* d_field, const qj_grid* grid
   return mc_run(ws, d_field, grid, iso, n_vertices, n_triangles, (hipStream_t)stream);
 }
 
+qj_status qj_mc_run_tiles(qj_workspace* ws, int32_t n, const double* const* d_fields, const qj_grid* grids,
+                          double iso, int64_t* n_vertices, int64_t* n_triangles, void* stream) {
+  if (!ws || n < 0 || (n > 0 && (!d_fields || !grids || !n_vertices || !n_triangles))) {
+    set_error("qj_mc_run_tiles: null argument or negative count");
+    return QJ_ERR_INVALID;
+  }
+  for (int k = 0; k < n; k++)
+    if (!d_fields[k]) {
+      set_error("qj_mc_run_tiles: field %d is null", k);
+      return QJ_ERR_INVALID;
+    }
+  std::lock_guard<std::mutex> lock(ws->mu);
+  return mc_run_tiles(ws, n, d_fields, grids, iso, n_vertices, n_triangles, (hipStream_t)stream);
+}
+
 qj_status qj_nlmc_run(qj_workspace* ws, const qj_grid* grid, const qj_rational* f, const qj_map_params* params,
                       double iso, qj_precision precision, double band, int32_t chunk_slices, int64_t* n_vertices,
                       int64_t* n_triangles, void* stream) {
@@ -498,6 +518,26 @@ qj_status qj_nlmc_run(qj_workspace* ws, const qj_grid* grid, const qj_rational* 
   int64_t bad = 0;
   return nlmc_run(ws, a, f, iso, precision, band, chunk_slices, n_vertices, n_triangles, &bad,
                   (hipStream_t)stream);
+}
+
+qj_status qj_nlmc_field(qj_workspace* ws, const double* d_field, const qj_grid* grid, const qj_rational* f,
+                        const qj_map_params* params, double iso, int64_t* n_vertices, int64_t* n_triangles,
+                        void* stream) {
+  if (!ws || !d_field || !grid || !f || !params || !n_vertices || !n_triangles) {
+    set_error("qj_nlmc_field: null argument");
+    return QJ_ERR_INVALID;
+  }
+  if (params->flags & (QJ_MAP_SKIP_ORIGIN | QJ_MAP_ADAPTIVE | QJ_MAP_LAGRANGE)) {
+    set_error("qj_nlmc_field: the mesher's map has no origin skip, adaptive re-pass or Lagrange order");
+    return QJ_ERR_INVALID;
+  }
+  MapArgs a;
+  qj_status st = make_map_args(grid, f, params, &a);
+  if (st) return st;
+  a.flags |= QJ_MAP_BLACK_HOLE;  // nonlinearValue's black-hole stop (TRIANGLE_MESH.cpp:1789-1791)
+  std::lock_guard<std::mutex> lock(ws->mu);
+  int64_t bad = 0;
+  return nlmc_run(ws, a, f, iso, QJ_FP64, 0.0, 64, n_vertices, n_triangles, &bad, (hipStream_t)stream, d_field);
 }
 
 qj_status qj_mc_copy(qj_workspace* ws, double* vertices, int32_t* triangles, void* stream) {
@@ -586,6 +626,11 @@ qj_status qj_ap_score_gradients(qj_workspace* ws, const qj_grid* grid, const qj_
     set_error("qj_ap_score_gradients: the AP gradients are defined for max_iter == 1 (toothOptimizer.cpp:402)");
     return QJ_ERR_INVALID;
   }
+  if (params->flags & QJ_MAP_EXTENDED_RANGE) {
+    set_error("qj_ap_score_gradients: the AP pass has its own extended-range re-pass; QJ_MAP_EXTENDED_RANGE is "
+              "not supported");
+    return QJ_ERR_INVALID;
+  }
   if (params->flags & QJ_MAP_LAGRANGE) {
     set_error("qj_ap_score_gradients: the AP gradients are of the factored order; QJ_MAP_LAGRANGE is not supported");
     return QJ_ERR_INVALID;
@@ -608,6 +653,10 @@ qj_status qj_points_adjoint(qj_workspace* ws, const double* d_points, int64_t n,
     return QJ_ERR_INVALID;
   }
   if (precision != QJ_FP64 && precision != QJ_FP32_BAND) return QJ_ERR_INVALID;
+  if (params && (params->flags & QJ_MAP_EXTENDED_RANGE)) {
+    set_error("qj_points_adjoint: the adjoint runs in FP64 / FP32 range; QJ_MAP_EXTENDED_RANGE is not supported");
+    return QJ_ERR_INVALID;
+  }
   if (params && (params->flags & QJ_MAP_LAGRANGE)) {
     set_error("qj_points_adjoint: the adjoint is of the factored order; QJ_MAP_LAGRANGE is not supported");
     return QJ_ERR_INVALID;

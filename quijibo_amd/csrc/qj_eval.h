@@ -242,4 +242,138 @@ __device__ __forceinline__ Res64 eval64(const MapArgs& a, RP roots, q64 q) {
 }
 
 
+// ---------------------------------------------------------------- extended range
+
+// A quaternion times 2^e. Normalised so the largest |component| is in [0.5, 1).
+struct xq {
+  q64 m;
+  int e;
+};
+__device__ __forceinline__ void xnorm(xq& v) {
+  const double mx = fmax(fmax(fabs(v.m.w), fabs(v.m.x)), fmax(fabs(v.m.y), fabs(v.m.z)));
+  if (mx > 0.0 && isfinite(mx)) {
+    int k;
+    (void)frexp(mx, &k);
+    v.m.w = ldexp(v.m.w, -k);
+    v.m.x = ldexp(v.m.x, -k);
+    v.m.y = ldexp(v.m.y, -k);
+    v.m.z = ldexp(v.m.z, -k);
+    v.e += k;
+  }
+}
+// QUATERNION::pow (QUATERNION.cpp:375-393) with exps = exp(e log|q| + lnoff)
+// kept as mantissa * 2^k; lnoff carries the scale of a normalised base
+// (e k ln 2 for a base m 2^k: the angle does not depend on the scale).
+__device__ __forceinline__ xq xpow(const q64& q, double e, double lnoff = 0.0) {
+  const double partial = q.x * q.x + q.y * q.y + q.z * q.z;
+  const double qMagnitude = sqrt(partial + q.w * q.w);
+  const double vMagnitude = sqrt(partial);
+  const double vMagnitudeInv = (vMagnitude > 0.0) ? 1.0 / vMagnitude : 0.0;
+  const double scale = e * acos(q.w / qMagnitude) * vMagnitudeInv;
+  const double magnitude = scale * vMagnitude;
+  const double magnitudeInv = (magnitude > 0.0) ? 1.0 / magnitude : 0.0;
+  const double lnE = e * log(qMagnitude) + lnoff;
+  int k = 0;
+  double em;
+  if (isfinite(lnE)) {
+    k = (int)floor(lnE * 1.4426950408889634);
+    em = exp(lnE - k * 0.6931471805599453);
+  } else {
+    em = exp(lnE);
+  }
+  double s, c;
+  sincos(magnitude, &s, &c);
+  const double scale2 = scale * em * magnitudeInv * s;
+  xq r{{em * c, scale2 * q.x, scale2 * q.y, scale2 * q.z}, k};
+  xnorm(r);
+  return r;
+}
+
+// (q - r)^e with a separate binary exponent, for QJ_MAP_EXTENDED_RANGE: the
+// base is normalised first, so neither its square nor the polar magnitude
+// leaves FP64's range; exponents 1 and 2 (kPow != kPolar) as the exact
+// products with the polar form's special cases (b = 0: NaN; v = 0:
+// (|w|^e, +0 v)), decided on the normalised base as long double would.
+template <int kPow>
+__device__ __forceinline__ xq term_ext(const RootRec64& r, const q64& q) {
+  xq n{qsub64x(q, r), 0};
+  xnorm(n);
+  if (kPow == kPolar || !(kPow == kExactOnly || is_one(r.e) || is_two(r.e)))
+    return xpow(n.m, r.e, r.e * (double)n.e * 0.6931471805599453);
+  const double partial = partial64(n.m);
+  if (!(partial + n.m.w * n.m.w > 0.0)) {
+    const double nan = __builtin_nan("");
+    return xq{{nan, nan, nan, nan}, 0};
+  }
+  if (!(partial > 0.0)) {
+    xq v = is_one(r.e) ? xq{{fabs(n.m.w), 0.0 * n.m.x, 0.0 * n.m.y, 0.0 * n.m.z}, n.e}
+                       : xq{{n.m.w * n.m.w, 0.0 * n.m.x, 0.0 * n.m.y, 0.0 * n.m.z}, 2 * n.e};
+    xnorm(v);
+    return v;
+  }
+  if (is_one(r.e)) return n;
+  xq v{square64(n.m, partial), 2 * n.e};
+  xnorm(v);
+  return v;
+}
+
+template <int kPow, class RP>
+__device__ __forceinline__ xq poly_ext(RP r, int n, const q64& q) {
+  xq res = term_ext<kPow>(load_root64(r), q);
+  for (int i = 1; i < n; i++) {
+    const xq t = term_ext<kPow>(load_root64(r + i), q);
+    res.m = qmul64x(res.m, t.m);
+    res.e += t.e;
+    xnorm(res);
+  }
+  return res;
+}
+
+// One voxel with the range of the reference's default build (Real = long
+// double, SETTINGS.h:10; QJ_MAP_EXTENDED_RANGE): the factored products, the
+// quotient and |q| are FP64 mantissas with separate binary exponents, so
+// nothing overflows or underflows where x87 long double does not. The
+// escape and black-hole tests compare the FP64 value of |q| (exact whenever
+// it is representable; +inf or 0 beyond, which decides them the same way);
+// F = log|mantissa| + exponent ln 2. An iterate that continues below FP64's
+// range (no black-hole stop) continues from its FP64 value.
+template <int kPow = kPolar, class RP = const RootRec64*>
+__device__ __forceinline__ Res64 eval64e(const MapArgs& a, RP roots, q64 q) {
+  Res64 r{0.0, 0, true, false};
+  double magnitude = qmag(q);
+  if ((a.flags & QJ_MAP_SKIP_ORIGIN) && magnitude < 1e-8) {
+    r.written = false;
+    return r;
+  }
+  double logmag = log(magnitude);
+  int it = 0;
+  while (magnitude < a.escape && it < a.max_iter) {
+    xq v = poly_ext<kPow>(roots, a.nt, q);
+    if (a.nb > 0) {
+      const xq B = poly_ext<kPow>(roots + a.nt, a.nb, q);
+      v.m = qdiv(v.m, B.m);
+      v.e -= B.e;
+    }
+    v.m = qscale(v.m, a.exp_scaling);
+    xnorm(v);
+    const double mm = qmag(v.m);
+    logmag = log(mm) + (double)v.e * 0.6931471805599453;
+    magnitude = ldexp(mm, v.e);
+    q = q64{ldexp(v.m.w, v.e), ldexp(v.m.x, v.e), ldexp(v.m.y, v.e), ldexp(v.m.z, v.e)};
+    it++;
+    r.iters++;
+    if ((a.flags & QJ_MAP_BLACK_HOLE) && magnitude < kBlackHole) it = a.max_iter;
+  }
+  r.F = logmag;
+  return r;
+}
+
+// eval64 or, with QJ_MAP_EXTENDED_RANGE, eval64e (a uniform branch where a
+// wave's voxels share their map arguments)
+template <int kPow = kPolar, class RP = const RootRec64*>
+__device__ __forceinline__ Res64 eval64_any(const MapArgs& a, RP roots, const q64& q) {
+  if (a.flags & QJ_MAP_EXTENDED_RANGE) return eval64e<kPow>(a, roots, q);
+  return eval64<kPow>(a, roots, q);
+}
+
 }  // namespace qj

@@ -55,50 +55,8 @@ struct PointSource {
   }
 };
 
-// ---------------------------------------------------------------- FP64 extended range
+// ---------------------------------------------------------------- FP64 extended range (AP re-pass)
 
-// A quaternion times 2^e. Normalised so the largest |component| is in [0.5, 1).
-struct xq {
-  q64 m;
-  int e;
-};
-__device__ __forceinline__ void xnorm(xq& v) {
-  const double mx = fmax(fmax(fabs(v.m.w), fabs(v.m.x)), fmax(fabs(v.m.y), fabs(v.m.z)));
-  if (mx > 0.0 && isfinite(mx)) {
-    int k;
-    (void)frexp(mx, &k);
-    v.m.w = ldexp(v.m.w, -k);
-    v.m.x = ldexp(v.m.x, -k);
-    v.m.y = ldexp(v.m.y, -k);
-    v.m.z = ldexp(v.m.z, -k);
-    v.e += k;
-  }
-}
-// QUATERNION::pow with exps = exp(e log|q|) kept as mantissa * 2^k.
-__device__ __forceinline__ xq xpow(const q64& q, double e) {
-  const double partial = q.x * q.x + q.y * q.y + q.z * q.z;
-  const double qMagnitude = sqrt(partial + q.w * q.w);
-  const double vMagnitude = sqrt(partial);
-  const double vMagnitudeInv = (vMagnitude > 0.0) ? 1.0 / vMagnitude : 0.0;
-  const double scale = e * acos(q.w / qMagnitude) * vMagnitudeInv;
-  const double magnitude = scale * vMagnitude;
-  const double magnitudeInv = (magnitude > 0.0) ? 1.0 / magnitude : 0.0;
-  const double lnE = e * log(qMagnitude);
-  int k = 0;
-  double em;
-  if (isfinite(lnE)) {
-    k = (int)floor(lnE * 1.4426950408889634);
-    em = exp(lnE - k * 0.6931471805599453);
-  } else {
-    em = exp(lnE);
-  }
-  double s, c;
-  sincos(magnitude, &s, &c);
-  const double scale2 = scale * em * magnitudeInv * s;
-  xq r{{em * c, scale2 * q.x, scale2 * q.y, scale2 * q.z}, k};
-  xnorm(r);
-  return r;
-}
 __device__ __forceinline__ xq xpoly(const RootRec64* __restrict__ r, int n, const q64& q) {
   xq res = xpow(qsub(q, q64{r[0].w, r[0].x, r[0].y, r[0].z}), r[0].e);
   for (int i = 1; i < n; i++) {
@@ -150,8 +108,9 @@ __device__ __forceinline__ Res64 eval64x(const MapArgs& a, const RootRec64* __re
 // kLagrange: the map may be in Lagrange order (QJ_MAP_LAGRANGE); the
 // factored-order instance carries no Lagrange code and needs fewer registers.
 // kPow: kPolar is the reference method (QJ_FP64); kExactOnly / kExactInt take
-// exponents 1 and 2 as exact products (QJ_FP64_EXACT).
-template <bool kLagrange, int kPow = kPolar>
+// exponents 1 and 2 as exact products (QJ_FP64_EXACT). kExt: the
+// extended-range evaluator (QJ_MAP_EXTENDED_RANGE).
+template <bool kLagrange, int kPow = kPolar, bool kExt = false>
 __global__ __launch_bounds__(kBlock) void k_grid64(MapArgs a, const RootRec64* __restrict__ roots,
                                                    double* __restrict__ out, unsigned long long* __restrict__ cnt,
                                                    uint32_t* __restrict__ alist, uint32_t* __restrict__ acount) {
@@ -163,7 +122,8 @@ __global__ __launch_bounds__(kBlock) void k_grid64(MapArgs a, const RootRec64* _
   if (valid) {
     double c[3];
     cell_center(a, x, y, z + a.zoff, c);
-    const Res64 r = eval64<kPow, const RootRec64*, kLagrange>(a, roots, q64{c[0], c[1], c[2], a.slice});
+    const q64 q0{c[0], c[1], c[2], a.slice};
+    const Res64 r = kExt ? eval64e<kPow>(a, roots, q0) : eval64<kPow, const RootRec64*, kLagrange>(a, roots, q0);
     if (r.written) {
       out[idx] = r.F;
       nonfinite = !isfinite(r.F);
@@ -236,7 +196,7 @@ __global__ __launch_bounds__(kBlock) void k_list64(MapArgs a, const RootRec64* _
     if (i < n) {
       idx = list ? list[i] : (uint32_t)i;
       const q64 q0 = Src::start(a, pts, idx);
-      const Res64 r = kExtended ? eval64x(a, roots, q0) : eval64<kPow>(a, roots, q0);
+      const Res64 r = kExtended ? eval64x(a, roots, q0) : eval64_any<kPow>(a, roots, q0);
       if (r.written) out[idx] = r.F;
       else if (pts) out[idx] = __builtin_nan("");
       vox++;
@@ -253,6 +213,7 @@ __global__ __launch_bounds__(kBlock) void k_list64(MapArgs a, const RootRec64* _
 
 // Block maximum of a non-negative double into counter slot `slot` (the bits
 // of non-negative doubles order like unsigned integers).
+template <int kSpread = kCounterSpread>
 __device__ __forceinline__ void block_max(double v, unsigned long long* cnt, int slot) {
   __shared__ unsigned long long s_part[kBlock / 64];
   unsigned long long b = __builtin_bit_cast(unsigned long long, v > 0.0 ? v : 0.0);
@@ -266,7 +227,7 @@ __device__ __forceinline__ void block_max(double v, unsigned long long* cnt, int
   if (threadIdx.x == 0) {
     unsigned long long m = 0;
     for (int i = 0; i < kBlock / 64; i++) m = s_part[i] > m ? s_part[i] : m;
-    if (m) atomicMax(&cnt[slot * kCounterSpread + (blockIdx.x % kCounterSpread)], m);
+    if (m) atomicMax(&cnt[slot * kSpread + (blockIdx.x % kSpread)], m);
   }
   __syncthreads();
 }
@@ -278,6 +239,9 @@ __device__ __forceinline__ void block_max(double v, unsigned long long* cnt, int
 // sign disagreements of (float)F > 0 (the mesher's predicate) and class
 // disagreements (finite / NaN / +-inf), and the largest absolute and relative
 // (to max(1, |F64|)) differences over cells finite in both.
+// kSpread: copies per counter slot (a workspace's counters), or 1 for the
+// request loop's per-result block of CNT_N counters.
+template <int kSpread = kCounterSpread>
 __global__ __launch_bounds__(kBlock) void k_check64(MapArgs a, const RootRec64* __restrict__ roots,
                                                     const double* __restrict__ field, int64_t nvox, int64_t samples,
                                                     unsigned long long* __restrict__ cnt) {
@@ -287,7 +251,8 @@ __global__ __launch_bounds__(kBlock) void k_check64(MapArgs a, const RootRec64* 
   if (i < samples) {
     const int64_t idx = ((2 * i + 1) * nvox) / (2 * samples);
     const q64 q0 = GridSource::start(a, nullptr, idx);
-    Res64 r = eval64<kPolar, const RootRec64*, true>(a, roots, q0);
+    Res64 r = (a.flags & QJ_MAP_EXTENDED_RANGE) ? eval64e<kPolar>(a, roots, q0)
+                                                : eval64<kPolar, const RootRec64*, true>(a, roots, q0);
     if ((a.flags & QJ_MAP_ADAPTIVE) && r.overflow) r = eval64x(a, roots, q0);
     if (r.written) {
       const double F = field[idx];
@@ -302,11 +267,28 @@ __global__ __launch_bounds__(kBlock) void k_check64(MapArgs a, const RootRec64* 
       flips = (((float)F > 0.0f) != ((float)r.F > 0.0f)) ? 1 : 0;
     }
   }
-  block_count(checked, cnt, CNT_CHECKED);
-  block_count(flips, cnt, CNT_CHECK_FLIPS);
-  block_count(cls, cnt, CNT_CHECK_CLASS);
-  block_max(dabs, cnt, CNT_CHECK_MAXABS);
-  block_max(drel, cnt, CNT_CHECK_MAXREL);
+  block_count<kSpread>(checked, cnt, CNT_CHECKED);
+  block_count<kSpread>(flips, cnt, CNT_CHECK_FLIPS);
+  block_count<kSpread>(cls, cnt, CNT_CHECK_CLASS);
+  block_max<kSpread>(dabs, cnt, CNT_CHECK_MAXABS);
+  block_max<kSpread>(drel, cnt, CNT_CHECK_MAXREL);
+}
+
+__global__ void k_zero_u64(unsigned long long* p, int n) {
+  if ((int)threadIdx.x < n) p[threadIdx.x] = 0ull;
+}
+
+qj_status launch_check_one(const MapArgs& a, const RootRec64* roots, const double* field, int64_t samples,
+                           unsigned long long* d_out, hipStream_t s) {
+  const int64_t nvox = (int64_t)a.res[0] * a.res[1] * a.res[2];
+  const int64_t m = samples < nvox ? samples : nvox;
+  hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(64), 0, s, d_out, (int)CNT_N);
+  QJ_HIP(hipGetLastError());
+  if (m <= 0) return QJ_OK;
+  hipLaunchKernelGGL(k_check64<1>, dim3((unsigned)((m + kBlock - 1) / kBlock)), dim3(kBlock), 0, s, a, roots, field,
+                     nvox, m, d_out);
+  QJ_HIP(hipGetLastError());
+  return QJ_OK;
 }
 
 // ---------------------------------------------------------------- batched chunks (request loop)
@@ -431,12 +413,12 @@ __global__ __launch_bounds__(kBlock) void k_batch64_list(const ChunkDesc* __rest
     if (__ballot(dl.r64 != dw.r64) == 0ull) {
       const int64_t idx = dl.start + local;
       typedef const __attribute__((address_space(4))) RootRec64 cR64;
-      const Res64 r = eval64<kPow>(dw.a, (cR64*)dw.r64, chunk_start(dw.a, idx));
+      const Res64 r = eval64_any<kPow>(dw.a, (cR64*)dw.r64, chunk_start(dw.a, idx));
       if (r.written) dw.out[idx] = r.F;
     } else {
       const ChunkDesc& d = descs[c];
       const int64_t idx = d.start + local;
-      const Res64 r = eval64<kPow>(d.a, d.r64, chunk_start(d.a, idx));
+      const Res64 r = eval64_any<kPow>(d.a, d.r64, chunk_start(d.a, idx));
       if (r.written) d.out[idx] = r.F;
     }
   }
@@ -452,10 +434,34 @@ __global__ __launch_bounds__(kBlock) void k_batch64(const ChunkDesc* __restrict_
     const ChunkDesc& d = descs[c];
     if (local < d.count) {
       const int64_t idx = d.start + local;
-      const Res64 r = eval64(d.a, d.r64, chunk_start(d.a, idx));
+      const Res64 r = eval64_any(d.a, d.r64, chunk_start(d.a, idx));
       if (r.written) d.out[idx] = r.F;
     }
   }
+}
+
+// The launch's descriptors from pinned host memory (coherent, read over the
+// bus) into device memory: the first node of every request-loop graph. Only
+// the descriptors in use (the first descriptor's total) are copied.
+__global__ __launch_bounds__(kBlock) void k_desc_fetch(const ChunkDesc* __restrict__ h_desc,
+                                                       ChunkDesc* __restrict__ d_desc, int max_chunks) {
+  static_assert(sizeof(ChunkDesc) % 8 == 0, "descriptors copy in 8-byte words");
+  const int total = h_desc[0].total;
+  const int n = total < max_chunks ? (total > 1 ? total : 1) : max_chunks;
+  const int64_t words = (int64_t)n * (int64_t)(sizeof(ChunkDesc) / 8);
+  const unsigned long long* src = (const unsigned long long*)h_desc;
+  unsigned long long* dst = (unsigned long long*)d_desc;
+  for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < words; i += (int64_t)gridDim.x * kBlock)
+    dst[i] = src[i];
+}
+
+qj_status launch_desc_fetch(const ChunkDesc* h_desc, ChunkDesc* d_desc, int max_chunks, hipStream_t s) {
+  const int64_t words = (int64_t)max_chunks * (int64_t)(sizeof(ChunkDesc) / 8);
+  const int64_t blocks = (words + kBlock - 1) / kBlock;
+  hipLaunchKernelGGL(k_desc_fetch, dim3((unsigned)(blocks < 1 ? 1 : blocks)), dim3(kBlock), 0, s, h_desc, d_desc,
+                     max_chunks);
+  QJ_HIP(hipGetLastError());
+  return QJ_OK;
 }
 
 static int resident_blocks(const void* kernel) {
@@ -587,13 +593,27 @@ static qj_status launch_field_bufs(const FieldBufs& fb, const MapArgs& a, const 
   const int64_t nblocks = ntx * nty * ntz;
   if (nblocks > INT32_MAX) return QJ_ERR_CAPACITY;
   mark(fb, s);
+  const bool ext = (a.flags & QJ_MAP_EXTENDED_RANGE) != 0;
+  const dim3 g64((unsigned)nblocks), b64(kBlock);
   if (prec == QJ_FP64_EXACT) {
-    if (a.gen)
-      hipLaunchKernelGGL((k_grid64<false, kExactInt>), dim3((unsigned)nblocks), dim3(kBlock), 0, s, a, f->d64, d_out,
-                         fb.counters, fb.alist, fb.list_count + 1);
+    if (a.gen && ext)
+      hipLaunchKernelGGL((k_grid64<false, kExactInt, true>), g64, b64, 0, s, a, f->d64, d_out, fb.counters, fb.alist,
+                         fb.list_count + 1);
+    else if (a.gen)
+      hipLaunchKernelGGL((k_grid64<false, kExactInt>), g64, b64, 0, s, a, f->d64, d_out, fb.counters, fb.alist,
+                         fb.list_count + 1);
+    else if (ext)
+      hipLaunchKernelGGL((k_grid64<false, kExactOnly, true>), g64, b64, 0, s, a, f->d64, d_out, fb.counters,
+                         fb.alist, fb.list_count + 1);
     else
-      hipLaunchKernelGGL((k_grid64<false, kExactOnly>), dim3((unsigned)nblocks), dim3(kBlock), 0, s, a, f->d64,
-                         d_out, fb.counters, fb.alist, fb.list_count + 1);
+      hipLaunchKernelGGL((k_grid64<false, kExactOnly>), g64, b64, 0, s, a, f->d64, d_out, fb.counters, fb.alist,
+                         fb.list_count + 1);
+    QJ_HIP(hipGetLastError());
+    mark(fb, s);
+    mark(fb, s);
+  } else if (prec == QJ_FP64 && ext) {
+    hipLaunchKernelGGL((k_grid64<false, kPolar, true>), g64, b64, 0, s, a, f->d64, d_out, fb.counters, fb.alist,
+                       fb.list_count + 1);
     QJ_HIP(hipGetLastError());
     mark(fb, s);
     mark(fb, s);
@@ -652,7 +672,7 @@ qj_status launch_field(qj_workspace* ws, const MapArgs& a, const qj_rational* f,
   if ((st = launch_field_bufs(ws_bufs(ws), a, f, prec, band, d_out, s))) return st;
   if (ws->check_samples > 0) {
     const int64_t m = ws->check_samples < nvox ? ws->check_samples : nvox;
-    hipLaunchKernelGGL(k_check64, dim3((unsigned)((m + kBlock - 1) / kBlock)), dim3(kBlock), 0, s, a, f->d64,
+    hipLaunchKernelGGL(k_check64<>, dim3((unsigned)((m + kBlock - 1) / kBlock)), dim3(kBlock), 0, s, a, f->d64,
                        (const double*)d_out, nvox, m, ws->d_counters);
     QJ_HIP(hipGetLastError());
   }

@@ -17,7 +17,9 @@ __device__ __forceinline__ unsigned long long wave_sum_u64(unsigned long long v)
   return v;
 }
 
-// Sum v over the block and add it to counter slot `slot` (spread copies).
+// Sum v over the block and add it to counter slot `slot` (kSpread copies per
+// slot, one chosen by the block index, to spread the atomics).
+template <int kSpread = kCounterSpread>
 __device__ __forceinline__ void block_count(unsigned long long v, unsigned long long* cnt, int slot) {
   __shared__ unsigned long long s_part[kBlock / 64];
   v = wave_sum_u64(v);
@@ -27,7 +29,7 @@ __device__ __forceinline__ void block_count(unsigned long long v, unsigned long 
   if (threadIdx.x == 0) {
     unsigned long long t = 0;
     for (int i = 0; i < kBlock / 64; i++) t += s_part[i];
-    if (t) atomicAdd(&cnt[slot * kCounterSpread + (blockIdx.x % kCounterSpread)], t);
+    if (t) atomicAdd(&cnt[slot * kSpread + (blockIdx.x % kSpread)], t);
   }
   __syncthreads();
 }
@@ -292,7 +294,10 @@ struct Res32 {
 // range (term_bits covers the product between two renormalisations); then
 // |B| > 2^512.5 means |B|^2 overflows in FP64, B^-1 = conj(B) * 0 and q = 0
 // exactly (F = -inf under the black-hole stop); anything closer to a range
-// edge, or outside it, goes to FP64.
+// edge, or outside it, goes to FP64. With QJ_MAP_EXTENDED_RANGE there is no
+// range edge (the products' exponents are integers): only an iterate that
+// continues below 2^-100, where FP32 would lose its precision, goes to FP64
+// (the extended-range FP64 evaluator).
 // top / bot: root accessors of the two polynomials (a.nt and a.nb roots).
 template <int kRN, bool kGeneral, class AccT, class AccB>
 __device__ __forceinline__ Res32 eval32(const MapArgs& a, const AccT& top, const AccB& bot, const Consts32& k,
@@ -310,6 +315,7 @@ __device__ __forceinline__ Res32 eval32(const MapArgs& a, const AccT& top, const
     return res;
   }
   const bool black_hole = (a.flags & QJ_MAP_BLACK_HOLE) != 0;
+  const bool ext = (a.flags & QJ_MAP_EXTENDED_RANGE) != 0;
   q32 q{(float)q0.w, (float)q0.x, (float)q0.y, (float)q0.z};
   float log2mag = 0.0f;
   float vx = __builtin_inff();
@@ -325,21 +331,21 @@ __device__ __forceinline__ Res32 eval32(const MapArgs& a, const AccT& top, const
     int ET = 0, EB = 0;
     float L = 0.0f, hiT = -1e30f, loT = 1e30f;
     const q32 T = poly32<kRN, kGeneral>(top, a.nt, q, ET, L, hiT, loT, bad, vx);
-    bad |= !(hiT < lim && loT > -lim_lo);
+    bad |= !ext && !(hiT < lim && loT > -lim_lo);
     float l2 = k.log2c;
     if (a.nb > 0) {
       float LB = 0.0f, hiB = -1e30f, loB = 1e30f;
       const q32 B = poly32<kRN, kGeneral>(bot, a.nb, q, EB, LB, hiB, loB, bad, vx);
-      bad |= !(hiB < lim && loB > -lim_lo);
+      bad |= !ext && !(hiB < lim && loB > -lim_lo);
       const float nb2 = B.w * B.w + B.x * B.x + B.y * B.y + B.z * B.z;
       const float lB = 0.5f * __log2f(nb2) + (float)EB + LB;
       const q32 D = qmul32(T, q32{B.w, -B.x, -B.y, -B.z});  // T conj(B) = T B^-1 |B|^2
       const float nd2 = D.w * D.w + D.x * D.x + D.y * D.y + D.z * D.z;
       l2 += 0.5f * __log2f(nd2) - __log2f(nb2) + (float)(ET - EB) + (L - LB);
       bad |= !(nb2 > 0.0f) || !(nd2 > 0.0f);
-      if (lB > 512.5f) {
+      if (!ext && lB > 512.5f) {
         zero = true;  // FP64: |B|^2 = inf, 1/inf = 0, q = T * 0 = 0
-      } else if (lB > 511.5f || lB < -510.0f) {
+      } else if (!ext && (lB > 511.5f || lB < -510.0f)) {
         bad = true;   // within FP32 error of the FP64 overflow edge of |B|^2 or 1/|B|^2
       } else if (!bad) {
         const float sc = exp2f(k.log2c + (float)(ET - EB) + (L - LB)) / nb2;
@@ -360,11 +366,16 @@ __device__ __forceinline__ Res32 eval32(const MapArgs& a, const AccT& top, const
       break;
     }
     log2mag = l2;
-    bad |= !isfinite(l2) || l2 > 511.0f || l2 < -530.0f;  // |q|^2 over/underflow in FP64
+    if (ext) bad |= !isfinite(l2);
+    else bad |= !isfinite(l2) || l2 > 511.0f || l2 < -530.0f;  // |q|^2 over/underflow in FP64
     amb |= fabsf(l2 - k.log2esc) < k.margin;
     const bool bh = black_hole && l2 < k.log2bh;
     if (black_hole) amb |= fabsf(l2 - k.log2bh) < k.margin;
     if (bad || l2 >= k.log2esc || bh || it >= a.max_iter) break;
+    if (ext && l2 < -100.0f) {
+      bad = true;  // the next iteration would start below FP32's precise range
+      break;
+    }
   }
   res.F = zero ? -__builtin_inff() : log2mag * 0.69314718f;
   res.need64 = bad || amb || !(vx > 0.0f) || !(fabsf(res.F) >= k.band);

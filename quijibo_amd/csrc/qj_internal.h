@@ -121,6 +121,13 @@ struct ChunkDesc {
   int32_t g1, g2, g3;  // variant groups (first descriptor): [0,g1) general, [g1,g2) RN4 integer, [g2,g3) RN2
                        // integer, [g3,total) RN8 integer
 };
+// The request loop's descriptors copied from pinned host memory to the
+// device by a kernel (the first node of its graphs).
+qj_status launch_desc_fetch(const ChunkDesc* h_desc, ChunkDesc* d_desc, int max_chunks, hipStream_t s);
+// One field's sampled FP64 cross-check (the kernel of qj_workspace_check)
+// into d_out[CNT_N] (CNT_CHECKED ... CNT_CHECK_MAXREL), cleared first.
+qj_status launch_check_one(const MapArgs& a, const RootRec64* roots, const double* field, int64_t samples,
+                           unsigned long long* d_out, hipStream_t s);
 // FP32 variant of a chunk's root set: 0 general exponents; integer exponents
 // renormalised every 4 terms (1), every 2 (2) or every 8 (3) (rn_of).
 int batch_variant(const MapArgs& a);
@@ -138,6 +145,8 @@ qj_status launch_batch(const ChunkDesc* d_desc, int nchunks, int chunk_voxels, q
 // marching cubes (qj_mc.hip)
 qj_status mc_run(qj_workspace* ws, const double* d_field, const qj_grid* g, double iso, int64_t* nv,
                  int64_t* nt, hipStream_t s);
+qj_status mc_run_tiles(qj_workspace* ws, int n, const double* const* fields, const qj_grid* grids, double iso,
+                       int64_t* nv, int64_t* nt, hipStream_t s);
 qj_status mc_copy(qj_workspace* ws, double* v, int32_t* t, hipStream_t s);
 qj_status nlmc_cache(qj_workspace* ws, int64_t* n_edges, int32_t* edges, int64_t* n_cells, int32_t* cells,
                      hipStream_t s);
@@ -145,7 +154,7 @@ qj_status nlmc_resume(qj_workspace* ws, const MapArgs& a, const qj_rational* f, 
                       const int32_t* h_edges, int64_t nc, const int32_t* h_cells, int64_t* nv_out, int64_t* nt_out,
                       hipStream_t s);
 qj_status nlmc_run(qj_workspace* ws, const MapArgs& a, const qj_rational* f, double iso, qj_precision prec,
-                   double band, int chunk, int64_t* nv, int64_t* nt, int64_t* n_bad, hipStream_t s);
+                   double band, int chunk, int64_t* nv, int64_t* nt, int64_t* n_bad, hipStream_t s, const double* src_field = nullptr);
 qj_status mc_ptrs(qj_workspace* ws, const double** v, const int32_t** t);
 void mc_free(qj_workspace* ws);
 
@@ -165,6 +174,14 @@ qj_status scratch_setup(int device, unsigned char** out);
 qj_status field_clamp_to_mean(qj_workspace* ws, double* d_field, int64_t n, bool nans, double* mean_out,
                               hipStream_t s);
 qj_status field_set_borders(const int32_t* res, double value, double* d_field, hipStream_t s);
+// invertedDistance / invertedDistanceNormalized of a signed distance field
+// (qj_fieldops.hip); abs_sum (host, optional): sum of |result|
+qj_status field_inverted_distance(qj_workspace* ws, const double* d_dist, const qj_grid* grid, bool normalized,
+                                  double* d_out, double* abs_sum, hipStream_t s);
+// weightScore for each threshold (and the maximum score) of a field
+qj_status weight_scores(qj_workspace* ws, const double* d_field, const qj_grid* grid, const qj_grid* wgrid,
+                        const double* d_weight, double alpha, const double* thresholds, int nthr, double* scores,
+                        double* max_score, hipStream_t s);
 
 // exclusive scan of uint32 (qj_scan.hip); n elements, temp provided by ws
 qj_status scan_u32(const uint32_t* d_in, uint32_t* d_out, int64_t n, uint32_t* d_total, void* d_tmp,

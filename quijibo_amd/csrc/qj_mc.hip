@@ -3,27 +3,26 @@
 // CPU mesher: same flags, same vertex set and numbering, same vertex
 // arithmetic, same triangle order, same degenerate filter.
 //
-// Pipeline (all HBM-bound; one lane per grid point / cell):
-//   k_mc_flags      cell case from the float sign test, per-block active count (wave ballot)
-//   scan            block counts -> block offsets (order preserving)
-//   k_mc_active     ballot-compacted list of active cells, in cell order; per
-//                   block of cells its triangle count, and each active cell's
-//                   first triangle within the block
+// Pipeline per field (all HBM-bound; one lane per grid point / cell, point
+// blocks dealt to the XCDs in contiguous runs; no host synchronisation
+// between fields, see mc_run_tiles):
+//   k_mc_flags      cell case from the float sign test (the field read once)
 //   k_mc_edges      per grid point: which of its +x,+y,+z edges some adjacent
 //                   cell's case row references (gather of <= 7 neighbour flags);
 //                   per block of points: vertex count
 //   scan            block vertex counts -> first vertex id per block (the
 //                   reference numbers vertices by std::map<pair<int,int>> order
-//                   = lower endpoint, then x<y<z); block triangle counts ->
-//                   first triangle per block
-//   (one read-back of the three totals)
+//                   = lower endpoint, then x<y<z)
 //   k_mc_vertices   ids within the block, p0 + d0/(d0-d1) (p1-p0) at cell
-//                   centres, no contraction
-//   k_mc_triangles / scan / k_mc_squeeze: triangles in cell then table order,
-//                   degenerate ones (edge < 1e-7) dropped
+//                   centres, no contraction; near-edge-end marks
+//   k_mc_tricount   per block of cells: triangles that pass the degenerate
+//                   filter (edge < 1e-7 dropped)
+//   scan            block counts -> first triangle per block
+//   k_mc_tris       each kept triangle written once, in cell then table order
 #include <float.h>
 #include <math.h>
 
+#include <algorithm>
 #include <vector>
 
 #include "qj_eval.h"
@@ -132,6 +131,11 @@ struct McState {
   int32_t* c_cells = nullptr;
   int64_t cap_ce = 0, cap_cc = 0, ncells = 0;
   bool cache_valid = false;
+  // linear mesher over several fields: running vertex / triangle totals and
+  // per-field counts on the device
+  uint64_t* d_base = nullptr;
+  int64_t* d_counts = nullptr;
+  int64_t cap_counts = 0;
 };
 
 __device__ __forceinline__ int cell_flag(const double* __restrict__ f, int64_t i, int64_t X, int64_t XY, double iso) {
@@ -157,30 +161,25 @@ __device__ __forceinline__ void point_xyz(const GridGeom& g, int64_t p, int64_t&
   x = r - yu * (uint32_t)g.res[0];
 }
 
-__global__ __launch_bounds__(kMcBlock) void k_mc_flags(const double* __restrict__ field, GridGeom g, double iso,
-                                                       uint8_t* __restrict__ flags, uint32_t* __restrict__ blk) {
-  __shared__ uint32_t s_n[kMcBlock / 64];
-  const int64_t X = g.res[0], Y = g.res[1], Z = g.res[2], XY = X * Y, N = XY * Z;
-  const int64_t p = (int64_t)blockIdx.x * kMcBlock + threadIdx.x;
-  bool act = false;
-  if (p < N) {
-    int64_t x, y, z;
-    point_xyz(g, p, x, y, z);
-    int flag = 0;
-    if (x < X - 1 && y < Y - 1 && z < Z - 1) flag = cell_flag(field, p, X, XY, iso);
-    flags[p] = (uint8_t)flag;
-    act = kMc.ntri[flag] > 0;
-  }
-  const unsigned long long m = __ballot(act);
-  if ((threadIdx.x & 63) == 0) s_n[threadIdx.x >> 6] = (uint32_t)__popcll(m);
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    uint32_t t = 0;
-    for (int i = 0; i < kMcBlock / 64; i++) t += s_n[i];
-    blk[blockIdx.x] = t;
-  }
+// Workgroups go to the 8 XCDs round-robin (block b to XCD b % 8); the point
+// kernels deal each XCD a contiguous run of point blocks instead, so the
+// field rows a block shares with its neighbours are fetched into one L2.
+__device__ __forceinline__ int64_t xcd_block(int64_t nb) {
+  const int64_t b = blockIdx.x, x = b & 7, k = b >> 3, per = nb >> 3, rem = nb & 7;
+  return x * per + (x < rem ? x : rem) + k;
 }
 
+__global__ __launch_bounds__(kMcBlock) void k_mc_flags(const double* __restrict__ field, GridGeom g, double iso,
+                                                       uint8_t* __restrict__ flags) {
+  const int64_t X = g.res[0], Y = g.res[1], Z = g.res[2], XY = X * Y, N = XY * Z;
+  const int64_t p = xcd_block((N + kMcBlock - 1) / kMcBlock) * kMcBlock + threadIdx.x;
+  if (p >= N) return;
+  int64_t x, y, z;
+  point_xyz(g, p, x, y, z);
+  int flag = 0;
+  if (x < X - 1 && y < Y - 1 && z < Z - 1) flag = cell_flag(field, p, X, XY, iso);
+  flags[p] = (uint8_t)flag;
+}
 
 // Block-wide exclusive prefix and total of a small per-thread count given as
 // three bit flags of weights W0, W1, W2 (edge bits 1,1,1; a triangle count
@@ -207,32 +206,6 @@ __device__ __forceinline__ BlockPrefix mc_block_prefix3(unsigned v) {
   return BlockPrefix{before + within, total};
 }
 
-// Active cells in cell order (ballot compaction; the block's first position
-// comes from blk_off), and per block of cells its triangle count (blkt) with
-// each active cell's first triangle within the block (tloc16, at the cell's
-// list position).
-__global__ __launch_bounds__(kMcBlock) void k_mc_active(const uint8_t* __restrict__ flags, int64_t N,
-                                                        const uint32_t* __restrict__ blk_off,
-                                                        uint32_t* __restrict__ active, uint16_t* __restrict__ tloc16,
-                                                        uint32_t* __restrict__ blkt) {
-  __shared__ uint32_t s_n[kMcBlock / 64];
-  const int64_t p = (int64_t)blockIdx.x * kMcBlock + threadIdx.x;
-  const int nt = p < N ? kMc.ntri[flags[p]] : 0;
-  const bool act = nt > 0;
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  const unsigned long long m = __ballot(act);
-  if (lane == 0) s_n[wave] = (uint32_t)__popcll(m);
-  const BlockPrefix tp = mc_block_prefix3<1, 2, 4>((unsigned)nt);  // synchronises the block
-  uint32_t off = blk_off[blockIdx.x];
-  for (int i = 0; i < wave; i++) off += s_n[i];
-  if (act) {
-    const uint32_t k = off + __popcll(m & ((1ull << lane) - 1ull));
-    active[k] = (uint32_t)p;
-    tloc16[k] = (uint16_t)tp.excl;
-  }
-  if (threadIdx.x == 0) blkt[blockIdx.x] = tp.total;
-}
-
 // Edge masks (bit axis set when the +axis edge from the point carries a vertex)
 // and the vertex count of each block of points.
 __global__ __launch_bounds__(kMcBlock) void k_mc_edges(const uint8_t* __restrict__ flags, GridGeom g,
@@ -240,7 +213,8 @@ __global__ __launch_bounds__(kMcBlock) void k_mc_edges(const uint8_t* __restrict
   __shared__ unsigned short s_em[256];
   stage_emask(s_em);
   const int64_t X = g.res[0], Y = g.res[1], Z = g.res[2], XY = X * Y, N = XY * Z;
-  const int64_t p = (int64_t)blockIdx.x * kMcBlock + threadIdx.x;
+  const int64_t lb = xcd_block((N + kMcBlock - 1) / kMcBlock);
+  const int64_t p = lb * kMcBlock + threadIdx.x;
   unsigned bits = 0;
   if (p < N) {
     int64_t x, y, z;
@@ -272,7 +246,7 @@ __global__ __launch_bounds__(kMcBlock) void k_mc_edges(const uint8_t* __restrict
     emask[p] = (uint8_t)bits;
   }
   const BlockPrefix vp = mc_block_prefix3<1, 1, 1>(bits);
-  if (threadIdx.x == 0) blkv[blockIdx.x] = vp.total;
+  if (threadIdx.x == 0) blkv[lb] = vp.total;
 }
 
 // computeEdgeInterpolations (TRIANGLE_MESH.cpp:2373-2425). Vertex ids follow
@@ -290,10 +264,14 @@ __global__ __launch_bounds__(kMcBlock) void k_mc_edges(const uint8_t* __restrict
 constexpr double kNearEnd = 2e-7;
 constexpr double kPrefilterSpacing = 1e-6;
 
+// The tile's vertices go to verts / vnear from the running vertex total
+// base[0] (several tiles share the buffers); ids at or beyond cap are not
+// written (the host sees the overflow in the totals and meshes again).
 __global__ __launch_bounds__(kMcBlock) void k_mc_vertices(const double* __restrict__ field, GridGeom g, double iso,
                                                           const uint8_t* __restrict__ emask,
                                                           const uint32_t* __restrict__ blk_off,
-                                                          uint16_t* __restrict__ vb16, double* __restrict__ verts,
+                                                          uint16_t* __restrict__ vb16, const uint64_t* __restrict__ base,
+                                                          int64_t cap, double* __restrict__ verts,
                                                           uint8_t* __restrict__ vnear) {
 #pragma clang fp contract(off)
   // the block's vertices (at most 3 per point) are listed in LDS in id order
@@ -302,7 +280,7 @@ __global__ __launch_bounds__(kMcBlock) void k_mc_vertices(const double* __restri
   __shared__ uint8_t s_pt[3 * kMcBlock];
   __shared__ uint8_t s_ax[3 * kMcBlock];
   const int64_t X = g.res[0], XY = X * g.res[1], N = XY * g.res[2];
-  const int64_t p0 = (int64_t)blockIdx.x * kMcBlock;
+  const int64_t p0 = xcd_block((N + kMcBlock - 1) / kMcBlock) * kMcBlock;
   const int64_t p = p0 + threadIdx.x;
   const unsigned bits = p < N ? emask[p] : 0u;
   const BlockPrefix vp = mc_block_prefix3<1, 1, 1>(bits);
@@ -316,8 +294,10 @@ __global__ __launch_bounds__(kMcBlock) void k_mc_vertices(const double* __restri
     }
   }
   __syncthreads();
-  const uint32_t base = blk_off[blockIdx.x];
+  const int64_t first = (int64_t)base[0] + blk_off[p0 / kMcBlock];
   for (uint32_t j = threadIdx.x; j < vp.total; j += kMcBlock) {
+    const int64_t id = first + j;
+    if (id >= cap) break;
     const int64_t q = p0 + s_pt[j];
     const int axis = s_ax[j];
     int64_t x64, y64, z64;
@@ -331,7 +311,6 @@ __global__ __launch_bounds__(kMcBlock) void k_mc_vertices(const double* __restri
     const double d1 = field[q + (axis == 0 ? 1 : (axis == 1 ? X : XY))] - iso;
     const double interpolation = d0 / (d0 - d1);
     const double v0 = a[0] + interpolation * off0, v1 = a[1] + interpolation * off1, v2 = a[2] + interpolation * off2;
-    const int64_t id = (int64_t)base + j;
     verts[3 * id + 0] = v0;
     verts[3 * id + 1] = v1;
     verts[3 * id + 2] = v2;
@@ -346,50 +325,126 @@ __device__ __forceinline__ double edge_len(const double* a, const double* b) {
   return sqrt(e0 * e0 + e1 * e1 + e2 * e2);  // norm(v0 - v1), VEC3.h:215-216
 }
 
-// addVertexPairTriangle (TRIANGLE_MESH.cpp:2490-2503) + degenerate filter (:1088-1102)
-__global__ __launch_bounds__(kMcBlock) void k_mc_triangles(const uint32_t* __restrict__ active, int64_t na,
-                                                           const uint8_t* __restrict__ flags, GridGeom g,
-                                                           const uint8_t* __restrict__ emask,
-                                                           const uint32_t* __restrict__ blk_off,
-                                                           const uint16_t* __restrict__ vb16,
-                                                           const double* __restrict__ verts,
-                                                           const uint8_t* __restrict__ vnear,
-                                                           const uint32_t* __restrict__ blk_toff,
-                                                           const uint16_t* __restrict__ tloc16,
-                                                           int32_t* __restrict__ tris_pre,
-                                                           uint32_t* __restrict__ keep) {
+// The vertex ids of triangle t of a cell's case row (tile-local): the owner
+// point of each edge, its block's first id, its first id within the block
+// and the edges of lower axis it also owns.
+__device__ __forceinline__ void tri_vertices(const uint32_t* s_tri, int flag, int t, int64_t p, int64_t X, int64_t XY,
+                                             const uint8_t* __restrict__ emask, const uint32_t* __restrict__ blk_off,
+                                             const uint16_t* __restrict__ vb16, int32_t* v) {
+  for (int c = 0; c < 3; c++) {
+    int dx, dy, dz, axis;
+    edge_geom(tri_edge(s_tri, flag, 3 * t + c), dx, dy, dz, axis);
+    const int64_t owner = p + dx + dy * X + dz * XY;
+    v[c] = (int32_t)(blk_off[owner / kMcBlock] + vb16[owner] + __popc(emask[owner] & ((1u << axis) - 1u)));
+  }
+}
+
+// The degenerate filter (TRIANGLE_MESH.cpp:1088-1102): a side shorter than
+// 1e-7. With the near-end marks (prefilter) only triangles with two or three
+// marked vertices are measured; vertices beyond the buffer's capacity (a
+// pass whose results the host discards) are not read.
+__device__ __forceinline__ bool degenerate_tri(const int32_t* v, const double* __restrict__ verts,
+                                               const uint8_t* __restrict__ vnear, bool prefilter, int64_t vfirst,
+                                               int64_t cap) {
+  if (vfirst + v[0] >= cap || vfirst + v[1] >= cap || vfirst + v[2] >= cap) return false;
+  if (prefilter &&
+      (int)vnear[vfirst + v[0]] + (int)vnear[vfirst + v[1]] + (int)vnear[vfirst + v[2]] < 2)
+    return false;
+  const double eps = 1e-7;
+  const double* p0 = verts + 3 * (vfirst + v[0]);
+  const double* p1 = verts + 3 * (vfirst + v[1]);
+  const double* p2 = verts + 3 * (vfirst + v[2]);
+  return edge_len(p0, p1) < eps || edge_len(p0, p2) < eps || edge_len(p1, p2) < eps;
+}
+
+// Kept triangles per block of cells (the filter applied before any triangle is
+// written, so each triangle is written once, at its final position).
+__global__ __launch_bounds__(kMcBlock) void k_mc_tricount(const uint8_t* __restrict__ flags, GridGeom g,
+                                                          const uint8_t* __restrict__ emask,
+                                                          const uint32_t* __restrict__ blk_off,
+                                                          const uint16_t* __restrict__ vb16,
+                                                          const uint64_t* __restrict__ base, int64_t vcap,
+                                                          const double* __restrict__ verts,
+                                                          const uint8_t* __restrict__ vnear, int prefilter,
+                                                          uint32_t* __restrict__ blkt) {
   __shared__ uint32_t s_tri[1024];
   __shared__ uint8_t s_ntri[256];
   stage_tri(s_tri, s_ntri);
-  const int64_t i = (int64_t)blockIdx.x * kMcBlock + threadIdx.x;
-  if (i >= na) return;
-  const int64_t X = g.res[0], XY = X * g.res[1];
-  const int64_t p = active[i];
-  const int flag = flags[p];
+  const int64_t X = g.res[0], XY = X * g.res[1], N = XY * g.res[2];
+  const int64_t lb = xcd_block((N + kMcBlock - 1) / kMcBlock);
+  const int64_t p = lb * kMcBlock + threadIdx.x;
+  const int flag = p < N ? flags[p] : 0;
   const int n = s_ntri[flag];
-  uint32_t o = blk_toff[p / kMcBlock] + tloc16[i];
+  const int64_t vfirst = (int64_t)base[0];
+  unsigned kept = 0;
   for (int t = 0; t < n; t++) {
     int32_t v[3];
-    for (int c = 0; c < 3; c++) {
-      int dx, dy, dz, axis;
-      edge_geom(tri_edge(s_tri, flag, 3 * t + c), dx, dy, dz, axis);
-      const int64_t owner = p + dx + dy * X + dz * XY;
-      v[c] = (int32_t)(blk_off[owner / kMcBlock] + vb16[owner] + __popc(emask[owner] & ((1u << axis) - 1u)));
+    tri_vertices(s_tri, flag, t, p, X, XY, emask, blk_off, vb16, v);
+    kept += degenerate_tri(v, verts, vnear, prefilter != 0, vfirst, vcap) ? 0u : 1u;
+  }
+  const BlockPrefix tp = mc_block_prefix3<1, 2, 4>(kept);
+  if (threadIdx.x == 0) blkt[lb] = tp.total;
+}
+
+// addVertexPairTriangle (TRIANGLE_MESH.cpp:2490-2503) in cell order then
+// table order, degenerate triangles dropped: each cell's kept triangles go
+// to the running triangle total base[1] + its block's first triangle + the
+// kept triangles of earlier cells of the block. Ids at or beyond tcap are not
+// written.
+__global__ __launch_bounds__(kMcBlock) void k_mc_tris(const uint8_t* __restrict__ flags, GridGeom g,
+                                                      const uint8_t* __restrict__ emask,
+                                                      const uint32_t* __restrict__ blk_off,
+                                                      const uint16_t* __restrict__ vb16,
+                                                      const uint64_t* __restrict__ base, int64_t vcap,
+                                                      const double* __restrict__ verts,
+                                                      const uint8_t* __restrict__ vnear, int prefilter,
+                                                      const uint32_t* __restrict__ blk_toff, int64_t tcap,
+                                                      int32_t* __restrict__ tris) {
+  __shared__ uint32_t s_tri[1024];
+  __shared__ uint8_t s_ntri[256];
+  stage_tri(s_tri, s_ntri);
+  const int64_t X = g.res[0], XY = X * g.res[1], N = XY * g.res[2];
+  const int64_t lb = xcd_block((N + kMcBlock - 1) / kMcBlock);
+  const int64_t p = lb * kMcBlock + threadIdx.x;
+  const int flag = p < N ? flags[p] : 0;
+  const int n = s_ntri[flag];
+  const int64_t vfirst = (int64_t)base[0];
+  int32_t v[5][3];
+  unsigned keepm = 0, kept = 0;
+  for (int t = 0; t < n; t++) {
+    tri_vertices(s_tri, flag, t, p, X, XY, emask, blk_off, vb16, v[t]);
+    if (!degenerate_tri(v[t], verts, vnear, prefilter != 0, vfirst, vcap)) {
+      keepm |= 1u << t;
+      kept++;
     }
-    bool degenerate = false;
-    if (!vnear || (int)vnear[v[0]] + (int)vnear[v[1]] + (int)vnear[v[2]] >= 2) {
-      const double eps = 1e-7;
-      const double* p0 = verts + 3 * (int64_t)v[0];
-      const double* p1 = verts + 3 * (int64_t)v[1];
-      const double* p2 = verts + 3 * (int64_t)v[2];
-      degenerate = edge_len(p0, p1) < eps || edge_len(p0, p2) < eps || edge_len(p1, p2) < eps;
+  }
+  const BlockPrefix tp = mc_block_prefix3<1, 2, 4>(kept);
+  int64_t o = (int64_t)base[1] + blk_toff[lb] + tp.excl;
+  for (int t = 0; t < n; t++) {
+    if (!(keepm & (1u << t))) continue;
+    if (o < tcap) {
+      tris[3 * o + 0] = v[t][0];
+      tris[3 * o + 1] = v[t][1];
+      tris[3 * o + 2] = v[t][2];
     }
-    tris_pre[3 * (int64_t)o + 0] = v[0];
-    tris_pre[3 * (int64_t)o + 1] = v[1];
-    tris_pre[3 * (int64_t)o + 2] = v[2];
-    keep[o] = degenerate ? 0u : 1u;
     o++;
   }
+}
+
+// After a tile: its vertex and triangle counts (the scans' totals) recorded
+// and added to the running totals.
+__global__ void k_mc_advance(const uint32_t* __restrict__ totals, uint64_t* __restrict__ base, int tile,
+                             int64_t* __restrict__ counts) {
+  if (threadIdx.x != 0) return;
+  const uint64_t nv = totals[1], nt = totals[2];
+  counts[2 * tile + 0] = (int64_t)nv;
+  counts[2 * tile + 1] = (int64_t)nt;
+  base[0] += nv;
+  base[1] += nt;
+}
+
+__global__ void k_mc_reset(uint64_t* __restrict__ base) {
+  if (threadIdx.x < 2) base[threadIdx.x] = 0;
 }
 
 // koff is the exclusive scan of the keep flags: triangle i is kept when its
@@ -427,31 +482,58 @@ static qj_status grow(T*& p, int64_t& cap, int64_t n, hipStream_t s) {
 
 static unsigned nblocks(int64_t n) { return (unsigned)((n + kMcBlock - 1) / kMcBlock); }
 
-qj_status mc_run(qj_workspace* ws, const double* d_field, const qj_grid* grid, double iso, int64_t* nv_out,
-                 int64_t* nt_out, hipStream_t s) {
+__global__ void k_mc_zero_totals(uint32_t* __restrict__ totals) {
+  if (threadIdx.x < 4) totals[threadIdx.x] = 0u;
+}
+
+// Meshes n fields one after another on the stream without waiting on the
+// host: every count stays on the device (running vertex and triangle totals,
+// per-tile counts) and is read back once at the end. The vertex and triangle
+// buffers keep their size from earlier calls; a pass that outgrows them
+// writes nothing beyond them, and the host then sizes them from the totals it
+// read and runs the pass again.
+qj_status mc_run_tiles(qj_workspace* ws, int n, const double* const* fields, const qj_grid* grids, double iso,
+                       int64_t* nv_out, int64_t* nt_out, hipStream_t s) {
   if (!ws->mc) ws->mc = new McState();
   McState& m = *(McState*)ws->mc;
   m.nv = m.nt = 0;
   m.cache_valid = false;
-  GridGeom g;
-  make_geom(grid, &g);
-  const int64_t N = (int64_t)g.res[0] * g.res[1] * g.res[2];
-  const int64_t cells = (int64_t)(g.res[0] - 1) * (g.res[1] - 1) * (g.res[2] - 1);
-  if (N > (int64_t)UINT32_MAX || 5 * cells > (int64_t)INT32_MAX) {
-    set_error("marching cubes on %lld points exceeds 32-bit vertex/triangle ids; split the field in tiles",
-              (long long)N);
-    return QJ_ERR_CAPACITY;
+  if (n <= 0) return QJ_OK;
+  std::vector<GridGeom> geo((size_t)n);
+  std::vector<bool> prefilter((size_t)n);
+  int64_t nmax = 0, total_pts = 0;
+  for (int k = 0; k < n; k++) {
+    GridGeom& g = geo[(size_t)k];
+    make_geom(&grids[k], &g);
+    const int64_t N = (int64_t)g.res[0] * g.res[1] * g.res[2];
+    const int64_t cells = (int64_t)(g.res[0] - 1) * (g.res[1] - 1) * (g.res[2] - 1);
+    if (N > (int64_t)UINT32_MAX || 5 * cells > (int64_t)INT32_MAX) {
+      set_error("marching cubes on %lld points exceeds 32-bit vertex/triangle ids; split the field in tiles",
+                (long long)N);
+      return QJ_ERR_CAPACITY;
+    }
+    nmax = N > nmax ? N : nmax;
+    total_pts += N;
+    // the degenerate-test prefilter holds when the cell-centre map is a
+    // rotation (edges meeting at a corner stay perpendicular) and every grid
+    // spacing is at least kPrefilterSpacing
+    bool pf = true;
+    for (int i = 0; i < 3; i++) {
+      pf &= fabs(g.dx[i]) >= kPrefilterSpacing;
+      for (int j = 0; j < 3; j++) {
+        const double d = g.rot[i] * g.rot[j] + g.rot[3 + i] * g.rot[3 + j] + g.rot[6 + i] * g.rot[6 + j];
+        pf &= fabs(d - (i == j ? 1.0 : 0.0)) < 1e-9;
+      }
+    }
+    prefilter[(size_t)k] = pf;
   }
-  *nv_out = *nt_out = 0;
-  if (cells <= 0) return QJ_OK;
   qj_status st;
-  const int64_t nb = nblocks(N);
-  if ((st = grow(m.flags, m.cap_flags, N, s)) || (st = grow(m.emask, m.cap_emask, N, s)) ||
-      (st = grow(m.vb16, m.cap_vb16, N, s)) || (st = grow(m.active, m.cap_active, N, s)) ||
-      (st = grow(m.tloc16, m.cap_tloc16, N, s)) || (st = grow(m.blk, m.cap_blk, nb, s)) ||
-      (st = grow(m.blkv, m.cap_blkv, nb, s)) || (st = grow(m.blkt, m.cap_blkt, nb, s)))
+  const int64_t nbmax = nblocks(nmax);
+  if ((st = grow(m.flags, m.cap_flags, nmax, s)) || (st = grow(m.emask, m.cap_emask, nmax, s)) ||
+      (st = grow(m.vb16, m.cap_vb16, nmax, s)) || (st = grow(m.blkv, m.cap_blkv, nbmax, s)) ||
+      (st = grow(m.blkt, m.cap_blkt, nbmax, s)) || (st = grow(m.d_counts, m.cap_counts, 2 * (int64_t)n, s)))
     return st;
-  const size_t need_tmp = scan_tmp_bytes(5 * N);  // the largest scan: pre-filter triangles <= 5 per cell
+  const size_t need_tmp = scan_tmp_bytes(nbmax > 4096 ? nbmax : 4096);
   if (need_tmp > m.tmp_bytes) {
     if (m.tmp) {
       QJ_HIP(hipStreamSynchronize(s));
@@ -463,65 +545,74 @@ qj_status mc_run(qj_workspace* ws, const double* d_field, const qj_grid* grid, d
     m.tmp_bytes = need_tmp;
   }
   if (!m.totals) QJ_HIP(hipMalloc(&m.totals, 4 * sizeof(uint32_t)));
-
-  // counts: active cells (totals[0]), vertices (totals[1]) and pre-filter
-  // triangles (totals[2]), read back together
-  hipLaunchKernelGGL(k_mc_flags, dim3((unsigned)nb), dim3(kMcBlock), 0, s, d_field, g, iso, m.flags, m.blk);
-  QJ_HIP(hipGetLastError());
-  if ((st = scan_u32(m.blk, m.blk, nb, m.totals + 0, m.tmp, m.tmp_bytes, s))) return st;
-  hipLaunchKernelGGL(k_mc_active, dim3((unsigned)nb), dim3(kMcBlock), 0, s, m.flags, N, m.blk, m.active, m.tloc16,
-                     m.blkt);
-  QJ_HIP(hipGetLastError());
-  hipLaunchKernelGGL(k_mc_edges, dim3((unsigned)nb), dim3(kMcBlock), 0, s, m.flags, g, m.emask, m.blkv);
-  QJ_HIP(hipGetLastError());
-  if ((st = scan_u32(m.blkv, m.blkv, nb, m.totals + 1, m.tmp, m.tmp_bytes, s))) return st;
-  if ((st = scan_u32(m.blkt, m.blkt, nb, m.totals + 2, m.tmp, m.tmp_bytes, s))) return st;
-  uint32_t h[4];
-  QJ_HIP(hipMemcpyAsync(h, m.totals, 3 * sizeof(uint32_t), hipMemcpyDeviceToHost, s));
-  QJ_HIP(hipStreamSynchronize(s));
-  const int64_t na = h[0], nv = h[1], ntp = h[2];
-  if ((st = grow(m.verts, m.cap_v, 3 * nv, s)) || (st = grow(m.vnear, m.cap_vnear, nv, s)) ||
-      (st = grow(m.tris_pre, m.cap_tp, 3 * ntp, s)) || (st = grow(m.keep, m.cap_keep, ntp, s)) ||
-      (st = grow(m.tris, m.cap_t, 3 * ntp, s)))
-    return st;
-  if (nv > 0) {
-    hipLaunchKernelGGL(k_mc_vertices, dim3((unsigned)nb), dim3(kMcBlock), 0, s, d_field, g, iso, m.emask, m.blkv,
-                       m.vb16, m.verts, m.vnear);
+  if (!m.d_base) QJ_HIP(hipMalloc(&m.d_base, 2 * sizeof(uint64_t)));
+  // first sizes: a quarter of the points as vertices, half as triangles
+  // (a 256^3 torus tile holds 0.42 and 0.73 of its points)
+  if (m.cap_v == 0 && (st = grow(m.verts, m.cap_v, 3 * (total_pts / 4 + 1024), s))) return st;
+  if (m.cap_vnear == 0 && (st = grow(m.vnear, m.cap_vnear, total_pts / 4 + 1024, s))) return st;
+  if (m.cap_t == 0 && (st = grow(m.tris, m.cap_t, 3 * (total_pts / 2 + 1024), s))) return st;
+  std::vector<int64_t> counts(2 * (size_t)n);
+  uint64_t totals[2] = {0, 0};
+  for (int pass = 0; pass < 2; pass++) {
+    const int64_t vcap = std::min(m.cap_v / 3, m.cap_vnear), tcap = m.cap_t / 3;
+    hipLaunchKernelGGL(k_mc_reset, dim3(1), dim3(64), 0, s, m.d_base);
     QJ_HIP(hipGetLastError());
-  }
-  // the degenerate-test prefilter holds when the cell-centre map is a rotation
-  // (edges meeting at a corner stay perpendicular) and every grid spacing is at
-  // least kPrefilterSpacing
-  bool prefilter = true;
-  for (int i = 0; i < 3; i++) {
-    prefilter &= fabs(g.dx[i]) >= kPrefilterSpacing;
-    for (int j = 0; j < 3; j++) {
-      const double d = g.rot[i] * g.rot[j] + g.rot[3 + i] * g.rot[3 + j] + g.rot[6 + i] * g.rot[6 + j];
-      prefilter &= fabs(d - (i == j ? 1.0 : 0.0)) < 1e-9;
+    for (int k = 0; k < n; k++) {
+      const GridGeom& g = geo[(size_t)k];
+      const int64_t N = (int64_t)g.res[0] * g.res[1] * g.res[2];
+      const int64_t cells = (int64_t)(g.res[0] - 1) * (g.res[1] - 1) * (g.res[2] - 1);
+      const dim3 nb(nblocks(N));
+      if (cells <= 0) {
+        hipLaunchKernelGGL(k_mc_zero_totals, dim3(1), dim3(64), 0, s, m.totals);
+        QJ_HIP(hipGetLastError());
+      } else {
+        hipLaunchKernelGGL(k_mc_flags, nb, dim3(kMcBlock), 0, s, fields[k], g, iso, m.flags);
+        QJ_HIP(hipGetLastError());
+        hipLaunchKernelGGL(k_mc_edges, nb, dim3(kMcBlock), 0, s, m.flags, g, m.emask, m.blkv);
+        QJ_HIP(hipGetLastError());
+        if ((st = scan_u32(m.blkv, m.blkv, nb.x, m.totals + 1, m.tmp, m.tmp_bytes, s))) return st;
+        hipLaunchKernelGGL(k_mc_vertices, nb, dim3(kMcBlock), 0, s, fields[k], g, iso, m.emask, m.blkv, m.vb16,
+                           (const uint64_t*)m.d_base, vcap, m.verts, m.vnear);
+        QJ_HIP(hipGetLastError());
+        const int pf = prefilter[(size_t)k] ? 1 : 0;
+        hipLaunchKernelGGL(k_mc_tricount, nb, dim3(kMcBlock), 0, s, m.flags, g, m.emask, m.blkv, m.vb16,
+                           (const uint64_t*)m.d_base, vcap, m.verts, m.vnear, pf, m.blkt);
+        QJ_HIP(hipGetLastError());
+        if ((st = scan_u32(m.blkt, m.blkt, nb.x, m.totals + 2, m.tmp, m.tmp_bytes, s))) return st;
+        hipLaunchKernelGGL(k_mc_tris, nb, dim3(kMcBlock), 0, s, m.flags, g, m.emask, m.blkv, m.vb16,
+                           (const uint64_t*)m.d_base, vcap, m.verts, m.vnear, pf, m.blkt, tcap, m.tris);
+        QJ_HIP(hipGetLastError());
+      }
+      hipLaunchKernelGGL(k_mc_advance, dim3(1), dim3(64), 0, s, m.totals, m.d_base, k, m.d_counts);
+      QJ_HIP(hipGetLastError());
     }
-  }
-  m.nv = nv;
-  if (na == 0 || ntp == 0) {
+    QJ_HIP(hipMemcpyAsync(counts.data(), m.d_counts, sizeof(int64_t) * 2 * (size_t)n, hipMemcpyDeviceToHost, s));
+    QJ_HIP(hipMemcpyAsync(totals, m.d_base, sizeof(totals), hipMemcpyDeviceToHost, s));
     QJ_HIP(hipStreamSynchronize(s));
-    *nv_out = nv;
-    return QJ_OK;
+    if ((int64_t)totals[0] <= vcap && (int64_t)totals[1] <= tcap) break;
+    if (pass == 1) {
+      set_error("marching cubes: counts changed between two passes over the same fields");
+      return QJ_ERR_NUMERIC;
+    }
+    // outgrown: size the buffers from this pass's totals and mesh again
+    if ((st = grow(m.verts, m.cap_v, 3 * (int64_t)totals[0], s)) ||
+        (st = grow(m.vnear, m.cap_vnear, (int64_t)totals[0], s)) ||
+        (st = grow(m.tris, m.cap_t, 3 * (int64_t)totals[1], s)))
+      return st;
   }
-  hipLaunchKernelGGL(k_mc_triangles, dim3(nblocks(na)), dim3(kMcBlock), 0, s, m.active, na, m.flags, g, m.emask,
-                     m.blkv, m.vb16, m.verts, prefilter ? (const uint8_t*)m.vnear : nullptr, m.blkt, m.tloc16,
-                     m.tris_pre, m.keep);
-  QJ_HIP(hipGetLastError());
-  if ((st = scan_u32(m.keep, m.keep, ntp, m.totals + 3, m.tmp, m.tmp_bytes, s))) return st;
-  // tris holds up to ntp triangles, so the squeeze needs no count from the host
-  hipLaunchKernelGGL(k_mc_squeeze, dim3(nblocks(ntp)), dim3(kMcBlock), 0, s, m.tris_pre, m.keep, ntp, m.totals + 3,
-                     m.tris);
-  QJ_HIP(hipGetLastError());
-  QJ_HIP(hipMemcpyAsync(h + 3, m.totals + 3, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
-  QJ_HIP(hipStreamSynchronize(s));
-  const int64_t nt = h[3];
-  m.nt = nt;
-  *nv_out = nv;
-  *nt_out = nt;
+  m.nv = (int64_t)totals[0];
+  m.nt = (int64_t)totals[1];
+  for (int k = 0; k < n; k++) {
+    if (nv_out) nv_out[k] = counts[2 * (size_t)k];
+    if (nt_out) nt_out[k] = counts[2 * (size_t)k + 1];
+  }
   return QJ_OK;
+}
+
+qj_status mc_run(qj_workspace* ws, const double* d_field, const qj_grid* grid, double iso, int64_t* nv_out,
+                 int64_t* nt_out, hipStream_t s) {
+  *nv_out = *nt_out = 0;
+  return mc_run_tiles(ws, 1, &d_field, grid, iso, nv_out, nt_out, s);
 }
 
 qj_status mc_copy(qj_workspace* ws, double* v, int32_t* t, hipStream_t s) {
@@ -566,6 +657,8 @@ void mc_free(qj_workspace* ws) {
   hipFree(m->d_bad);
   hipFree(m->c_edges);
   hipFree(m->c_cells);
+  hipFree(m->d_base);
+  hipFree(m->d_counts);
   delete m;
   ws->mc = nullptr;
 }
@@ -614,7 +707,7 @@ __device__ __forceinline__ double nonlinear_value(const MapArgs& a, const RootRe
     const double v = log(magnitude) - iso;
     return isinf(v) ? (double)FLT_MAX : v;
   }
-  const Res64 r = eval64<kPow>(a, roots, q64{p[0], p[1], p[2], a.slice});
+  const Res64 r = eval64_any<kPow>(a, roots, q64{p[0], p[1], p[2], a.slice});
   return r.F - iso;
 }
 
@@ -899,7 +992,8 @@ static qj_status grow_keep(T*& p, int64_t& cap, int64_t n, int64_t used, hipStre
 }
 
 qj_status nlmc_run(qj_workspace* ws, const MapArgs& a0, const qj_rational* f, double iso, qj_precision prec,
-                   double band, int chunk, int64_t* nv_out, int64_t* nt_out, int64_t* n_bad, hipStream_t s) {
+                   double band, int chunk, int64_t* nv_out, int64_t* nt_out, int64_t* n_bad, hipStream_t s,
+                   const double* src_field) {
   if (!ws->mc) ws->mc = new McState();
   McState& m = *(McState*)ws->mc;
   m.nv = m.nt = 0;
@@ -916,7 +1010,7 @@ qj_status nlmc_run(qj_workspace* ws, const MapArgs& a0, const qj_rational* f, do
   }
   const int64_t slab_pts = XY * (chunk + 3), own_pts = XY * (chunk + 1);
   qj_status st;
-  if ((st = grow(m.slab, m.cap_slab, slab_pts, s)) || (st = grow(m.flags, m.cap_flags, slab_pts, s)) ||
+  if ((!src_field && (st = grow(m.slab, m.cap_slab, slab_pts, s))) || (st = grow(m.flags, m.cap_flags, slab_pts, s)) ||
       (st = grow(m.emask, m.cap_emask, own_pts, s)) || (st = grow(m.cnt, m.cap_cnt, own_pts, s)) ||
       (st = grow(m.active, m.cap_active, own_pts, s)) || (st = grow(m.tri_off, m.cap_toff, own_pts, s)) ||
       (st = grow(m.blk, m.cap_blk, (int64_t)nblocks(own_pts), s)) ||
@@ -944,14 +1038,17 @@ qj_status nlmc_run(qj_workspace* ws, const MapArgs& a0, const qj_rational* f, do
     const int64_t s0 = z0 > 0 ? z0 - 1 : 0;
     const int64_t s1 = z1 + 1 < Z - 1 ? z1 + 1 : Z - 1;
     const int64_t ns = s1 - s0 + 1;
-    // 1. slab of log|q|
-    MapArgs sa = a;
-    sa.res[2] = (int32_t)ns;
-    sa.zoff = (int32_t)s0;
-    // the Lagrange order has no FP32 path: its slabs are FP64
-    if ((st = launch_field(ws, sa, f, (a.flags & QJ_MAP_LAGRANGE) ? QJ_FP64 : prec, band, m.slab, s))) return st;
+    // 1. slab of log|q| (or the caller's field, TRIANGLE_MESH.cpp:152-215)
+    const double* slab = src_field ? src_field + s0 * XY : m.slab;
+    if (!src_field) {
+      MapArgs sa = a;
+      sa.res[2] = (int32_t)ns;
+      sa.zoff = (int32_t)s0;
+      // the Lagrange order has no FP32 path: its slabs are FP64
+      if ((st = launch_field(ws, sa, f, (a.flags & QJ_MAP_LAGRANGE) ? QJ_FP64 : prec, band, m.slab, s))) return st;
+    }
     // 2. cases
-    hipLaunchKernelGGL(k_nl_flags, dim3(nblocks(XY * ns)), dim3(kMcBlock), 0, s, m.slab, X, Y, Z, s0, ns, m.flags);
+    hipLaunchKernelGGL(k_nl_flags, dim3(nblocks(XY * ns)), dim3(kMcBlock), 0, s, slab, X, Y, Z, s0, ns, m.flags);
     QJ_HIP(hipGetLastError());
     // 3. edge masks of points [z0, z1] (z1 only when it exists)
     const int64_t np = (z1 < Z ? z1 + 1 : Z) - z0;

@@ -1,17 +1,23 @@
 // Request loop: continuous batching of field solves on one GPU.
 //
 // Requests (any grid size, any root count up to max_roots) are copied on
-// submit and wait in a pending list. A scheduler thread admits them while
-// their HBM footprint (field + root records) fits a preallocated arena, splits
-// admitted requests into fixed-size voxel chunks and packs chunks of several
-// requests into one launch, shortest remaining work first. Each launch is a
-// hipGraph captured once per in-flight slot and set of FP32 variants present
-// ([descriptor upload, FP32 bulk per variant, FP64 band] or [descriptor
-// upload, FP64]) and replayed with new descriptors;
-// requests that arrive while launches are on the GPU join the next launch.
-// Fields stay resident in the arena until completion (or until released when
-// the caller asked to keep them). The reference has no serving path; its
-// analogue is one OPTIMIZE_3D::computeLogScaledPowerRationalMap call per solve
+// submit and wait in a pending list; past max_pending_work of admitted and
+// waiting work a submission is refused (QJ_ERR_BUSY). A scheduler thread
+// admits them while their HBM footprint (field + root records) fits a
+// preallocated arena, smallest first, splits admitted requests into
+// fixed-size voxel chunks and packs chunks of several requests into one
+// launch, shortest remaining work first; a request older than aging_s goes
+// before all others in both, oldest first, so large requests are not starved.
+// Each launch is a hipGraph captured once per in-flight slot and set of FP32
+// variants present ([descriptor fetch, FP32 bulk per variant, FP64 band] or
+// [descriptor fetch, FP64]) and replayed with new descriptors, which the
+// graph's first kernel reads from coherent pinned host memory; requests that
+// arrive while launches are on the GPU join the next launch. A finished
+// field gets its sampled FP64 cross-check (check_samples) on the server's own
+// stream before its completion is posted. Fields stay resident in the arena
+// until completion (or until released when the caller asked to keep them).
+// The reference has no serving path; its analogue is one
+// OPTIMIZE_3D::computeLogScaledPowerRationalMap call per solve
 // (OPTIMIZE_3D.cpp:1272-1370).
 #include <math.h>
 #include <string.h>
@@ -98,6 +104,10 @@ struct Req {
   int nt = 0, nb = 0;
   double sct = 1, scb = 1;
   int64_t voxels = 0, next = 0, chunks = 0, chunks_done = 0;
+  double work = 0;       // root-term iterations: voxels x max_iter x roots
+  double submitted = 0;  // server clock at submission (aging)
+  int check_slot = -1;   // per-result cross-check in flight
+  unsigned long long chk[CNT_N] = {};
   size_t off = 0, bytes = 0, off64 = 0, off32 = 0, off_field = 0;
   bool admitted = false;
   qj_status status = QJ_OK;
@@ -142,6 +152,15 @@ struct qj_server {
   std::vector<std::unique_ptr<qj::Req>> pending;              // not admitted yet
   std::unordered_map<uint64_t, std::unique_ptr<qj::Req>> live; // admitted, running
   std::unordered_map<uint64_t, std::unique_ptr<qj::Req>> kept; // finished, field kept
+  std::unordered_map<uint64_t, std::unique_ptr<qj::Req>> checking; // finished, cross-check in flight
+  // per-result cross-checks: kCheckSlots blocks of CNT_N counters on the
+  // device and in pinned host memory, one event each
+  static constexpr int kCheckSlots = 64;
+  unsigned long long* d_check = nullptr;
+  unsigned long long* h_check = nullptr;
+  hipEvent_t check_ev[kCheckSlots] = {};
+  std::vector<int> check_free;
+  double pending_work = 0;  // guarded by mu: work submitted and not yet completed
   std::vector<uint64_t> release_q;
   size_t slot_next = 0;
   qj_status fatal = QJ_OK;
@@ -163,9 +182,30 @@ static void finish(qj_server* s, Req* r, qj_status st) {
   c.done_time = r->done;
   c.voxels = r->voxels;
   c.d_field = (r->keep && st == QJ_OK && r->admitted) ? (const double*)(s->arena.base() + r->off_field) : nullptr;
+  c.checked = (int64_t)r->chk[CNT_CHECKED];
+  c.check_sign_flips = (int64_t)r->chk[CNT_CHECK_FLIPS];
+  c.check_class_mismatch = (int64_t)r->chk[CNT_CHECK_CLASS];
+  memcpy(&c.check_max_abs_dF, &r->chk[CNT_CHECK_MAXABS], sizeof(double));
+  memcpy(&c.check_max_rel_dF, &r->chk[CNT_CHECK_MAXREL], sizeof(double));
   std::lock_guard<std::mutex> lk(s->mu);
+  s->pending_work -= r->work;
+  if (s->pending_work < 0.5) s->pending_work = 0.0;  // rounding of the running sum
   s->completions.push_back(c);
   s->cv_done.notify_all();
+}
+
+static bool aged(const qj_server* s, const Req* r, double now) { return now - r->submitted > s->cfg.aging_s; }
+
+// Posts a finished request: its completion, then the arena space freed or
+// the field kept for the caller.
+static void retire(qj_server* s, std::unique_ptr<Req> r) {
+  finish(s, r.get(), r->status);
+  if (r->keep && r->status == QJ_OK) {
+    const uint64_t id = r->id;
+    s->kept[id] = std::move(r);
+  } else {
+    s->arena.free(r->off, r->bytes);
+  }
 }
 
 static bool try_admit(qj_server* s, Req* r) {
@@ -195,7 +235,11 @@ static int build_batch(qj_server* s, Slot& sl) {
   std::vector<Req*> order;
   for (auto& kv : s->live)
     if (kv.second->next < kv.second->voxels) order.push_back(kv.second.get());
-  std::sort(order.begin(), order.end(), [](const Req* a, const Req* b) {
+  const double now = s->now();
+  std::sort(order.begin(), order.end(), [s, now](const Req* a, const Req* b) {
+    const bool oa = aged(s, a, now), ob = aged(s, b, now);
+    if (oa != ob) return oa;
+    if (oa) return a->submitted < b->submitted;
     const int64_t ra = a->voxels - a->next, rb = b->voxels - b->next;
     return ra != rb ? ra < rb : a->arrival < b->arrival;
   });
@@ -251,13 +295,27 @@ static void complete_slot(qj_server* s, Slot& sl, qj_status st) {
     r->chunks_done += w.second;
     if (st != QJ_OK) r->status = st;
     if (r->chunks_done >= r->chunks) {
-      finish(s, r, r->status);
-      if (r->keep && r->status == QJ_OK) {
-        s->kept[r->id] = std::move(it->second);
-      } else {
-        s->arena.free(r->off, r->bytes);
-      }
+      std::unique_ptr<Req> own = std::move(it->second);
       s->live.erase(it);
+      if (r->status == QJ_OK && s->cfg.check_samples > 0 && !s->check_free.empty()) {
+        // the field is complete (every launch that held its chunks has
+        // finished): its cross-check runs on the server's stream
+        const int slot = s->check_free.back();
+        unsigned long long* d = s->d_check + (size_t)slot * CNT_N;
+        char* b = s->arena.base();
+        qj_status cs = launch_check_one(r->args, (const RootRec64*)(b + r->off64), (const double*)(b + r->off_field),
+                                        s->cfg.check_samples, d, s->stream);
+        if (cs == QJ_OK &&
+            hipMemcpyAsync(s->h_check + (size_t)slot * CNT_N, d, sizeof(unsigned long long) * CNT_N,
+                           hipMemcpyDeviceToHost, s->stream) == hipSuccess &&
+            hipEventRecord(s->check_ev[slot], s->stream) == hipSuccess) {
+          s->check_free.pop_back();
+          r->check_slot = slot;
+          s->checking[r->id] = std::move(own);
+          continue;
+        }
+      }
+      retire(s, std::move(own));
     }
   }
   sl.work.clear();
@@ -283,28 +341,50 @@ static void worker_main(qj_server* s) {
       }
       s->release_q.clear();
     }
-    // 2. admit, smallest footprint first, while the arena has room
+    // 2. admit while the arena has room: requests older than aging_s first,
+    //    oldest first (and nothing after an aged request that does not fit, so
+    //    the space it needs frees up), then smallest footprint first
     if (!s->pending.empty()) {
+      const double now = s->now();
       std::stable_sort(s->pending.begin(), s->pending.end(),
-                       [](const std::unique_ptr<Req>& a, const std::unique_ptr<Req>& b) { return a->bytes < b->bytes; });
+                       [s, now](const std::unique_ptr<Req>& a, const std::unique_ptr<Req>& b) {
+                         const bool oa = aged(s, a.get(), now), ob = aged(s, b.get(), now);
+                         if (oa != ob) return oa;
+                         if (oa) return a->submitted < b->submitted;
+                         return a->bytes < b->bytes;
+                       });
       std::vector<std::unique_ptr<Req>> keep;
+      bool blocked = false;
       for (auto& r : s->pending) {
         if (r->bytes > s->arena.size()) {
           finish(s, r.get(), QJ_ERR_CAPACITY);
           continue;
         }
-        if (try_admit(s, r.get())) {
+        if (!blocked && try_admit(s, r.get())) {
           const uint64_t id = r->id;
           s->live[id] = std::move(r);
         } else {
+          blocked |= aged(s, r.get(), now);
           keep.push_back(std::move(r));
         }
       }
       s->pending.swap(keep);
     }
-    // 3. retire finished launches
+    // 3. retire finished launches, then fields whose cross-check is done
     for (Slot& sl : s->slots)
       if (sl.busy && hipEventQuery(sl.done) == hipSuccess) complete_slot(s, sl, QJ_OK);
+    for (auto it = s->checking.begin(); it != s->checking.end();) {
+      Req* r = it->second.get();
+      if (hipEventQuery(s->check_ev[r->check_slot]) != hipSuccess) {
+        ++it;
+        continue;
+      }
+      memcpy(r->chk, s->h_check + (size_t)r->check_slot * CNT_N, sizeof(r->chk));
+      s->check_free.push_back(r->check_slot);
+      std::unique_ptr<Req> own = std::move(it->second);
+      it = s->checking.erase(it);
+      retire(s, std::move(own));
+    }
     // 4. launch while a slot is free and there is work
     bool launched = false;
     bool has_work = false;
@@ -327,7 +407,7 @@ static void worker_main(qj_server* s) {
       }
     }
     if (launched) continue;
-    bool any_busy = false;
+    bool any_busy = !s->checking.empty();
     for (Slot& sl : s->slots) any_busy |= sl.busy;
     if (s->stop.load() && !any_busy && s->live.empty()) {
       std::lock_guard<std::mutex> lk(s->mu);
@@ -349,7 +429,8 @@ static void worker_main(qj_server* s) {
 
 static qj_status capture_slot(qj_server* s, Slot& sl) {
   const int nch = s->cfg.chunks_per_launch;
-  QJ_HIP(hipHostMalloc((void**)&sl.h_desc, sizeof(ChunkDesc) * nch, hipHostMallocDefault));
+  // coherent: the graph's first kernel reads the descriptors over the bus
+  QJ_HIP(hipHostMalloc((void**)&sl.h_desc, sizeof(ChunkDesc) * nch, hipHostMallocCoherent));
   memset(sl.h_desc, 0, sizeof(ChunkDesc) * nch);
   QJ_HIP(hipMalloc(&sl.d_desc, sizeof(ChunkDesc) * nch));
   QJ_HIP(hipMemset(sl.d_desc, 0, sizeof(ChunkDesc) * nch));
@@ -365,16 +446,14 @@ static qj_status capture_slot(qj_server* s, Slot& sl) {
   for (unsigned v = first; v <= last; v++) {
     hipGraph_t g;
     QJ_HIP(hipStreamBeginCapture(sl.stream, hipStreamCaptureModeThreadLocal));
-    hipError_t e = hipMemcpyAsync(sl.d_desc, sl.h_desc, sizeof(ChunkDesc) * nch, hipMemcpyHostToDevice, sl.stream);
-    qj_status st = QJ_OK;
-    if (e == hipSuccess)
+    qj_status st = launch_desc_fetch(sl.h_desc, sl.d_desc, nch, sl.stream);
+    if (st == QJ_OK)
       st = launch_batch(sl.d_desc, nch, s->cfg.chunk_voxels, (qj_precision)s->cfg.precision, sl.d_band,
                         sl.d_band_count, v, sl.stream);
     hipError_t e2 = hipStreamEndCapture(sl.stream, &g);
-    if (e != hipSuccess) return hip_fail(e, "capture memcpy");
     if (st) return st;
     if (e2 != hipSuccess) return hip_fail(e2, "hipStreamEndCapture");
-    e = hipGraphInstantiate(&sl.exec[v], g, nullptr, nullptr, 0);
+    const hipError_t e = hipGraphInstantiate(&sl.exec[v], g, nullptr, nullptr, 0);
     (void)hipGraphDestroy(g);
     if (e != hipSuccess) return hip_fail(e, "hipGraphInstantiate");
   }
@@ -394,6 +473,11 @@ static void destroy_server(qj_server* s) {
     if (sl.stream) (void)hipStreamDestroy(sl.stream);
   }
   s->arena.release();
+  if (s->stream) (void)hipStreamSynchronize(s->stream);
+  for (hipEvent_t e : s->check_ev)
+    if (e) (void)hipEventDestroy(e);
+  if (s->d_check) (void)hipFree(s->d_check);
+  if (s->h_check) (void)hipHostFree(s->h_check);
   if (s->stream) (void)hipStreamDestroy(s->stream);
   delete s;
 }
@@ -417,6 +501,12 @@ qj_status qj_server_create(const qj_server_config* cfg, qj_server** out) {
   if (c.inflight_launches <= 0) c.inflight_launches = 2;
   if (c.launch_work <= 0) c.launch_work = 3e9;
   if (c.band <= 0) c.band = 0.05;
+  if (c.aging_s <= 0) c.aging_s = 0.25;
+  if (c.max_pending_work < 0 || c.check_samples < 0 || c.check_samples > (1 << 24)) {
+    delete s;
+    set_error("qj_server_create: max_pending_work must be >= 0 and check_samples in [0, 2^24]");
+    return QJ_ERR_INVALID;
+  }
   if (c.precision != QJ_FP64 && c.precision != QJ_FP32_BAND) {
     delete s;
     set_error("qj_server_create: unknown precision");
@@ -437,6 +527,20 @@ qj_status qj_server_create(const qj_server_config* cfg, qj_server** out) {
     destroy_server(s);
     set_error("qj_server_create: cannot allocate %lld bytes of HBM", (long long)c.hbm_budget_bytes);
     return QJ_ERR_NOMEM;
+  }
+  if (hipMalloc(&s->d_check, sizeof(unsigned long long) * CNT_N * qj_server::kCheckSlots) != hipSuccess ||
+      hipHostMalloc((void**)&s->h_check, sizeof(unsigned long long) * CNT_N * qj_server::kCheckSlots,
+                    hipHostMallocDefault) != hipSuccess) {
+    destroy_server(s);
+    set_error("qj_server_create: cannot allocate the cross-check buffers");
+    return QJ_ERR_NOMEM;
+  }
+  for (int k = 0; k < qj_server::kCheckSlots; k++) {
+    if (hipEventCreateWithFlags(&s->check_ev[k], hipEventDisableTiming) != hipSuccess) {
+      destroy_server(s);
+      return hip_fail(hipErrorOutOfMemory, "qj_server_create: check events");
+    }
+    s->check_free.push_back(k);
   }
   s->slots.resize(c.inflight_launches);
   for (Slot& sl : s->slots) {
@@ -516,10 +620,19 @@ qj_status qj_server_submit(qj_server* s, const qj_request* req, uint64_t* id) {
   r->chunks = (r->voxels + s->cfg.chunk_voxels - 1) / s->cfg.chunk_voxels;
   r->bytes = (((size_t)r->voxels * sizeof(double) + 255) & ~(size_t)255) +
              ((r->h64.size() * sizeof(RootRec64) + 255) & ~(size_t)255) + r->blk32.size();
-  r->id = s->next_id.fetch_add(1);
-  if (id) *id = r->id;
+  r->work = (double)r->voxels * (double)r->params.max_iter * (double)(r->nt + r->nb);
+  r->submitted = s->now();
   {
     std::lock_guard<std::mutex> lk(s->mu);
+    if (s->cfg.max_pending_work > 0 && s->pending_work > 0 &&
+        s->pending_work + r->work > s->cfg.max_pending_work) {
+      set_error("qj_server_submit: refused, %.3g root-term iterations pending (limit %.3g)", s->pending_work,
+                s->cfg.max_pending_work);
+      return QJ_ERR_BUSY;
+    }
+    s->pending_work += r->work;
+    r->id = s->next_id.fetch_add(1);
+    if (id) *id = r->id;
     s->incoming.push_back(std::move(r));
   }
   s->cv_submit.notify_one();

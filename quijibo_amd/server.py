@@ -12,9 +12,11 @@ from ._lib import FP32_BAND, MAP_BLACK_HOLE, check
 
 class Server:
     def __init__(self, hbm_budget_bytes=8 << 30, chunk_voxels=65536, chunks_per_launch=512, max_roots=256,
-                 inflight_launches=2, precision=FP32_BAND, band=0.05, device=0, launch_work=0.0):
+                 inflight_launches=2, precision=FP32_BAND, band=0.05, device=0, launch_work=0.0,
+                 max_pending_work=0.0, aging_s=0.0, check_samples=0):
         cfg = _lib.ServerConfig(int(hbm_budget_bytes), int(chunk_voxels), int(chunks_per_launch), int(max_roots),
-                                int(inflight_launches), int(precision), float(band), int(device), float(launch_work))
+                                int(inflight_launches), int(precision), float(band), int(device), float(launch_work),
+                                float(max_pending_work), float(aging_s), int(check_samples))
         h = ctypes.c_void_p()
         check(_lib.load().qj_server_create(ctypes.byref(cfg), ctypes.byref(h)))
         self.handle = h
@@ -24,7 +26,8 @@ class Server:
         return _lib.load().qj_server_now(self.handle)
 
     def submit(self, w, keep_field=False, arrival_time=None, flags=MAP_BLACK_HOLE):
-        """Submit a workloads.Workload; returns the request id."""
+        """Submit a workloads.Workload; returns the request id. Raises QJError
+        with status 7 (QJ_ERR_BUSY) when the server refuses it."""
         tr = np.ascontiguousarray(w.top.roots, dtype=np.float64)
         tp = np.ascontiguousarray(w.top.powers, dtype=np.float64)
         br = np.ascontiguousarray(w.bottom.roots, dtype=np.float64).reshape(-1, 4)
@@ -86,14 +89,16 @@ class Server:
             pass
 
 
-def replay(server, timed_requests, log=None):
-    """Submit (t, workload) pairs at their offsets on the server clock; return completions.
+def replay(server, timed_requests, log=None, flags=MAP_BLACK_HOLE):
+    """Submit (t, workload) pairs at their offsets on the server clock; return
+    (completions, id -> workload, start, refused count).
 
     Latency of a request = done_time - arrival_time, both on the server clock;
     arrival_time is the scheduled arrival, so submission delay counts against it."""
     start = server.now() + 0.05
     ids = {}
     done = []
+    refused = 0
     for i, (t, w) in enumerate(timed_requests):
         target = start + t
         while True:
@@ -102,16 +107,22 @@ def replay(server, timed_requests, log=None):
                 break
             done.extend(server.poll(256, 0.0))
             time.sleep(min(0.0005, target - now))
-        rid = server.submit(w, keep_field=False, arrival_time=target)
+        try:
+            rid = server.submit(w, keep_field=False, arrival_time=target, flags=flags)
+        except _lib.QJError as e:
+            if e.status != 7:
+                raise
+            refused += 1
+            continue
         ids[rid] = w
         if log and i % 200 == 0:
             log("submitted %d/%d at t=%.2fs, completed %d" % (i, len(timed_requests), server.now() - start, len(done)))
-    while len(done) < len(timed_requests):
+    while len(done) < len(ids):
         got = server.poll(256, 0.1)
         done.extend(got)
         if log and got and len(done) % 200 < len(got):
             log("completed %d/%d at t=%.2fs" % (len(done), len(timed_requests), server.now() - start))
-    return done, ids, start
+    return done, ids, start, refused
 
 
 def latency_summary(done, ids):

@@ -111,3 +111,41 @@ def test_c3_full_tile_mesh_is_bit_exact(gpu):
     print("c3 tile: %d vertices %d triangles" % (len(v), len(t)), flush=True)
     assert len(t) > 10000
     assert_same_mesh(v, t, ov, ot)  # vertices on edges with an infinite end are NaN on both sides
+
+
+def test_tiles_in_one_call_match_single_calls_and_the_oracle(gpu):
+    """qj_mc_run_tiles: several fields of different sizes meshed with one
+    read-back; each field's part equals qj_mc_run on it alone and the oracle's
+    mesh. A fresh workspace whose first guess of the buffer sizes is too small
+    (random signs: about 1.5 vertices per point) meshes again at the right size."""
+    rng = np.random.default_rng(4)
+    specs = [((33, 29, 31), (0.1, 0.0, -0.2), (2.0, 1.8, 1.9)), ((40, 40, 40), (0, 0, 0), (3, 3, 3)),
+             ((17, 23, 9), (0.3, -0.1, 0.0), (1.0, 1.5, 0.7)), ((2, 2, 2), (0, 0, 0), (1, 1, 1))]
+    fields, grids, hosts = [], [], []
+    for k, (res, c, L) in enumerate(specs):
+        if k == 1:
+            ax = -1.5 + (np.arange(40) + 0.5) * 3.0 / 40
+            Z, Y, X = np.meshgrid(ax, ax, ax, indexing="ij")
+            f = (0.9 - np.sqrt(X ** 2 + (Y - 0.1) ** 2 + Z ** 2)).reshape(-1)
+        else:
+            f = rng.normal(size=int(np.prod(res)))
+        hosts.append(f)
+        fields.append(torch.from_numpy(np.ascontiguousarray(f)).cuda())
+        grids.append(api.make_grid(res, c, L))
+    ws = api.Workspace()
+    v, t, counts = api.marching_cubes_tiles(fields, grids, workspace=ws)
+    v, t = v.cpu().numpy(), t.cpu().numpy()
+    assert counts[0][0] > np.prod(specs[0][0]) // 4  # more vertices than the first guess
+    ov0 = ot0 = 0
+    for k, (res, c, L) in enumerate(specs):
+        nv, nt = counts[k]
+        sv, st = api.marching_cubes(fields[k], grids[k], workspace=api.Workspace())
+        assert np.array_equal(v[ov0:ov0 + nv], sv) and np.array_equal(t[ot0:ot0 + nt], st), k
+        ov, ot = O.marching_cubes(O.make_grid(res, c, L), hosts[k])
+        assert_same_mesh(sv, st, ov, ot)
+        ov0 += nv
+        ot0 += nt
+    assert ov0 == len(v) and ot0 == len(t)
+    # the same workspace again: buffers now large enough, one pass
+    v2, t2, counts2 = api.marching_cubes_tiles(fields, grids, workspace=ws)
+    assert counts2 == counts and torch.equal(v2.cpu(), torch.from_numpy(v)) and np.array_equal(t2.cpu().numpy(), t)

@@ -168,3 +168,32 @@ def test_lagrange_points_eval_matches_oracle(gpu, cfg):
                             O.poly(w.bottom.roots, w.bottom.powers * 1.1),
                             O.make_params(w.max_iter, 2000.0, w.exp_scaling, w.slice, O.BLACK_HOLE))
         assert np.max(np.abs(std[fin] - ref[fin])) > 1e-3
+
+
+@pytest.mark.parametrize("cfg", ["c1", "c2"])
+def test_nonlinear_mesher_on_a_field(gpu, cfg):
+    """TRIANGLE_MESH(field, top, bottom, c, maxIter, slice, iso): cases from the
+    given field, vertices by bisection on the map; against the oracle's
+    restatement of the same constructor on the same field."""
+    w = W.config1(8) if cfg == "c1" else W.config2(8)
+    res = (38, 35, 33)
+    g, r, p = _setup(w, res)
+    field = api.field_eval(g, r, p, FP64)
+    # a field of its own: the map's field with a few signs flipped in one corner,
+    # so the cases are the field's and not the map's
+    f = field.clone()
+    f.view(res[2], res[1], res[0])[:6, :6, :6] *= -1.0
+    v, t = api.nonlinear_mesh_field(f, g, r, p, 0.0, workspace=api.Workspace(), allow_inconsistent=True)
+    og = O.make_grid(res, w.center, w.lengths)
+    ov, ot, bad = O.nonlinear_mc_field(og, f.cpu().numpy(), O.poly(w.top.roots, w.top.powers),
+                                       O.poly(w.bottom.roots, w.bottom.powers),
+                                       O.make_params(w.max_iter, 2000.0, w.exp_scaling, w.slice, O.BLACK_HOLE))
+    print("field mesher %s: %d vertices %d triangles, %d edges not straddling" % (cfg, len(v), len(t), bad),
+          flush=True)
+    assert np.array_equal(t, ot) and len(t) > 100
+    d = np.max(np.abs(v - ov), axis=1)
+    assert (d > 1e-12).mean() <= 0.005 and np.all(d <= 3.0 / min(res) / 2 ** 6 * 1.0001)
+    # on the map's own field the constructor gives the slab mesher's mesh
+    v2, t2 = api.nonlinear_mesh_field(field, g, r, p, 0.0, workspace=api.Workspace())
+    v3, t3 = api.nonlinear_mesh(g, r, p, 0.0, FP64, 0.05, 64, workspace=api.Workspace())
+    assert np.array_equal(t2, t3) and np.array_equal(v2, v3)

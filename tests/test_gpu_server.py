@@ -140,3 +140,95 @@ def test_lagrange_request_matches_the_direct_evaluation(gpu):
     std = _direct(w, FP64)
     fin = np.isfinite(ref) & np.isfinite(std)
     assert np.max(np.abs(std[fin] - ref[fin])) > 1e-3  # the Lagrange order is a different map
+
+
+def test_refusal_aging_and_per_result_check(gpu):
+    """max_pending_work refuses a submission past the limit (QJ_ERR_BUSY) but
+    never one that arrives at an idle server; every completion carries its
+    sampled FP64 cross-check; an old request is admitted before smaller new
+    ones."""
+    from quijibo_amd._lib import QJError
+    w = W.config2(64)
+    work = 64 ** 3 * w.max_iter * (w.top.n + w.bottom.n)
+    srv = S.Server(hbm_budget_bytes=256 << 20, chunk_voxels=16384, chunks_per_launch=8,
+                   max_pending_work=1.5 * work, check_samples=700)
+    big = W.config2(96)  # more work than the limit alone: accepted at an idle server
+    a = srv.submit(big)
+    with pytest.raises(QJError) as ei:
+        srv.submit(w)
+    assert ei.value.status == 7
+    done = srv.wait_all(1, timeout=60)
+    assert len(done) == 1 and done[0].id == a and done[0].status == 0
+    c = done[0]
+    assert c.checked == 700 and c.check_sign_flips == 0 and c.check_class_mismatch == 0
+    assert 0 < c.check_max_abs_dF < 0.05 and c.check_max_rel_dF <= c.check_max_abs_dF
+    b = srv.submit(w)  # idle again: accepted
+    done = srv.wait_all(1, timeout=60)
+    assert done[0].id == b and done[0].status == 0 and done[0].checked == 700
+    srv.close()
+
+
+def test_aged_request_is_admitted_first(gpu):
+    """Once a waiting request is older than aging_s nothing behind it is
+    admitted before it: every small request admitted later than the big one's
+    aging time (plus a margin for one admission pass) is admitted after it, and
+    the big one is admitted while small ones keep arriving (no starvation)."""
+    import time
+    small = W.config1(32)
+    big = W.config1(96)  # 6.75 MiB field: fits the 8 MiB budget only beside a few small ones
+    aging = 0.05
+    srv = S.Server(hbm_budget_bytes=8 << 20, chunk_voxels=16384, chunks_per_launch=4, aging_s=aging)
+    ids_small = [srv.submit(small) for _ in range(40)]  # about 10 MiB: more than the budget
+    bid = srv.submit(big)
+    t_end = srv.now() + 0.3
+    while srv.now() < t_end:
+        ids_small.append(srv.submit(small))
+        time.sleep(0.002)
+    done = srv.wait_all(len(ids_small) + 1, timeout=120)
+    srv.close()
+    assert len(done) == len(ids_small) + 1 and all(c.status == 0 for c in done)
+    by = {c.id: c for c in done}
+    b = by[bid]
+    late = [by[i] for i in ids_small if by[i].admit_time > b.arrival_time + aging + 0.02]
+    print("big waited %.1f ms; %d small requests admitted after its aging time" %
+          (1e3 * (b.admit_time - b.arrival_time), len(late)), flush=True)
+    assert all(c.admit_time >= b.admit_time for c in late)
+    assert b.admit_time <= b.arrival_time + aging + 0.25
+
+
+def test_config4_slice_against_the_oracle(gpu):
+    """The first requests of the serving stream with their grids shrunk (edge /
+    16) through the request loop, each result against the oracle: FP64 range
+    against the FP64 oracle (zero sign flips, FP64 tolerance in the band) and
+    the extended range against the long double oracle."""
+    import oracle as O
+    from quijibo_amd._lib import MAP_EXTENDED_RANGE
+    reqs = [w for _, w in W.config4_requests(n=10)]
+    for w in reqs:
+        e = max(4, w.res[0] // 16)
+        w.res = (e, e, e)
+    srv = S.Server(hbm_budget_bytes=64 << 20, chunk_voxels=4096, chunks_per_launch=16, check_samples=256)
+    sent = []
+    for k, w in enumerate(reqs):
+        flags = MAP_BLACK_HOLE | (MAP_EXTENDED_RANGE if k % 2 else 0)
+        sent.append((srv.submit(w, keep_field=True, flags=flags), w, flags))
+    done = {c.id: c for c in srv.wait_all(len(sent), timeout=120)}
+    for rid, w, flags in sent:
+        c = done[rid]
+        assert c.status == 0
+        got = srv.fetch(c, device=False)
+        srv.release(rid)
+        oflags = O.BLACK_HOLE | (O.EXTENDED if flags & MAP_EXTENDED_RANGE else 0)
+        ref = O.field_map(O.make_grid(w.res, w.center, w.lengths), O.poly(w.top.roots, w.top.powers),
+                          O.poly(w.bottom.roots, w.bottom.powers),
+                          O.make_params(w.max_iter, w.escape, w.exp_scaling, w.slice, oflags))
+        assert np.array_equal(np.isfinite(got), np.isfinite(ref)) and np.array_equal(np.isnan(got), np.isnan(ref))
+        flips = int(np.count_nonzero((got.astype(np.float32) > 0) != (ref.astype(np.float32) > 0)))
+        inband = np.abs(ref) < 0.04
+        print("req %d (%d roots, %d^3, ext %d): flips %d, band %d, checked %d" %
+              (rid, w.top.n + w.bottom.n, w.res[0], bool(flags & MAP_EXTENDED_RANGE), flips, inband.sum(),
+               c.checked), flush=True)
+        assert flips == 0
+        assert np.all(np.abs(got[inband] - ref[inband]) <= 1e-8 * np.maximum(1.0, np.abs(ref[inband])))
+        assert c.checked == min(256, got.size) and c.check_sign_flips == 0
+    srv.close()
