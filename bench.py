@@ -87,6 +87,7 @@ def main():
     ap.add_argument("--serve-rate", type=float, default=200.0)
     ap.add_argument("--serve-budget-gib", type=float, default=64.0)
     ap.add_argument("--probe", action="store_true", help="counter probe: two field launches and a 1 GiB copy")
+    ap.add_argument("--probe-precision", default="fp32_band", choices=["fp32_band", "fp64", "fp64_exact"])
     ap.add_argument("--mc-probe", action="store_true",
                     help="counter probe: marching cubes of one 256^3 torus tile (twice), then a 1 GiB copy")
     ap.add_argument("--serve-only", action="store_true", help="run only the serving stream")
@@ -105,7 +106,7 @@ def main():
 
     import torch
     from quijibo_amd import api, workloads as W
-    from quijibo_amd._lib import FP32_BAND, FP64, MAP_BLACK_HOLE
+    from quijibo_amd._lib import FP32_BAND, FP64, FP64_EXACT, MAP_BLACK_HOLE
 
     torch.cuda.set_device(0)
     w = W.headline(args.res)
@@ -131,9 +132,11 @@ def main():
         # Counter probe for rocprofv3 --pmc: one FP32+band field after a warm-up, then a
         # plain 1 GiB device-to-device copy whose FETCH_SIZE / WRITE_SIZE readings give
         # the scale factor for the field kernel's readings.
+        prec = {"fp32_band": FP32_BAND, "fp64": FP64, "fp64_exact": FP64_EXACT}[args.probe_precision]
         for i in range(2):
-            log("probe: fp32+band field %d" % i)
-            api.field_eval(grid, r, params, FP32_BAND, args.band, out=out32, workspace=ws)
+            log("probe: %s field %d" % (args.probe_precision, i))
+            api.field_eval(grid, r, params, prec, args.band, out=out32 if prec == FP32_BAND else out64,
+                           workspace=ws)
             torch.cuda.synchronize()
         src = torch.ones(1 << 27, dtype=torch.float64, device="cuda")
         dst = torch.empty_like(src)
@@ -141,7 +144,8 @@ def main():
         log("probe: copy 1 GiB")
         dst.copy_(src)
         torch.cuda.synchronize()
-        print(json.dumps({"probe": "k_grid32 512^3 x2, then copy of %d bytes" % (8 << 27)}))
+        print(json.dumps({"probe": "%s field %d^3 x2, then copy of %d bytes" % (args.probe_precision, args.res,
+                                                                                 8 << 27)}))
         return
 
     # plain FP64 kernel
@@ -162,7 +166,6 @@ def main():
     # FP64 with exponents 1 and 2 as exact products: the same precision as the
     # plain FP64 kernel, the algorithm of the FP32 path, so the two ratios
     # below separate the algorithm's share of the speedup from precision's
-    from quijibo_amd._lib import FP64_EXACT
     out64x = torch.empty(w.voxels, dtype=torch.float64, device="cuda")
     log("fp64 exact: warmup")
     api.field_eval(grid, r, params, FP64_EXACT, out=out64x, workspace=ws)
@@ -287,10 +290,11 @@ def main():
 def serve_stream(args, log):
     """Config 4: Poisson arrivals at serve_rate, grid edge in {64,128,256,512}, 8-256 roots
     (half top, half bottom, Gaussian sigma 0.7), maxIter 3-5, through the request loop, in the
-    reference's default-build range (QJ_MAP_EXTENDED_RANGE, Real = long double). Submissions
-    beyond serve_pending_s of backlog (in seconds of the loop's measured root-term rate) are
-    refused; requests waiting longer than serve_aging_s go first; every result carries its
-    sampled FP64 cross-check."""
+    reference's default-build range (QJ_MAP_EXTENDED_RANGE, Real = long double). A submission
+    is refused when more than serve_pending_s of work (in seconds of the loop's measured
+    root-term rate) is queued ahead of it in shortest-first order; requests waiting longer
+    than serve_aging_s are admitted first and take up to a quarter of each launch; every result
+    carries its sampled FP64 cross-check."""
     from quijibo_amd import server as S, workloads as W
     from quijibo_amd._lib import FP32_BAND, MAP_BLACK_HOLE, MAP_EXTENDED_RANGE
     reqs = W.config4_requests(n=args.serve, rate=args.serve_rate)
@@ -324,7 +328,8 @@ def serve_stream(args, log):
             "error_statement": check,
             "config": "hbm budget %.0f GiB, 65536-voxel chunks, launches of <= 512 chunks and ~%s root-term "
                       "iterations, %s launches in flight, fp32+fp64 band, extended range, shortest-remaining-work "
-                      "first, aging %.2f s, refusal beyond %.1f s of backlog (%.2g root-term iterations)"
+                      "first, aging %.2f s (aged requests take up to a quarter of each launch), refusal when more "
+                      "than %.1f s of work (%.2g root-term iterations) is queued ahead in shortest-first order"
                       % (args.serve_budget_gib, ("%.1e" % args.serve_launch_work) if args.serve_launch_work else "3e9",
                          args.serve_inflight or 2, args.serve_aging_s, args.serve_pending_s,
                          args.serve_pending_s * rate)}
@@ -411,18 +416,18 @@ def gradients(args, log):
 
 def mc_probe(args, log):
     """Counter probe for marching cubes: the 256^3 tile (1, 1, 0) of the 1024^3 torus workload
-    (it holds part of the surface) evaluated FP32+band, meshed twice (a warm-up and the run the
+    (it holds part of the surface) evaluated FP32+band in the extended range, meshed twice (a warm-up and the run the
     counters are read from), then a plain 1 GiB device-to-device copy whose FETCH_SIZE /
     WRITE_SIZE readings scale the others."""
     import ctypes
     import torch
     from quijibo_amd import api, workloads as W
-    from quijibo_amd._lib import FP32_BAND, MAP_BLACK_HOLE
+    from quijibo_amd._lib import FP32_BAND, MAP_BLACK_HOLE, MAP_EXTENDED_RANGE
     w = W.config3(1024, 256)
     c, tl = W.tile_grid(w, 1, 1, 0, 256)
     g = api.make_grid((256, 256, 256), c, tl)
     r = api.Rational(api.POLYNOMIAL_4D(w.top.roots, w.top.powers), api.POLYNOMIAL_4D(w.bottom.roots, w.bottom.powers))
-    params = api.make_params(w.max_iter, w.escape, w.exp_scaling, w.slice, MAP_BLACK_HOLE)
+    params = api.make_params(w.max_iter, w.escape, w.exp_scaling, w.slice, MAP_BLACK_HOLE | MAP_EXTENDED_RANGE)
     ws = api.Workspace()
     fld = api.field_eval(g, r, params, FP32_BAND, args.band, workspace=ws)
     torch.cuda.synchronize()

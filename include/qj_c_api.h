@@ -20,6 +20,10 @@
  *     compute{Top,Bottom}BulkScalarGradientAP       src/solvers/OPTIMIZE_3D.cpp:3563, 3589, 8905, 10012, 9250, 10360
  *   OPTIMIZE_3D::computeLogRationalCenteredGradient (root-position gradient) src/solvers/OPTIMIZE_3D.cpp:2872-2951
  *   FIELD_3D::clampInfsToMean / clampNansToMean / setBorders  src/geometry/FIELD_3D.cpp:5182-5234, 5357-5382
+ *   FIELD_3D::fieldMin / fieldMax / invertedDistance / invertedDistanceNormalized
+ *                                                    src/geometry/FIELD_3D.cpp:2960-3032, 5241-5352
+ *   OPTIMIZE_3D::weightScore / inverseDistanceScore / computeMaximumScaledPowerRationalScore
+ *                                                    src/solvers/OPTIMIZE_3D.cpp:4484-4563
  */
 #ifndef QJ_C_API_H
 #define QJ_C_API_H
@@ -304,6 +308,41 @@ qj_status qj_field_clamp_nans_to_mean(qj_workspace* ws, double* d_field, int64_t
  * the grid take `value`. Enqueued on `stream`. */
 qj_status qj_field_set_borders(const qj_grid* grid, double value, double* d_field, void* stream);
 
+/* FIELD_3D::fieldMin / fieldMax (FIELD_3D.cpp:2960-2971, 3021-3032) of d_field
+ * (device, n >= 1 doubles), with the serial scan's behaviour: NaN cells are
+ * skipped unless cell 0 is NaN (then both are NaN), and of equal values the
+ * first cell's wins. Synchronizes `stream`; results on the host. */
+qj_status qj_field_min_max(qj_workspace* ws, const double* d_field, int64_t n, double* min_out, double* max_out,
+                           void* stream);
+
+/* FIELD_3D::invertedDistance (FIELD_3D.cpp:5241-5297) of a signed distance
+ * field d_dist on `grid` (device): the cells with a 26-neighbour of the other
+ * sign (outerShell + innerShell, :5070-5145) get 1, every other cell 1 + its
+ * 6-neighbour layer number (sweep after sweep, each reading the previous
+ * one), then 1 / number, negated inside (distance < 0) and quartered outside.
+ * normalized != 0: invertedDistanceNormalized (:5316-5352), the result over
+ * the sum of its absolute values, then the inside and the outside each scaled
+ * to absolute sum 0.5. abs_sum (host, may be NULL): the absolute sum of the
+ * result (OPTIMIZE_3D::read's _inverseFieldSum, OPTIMIZE_3D.cpp:3375-3378).
+ * Sums in double-double (the reference's serial sums differ by at most
+ * (n - 1) 2^-53 of the absolute sum). A grid where some cell can never be
+ * numbered (no sign change anywhere: the reference sweeps forever) returns
+ * QJ_ERR_NUMERIC. Synchronizes `stream`. */
+qj_status qj_field_inverted_distance(qj_workspace* ws, const qj_grid* grid, const double* d_dist, int32_t normalized,
+                                     double* d_out, double* abs_sum, void* stream);
+
+/* OPTIMIZE_3D::weightScore (OPTIMIZE_3D.cpp:4502-4517) and
+ * inverseDistanceScore (:4522-4563) for n_thresholds thresholds t (host
+ * array): scores[k] = sum over the cells of `grid` of
+ * -tanh(tanh_alpha (F - t_k)) w(cell centre), with w the trilinear lookup of
+ * the weight field (wgrid, d_weight; FIELD_3D.cpp:966-1029). max_score (host,
+ * may be NULL): computeMaximumScaledPowerRationalScore (:4484-4498), the sum
+ * of |w(cell centre)|. One launch for all thresholds (at most 65534); sums
+ * in double-double. Synchronizes `stream`. */
+qj_status qj_weight_scores(qj_workspace* ws, const qj_grid* grid, const double* d_field, const qj_grid* wgrid,
+                           const double* d_weight, double tanh_alpha, const double* thresholds, int32_t n_thresholds,
+                           double* scores, double* max_score, void* stream);
+
 /* ---- gradients ---- */
 
 /* One-iteration AP score and exponent gradients (max_iter must be 1, as the
@@ -357,11 +396,14 @@ typedef struct qj_server_config {
   int32_t device;
   double launch_work;         /* root-term iterations per launch (voxels x max_iter x roots), so every
                                  launch takes about the same GPU time (0 = 3e9, about 1.5 ms) */
-  double max_pending_work;    /* root-term iterations admitted or waiting beyond which qj_server_submit
-                                 refuses a request with QJ_ERR_BUSY (0 = no limit; a request arriving at an
-                                 idle server is always taken) */
-  double aging_s;             /* a request waiting longer than this many seconds is admitted and packed
-                                 before the others, oldest first (0 = 0.25) */
+  double max_pending_work;    /* qj_server_submit refuses a request with QJ_ERR_BUSY when the root-term
+                                 iterations queued ahead of it in shortest-first order (the outstanding
+                                 requests with no more work than it, its own included) exceed this
+                                 (0 = no limit; a request arriving at an idle server is always taken) */
+  double aging_s;             /* a request waiting longer than this many seconds is admitted before the
+                                 others (oldest first; while it does not fit, requests of a quarter of its
+                                 footprint or more wait behind it) and takes up to a quarter of each
+                                 launch (0 = 0.25) */
   int32_t check_samples;      /* per-result error statement: cells of every finished field re-evaluated with
                                  the reference method in FP64 and compared (qj_completion.checked ...; 0 = off) */
 } qj_server_config;
@@ -396,7 +438,8 @@ typedef struct qj_completion {
 qj_status qj_server_create(const qj_server_config* cfg, qj_server** out);
 double qj_server_now(qj_server* s); /* seconds on the server's monotonic clock */
 /* Copies the request (roots included); never blocks on the GPU. Returns
- * QJ_ERR_BUSY (no id, nothing queued) when max_pending_work would be exceeded. */
+ * QJ_ERR_BUSY (no id, nothing queued) when the work queued ahead of it would
+ * exceed max_pending_work. */
 qj_status qj_server_submit(qj_server* s, const qj_request* req, uint64_t* id);
 /* Up to max completions; waits up to timeout_s for at least one (0 = poll). */
 qj_status qj_server_poll(qj_server* s, qj_completion* out, int32_t max, int32_t* n_out, double timeout_s);

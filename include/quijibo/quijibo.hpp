@@ -12,6 +12,7 @@
 
 #include <stdio.h>
 
+#include <functional>
 #include <memory>
 #include <stdexcept>
 #include <string>
@@ -77,6 +78,14 @@ class POLYNOMIAL_4D {
   Real& powerScalar() { return _powerScalar; }
   Real powerScalar() const { return _powerScalar; }
 
+  // every root times alpha, all four components (POLYNOMIAL_4D.cpp:1184-1190)
+  POLYNOMIAL_4D& operator*=(const Real& alpha);
+  // roots moved by (v, 0) (POLYNOMIAL_4D.cpp:1195-1216); translateExceptFirst
+  // leaves root 0 where it is (:1220-1226)
+  POLYNOMIAL_4D& operator+=(const VEC3F& v);
+  POLYNOMIAL_4D& operator-=(const VEC3F& v);
+  void translateExceptFirst(const VEC3F& v);
+
   void addRoot(const QUATERNION& r, const Real& power = 1.0);
   void modifyRoot(int which, const QUATERNION& r) { _roots.at(which) = r; }
   void changePower(int which, const Real& p) { _rootPowers.at(which) = p; }
@@ -120,6 +129,9 @@ class FIELD_3D {
   Real dz() const { return _lengths[2] / _zRes; }
   const VEC3F& center() const { return _center; }
   const VEC3F& lengths() const { return _lengths; }
+  VEC3F& center() { return _center; }
+  // lengths times scale (FIELD_3D.cpp:5302-5311; the spacings follow)
+  void scaleLengths(const Real& scale);
   QUATERNION& rotation() { return _rotation; }
   const QUATERNION& rotation() const { return _rotation; }
   Real* data() { return _data.data(); }
@@ -144,6 +156,23 @@ class FIELD_3D {
   void clampInfsToMean();
   void clampNansToMean();
   void setBorders(const Real& value);
+
+  FIELD_3D& operator*=(const Real& alpha);  // every cell times alpha
+  // serial sums in index order (FIELD_3D.cpp:2820-2839), host side
+  Real sum() const;
+  Real absSum() const;
+  // FIELD_3D.cpp:2960-2971, 3021-3032 on the GPU: NaN cells are skipped
+  // unless cell 0 is NaN, ties keep the first cell
+  Real fieldMax() const;
+  Real fieldMin() const;
+  // Of a signed distance field, on the GPU (FIELD_3D.cpp:5241-5297,
+  // 5316-5352): 1 on the cells next to a sign change, then 1 over the
+  // 6-neighbour layer number, negated inside and quartered outside; the
+  // normalized version scales the inside and the outside to absolute sums
+  // of 0.5 each. A field without a sign change throws (the reference sweeps
+  // forever). absSum, when given, receives the result's absolute sum.
+  FIELD_3D invertedDistance() const;
+  FIELD_3D invertedDistanceNormalized(Real* absSum = nullptr) const;
 
  private:
   int _xRes = 0, _yRes = 0, _zRes = 0;
@@ -209,6 +238,8 @@ class OPTIMIZE_3D {
   FIELD_3D& fractal() { return _fractal; }
   FIELD_3D& weightField() { return _weightField; }
   FIELD_3D& distanceField() { return _distanceField; }
+  FIELD_3D& inverseDistanceField() { return _inverseDistanceField; }
+  Real inverseFieldSum() const { return _inverseFieldSum; }
   int& maxIterations() { return _maxIterations; }
   Real& expScaling() { return _expScaling; }
   Real& quaternionSlice() { return _quaternionSlice; }
@@ -226,6 +257,43 @@ class OPTIMIZE_3D {
   // OPTIMIZE_3D.cpp:1374-1568 (escape 20, origin skipped, overflowed cells re-passed)
   void computeRationalMapAP(const POLYNOMIAL_4D& top, const POLYNOMIAL_4D& bottom, FIELD_3D& fractal,
                             bool verbose = false);
+  // OPTIMIZE_3D.cpp:1572-1718 (escape 20, cells at the origin keep their
+  // value, no black-hole stop), FP64
+  void computeLogScaledPowerRationalMapCached(const POLYNOMIAL_4D& top, const POLYNOMIAL_4D& bottom,
+                                              FIELD_3D& fractal, bool verbose = false);
+  // computeLogPowerRationalMap(bool) -> computeLogPowerRationalMapCached
+  // (OPTIMIZE_3D.h:164, OPTIMIZE_3D.cpp:1165-1269): the same map with the
+  // power scalars left out (evaluatePowerFactored, POLYNOMIAL_4D.cpp:966-1003)
+  // and no origin test, FP64
+  void computeLogPowerRationalMap(bool verbose = false);
+
+  // Scores (OPTIMIZE_3D.cpp:3573-3585, 4484-4563), sums on the GPU:
+  // weightScore = sum over the cells of -tanh(alpha F) w(cell centre);
+  // inverseDistanceScore the same with F - threshold (inverseSum unused, as
+  // in the reference); the batched form scores many thresholds in one pass.
+  Real computeLogScaledPowerRationalScore();
+  Real computeMaximumScaledPowerRationalScore();
+  Real weightScore(const FIELD_3D& logFractal, const FIELD_3D& weightField);
+  Real inverseDistanceScore(const FIELD_3D& logFractal, const FIELD_3D& inverseDistanceField, const Real& inverseSum,
+                            const Real& threshold);
+  std::vector<Real> inverseDistanceScores(const FIELD_3D& logFractal, const FIELD_3D& inverseDistanceField,
+                                          const std::vector<Real>& thresholds);
+  // OPTIMIZE_3D.cpp:4694-4772: the unscaled one-iteration map at the
+  // fractal's resolution, clampInfsToMean, then four scans of 100 thresholds
+  // between the field's min and max for the lowest inverseDistanceScore
+  // against invertedDistance(distance field); expScaling becomes
+  // exp(-threshold). Returns the lowest score. A scan whose threshold stops
+  // advancing throws (the reference would loop forever).
+  Real estimateConformalRadius();
+
+  // OPTIMIZE_3D.cpp:3442-3471: resolution, centre and lengths from the
+  // distance field; the fractal becomes a zero field on that grid
+  void setScoreFields(const FIELD_3D& distanceField, const FIELD_3D& curvatureField);
+  void computeWeightField() { _weightField = _inverseDistanceField; }  // OPTIMIZE_3D.cpp:10840-10843
+  // OPTIMIZE_3D.cpp:4338-4394: the problem scaled about the origin / moved
+  void scaleEverything(const Real& scale);
+  void translateEverything(const VEC3F& translation);
+  void translateEverythingExceptFirst(const VEC3F& translation);
 
   // AP score and gradients at one iteration (OPTIMIZE_3D.cpp:3563, 3589, 8905, 10012, 9250, 10360).
   // One GPU pass computes all of them for the current state; the accessors
@@ -241,11 +309,17 @@ class OPTIMIZE_3D {
   // .o3d layout (OPTIMIZE_3D.cpp:3254-3346): int xRes, yRes, zRes; center;
   // lengths; top and bottom POLYNOMIAL_4D; double expScaling; int history
   // size (0, the history is never written); the fractal, distance and
-  // curvature FIELD_3Ds. read() restores these members; the weight field is
-  // the caller's to set (the reference derives it from the distance field,
-  // FIELD_3D.cpp:5316, outside the map/score path).
+  // curvature FIELD_3Ds. read() then does what the reference's read does
+  // (:3348-3388): setScoreFields(distance, distance) (so the curvature field
+  // becomes a copy of the distance field and the fractal a zero field on the
+  // distance field's grid) and, unless an inverse distance field is already
+  // set, inverseDistanceField = invertedDistanceNormalized(distance) with its
+  // absolute sum, then weightField = inverseDistanceField.
   void write(const std::string& filename) const;
   void read(const std::string& filename);
+  // the stored members only, without read()'s set-up (no GPU work): the
+  // .o3d layout as written
+  void readState(const std::string& filename);
   FIELD_3D& curvatureField() { return _curvatureField; }
   const VEC3F& center() const { return _center; }
   const VEC3F& lengths() const { return _lengths; }
@@ -255,13 +329,18 @@ class OPTIMIZE_3D {
 
  private:
   void sync_rational(const POLYNOMIAL_4D& top, const POLYNOMIAL_4D& bottom);
+  void scores_on_device(const FIELD_3D& logFractal, const FIELD_3D& weight, const std::vector<Real>& thresholds,
+                        std::vector<Real>* scores, Real* maxScore);
+  void map_into(const POLYNOMIAL_4D& top, const POLYNOMIAL_4D& bottom, FIELD_3D& fractal, const qj_map_params& p,
+                bool keepSkipped);
   void run_ap();
   void ensure_device(size_t cells);
 
   int _xRes = 0, _yRes = 0, _zRes = 0;
   VEC3F _center, _lengths{1, 1, 1};
   POLYNOMIAL_4D _top, _bottom;
-  FIELD_3D _fractal, _distanceField, _curvatureField, _weightField;
+  FIELD_3D _fractal, _distanceField, _curvatureField, _weightField, _inverseDistanceField;
+  Real _inverseFieldSum = 0;
   int _maxIterations = 1;
   Real _expScaling = 1.0;
   Real _quaternionSlice = 0.0;
@@ -337,5 +416,63 @@ class TAO_SOLVER {
   SOLVER_STATE _reason = CONTINUE_ITERATING;
   std::vector<Real> _history;
 };
+
+// The shape-optimizer driver of projects/toothOptimizer/toothOptimizer.cpp (a
+// program in the reference, a function here): read() has already set the
+// weight field; the set-up of its main() (:498-520) moves and scales the
+// problem to the fractal radius, multiplies the root powers by the initial
+// amplitude and resets both power scalars to 1; then optimizeSpecificResolution
+// (:385-457) runs for each resolution: at 25 the conformal radius estimate and
+// the fractal mesh, the initial and maximum scores, a BLMVM solve over
+// (log c, top scalar, bottom scalar) then one over every exponent
+// (FormScaledOnlyFunctionGradient / FormScaledFunctionGradient, :64-134, with
+// the GPU AP pass as the callback), a solve with the inside-reweighted weight
+// field (buildReweightedInsideField, :276-381), a last solve with the
+// original weights, and the state written to <path><outputFilename>_res_<res>.o3d.
+// monitor (:48-61) writes <path>output.o3d on every checkpointEvery-th BLMVM
+// iteration. The reweighting needs the signed distance field of the fractal's
+// mesh, which the reference gets from an external program (sdfGen on
+// temp.obj); here the caller supplies it (fractalDistance), and without it
+// the reweighted solve is skipped. One schedule runs at a time per process
+// (the callbacks reach the optimizer through process-wide state, as the
+// reference's globals do).
+struct ToothOptimizerSettings {
+  std::string path = "./temp/";                 // "path"
+  std::string outputFilename = "temp.o3d";      // "optimization output filename"
+  std::vector<int> resolutions{25, 50, 97};
+  Real fractalRadius = 4.0;                     // "fractal radius"
+  Real initialBulkScalarAmplitude = -1;         // "initial bulk scalar amplitude"
+  int maxTaoIterations = 2000;                  // TAO's default (the reference parses but never applies its own)
+  int checkpointEvery = 10;
+  bool prepare = true;                          // run main()'s set-up first
+  // the fractal mesh's signed distance on the grid of meshDistance (the
+  // optimizer's distance field), one value per cell
+  std::function<FIELD_3D(const TRIANGLE_MESH& fractalMesh, const FIELD_3D& meshDistance)> fractalDistance;
+};
+
+struct ToothSolveRecord {
+  std::string name;  // "scaled-only", "scaled", "reweighted", "unweighted"
+  int iterations = 0, reason = 0, evaluations = 0;
+  Real value = 0, maxScore = 0;
+  std::vector<Real> solution, history;
+};
+
+struct ToothResolutionRecord {
+  int res = 0;
+  bool conformal = false;
+  Real conformalScore = 0, conformalExpScaling = 0;
+  int64_t fractalVertices = 0, fractalTriangles = 0;      // computeFractal at resolution 25
+  int64_t reweightVertices = 0, reweightTriangles = 0;    // the mesh the reweighting starts from
+  Real initialScore = 0, maxScore = 0;
+  std::vector<ToothSolveRecord> solves;
+  std::string output;
+};
+
+struct ToothOptimizerLog {
+  std::vector<ToothResolutionRecord> resolutions;
+  int checkpoints = 0;
+};
+
+void optimizeTooth(OPTIMIZE_3D& optimize3D, const ToothOptimizerSettings& settings, ToothOptimizerLog* log = nullptr);
 
 }  // namespace quijibo

@@ -92,6 +92,12 @@ def lib():
         L.qo_clamp_nans_to_mean.argtypes = [dp, ctypes.c_int64]
         L.qo_clamp_nans_to_mean.restype = ctypes.c_double
         L.qo_set_borders.argtypes = [ctypes.POINTER(Grid), dp, ctypes.c_double]
+        L.qo_inverted_distance.argtypes = [ctypes.POINTER(Grid), dp, dp]
+        L.qo_inverted_distance.restype = ctypes.c_int
+        L.qo_inverted_distance_normalized.argtypes = [ctypes.POINTER(Grid), dp, dp, dp]
+        L.qo_inverted_distance_normalized.restype = ctypes.c_int
+        L.qo_weight_score.argtypes = [ctypes.POINTER(Grid), dp, ctypes.POINTER(Grid), dp, ctypes.c_double,
+                                      ctypes.c_double, dp, dp]
         _lib = L
     return _lib
 
@@ -284,3 +290,45 @@ def set_borders(grid, field, value):
     f = np.array(field, dtype=np.float64).reshape(-1).copy()
     lib().qo_set_borders(ctypes.byref(grid), _dp(f), float(value))
     return f
+
+
+def inverted_distance(grid, dist, normalized=False):
+    """FIELD_3D::invertedDistance / invertedDistanceNormalized of a signed
+    distance field (FIELD_3D.cpp:5241-5352). Returns (field, abs_sum) with
+    abs_sum the serial absolute sum of the result (normalized only, else
+    None); raises ValueError where the reference would sweep forever."""
+    d = np.ascontiguousarray(dist, dtype=np.float64).reshape(-1)
+    out = np.empty_like(d)
+    a = ctypes.c_double(0.0)
+    if normalized:
+        st = lib().qo_inverted_distance_normalized(ctypes.byref(grid), _dp(d), _dp(out), ctypes.byref(a))
+    else:
+        st = lib().qo_inverted_distance(ctypes.byref(grid), _dp(d), _dp(out))
+    if st != 0:
+        raise ValueError("inverted distance: some cell can never be numbered")
+    return out, (a.value if normalized else None)
+
+
+def weight_score(grid, field, wgrid, weight, alpha, threshold=0.0):
+    """OPTIMIZE_3D::weightScore / inverseDistanceScore (OPTIMIZE_3D.cpp:4502-4563)
+    and computeMaximumScaledPowerRationalScore (:4484-4498): (score, max_score)."""
+    f = np.ascontiguousarray(field, dtype=np.float64).reshape(-1)
+    w = np.ascontiguousarray(weight, dtype=np.float64).reshape(-1)
+    sc, mx = ctypes.c_double(0.0), ctypes.c_double(0.0)
+    lib().qo_weight_score(ctypes.byref(grid), _dp(f), ctypes.byref(wgrid), _dp(w), float(alpha), float(threshold),
+                          ctypes.byref(sc), ctypes.byref(mx))
+    return sc.value, mx.value
+
+
+def field_min_max(field):
+    """FIELD_3D::fieldMin / fieldMax (FIELD_3D.cpp:2960-2971, 3021-3032): the
+    serial scan from cell 0 keeping the first cell that compares smaller /
+    larger (pure Python; small fields)."""
+    f = [float(v) for v in np.asarray(field, dtype=np.float64).reshape(-1)]
+    mn = mx = f[0]
+    for v in f:
+        if v < mn:
+            mn = v
+        if v > mx:
+            mx = v
+    return mn, mx

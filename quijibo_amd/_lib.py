@@ -122,6 +122,11 @@ SIGNATURES = [
     ("qj_field_clamp_infs_to_mean", ctypes.c_int32, [_P, _P, ctypes.c_int64, c_double_p, _P]),
     ("qj_field_clamp_nans_to_mean", ctypes.c_int32, [_P, _P, ctypes.c_int64, c_double_p, _P]),
     ("qj_field_set_borders", ctypes.c_int32, [ctypes.POINTER(Grid), ctypes.c_double, _P, _P]),
+    ("qj_field_min_max", ctypes.c_int32, [_P, _P, ctypes.c_int64, c_double_p, c_double_p, _P]),
+    ("qj_field_inverted_distance", ctypes.c_int32, [_P, ctypes.POINTER(Grid), _P, ctypes.c_int32, _P, c_double_p,
+                                                    _P]),
+    ("qj_weight_scores", ctypes.c_int32, [_P, ctypes.POINTER(Grid), _P, ctypes.POINTER(Grid), _P, ctypes.c_double,
+                                          c_double_p, ctypes.c_int32, c_double_p, c_double_p, _P]),
     ("qj_ap_score_gradients", ctypes.c_int32, [_P, ctypes.POINTER(Grid), _P, ctypes.POINTER(MapParams),
                                                ctypes.POINTER(Grid), _P, ctypes.c_double, ctypes.c_double, _P, _P,
                                                _P, ctypes.POINTER(APGrads), _P]),

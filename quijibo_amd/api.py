@@ -207,6 +207,26 @@ class FIELD_3D:
         g = self.grid()
         check(_lib.load().qj_field_set_borders(ctypes.byref(g), float(value), _ptr(self.data), _stream_ptr(stream)))
 
+    def fieldMin(self, workspace=None, stream=None):
+        """FIELD_3D::fieldMin (FIELD_3D.cpp:3021-3032) on the device."""
+        return field_min_max(self.data.view(-1), workspace, stream)[0]
+
+    def fieldMax(self, workspace=None, stream=None):
+        """FIELD_3D::fieldMax (FIELD_3D.cpp:2960-2971) on the device."""
+        return field_min_max(self.data.view(-1), workspace, stream)[1]
+
+    def invertedDistance(self, workspace=None, stream=None):
+        """FIELD_3D::invertedDistance (FIELD_3D.cpp:5241-5297) of this signed
+        distance field, as a new FIELD_3D on the same grid."""
+        out, _ = inverted_distance(self.data.view(-1), self.grid(), False, workspace, stream)
+        return FIELD_3D(*self.res, center=self.center, lengths=self.lengths, data=out, rotation=self.rotation)
+
+    def invertedDistanceNormalized(self, workspace=None, stream=None):
+        """FIELD_3D::invertedDistanceNormalized (FIELD_3D.cpp:5316-5352):
+        (field, absolute sum of the result)."""
+        out, a = inverted_distance(self.data.view(-1), self.grid(), True, workspace, stream)
+        return FIELD_3D(*self.res, center=self.center, lengths=self.lengths, data=out, rotation=self.rotation), a
+
 
 def _clamp(field, fn, workspace, stream):
     _device_tensor(field.data, "field", field.totalCells())
@@ -215,6 +235,52 @@ def _clamp(field, fn, workspace, stream):
     check(getattr(_lib.load(), fn)(ws.handle, _ptr(field.data), field.totalCells(), ctypes.byref(mean),
                                    _stream_ptr(stream)))
     return mean.value
+
+
+def field_min_max(field, workspace=None, stream=None):
+    """qj_field_min_max: (fieldMin, fieldMax) of a float64 CUDA tensor with the
+    reference's serial-scan semantics (FIELD_3D.cpp:2960-3032)."""
+    n = field.numel()
+    _device_tensor(field, "field", n)
+    if n < 1:
+        raise ValueError("field_min_max: empty field")
+    ws = workspace or default_workspace()
+    mn, mx = ctypes.c_double(), ctypes.c_double()
+    check(_lib.load().qj_field_min_max(ws.handle, _ptr(field), n, ctypes.byref(mn), ctypes.byref(mx),
+                                       _stream_ptr(stream)))
+    return mn.value, mx.value
+
+
+def inverted_distance(dist, grid, normalized=False, workspace=None, stream=None, out=None):
+    """qj_field_inverted_distance: FIELD_3D::invertedDistance(Normalized) of a
+    signed distance field (float64 CUDA tensor on `grid`). Returns (out, abs
+    sum of out)."""
+    n = _cells(grid)
+    _device_tensor(dist, "dist", n)
+    if out is None:
+        out = torch.empty(n, dtype=torch.float64, device=dist.device)
+    _device_tensor(out, "out", n)
+    ws = workspace or default_workspace()
+    a = ctypes.c_double()
+    check(_lib.load().qj_field_inverted_distance(ws.handle, ctypes.byref(grid), _ptr(dist), 1 if normalized else 0,
+                                                 _ptr(out), ctypes.byref(a), _stream_ptr(stream)))
+    return out, a.value
+
+
+def weight_scores(field, grid, weight, wgrid, alpha, thresholds=(0.0,), workspace=None, stream=None):
+    """qj_weight_scores: OPTIMIZE_3D::weightScore / inverseDistanceScore for each
+    threshold and computeMaximumScaledPowerRationalScore (OPTIMIZE_3D.cpp:4484-4563).
+    Returns (scores as a numpy array, max_score)."""
+    _device_tensor(field, "field", _cells(grid))
+    _device_tensor(weight, "weight", _cells(wgrid))
+    t = np.ascontiguousarray(thresholds, dtype=np.float64).reshape(-1)
+    sc = np.zeros(t.size, dtype=np.float64)
+    mx = ctypes.c_double()
+    ws = workspace or default_workspace()
+    check(_lib.load().qj_weight_scores(ws.handle, ctypes.byref(grid), _ptr(field), ctypes.byref(wgrid), _ptr(weight),
+                                       float(alpha), t.ctypes.data_as(_lib.c_double_p), int(t.size),
+                                       sc.ctypes.data_as(_lib.c_double_p), ctypes.byref(mx), _stream_ptr(stream)))
+    return sc, mx.value
 
 
 def field_eval(grid, rational, params, precision=FP64, band=DEFAULT_BAND, out=None, workspace=None, stream=None):

@@ -15,7 +15,7 @@ HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("QJ_OFFLOAD_ARCH", "gfx950")
 
 SOURCES = ["csrc/qj_api.cpp", "csrc/qj_field.hip", "csrc/qj_mc.hip", "csrc/qj_scan.hip", "csrc/qj_grad.hip", "csrc/qj_fieldops.hip",
-           "csrc/qj_server.cpp", "cpp/quijibo.cpp", "cpp/toupee.cpp"]
+           "csrc/qj_server.cpp", "cpp/quijibo.cpp", "cpp/toupee.cpp", "cpp/tooth_optimizer.cpp"]
 
 # Packed FP32 (v_pk_*_f32) is disabled for all device code.
 DEVICE_FLAGS = ["-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"]

@@ -5,6 +5,7 @@
 #include "../../include/quijibo/quijibo.hpp"
 
 #include <hip/hip_runtime_api.h>
+#include <float.h>
 #include <math.h>
 #include <string.h>
 #include <zlib.h>
@@ -189,6 +190,29 @@ POLYNOMIAL_4D::POLYNOMIAL_4D(const std::vector<QUATERNION>& roots, const std::ve
 void POLYNOMIAL_4D::addRoot(const QUATERNION& r, const Real& power) {
   _roots.push_back(r);
   _rootPowers.push_back(power);
+}
+
+POLYNOMIAL_4D& POLYNOMIAL_4D::operator*=(const Real& alpha) {
+  for (QUATERNION& r : _roots)
+    for (int i = 0; i < 4; i++) r[i] *= alpha;
+  return *this;
+}
+
+POLYNOMIAL_4D& POLYNOMIAL_4D::operator+=(const VEC3F& v) {
+  for (QUATERNION& r : _roots)
+    for (int i = 0; i < 3; i++) r[i] += v[i];
+  return *this;
+}
+
+POLYNOMIAL_4D& POLYNOMIAL_4D::operator-=(const VEC3F& v) {
+  for (QUATERNION& r : _roots)
+    for (int i = 0; i < 3; i++) r[i] -= v[i];
+  return *this;
+}
+
+void POLYNOMIAL_4D::translateExceptFirst(const VEC3F& v) {
+  for (size_t k = 1; k < _roots.size(); k++)
+    for (int i = 0; i < 3; i++) _roots[k][i] += v[i];
 }
 
 void POLYNOMIAL_4D::clear() {
@@ -406,6 +430,68 @@ void FIELD_3D::setBorders(const Real& value) {
   const qj_grid g = grid();
   device_pass(*this, [&](double* d) { check(qj_field_set_borders(&g, value, d, nullptr), "qj_field_set_borders"); });
 }
+
+void FIELD_3D::scaleLengths(const Real& scale) {
+  for (int i = 0; i < 3; i++) _lengths[i] *= scale;
+}
+
+FIELD_3D& FIELD_3D::operator*=(const Real& alpha) {
+  for (Real& v : _data) v *= alpha;
+  return *this;
+}
+
+Real FIELD_3D::sum() const {
+  Real s = 0;
+  for (Real v : _data) s += v;
+  return s;
+}
+
+Real FIELD_3D::absSum() const {
+  Real s = 0;
+  for (Real v : _data) s += fabs(v);
+  return s;
+}
+
+namespace {
+void field_range(const FIELD_3D& f, Real* mn, Real* mx) {
+  const size_t n = f.totalCells();
+  if (!n) throw Error(QJ_ERR_INVALID, "FIELD_3D fieldMin/fieldMax: empty field");
+  DeviceBuffer d(sizeof(double) * n);
+  hip_check(hipMemcpy(d.p, f.dataConst(), sizeof(double) * n, hipMemcpyHostToDevice), "hipMemcpy field");
+  check(qj_field_min_max(shared_workspace(), (const double*)d.p, (int64_t)n, mn, mx, nullptr), "qj_field_min_max");
+}
+
+FIELD_3D inverted(const FIELD_3D& dist, bool normalized, Real* abs_sum) {
+  FIELD_3D out(dist);
+  const size_t n = dist.totalCells();
+  if (abs_sum) *abs_sum = 0;
+  if (!n) return out;
+  DeviceBuffer d(sizeof(double) * n), o(sizeof(double) * n);
+  hip_check(hipMemcpy(d.p, dist.dataConst(), sizeof(double) * n, hipMemcpyHostToDevice), "hipMemcpy field");
+  const qj_grid g = dist.grid();
+  check(qj_field_inverted_distance(shared_workspace(), &g, (const double*)d.p, normalized ? 1 : 0, (double*)o.p,
+                                   abs_sum, nullptr),
+        "qj_field_inverted_distance");
+  hip_check(hipMemcpy(out.data(), o.p, sizeof(double) * n, hipMemcpyDeviceToHost), "hipMemcpy field");
+  return out;
+}
+}  // namespace
+
+Real FIELD_3D::fieldMax() const {
+  Real mn, mx;
+  field_range(*this, &mn, &mx);
+  return mx;
+}
+
+Real FIELD_3D::fieldMin() const {
+  Real mn, mx;
+  field_range(*this, &mn, &mx);
+  return mn;
+}
+
+FIELD_3D FIELD_3D::invertedDistance() const { return inverted(*this, false, nullptr); }
+
+FIELD_3D FIELD_3D::invertedDistanceNormalized(Real* absSum) const { return inverted(*this, true, absSum); }
 
 TRIANGLE_MESH::TRIANGLE_MESH(const FIELD_3D& field, Real isosurface) {
   qj_workspace* ws = shared_workspace();
@@ -661,6 +747,181 @@ void OPTIMIZE_3D::computeRationalMapAP(const POLYNOMIAL_4D& top, const POLYNOMIA
   _apValid = false;
 }
 
+void OPTIMIZE_3D::map_into(const POLYNOMIAL_4D& top, const POLYNOMIAL_4D& bottom, FIELD_3D& fractal,
+                           const qj_map_params& p, bool keepSkipped) {
+  sync_rational(top, bottom);
+  const size_t n = fractal.totalCells();
+  if (!n) return;
+  ensure_device(n);
+  if (keepSkipped)
+    hip_check(hipMemcpy(_dField, fractal.dataConst(), sizeof(double) * n, hipMemcpyHostToDevice), "hipMemcpy field");
+  const qj_grid g = fractal.grid();
+  check(qj_field_eval(_ws, &g, _rational, &p, QJ_FP64, 0.0, _dField, nullptr), "qj_field_eval");
+  hip_check(hipMemcpy(fractal.data(), _dField, sizeof(double) * n, hipMemcpyDeviceToHost), "hipMemcpy field");
+  _apValid = false;
+}
+
+void OPTIMIZE_3D::computeLogScaledPowerRationalMapCached(const POLYNOMIAL_4D& top, const POLYNOMIAL_4D& bottom,
+                                                         FIELD_3D& fractal, bool verbose) {
+  (void)verbose;
+  // cells at the origin are skipped and keep their previous value (OPTIMIZE_3D.cpp:1624-1625)
+  const qj_map_params p{_maxIterations, 20.0, _expScaling, _quaternionSlice, QJ_MAP_SKIP_ORIGIN};
+  map_into(top, bottom, fractal, p, true);
+}
+
+void OPTIMIZE_3D::computeLogPowerRationalMap(bool verbose) {
+  (void)verbose;
+  // evaluatePowerFactored: the root powers without the power scalar
+  POLYNOMIAL_4D top(_top), bottom(_bottom);
+  top.powerScalar() = 1.0;
+  bottom.powerScalar() = 1.0;
+  const qj_map_params p{_maxIterations, 20.0, _expScaling, _quaternionSlice, 0u};
+  map_into(top, bottom, _fractal, p, false);
+}
+
+void OPTIMIZE_3D::scores_on_device(const FIELD_3D& logFractal, const FIELD_3D& weight,
+                                   const std::vector<Real>& thresholds, std::vector<Real>* scores, Real* maxScore) {
+  const size_t n = logFractal.totalCells(), wn = weight.totalCells();
+  if (!n) throw Error(QJ_ERR_INVALID, "OPTIMIZE_3D score: the fractal is empty");
+  if (!wn) throw Error(QJ_ERR_INVALID, "OPTIMIZE_3D score: the weight field is empty");
+  if (!_ws) check(qj_workspace_create(&_ws), "qj_workspace_create");
+  ensure_device(n);
+  hip_check(hipMemcpy(_dField, logFractal.dataConst(), sizeof(double) * n, hipMemcpyHostToDevice),
+            "hipMemcpy field");
+  if (wn > _dWeightCells || !_dWeight) {
+    if (_dWeight) hip_check(hipFree(_dWeight), "hipFree");
+    _dWeight = nullptr;
+    hip_check(hipMalloc((void**)&_dWeight, sizeof(double) * wn), "hipMalloc weight");
+    _dWeightCells = wn;
+  }
+  hip_check(hipMemcpy(_dWeight, weight.dataConst(), sizeof(double) * wn, hipMemcpyHostToDevice), "hipMemcpy weight");
+  const qj_grid g = logFractal.grid(), wg = weight.grid();
+  if (scores) scores->assign(thresholds.size(), 0.0);
+  check(qj_weight_scores(_ws, &g, _dField, &wg, _dWeight, _tanhAlpha, thresholds.data(), (int32_t)thresholds.size(),
+                         scores ? scores->data() : nullptr, maxScore, nullptr),
+        "qj_weight_scores");
+}
+
+Real OPTIMIZE_3D::weightScore(const FIELD_3D& logFractal, const FIELD_3D& weightField) {
+  std::vector<Real> s;
+  scores_on_device(logFractal, weightField, std::vector<Real>{0.0}, &s, nullptr);
+  return s[0];
+}
+
+Real OPTIMIZE_3D::inverseDistanceScore(const FIELD_3D& logFractal, const FIELD_3D& inverseDistanceField,
+                                       const Real& inverseSum, const Real& threshold) {
+  (void)inverseSum;  // the reference leaves its normalisation commented out (OPTIMIZE_3D.cpp:4560)
+  std::vector<Real> s;
+  scores_on_device(logFractal, inverseDistanceField, std::vector<Real>{threshold}, &s, nullptr);
+  return s[0];
+}
+
+std::vector<Real> OPTIMIZE_3D::inverseDistanceScores(const FIELD_3D& logFractal, const FIELD_3D& inverseDistanceField,
+                                                     const std::vector<Real>& thresholds) {
+  std::vector<Real> s;
+  scores_on_device(logFractal, inverseDistanceField, thresholds, &s, nullptr);
+  return s;
+}
+
+Real OPTIMIZE_3D::computeLogScaledPowerRationalScore() {
+  computeLogScaledPowerRationalMapCached(_top, _bottom, _fractal);
+  return weightScore(_fractal, _weightField);
+}
+
+Real OPTIMIZE_3D::computeMaximumScaledPowerRationalScore() {
+  Real m = 0;
+  scores_on_device(_fractal, _weightField, std::vector<Real>(), nullptr, &m);
+  return m;
+}
+
+Real OPTIMIZE_3D::estimateConformalRadius() {
+  const VEC3F center = _fractal.center(), lengths = _fractal.lengths();
+  const int fractalRes = _fractal.xRes();
+  _fractal = FIELD_3D(fractalRes, fractalRes, fractalRes, center, lengths);
+  const int oldMaxIterations = _maxIterations;
+  _expScaling = 1;
+  _maxIterations = 1;
+  computeLogPowerRationalMap(true);
+  _fractal.clampInfsToMean();
+  const FIELD_3D inverseDistanceField = _distanceField.invertedDistance();
+  Real minScore = _fractal.fieldMin(), maxScore = _fractal.fieldMax();
+  Real minThreshold = minScore;
+  Real minScoreFound = FLT_MAX;
+  for (int i = 0; i < 4; i++) {
+    const Real increment = (maxScore - minScore) / 100;
+    minScoreFound = FLT_MAX;
+    // the reference's scan, all thresholds of a pass scored in one GPU pass
+    std::vector<Real> xs;
+    for (Real x = minScore; x < maxScore; x += increment) {
+      if (xs.size() >= 4096 || x + increment == x)
+        throw Error(QJ_ERR_NUMERIC, "estimateConformalRadius: the threshold scan does not advance");
+      xs.push_back(x);
+    }
+    const std::vector<Real> sc = xs.empty() ? std::vector<Real>() : inverseDistanceScores(_fractal,
+                                                                                          inverseDistanceField, xs);
+    for (size_t k = 0; k < xs.size(); k++)
+      if (sc[k] < minScoreFound) {
+        minScoreFound = sc[k];
+        minThreshold = xs[k];
+      }
+    minScore = minThreshold - 5 * increment;
+    maxScore = minThreshold + 5 * increment;
+  }
+  _expScaling = exp(-minThreshold);
+  _maxIterations = oldMaxIterations;
+  return minScoreFound;
+}
+
+void OPTIMIZE_3D::setScoreFields(const FIELD_3D& distanceField, const FIELD_3D& curvatureField) {
+  if (distanceField.xRes() != curvatureField.xRes() || distanceField.yRes() != curvatureField.yRes() ||
+      distanceField.zRes() != curvatureField.zRes())
+    throw Error(QJ_ERR_INVALID, "setScoreFields: the distance and curvature fields differ in resolution");
+  _xRes = distanceField.xRes();
+  _yRes = distanceField.yRes();
+  _zRes = distanceField.zRes();
+  _center = distanceField.center();
+  _lengths = distanceField.lengths();
+  FIELD_3D d(distanceField.dataConst(), _xRes, _yRes, _zRes, _center, _lengths);
+  FIELD_3D c(curvatureField.dataConst(), _xRes, _yRes, _zRes, _center, _lengths);
+  _distanceField = std::move(d);
+  _curvatureField = std::move(c);
+  _fractal = FIELD_3D(_xRes, _yRes, _zRes, _center, _lengths);
+  _apValid = false;
+}
+
+void OPTIMIZE_3D::scaleEverything(const Real& scale) {
+  for (int i = 0; i < 3; i++) {
+    _lengths[i] *= scale;
+    _center[i] *= scale;
+  }
+  for (FIELD_3D* f : {&_distanceField, &_inverseDistanceField, &_weightField, &_curvatureField, &_fractal}) {
+    f->scaleLengths(scale);
+    for (int i = 0; i < 3; i++) f->center()[i] *= scale;
+  }
+  _distanceField *= scale;
+  _top *= scale;
+  _bottom *= scale;
+  _apValid = false;
+}
+
+void OPTIMIZE_3D::translateEverything(const VEC3F& translation) {
+  for (int i = 0; i < 3; i++) _center[i] += translation[i];
+  for (FIELD_3D* f : {&_distanceField, &_inverseDistanceField, &_weightField, &_curvatureField, &_fractal})
+    for (int i = 0; i < 3; i++) f->center()[i] += translation[i];
+  _top += translation;
+  _bottom += translation;
+  _apValid = false;
+}
+
+void OPTIMIZE_3D::translateEverythingExceptFirst(const VEC3F& translation) {
+  for (int i = 0; i < 3; i++) _center[i] += translation[i];
+  for (FIELD_3D* f : {&_distanceField, &_inverseDistanceField, &_weightField, &_curvatureField, &_fractal})
+    for (int i = 0; i < 3; i++) f->center()[i] += translation[i];
+  _top.translateExceptFirst(translation);
+  _bottom.translateExceptFirst(translation);
+  _apValid = false;
+}
+
 // One fused GPU pass gives the AP field, the score and every gradient for the
 // current state; the accessors reuse it while the state (roots, powers,
 // scalars, c, alpha, threshold, weight field) is unchanged.
@@ -782,6 +1043,17 @@ void OPTIMIZE_3D::write(const std::string& filename) const {
 }
 
 void OPTIMIZE_3D::read(const std::string& filename) {
+  readState(filename);
+  // the reference's post-read set-up (OPTIMIZE_3D.cpp:3348-3388)
+  setScoreFields(_distanceField, _distanceField);
+  if (_inverseDistanceField.totalCells() == 0) {
+    _inverseDistanceField = _distanceField.invertedDistanceNormalized();
+    _inverseFieldSum = _inverseDistanceField.absSum();
+  }
+  computeWeightField();
+}
+
+void OPTIMIZE_3D::readState(const std::string& filename) {
   Reader r;
   r.f = fopen(filename.c_str(), "rb");
   if (!r.f) throw Error(QJ_ERR_INVALID, "OPTIMIZE_3D read: cannot open " + filename);

@@ -614,6 +614,43 @@ qj_status qj_field_set_borders(const qj_grid* grid, double value, double* d_fiel
   return field_set_borders(grid->res, value, d_field, (hipStream_t)stream);
 }
 
+static bool grid_ok(const qj_grid* g) {
+  return g && g->res[0] >= 1 && g->res[1] >= 1 && g->res[2] >= 1;
+}
+
+qj_status qj_field_min_max(qj_workspace* ws, const double* d_field, int64_t n, double* min_out, double* max_out,
+                           void* stream) {
+  if (!ws || !d_field || !min_out || !max_out || n < 1) {
+    set_error("qj_field_min_max: null argument or empty field");
+    return QJ_ERR_INVALID;
+  }
+  std::lock_guard<std::mutex> lock(ws->mu);
+  return field_min_max(ws, d_field, n, min_out, max_out, (hipStream_t)stream);
+}
+
+qj_status qj_field_inverted_distance(qj_workspace* ws, const qj_grid* grid, const double* d_dist, int32_t normalized,
+                                     double* d_out, double* abs_sum, void* stream) {
+  if (!ws || !grid_ok(grid) || !d_dist || !d_out || d_dist == d_out) {
+    set_error("qj_field_inverted_distance: null argument, empty grid, or output aliasing the distance field");
+    return QJ_ERR_INVALID;
+  }
+  std::lock_guard<std::mutex> lock(ws->mu);
+  return field_inverted_distance(ws, d_dist, grid, normalized != 0, d_out, abs_sum, (hipStream_t)stream);
+}
+
+qj_status qj_weight_scores(qj_workspace* ws, const qj_grid* grid, const double* d_field, const qj_grid* wgrid,
+                           const double* d_weight, double tanh_alpha, const double* thresholds, int32_t n_thresholds,
+                           double* scores, double* max_score, void* stream) {
+  if (!ws || !grid_ok(grid) || !grid_ok(wgrid) || !d_field || !d_weight || n_thresholds < 0 ||
+      (n_thresholds > 0 && (!thresholds || !scores)) || n_thresholds > 65534) {
+    set_error("qj_weight_scores: null argument, empty grid, or threshold count outside [0, 65534]");
+    return QJ_ERR_INVALID;
+  }
+  std::lock_guard<std::mutex> lock(ws->mu);
+  return weight_scores(ws, d_field, grid, wgrid, d_weight, tanh_alpha, thresholds, n_thresholds, scores, max_score,
+                       (hipStream_t)stream);
+}
+
 qj_status qj_ap_score_gradients(qj_workspace* ws, const qj_grid* grid, const qj_rational* f,
                                 const qj_map_params* params, const qj_grid* wgrid, const double* d_weight,
                                 double tanh_alpha, double tanh_threshold, double* d_field, double* grad_top,

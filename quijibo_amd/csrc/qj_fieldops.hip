@@ -332,6 +332,64 @@ __global__ __launch_bounds__(kOpsBlock) void k_weight_score(const double* __rest
   }
 }
 
+// fieldMin / fieldMax (FIELD_3D.cpp:2960-2971, 3021-3032): the serial scan
+// keeps the first cell that compares smaller (larger), starting from cell 0,
+// so NaN cells are skipped unless cell 0 is NaN, and of equal values (0 and
+// -0) the first wins. Reduced as (value, index) pairs with the index breaking
+// ties; NaN cells never win.
+struct VI {
+  double v;
+  int64_t i;
+};
+__device__ __forceinline__ VI vi_min(VI a, VI b) {
+  if (b.i < 0) return a;
+  if (a.i < 0) return b;
+  return (b.v < a.v || (b.v == a.v && b.i < a.i)) ? b : a;
+}
+__device__ __forceinline__ VI vi_max(VI a, VI b) {
+  if (b.i < 0) return a;
+  if (a.i < 0) return b;
+  return (b.v > a.v || (b.v == a.v && b.i < a.i)) ? b : a;
+}
+
+__global__ __launch_bounds__(kOpsBlock) void k_min_max(const double* __restrict__ f, int64_t n,
+                                                       double* __restrict__ part) {
+  __shared__ double s_v[2][kOpsBlock];
+  __shared__ int64_t s_i[2][kOpsBlock];
+  VI mn{0.0, -1}, mx{0.0, -1};
+  const int64_t stride = (int64_t)gridDim.x * kOpsBlock;
+  for (int64_t i = (int64_t)blockIdx.x * kOpsBlock + threadIdx.x; i < n; i += stride) {
+    const double v = f[i];
+    if (isnan(v)) continue;
+    mn = vi_min(mn, VI{v, i});
+    mx = vi_max(mx, VI{v, i});
+  }
+  const int t = threadIdx.x;
+  s_v[0][t] = mn.v;
+  s_i[0][t] = mn.i;
+  s_v[1][t] = mx.v;
+  s_i[1][t] = mx.i;
+  __syncthreads();
+  for (int w = kOpsBlock / 2; w > 0; w >>= 1) {
+    if (t < w) {
+      const VI a = vi_min(VI{s_v[0][t], s_i[0][t]}, VI{s_v[0][t + w], s_i[0][t + w]});
+      const VI b = vi_max(VI{s_v[1][t], s_i[1][t]}, VI{s_v[1][t + w], s_i[1][t + w]});
+      s_v[0][t] = a.v;
+      s_i[0][t] = a.i;
+      s_v[1][t] = b.v;
+      s_i[1][t] = b.i;
+    }
+    __syncthreads();
+  }
+  if (t == 0) {
+    double* p = part + 4 * (int64_t)blockIdx.x;
+    p[0] = s_v[0][0];
+    p[1] = (double)s_i[0][0];  // exact: indexes stay below 2^53
+    p[2] = s_v[1][0];
+    p[3] = (double)s_i[1][0];
+  }
+}
+
 // host: the sum of nparts partials (4 doubles each) read back
 static double host_sum(const std::vector<double>& p, int64_t first, int64_t nparts) {
   double hi = 0, lo = 0, special = 0;
@@ -456,6 +514,42 @@ qj_status field_inverted_distance(qj_workspace* ws, const double* d_dist, const 
     QJ_HIP(hipGetLastError());
   }
   if (abs_sum && (st = sum(0, abs_sum))) return st;
+  return QJ_OK;
+}
+
+qj_status field_min_max(qj_workspace* ws, const double* d_field, int64_t n, double* mn, double* mx, hipStream_t s) {
+  if (n <= 0) return QJ_ERR_INVALID;
+  const int64_t want = (n + kOpsBlock - 1) / kOpsBlock;
+  const int grid = (int)(want < kOpsGrid ? want : kOpsGrid);
+  qj_status st = ensure_partials(ws, 4 * (int64_t)grid, s);
+  if (st) return st;
+  hipLaunchKernelGGL(k_min_max, dim3(grid), dim3(kOpsBlock), 0, s, d_field, n, ws->d_partials);
+  QJ_HIP(hipGetLastError());
+  std::vector<double> part(4 * (size_t)grid);
+  double first = 0;
+  QJ_HIP(hipMemcpyAsync(part.data(), ws->d_partials, sizeof(double) * part.size(), hipMemcpyDeviceToHost, s));
+  QJ_HIP(hipMemcpyAsync(&first, d_field, sizeof(double), hipMemcpyDeviceToHost, s));
+  QJ_HIP(hipStreamSynchronize(s));
+  if (isnan(first)) {  // nothing compares against a NaN start
+    *mn = *mx = first;
+    return QJ_OK;
+  }
+  double bv = 0, tv = 0;
+  int64_t bi = -1, ti = -1;
+  for (int b = 0; b < grid; b++) {
+    const int64_t i0 = (int64_t)part[4 * b + 1], i1 = (int64_t)part[4 * b + 3];
+    const double v0 = part[4 * b], v1 = part[4 * b + 2];
+    if (i0 >= 0 && (bi < 0 || v0 < bv || (v0 == bv && i0 < bi))) {
+      bv = v0;
+      bi = i0;
+    }
+    if (i1 >= 0 && (ti < 0 || v1 > tv || (v1 == tv && i1 < ti))) {
+      tv = v1;
+      ti = i1;
+    }
+  }
+  *mn = bv;
+  *mx = tv;
   return QJ_OK;
 }
 

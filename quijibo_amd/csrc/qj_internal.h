@@ -178,6 +178,8 @@ qj_status field_set_borders(const int32_t* res, double value, double* d_field, h
 // (qj_fieldops.hip); abs_sum (host, optional): sum of |result|
 qj_status field_inverted_distance(qj_workspace* ws, const double* d_dist, const qj_grid* grid, bool normalized,
                                   double* d_out, double* abs_sum, hipStream_t s);
+// fieldMin / fieldMax with the serial scan's NaN and tie behaviour (host results)
+qj_status field_min_max(qj_workspace* ws, const double* d_field, int64_t n, double* mn, double* mx, hipStream_t s);
 // weightScore for each threshold (and the maximum score) of a field
 qj_status weight_scores(qj_workspace* ws, const double* d_field, const qj_grid* grid, const qj_grid* wgrid,
                         const double* d_weight, double alpha, const double* thresholds, int nthr, double* scores,

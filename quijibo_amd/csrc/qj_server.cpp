@@ -1,13 +1,18 @@
 // Request loop: continuous batching of field solves on one GPU.
 //
 // Requests (any grid size, any root count up to max_roots) are copied on
-// submit and wait in a pending list; past max_pending_work of admitted and
-// waiting work a submission is refused (QJ_ERR_BUSY). A scheduler thread
-// admits them while their HBM footprint (field + root records) fits a
-// preallocated arena, smallest first, splits admitted requests into
+// submit and wait in a pending list; a submission is refused (QJ_ERR_BUSY)
+// when the work queued ahead of it in shortest-first order (the outstanding
+// requests no larger than it, its own included) exceeds max_pending_work, so
+// under overload the large requests are the ones turned away. A scheduler
+// thread admits requests while their HBM footprint (field + root records)
+// fits a preallocated arena, smallest first, splits admitted requests into
 // fixed-size voxel chunks and packs chunks of several requests into one
-// launch, shortest remaining work first; a request older than aging_s goes
-// before all others in both, oldest first, so large requests are not starved.
+// launch, shortest remaining work first. A request older than aging_s is
+// admitted before all others (oldest first; while it does not fit, requests
+// of a quarter of its footprint or more wait behind it) and takes up to a
+// quarter of each launch, so large requests are not starved and short ones
+// keep flowing.
 // Each launch is a hipGraph captured once per in-flight slot and set of FP32
 // variants present ([descriptor fetch, FP32 bulk per variant, FP64 band] or
 // [descriptor fetch, FP64]) and replayed with new descriptors, which the
@@ -29,6 +34,7 @@
 #include <deque>
 #include <map>
 #include <memory>
+#include <set>
 #include <thread>
 #include <unordered_map>
 
@@ -161,6 +167,7 @@ struct qj_server {
   hipEvent_t check_ev[kCheckSlots] = {};
   std::vector<int> check_free;
   double pending_work = 0;  // guarded by mu: work submitted and not yet completed
+  std::multiset<double> outstanding;  // guarded by mu: the work of each submitted, not completed request
   std::vector<uint64_t> release_q;
   size_t slot_next = 0;
   qj_status fatal = QJ_OK;
@@ -189,6 +196,8 @@ static void finish(qj_server* s, Req* r, qj_status st) {
   memcpy(&c.check_max_rel_dF, &r->chk[CNT_CHECK_MAXREL], sizeof(double));
   std::lock_guard<std::mutex> lk(s->mu);
   s->pending_work -= r->work;
+  auto ow = s->outstanding.find(r->work);
+  if (ow != s->outstanding.end()) s->outstanding.erase(ow);
   if (s->pending_work < 0.5) s->pending_work = 0.0;  // rounding of the running sum
   s->completions.push_back(c);
   s->cv_done.notify_all();
@@ -229,31 +238,39 @@ static bool try_admit(qj_server* s, Req* r) {
   return true;
 }
 
-// Fills slot descriptors with up to chunks_per_launch chunks, shortest remaining
-// work first, until the launch holds about launch_work root-term iterations.
+// Fills slot descriptors with up to chunks_per_launch chunks until the launch
+// holds about launch_work root-term iterations: requests older than aging_s
+// (oldest first) take up to kAgedShare of it while younger ones wait, the
+// rest goes shortest remaining work first, and aged requests take whatever
+// room is left. An aged request thus progresses at a quarter of the device
+// or more without stalling the short requests behind it.
+constexpr double kAgedShare = 0.25;
+
 static int build_batch(qj_server* s, Slot& sl) {
-  std::vector<Req*> order;
-  for (auto& kv : s->live)
-    if (kv.second->next < kv.second->voxels) order.push_back(kv.second.get());
+  std::vector<Req*> aged_q, young;
   const double now = s->now();
-  std::sort(order.begin(), order.end(), [s, now](const Req* a, const Req* b) {
-    const bool oa = aged(s, a, now), ob = aged(s, b, now);
-    if (oa != ob) return oa;
-    if (oa) return a->submitted < b->submitted;
+  for (auto& kv : s->live) {
+    Req* r = kv.second.get();
+    if (r->next < r->voxels) (aged(s, r, now) ? aged_q : young).push_back(r);
+  }
+  std::sort(aged_q.begin(), aged_q.end(), [](const Req* a, const Req* b) { return a->submitted < b->submitted; });
+  std::sort(young.begin(), young.end(), [](const Req* a, const Req* b) {
     const int64_t ra = a->voxels - a->next, rb = b->voxels - b->next;
     return ra != rb ? ra < rb : a->arrival < b->arrival;
   });
   const int maxc = s->cfg.chunks_per_launch, cv = s->cfg.chunk_voxels;
+  const double full = s->cfg.launch_work;
   int n = 0;
   double work = 0.0;
   sl.work.clear();
   char* b = s->arena.base();
   std::vector<ChunkDesc>& tmp = s->batch_tmp;
   tmp.resize(maxc);
-  for (Req* r : order) {
+  // chunks of r while the launch has room below `limit`
+  auto take = [&](Req* r, double limit) {
     int taken = 0;
     const double per_voxel = (double)r->params.max_iter * (double)(r->nt + r->nb);
-    while (n < maxc && r->next < r->voxels && (n == 0 || work < s->cfg.launch_work)) {
+    while (n < maxc && r->next < r->voxels && (n == 0 || work < limit)) {
       ChunkDesc& d = tmp[n++];
       d.a = r->args;
       d.r64 = (const RootRec64*)(b + r->off64);
@@ -268,7 +285,19 @@ static int build_batch(qj_server* s, Slot& sl) {
       taken++;
     }
     if (taken) sl.work.emplace_back(r->id, taken);
-    if (n == maxc || work >= s->cfg.launch_work) break;
+  };
+  const double aged_limit = young.empty() ? full : full * kAgedShare;
+  for (Req* r : aged_q) {
+    if (n == maxc || work >= aged_limit) break;
+    take(r, aged_limit);
+  }
+  for (Req* r : young) {
+    if (n == maxc || work >= full) break;
+    take(r, full);
+  }
+  for (Req* r : aged_q) {
+    if (n == maxc || work >= full) break;
+    take(r, full);
   }
   // group by FP32 variant (the kernels take one group each); order inside a
   // group keeps the shortest-remaining-work order
@@ -353,18 +382,23 @@ static void worker_main(qj_server* s) {
                          if (oa) return a->submitted < b->submitted;
                          return a->bytes < b->bytes;
                        });
+      // An aged request that does not fit holds back the requests behind it
+      // that would compete for its memory (a quarter of its footprint or
+      // more); smaller ones still go, since shortest-first packing finishes
+      // them, and frees their memory, long before it.
       std::vector<std::unique_ptr<Req>> keep;
-      bool blocked = false;
+      size_t blocked_bytes = SIZE_MAX;
       for (auto& r : s->pending) {
         if (r->bytes > s->arena.size()) {
           finish(s, r.get(), QJ_ERR_CAPACITY);
           continue;
         }
-        if (!blocked && try_admit(s, r.get())) {
+        const bool held = blocked_bytes != SIZE_MAX && r->bytes >= blocked_bytes / 4;
+        if (!held && try_admit(s, r.get())) {
           const uint64_t id = r->id;
           s->live[id] = std::move(r);
         } else {
-          blocked |= aged(s, r.get(), now);
+          if (!held && aged(s, r.get(), now) && r->bytes < blocked_bytes) blocked_bytes = r->bytes;
           keep.push_back(std::move(r));
         }
       }
@@ -624,13 +658,17 @@ qj_status qj_server_submit(qj_server* s, const qj_request* req, uint64_t* id) {
   r->submitted = s->now();
   {
     std::lock_guard<std::mutex> lk(s->mu);
-    if (s->cfg.max_pending_work > 0 && s->pending_work > 0 &&
-        s->pending_work + r->work > s->cfg.max_pending_work) {
-      set_error("qj_server_submit: refused, %.3g root-term iterations pending (limit %.3g)", s->pending_work,
-                s->cfg.max_pending_work);
+    // work queued ahead of this request in shortest-first order, its own
+    // included: outstanding requests no larger than it
+    double ahead = r->work;
+    for (auto it = s->outstanding.begin(); it != s->outstanding.end() && *it <= r->work; ++it) ahead += *it;
+    if (s->cfg.max_pending_work > 0 && !s->outstanding.empty() && ahead > s->cfg.max_pending_work) {
+      set_error("qj_server_submit: refused, %.3g root-term iterations queued ahead of it in shortest-first order "
+                "(limit %.3g)", ahead, s->cfg.max_pending_work);
       return QJ_ERR_BUSY;
     }
     s->pending_work += r->work;
+    s->outstanding.insert(r->work);
     r->id = s->next_id.fetch_add(1);
     if (id) *id = r->id;
     s->incoming.push_back(std::move(r));

@@ -6,7 +6,7 @@
 //       writes a POLYNOMIAL_4D, FIELD_3D (plain and .gz) and .o3d file into
 //       <dir>, reads each back and checks the round trip field by field.
 //   host_layer_driver copy <in.o3d> <out.o3d>
-//       OPTIMIZE_3D::read then OPTIMIZE_3D::write.
+//       OPTIMIZE_3D::readState then OPTIMIZE_3D::write (the layout round trip).
 //   host_layer_driver map <in.o3d> <maxIterations> <precision 0|1> <outdir>
 //       OPTIMIZE_3D::read, computeLogScaledPowerRationalMap into the fractal
 //       -> <outdir>/field.bin; TRIANGLE_MESH(fractal) -> linear.obj; the
@@ -19,12 +19,19 @@
 //       one-iteration AP score and gradients printed as JSON.
 //   host_layer_driver blmvm rosenbrock|boxquad
 //       TAO_SOLVER::optimizeBLMVM on a test problem (no GPU); JSON result.
+//   host_layer_driver readsetup <in.o3d> <outdir>
+//       OPTIMIZE_3D::read with its set-up; weight, inverse distance,
+//       curvature and fractal fields -> <outdir>; JSON summary.
+//   host_layer_driver tooth <in.o3d> <path/> <maxTao> <amplitude> <radius> <res,res,...>
+//       the toothOptimizer schedule (optimizeTooth) with a sphere of radius 1
+//       as the fractal's distance field; JSON log.
 //   host_layer_driver optimize <in.o3d> <weight.field> <alpha> <iterations>
 //       the shape optimizer's exponent fit (toothOptimizer.cpp:65-110,
 //       160-218): BLMVM over [log c, bottom powers, top powers, top scalar,
 //       bottom scalar] with the GPU AP score and gradients as the callback.
 #include <math.h>
 #include <stdio.h>
+#include <stdlib.h>
 #include <string.h>
 
 #include <string>
@@ -90,7 +97,7 @@ static int run_io(const std::string& dir) {
   opt.distanceField()[0] = 42.0;
   opt.write(dir + "/state.o3d");
   OPTIMIZE_3D in;
-  in.read(dir + "/state.o3d");
+  in.readState(dir + "/state.o3d");
   if (in.xRes() != 5 || in.yRes() != 4 || in.zRes() != 3) return fail(".o3d resolution");
   if (!same_poly(in.top(), top) || !same_poly(in.bottom(), bottom)) return fail(".o3d polynomials");
   if (in.expScaling() != 0.75) return fail(".o3d expScaling");
@@ -106,7 +113,7 @@ static int run_io(const std::string& dir) {
   bool threw = false;
   try {
     OPTIMIZE_3D s;
-    s.read(dir + "/short.o3d");
+    s.readState(dir + "/short.o3d");
   } catch (const Error& e) {
     threw = e.status == QJ_ERR_INVALID;
   }
@@ -284,13 +291,87 @@ static int run_optimize(const std::string& in, const std::string& weight, double
   return 0;
 }
 
+// read() with the reference's set-up: the weight and inverse distance fields
+// and the fractal's new grid go to <outdir>
+static int run_readsetup(const std::string& in, const std::string& out) {
+  OPTIMIZE_3D opt;
+  opt.read(in);
+  opt.weightField().write(out + "/weight.bin");
+  opt.inverseDistanceField().write(out + "/inverse.bin");
+  opt.curvatureField().write(out + "/curvature.bin");
+  opt.fractal().write(out + "/fractal.bin");
+  printf("{\"res\": [%d, %d, %d], \"inverse_sum\": %.17g}\n", opt.xRes(), opt.yRes(), opt.zRes(),
+         opt.inverseFieldSum());
+  return 0;
+}
+
+// the fractal distance the reweighted solve uses: a sphere of radius 1 about
+// the origin, on the grid of the optimizer's distance field
+static FIELD_3D sphere_distance(const TRIANGLE_MESH& mesh, const FIELD_3D& grid) {
+  (void)mesh;
+  FIELD_3D f(grid);
+  for (int z = 0; z < f.zRes(); z++)
+    for (int y = 0; y < f.yRes(); y++)
+      for (int x = 0; x < f.xRes(); x++) {
+        const VEC3F c = f.cellCenter(x, y, z);
+        f(x, y, z) = sqrt(c[0] * c[0] + c[1] * c[1] + c[2] * c[2]) - 1.0;
+      }
+  return f;
+}
+
+static int run_tooth(const std::string& in, const std::string& path, int maxTao, double amp, double radius,
+                     const std::string& resList) {
+  OPTIMIZE_3D opt;
+  opt.read(in);
+  ToothOptimizerSettings s;
+  s.path = path;
+  s.outputFilename = "tooth.o3d";
+  s.resolutions.clear();
+  for (size_t a = 0; a < resList.size();) {
+    size_t b = resList.find(',', a);
+    if (b == std::string::npos) b = resList.size();
+    s.resolutions.push_back(atoi(resList.substr(a, b - a).c_str()));
+    a = b + 1;
+  }
+  s.maxTaoIterations = maxTao;
+  s.initialBulkScalarAmplitude = amp;
+  s.fractalRadius = radius;
+  s.fractalDistance = sphere_distance;
+  ToothOptimizerLog log;
+  optimizeTooth(opt, s, &log);
+  printf("{\"checkpoints\": %d, \"resolutions\": [", log.checkpoints);
+  for (size_t i = 0; i < log.resolutions.size(); i++) {
+    const ToothResolutionRecord& r = log.resolutions[i];
+    printf("%s{\"res\": %d, \"conformal\": %s, \"conformal_score\": %.17g, \"conformal_exp_scaling\": %.17g, "
+           "\"fractal_vertices\": %lld, \"fractal_triangles\": %lld, \"reweight_triangles\": %lld, "
+           "\"initial_score\": %.17g, "
+           "\"max_score\": %.17g, \"output\": \"%s\", \"solves\": [",
+           i ? ", " : "", r.res, r.conformal ? "true" : "false", r.conformalScore, r.conformalExpScaling,
+           (long long)r.fractalVertices, (long long)r.fractalTriangles, (long long)r.reweightTriangles, r.initialScore,
+           r.maxScore, r.output.c_str());
+    for (size_t k = 0; k < r.solves.size(); k++) {
+      const ToothSolveRecord& v = r.solves[k];
+      printf("%s{\"name\": \"%s\", \"iterations\": %d, \"reason\": %d, \"evaluations\": %d, \"value\": %.17g, "
+             "\"max_score\": %.17g, ",
+             k ? ", " : "", v.name.c_str(), v.iterations, v.reason, v.evaluations, v.value, v.maxScore);
+      print_vec("x", v.solution);
+      printf(", ");
+      print_vec("history", v.history);
+      printf("}");
+    }
+    printf("]}");
+  }
+  printf("]}\n");
+  return 0;
+}
+
 int main(int argc, char** argv) {
   try {
     const std::string mode = argc > 1 ? argv[1] : "";
     if (mode == "io" && argc == 3) return run_io(argv[2]);
     if (mode == "copy" && argc == 4) {
       OPTIMIZE_3D o;
-      o.read(argv[2]);
+      o.readState(argv[2]);
       o.write(argv[3]);
       printf("copy ok\n");
       return 0;
@@ -299,6 +380,9 @@ int main(int argc, char** argv) {
     if (mode == "ap" && argc == 6) return run_ap(argv[2], argv[3], atof(argv[4]), atof(argv[5]));
     if (mode == "blmvm" && argc == 3) return run_blmvm(argv[2]);
     if (mode == "optimize" && argc == 6) return run_optimize(argv[2], argv[3], atof(argv[4]), atoi(argv[5]));
+    if (mode == "readsetup" && argc == 4) return run_readsetup(argv[2], argv[3]);
+    if (mode == "tooth" && argc == 8)
+      return run_tooth(argv[2], argv[3], atoi(argv[4]), atof(argv[5]), atof(argv[6]), argv[7]);
   } catch (const Error& e) {
     fprintf(stderr, "quijibo::Error %d: %s\n", e.status, e.what());
     return 2;

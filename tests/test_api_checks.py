@@ -46,3 +46,13 @@ def test_device_tensor_checks_dtype_contiguity_and_size():
         api._device_tensor(t, "x", 12)
     with pytest.raises(ValueError, match="torch tensor"):
         api._device_tensor(t.numpy(), "x")
+
+
+def test_scoring_fields_reject_host_tensors():
+    with pytest.raises(ValueError, match="CUDA"):
+        api.inverted_distance(torch.zeros(120, dtype=torch.float64), _grid())
+    with pytest.raises(ValueError, match="CUDA"):
+        api.weight_scores(torch.zeros(120, dtype=torch.float64), _grid(), torch.zeros(120, dtype=torch.float64),
+                          _grid(), 10.0)
+    with pytest.raises(ValueError, match="CUDA"):
+        api.field_min_max(torch.zeros(120, dtype=torch.float64))

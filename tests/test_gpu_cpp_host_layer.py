@@ -18,10 +18,19 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 DRIVER = os.path.join(HERE, "cpp", "_build", "host_layer_driver")
 
 
+def _sphere_distance(res, center, lengths, r=0.8):
+    ax = [center[i] - lengths[i] / 2 + (np.arange(res[i]) + 0.5) * lengths[i] / res[i] for i in range(3)]
+    Z, Y, X = np.meshgrid(ax[2], ax[1], ax[0], indexing="ij")
+    return (np.sqrt(X ** 2 + Y ** 2 + Z ** 2) - r).reshape(-1)
+
+
 def _o3d(tmp_path, w, res):
+    """The workload as a .o3d; its distance field (a sphere) sets the grid that
+    OPTIMIZE_3D::read gives the fractal (setScoreFields, OPTIMIZE_3D.cpp:3442-3471)."""
     s = FM.O3D(res, w.center, w.lengths, FM.Polynomial4D(w.top.roots, w.top.powers, w.top.power_scalar),
                FM.Polynomial4D(w.bottom.roots, w.bottom.powers, w.bottom.power_scalar), w.exp_scaling, 0,
-               FM.Field3D(res, w.center, w.lengths))
+               FM.Field3D(res, w.center, w.lengths),
+               FM.Field3D(res, w.center, w.lengths, _sphere_distance(res, w.center, w.lengths)))
     p = tmp_path / "in.o3d"
     FM.write_o3d(p, s)
     return p
@@ -167,3 +176,37 @@ def test_blmvm_exponent_fit_follows_the_oracle(gpu, tmp_path):
     # objective amplify that to ~1e-6 in the state (the objective agrees to ~1e-8)
     assert np.allclose(out["x"], x, rtol=2e-5, atol=1e-8)
     assert hist[-1] < hist[0]
+
+
+def test_read_sets_up_the_score_fields(gpu, tmp_path):
+    """OPTIMIZE_3D::read's set-up (OPTIMIZE_3D.cpp:3348-3388): the grid from the
+    distance field, the fractal zeroed on it, the curvature field a copy of
+    the distance field, the weight field invertedDistanceNormalized(distance)
+    and its absolute sum, against the oracle."""
+    w = W.config2(8)
+    res = (22, 20, 18)
+    center, lengths = (0.2, -0.1, 0.05), (2.6, 2.4, 2.2)
+    dist = _sphere_distance(res, center, lengths, 0.7)
+    s = FM.O3D((5, 5, 5), (0.0, 0.0, 0.0), (3.0, 3.0, 3.0),
+               FM.Polynomial4D(w.top.roots, w.top.powers), FM.Polynomial4D(w.bottom.roots, w.bottom.powers), 1.0, 0,
+               FM.Field3D((5, 5, 5), (0.0, 0.0, 0.0), (3.0, 3.0, 3.0), np.arange(125.0)),
+               FM.Field3D(res, center, lengths, dist), FM.Field3D((2, 2, 2), data=np.ones(8)))
+    FM.write_o3d(tmp_path / "in.o3d", s)
+    out = json.loads(_run(["readsetup", str(tmp_path / "in.o3d"), str(tmp_path)]))
+    assert tuple(out["res"]) == res
+    fr = FM.read_field(tmp_path / "fractal.bin")
+    assert fr.res == res and fr.center == center and fr.lengths == lengths and not fr.data.any()
+    cv = FM.read_field(tmp_path / "curvature.bin")
+    assert cv.res == res and np.array_equal(cv.data, dist)
+    ref, a = O.inverted_distance(O.make_grid(res, center, lengths), dist, normalized=True)
+    wt = FM.read_field(tmp_path / "weight.bin")
+    inv = FM.read_field(tmp_path / "inverse.bin")
+    assert np.array_equal(wt.data, inv.data)
+    # the layer numbers are exact; the normalising sums are double-double on
+    # the device and serial in the oracle, so the scale factors differ by up
+    # to n u (u = 2^-53) relative
+    raw, _ = O.inverted_distance(O.make_grid(res, center, lengths), dist)
+    assert np.array_equal(np.sign(wt.data), np.sign(raw))
+    tol = ref.size * 2.0 ** -52
+    assert np.max(np.abs(wt.data - ref) / np.abs(ref)) <= tol
+    assert abs(out["inverse_sum"] - a) <= tol

@@ -143,10 +143,11 @@ def test_lagrange_request_matches_the_direct_evaluation(gpu):
 
 
 def test_refusal_aging_and_per_result_check(gpu):
-    """max_pending_work refuses a submission past the limit (QJ_ERR_BUSY) but
-    never one that arrives at an idle server; every completion carries its
-    sampled FP64 cross-check; an old request is admitted before smaller new
-    ones."""
+    """max_pending_work refuses a submission whose work queued ahead of it in
+    shortest-first order (its own included) passes the limit (QJ_ERR_BUSY),
+    but never one that arrives at an idle server, and not a smaller request
+    that only the larger ones are ahead of; every completion carries its
+    sampled FP64 cross-check."""
     from quijibo_amd._lib import QJError
     w = W.config2(64)
     work = 64 ** 3 * w.max_iter * (w.top.n + w.bottom.n)
@@ -155,43 +156,50 @@ def test_refusal_aging_and_per_result_check(gpu):
     big = W.config2(96)  # more work than the limit alone: accepted at an idle server
     a = srv.submit(big)
     with pytest.raises(QJError) as ei:
-        srv.submit(w)
+        srv.submit(big)  # the first one is ahead of it: 2 x 3.4 work
     assert ei.value.status == 7
+    b = srv.submit(w)  # only its own work is ahead of it in shortest-first order
+    done = srv.wait_all(2, timeout=60)
+    assert sorted(c.id for c in done) == sorted([a, b]) and all(c.status == 0 for c in done)
+    for c in done:
+        assert c.checked == 700 and c.check_sign_flips == 0 and c.check_class_mismatch == 0
+        assert 0 < c.check_max_abs_dF < 0.05 and c.check_max_rel_dF <= c.check_max_abs_dF
+    d = srv.submit(big)  # idle again: accepted
     done = srv.wait_all(1, timeout=60)
-    assert len(done) == 1 and done[0].id == a and done[0].status == 0
-    c = done[0]
-    assert c.checked == 700 and c.check_sign_flips == 0 and c.check_class_mismatch == 0
-    assert 0 < c.check_max_abs_dF < 0.05 and c.check_max_rel_dF <= c.check_max_abs_dF
-    b = srv.submit(w)  # idle again: accepted
-    done = srv.wait_all(1, timeout=60)
-    assert done[0].id == b and done[0].status == 0 and done[0].checked == 700
+    assert done[0].id == d and done[0].status == 0 and done[0].checked == 700
     srv.close()
 
 
 def test_aged_request_is_admitted_first(gpu):
-    """Once a waiting request is older than aging_s nothing behind it is
-    admitted before it: every small request admitted later than the big one's
-    aging time (plus a margin for one admission pass) is admitted after it, and
-    the big one is admitted while small ones keep arriving (no starvation)."""
+    """Once a waiting request is older than aging_s, requests of a quarter of
+    its footprint or more are not admitted before it: every medium request
+    admitted later than the big one's aging time (plus a margin for one
+    admission pass) is admitted after it, and the big one is admitted while
+    medium ones keep arriving (no starvation). Small requests are not held."""
     import time
-    small = W.config1(32)
-    big = W.config1(96)  # 6.75 MiB field: fits the 8 MiB budget only beside a few small ones
+    small = W.config1(16)   # 32 KiB field: below a quarter of the big one's footprint
+    medium = W.config1(64)  # 2 MiB field: at least a quarter of it
+    big = W.config1(96)     # 6.75 MiB field: fits the 8 MiB budget only beside no medium one
     aging = 0.05
     srv = S.Server(hbm_budget_bytes=8 << 20, chunk_voxels=16384, chunks_per_launch=4, aging_s=aging)
-    ids_small = [srv.submit(small) for _ in range(40)]  # about 10 MiB: more than the budget
+    ids_med = [srv.submit(medium) for _ in range(12)]  # 24 MiB: three budgets' worth
     bid = srv.submit(big)
+    ids_small = []
     t_end = srv.now() + 0.3
     while srv.now() < t_end:
+        ids_med.append(srv.submit(medium))
         ids_small.append(srv.submit(small))
         time.sleep(0.002)
-    done = srv.wait_all(len(ids_small) + 1, timeout=120)
+    n = len(ids_med) + len(ids_small) + 1
+    done = srv.wait_all(n, timeout=120)
     srv.close()
-    assert len(done) == len(ids_small) + 1 and all(c.status == 0 for c in done)
+    assert len(done) == n and all(c.status == 0 for c in done)
     by = {c.id: c for c in done}
     b = by[bid]
-    late = [by[i] for i in ids_small if by[i].admit_time > b.arrival_time + aging + 0.02]
-    print("big waited %.1f ms; %d small requests admitted after its aging time" %
-          (1e3 * (b.admit_time - b.arrival_time), len(late)), flush=True)
+    late = [by[i] for i in ids_med if by[i].admit_time > b.arrival_time + aging + 0.02]
+    held_small = [by[i] for i in ids_small if b.arrival_time + aging + 0.02 < by[i].admit_time < b.admit_time]
+    print("big waited %.1f ms; %d medium requests admitted after its aging time, %d small ones admitted while "
+          "it waited" % (1e3 * (b.admit_time - b.arrival_time), len(late), len(held_small)), flush=True)
     assert all(c.admit_time >= b.admit_time for c in late)
     assert b.admit_time <= b.arrival_time + aging + 0.25
 
