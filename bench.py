@@ -98,6 +98,8 @@ def main():
                     help="refuse submissions beyond this many seconds of backlog (0 = never refuse)")
     ap.add_argument("--serve-aging-s", type=float, default=0.5, help="requests waiting longer go first")
     ap.add_argument("--serve-check", type=int, default=512, help="cross-check samples per served result")
+    ap.add_argument("--serve-escalate", type=float, default=0.025,
+                    help="recompute a result in FP64 when its sampled max |dF| exceeds this (0 = never)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: the field, mesher and gradient paths run on a single MI355X "
@@ -302,7 +304,7 @@ def serve_stream(args, log):
     srv = S.Server(hbm_budget_bytes=int(args.serve_budget_gib * (1 << 30)), precision=FP32_BAND, band=args.band,
                    launch_work=args.serve_launch_work, inflight_launches=args.serve_inflight,
                    max_pending_work=args.serve_pending_s * rate, aging_s=args.serve_aging_s,
-                   check_samples=args.serve_check)
+                   check_samples=args.serve_check, escalate_abs_dF=args.serve_escalate)
     flags = MAP_BLACK_HOLE | MAP_EXTENDED_RANGE
     # warm the graphs with one small request
     srv.submit(reqs[0][1], flags=flags)
@@ -320,7 +322,9 @@ def serve_stream(args, log):
              "class_mismatch": int(sum(c.check_class_mismatch for c in ok)),
              "max_abs_dF": max((c.check_max_abs_dF for c in ok), default=0.0),
              "max_rel_dF": max((c.check_max_rel_dF for c in ok), default=0.0),
-             "samples_per_result": args.serve_check}
+             "samples_per_result": args.serve_check,
+             "escalated_to_fp64": int(sum(c.escalated for c in ok)),
+             "escalate_above_abs_dF": args.serve_escalate}
     return {"requests": len(reqs), "offered_rate_req_s": args.serve_rate, "completed": len(ok),
             "refused": refused, "wall_s": round(end - start, 3),
             "throughput_req_s": round(len(ok) / max(1e-9, end - start), 2),

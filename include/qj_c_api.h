@@ -406,6 +406,10 @@ typedef struct qj_server_config {
                                  launch (0 = 0.25) */
   int32_t check_samples;      /* per-result error statement: cells of every finished field re-evaluated with
                                  the reference method in FP64 and compared (qj_completion.checked ...; 0 = off) */
+  double escalate_abs_dF;     /* FP32_BAND servers with check_samples: a result whose sample shows a sign flip,
+                                 a class mismatch or max |F - F_fp64| above this is computed again with every
+                                 voxel in FP64 (exact integer powers) before it completes; its completion says
+                                 escalated = 1 and carries the second cross-check (0 = off) */
 } qj_server_config;
 
 typedef struct qj_request {
@@ -433,6 +437,7 @@ typedef struct qj_completion {
   int64_t check_class_mismatch;
   double check_max_abs_dF;
   double check_max_rel_dF;
+  int32_t escalated;        /* 1: recomputed in FP64 after its FP32 cross-check (escalate_abs_dF) */
 } qj_completion;
 
 qj_status qj_server_create(const qj_server_config* cfg, qj_server** out);

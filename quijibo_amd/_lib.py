@@ -62,7 +62,7 @@ class ServerConfig(ctypes.Structure):
                 ("inflight_launches", ctypes.c_int32), ("precision", ctypes.c_int32), ("band", ctypes.c_double),
                 ("device", ctypes.c_int32), ("launch_work", ctypes.c_double),
                 ("max_pending_work", ctypes.c_double), ("aging_s", ctypes.c_double),
-                ("check_samples", ctypes.c_int32)]
+                ("check_samples", ctypes.c_int32), ("escalate_abs_dF", ctypes.c_double)]
 
 
 class Request(ctypes.Structure):
@@ -75,7 +75,7 @@ class Completion(ctypes.Structure):
                 ("admit_time", ctypes.c_double), ("done_time", ctypes.c_double), ("d_field", ctypes.c_void_p),
                 ("voxels", ctypes.c_int64), ("checked", ctypes.c_int64), ("check_sign_flips", ctypes.c_int64),
                 ("check_class_mismatch", ctypes.c_int64), ("check_max_abs_dF", ctypes.c_double),
-                ("check_max_rel_dF", ctypes.c_double)]
+                ("check_max_rel_dF", ctypes.c_double), ("escalated", ctypes.c_int32)]
 
 
 # every symbol declared in include/qj_c_api.h: (name, restype, argtypes)

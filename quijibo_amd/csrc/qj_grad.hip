@@ -16,6 +16,10 @@
 //
 // k_adjoint: reverse mode through max_iter iterations of the map at points,
 //   recomputing every iteration's products instead of storing trajectories.
+//   Each point is a group of 8 lanes, each lane taking a share of the roots;
+//   the ordered product of the shares is a prefix/suffix scan of quaternion
+//   products over the group, and each root's gradient is summed over the
+//   wave's points by shuffles before one lane adds it to the block's LDS.
 #include <float.h>
 
 #include <string.h>
@@ -459,41 +463,85 @@ __device__ __forceinline__ Q4<T> pow_vjp(const Term<T>& t, const Q4<T>& tb, T e)
                coef * t.b.z + t.C * tb.z};
 }
 
-template <class T>
-struct Fwd {
-  Q4<T> Th, Bh;  // unit products
-  T Lt, Lb;      // log magnitudes
-};
-template <class T>
-__device__ __forceinline__ Fwd<T> forward(const MapArgs& a, const RootRec64* __restrict__ roots, const Q4<T>& q) {
-  Fwd<T> f;
-  f.Th = Q4<T>{1, 0, 0, 0};
-  f.Bh = Q4<T>{1, 0, 0, 0};
-  f.Lt = 0;
-  f.Lb = 0;
-  for (int i = 0; i < a.nt; i++) {
-    const RootRec64 rr = roots[i];
-    const Term<T> t = make_term(q, rr, T(rr.e));
-    f.Th = mul(f.Th, t.that);
-    f.Lt += T(rr.e) * t.lb;
-    if ((i & 7) == 7) f.Th = unit(f.Th);
-  }
-  for (int i = 0; i < a.nb; i++) {
-    const RootRec64 rr = roots[a.nt + i];
-    const Term<T> t = make_term(q, rr, T(rr.e));
-    f.Bh = mul(f.Bh, t.that);
-    f.Lb += T(rr.e) * t.lb;
-    if ((i & 7) == 7) f.Bh = unit(f.Bh);
-  }
-  f.Th = unit(f.Th);
-  f.Bh = unit(f.Bh);
-  return f;
-}
-
 constexpr int kAdjBlock = 128;
 constexpr int kMaxAdjIter = 16;
+// lanes per point: each lane of a group takes a contiguous share of the top
+// roots and of the bottom roots, so 40k points keep every SIMD busy
+constexpr int kAdjLanes = 8;
+
+template <class T>
+__device__ __forceinline__ Q4<T> shfl_up_q(const Q4<T>& a, int d) {
+  return {__shfl_up(a.w, d, kAdjLanes), __shfl_up(a.x, d, kAdjLanes), __shfl_up(a.y, d, kAdjLanes),
+          __shfl_up(a.z, d, kAdjLanes)};
+}
+template <class T>
+__device__ __forceinline__ Q4<T> shfl_down_q(const Q4<T>& a, int d) {
+  return {__shfl_down(a.w, d, kAdjLanes), __shfl_down(a.x, d, kAdjLanes), __shfl_down(a.y, d, kAdjLanes),
+          __shfl_down(a.z, d, kAdjLanes)};
+}
+template <class T>
+__device__ __forceinline__ Q4<T> shfl_q(const Q4<T>& a, int src) {
+  return {__shfl(a.w, src, kAdjLanes), __shfl(a.x, src, kAdjLanes), __shfl(a.y, src, kAdjLanes),
+          __shfl(a.z, src, kAdjLanes)};
+}
+// sum over the lanes of a group
+template <class T>
+__device__ __forceinline__ T group_sum(T v) {
+  for (int o = kAdjLanes / 2; o > 0; o >>= 1) v += __shfl_xor(v, o);
+  return v;
+}
+template <class T>
+__device__ __forceinline__ Q4<T> group_sum_q(const Q4<T>& a) {
+  return {group_sum(a.w), group_sum(a.x), group_sum(a.y), group_sum(a.z)};
+}
+// sum over the groups of a wave, lane by lane (the lanes that hold the same roots)
+__device__ __forceinline__ double across_groups(double v) {
+  for (int o = kAdjLanes; o < 64; o <<= 1) v += __shfl_xor(v, o);
+  return v;
+}
+
+// The unit product of polynomial roots [lo, hi) in order, its log magnitude,
+// and over the group: the whole polynomial's unit product and log magnitude,
+// and the unit product of the roots after this lane's share.
+template <class T>
+struct Share {
+  Q4<T> all, after;
+  T L;
+};
+template <class T>
+__device__ __forceinline__ Share<T> group_product(const RootRec64* __restrict__ roots, int lo, int hi, const Q4<T>& q,
+                                                  int sub) {
+  Q4<T> p{1, 0, 0, 0};
+  T l = 0;
+  for (int i = lo; i < hi; i++) {
+    const RootRec64 rr = roots[i];
+    const Term<T> t = make_term(q, rr, T(rr.e));
+    p = mul(p, t.that);
+    l += T(rr.e) * t.lb;
+    if (((i - lo) & 7) == 7) p = unit(p);
+  }
+  p = unit(p);
+  // ordered inclusive prefix and suffix products over the group (Hillis-Steele)
+  Q4<T> pre = p, suf = p;
+  for (int d = 1; d < kAdjLanes; d <<= 1) {
+    const Q4<T> y = shfl_up_q(pre, d);
+    const Q4<T> z = shfl_down_q(suf, d);
+    if (sub >= d) pre = mul(y, pre);
+    if (sub + d < kAdjLanes) suf = mul(suf, z);
+  }
+  Share<T> s;
+  s.all = unit(shfl_q(pre, kAdjLanes - 1));
+  const Q4<T> nxt = shfl_down_q(suf, 1);
+  s.after = sub + 1 < kAdjLanes ? unit(nxt) : Q4<T>{1, 0, 0, 0};
+  s.L = group_sum(l);
+  return s;
+}
 
 // slot layout per root i (top then bottom): 5 i + {0..3} = d/d root component, 5 i + 4 = d/d e_i
+// One group of kAdjLanes lanes per point; lane `sub` of a group takes top
+// roots [sub ct, sub ct + ct) and bottom roots [sub cb, sub cb + cb). Control
+// flow is uniform over the wave (every group runs max_iter steps and ct / cb
+// root steps, masked), so the shuffles always see active partners.
 template <class T>
 __global__ __launch_bounds__(kAdjBlock) void k_adjoint(MapArgs a, const RootRec64* __restrict__ roots,
                                                        const double* __restrict__ pts, int64_t n,
@@ -502,23 +550,32 @@ __global__ __launch_bounds__(kAdjBlock) void k_adjoint(MapArgs a, const RootRec6
   extern __shared__ double acc[];  // (kAdjBlock/64) x K
   for (int i = threadIdx.x; i < (kAdjBlock / 64) * K; i += kAdjBlock) acc[i] = 0.0;
   __syncthreads();
-  const int64_t pid = (int64_t)blockIdx.x * kAdjBlock + threadIdx.x;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int sub = lane & (kAdjLanes - 1);
+  const int64_t pid = ((int64_t)blockIdx.x * kAdjBlock + threadIdx.x) / kAdjLanes;
   const bool active = pid < n;
+  const int ct = (a.nt + kAdjLanes - 1) / kAdjLanes, cb = (a.nb + kAdjLanes - 1) / kAdjLanes;
+  const int t0 = sub * ct, t1 = min(a.nt, t0 + ct);
+  const int b0 = sub * cb, b1 = min(a.nb, b0 + cb);
   Q4<T> qs[kMaxAdjIter + 1];
   int Kx = 0;
   T gF = 0;
-  if (active) {
-    qs[0] = Q4<T>{T(pts[3 * pid]), T(pts[3 * pid + 1]), T(pts[3 * pid + 2]), T(a.slice)};
-    gF = T(dJdF[pid]);
-    T mag = sqrt(dot(qs[0], qs[0]));
-    while (mag < T(a.escape) && Kx < a.max_iter) {
-      const Fwd<T> f = forward(a, roots, qs[Kx]);
-      const Q4<T> Qh = mul(f.Th, conj(f.Bh));
-      const T scale = T(a.exp_scaling) * exp(f.Lt - f.Lb);
-      qs[Kx + 1] = sc(Qh, scale);
-      mag = sqrt(dot(qs[Kx + 1], qs[Kx + 1]));
-      Kx++;
-      if ((a.flags & QJ_MAP_BLACK_HOLE) && mag < T(10.0 * (double)FLT_MIN)) break;
+  qs[0] = active ? Q4<T>{T(pts[3 * pid]), T(pts[3 * pid + 1]), T(pts[3 * pid + 2]), T(a.slice)}
+                 : Q4<T>{1, 0, 0, 0};
+  if (active) gF = T(dJdF[pid]);
+  bool running = active && sqrt(dot(qs[0], qs[0])) < T(a.escape) && a.max_iter > 0;
+  for (int k = 0; k < a.max_iter; k++) {
+    const Q4<T> q = running ? qs[k] : Q4<T>{1, 0, 0, 0};
+    const Share<T> tp = group_product(roots, t0, t1, q, sub);
+    const Share<T> bt = group_product(roots + a.nt, b0, b1, q, sub);
+    if (running) {
+      const Q4<T> Qh = a.nb > 0 ? mul(tp.all, conj(bt.all)) : tp.all;
+      const T scale = T(a.exp_scaling) * exp(tp.L - (a.nb > 0 ? bt.L : T(0)));
+      qs[k + 1] = sc(Qh, scale);
+      Kx = k + 1;
+      const T mag = sqrt(dot(qs[k + 1], qs[k + 1]));
+      if (!(mag < T(a.escape)) || ((a.flags & QJ_MAP_BLACK_HOLE) && mag < T(10.0 * (double)FLT_MIN)))
+        running = false;
     }
   }
   // dJ/dq_K for F = log|q_K|
@@ -529,23 +586,27 @@ __global__ __launch_bounds__(kAdjBlock) void k_adjoint(MapArgs a, const RootRec6
   }
   for (int k = a.max_iter - 1; k >= 0; k--) {
     const bool live = active && k < Kx;
-    Q4<T> q = live ? qs[k] : Q4<T>{1, 0, 0, 0};
-    Fwd<T> f = forward(a, roots, q);
-    const T Qmag = exp(f.Lt - f.Lb);
+    const Q4<T> q = live ? qs[k] : Q4<T>{1, 0, 0, 0};
+    const Share<T> tp = group_product(roots, t0, t1, q, sub);
+    const Share<T> bt = group_product(roots + a.nt, b0, b1, q, sub);
+    const Q4<T> Th = tp.all;
+    const Q4<T> Bh = a.nb > 0 ? bt.all : Q4<T>{1, 0, 0, 0};
+    const T Qmag = exp(tp.L - (a.nb > 0 ? bt.L : T(0)));
     const Q4<T> Qbar = sc(qbar, T(a.exp_scaling));  // q_{k+1} = c Q
     Q4<T> qnew{0, 0, 0, 0};
     // top terms: tbar' = conj(Phat) Qbar Bhat conj(Shat_{i+1})
     {
-      const Q4<T> QB = mul(Qbar, f.Bh);
-      Q4<T> S{1, 0, 0, 0};
-      for (int i = a.nt - 1; i >= 0; i--) {
-        const RootRec64 rr = roots[i];
+      const Q4<T> QB = mul(Qbar, Bh);
+      Q4<T> S = tp.after;
+      for (int j = 0; j < ct; j++) {
+        const int i = t0 + ct - 1 - j;
         double g0 = 0, g1 = 0, g2 = 0, g3 = 0, g4 = 0;
-        if (live) {
+        if (live && i < t1) {
+          const RootRec64 rr = roots[i];
           const Term<T> t = make_term(q, rr, T(rr.e));
           const Q4<T> right = S;
           S = mul(t.that, S);
-          const Q4<T> P = (i > 0) ? mul(f.Th, conj(S)) : Q4<T>{1, 0, 0, 0};
+          const Q4<T> P = (i > 0) ? mul(Th, conj(S)) : Q4<T>{1, 0, 0, 0};
           const Q4<T> tb = mul(mul(conj(P), QB), conj(right));
           const Q4<T> bb = sc(pow_vjp(t, tb, T(rr.e)), Qmag);
           // d t / d e = t log(b); log(b) = (log|b|, th v/|v|)
@@ -554,27 +615,32 @@ __global__ __launch_bounds__(kAdjBlock) void k_adjoint(MapArgs a, const RootRec6
           const T ge = Qmag * dot(tb, mul(t.that, lg));
           qnew = Q4<T>{qnew.w + bb.w, qnew.x + bb.x, qnew.y + bb.y, qnew.z + bb.z};
           g0 = -(double)bb.w; g1 = -(double)bb.x; g2 = -(double)bb.y; g3 = -(double)bb.z; g4 = (double)ge;
-          if ((i & 7) == 0) S = unit(S);
+          if ((j & 7) == 7) S = unit(S);
         }
-        wave_acc(acc, K, 5 * i + 0, g0);
-        wave_acc(acc, K, 5 * i + 1, g1);
-        wave_acc(acc, K, 5 * i + 2, g2);
-        wave_acc(acc, K, 5 * i + 3, g3);
-        wave_acc(acc, K, 5 * i + 4, g4);
+        g0 = across_groups(g0);
+        g1 = across_groups(g1);
+        g2 = across_groups(g2);
+        g3 = across_groups(g3);
+        g4 = across_groups(g4);
+        if (lane < kAdjLanes && i < t1) {
+          double* r = acc + wave * K + 5 * i;
+          r[0] += g0; r[1] += g1; r[2] += g2; r[3] += g3; r[4] += g4;
+        }
       }
     }
     // bottom terms: ubar' = -conj(PBhat) Bhat conj(That) Qbar Bhat conj(SBhat_{j+1})
     if (a.nb > 0) {
-      const Q4<T> M = mul(mul(f.Bh, conj(f.Th)), mul(Qbar, f.Bh));
-      Q4<T> S{1, 0, 0, 0};
-      for (int j = a.nb - 1; j >= 0; j--) {
-        const RootRec64 rr = roots[a.nt + j];
+      const Q4<T> M = mul(mul(Bh, conj(Th)), mul(Qbar, Bh));
+      Q4<T> S = bt.after;
+      for (int jj = 0; jj < cb; jj++) {
+        const int j = b0 + cb - 1 - jj;
         double g0 = 0, g1 = 0, g2 = 0, g3 = 0, g4 = 0;
-        if (live) {
+        if (live && j < b1) {
+          const RootRec64 rr = roots[a.nt + j];
           const Term<T> t = make_term(q, rr, T(rr.e));
           const Q4<T> right = S;
           S = mul(t.that, S);
-          const Q4<T> P = (j > 0) ? mul(f.Bh, conj(S)) : Q4<T>{1, 0, 0, 0};
+          const Q4<T> P = (j > 0) ? mul(Bh, conj(S)) : Q4<T>{1, 0, 0, 0};
           const Q4<T> ub = sc(mul(mul(conj(P), M), conj(right)), T(-1));
           const Q4<T> bb = sc(pow_vjp(t, ub, T(rr.e)), Qmag);
           const T lv = t.rho > T(0) ? t.th / t.rho : T(0);
@@ -582,16 +648,20 @@ __global__ __launch_bounds__(kAdjBlock) void k_adjoint(MapArgs a, const RootRec6
           const T ge = Qmag * dot(ub, mul(t.that, lg));
           qnew = Q4<T>{qnew.w + bb.w, qnew.x + bb.x, qnew.y + bb.y, qnew.z + bb.z};
           g0 = -(double)bb.w; g1 = -(double)bb.x; g2 = -(double)bb.y; g3 = -(double)bb.z; g4 = (double)ge;
-          if ((j & 7) == 0) S = unit(S);
+          if ((jj & 7) == 7) S = unit(S);
         }
-        const int base = 5 * (a.nt + j);
-        wave_acc(acc, K, base + 0, g0);
-        wave_acc(acc, K, base + 1, g1);
-        wave_acc(acc, K, base + 2, g2);
-        wave_acc(acc, K, base + 3, g3);
-        wave_acc(acc, K, base + 4, g4);
+        g0 = across_groups(g0);
+        g1 = across_groups(g1);
+        g2 = across_groups(g2);
+        g3 = across_groups(g3);
+        g4 = across_groups(g4);
+        if (lane < kAdjLanes && j < b1) {
+          double* r = acc + wave * K + 5 * (a.nt + j);
+          r[0] += g0; r[1] += g1; r[2] += g2; r[3] += g3; r[4] += g4;
+        }
       }
     }
+    qnew = group_sum_q(qnew);
     if (live) qbar = qnew;
   }
   __syncthreads();
@@ -634,7 +704,8 @@ qj_status points_adjoint(qj_workspace* ws, const MapArgs& a, const qj_rational* 
     set_error("too many roots for the adjoint kernel (%d)", nr);
     return QJ_ERR_CAPACITY;
   }
-  const int64_t nblocks = n > 0 ? (n + kAdjBlock - 1) / kAdjBlock : 1;
+  constexpr int64_t kPointsPerBlock = kAdjBlock / kAdjLanes;
+  const int64_t nblocks = n > 0 ? (n + kPointsPerBlock - 1) / kPointsPerBlock : 1;
   qj_status st = ensure_partials(ws, nblocks * K, s);
   if (st) return st;
   if (n > 0) {

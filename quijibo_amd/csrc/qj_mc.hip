@@ -3,10 +3,16 @@
 // CPU mesher: same flags, same vertex set and numbering, same vertex
 // arithmetic, same triangle order, same degenerate filter.
 //
-// Pipeline per field (all HBM-bound; one lane per grid point / cell, point
-// blocks dealt to the XCDs in contiguous runs; no host synchronisation
-// between fields, see mc_run_tiles):
-//   k_mc_flags      cell case from the float sign test (the field read once)
+// Pipeline per field (one lane per grid point / cell, point blocks dealt to
+// the XCDs in contiguous runs; no host synchronisation between fields, see
+// mc_run_tiles):
+//   k_mc_cells      cell case from the float sign test (the 8 corners read
+//                   once), and which of the case row's triangles survive the
+//                   degenerate filter (edge < 1e-7 dropped): an FP32 estimate
+//                   of where each vertex sits on its edge clears every
+//                   triangle with fewer than two vertices near an edge end,
+//                   the rest are measured with the FP64 vertices; per block
+//                   of cells: kept-triangle count
 //   k_mc_edges      per grid point: which of its +x,+y,+z edges some adjacent
 //                   cell's case row references (gather of <= 7 neighbour flags);
 //                   per block of points: vertex count
@@ -14,10 +20,8 @@
 //                   reference numbers vertices by std::map<pair<int,int>> order
 //                   = lower endpoint, then x<y<z)
 //   k_mc_vertices   ids within the block, p0 + d0/(d0-d1) (p1-p0) at cell
-//                   centres, no contraction; near-edge-end marks
-//   k_mc_tricount   per block of cells: triangles that pass the degenerate
-//                   filter (edge < 1e-7 dropped)
-//   scan            block counts -> first triangle per block
+//                   centres, no contraction
+//   scan            block triangle counts -> first triangle per block
 //   k_mc_tris       each kept triangle written once, in cell then table order
 #include <float.h>
 #include <math.h>
@@ -34,14 +38,30 @@ namespace qj {
 
 constexpr int kMcBlock = 256;
 
+// edge id -> lower endpoint offset (dx,dy,dz) and axis
+constexpr void edge_parts(int e, int& dx, int& dy, int& dz, int& axis) {
+  if (e < 4) {
+    axis = 0; dx = 0; dy = e & 1; dz = e >> 1;
+  } else if (e < 8) {
+    axis = 1; dx = e & 1; dy = 0; dz = (e >> 1) & 1;
+  } else {
+    axis = 2; dx = e & 1; dy = (e >> 1) & 1; dz = 0;
+  }
+}
+
 struct McTables {
   signed char tri[256][16];
   unsigned char ntri[256];
   unsigned short emask[256];
+  // per case row and triangle: the bit mask of its 3 edges, and its 3 corners
+  // in table order, 5 bits each: dx | (dy + 2 dz) << 1 | axis << 3 (the row
+  // of owner points and the axis the triangle kernel looks up)
+  unsigned short tmask[256][5];
+  unsigned short tcorner[256][5];
 };
 
 constexpr McTables make_tables() {
-  McTables t{QJ_MC_TRI_INIT, {}, {}};
+  McTables t{QJ_MC_TRI_INIT, {}, {}, {}, {}};
   for (int c = 0; c < 256; c++) {
     int n = 0;
     unsigned m = 0;
@@ -51,6 +71,18 @@ constexpr McTables make_tables() {
     }
     t.ntri[c] = (unsigned char)(n / 3);
     t.emask[c] = (unsigned short)m;
+    for (int q = 0; q < n / 3; q++) {
+      unsigned tm = 0, tc = 0;
+      for (int j = 0; j < 3; j++) {
+        const int e = t.tri[c][3 * q + j];
+        int dx = 0, dy = 0, dz = 0, axis = 0;
+        edge_parts(e, dx, dy, dz, axis);
+        tm |= 1u << e;
+        tc |= (unsigned)(dx | (dy + 2 * dz) << 1 | axis << 3) << (5 * j);
+      }
+      t.tmask[c][q] = (unsigned short)tm;
+      t.tcorner[c][q] = (unsigned short)tc;
+    }
   }
   return t;
 }
@@ -79,15 +111,15 @@ __device__ __forceinline__ int tri_edge(const uint32_t* s_tri, int flag, int k) 
   return ((const signed char*)s_tri)[flag * 16 + k];
 }
 
-// edge id -> lower endpoint offset (dx,dy,dz) and axis
 __device__ __forceinline__ void edge_geom(int e, int& dx, int& dy, int& dz, int& axis) {
-  if (e < 4) {
-    axis = 0; dx = 0; dy = e & 1; dz = e >> 1;
-  } else if (e < 8) {
-    axis = 1; dx = e & 1; dy = 0; dz = (e >> 1) & 1;
-  } else {
-    axis = 2; dx = e & 1; dy = (e >> 1) & 1; dz = 0;
-  }
+  edge_parts(e, dx, dy, dz, axis);
+}
+
+// per-triangle tables (5 per case row) staged in LDS as 640 words
+__device__ __forceinline__ void stage_tri5(unsigned short* dst, const unsigned short (*src)[5]) {
+  const uint32_t* s32 = (const uint32_t*)&src[0][0];
+  uint32_t* d32 = (uint32_t*)dst;
+  for (int i = threadIdx.x; i < 640; i += kMcBlock) d32[i] = s32[i];
 }
 
 __device__ __forceinline__ double sq3(double e0, double e1, double e2) {
@@ -102,10 +134,11 @@ struct McState {
   uint8_t* emask = nullptr;
   uint32_t* cnt = nullptr;     // per point edge count -> vertex base (scanned in place; non-linear mesher)
   uint16_t* vb16 = nullptr;    // per point first vertex id within its block (linear mesher)
-  uint8_t* vnear = nullptr;    // per vertex: within kNearEnd of an edge end (or off the edge)
+  uint8_t* kmask = nullptr;    // per cell: bit t set when triangle t of its case row is kept
+  int64_t cap_kmask = 0;
   uint16_t* tloc16 = nullptr;  // per active cell first triangle within its block of cells
   uint32_t* blkt = nullptr;    // per block of cells triangle count -> first triangle of the block
-  int64_t cap_vnear = 0, cap_tloc16 = 0, cap_blkt = 0;
+  int64_t cap_tloc16 = 0, cap_blkt = 0;
   uint32_t* blkv = nullptr;    // per block vertex count -> first vertex id of the block
   int64_t cap_vb16 = 0, cap_blkv = 0;
   uint32_t* blk = nullptr;     // per block active count -> offset
@@ -169,18 +202,6 @@ __device__ __forceinline__ int64_t xcd_block(int64_t nb) {
   return x * per + (x < rem ? x : rem) + k;
 }
 
-__global__ __launch_bounds__(kMcBlock) void k_mc_flags(const double* __restrict__ field, GridGeom g, double iso,
-                                                       uint8_t* __restrict__ flags) {
-  const int64_t X = g.res[0], Y = g.res[1], Z = g.res[2], XY = X * Y, N = XY * Z;
-  const int64_t p = xcd_block((N + kMcBlock - 1) / kMcBlock) * kMcBlock + threadIdx.x;
-  if (p >= N) return;
-  int64_t x, y, z;
-  point_xyz(g, p, x, y, z);
-  int flag = 0;
-  if (x < X - 1 && y < Y - 1 && z < Z - 1) flag = cell_flag(field, p, X, XY, iso);
-  flags[p] = (uint8_t)flag;
-}
-
 // Block-wide exclusive prefix and total of a small per-thread count given as
 // three bit flags of weights W0, W1, W2 (edge bits 1,1,1; a triangle count
 // 0..5 in binary 1,2,4): three ballots and masked popcounts, no shuffles.
@@ -211,10 +232,23 @@ __device__ __forceinline__ BlockPrefix mc_block_prefix3(unsigned v) {
 __global__ __launch_bounds__(kMcBlock) void k_mc_edges(const uint8_t* __restrict__ flags, GridGeom g,
                                                        uint8_t* __restrict__ emask, uint32_t* __restrict__ blkv) {
   __shared__ unsigned short s_em[256];
-  stage_emask(s_em);
+  // the cells around the block's points: rows -a*X - b*XY (a, b in {0,1})
+  // from the block's first point minus one, 257 flags each
+  __shared__ uint8_t s_fl[4][kMcBlock + 1];
   const int64_t X = g.res[0], Y = g.res[1], Z = g.res[2], XY = X * Y, N = XY * Z;
   const int64_t lb = xcd_block((N + kMcBlock - 1) / kMcBlock);
-  const int64_t p = lb * kMcBlock + threadIdx.x;
+  const int64_t p0 = lb * kMcBlock;
+  const int t = threadIdx.x;
+  s_em[t] = kMc.emask[t];
+  for (int r = 0; r < 4; r++) {
+    const int64_t row = p0 - (r & 1) * X - (r >> 1) * XY - 1;
+    for (int j = t; j < kMcBlock + 1; j += kMcBlock) {
+      const int64_t q = row + j;
+      s_fl[r][j] = (q >= 0 && q < N) ? flags[q] : (uint8_t)0;
+    }
+  }
+  __syncthreads();
+  const int64_t p = p0 + t;
   unsigned bits = 0;
   if (p < N) {
     int64_t x, y, z;
@@ -224,7 +258,7 @@ __global__ __launch_bounds__(kMcBlock) void k_mc_edges(const uint8_t* __restrict
       unsigned ref = 0;
       for (int b = 0; b < 2; b++)
         for (int a = 0; a < 2; a++)
-          if (y >= a && z >= b) ref |= (s_em[flags[p - a * X - b * XY]] >> (a + 2 * b)) & 1u;
+          if (y >= a && z >= b) ref |= (s_em[s_fl[a + 2 * b][t + 1]] >> (a + 2 * b)) & 1u;
       bits |= ref;
     }
     // y-edge (p, p+X): cells p - (a,0,b); local id 4 + a + 2b
@@ -232,7 +266,7 @@ __global__ __launch_bounds__(kMcBlock) void k_mc_edges(const uint8_t* __restrict
       unsigned ref = 0;
       for (int b = 0; b < 2; b++)
         for (int a = 0; a < 2; a++)
-          if (x >= a && z >= b) ref |= (s_em[flags[p - a - b * XY]] >> (4 + a + 2 * b)) & 1u;
+          if (x >= a && z >= b) ref |= (s_em[s_fl[2 * b][t + 1 - a]] >> (4 + a + 2 * b)) & 1u;
       bits |= ref << 1;
     }
     // z-edge (p, p+XY): cells p - (a,b,0); local id 8 + a + 2b
@@ -240,7 +274,7 @@ __global__ __launch_bounds__(kMcBlock) void k_mc_edges(const uint8_t* __restrict
       unsigned ref = 0;
       for (int b = 0; b < 2; b++)
         for (int a = 0; a < 2; a++)
-          if (x >= a && y >= b) ref |= (s_em[flags[p - a - b * X]] >> (8 + a + 2 * b)) & 1u;
+          if (x >= a && y >= b) ref |= (s_em[s_fl[b][t + 1 - a]] >> (8 + a + 2 * b)) & 1u;
       bits |= ref << 2;
     }
     emask[p] = (uint8_t)bits;
@@ -249,31 +283,140 @@ __global__ __launch_bounds__(kMcBlock) void k_mc_edges(const uint8_t* __restrict
   if (threadIdx.x == 0) blkv[lb] = vp.total;
 }
 
+// One vertex of computeEdgeInterpolations (TRIANGLE_MESH.cpp:2373-2425): the
+// edge from point (x, y, z) along `axis`, p0 + d0/(d0-d1) (p1-p0) between the
+// two cell centres, in the reference's operation order with no contraction.
+// k_mc_vertices writes it; k_mc_cells measures triangles with it, so both
+// see the same bits.
+__device__ __forceinline__ void mc_vertex(const GridGeom& g, int x, int y, int z, int axis, double f0, double f1,
+                                          double iso, double* v) {
+#pragma clang fp contract(off)
+  double a[3], b[3];
+  cell_center(g, x, y, z, a);
+  cell_center(g, x + (axis == 0), y + (axis == 1), z + (axis == 2), b);
+  const double d0 = f0 - iso;
+  const double off0 = b[0] - a[0], off1 = b[1] - a[1], off2 = b[2] - a[2];
+  const double d1 = f1 - iso;
+  const double interpolation = d0 / (d0 - d1);
+  v[0] = a[0] + interpolation * off0;
+  v[1] = a[1] + interpolation * off1;
+  v[2] = a[2] + interpolation * off2;
+}
+
+__device__ __forceinline__ double edge_len(const double* a, const double* b) {
+#pragma clang fp contract(off)
+  const double e0 = a[0] - b[0], e1 = a[1] - b[1], e2 = a[2] - b[2];
+  return sqrt(e0 * e0 + e1 * e1 + e2 * e2);  // norm(v0 - v1), VEC3.h:215-216
+}
+
+// The degenerate filter (TRIANGLE_MESH.cpp:1088-1102) drops triangles with a
+// side shorter than 1e-7. Two vertices of one triangle lie on edges of one
+// cell: parallel or skew edges are at least one grid spacing h apart, and on
+// two edges meeting at a corner c the vertices satisfy |v1 - v2| >=
+// max |vi - c| (the edges are perpendicular). So with h >= kPrefilterSpacing
+// a triangle with at most one vertex within 2e-7 of an end of its edge has
+// every side longer than 1e-7. k_mc_cells marks a vertex "near" from an FP32
+// estimate t' of its place t on the edge: near unless t' h >= kNearLimit and
+// (1 - t') h >= kNearLimit. |t' - t| stays below 2.5e-7 (corner values
+// rounded to float, one division and one subtraction), so with
+// h <= kPrefilterMaxSpacing every vertex within 2e-7 of an end is marked
+// (2e-7 + 2.5e-7 h < 1e-6); NaN or infinite corners mark it too. Triangles
+// with two or three marked vertices are measured with the FP64 vertices.
+constexpr float kNearLimit = 1e-6f;
+constexpr double kPrefilterSpacing = 1e-6;
+constexpr double kPrefilterMaxSpacing = 3.0;
+
+// Cell case and kept triangles of every cell. flags[p] is the case of the
+// cell with lowest corner p (0 on the last layer of each axis), kmask[p] bit t
+// is set when triangle t of that case row survives the filter, blkt[block]
+// counts the block's kept triangles. prefilter = 0 measures every triangle.
+// The near test runs on all 12 edges with fixed corner indices (a vertex at
+// place t on a crossing edge has t = |v0| / |v0 - v1| and 1 - t = |v1| /
+// |v0 - v1|, so no division), masked by the case row's edges.
+__global__ __launch_bounds__(kMcBlock) void k_mc_cells(const double* __restrict__ field, GridGeom g, double iso,
+                                                       int prefilter, uint8_t* __restrict__ flags,
+                                                       uint8_t* __restrict__ kmask, uint32_t* __restrict__ blkt) {
+  __shared__ uint32_t s_tm32[256 * 5 / 2];
+  __shared__ unsigned short s_em[256];
+  __shared__ uint8_t s_ntri[256];
+  unsigned short* s_tm = (unsigned short*)s_tm32;
+  stage_tri5(s_tm, kMc.tmask);
+  s_em[threadIdx.x] = kMc.emask[threadIdx.x];
+  s_ntri[threadIdx.x] = kMc.ntri[threadIdx.x];
+  __syncthreads();
+  const int64_t X = g.res[0], Y = g.res[1], Z = g.res[2], XY = X * Y, N = XY * Z;
+  const int64_t lb = xcd_block((N + kMcBlock - 1) / kMcBlock);
+  const int64_t p = lb * kMcBlock + threadIdx.x;
+  int flag = 0;
+  unsigned keep = 0;
+  if (p < N) {
+    int64_t x, y, z;
+    point_xyz(g, p, x, y, z);
+    if (x < X - 1 && y < Y - 1 && z < Z - 1) {
+      float v[8];
+#pragma unroll
+      for (int k = 0; k < 8; k++) {
+        const int dx = k >> 2, dy = (k >> 1) & 1, dz = k & 1;
+        // CUBE corners are floats (TRIANGLE_MESH.h:283-288); test at TRIANGLE_MESH.cpp:1037-1039
+        v[k] = (float)(field[p + dx + dy * X + dz * XY] - iso);
+        flag |= (v[k] > 0.0f) << k;
+      }
+      const int n = s_ntri[flag];
+      if (n) {
+        const float h[3] = {fabsf((float)g.dx[0]), fabsf((float)g.dx[1]), fabsf((float)g.dx[2])};
+        unsigned near = 0;
+#pragma unroll
+        for (int e = 0; e < 12; e++) {
+          int dx = 0, dy = 0, dz = 0, axis = 0;
+          edge_parts(e, dx, dy, dz, axis);
+          const int k0 = 4 * dx + 2 * dy + dz, k1 = k0 + (axis == 0 ? 4 : (axis == 1 ? 2 : 1));
+          const float a0 = fabsf(v[k0]), a1 = fabsf(v[k1]), lim = kNearLimit * fabsf(v[k0] - v[k1]);
+          near |= (a0 * h[axis] >= lim && a1 * h[axis] >= lim) ? 0u : (1u << e);
+        }
+        near &= s_em[flag];
+        for (int t = 0; t < n; t++) {
+          const unsigned tm = s_tm[flag * 5 + t];
+          bool degenerate = false;
+          if (!prefilter || __popc(near & tm) >= 2) {
+            // measured with the FP64 vertices (the corner values read again:
+            // this path is rare, and a register array indexed per lane would
+            // go to scratch)
+            double w[3][3];
+            unsigned r = tm;
+            for (int j = 0; j < 3; j++) {
+              const int e = __ffs(r) - 1;
+              r &= r - 1;
+              int dx, dy, dz, axis;
+              edge_geom(e, dx, dy, dz, axis);
+              const int64_t q0 = p + dx + dy * X + dz * XY;
+              const int64_t q1 = q0 + (axis == 0 ? 1 : (axis == 1 ? X : XY));
+              mc_vertex(g, (int)x + dx, (int)y + dy, (int)z + dz, axis, field[q0], field[q1], iso, w[j]);
+            }
+            const double eps = 1e-7;
+            degenerate = edge_len(w[0], w[1]) < eps || edge_len(w[0], w[2]) < eps || edge_len(w[1], w[2]) < eps;
+          }
+          if (!degenerate) keep |= 1u << t;
+        }
+      }
+    }
+    flags[p] = (uint8_t)flag;
+    kmask[p] = (uint8_t)keep;
+  }
+  const BlockPrefix tp = mc_block_prefix3<1, 2, 4>((unsigned)__popc(keep));
+  if (threadIdx.x == 0) blkt[lb] = tp.total;
+}
+
 // computeEdgeInterpolations (TRIANGLE_MESH.cpp:2373-2425). Vertex ids follow
 // the point order (the block's first id from blk_off, then the prefix within
-// the block, which is also kept in vb16 for the triangle kernel).
-//
-// vnear marks vertices closer than kNearEnd to an end of their edge (or not
-// between its ends). Two vertices of one triangle lie on edges of one cell:
-// parallel or skew edges are at least one grid spacing h apart, and on two
-// edges meeting at a corner c the vertices satisfy |v1 - v2| >= max |vi - c|.
-// With h >= kPrefilterSpacing, a triangle with fewer than two marked vertices
-// therefore has every side longer than the 1e-7 of the degenerate test, which
-// the triangle kernel then skips (it reads three bytes instead of nine
-// coordinates); triangles with two or three marked vertices take the test.
-constexpr double kNearEnd = 2e-7;
-constexpr double kPrefilterSpacing = 1e-6;
-
-// The tile's vertices go to verts / vnear from the running vertex total
-// base[0] (several tiles share the buffers); ids at or beyond cap are not
-// written (the host sees the overflow in the totals and meshes again).
+// the block, which is also kept in vb16 for the triangle kernel). The tile's
+// vertices go to verts from the running vertex total base[0] (several tiles
+// share the buffer); ids at or beyond cap are not written (the host sees the
+// overflow in the totals and meshes again).
 __global__ __launch_bounds__(kMcBlock) void k_mc_vertices(const double* __restrict__ field, GridGeom g, double iso,
                                                           const uint8_t* __restrict__ emask,
                                                           const uint32_t* __restrict__ blk_off,
                                                           uint16_t* __restrict__ vb16, const uint64_t* __restrict__ base,
-                                                          int64_t cap, double* __restrict__ verts,
-                                                          uint8_t* __restrict__ vnear) {
-#pragma clang fp contract(off)
+                                                          int64_t cap, double* __restrict__ verts) {
   // the block's vertices (at most 3 per point) are listed in LDS in id order
   // and shared out one per thread, so the FP64 work is not spread over
   // mostly idle lanes
@@ -302,130 +445,76 @@ __global__ __launch_bounds__(kMcBlock) void k_mc_vertices(const double* __restri
     const int axis = s_ax[j];
     int64_t x64, y64, z64;
     point_xyz(g, q, x64, y64, z64);
-    const int x = (int)x64, y = (int)y64, z = (int)z64;
-    double a[3], b[3];
-    cell_center(g, x, y, z, a);
-    cell_center(g, x + (axis == 0), y + (axis == 1), z + (axis == 2), b);
-    const double d0 = field[q] - iso;
-    const double off0 = b[0] - a[0], off1 = b[1] - a[1], off2 = b[2] - a[2];
-    const double d1 = field[q + (axis == 0 ? 1 : (axis == 1 ? X : XY))] - iso;
-    const double interpolation = d0 / (d0 - d1);
-    const double v0 = a[0] + interpolation * off0, v1 = a[1] + interpolation * off1, v2 = a[2] + interpolation * off2;
-    verts[3 * id + 0] = v0;
-    verts[3 * id + 1] = v1;
-    verts[3 * id + 2] = v2;
-    const double da = sq3(v0 - a[0], v1 - a[1], v2 - a[2]), db = sq3(v0 - b[0], v1 - b[1], v2 - b[2]);
-    vnear[id] = (da >= kNearEnd * kNearEnd && db >= kNearEnd * kNearEnd) ? 0 : 1;
+    double w[3];
+    mc_vertex(g, (int)x64, (int)y64, (int)z64, axis, field[q], field[q + (axis == 0 ? 1 : (axis == 1 ? X : XY))], iso,
+              w);
+    verts[3 * id + 0] = w[0];
+    verts[3 * id + 1] = w[1];
+    verts[3 * id + 2] = w[2];
   }
 }
 
-__device__ __forceinline__ double edge_len(const double* a, const double* b) {
-#pragma clang fp contract(off)
-  const double e0 = a[0] - b[0], e1 = a[1] - b[1], e2 = a[2] - b[2];
-  return sqrt(e0 * e0 + e1 * e1 + e2 * e2);  // norm(v0 - v1), VEC3.h:215-216
-}
+// The vertex ids of the points that own the edges of the block's cells: the
+// owner of an edge of cell p is p + dx + dy*X + dz*XY, so they lie in 4 rows
+// of 257 points from the block's first point (this row, +y, +z, +y+z). Per
+// owner: its first vertex id (its block's first id + its first id within the
+// block) and its edge mask, staged in LDS so each triangle corner is an LDS
+// read instead of three gathers.
+struct OwnerRows {
+  uint32_t vid[4][kMcBlock + 1];
+  uint8_t em[4][kMcBlock + 1];
+};
 
-// The vertex ids of triangle t of a cell's case row (tile-local): the owner
-// point of each edge, its block's first id, its first id within the block
-// and the edges of lower axis it also owns.
-__device__ __forceinline__ void tri_vertices(const uint32_t* s_tri, int flag, int t, int64_t p, int64_t X, int64_t XY,
+__device__ __forceinline__ void stage_owners(OwnerRows& o, int64_t p0, int64_t X, int64_t XY, int64_t N,
                                              const uint8_t* __restrict__ emask, const uint32_t* __restrict__ blk_off,
-                                             const uint16_t* __restrict__ vb16, int32_t* v) {
-  for (int c = 0; c < 3; c++) {
-    int dx, dy, dz, axis;
-    edge_geom(tri_edge(s_tri, flag, 3 * t + c), dx, dy, dz, axis);
-    const int64_t owner = p + dx + dy * X + dz * XY;
-    v[c] = (int32_t)(blk_off[owner / kMcBlock] + vb16[owner] + __popc(emask[owner] & ((1u << axis) - 1u)));
+                                             const uint16_t* __restrict__ vb16) {
+  for (int r = 0; r < 4; r++) {
+    const int64_t row = p0 + (r & 1) * X + (r >> 1) * XY;
+    for (int j = threadIdx.x; j < kMcBlock + 1; j += kMcBlock) {
+      const int64_t q = row + j;
+      const bool in = q < N;
+      o.vid[r][j] = in ? blk_off[q / kMcBlock] + vb16[q] : 0u;
+      o.em[r][j] = in ? emask[q] : (uint8_t)0;
+    }
   }
-}
-
-// The degenerate filter (TRIANGLE_MESH.cpp:1088-1102): a side shorter than
-// 1e-7. With the near-end marks (prefilter) only triangles with two or three
-// marked vertices are measured; vertices beyond the buffer's capacity (a
-// pass whose results the host discards) are not read.
-__device__ __forceinline__ bool degenerate_tri(const int32_t* v, const double* __restrict__ verts,
-                                               const uint8_t* __restrict__ vnear, bool prefilter, int64_t vfirst,
-                                               int64_t cap) {
-  if (vfirst + v[0] >= cap || vfirst + v[1] >= cap || vfirst + v[2] >= cap) return false;
-  if (prefilter &&
-      (int)vnear[vfirst + v[0]] + (int)vnear[vfirst + v[1]] + (int)vnear[vfirst + v[2]] < 2)
-    return false;
-  const double eps = 1e-7;
-  const double* p0 = verts + 3 * (vfirst + v[0]);
-  const double* p1 = verts + 3 * (vfirst + v[1]);
-  const double* p2 = verts + 3 * (vfirst + v[2]);
-  return edge_len(p0, p1) < eps || edge_len(p0, p2) < eps || edge_len(p1, p2) < eps;
-}
-
-// Kept triangles per block of cells (the filter applied before any triangle is
-// written, so each triangle is written once, at its final position).
-__global__ __launch_bounds__(kMcBlock) void k_mc_tricount(const uint8_t* __restrict__ flags, GridGeom g,
-                                                          const uint8_t* __restrict__ emask,
-                                                          const uint32_t* __restrict__ blk_off,
-                                                          const uint16_t* __restrict__ vb16,
-                                                          const uint64_t* __restrict__ base, int64_t vcap,
-                                                          const double* __restrict__ verts,
-                                                          const uint8_t* __restrict__ vnear, int prefilter,
-                                                          uint32_t* __restrict__ blkt) {
-  __shared__ uint32_t s_tri[1024];
-  __shared__ uint8_t s_ntri[256];
-  stage_tri(s_tri, s_ntri);
-  const int64_t X = g.res[0], XY = X * g.res[1], N = XY * g.res[2];
-  const int64_t lb = xcd_block((N + kMcBlock - 1) / kMcBlock);
-  const int64_t p = lb * kMcBlock + threadIdx.x;
-  const int flag = p < N ? flags[p] : 0;
-  const int n = s_ntri[flag];
-  const int64_t vfirst = (int64_t)base[0];
-  unsigned kept = 0;
-  for (int t = 0; t < n; t++) {
-    int32_t v[3];
-    tri_vertices(s_tri, flag, t, p, X, XY, emask, blk_off, vb16, v);
-    kept += degenerate_tri(v, verts, vnear, prefilter != 0, vfirst, vcap) ? 0u : 1u;
-  }
-  const BlockPrefix tp = mc_block_prefix3<1, 2, 4>(kept);
-  if (threadIdx.x == 0) blkt[lb] = tp.total;
 }
 
 // addVertexPairTriangle (TRIANGLE_MESH.cpp:2490-2503) in cell order then
-// table order, degenerate triangles dropped: each cell's kept triangles go
-// to the running triangle total base[1] + its block's first triangle + the
-// kept triangles of earlier cells of the block. Ids at or beyond tcap are not
-// written.
-__global__ __launch_bounds__(kMcBlock) void k_mc_tris(const uint8_t* __restrict__ flags, GridGeom g,
+// table order, degenerate triangles dropped (kmask from k_mc_cells): each
+// cell's kept triangles go to the running triangle total base[1] + its
+// block's first triangle + the kept triangles of earlier cells of the block.
+// Ids at or beyond tcap are not written.
+__global__ __launch_bounds__(kMcBlock) void k_mc_tris(const uint8_t* __restrict__ flags,
+                                                      const uint8_t* __restrict__ kmask, GridGeom g,
                                                       const uint8_t* __restrict__ emask,
                                                       const uint32_t* __restrict__ blk_off,
                                                       const uint16_t* __restrict__ vb16,
-                                                      const uint64_t* __restrict__ base, int64_t vcap,
-                                                      const double* __restrict__ verts,
-                                                      const uint8_t* __restrict__ vnear, int prefilter,
+                                                      const uint64_t* __restrict__ base,
                                                       const uint32_t* __restrict__ blk_toff, int64_t tcap,
                                                       int32_t* __restrict__ tris) {
-  __shared__ uint32_t s_tri[1024];
-  __shared__ uint8_t s_ntri[256];
-  stage_tri(s_tri, s_ntri);
+  __shared__ uint32_t s_tc32[256 * 5 / 2];
+  __shared__ OwnerRows s_own;
+  unsigned short* s_tc = (unsigned short*)s_tc32;
   const int64_t X = g.res[0], XY = X * g.res[1], N = XY * g.res[2];
   const int64_t lb = xcd_block((N + kMcBlock - 1) / kMcBlock);
   const int64_t p = lb * kMcBlock + threadIdx.x;
   const int flag = p < N ? flags[p] : 0;
-  const int n = s_ntri[flag];
-  const int64_t vfirst = (int64_t)base[0];
-  int32_t v[5][3];
-  unsigned keepm = 0, kept = 0;
-  for (int t = 0; t < n; t++) {
-    tri_vertices(s_tri, flag, t, p, X, XY, emask, blk_off, vb16, v[t]);
-    if (!degenerate_tri(v[t], verts, vnear, prefilter != 0, vfirst, vcap)) {
-      keepm |= 1u << t;
-      kept++;
-    }
-  }
-  const BlockPrefix tp = mc_block_prefix3<1, 2, 4>(kept);
+  const unsigned keep = p < N ? kmask[p] : 0u;
+  stage_owners(s_own, lb * kMcBlock, X, XY, N, emask, blk_off, vb16);
+  stage_tri5(s_tc, kMc.tcorner);
+  __syncthreads();
+  const BlockPrefix tp = mc_block_prefix3<1, 2, 4>((unsigned)__popc(keep));
   int64_t o = (int64_t)base[1] + blk_toff[lb] + tp.excl;
-  for (int t = 0; t < n; t++) {
-    if (!(keepm & (1u << t))) continue;
+  for (unsigned m = keep; m; m &= m - 1) {
+    const int t = __ffs(m) - 1;
     if (o < tcap) {
-      tris[3 * o + 0] = v[t][0];
-      tris[3 * o + 1] = v[t][1];
-      tris[3 * o + 2] = v[t][2];
+      const unsigned tc = s_tc[flag * 5 + t];
+      for (int c = 0; c < 3; c++) {
+        const unsigned d = (tc >> (5 * c)) & 31u;
+        const int dx = d & 1, r = (d >> 1) & 3, axis = d >> 3;
+        const int j = (int)threadIdx.x + dx;
+        tris[3 * o + c] = (int32_t)(s_own.vid[r][j] + __popc(s_own.em[r][j] & ((1u << axis) - 1u)));
+      }
     }
     o++;
   }
@@ -516,10 +605,10 @@ qj_status mc_run_tiles(qj_workspace* ws, int n, const double* const* fields, con
     total_pts += N;
     // the degenerate-test prefilter holds when the cell-centre map is a
     // rotation (edges meeting at a corner stay perpendicular) and every grid
-    // spacing is at least kPrefilterSpacing
+    // spacing lies in [kPrefilterSpacing, kPrefilterMaxSpacing]
     bool pf = true;
     for (int i = 0; i < 3; i++) {
-      pf &= fabs(g.dx[i]) >= kPrefilterSpacing;
+      pf &= fabs(g.dx[i]) >= kPrefilterSpacing && fabs(g.dx[i]) <= kPrefilterMaxSpacing;
       for (int j = 0; j < 3; j++) {
         const double d = g.rot[i] * g.rot[j] + g.rot[3 + i] * g.rot[3 + j] + g.rot[6 + i] * g.rot[6 + j];
         pf &= fabs(d - (i == j ? 1.0 : 0.0)) < 1e-9;
@@ -529,7 +618,8 @@ qj_status mc_run_tiles(qj_workspace* ws, int n, const double* const* fields, con
   }
   qj_status st;
   const int64_t nbmax = nblocks(nmax);
-  if ((st = grow(m.flags, m.cap_flags, nmax, s)) || (st = grow(m.emask, m.cap_emask, nmax, s)) ||
+  if ((st = grow(m.flags, m.cap_flags, nmax, s)) || (st = grow(m.kmask, m.cap_kmask, nmax, s)) ||
+      (st = grow(m.emask, m.cap_emask, nmax, s)) ||
       (st = grow(m.vb16, m.cap_vb16, nmax, s)) || (st = grow(m.blkv, m.cap_blkv, nbmax, s)) ||
       (st = grow(m.blkt, m.cap_blkt, nbmax, s)) || (st = grow(m.d_counts, m.cap_counts, 2 * (int64_t)n, s)))
     return st;
@@ -549,12 +639,11 @@ qj_status mc_run_tiles(qj_workspace* ws, int n, const double* const* fields, con
   // first sizes: a quarter of the points as vertices, half as triangles
   // (a 256^3 torus tile holds 0.42 and 0.73 of its points)
   if (m.cap_v == 0 && (st = grow(m.verts, m.cap_v, 3 * (total_pts / 4 + 1024), s))) return st;
-  if (m.cap_vnear == 0 && (st = grow(m.vnear, m.cap_vnear, total_pts / 4 + 1024, s))) return st;
   if (m.cap_t == 0 && (st = grow(m.tris, m.cap_t, 3 * (total_pts / 2 + 1024), s))) return st;
   std::vector<int64_t> counts(2 * (size_t)n);
   uint64_t totals[2] = {0, 0};
   for (int pass = 0; pass < 2; pass++) {
-    const int64_t vcap = std::min(m.cap_v / 3, m.cap_vnear), tcap = m.cap_t / 3;
+    const int64_t vcap = m.cap_v / 3, tcap = m.cap_t / 3;
     hipLaunchKernelGGL(k_mc_reset, dim3(1), dim3(64), 0, s, m.d_base);
     QJ_HIP(hipGetLastError());
     for (int k = 0; k < n; k++) {
@@ -566,21 +655,18 @@ qj_status mc_run_tiles(qj_workspace* ws, int n, const double* const* fields, con
         hipLaunchKernelGGL(k_mc_zero_totals, dim3(1), dim3(64), 0, s, m.totals);
         QJ_HIP(hipGetLastError());
       } else {
-        hipLaunchKernelGGL(k_mc_flags, nb, dim3(kMcBlock), 0, s, fields[k], g, iso, m.flags);
+        const int pf = prefilter[(size_t)k] ? 1 : 0;
+        hipLaunchKernelGGL(k_mc_cells, nb, dim3(kMcBlock), 0, s, fields[k], g, iso, pf, m.flags, m.kmask, m.blkt);
         QJ_HIP(hipGetLastError());
         hipLaunchKernelGGL(k_mc_edges, nb, dim3(kMcBlock), 0, s, m.flags, g, m.emask, m.blkv);
         QJ_HIP(hipGetLastError());
         if ((st = scan_u32(m.blkv, m.blkv, nb.x, m.totals + 1, m.tmp, m.tmp_bytes, s))) return st;
         hipLaunchKernelGGL(k_mc_vertices, nb, dim3(kMcBlock), 0, s, fields[k], g, iso, m.emask, m.blkv, m.vb16,
-                           (const uint64_t*)m.d_base, vcap, m.verts, m.vnear);
-        QJ_HIP(hipGetLastError());
-        const int pf = prefilter[(size_t)k] ? 1 : 0;
-        hipLaunchKernelGGL(k_mc_tricount, nb, dim3(kMcBlock), 0, s, m.flags, g, m.emask, m.blkv, m.vb16,
-                           (const uint64_t*)m.d_base, vcap, m.verts, m.vnear, pf, m.blkt);
+                           (const uint64_t*)m.d_base, vcap, m.verts);
         QJ_HIP(hipGetLastError());
         if ((st = scan_u32(m.blkt, m.blkt, nb.x, m.totals + 2, m.tmp, m.tmp_bytes, s))) return st;
-        hipLaunchKernelGGL(k_mc_tris, nb, dim3(kMcBlock), 0, s, m.flags, g, m.emask, m.blkv, m.vb16,
-                           (const uint64_t*)m.d_base, vcap, m.verts, m.vnear, pf, m.blkt, tcap, m.tris);
+        hipLaunchKernelGGL(k_mc_tris, nb, dim3(kMcBlock), 0, s, m.flags, m.kmask, g, m.emask, m.blkv, m.vb16,
+                           (const uint64_t*)m.d_base, m.blkt, tcap, m.tris);
         QJ_HIP(hipGetLastError());
       }
       hipLaunchKernelGGL(k_mc_advance, dim3(1), dim3(64), 0, s, m.totals, m.d_base, k, m.d_counts);
@@ -596,7 +682,6 @@ qj_status mc_run_tiles(qj_workspace* ws, int n, const double* const* fields, con
     }
     // outgrown: size the buffers from this pass's totals and mesh again
     if ((st = grow(m.verts, m.cap_v, 3 * (int64_t)totals[0], s)) ||
-        (st = grow(m.vnear, m.cap_vnear, (int64_t)totals[0], s)) ||
         (st = grow(m.tris, m.cap_t, 3 * (int64_t)totals[1], s)))
       return st;
   }
@@ -639,7 +724,7 @@ void mc_free(qj_workspace* ws) {
   hipFree(m->emask);
   hipFree(m->cnt);
   hipFree(m->vb16);
-  hipFree(m->vnear);
+  hipFree(m->kmask);
   hipFree(m->tloc16);
   hipFree(m->blkt);
   hipFree(m->blkv);

@@ -19,7 +19,9 @@
 // graph's first kernel reads from coherent pinned host memory; requests that
 // arrive while launches are on the GPU join the next launch. A finished
 // field gets its sampled FP64 cross-check (check_samples) on the server's own
-// stream before its completion is posted. Fields stay resident in the arena
+// stream before its completion is posted; with escalate_abs_dF set, an FP32
+// result whose sample shows a sign flip, a class mismatch or a larger error
+// is computed again with every voxel in the FP64 band kernels. Fields stay resident in the arena
 // until completion (or until released when the caller asked to keep them).
 // The reference has no serving path; its analogue is one
 // OPTIMIZE_3D::computeLogScaledPowerRationalMap call per solve
@@ -113,6 +115,7 @@ struct Req {
   double work = 0;       // root-term iterations: voxels x max_iter x roots
   double submitted = 0;  // server clock at submission (aging)
   int check_slot = -1;   // per-result cross-check in flight
+  bool escalated = false;  // recomputed in FP64 after its FP32 cross-check
   unsigned long long chk[CNT_N] = {};
   size_t off = 0, bytes = 0, off64 = 0, off32 = 0, off_field = 0;
   bool admitted = false;
@@ -189,6 +192,7 @@ static void finish(qj_server* s, Req* r, qj_status st) {
   c.done_time = r->done;
   c.voxels = r->voxels;
   c.d_field = (r->keep && st == QJ_OK && r->admitted) ? (const double*)(s->arena.base() + r->off_field) : nullptr;
+  c.escalated = r->escalated ? 1 : 0;
   c.checked = (int64_t)r->chk[CNT_CHECKED];
   c.check_sign_flips = (int64_t)r->chk[CNT_CHECK_FLIPS];
   c.check_class_mismatch = (int64_t)r->chk[CNT_CHECK_CLASS];
@@ -417,6 +421,21 @@ static void worker_main(qj_server* s) {
       s->check_free.push_back(r->check_slot);
       std::unique_ptr<Req> own = std::move(it->second);
       it = s->checking.erase(it);
+      double max_abs;
+      memcpy(&max_abs, &own->chk[CNT_CHECK_MAXABS], sizeof max_abs);
+      if (s->cfg.escalate_abs_dF > 0 && s->cfg.precision == QJ_FP32_BAND && !own->escalated &&
+          (own->chk[CNT_CHECK_FLIPS] > 0 || own->chk[CNT_CHECK_CLASS] > 0 || !(max_abs <= s->cfg.escalate_abs_dF))) {
+        // the FP32 bulk missed the tolerance on the sample: every voxel of the
+        // request goes again, this time to the FP64 band kernels (a band of
+        // infinite width), keeping its arena space and its place in the queue
+        own->escalated = true;
+        own->next = 0;
+        own->chunks_done = 0;
+        own->k32.band = __builtin_inff();
+        const uint64_t id = own->id;
+        s->live[id] = std::move(own);
+        continue;
+      }
       retire(s, std::move(own));
     }
     // 4. launch while a slot is free and there is work
@@ -536,9 +555,11 @@ qj_status qj_server_create(const qj_server_config* cfg, qj_server** out) {
   if (c.launch_work <= 0) c.launch_work = 3e9;
   if (c.band <= 0) c.band = 0.05;
   if (c.aging_s <= 0) c.aging_s = 0.25;
-  if (c.max_pending_work < 0 || c.check_samples < 0 || c.check_samples > (1 << 24)) {
+  if (c.max_pending_work < 0 || c.check_samples < 0 || c.check_samples > (1 << 24) || !(c.escalate_abs_dF >= 0) ||
+      (c.escalate_abs_dF > 0 && c.check_samples == 0)) {
     delete s;
-    set_error("qj_server_create: max_pending_work must be >= 0 and check_samples in [0, 2^24]");
+    set_error("qj_server_create: max_pending_work must be >= 0, check_samples in [0, 2^24], escalate_abs_dF >= 0 "
+              "(and > 0 only with check_samples)");
     return QJ_ERR_INVALID;
   }
   if (c.precision != QJ_FP64 && c.precision != QJ_FP32_BAND) {

@@ -13,10 +13,10 @@ from ._lib import FP32_BAND, MAP_BLACK_HOLE, check
 class Server:
     def __init__(self, hbm_budget_bytes=8 << 30, chunk_voxels=65536, chunks_per_launch=512, max_roots=256,
                  inflight_launches=2, precision=FP32_BAND, band=0.05, device=0, launch_work=0.0,
-                 max_pending_work=0.0, aging_s=0.0, check_samples=0):
+                 max_pending_work=0.0, aging_s=0.0, check_samples=0, escalate_abs_dF=0.0):
         cfg = _lib.ServerConfig(int(hbm_budget_bytes), int(chunk_voxels), int(chunks_per_launch), int(max_roots),
                                 int(inflight_launches), int(precision), float(band), int(device), float(launch_work),
-                                float(max_pending_work), float(aging_s), int(check_samples))
+                                float(max_pending_work), float(aging_s), int(check_samples), float(escalate_abs_dF))
         h = ctypes.c_void_p()
         check(_lib.load().qj_server_create(ctypes.byref(cfg), ctypes.byref(h)))
         self.handle = h

@@ -36,10 +36,10 @@ def _mixed(n=12, seed=5):
     return out
 
 
-def _direct(w, precision):
+def _direct(w, precision, flags=MAP_BLACK_HOLE):
     g = api.make_grid(w.res, w.center, w.lengths)
     r = api.Rational(api.POLYNOMIAL_4D(w.top.roots, w.top.powers), api.POLYNOMIAL_4D(w.bottom.roots, w.bottom.powers))
-    p = api.make_params(w.max_iter, w.escape, w.exp_scaling, w.slice, MAP_BLACK_HOLE)
+    p = api.make_params(w.max_iter, w.escape, w.exp_scaling, w.slice, flags)
     return api.field_eval(g, r, p, precision, 0.05).cpu().numpy()
 
 
@@ -240,3 +240,35 @@ def test_config4_slice_against_the_oracle(gpu):
         assert np.all(np.abs(got[inband] - ref[inband]) <= 1e-8 * np.maximum(1.0, np.abs(ref[inband])))
         assert c.checked == min(256, got.size) and c.check_sign_flips == 0
     srv.close()
+
+
+def test_escalation_recomputes_inaccurate_fp32_results_in_fp64(gpu):
+    """A served FP32 result whose sampled cross-check misses escalate_abs_dF is
+    computed again with every voxel in FP64 (exact integer powers): request 38
+    of the config-4 stream (128^3, 109 + 109 roots, maxIter 5, extended range)
+    has FP32 errors above 0.05 and sign flips; the escalated result equals the
+    FP64-exact field and has none. An accurate request is not escalated."""
+    from quijibo_amd._lib import FP64_EXACT, MAP_EXTENDED_RANGE
+    flags = MAP_BLACK_HOLE | MAP_EXTENDED_RANGE
+    hard = W.config4_requests()[38][1]  # the full 2000-request stream's draw
+    assert hard.res == (128, 128, 128) and hard.max_iter == 5
+    easy = W.config2(48)
+    srv = S.Server(hbm_budget_bytes=256 << 20, chunk_voxels=16384, chunks_per_launch=16, check_samples=512,
+                   escalate_abs_dF=0.025)
+    a = srv.submit(hard, keep_field=True, flags=flags)
+    b = srv.submit(easy, keep_field=True, flags=flags)
+    done = {c.id: c for c in srv.wait_all(2, timeout=120)}
+    assert done[a].status == 0 and done[b].status == 0
+    print("hard: escalated %d, check max |dF| %.3g; easy: escalated %d, check max |dF| %.3g" % (
+        done[a].escalated, done[a].check_max_abs_dF, done[b].escalated, done[b].check_max_abs_dF), flush=True)
+    assert done[a].escalated == 1 and done[b].escalated == 0
+    got = srv.fetch(done[a], device=False)
+    srv.release(a)
+    srv.release(b)
+    srv.close()
+    ref = _direct(hard, FP64_EXACT, flags=flags)
+    assert done[a].check_sign_flips == 0 and done[a].check_max_abs_dF < 1e-9
+    fin = np.isfinite(ref)
+    assert np.array_equal(np.isfinite(got), fin)
+    assert np.max(np.abs(got[fin] - ref[fin]) / np.maximum(1.0, np.abs(ref[fin]))) <= 1e-12
+    assert np.count_nonzero((got.astype(np.float32) > 0) != (ref.astype(np.float32) > 0)) == 0

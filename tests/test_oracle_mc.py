@@ -68,6 +68,23 @@ def test_oracle_table_is_the_reference_table():
         assert ot[c] == ref[c], c
 
 
+def test_oracle_table_matches_the_reference_digest():
+    """The same pin without the reference sources: the table's serialisation
+    against the SHA-256 that tests/golden/make_mc_table_digest.py took of the
+    reference's table file."""
+    import hashlib
+    import sys
+    sys.path.insert(0, os.path.join(HERE, "golden"))
+    from make_mc_table_digest import serialise
+    want = open(os.path.join(HERE, "golden", "mc_table_digest.txt")).read().split()[0]
+    got = hashlib.sha256(serialise(oracle_table()).encode()).hexdigest()
+    assert got == want
+    # one changed triangle changes the digest
+    t = [list(r) for r in oracle_table()]
+    t[158] = t[158] + [(10, 5, 6)]
+    assert hashlib.sha256(serialise(t).encode()).hexdigest() != want
+
+
 def _crossing_edges(flag):
     out = set()
     for e, (o, a) in REF_EDGES.items():
