@@ -227,60 +227,89 @@ __device__ __forceinline__ BlockPrefix mc_block_prefix3(unsigned v) {
   return BlockPrefix{before + within, total};
 }
 
-// Edge masks (bit axis set when the +axis edge from the point carries a vertex)
-// and the vertex count of each block of points.
-__global__ __launch_bounds__(kMcBlock) void k_mc_edges(const uint8_t* __restrict__ flags, GridGeom g,
-                                                       uint8_t* __restrict__ emask, uint32_t* __restrict__ blkv) {
+// ---- linear mesher: one wave per block of 256 points (the unit of the
+// vertex and triangle numbering), four blocks per workgroup. Lane l of a wave
+// takes points l, l + 64, l + 128 and l + 192 of its block: every load
+// instruction is contiguous over the wave, each lane has four independent
+// points in flight, and prefixes in point order are wave ballots with no
+// barrier.
+constexpr int kMcWaves = 4;              // blocks per workgroup
+constexpr int kMcGroup = 64 * kMcWaves;  // threads per workgroup
+static_assert(kMcGroup == kMcBlock, "the table stagings use one thread per case");
+
+struct WavePrefix {
+  uint32_t excl;
+  uint32_t total;
+};
+// exclusive prefix over the wave's lanes and wave total of a count 0..7 given
+// as three bits of weights W0, W1, W2
+template <int W0, int W1, int W2>
+__device__ __forceinline__ WavePrefix wave_prefix3(unsigned v) {
+  const int lane = threadIdx.x & 63;
+  const unsigned long long lt = (1ull << lane) - 1ull;
+  const unsigned long long b0 = __ballot(v & 1u), b1 = __ballot(v & 2u), b2 = __ballot(v & 4u);
+  return WavePrefix{(uint32_t)(W0 * __popcll(b0 & lt) + W1 * __popcll(b1 & lt) + W2 * __popcll(b2 & lt)),
+                    (uint32_t)(W0 * __popcll(b0) + W1 * __popcll(b1) + W2 * __popcll(b2))};
+}
+
+// this wave's block of points (blocks of a workgroup are consecutive), or -1
+// past the last block
+__device__ __forceinline__ int64_t wave_block(int64_t nb) {
+  const int64_t B = xcd_block((nb + kMcWaves - 1) / kMcWaves) * kMcWaves + (threadIdx.x >> 6);
+  return B < nb ? B : -1;
+}
+
+// Edge masks (bit axis set when the +axis edge from the point carries a
+// vertex), each point's first vertex id within its block (vb16), and each
+// block's vertex count.
+__global__ __launch_bounds__(kMcGroup) void k_mc_edges(const uint8_t* __restrict__ flags, GridGeom g,
+                                                       uint8_t* __restrict__ emask, uint16_t* __restrict__ vb16,
+                                                       uint32_t* __restrict__ blkv) {
   __shared__ unsigned short s_em[256];
-  // the cells around the block's points: rows -a*X - b*XY (a, b in {0,1})
-  // from the block's first point minus one, 257 flags each
-  __shared__ uint8_t s_fl[4][kMcBlock + 1];
-  const int64_t X = g.res[0], Y = g.res[1], Z = g.res[2], XY = X * Y, N = XY * Z;
-  const int64_t lb = xcd_block((N + kMcBlock - 1) / kMcBlock);
-  const int64_t p0 = lb * kMcBlock;
-  const int t = threadIdx.x;
-  s_em[t] = kMc.emask[t];
-  for (int r = 0; r < 4; r++) {
-    const int64_t row = p0 - (r & 1) * X - (r >> 1) * XY - 1;
-    for (int j = t; j < kMcBlock + 1; j += kMcBlock) {
-      const int64_t q = row + j;
-      s_fl[r][j] = (q >= 0 && q < N) ? flags[q] : (uint8_t)0;
-    }
-  }
+  s_em[threadIdx.x] = kMc.emask[threadIdx.x];
   __syncthreads();
-  const int64_t p = p0 + t;
-  unsigned bits = 0;
-  if (p < N) {
-    int64_t x, y, z;
-    point_xyz(g, p, x, y, z);
-    // x-edge (p, p+1): cells p - (0,a,b); local id a + 2b
-    if (x < X - 1) {
-      unsigned ref = 0;
-      for (int b = 0; b < 2; b++)
-        for (int a = 0; a < 2; a++)
-          if (y >= a && z >= b) ref |= (s_em[s_fl[a + 2 * b][t + 1]] >> (a + 2 * b)) & 1u;
-      bits |= ref;
+  const int64_t X = g.res[0], Y = g.res[1], Z = g.res[2], XY = X * Y, N = XY * Z;
+  const int64_t B = wave_block((N + kMcBlock - 1) / kMcBlock);
+  if (B < 0) return;
+  const int lane = threadIdx.x & 63;
+  uint32_t base = 0;
+  for (int sub = 0; sub < 4; sub++) {
+    const int64_t p = B * kMcBlock + sub * 64 + lane;
+    unsigned bits = 0;
+    if (p < N) {
+      int64_t x, y, z;
+      point_xyz(g, p, x, y, z);
+      // x-edge (p, p+1): cells p - (0,a,b); local id a + 2b
+      if (x < X - 1) {
+        unsigned ref = 0;
+        for (int b = 0; b < 2; b++)
+          for (int a = 0; a < 2; a++)
+            if (y >= a && z >= b) ref |= (s_em[flags[p - a * X - b * XY]] >> (a + 2 * b)) & 1u;
+        bits |= ref;
+      }
+      // y-edge (p, p+X): cells p - (a,0,b); local id 4 + a + 2b
+      if (y < Y - 1) {
+        unsigned ref = 0;
+        for (int b = 0; b < 2; b++)
+          for (int a = 0; a < 2; a++)
+            if (x >= a && z >= b) ref |= (s_em[flags[p - a - b * XY]] >> (4 + a + 2 * b)) & 1u;
+        bits |= ref << 1;
+      }
+      // z-edge (p, p+XY): cells p - (a,b,0); local id 8 + a + 2b
+      if (z < Z - 1) {
+        unsigned ref = 0;
+        for (int b = 0; b < 2; b++)
+          for (int a = 0; a < 2; a++)
+            if (x >= a && y >= b) ref |= (s_em[flags[p - a - b * X]] >> (8 + a + 2 * b)) & 1u;
+        bits |= ref << 2;
+      }
+      emask[p] = (uint8_t)bits;
     }
-    // y-edge (p, p+X): cells p - (a,0,b); local id 4 + a + 2b
-    if (y < Y - 1) {
-      unsigned ref = 0;
-      for (int b = 0; b < 2; b++)
-        for (int a = 0; a < 2; a++)
-          if (x >= a && z >= b) ref |= (s_em[s_fl[2 * b][t + 1 - a]] >> (4 + a + 2 * b)) & 1u;
-      bits |= ref << 1;
-    }
-    // z-edge (p, p+XY): cells p - (a,b,0); local id 8 + a + 2b
-    if (z < Z - 1) {
-      unsigned ref = 0;
-      for (int b = 0; b < 2; b++)
-        for (int a = 0; a < 2; a++)
-          if (x >= a && y >= b) ref |= (s_em[s_fl[b][t + 1 - a]] >> (8 + a + 2 * b)) & 1u;
-      bits |= ref << 2;
-    }
-    emask[p] = (uint8_t)bits;
+    const WavePrefix vp = wave_prefix3<1, 1, 1>(bits);
+    if (p < N) vb16[p] = (uint16_t)(base + vp.excl);
+    base += vp.total;
   }
-  const BlockPrefix vp = mc_block_prefix3<1, 1, 1>(bits);
-  if (threadIdx.x == 0) blkv[lb] = vp.total;
+  if (lane == 0) blkv[B] = base;
 }
 
 // One vertex of computeEdgeInterpolations (TRIANGLE_MESH.cpp:2373-2425): the
@@ -332,7 +361,10 @@ constexpr double kPrefilterMaxSpacing = 3.0;
 // counts the block's kept triangles. prefilter = 0 measures every triangle.
 // The near test runs on all 12 edges with fixed corner indices (a vertex at
 // place t on a crossing edge has t = |v0| / |v0 - v1| and 1 - t = |v1| /
-// |v0 - v1|, so no division), masked by the case row's edges.
+// |v0 - v1|, so no division), masked by the case row's edges. One cell per
+// thread, a block of 256 cells per block of the numbering (four cells per
+// lane, as in the kernels below, measured slower here: the triangle loops of
+// a lane's cells run one after the other).
 __global__ __launch_bounds__(kMcBlock) void k_mc_cells(const double* __restrict__ field, GridGeom g, double iso,
                                                        int prefilter, uint8_t* __restrict__ flags,
                                                        uint8_t* __restrict__ kmask, uint32_t* __restrict__ blkt) {
@@ -407,84 +439,77 @@ __global__ __launch_bounds__(kMcBlock) void k_mc_cells(const double* __restrict_
 }
 
 // computeEdgeInterpolations (TRIANGLE_MESH.cpp:2373-2425). Vertex ids follow
-// the point order (the block's first id from blk_off, then the prefix within
-// the block, which is also kept in vb16 for the triangle kernel). The tile's
+// the point order: the block's first id from blk_off, then vb16. The tile's
 // vertices go to verts from the running vertex total base[0] (several tiles
 // share the buffer); ids at or beyond cap are not written (the host sees the
 // overflow in the totals and meshes again).
-__global__ __launch_bounds__(kMcBlock) void k_mc_vertices(const double* __restrict__ field, GridGeom g, double iso,
+__global__ __launch_bounds__(kMcGroup) void k_mc_vertices(const double* __restrict__ field, GridGeom g, double iso,
                                                           const uint8_t* __restrict__ emask,
+                                                          const uint16_t* __restrict__ vb16,
                                                           const uint32_t* __restrict__ blk_off,
-                                                          uint16_t* __restrict__ vb16, const uint64_t* __restrict__ base,
-                                                          int64_t cap, double* __restrict__ verts) {
-  // the block's vertices (at most 3 per point) are listed in LDS in id order
-  // and shared out one per thread, so the FP64 work is not spread over
-  // mostly idle lanes
-  __shared__ uint8_t s_pt[3 * kMcBlock];
-  __shared__ uint8_t s_ax[3 * kMcBlock];
+                                                          const uint64_t* __restrict__ base, int64_t cap,
+                                                          double* __restrict__ verts) {
+  // each wave lists its block's vertices (at most 3 per point) in LDS in id
+  // order and computes them one per lane, so the FP64 work is not spread
+  // over mostly idle lanes
+  __shared__ uint8_t s_pt[kMcWaves][3 * kMcBlock];
+  __shared__ uint8_t s_ax[kMcWaves][3 * kMcBlock];
   const int64_t X = g.res[0], XY = X * g.res[1], N = XY * g.res[2];
-  const int64_t p0 = xcd_block((N + kMcBlock - 1) / kMcBlock) * kMcBlock;
-  const int64_t p = p0 + threadIdx.x;
-  const unsigned bits = p < N ? emask[p] : 0u;
-  const BlockPrefix vp = mc_block_prefix3<1, 1, 1>(bits);
-  if (p < N) vb16[p] = (uint16_t)vp.excl;
-  uint32_t k = vp.excl;
-  for (int axis = 0; axis < 3; axis++) {
-    if (bits & (1u << axis)) {
-      s_pt[k] = (uint8_t)threadIdx.x;
-      s_ax[k] = (uint8_t)axis;
-      k++;
+  const int64_t B = wave_block((N + kMcBlock - 1) / kMcBlock);
+  const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  uint32_t total = 0;
+  if (B >= 0) {
+    for (int sub = 0; sub < 4; sub++) {
+      const int64_t p = B * kMcBlock + sub * 64 + lane;
+      unsigned bits = 0;
+      if (p < N) {
+        bits = emask[p];
+        uint32_t k = vb16[p];
+        for (int axis = 0; axis < 3; axis++)
+          if (bits & (1u << axis)) {
+            s_pt[w][k] = (uint8_t)(sub * 64 + lane);
+            s_ax[w][k] = (uint8_t)axis;
+            k++;
+          }
+      }
+      total += wave_prefix3<1, 1, 1>(bits).total;
     }
   }
   __syncthreads();
-  const int64_t first = (int64_t)base[0] + blk_off[p0 / kMcBlock];
-  for (uint32_t j = threadIdx.x; j < vp.total; j += kMcBlock) {
+  if (B < 0) return;
+  const int64_t first = (int64_t)base[0] + blk_off[B];
+  for (uint32_t j = lane; j < total; j += 64) {
     const int64_t id = first + j;
     if (id >= cap) break;
-    const int64_t q = p0 + s_pt[j];
-    const int axis = s_ax[j];
+    const int64_t q = B * kMcBlock + s_pt[w][j];
+    const int axis = s_ax[w][j];
     int64_t x64, y64, z64;
     point_xyz(g, q, x64, y64, z64);
-    double w[3];
+    double v[3];
     mc_vertex(g, (int)x64, (int)y64, (int)z64, axis, field[q], field[q + (axis == 0 ? 1 : (axis == 1 ? X : XY))], iso,
-              w);
-    verts[3 * id + 0] = w[0];
-    verts[3 * id + 1] = w[1];
-    verts[3 * id + 2] = w[2];
+              v);
+    verts[3 * id + 0] = v[0];
+    verts[3 * id + 1] = v[1];
+    verts[3 * id + 2] = v[2];
   }
 }
 
-// The vertex ids of the points that own the edges of the block's cells: the
+// The vertex ids of the points that own the edges of a block's cells: the
 // owner of an edge of cell p is p + dx + dy*X + dz*XY, so they lie in 4 rows
 // of 257 points from the block's first point (this row, +y, +z, +y+z). Per
-// owner: its first vertex id (its block's first id + its first id within the
-// block) and its edge mask, staged in LDS so each triangle corner is an LDS
-// read instead of three gathers.
+// owner: its first vertex id (its block's first id + vb16) and its edge mask,
+// staged in LDS per wave so each triangle corner is an LDS read.
 struct OwnerRows {
   uint32_t vid[4][kMcBlock + 1];
   uint8_t em[4][kMcBlock + 1];
 };
-
-__device__ __forceinline__ void stage_owners(OwnerRows& o, int64_t p0, int64_t X, int64_t XY, int64_t N,
-                                             const uint8_t* __restrict__ emask, const uint32_t* __restrict__ blk_off,
-                                             const uint16_t* __restrict__ vb16) {
-  for (int r = 0; r < 4; r++) {
-    const int64_t row = p0 + (r & 1) * X + (r >> 1) * XY;
-    for (int j = threadIdx.x; j < kMcBlock + 1; j += kMcBlock) {
-      const int64_t q = row + j;
-      const bool in = q < N;
-      o.vid[r][j] = in ? blk_off[q / kMcBlock] + vb16[q] : 0u;
-      o.em[r][j] = in ? emask[q] : (uint8_t)0;
-    }
-  }
-}
 
 // addVertexPairTriangle (TRIANGLE_MESH.cpp:2490-2503) in cell order then
 // table order, degenerate triangles dropped (kmask from k_mc_cells): each
 // cell's kept triangles go to the running triangle total base[1] + its
 // block's first triangle + the kept triangles of earlier cells of the block.
 // Ids at or beyond tcap are not written.
-__global__ __launch_bounds__(kMcBlock) void k_mc_tris(const uint8_t* __restrict__ flags,
+__global__ __launch_bounds__(kMcGroup) void k_mc_tris(const uint8_t* __restrict__ flags,
                                                       const uint8_t* __restrict__ kmask, GridGeom g,
                                                       const uint8_t* __restrict__ emask,
                                                       const uint32_t* __restrict__ blk_off,
@@ -493,30 +518,48 @@ __global__ __launch_bounds__(kMcBlock) void k_mc_tris(const uint8_t* __restrict_
                                                       const uint32_t* __restrict__ blk_toff, int64_t tcap,
                                                       int32_t* __restrict__ tris) {
   __shared__ uint32_t s_tc32[256 * 5 / 2];
-  __shared__ OwnerRows s_own;
+  __shared__ OwnerRows s_own[kMcWaves];
   unsigned short* s_tc = (unsigned short*)s_tc32;
-  const int64_t X = g.res[0], XY = X * g.res[1], N = XY * g.res[2];
-  const int64_t lb = xcd_block((N + kMcBlock - 1) / kMcBlock);
-  const int64_t p = lb * kMcBlock + threadIdx.x;
-  const int flag = p < N ? flags[p] : 0;
-  const unsigned keep = p < N ? kmask[p] : 0u;
-  stage_owners(s_own, lb * kMcBlock, X, XY, N, emask, blk_off, vb16);
   stage_tri5(s_tc, kMc.tcorner);
-  __syncthreads();
-  const BlockPrefix tp = mc_block_prefix3<1, 2, 4>((unsigned)__popc(keep));
-  int64_t o = (int64_t)base[1] + blk_toff[lb] + tp.excl;
-  for (unsigned m = keep; m; m &= m - 1) {
-    const int t = __ffs(m) - 1;
-    if (o < tcap) {
-      const unsigned tc = s_tc[flag * 5 + t];
-      for (int c = 0; c < 3; c++) {
-        const unsigned d = (tc >> (5 * c)) & 31u;
-        const int dx = d & 1, r = (d >> 1) & 3, axis = d >> 3;
-        const int j = (int)threadIdx.x + dx;
-        tris[3 * o + c] = (int32_t)(s_own.vid[r][j] + __popc(s_own.em[r][j] & ((1u << axis) - 1u)));
+  const int64_t X = g.res[0], XY = X * g.res[1], N = XY * g.res[2];
+  const int64_t B = wave_block((N + kMcBlock - 1) / kMcBlock);
+  const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  if (B >= 0) {
+    const int64_t p0 = B * kMcBlock;
+    for (int r = 0; r < 4; r++) {
+      const int64_t row = p0 + (r & 1) * X + (r >> 1) * XY;
+      for (int j = lane; j < kMcBlock + 1; j += 64) {
+        const int64_t q = row + j;
+        const bool in = q < N;
+        s_own[w].vid[r][j] = in ? blk_off[q / kMcBlock] + vb16[q] : 0u;
+        s_own[w].em[r][j] = in ? emask[q] : (uint8_t)0;
       }
     }
-    o++;
+  }
+  __syncthreads();
+  if (B < 0) return;
+  const OwnerRows& own = s_own[w];
+  int64_t o = (int64_t)base[1] + blk_toff[B];
+  for (int sub = 0; sub < 4; sub++) {
+    const int64_t p = B * kMcBlock + sub * 64 + lane;
+    const int flag = p < N ? flags[p] : 0;
+    const unsigned keep = p < N ? kmask[p] : 0u;
+    const WavePrefix tp = wave_prefix3<1, 2, 4>((unsigned)__popc(keep));
+    int64_t t_id = o + tp.excl;
+    for (unsigned m = keep; m; m &= m - 1) {
+      const int t = __ffs(m) - 1;
+      if (t_id < tcap) {
+        const unsigned tc = s_tc[flag * 5 + t];
+        for (int c = 0; c < 3; c++) {
+          const unsigned d = (tc >> (5 * c)) & 31u;
+          const int dx = d & 1, r = (d >> 1) & 3, axis = d >> 3;
+          const int j = sub * 64 + lane + dx;
+          tris[3 * t_id + c] = (int32_t)(own.vid[r][j] + __popc(own.em[r][j] & ((1u << axis) - 1u)));
+        }
+      }
+      t_id++;
+    }
+    o += tp.total;
   }
 }
 
@@ -656,16 +699,17 @@ qj_status mc_run_tiles(qj_workspace* ws, int n, const double* const* fields, con
         QJ_HIP(hipGetLastError());
       } else {
         const int pf = prefilter[(size_t)k] ? 1 : 0;
+        const dim3 nwg((nb.x + kMcWaves - 1) / kMcWaves);
         hipLaunchKernelGGL(k_mc_cells, nb, dim3(kMcBlock), 0, s, fields[k], g, iso, pf, m.flags, m.kmask, m.blkt);
         QJ_HIP(hipGetLastError());
-        hipLaunchKernelGGL(k_mc_edges, nb, dim3(kMcBlock), 0, s, m.flags, g, m.emask, m.blkv);
+        hipLaunchKernelGGL(k_mc_edges, nwg, dim3(kMcGroup), 0, s, m.flags, g, m.emask, m.vb16, m.blkv);
         QJ_HIP(hipGetLastError());
         if ((st = scan_u32(m.blkv, m.blkv, nb.x, m.totals + 1, m.tmp, m.tmp_bytes, s))) return st;
-        hipLaunchKernelGGL(k_mc_vertices, nb, dim3(kMcBlock), 0, s, fields[k], g, iso, m.emask, m.blkv, m.vb16,
+        hipLaunchKernelGGL(k_mc_vertices, nwg, dim3(kMcGroup), 0, s, fields[k], g, iso, m.emask, m.vb16, m.blkv,
                            (const uint64_t*)m.d_base, vcap, m.verts);
         QJ_HIP(hipGetLastError());
         if ((st = scan_u32(m.blkt, m.blkt, nb.x, m.totals + 2, m.tmp, m.tmp_bytes, s))) return st;
-        hipLaunchKernelGGL(k_mc_tris, nb, dim3(kMcBlock), 0, s, m.flags, m.kmask, g, m.emask, m.blkv, m.vb16,
+        hipLaunchKernelGGL(k_mc_tris, nwg, dim3(kMcGroup), 0, s, m.flags, m.kmask, g, m.emask, m.blkv, m.vb16,
                            (const uint64_t*)m.d_base, m.blkt, tcap, m.tris);
         QJ_HIP(hipGetLastError());
       }

@@ -267,7 +267,8 @@ def test_escalation_recomputes_inaccurate_fp32_results_in_fp64(gpu):
     srv.release(b)
     srv.close()
     ref = _direct(hard, FP64_EXACT, flags=flags)
-    assert done[a].check_sign_flips == 0 and done[a].check_max_abs_dF < 1e-9
+    # the second cross-check compares exact products with the polar form: FP64 rounding only
+    assert done[a].check_sign_flips == 0 and done[a].check_max_rel_dF <= 1e-8
     fin = np.isfinite(ref)
     assert np.array_equal(np.isfinite(got), fin)
     assert np.max(np.abs(got[fin] - ref[fin]) / np.maximum(1.0, np.abs(ref[fin]))) <= 1e-12
