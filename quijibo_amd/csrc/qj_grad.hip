@@ -338,13 +338,23 @@ __global__ __launch_bounds__(kGBlock) void k_ap_grads(APArgs g, const RootRec64*
   }
 }
 
-// sums partials[b][k] over b in a fixed order
-__global__ void k_sum_partials(const double* __restrict__ partials, int64_t nb, int K, double* __restrict__ out) {
-  const int k = blockIdx.x * blockDim.x + threadIdx.x;
-  if (k >= K) return;
-  double s = 0.0;
-  for (int64_t b = 0; b < nb; b++) s += partials[b * K + k];
-  out[k] = s;
+// sums partials[b][k] over b: one block per k, each thread a strided run of
+// blocks in order, then a fixed tree over the threads (deterministic; a serial
+// sum over thousands of blocks per k took most of a millisecond)
+constexpr int kSumThreads = 256;
+__global__ __launch_bounds__(kSumThreads) void k_sum_partials(const double* __restrict__ partials, int64_t nb, int K,
+                                                              double* __restrict__ out) {
+  __shared__ double s_v[kSumThreads];
+  const int k = blockIdx.x;
+  double v = 0.0;
+  for (int64_t b = threadIdx.x; b < nb; b += kSumThreads) v += partials[b * K + k];
+  s_v[threadIdx.x] = v;
+  __syncthreads();
+  for (int w = kSumThreads / 2; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) s_v[threadIdx.x] += s_v[threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) out[k] = s_v[0];
 }
 
 qj_status ap_grads(qj_workspace* ws, const MapArgs& a, const qj_rational* f, const qj_grid* wg, const double* d_weight,
@@ -371,7 +381,7 @@ qj_status ap_grads(qj_workspace* ws, const MapArgs& a, const qj_rational* f, con
   hipLaunchKernelGGL(k_ap_grads, dim3((unsigned)nblocks), dim3(kGBlock), lds, s, g, f->d64, d_weight, d_field,
                      ws->d_partials, K, nad);
   QJ_HIP(hipGetLastError());
-  hipLaunchKernelGGL(k_sum_partials, dim3((K + 127) / 128), dim3(128), 0, s, ws->d_partials, nblocks, K, sums);
+  hipLaunchKernelGGL(k_sum_partials, dim3(K), dim3(kSumThreads), 0, s, ws->d_partials, nblocks, K, sums);
   QJ_HIP(hipGetLastError());
   std::vector<double> h(K + 1);
   QJ_HIP(hipMemcpyAsync(h.data(), sums, sizeof(double) * (K + 1), hipMemcpyDeviceToHost, s));
@@ -706,8 +716,9 @@ qj_status points_adjoint(qj_workspace* ws, const MapArgs& a, const qj_rational* 
   }
   constexpr int64_t kPointsPerBlock = kAdjBlock / kAdjLanes;
   const int64_t nblocks = n > 0 ? (n + kPointsPerBlock - 1) / kPointsPerBlock : 1;
-  qj_status st = ensure_partials(ws, nblocks * K, s);
+  qj_status st = ensure_partials(ws, nblocks * K + K, s);
   if (st) return st;
+  double* sums = ws->d_partials + nblocks * K;
   if (n > 0) {
     if (prec == QJ_FP64)
       hipLaunchKernelGGL(k_adjoint<double>, dim3((unsigned)nblocks), dim3(kAdjBlock), lds, s, a, f->d64, d_pts, n,
@@ -716,10 +727,12 @@ qj_status points_adjoint(qj_workspace* ws, const MapArgs& a, const qj_rational* 
       hipLaunchKernelGGL(k_adjoint<float>, dim3((unsigned)nblocks), dim3(kAdjBlock), lds, s, a, f->d64, d_pts, n,
                          d_dJdF, ws->d_partials, K);
     QJ_HIP(hipGetLastError());
+    hipLaunchKernelGGL(k_sum_partials, dim3(K), dim3(kSumThreads), 0, s, ws->d_partials, nblocks, K, sums);
+    QJ_HIP(hipGetLastError());
   } else {
-    QJ_HIP(hipMemsetAsync(ws->d_partials, 0, sizeof(double) * K, s));
+    QJ_HIP(hipMemsetAsync(sums, 0, sizeof(double) * K, s));
   }
-  hipLaunchKernelGGL(k_adjoint_finish, dim3((K + 127) / 128), dim3(128), 0, s, ws->d_partials, nblocks, a.nt, nr,
+  hipLaunchKernelGGL(k_adjoint_finish, dim3((K + 127) / 128), dim3(128), 0, s, sums, (int64_t)1, a.nt, nr,
                      a.sc_top, a.sc_bottom, d_gr, d_gp);
   QJ_HIP(hipGetLastError());
   return QJ_OK;

@@ -14,8 +14,10 @@
 //                   the rest are measured with the FP64 vertices; per block
 //                   of cells: kept-triangle count
 //   k_mc_edges      per grid point: which of its +x,+y,+z edges some adjacent
-//                   cell's case row references (gather of <= 7 neighbour flags);
-//                   per block of points: vertex count
+//                   cell's case row references (the edge-use masks of the four
+//                   cell rows around a block staged in LDS), packed with the
+//                   point's first vertex id within its block; per block of
+//                   points: vertex count
 //   scan            block vertex counts -> first vertex id per block (the
 //                   reference numbers vertices by std::map<pair<int,int>> order
 //                   = lower endpoint, then x<y<z)
@@ -53,6 +55,8 @@ struct McTables {
   signed char tri[256][16];
   unsigned char ntri[256];
   unsigned short emask[256];
+  // edge mask | triangle count << 12
+  unsigned short emn[256];
   // per case row and triangle: the bit mask of its 3 edges, and its 3 corners
   // in table order, 5 bits each: dx | (dy + 2 dz) << 1 | axis << 3 (the row
   // of owner points and the axis the triangle kernel looks up)
@@ -61,7 +65,7 @@ struct McTables {
 };
 
 constexpr McTables make_tables() {
-  McTables t{QJ_MC_TRI_INIT, {}, {}, {}, {}};
+  McTables t{QJ_MC_TRI_INIT, {}, {}, {}, {}, {}};
   for (int c = 0; c < 256; c++) {
     int n = 0;
     unsigned m = 0;
@@ -71,6 +75,7 @@ constexpr McTables make_tables() {
     }
     t.ntri[c] = (unsigned char)(n / 3);
     t.emask[c] = (unsigned short)m;
+    t.emn[c] = (unsigned short)(m | (unsigned)(n / 3) << 12);
     for (int q = 0; q < n / 3; q++) {
       unsigned tm = 0, tc = 0;
       for (int j = 0; j < 3; j++) {
@@ -133,14 +138,14 @@ struct McState {
   uint8_t* flags = nullptr;
   uint8_t* emask = nullptr;
   uint32_t* cnt = nullptr;     // per point edge count -> vertex base (scanned in place; non-linear mesher)
-  uint16_t* vb16 = nullptr;    // per point first vertex id within its block (linear mesher)
+  uint16_t* pk16 = nullptr;    // linear mesher, per point: first vertex id within its block << 3 | edge mask
   uint8_t* kmask = nullptr;    // per cell: bit t set when triangle t of its case row is kept
   int64_t cap_kmask = 0;
   uint16_t* tloc16 = nullptr;  // per active cell first triangle within its block of cells
   uint32_t* blkt = nullptr;    // per block of cells triangle count -> first triangle of the block
   int64_t cap_tloc16 = 0, cap_blkt = 0;
   uint32_t* blkv = nullptr;    // per block vertex count -> first vertex id of the block
-  int64_t cap_vb16 = 0, cap_blkv = 0;
+  int64_t cap_pk16 = 0, cap_blkv = 0;
   uint32_t* blk = nullptr;     // per block active count -> offset
   uint32_t* active = nullptr;  // active cells
   uint32_t* tri_off = nullptr; // per active cell triangle count -> offset
@@ -157,6 +162,9 @@ struct McState {
   int64_t cap_slab = 0;
   uint32_t* slice_off = nullptr;
   int64_t cap_slice = 0;
+  uint32_t* vpt = nullptr;  // vertices of a range of points: owning point, edge axis
+  uint8_t* vax = nullptr;
+  int64_t cap_vpt = 0, cap_vax = 0;
   unsigned int* d_bad = nullptr;
   // edge cache of the last non-linear mesh (TRIANGLE_MESH.cpp:431-586): per
   // vertex its edge (lower endpoint x, y, z, axis), per active cell (x, y, z, flag)
@@ -260,53 +268,79 @@ __device__ __forceinline__ int64_t wave_block(int64_t nb) {
 }
 
 // Edge masks (bit axis set when the +axis edge from the point carries a
-// vertex), each point's first vertex id within its block (vb16), and each
-// block's vertex count.
+// vertex) and each point's first vertex id within its block, packed as
+// pk16[p] = id << 3 | mask (ids within a block stay below 3 x 256), and each
+// block's vertex count. The edges of point p are referenced by the cells
+// p - a X - b XY (x edge), p - a - b XY (y edge) and p - a - b X (z edge),
+// a, b in {0, 1}: entries -1..255 of the four cell rows that start at
+// p0 - a X - b XY. Each wave stages those rows' edge-use masks (the case
+// table's emask of each cell's flag) in LDS, so a point's 12 tests are LDS
+// reads and each flag is looked up once.
 __global__ __launch_bounds__(kMcGroup) void k_mc_edges(const uint8_t* __restrict__ flags, GridGeom g,
-                                                       uint8_t* __restrict__ emask, uint16_t* __restrict__ vb16,
-                                                       uint32_t* __restrict__ blkv) {
+                                                       uint16_t* __restrict__ pk16, uint32_t* __restrict__ blkv) {
   __shared__ unsigned short s_em[256];
+  __shared__ unsigned short s_ce[kMcWaves][4][kMcBlock + 1];
   s_em[threadIdx.x] = kMc.emask[threadIdx.x];
   __syncthreads();
   const int64_t X = g.res[0], Y = g.res[1], Z = g.res[2], XY = X * Y, N = XY * Z;
   const int64_t B = wave_block((N + kMcBlock - 1) / kMcBlock);
+  const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  if (B >= 0) {
+    // all 20 loads issued before the first table lookup (case 0 uses no edge)
+    const int64_t p0 = B * kMcBlock;
+    unsigned f[4][5];
+#pragma unroll
+    for (int r = 0; r < 4; r++) {
+      const int64_t c0 = p0 - (r & 1) * X - (r >> 1) * XY - 1;
+#pragma unroll
+      for (int k = 0; k < 5; k++) {
+        const int64_t c = c0 + lane + 64 * k;
+        f[r][k] = ((k < 4 || lane == 0) && c >= 0 && c < N) ? (unsigned)flags[c] : 0u;
+      }
+    }
+#pragma unroll
+    for (int r = 0; r < 4; r++)
+#pragma unroll
+      for (int k = 0; k < 5; k++)
+        if (k < 4 || lane == 0) s_ce[w][r][lane + 64 * k] = s_em[f[r][k]];
+  }
+  __syncthreads();
   if (B < 0) return;
-  const int lane = threadIdx.x & 63;
   uint32_t base = 0;
   for (int sub = 0; sub < 4; sub++) {
-    const int64_t p = B * kMcBlock + sub * 64 + lane;
+    const int j = sub * 64 + lane;
+    const int64_t p = B * kMcBlock + j;
     unsigned bits = 0;
     if (p < N) {
       int64_t x, y, z;
       point_xyz(g, p, x, y, z);
-      // x-edge (p, p+1): cells p - (0,a,b); local id a + 2b
+      // x-edge (p, p+1): cells p - (0,a,b), row a + 2b, local edge a + 2b
       if (x < X - 1) {
         unsigned ref = 0;
         for (int b = 0; b < 2; b++)
           for (int a = 0; a < 2; a++)
-            if (y >= a && z >= b) ref |= (s_em[flags[p - a * X - b * XY]] >> (a + 2 * b)) & 1u;
+            if (y >= a && z >= b) ref |= (s_ce[w][a + 2 * b][j + 1] >> (a + 2 * b)) & 1u;
         bits |= ref;
       }
-      // y-edge (p, p+X): cells p - (a,0,b); local id 4 + a + 2b
+      // y-edge (p, p+X): cells p - (a,0,b), row 2b, local edge 4 + a + 2b
       if (y < Y - 1) {
         unsigned ref = 0;
         for (int b = 0; b < 2; b++)
           for (int a = 0; a < 2; a++)
-            if (x >= a && z >= b) ref |= (s_em[flags[p - a - b * XY]] >> (4 + a + 2 * b)) & 1u;
+            if (x >= a && z >= b) ref |= (s_ce[w][2 * b][j + 1 - a] >> (4 + a + 2 * b)) & 1u;
         bits |= ref << 1;
       }
-      // z-edge (p, p+XY): cells p - (a,b,0); local id 8 + a + 2b
+      // z-edge (p, p+XY): cells p - (a,b,0), row b, local edge 8 + a + 2b
       if (z < Z - 1) {
         unsigned ref = 0;
         for (int b = 0; b < 2; b++)
           for (int a = 0; a < 2; a++)
-            if (x >= a && y >= b) ref |= (s_em[flags[p - a - b * X]] >> (8 + a + 2 * b)) & 1u;
+            if (x >= a && y >= b) ref |= (s_ce[w][b][j + 1 - a] >> (8 + a + 2 * b)) & 1u;
         bits |= ref << 2;
       }
-      emask[p] = (uint8_t)bits;
     }
     const WavePrefix vp = wave_prefix3<1, 1, 1>(bits);
-    if (p < N) vb16[p] = (uint16_t)(base + vp.excl);
+    if (p < N) pk16[p] = (uint16_t)((base + vp.excl) << 3 | bits);
     base += vp.total;
   }
   if (lane == 0) blkv[B] = base;
@@ -361,20 +395,17 @@ constexpr double kPrefilterMaxSpacing = 3.0;
 // counts the block's kept triangles. prefilter = 0 measures every triangle.
 // The near test runs on all 12 edges with fixed corner indices (a vertex at
 // place t on a crossing edge has t = |v0| / |v0 - v1| and 1 - t = |v1| /
-// |v0 - v1|, so no division), masked by the case row's edges. One cell per
+// |v0 - v1|, so no division), masked by the case row's edges; a cell with
+// fewer than two near vertices keeps all its triangles without looking at
+// them, and only the others read the per-triangle edge masks. One cell per
 // thread, a block of 256 cells per block of the numbering (four cells per
 // lane, as in the kernels below, measured slower here: the triangle loops of
 // a lane's cells run one after the other).
 __global__ __launch_bounds__(kMcBlock) void k_mc_cells(const double* __restrict__ field, GridGeom g, double iso,
                                                        int prefilter, uint8_t* __restrict__ flags,
                                                        uint8_t* __restrict__ kmask, uint32_t* __restrict__ blkt) {
-  __shared__ uint32_t s_tm32[256 * 5 / 2];
-  __shared__ unsigned short s_em[256];
-  __shared__ uint8_t s_ntri[256];
-  unsigned short* s_tm = (unsigned short*)s_tm32;
-  stage_tri5(s_tm, kMc.tmask);
-  s_em[threadIdx.x] = kMc.emask[threadIdx.x];
-  s_ntri[threadIdx.x] = kMc.ntri[threadIdx.x];
+  __shared__ unsigned short s_emn[256];
+  s_emn[threadIdx.x] = kMc.emn[threadIdx.x];
   __syncthreads();
   const int64_t X = g.res[0], Y = g.res[1], Z = g.res[2], XY = X * Y, N = XY * Z;
   const int64_t lb = xcd_block((N + kMcBlock - 1) / kMcBlock);
@@ -393,23 +424,28 @@ __global__ __launch_bounds__(kMcBlock) void k_mc_cells(const double* __restrict_
         v[k] = (float)(field[p + dx + dy * X + dz * XY] - iso);
         flag |= (v[k] > 0.0f) << k;
       }
-      const int n = s_ntri[flag];
+      const unsigned emn = s_emn[flag];
+      const int n = (int)(emn >> 12);
       if (n) {
-        const float h[3] = {fabsf((float)g.dx[0]), fabsf((float)g.dx[1]), fabsf((float)g.dx[2])};
-        unsigned near = 0;
+        keep = (1u << n) - 1u;
+        unsigned near = ~0u;  // no prefilter: every triangle is measured
+        if (prefilter) {
+          const float h[3] = {fabsf((float)g.dx[0]), fabsf((float)g.dx[1]), fabsf((float)g.dx[2])};
+          near = 0;
 #pragma unroll
-        for (int e = 0; e < 12; e++) {
-          int dx = 0, dy = 0, dz = 0, axis = 0;
-          edge_parts(e, dx, dy, dz, axis);
-          const int k0 = 4 * dx + 2 * dy + dz, k1 = k0 + (axis == 0 ? 4 : (axis == 1 ? 2 : 1));
-          const float a0 = fabsf(v[k0]), a1 = fabsf(v[k1]), lim = kNearLimit * fabsf(v[k0] - v[k1]);
-          near |= (a0 * h[axis] >= lim && a1 * h[axis] >= lim) ? 0u : (1u << e);
+          for (int e = 0; e < 12; e++) {
+            int dx = 0, dy = 0, dz = 0, axis = 0;
+            edge_parts(e, dx, dy, dz, axis);
+            const int k0 = 4 * dx + 2 * dy + dz, k1 = k0 + (axis == 0 ? 4 : (axis == 1 ? 2 : 1));
+            const float a0 = fabsf(v[k0]), a1 = fabsf(v[k1]), lim = kNearLimit * fabsf(v[k0] - v[k1]);
+            near |= (a0 * h[axis] >= lim && a1 * h[axis] >= lim) ? 0u : (1u << e);
+          }
+          near &= emn & 0xfffu;
         }
-        near &= s_em[flag];
-        for (int t = 0; t < n; t++) {
-          const unsigned tm = s_tm[flag * 5 + t];
-          bool degenerate = false;
-          if (!prefilter || __popc(near & tm) >= 2) {
+        if (__popc(near) >= 2) {
+          for (int t = 0; t < n; t++) {
+            const unsigned tm = kMc.tmask[flag][t];
+            if (__popc(near & tm) < 2) continue;
             // measured with the FP64 vertices (the corner values read again:
             // this path is rare, and a register array indexed per lane would
             // go to scratch)
@@ -425,9 +461,9 @@ __global__ __launch_bounds__(kMcBlock) void k_mc_cells(const double* __restrict_
               mc_vertex(g, (int)x + dx, (int)y + dy, (int)z + dz, axis, field[q0], field[q1], iso, w[j]);
             }
             const double eps = 1e-7;
-            degenerate = edge_len(w[0], w[1]) < eps || edge_len(w[0], w[2]) < eps || edge_len(w[1], w[2]) < eps;
+            if (edge_len(w[0], w[1]) < eps || edge_len(w[0], w[2]) < eps || edge_len(w[1], w[2]) < eps)
+              keep &= ~(1u << t);
           }
-          if (!degenerate) keep |= 1u << t;
         }
       }
     }
@@ -439,13 +475,12 @@ __global__ __launch_bounds__(kMcBlock) void k_mc_cells(const double* __restrict_
 }
 
 // computeEdgeInterpolations (TRIANGLE_MESH.cpp:2373-2425). Vertex ids follow
-// the point order: the block's first id from blk_off, then vb16. The tile's
+// the point order: the block's first id from blk_off, then pk16 >> 3. The tile's
 // vertices go to verts from the running vertex total base[0] (several tiles
 // share the buffer); ids at or beyond cap are not written (the host sees the
 // overflow in the totals and meshes again).
 __global__ __launch_bounds__(kMcGroup) void k_mc_vertices(const double* __restrict__ field, GridGeom g, double iso,
-                                                          const uint8_t* __restrict__ emask,
-                                                          const uint16_t* __restrict__ vb16,
+                                                          const uint16_t* __restrict__ pk16,
                                                           const uint32_t* __restrict__ blk_off,
                                                           const uint64_t* __restrict__ base, int64_t cap,
                                                           double* __restrict__ verts) {
@@ -459,12 +494,20 @@ __global__ __launch_bounds__(kMcGroup) void k_mc_vertices(const double* __restri
   const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
   uint32_t total = 0;
   if (B >= 0) {
+    unsigned pks[4];
+#pragma unroll
+    for (int sub = 0; sub < 4; sub++) {
+      const int64_t p = B * kMcBlock + sub * 64 + lane;
+      pks[sub] = p < N ? (unsigned)pk16[p] : 0u;
+    }
+#pragma unroll
     for (int sub = 0; sub < 4; sub++) {
       const int64_t p = B * kMcBlock + sub * 64 + lane;
       unsigned bits = 0;
       if (p < N) {
-        bits = emask[p];
-        uint32_t k = vb16[p];
+        const unsigned pk = pks[sub];
+        bits = pk & 7u;
+        uint32_t k = pk >> 3;
         for (int axis = 0; axis < 3; axis++)
           if (bits & (1u << axis)) {
             s_pt[w][k] = (uint8_t)(sub * 64 + lane);
@@ -497,69 +540,112 @@ __global__ __launch_bounds__(kMcGroup) void k_mc_vertices(const double* __restri
 // The vertex ids of the points that own the edges of a block's cells: the
 // owner of an edge of cell p is p + dx + dy*X + dz*XY, so they lie in 4 rows
 // of 257 points from the block's first point (this row, +y, +z, +y+z). Per
-// owner: its first vertex id (its block's first id + vb16) and its edge mask,
-// staged in LDS per wave so each triangle corner is an LDS read.
+// owner its pk16 word (first vertex id within its block << 3 | edge mask);
+// per row the first ids of the (at most two) blocks it spans and the entry
+// where the second begins.
 struct OwnerRows {
-  uint32_t vid[4][kMcBlock + 1];
-  uint8_t em[4][kMcBlock + 1];
+  uint16_t pk[4][kMcBlock + 1];
+  uint32_t bo[4][2];
+  int32_t jb[4];
+};
+
+// The block's kept triangles in id order, (cell << 3) | triangle of the case
+// row, and the cells' cases.
+struct TriList {
+  uint16_t e[5 * kMcBlock];
+  uint8_t flag[kMcBlock];
 };
 
 // addVertexPairTriangle (TRIANGLE_MESH.cpp:2490-2503) in cell order then
-// table order, degenerate triangles dropped (kmask from k_mc_cells): each
-// cell's kept triangles go to the running triangle total base[1] + its
-// block's first triangle + the kept triangles of earlier cells of the block.
-// Ids at or beyond tcap are not written.
+// table order, degenerate triangles dropped (kmask from k_mc_cells): the
+// block's kept triangles go to the running triangle total base[1] + the
+// block's first triangle. Each wave lists its block's kept triangles in LDS
+// and then writes them one per lane, so consecutive lanes write consecutive
+// triangles. Ids at or beyond tcap are not written.
 __global__ __launch_bounds__(kMcGroup) void k_mc_tris(const uint8_t* __restrict__ flags,
                                                       const uint8_t* __restrict__ kmask, GridGeom g,
-                                                      const uint8_t* __restrict__ emask,
+                                                      const uint16_t* __restrict__ pk16,
                                                       const uint32_t* __restrict__ blk_off,
-                                                      const uint16_t* __restrict__ vb16,
                                                       const uint64_t* __restrict__ base,
                                                       const uint32_t* __restrict__ blk_toff, int64_t tcap,
                                                       int32_t* __restrict__ tris) {
   __shared__ uint32_t s_tc32[256 * 5 / 2];
   __shared__ OwnerRows s_own[kMcWaves];
+  __shared__ TriList s_tl[kMcWaves];
   unsigned short* s_tc = (unsigned short*)s_tc32;
   stage_tri5(s_tc, kMc.tcorner);
   const int64_t X = g.res[0], XY = X * g.res[1], N = XY * g.res[2];
-  const int64_t B = wave_block((N + kMcBlock - 1) / kMcBlock);
+  const int64_t nb = (N + kMcBlock - 1) / kMcBlock;
+  const int64_t B = wave_block(nb);
   const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  OwnerRows& own = s_own[w];
+  TriList& tl = s_tl[w];
+  uint32_t total = 0;
   if (B >= 0) {
+    // every load of the staging issued before the first LDS write
     const int64_t p0 = B * kMcBlock;
+    unsigned v[4][5], fl[4], km[4], bo[4][2] = {{0u, 0u}, {0u, 0u}, {0u, 0u}, {0u, 0u}};
+#pragma unroll
     for (int r = 0; r < 4; r++) {
       const int64_t row = p0 + (r & 1) * X + (r >> 1) * XY;
-      for (int j = lane; j < kMcBlock + 1; j += 64) {
-        const int64_t q = row + j;
-        const bool in = q < N;
-        s_own[w].vid[r][j] = in ? blk_off[q / kMcBlock] + vb16[q] : 0u;
-        s_own[w].em[r][j] = in ? emask[q] : (uint8_t)0;
+#pragma unroll
+      for (int k = 0; k < 5; k++) {
+        const int64_t q = row + lane + 64 * k;
+        v[r][k] = ((k < 4 || lane == 0) && q < N) ? (unsigned)pk16[q] : 0u;
       }
+      const int64_t b0 = row / kMcBlock;
+      if (lane == 0) {
+        bo[r][0] = b0 < nb ? blk_off[b0] : 0u;
+        bo[r][1] = b0 + 1 < nb ? blk_off[b0 + 1] : 0u;
+      }
+    }
+#pragma unroll
+    for (int sub = 0; sub < 4; sub++) {
+      const int64_t p = p0 + sub * 64 + lane;
+      fl[sub] = p < N ? (unsigned)flags[p] : 0u;
+      km[sub] = p < N ? (unsigned)kmask[p] : 0u;
+    }
+#pragma unroll
+    for (int r = 0; r < 4; r++) {
+      const int64_t row = p0 + (r & 1) * X + (r >> 1) * XY;
+#pragma unroll
+      for (int k = 0; k < 5; k++)
+        if (k < 4 || lane == 0) own.pk[r][lane + 64 * k] = (uint16_t)v[r][k];
+      if (lane == 0) {
+        own.bo[r][0] = bo[r][0];
+        own.bo[r][1] = bo[r][1];
+        own.jb[r] = (int32_t)(kMcBlock - (row - (row / kMcBlock) * kMcBlock));
+      }
+    }
+#pragma unroll
+    for (int sub = 0; sub < 4; sub++) {
+      const int c = sub * 64 + lane;
+      const unsigned flag = fl[sub];
+      const unsigned keep = km[sub];
+      tl.flag[c] = (uint8_t)flag;
+      const WavePrefix tp = wave_prefix3<1, 2, 4>((unsigned)__popc(keep));
+      uint32_t k = total + tp.excl;
+      for (unsigned m = keep; m; m &= m - 1) tl.e[k++] = (uint16_t)(c << 3 | (__ffs(m) - 1));
+      total += tp.total;
     }
   }
   __syncthreads();
   if (B < 0) return;
-  const OwnerRows& own = s_own[w];
-  int64_t o = (int64_t)base[1] + blk_toff[B];
-  for (int sub = 0; sub < 4; sub++) {
-    const int64_t p = B * kMcBlock + sub * 64 + lane;
-    const int flag = p < N ? flags[p] : 0;
-    const unsigned keep = p < N ? kmask[p] : 0u;
-    const WavePrefix tp = wave_prefix3<1, 2, 4>((unsigned)__popc(keep));
-    int64_t t_id = o + tp.excl;
-    for (unsigned m = keep; m; m &= m - 1) {
-      const int t = __ffs(m) - 1;
-      if (t_id < tcap) {
-        const unsigned tc = s_tc[flag * 5 + t];
-        for (int c = 0; c < 3; c++) {
-          const unsigned d = (tc >> (5 * c)) & 31u;
-          const int dx = d & 1, r = (d >> 1) & 3, axis = d >> 3;
-          const int j = sub * 64 + lane + dx;
-          tris[3 * t_id + c] = (int32_t)(own.vid[r][j] + __popc(own.em[r][j] & ((1u << axis) - 1u)));
-        }
-      }
-      t_id++;
+  const int64_t o = (int64_t)base[1] + blk_toff[B];
+  for (uint32_t j = lane; j < total; j += 64) {
+    const int64_t t_id = o + j;
+    if (t_id >= tcap) break;
+    const unsigned e = tl.e[j];
+    const int c = (int)(e >> 3);
+    const unsigned tc = s_tc[tl.flag[c] * 5 + (e & 7u)];
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+      const unsigned d = (tc >> (5 * k)) & 31u;
+      const int dx = d & 1, r = (d >> 1) & 3, axis = d >> 3;
+      const int jj = c + dx;
+      const unsigned pk = own.pk[r][jj];
+      tris[3 * t_id + k] = (int32_t)(own.bo[r][jj >= own.jb[r]] + (pk >> 3) + __popc(pk & ((1u << axis) - 1u)));
     }
-    o += tp.total;
   }
 }
 
@@ -662,8 +748,7 @@ qj_status mc_run_tiles(qj_workspace* ws, int n, const double* const* fields, con
   qj_status st;
   const int64_t nbmax = nblocks(nmax);
   if ((st = grow(m.flags, m.cap_flags, nmax, s)) || (st = grow(m.kmask, m.cap_kmask, nmax, s)) ||
-      (st = grow(m.emask, m.cap_emask, nmax, s)) ||
-      (st = grow(m.vb16, m.cap_vb16, nmax, s)) || (st = grow(m.blkv, m.cap_blkv, nbmax, s)) ||
+      (st = grow(m.pk16, m.cap_pk16, nmax, s)) || (st = grow(m.blkv, m.cap_blkv, nbmax, s)) ||
       (st = grow(m.blkt, m.cap_blkt, nbmax, s)) || (st = grow(m.d_counts, m.cap_counts, 2 * (int64_t)n, s)))
     return st;
   const size_t need_tmp = scan_tmp_bytes(nbmax > 4096 ? nbmax : 4096);
@@ -702,14 +787,14 @@ qj_status mc_run_tiles(qj_workspace* ws, int n, const double* const* fields, con
         const dim3 nwg((nb.x + kMcWaves - 1) / kMcWaves);
         hipLaunchKernelGGL(k_mc_cells, nb, dim3(kMcBlock), 0, s, fields[k], g, iso, pf, m.flags, m.kmask, m.blkt);
         QJ_HIP(hipGetLastError());
-        hipLaunchKernelGGL(k_mc_edges, nwg, dim3(kMcGroup), 0, s, m.flags, g, m.emask, m.vb16, m.blkv);
+        hipLaunchKernelGGL(k_mc_edges, nwg, dim3(kMcGroup), 0, s, m.flags, g, m.pk16, m.blkv);
         QJ_HIP(hipGetLastError());
         if ((st = scan_u32(m.blkv, m.blkv, nb.x, m.totals + 1, m.tmp, m.tmp_bytes, s))) return st;
-        hipLaunchKernelGGL(k_mc_vertices, nwg, dim3(kMcGroup), 0, s, fields[k], g, iso, m.emask, m.vb16, m.blkv,
+        hipLaunchKernelGGL(k_mc_vertices, nwg, dim3(kMcGroup), 0, s, fields[k], g, iso, m.pk16, m.blkv,
                            (const uint64_t*)m.d_base, vcap, m.verts);
         QJ_HIP(hipGetLastError());
         if ((st = scan_u32(m.blkt, m.blkt, nb.x, m.totals + 2, m.tmp, m.tmp_bytes, s))) return st;
-        hipLaunchKernelGGL(k_mc_tris, nwg, dim3(kMcGroup), 0, s, m.flags, m.kmask, g, m.emask, m.blkv, m.vb16,
+        hipLaunchKernelGGL(k_mc_tris, nwg, dim3(kMcGroup), 0, s, m.flags, m.kmask, g, m.pk16, m.blkv,
                            (const uint64_t*)m.d_base, m.blkt, tcap, m.tris);
         QJ_HIP(hipGetLastError());
       }
@@ -767,7 +852,7 @@ void mc_free(qj_workspace* ws) {
   hipFree(m->flags);
   hipFree(m->emask);
   hipFree(m->cnt);
-  hipFree(m->vb16);
+  hipFree(m->pk16);
   hipFree(m->kmask);
   hipFree(m->tloc16);
   hipFree(m->blkt);
@@ -783,6 +868,8 @@ void mc_free(qj_workspace* ws) {
   hipFree(m->totals);
   hipFree(m->slab);
   hipFree(m->slice_off);
+  hipFree(m->vpt);
+  hipFree(m->vax);
   hipFree(m->d_bad);
   hipFree(m->c_edges);
   hipFree(m->c_cells);
@@ -939,64 +1026,57 @@ __device__ __forceinline__ void place_vertex(const MapArgs& a, const RootRec64* 
   out[2] = m[2];
 }
 
-// vertices of the owned points [p0, p0 + np) by bisection, and their edges.
-// About one point in seven owns an edge, so the block first lists its points
-// with edges and their vertices in LDS, then evaluates one listed point per
-// thread (the lower end value, shared by the point's edges) and bisects one
-// vertex per thread: the FP64 map evaluations run on full waves.
+// The vertices of a range of points (off: the exclusive scan of the edge
+// counts, from the range's first point) in id order: the owning point
+// (relative to the range) and the edge's axis.
+__global__ __launch_bounds__(kMcBlock) void k_nl_vlist(const uint8_t* __restrict__ emask,
+                                                       const uint32_t* __restrict__ off, int64_t n,
+                                                       uint32_t* __restrict__ vpt, uint8_t* __restrict__ vax) {
+  const int64_t j = (int64_t)blockIdx.x * kMcBlock + threadIdx.x;
+  if (j >= n) return;
+  const unsigned bits = emask[j];
+  if (!bits) return;
+  uint32_t k = off[j] - off[0];
+  for (int axis = 0; axis < 3; axis++)
+    if (bits & (1u << axis)) {
+      vpt[k] = (uint32_t)j;
+      vax[k] = (uint8_t)axis;
+      k++;
+    }
+}
+
+// Vertices by bisection, one per thread over the list: both end values, then
+// up to 6 midpoints (FP64 map evaluations on full waves; about one point in
+// ten owns an edge, so a thread per point would leave most lanes idle). The
+// range's vertex count is the scan total minus the scan at its first point;
+// vertex t gets id base + off[0] + t.
 template <int kPow, bool kLag>
 __global__ __launch_bounds__(kMcBlock) void k_nl_vertices(MapArgs a, const RootRec64* __restrict__ roots,
-                                                          double iso, int64_t p0, int64_t np,
-                                                          const uint8_t* __restrict__ emask,
-                                                          const uint32_t* __restrict__ off, uint32_t base,
+                                                          double iso, int64_t p0,
+                                                          const uint32_t* __restrict__ vpt,
+                                                          const uint8_t* __restrict__ vax,
+                                                          const uint32_t* __restrict__ off,
+                                                          const uint32_t* __restrict__ total, uint32_t base,
                                                           double* __restrict__ verts, int32_t* __restrict__ edges,
                                                           unsigned int* __restrict__ bad) {
-  __shared__ uint8_t s_lp[kMcBlock];        // listed point -> point within the block
-  __shared__ double s_va[kMcBlock];         // map value at the listed point
-  __shared__ uint8_t s_slp[3 * kMcBlock];   // vertex slot -> listed point
-  __shared__ uint8_t s_sax[3 * kMcBlock];   // vertex slot -> axis
-  const int64_t X = a.res[0], XY = X * a.res[1], N = XY * np;
-  const int64_t j0 = (int64_t)blockIdx.x * kMcBlock;
-  const int64_t j = j0 + threadIdx.x;
-  const unsigned bits = j < N ? emask[j] : 0u;
-  const BlockPrefix lp = mc_block_prefix3<1, 0, 0>(bits != 0u ? 1u : 0u);
-  const BlockPrefix vp = mc_block_prefix3<1, 1, 1>(bits);
-  if (bits) {
-    s_lp[lp.excl] = (uint8_t)threadIdx.x;
-    uint32_t k = vp.excl;
-    for (int axis = 0; axis < 3; axis++) {
-      if (bits & (1u << axis)) {
-        s_slp[k] = (uint8_t)lp.excl;
-        s_sax[k] = (uint8_t)axis;
-        k++;
-      }
-    }
-  }
-  __syncthreads();
-  for (uint32_t i = threadIdx.x; i < lp.total; i += kMcBlock) {
-    const int64_t q = j0 + s_lp[i];
-    const int64_t lz = q / XY, r = q - lz * XY;
-    const int y = (int)(r / X), x = (int)(r - (int64_t)y * X), z = (int)(p0 + lz);
-    double pa[3];
-    cell_center(a, x, y, z, pa);
-    s_va[i] = nonlinear_value<kPow, kLag>(a, roots, pa, iso);
-  }
-  __syncthreads();
-  const uint32_t first = base + off[j0];  // vertex id of the block's first listed edge
-  for (uint32_t sl = threadIdx.x; sl < vp.total; sl += kMcBlock) {
-    const int l = s_slp[sl], axis = s_sax[sl];
-    const int64_t q = j0 + s_lp[l];
-    const int64_t lz = q / XY, r = q - lz * XY;
-    const int y = (int)(r / X), x = (int)(r - (int64_t)y * X), z = (int)(p0 + lz);
-    double pa[3];
-    cell_center(a, x, y, z, pa);
-    const int64_t k = (int64_t)first + sl;
-    place_vertex<kPow, kLag>(a, roots, iso, x, y, z, axis, pa, s_va[l], verts + 3 * k, bad);
-    edges[4 * k + 0] = x;
-    edges[4 * k + 1] = y;
-    edges[4 * k + 2] = z;
-    edges[4 * k + 3] = axis;
-  }
+  const uint32_t first = off[0];
+  const uint32_t n = *total - first;
+  const uint32_t t = blockIdx.x * kMcBlock + threadIdx.x;
+  if (t >= n) return;
+  const int64_t X = a.res[0], XY = X * a.res[1];
+  const int64_t q = vpt[t];
+  const int axis = vax[t];
+  const int64_t lz = q / XY, r = q - lz * XY;
+  const int y = (int)(r / X), x = (int)(r - (int64_t)y * X), z = (int)(p0 + lz);
+  double pa[3];
+  cell_center(a, x, y, z, pa);
+  const double va = nonlinear_value<kPow, kLag>(a, roots, pa, iso);
+  const int64_t k = (int64_t)base + first + t;
+  place_vertex<kPow, kLag>(a, roots, iso, x, y, z, axis, pa, va, verts + 3 * k, bad);
+  edges[4 * k + 0] = x;
+  edges[4 * k + 1] = y;
+  edges[4 * k + 2] = z;
+  edges[4 * k + 3] = axis;
 }
 
 // Launch a mesher vertex kernel instantiated for the map's power mode and
@@ -1208,14 +1288,24 @@ qj_status nlmc_run(qj_workspace* ws, const MapArgs& a0, const qj_rational* f, do
                         4 * (int64_t)(nv + placed_next), s)))
       return st;
     const int64_t skip = z0 > 0 ? 1 : 0;
+    if ((st = grow(m.vpt, m.cap_vpt, (int64_t)(cnt_own + cnt_next), s))) return st;
+    if ((st = grow(m.vax, m.cap_vax, (int64_t)(cnt_own + cnt_next), s))) return st;
     if (cnt_own > 0 && z1 - z0 > skip) {
-      launch_nl_vertices(a, dim3(nblocks(XY * (z1 - z0 - skip))), s, f->d64, iso, z0 + skip, z1 - z0 - skip,
-                         m.emask + skip * XY, m.cnt + skip * XY, (uint32_t)nv, m.verts, m.c_edges, m.d_bad);
+      const int64_t n = XY * (z1 - z0 - skip);
+      hipLaunchKernelGGL(k_nl_vlist, dim3(nblocks(n)), dim3(kMcBlock), 0, s, m.emask + skip * XY,
+                         m.cnt + skip * XY, n, m.vpt, m.vax);
+      QJ_HIP(hipGetLastError());
+      launch_nl_vertices(a, dim3(nblocks((int64_t)cnt_own)), s, f->d64, iso, z0 + skip, m.vpt, m.vax,
+                         m.cnt + skip * XY, m.totals + 1, (uint32_t)nv, m.verts, m.c_edges, m.d_bad);
       QJ_HIP(hipGetLastError());
     }
     if (cnt_next > 0) {
-      launch_nl_vertices(a, dim3(nblocks(XY)), s, f->d64, iso, z1, (int64_t)1, m.emask + nown, m.slice_off,
-                         (uint32_t)(nv + cnt_own), m.verts, m.c_edges, m.d_bad);
+      hipLaunchKernelGGL(k_nl_vlist, dim3(nblocks(XY)), dim3(kMcBlock), 0, s, m.emask + nown, m.slice_off, XY,
+                         m.vpt + cnt_own, m.vax + cnt_own);
+      QJ_HIP(hipGetLastError());
+      launch_nl_vertices(a, dim3(nblocks((int64_t)cnt_next)), s, f->d64, iso, z1, m.vpt + cnt_own,
+                         m.vax + cnt_own, m.slice_off, m.totals + 3, (uint32_t)(nv + cnt_own), m.verts, m.c_edges,
+                         m.d_bad);
       QJ_HIP(hipGetLastError());
     }
     // 5. active cells of the chunk, in cell order
