@@ -4,6 +4,7 @@
 #include <stdio.h>
 #include <string.h>
 
+#include <algorithm>
 #include <new>
 
 #include "qj_internal.h"
@@ -119,17 +120,25 @@ qj_status make_map_args(const qj_grid* g, const qj_rational* f, const qj_map_par
   return QJ_OK;
 }
 
-qj_status ensure_list(qj_workspace* ws, int64_t n, hipStream_t s) {
-  if (n <= ws->list_cap) return QJ_OK;
+// The lists of an evaluation of n cells: a band list of a quarter of the
+// cells (at least 2^16, at most n; cells that need FP64 beyond it are marked
+// in the output and swept) and, for QJ_MAP_ADAPTIVE, an adaptive list of n.
+// At 512^3 that is 128 MiB of band list against the 1 GiB field.
+qj_status ensure_list(qj_workspace* ws, int64_t n, bool adaptive, hipStream_t s) {
+  const int64_t band = std::min(n, std::max(n / 4, (int64_t)1 << 16));
+  const int64_t adapt = adaptive ? n : 0;
+  if (band <= ws->list_cap && adapt <= ws->alist_cap && ws->d_list) return QJ_OK;
+  const int64_t nb = std::max(band, ws->list_cap), na = std::max(adapt, ws->alist_cap);
   if (ws->d_list) {
     QJ_HIP(hipStreamSynchronize(s));
     QJ_HIP(hipFree(ws->d_list));
-    ws->d_list = nullptr;
-    ws->list_cap = 0;
+    ws->d_list = ws->d_alist = nullptr;
+    ws->list_cap = ws->alist_cap = 0;
   }
-  // two segments: band list, adaptive (overflow) list
-  QJ_HIP(hipMalloc(&ws->d_list, sizeof(uint32_t) * 2 * (size_t)n));
-  ws->list_cap = n;
+  QJ_HIP(hipMalloc(&ws->d_list, sizeof(uint32_t) * (size_t)(nb + na + 1)));
+  ws->d_alist = ws->d_list + nb;
+  ws->list_cap = nb;
+  ws->alist_cap = na;
   return QJ_OK;
 }
 

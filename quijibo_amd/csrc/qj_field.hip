@@ -113,7 +113,8 @@ __device__ __forceinline__ Res64 eval64x(const MapArgs& a, const RootRec64* __re
 template <bool kLagrange, int kPow = kPolar, bool kExt = false>
 __global__ __launch_bounds__(kBlock) void k_grid64(MapArgs a, const RootRec64* __restrict__ roots,
                                                    double* __restrict__ out, unsigned long long* __restrict__ cnt,
-                                                   uint32_t* __restrict__ alist, uint32_t* __restrict__ acount) {
+                                                   uint32_t* __restrict__ alist, uint32_t* __restrict__ acount,
+                                                   uint32_t acap) {
   int x, y, z;
   const bool valid = tile_voxel(a, x, y, z);
   int iters = 0;
@@ -135,7 +136,7 @@ __global__ __launch_bounds__(kBlock) void k_grid64(MapArgs a, const RootRec64* _
   block_count(iters, cnt, CNT_ITERS);
   block_count(nonfinite ? 1 : 0, cnt, CNT_NONFINITE);
   if (a.flags & QJ_MAP_ADAPTIVE) {
-    block_append(ovf, (uint32_t)idx, alist, acount);
+    block_append(ovf, (uint32_t)idx, alist, acount, acap);
     block_count(ovf ? 1 : 0, cnt, CNT_ADAPTIVE);
   }
 }
@@ -143,17 +144,19 @@ __global__ __launch_bounds__(kBlock) void k_grid64(MapArgs a, const RootRec64* _
 template <int kRN, bool kGeneral>
 __global__ __launch_bounds__(kBlock) void k_grid32(MapArgs a, Consts32 k, const F4* __restrict__ roots,
                                                    double* __restrict__ out, unsigned long long* __restrict__ cnt,
-                                                   uint32_t* __restrict__ blist, uint32_t* __restrict__ bcount) {
+                                                   uint32_t* __restrict__ blist, uint32_t* __restrict__ bcount,
+                                                   uint32_t bcap) {
   const Roots32 R = roots32_view(roots, a.nt + a.nb);
   grid32_body<kRN, kGeneral>(a, k, root_acc(R.q, R.pat, R.e, 0), root_acc(R.q, R.pat, R.e, a.nt), out, cnt,
-                                    blist, bcount);
+                                    blist, bcount, bcap);
 }
 
 template <class Src, int kRN, bool kGeneral>
 __global__ __launch_bounds__(kBlock) void k_points32(MapArgs a, Consts32 k, const F4* __restrict__ roots,
                                                      const double* __restrict__ pts, int64_t n,
                                                      double* __restrict__ out, unsigned long long* __restrict__ cnt,
-                                                     uint32_t* __restrict__ blist, uint32_t* __restrict__ bcount) {
+                                                     uint32_t* __restrict__ blist, uint32_t* __restrict__ bcount,
+                                                     uint32_t bcap) {
   const Roots32 R = roots32_view(roots, a.nt + a.nb);
   const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
   const bool valid = i < n;
@@ -172,21 +175,21 @@ __global__ __launch_bounds__(kBlock) void k_points32(MapArgs a, Consts32 k, cons
   }
   block_count(valid ? 1 : 0, cnt, CNT_VOXELS);
   block_count(iters, cnt, CNT_ITERS);
-  block_append(need, (uint32_t)i, blist, bcount);
+  if (!block_append(need, (uint32_t)i, blist, bcount, bcap) && need) ((unsigned long long*)out)[i] = kBandMark;
 }
 
-// FP64 over a compacted list (count read on the device), or over all n
-// indices when list == nullptr. kPow: the PowMode (kExactInt / kExactOnly for
-// the FP32 path's band and range re-evaluation).
+// FP64 over a compacted list (count read on the device, at most cap entries
+// held), or over all n indices when list == nullptr. kPow: the PowMode
+// (kExactInt / kExactOnly for the FP32 path's band and range re-evaluation).
 template <class Src, bool kExtended, int kPow = kPolar>
 __global__ __launch_bounds__(kBlock) void k_list64(MapArgs a, const RootRec64* __restrict__ roots,
                                                    const double* __restrict__ pts,
                                                    const uint32_t* __restrict__ list,
-                                                   const uint32_t* __restrict__ count, int64_t n_all,
+                                                   const uint32_t* __restrict__ count, int64_t cap, int64_t n_all,
                                                    double* __restrict__ out, unsigned long long* __restrict__ cnt,
                                                    int slot_vox, int slot_iter, uint32_t* __restrict__ alist,
-                                                   uint32_t* __restrict__ acount) {
-  const int64_t n = list ? (int64_t)*count : n_all;
+                                                   uint32_t* __restrict__ acount, uint32_t acap) {
+  const int64_t n = list ? ((int64_t)*count < cap ? (int64_t)*count : cap) : n_all;
   const int64_t stride = (int64_t)gridDim.x * kBlock;
   unsigned long long vox = 0, iters = 0;
   for (int64_t base = (int64_t)blockIdx.x * kBlock; base < n; base += stride) {
@@ -203,10 +206,42 @@ __global__ __launch_bounds__(kBlock) void k_list64(MapArgs a, const RootRec64* _
       iters += r.iters;
       ovf = r.overflow;
     }
-    if (!kExtended && (a.flags & QJ_MAP_ADAPTIVE)) block_append(ovf, idx, alist, acount);
+    if (!kExtended && (a.flags & QJ_MAP_ADAPTIVE)) block_append(ovf, idx, alist, acount, acap);
   }
   block_count(vox, cnt, slot_vox);
   block_count(iters, cnt, slot_iter);
+}
+
+// The cells (or points) that the FP32 pass found in need of FP64 after the
+// band list had filled up (they hold kBandMark), evaluated as k_list64
+// evaluates the listed ones. Returns at once when the list held them all.
+template <class Src, int kPow>
+__global__ __launch_bounds__(kBlock) void k_band_sweep(MapArgs a, const RootRec64* __restrict__ roots,
+                                                       const double* __restrict__ pts,
+                                                       const uint32_t* __restrict__ count, int64_t cap,
+                                                       int64_t n_all, double* __restrict__ out,
+                                                       unsigned long long* __restrict__ cnt,
+                                                       uint32_t* __restrict__ alist, uint32_t* __restrict__ acount,
+                                                       uint32_t acap) {
+  if ((int64_t)*count <= cap) return;  // the same for every thread of the grid
+  const int64_t stride = (int64_t)gridDim.x * kBlock;
+  unsigned long long vox = 0, iters = 0;
+  for (int64_t base = (int64_t)blockIdx.x * kBlock; base < n_all; base += stride) {
+    const int64_t i = base + threadIdx.x;
+    bool ovf = false;
+    if (i < n_all && ((const unsigned long long*)out)[i] == kBandMark) {
+      const q64 q0 = Src::start(a, pts, (uint32_t)i);
+      const Res64 r = eval64_any<kPow>(a, roots, q0);
+      if (r.written) out[i] = r.F;
+      else if (pts) out[i] = __builtin_nan("");
+      vox++;
+      iters += r.iters;
+      ovf = r.overflow;
+    }
+    if (a.flags & QJ_MAP_ADAPTIVE) block_append(ovf, (uint32_t)i, alist, acount, acap);
+  }
+  block_count(vox, cnt, CNT_BAND_VOXELS);
+  block_count(iters, cnt, CNT_BAND_ITERS);
 }
 
 // ---------------------------------------------------------------- sampled FP64 cross-check
@@ -386,7 +421,7 @@ __global__ __launch_bounds__(kBlock) void k_batch32(const ChunkDesc* __restrict_
         blist[list_cap - 1 - (int64_t)k] = e;
       }
     } else {
-      block_append(need, e, blist, counters + BC_BAND_INT);
+      block_append(need, e, blist, counters + BC_BAND_INT, (uint32_t)list_cap);
     }
   }
 }
@@ -544,20 +579,39 @@ static int list_grid(int64_t n_hint) {
   return (int)(b < 1 ? 1 : (b > 8192 ? 8192 : b));
 }
 
-// Device buffers of one field evaluation: the band list and the adaptive
-// list (list_cap entries each), their two counts, and the statistics
-// counters. ws (may be null) only carries the optional kernel timing.
+// Device buffers of one field evaluation: the band list (list_cap entries;
+// what does not fit is marked in the output and swept), the adaptive list
+// (alist_cap entries, as many as cells when QJ_MAP_ADAPTIVE is set), their two
+// counts, and the statistics counters. ws (may be null) only carries the
+// optional kernel timing.
 struct FieldBufs {
   uint32_t* list;
   uint32_t* alist;
   int64_t list_cap;
+  int64_t alist_cap;
   uint32_t* list_count;
   unsigned long long* counters;
   qj_workspace* ws;
 };
 
 static FieldBufs ws_bufs(qj_workspace* ws) {
-  return FieldBufs{ws->d_list, ws->d_list + ws->list_cap, ws->list_cap, ws->d_list_count, ws->d_counters, ws};
+  return FieldBufs{ws->d_list, ws->d_alist, ws->list_cap, ws->alist_cap, ws->d_list_count, ws->d_counters, ws};
+}
+
+// The band sweep after the band list's FP64 pass (exits at once unless the
+// FP32 pass overflowed the list).
+template <class Src>
+static qj_status launch_sweep(const FieldBufs& fb, const MapArgs& a, const qj_rational* f, const double* pts,
+                              int64_t n, double* d_out, hipStream_t s) {
+  const dim3 g(list_grid(n / 16)), b(kBlock);
+  if (a.gen)
+    hipLaunchKernelGGL((k_band_sweep<Src, kExactInt>), g, b, 0, s, a, f->d64, pts, fb.list_count, fb.list_cap, n,
+                       d_out, fb.counters, fb.alist, fb.list_count + 1, (uint32_t)fb.alist_cap);
+  else
+    hipLaunchKernelGGL((k_band_sweep<Src, kExactOnly>), g, b, 0, s, a, f->d64, pts, fb.list_count, fb.list_cap, n,
+                       d_out, fb.counters, fb.alist, fb.list_count + 1, (uint32_t)fb.alist_cap);
+  QJ_HIP(hipGetLastError());
+  return QJ_OK;
 }
 
 static void mark(const FieldBufs& b, hipStream_t s) {
@@ -569,19 +623,19 @@ static qj_status run_adaptive(const FieldBufs& fb, const MapArgs& a, const qj_ra
   if (!(a.flags & QJ_MAP_ADAPTIVE)) return QJ_OK;
   if (pts)
     hipLaunchKernelGGL((k_list64<PointSource, true>), dim3(list_grid(n / 64 + 1)), dim3(kBlock), 0, s, a, f->d64,
-                       pts, fb.alist, fb.list_count + 1, (int64_t)0, d_out, fb.counters, (int)CNT_BAND_VOXELS,
-                       (int)CNT_BAND_ITERS, (uint32_t*)nullptr, (uint32_t*)nullptr);
+                       pts, fb.alist, fb.list_count + 1, fb.alist_cap, (int64_t)0, d_out, fb.counters,
+                       (int)CNT_BAND_VOXELS, (int)CNT_BAND_ITERS, (uint32_t*)nullptr, (uint32_t*)nullptr, 0u);
   else
     hipLaunchKernelGGL((k_list64<GridSource, true>), dim3(list_grid(n / 64 + 1)), dim3(kBlock), 0, s, a, f->d64,
-                       pts, fb.alist, fb.list_count + 1, (int64_t)0, d_out, fb.counters, (int)CNT_BAND_VOXELS,
-                       (int)CNT_BAND_ITERS, (uint32_t*)nullptr, (uint32_t*)nullptr);
+                       pts, fb.alist, fb.list_count + 1, fb.alist_cap, (int64_t)0, d_out, fb.counters,
+                       (int)CNT_BAND_VOXELS, (int)CNT_BAND_ITERS, (uint32_t*)nullptr, (uint32_t*)nullptr, 0u);
   QJ_HIP(hipGetLastError());
   return QJ_OK;
 }
 
 // One field (or one z-slab of it: a.res[2] slices from global slice a.zoff,
-// written from d_out on) with lists of at least a.res[0] a.res[1] a.res[2]
-// entries.
+// written from d_out on); with QJ_MAP_ADAPTIVE the adaptive list holds at
+// least a.res[0] a.res[1] a.res[2] entries.
 static qj_status launch_field_bufs(const FieldBufs& fb, const MapArgs& a, const qj_rational* f, qj_precision prec,
                                    double band, double* d_out, hipStream_t s) {
   const int64_t nvox = (int64_t)a.res[0] * a.res[1] * a.res[2];
@@ -598,32 +652,32 @@ static qj_status launch_field_bufs(const FieldBufs& fb, const MapArgs& a, const 
   if (prec == QJ_FP64_EXACT) {
     if (a.gen && ext)
       hipLaunchKernelGGL((k_grid64<false, kExactInt, true>), g64, b64, 0, s, a, f->d64, d_out, fb.counters, fb.alist,
-                         fb.list_count + 1);
+                         fb.list_count + 1, (uint32_t)fb.alist_cap);
     else if (a.gen)
       hipLaunchKernelGGL((k_grid64<false, kExactInt>), g64, b64, 0, s, a, f->d64, d_out, fb.counters, fb.alist,
-                         fb.list_count + 1);
+                         fb.list_count + 1, (uint32_t)fb.alist_cap);
     else if (ext)
       hipLaunchKernelGGL((k_grid64<false, kExactOnly, true>), g64, b64, 0, s, a, f->d64, d_out, fb.counters,
-                         fb.alist, fb.list_count + 1);
+                         fb.alist, fb.list_count + 1, (uint32_t)fb.alist_cap);
     else
       hipLaunchKernelGGL((k_grid64<false, kExactOnly>), g64, b64, 0, s, a, f->d64, d_out, fb.counters, fb.alist,
-                         fb.list_count + 1);
+                         fb.list_count + 1, (uint32_t)fb.alist_cap);
     QJ_HIP(hipGetLastError());
     mark(fb, s);
     mark(fb, s);
   } else if (prec == QJ_FP64 && ext) {
     hipLaunchKernelGGL((k_grid64<false, kPolar, true>), g64, b64, 0, s, a, f->d64, d_out, fb.counters, fb.alist,
-                       fb.list_count + 1);
+                       fb.list_count + 1, (uint32_t)fb.alist_cap);
     QJ_HIP(hipGetLastError());
     mark(fb, s);
     mark(fb, s);
   } else if (prec == QJ_FP64) {
     if (a.flags & QJ_MAP_LAGRANGE)
       hipLaunchKernelGGL(k_grid64<true>, dim3((unsigned)nblocks), dim3(kBlock), 0, s, a, f->d64, d_out,
-                         fb.counters, fb.alist, fb.list_count + 1);
+                         fb.counters, fb.alist, fb.list_count + 1, (uint32_t)fb.alist_cap);
     else
       hipLaunchKernelGGL(k_grid64<false>, dim3((unsigned)nblocks), dim3(kBlock), 0, s, a, f->d64, d_out,
-                         fb.counters, fb.alist, fb.list_count + 1);
+                         fb.counters, fb.alist, fb.list_count + 1, (uint32_t)fb.alist_cap);
     QJ_HIP(hipGetLastError());
     mark(fb, s);
     mark(fb, s);
@@ -632,28 +686,32 @@ static qj_status launch_field_bufs(const FieldBufs& fb, const MapArgs& a, const 
     const dim3 g((unsigned)nblocks), b(kBlock);
     if (a.gen)
       hipLaunchKernelGGL((k_grid32<2, true>), g, b, 0, s, a, k, (const F4*)f->d32, d_out, fb.counters, fb.list,
-                         fb.list_count);
+                         fb.list_count, (uint32_t)fb.list_cap);
     else if (rn_of(a) == 8)
       hipLaunchKernelGGL((k_grid32<8, false>), g, b, 0, s, a, k, (const F4*)f->d32, d_out, fb.counters, fb.list,
-                         fb.list_count);
+                         fb.list_count, (uint32_t)fb.list_cap);
     else if (rn_of(a) == 4)
       hipLaunchKernelGGL((k_grid32<4, false>), g, b, 0, s, a, k, (const F4*)f->d32, d_out, fb.counters, fb.list,
-                         fb.list_count);
+                         fb.list_count, (uint32_t)fb.list_cap);
     else
       hipLaunchKernelGGL((k_grid32<2, false>), g, b, 0, s, a, k, (const F4*)f->d32, d_out, fb.counters, fb.list,
-                         fb.list_count);
+                         fb.list_count, (uint32_t)fb.list_cap);
     QJ_HIP(hipGetLastError());
     mark(fb, s);
     mark(fb, s);
     if (a.gen)
       hipLaunchKernelGGL((k_list64<GridSource, false, kExactInt>), dim3(list_grid(nvox / 16)), dim3(kBlock), 0, s, a,
-                         f->d64, (const double*)nullptr, fb.list, fb.list_count, (int64_t)0, d_out, fb.counters,
-                         (int)CNT_BAND_VOXELS, (int)CNT_BAND_ITERS, fb.alist, fb.list_count + 1);
+                         f->d64, (const double*)nullptr, fb.list, fb.list_count, fb.list_cap, (int64_t)0, d_out,
+                         fb.counters, (int)CNT_BAND_VOXELS, (int)CNT_BAND_ITERS, fb.alist, fb.list_count + 1,
+                         (uint32_t)fb.alist_cap);
     else
       hipLaunchKernelGGL((k_list64<GridSource, false, kExactOnly>), dim3(list_grid(nvox / 16)), dim3(kBlock), 0, s,
-                         a, f->d64, (const double*)nullptr, fb.list, fb.list_count, (int64_t)0, d_out, fb.counters,
-                         (int)CNT_BAND_VOXELS, (int)CNT_BAND_ITERS, fb.alist, fb.list_count + 1);
+                         a, f->d64, (const double*)nullptr, fb.list, fb.list_count, fb.list_cap, (int64_t)0, d_out,
+                         fb.counters, (int)CNT_BAND_VOXELS, (int)CNT_BAND_ITERS, fb.alist, fb.list_count + 1,
+                         (uint32_t)fb.alist_cap);
     QJ_HIP(hipGetLastError());
+    qj_status st = launch_sweep<GridSource>(fb, a, f, nullptr, nvox, d_out, s);
+    if (st) return st;
   }
   mark(fb, s);
   return run_adaptive(fb, a, f, nullptr, nvox, d_out, s);
@@ -667,7 +725,7 @@ qj_status launch_field(qj_workspace* ws, const MapArgs& a, const qj_rational* f,
     set_error("grid of %lld cells exceeds 2^32; evaluate it in tiles", (long long)nvox);
     return QJ_ERR_CAPACITY;
   }
-  qj_status st = ensure_list(ws, nvox, s);
+  qj_status st = ensure_list(ws, nvox, (a.flags & QJ_MAP_ADAPTIVE) != 0, s);
   if (st) return st;
   if ((st = launch_field_bufs(ws_bufs(ws), a, f, prec, band, d_out, s))) return st;
   if (ws->check_samples > 0) {
@@ -683,7 +741,7 @@ qj_status launch_points(qj_workspace* ws, const MapArgs& a, const qj_rational* f
                         const double* d_pts, int64_t n, double* d_out, hipStream_t s) {
   if (n <= 0) return QJ_OK;
   if (n > (int64_t)UINT32_MAX) return QJ_ERR_CAPACITY;
-  qj_status st = ensure_list(ws, n, s);
+  qj_status st = ensure_list(ws, n, (a.flags & QJ_MAP_ADAPTIVE) != 0, s);
   if (st) return st;
   const FieldBufs fb = ws_bufs(ws);
   // list counters cleared by a kernel, not a memset node, so the call can be captured in a graph
@@ -691,18 +749,20 @@ qj_status launch_points(qj_workspace* ws, const MapArgs& a, const qj_rational* f
   QJ_HIP(hipGetLastError());
   if (prec == QJ_FP64) {
     hipLaunchKernelGGL((k_list64<PointSource, false>), dim3(list_grid(n)), dim3(kBlock), 0, s, a, f->d64, d_pts,
-                       (const uint32_t*)nullptr, (const uint32_t*)nullptr, n, d_out, fb.counters,
-                       (int)CNT_VOXELS, (int)CNT_ITERS, fb.alist, fb.list_count + 1);
+                       (const uint32_t*)nullptr, (const uint32_t*)nullptr, (int64_t)0, n, d_out, fb.counters,
+                       (int)CNT_VOXELS, (int)CNT_ITERS, fb.alist, fb.list_count + 1, (uint32_t)fb.alist_cap);
     QJ_HIP(hipGetLastError());
   } else if (prec == QJ_FP64_EXACT) {
     if (a.gen)
       hipLaunchKernelGGL((k_list64<PointSource, false, kExactInt>), dim3(list_grid(n)), dim3(kBlock), 0, s, a,
-                         f->d64, d_pts, (const uint32_t*)nullptr, (const uint32_t*)nullptr, n, d_out, fb.counters,
-                         (int)CNT_VOXELS, (int)CNT_ITERS, fb.alist, fb.list_count + 1);
+                         f->d64, d_pts, (const uint32_t*)nullptr, (const uint32_t*)nullptr, (int64_t)0, n, d_out,
+                         fb.counters, (int)CNT_VOXELS, (int)CNT_ITERS, fb.alist, fb.list_count + 1,
+                         (uint32_t)fb.alist_cap);
     else
       hipLaunchKernelGGL((k_list64<PointSource, false, kExactOnly>), dim3(list_grid(n)), dim3(kBlock), 0, s, a,
-                         f->d64, d_pts, (const uint32_t*)nullptr, (const uint32_t*)nullptr, n, d_out, fb.counters,
-                         (int)CNT_VOXELS, (int)CNT_ITERS, fb.alist, fb.list_count + 1);
+                         f->d64, d_pts, (const uint32_t*)nullptr, (const uint32_t*)nullptr, (int64_t)0, n, d_out,
+                         fb.counters, (int)CNT_VOXELS, (int)CNT_ITERS, fb.alist, fb.list_count + 1,
+                         (uint32_t)fb.alist_cap);
     QJ_HIP(hipGetLastError());
   } else {
     const int64_t nb = (n + kBlock - 1) / kBlock;
@@ -710,26 +770,29 @@ qj_status launch_points(qj_workspace* ws, const MapArgs& a, const qj_rational* f
     const dim3 g((unsigned)nb), b(kBlock);
     if (a.gen)
       hipLaunchKernelGGL((k_points32<PointSource, 2, true>), g, b, 0, s, a, k, (const F4*)f->d32, d_pts, n, d_out,
-                         fb.counters, fb.list, fb.list_count);
+                         fb.counters, fb.list, fb.list_count, (uint32_t)fb.list_cap);
     else if (rn_of(a) == 8)
       hipLaunchKernelGGL((k_points32<PointSource, 8, false>), g, b, 0, s, a, k, (const F4*)f->d32, d_pts, n, d_out,
-                         fb.counters, fb.list, fb.list_count);
+                         fb.counters, fb.list, fb.list_count, (uint32_t)fb.list_cap);
     else if (rn_of(a) == 4)
       hipLaunchKernelGGL((k_points32<PointSource, 4, false>), g, b, 0, s, a, k, (const F4*)f->d32, d_pts, n, d_out,
-                         fb.counters, fb.list, fb.list_count);
+                         fb.counters, fb.list, fb.list_count, (uint32_t)fb.list_cap);
     else
       hipLaunchKernelGGL((k_points32<PointSource, 2, false>), g, b, 0, s, a, k, (const F4*)f->d32, d_pts, n, d_out,
-                         fb.counters, fb.list, fb.list_count);
+                         fb.counters, fb.list, fb.list_count, (uint32_t)fb.list_cap);
     QJ_HIP(hipGetLastError());
     if (a.gen)
       hipLaunchKernelGGL((k_list64<PointSource, false, kExactInt>), dim3(list_grid(n / 16)), dim3(kBlock), 0, s, a,
-                         f->d64, d_pts, fb.list, fb.list_count, (int64_t)0, d_out, fb.counters,
-                         (int)CNT_BAND_VOXELS, (int)CNT_BAND_ITERS, fb.alist, fb.list_count + 1);
+                         f->d64, d_pts, fb.list, fb.list_count, fb.list_cap, (int64_t)0, d_out, fb.counters,
+                         (int)CNT_BAND_VOXELS, (int)CNT_BAND_ITERS, fb.alist, fb.list_count + 1,
+                         (uint32_t)fb.alist_cap);
     else
       hipLaunchKernelGGL((k_list64<PointSource, false, kExactOnly>), dim3(list_grid(n / 16)), dim3(kBlock), 0, s, a,
-                         f->d64, d_pts, fb.list, fb.list_count, (int64_t)0, d_out, fb.counters,
-                         (int)CNT_BAND_VOXELS, (int)CNT_BAND_ITERS, fb.alist, fb.list_count + 1);
+                         f->d64, d_pts, fb.list, fb.list_count, fb.list_cap, (int64_t)0, d_out, fb.counters,
+                         (int)CNT_BAND_VOXELS, (int)CNT_BAND_ITERS, fb.alist, fb.list_count + 1,
+                         (uint32_t)fb.alist_cap);
     QJ_HIP(hipGetLastError());
+    if ((st = launch_sweep<PointSource>(fb, a, f, d_pts, n, d_out, s))) return st;
   }
   return run_adaptive(fb, a, f, d_pts, n, d_out, s);
 }
@@ -803,7 +866,7 @@ extern "C" qj_status qj_field_eval_noalloc(const qj_grid* grid, const qj_rationa
   QJ_HIP(hipGetDevice(&dev));
   unsigned char* base = nullptr;
   if ((st = scratch_setup(dev, &base))) return st;
-  const FieldBufs fb{(uint32_t*)base, (uint32_t*)(base + kStaticListBytes), kStaticSlabVoxels,
+  const FieldBufs fb{(uint32_t*)base, (uint32_t*)(base + kStaticListBytes), kStaticSlabVoxels, kStaticSlabVoxels,
                      (uint32_t*)(base + kStaticCountOff), (unsigned long long*)(base + kStaticCounterOff), nullptr};
   const hipStream_t s = (hipStream_t)stream;
   const int64_t Z = a.res[2], nz = kStaticSlabVoxels / XY;

@@ -34,8 +34,11 @@ __device__ __forceinline__ void block_count(unsigned long long v, unsigned long 
   __syncthreads();
 }
 
-// Order-free compaction: flagged lanes append `value` to list; one atomic per block.
-__device__ __forceinline__ void block_append(bool flag, uint32_t value, uint32_t* list, uint32_t* count) {
+// Order-free compaction: flagged lanes append `value` to list; one atomic per
+// block. Entries at `cap` and beyond are not written (the count still counts
+// them); returns whether this lane's entry was written.
+__device__ __forceinline__ bool block_append(bool flag, uint32_t value, uint32_t* list, uint32_t* count,
+                                             uint32_t cap) {
   __shared__ uint32_t s_cnt[kBlock / 64];
   __shared__ uint32_t s_base;
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
@@ -53,8 +56,16 @@ __device__ __forceinline__ void block_append(bool flag, uint32_t value, uint32_t
     s_base = t ? atomicAdd(count, t) : 0u;
   }
   __syncthreads();
-  if (flag) list[s_base + s_cnt[wave] + rank] = value;
+  const uint32_t slot = s_base + s_cnt[wave] + rank;
+  const bool stored = flag && slot < cap;
+  if (stored) list[slot] = value;
+  return stored;
 }
+
+// What the FP32 pass writes into a cell that needs FP64 but found the band
+// list full (a signalling-NaN pattern no arithmetic produces); the band sweep
+// evaluates those cells.
+constexpr unsigned long long kBandMark = 0x7ff40000b0a4d5eeull;
 
 // ---------------------------------------------------------------- FP32 bulk
 
@@ -399,7 +410,8 @@ __device__ __forceinline__ bool tile_voxel(const MapArgs& a, int& x, int& y, int
 template <int kRN, bool kGeneral, class AccT, class AccB>
 __device__ __forceinline__ void grid32_body(const MapArgs& a, const Consts32& k, const AccT& top, const AccB& bot,
                                             double* __restrict__ out, unsigned long long* __restrict__ cnt,
-                                            uint32_t* __restrict__ blist, uint32_t* __restrict__ bcount) {
+                                            uint32_t* __restrict__ blist, uint32_t* __restrict__ bcount,
+                                            uint32_t bcap) {
   int x, y, z;
   const bool valid = tile_voxel(a, x, y, z);
   int iters = 0;
@@ -417,7 +429,7 @@ __device__ __forceinline__ void grid32_body(const MapArgs& a, const Consts32& k,
   }
   block_count(valid ? 1 : 0, cnt, CNT_VOXELS);
   block_count(iters, cnt, CNT_ITERS);
-  block_append(need, (uint32_t)idx, blist, bcount);
+  if (!block_append(need, (uint32_t)idx, blist, bcount, bcap) && need) ((unsigned long long*)out)[idx] = kBandMark;
 }
 
 }  // namespace qj

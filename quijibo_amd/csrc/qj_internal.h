@@ -63,8 +63,9 @@ struct qj_rational {
 struct qj_workspace {
   int device = 0;
   unsigned long long* d_counters = nullptr;  // CNT_N x kCounterSpread
-  uint32_t* d_list = nullptr;                // band / adaptive voxel list
-  int64_t list_cap = 0;
+  uint32_t* d_list = nullptr;                // band list (list_cap entries), then the adaptive list
+  uint32_t* d_alist = nullptr;               // adaptive list (alist_cap entries)
+  int64_t list_cap = 0, alist_cap = 0;
   uint32_t* d_list_count = nullptr;          // [0] band count, [1] adaptive count
   // marching cubes
   void* mc = nullptr;
@@ -95,7 +96,7 @@ void make_geom(const qj_grid* g, GridGeom* out);
 qj_status make_map_args(const qj_grid* g, const qj_rational* f, const qj_map_params* p, MapArgs* a);
 double term_bits(const std::vector<RootRec64>& roots, double escape);
 double near_bits(const std::vector<RootRec64>& roots);
-qj_status ensure_list(qj_workspace* ws, int64_t n, hipStream_t s);
+qj_status ensure_list(qj_workspace* ws, int64_t n, bool adaptive, hipStream_t s);
 qj_status ensure_partials(qj_workspace* ws, int64_t n, hipStream_t s);
 // Records an event on s when timing is enabled (slot order: bulk0, bulk1, band0, band1).
 void timing_mark(qj_workspace* ws, hipStream_t s);
