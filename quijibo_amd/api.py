@@ -466,8 +466,9 @@ def points_adjoint(points, rational, params, dJdF, precision=FP64, workspace=Non
 
 
 def nonlinear_mesh(grid, rational, params, iso=0.0, precision=FP32_BAND, band=DEFAULT_BAND, chunk_slices=64,
-                   workspace=None, stream=None, allow_inconsistent=False):
-    """qj_nlmc_run + copy: the non-linear low-memory mesher. Returns (vertices, triangles)."""
+                   workspace=None, stream=None, allow_inconsistent=False, device_output=False):
+    """qj_nlmc_run + copy: the non-linear low-memory mesher. Returns (vertices, triangles), as host
+    arrays, or as device tensors with device_output=True."""
     ws = workspace or default_workspace()
     nv = ctypes.c_int64()
     nt = ctypes.c_int64()
@@ -477,6 +478,12 @@ def nonlinear_mesh(grid, rational, params, iso=0.0, precision=FP32_BAND, band=DE
                        _stream_ptr(stream))
     if st != 0 and not (allow_inconsistent and st == 5):
         check(st)
+    if device_output:
+        v = torch.empty((nv.value, 3), dtype=torch.float64, device="cuda")
+        t = torch.empty((nt.value, 3), dtype=torch.int32, device="cuda")
+        check(L.qj_mc_copy(ws.handle, _ptr(v) if nv.value else None, _ptr(t) if nt.value else None,
+                           _stream_ptr(stream)))
+        return v, t
     v = np.empty((nv.value, 3), dtype=np.float64)
     t = np.empty((nt.value, 3), dtype=np.int32)
     check(L.qj_mc_copy(ws.handle, v.ctypes.data_as(ctypes.c_void_p) if nv.value else None,

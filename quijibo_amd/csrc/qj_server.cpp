@@ -355,6 +355,33 @@ static void complete_slot(qj_server* s, Slot& sl, qj_status st) {
   sl.busy = false;
 }
 
+// Launches a batch into every free slot while live requests have chunks left;
+// returns whether it launched anything.
+static bool launch_free_slots(qj_server* s) {
+  bool launched = false;
+  for (size_t k = 0; k < s->slots.size(); k++) {
+    Slot& sl = s->slots[(s->slot_next + k) % s->slots.size()];
+    if (sl.busy) continue;
+    bool has_work = false;
+    for (auto& kv : s->live)
+      if (kv.second->next < kv.second->voxels) {
+        has_work = true;
+        break;
+      }
+    if (!has_work || build_batch(s, sl) == 0) break;
+    hipError_t e = hipGraphLaunch(sl.exec[sl.variants], sl.stream);
+    if (e == hipSuccess) e = hipEventRecord(sl.done, sl.stream);
+    sl.busy = true;
+    if (e != hipSuccess) complete_slot(s, sl, QJ_ERR_HIP);
+    s->slot_next = (s->slot_next + k + 1) % s->slots.size();
+    launched = true;
+  }
+  return launched;
+}
+
+// The worker: each pass retires finished launches and refills their slots
+// first (so the GPU waits for the host as little as possible), then admits
+// requests and retires cross-checked results.
 static void worker_main(qj_server* s) {
   hipSetDevice(s->cfg.device);
   while (true) {
@@ -374,9 +401,14 @@ static void worker_main(qj_server* s) {
       }
       s->release_q.clear();
     }
-    // 2. admit while the arena has room: requests older than aging_s first,
+    // 2. retire finished launches and refill their slots
+    for (Slot& sl : s->slots)
+      if (sl.busy && hipEventQuery(sl.done) == hipSuccess) complete_slot(s, sl, QJ_OK);
+    bool launched = launch_free_slots(s);
+    // 3. admit while the arena has room: requests older than aging_s first,
     //    oldest first (and nothing after an aged request that does not fit, so
     //    the space it needs frees up), then smallest footprint first
+    bool admitted = false;
     if (!s->pending.empty()) {
       const double now = s->now();
       std::stable_sort(s->pending.begin(), s->pending.end(),
@@ -401,6 +433,7 @@ static void worker_main(qj_server* s) {
         if (!held && try_admit(s, r.get())) {
           const uint64_t id = r->id;
           s->live[id] = std::move(r);
+          admitted = true;
         } else {
           if (!held && aged(s, r.get(), now) && r->bytes < blocked_bytes) blocked_bytes = r->bytes;
           keep.push_back(std::move(r));
@@ -408,9 +441,7 @@ static void worker_main(qj_server* s) {
       }
       s->pending.swap(keep);
     }
-    // 3. retire finished launches, then fields whose cross-check is done
-    for (Slot& sl : s->slots)
-      if (sl.busy && hipEventQuery(sl.done) == hipSuccess) complete_slot(s, sl, QJ_OK);
+    // 4. retire fields whose cross-check is done
     for (auto it = s->checking.begin(); it != s->checking.end();) {
       Req* r = it->second.get();
       if (hipEventQuery(s->check_ev[r->check_slot]) != hipSuccess) {
@@ -434,31 +465,15 @@ static void worker_main(qj_server* s) {
         own->k32.band = __builtin_inff();
         const uint64_t id = own->id;
         s->live[id] = std::move(own);
+        admitted = true;
         continue;
       }
       retire(s, std::move(own));
     }
-    // 4. launch while a slot is free and there is work
-    bool launched = false;
-    bool has_work = false;
-    for (auto& kv : s->live)
-      if (kv.second->next < kv.second->voxels) has_work = true;
-    if (has_work) {
-      for (size_t k = 0; k < s->slots.size(); k++) {
-        Slot& sl = s->slots[(s->slot_next + k) % s->slots.size()];
-        if (sl.busy) continue;
-        if (build_batch(s, sl) == 0) break;
-        hipError_t e = hipGraphLaunch(sl.exec[sl.variants], sl.stream);
-        if (e == hipSuccess) e = hipEventRecord(sl.done, sl.stream);
-        sl.busy = true;
-        if (e != hipSuccess) {
-          complete_slot(s, sl, QJ_ERR_HIP);
-        }
-        s->slot_next = (s->slot_next + k + 1) % s->slots.size();
-        launched = true;
-        break;
-      }
-    }
+    // new work and a free slot: go round again at once
+    bool free_slot = false;
+    for (Slot& sl : s->slots) free_slot |= !sl.busy;
+    if (admitted && free_slot) continue;
     if (launched) continue;
     bool any_busy = !s->checking.empty();
     for (Slot& sl : s->slots) any_busy |= sl.busy;
@@ -467,8 +482,8 @@ static void worker_main(qj_server* s) {
       if (s->incoming.empty()) break;
     }
     if (any_busy) {
-      // all slots busy (or no new work): wait briefly for the oldest launch or a submission
-      std::this_thread::sleep_for(std::chrono::microseconds(20));
+      // all slots busy (or no new work): wait briefly for a launch to finish or a submission
+      std::this_thread::sleep_for(std::chrono::microseconds(10));
     } else {
       std::unique_lock<std::mutex> lk(s->mu);
       s->cv_submit.wait_for(lk, std::chrono::milliseconds(1),
