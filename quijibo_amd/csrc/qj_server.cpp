@@ -359,8 +359,9 @@ static void complete_slot(qj_server* s, Slot& sl, qj_status st) {
 // returns whether it launched anything.
 static bool launch_free_slots(qj_server* s) {
   bool launched = false;
+  const size_t first = s->slot_next;
   for (size_t k = 0; k < s->slots.size(); k++) {
-    Slot& sl = s->slots[(s->slot_next + k) % s->slots.size()];
+    Slot& sl = s->slots[(first + k) % s->slots.size()];
     if (sl.busy) continue;
     bool has_work = false;
     for (auto& kv : s->live)
@@ -373,7 +374,7 @@ static bool launch_free_slots(qj_server* s) {
     if (e == hipSuccess) e = hipEventRecord(sl.done, sl.stream);
     sl.busy = true;
     if (e != hipSuccess) complete_slot(s, sl, QJ_ERR_HIP);
-    s->slot_next = (s->slot_next + k + 1) % s->slots.size();
+    s->slot_next = (first + k + 1) % s->slots.size();
     launched = true;
   }
   return launched;
