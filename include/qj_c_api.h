@@ -180,8 +180,11 @@ qj_status qj_workspace_kernel_ms(qj_workspace* ws, double* bulk_ms, double* band
  * reference-facing default is 0.05). */
 /* qj_field_eval and qj_points_eval enqueue kernels only (no memset nodes, no
  * host synchronization), so they may be captured in a hipGraph once a first
- * call of at least the same size has sized the workspace's lists (a larger
- * call reallocates). */
+ * call of at least the same size (and with QJ_MAP_ADAPTIVE if the captured
+ * call has it) has sized the workspace's lists; a larger call reallocates.
+ * The band list holds max(n/4, 65536) of the n cells (at most n); cells that
+ * need FP64 beyond it are marked in d_field and evaluated by a sweep kernel,
+ * so every band width gives the same field. */
 qj_status qj_field_eval(qj_workspace* ws, const qj_grid* grid, const qj_rational* f,
                         const qj_map_params* params, qj_precision precision, double band,
                         double* d_field, void* stream);
