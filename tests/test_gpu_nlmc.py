@@ -53,6 +53,22 @@ def test_chunking_and_precision_do_not_change_the_mesh(gpu):
         assert np.array_equal(t, ref[1]) and np.array_equal(v, ref[0]), (prec, chunk)
 
 
+def test_device_output_is_the_host_mesh(gpu):
+    """device_output=True returns the same mesh as the host arrays, bit for bit,
+    including a mesh that spans several slabs (the slab-wide vertex lists)."""
+    w = W.config2(8)
+    res = (50, 47, 45)
+    g = api.make_grid(res, w.center, w.lengths)
+    r = api.Rational(api.POLYNOMIAL_4D(w.top.roots, w.top.powers), api.POLYNOMIAL_4D(w.bottom.roots, w.bottom.powers))
+    p = api.make_params(w.max_iter, 2000.0, w.exp_scaling, w.slice, MAP_BLACK_HOLE)
+    ws = api.Workspace()
+    v, t = api.nonlinear_mesh(g, r, p, 0.0, FP32_BAND, 0.05, 8, workspace=ws)
+    dv, dt = api.nonlinear_mesh(g, r, p, 0.0, FP32_BAND, 0.05, 8, workspace=ws, device_output=True)
+    assert isinstance(dv, torch.Tensor) and dv.is_cuda and dt.dtype == torch.int32
+    assert len(v) > 1000
+    assert np.array_equal(dv.cpu().numpy(), v) and np.array_equal(dt.cpu().numpy(), t)
+
+
 def _setup(w, res, flags=MAP_BLACK_HOLE):
     g = api.make_grid(res, w.center, w.lengths)
     r = api.Rational(api.POLYNOMIAL_4D(w.top.roots, w.top.powers), api.POLYNOMIAL_4D(w.bottom.roots, w.bottom.powers))
