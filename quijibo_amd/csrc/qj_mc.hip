@@ -11,8 +11,10 @@
 //                   degenerate filter (edge < 1e-7 dropped): an FP32 estimate
 //                   of where each vertex sits on its edge clears every
 //                   triangle with fewer than two vertices near an edge end,
-//                   the rest are measured with the FP64 vertices; per block
-//                   of cells: kept-triangle count
+//                   the other cells go to a list; per block of cells:
+//                   kept-triangle count
+//   k_mc_measure    the listed cells' triangles measured with the FP64
+//                   vertices, one cell per lane
 //   k_mc_edges      per grid point: which of its +x,+y,+z edges some adjacent
 //                   cell's case row references (the edge-use masks of the four
 //                   cell rows around a block staged in LDS), packed with the
@@ -141,6 +143,9 @@ struct McState {
   uint16_t* pk16 = nullptr;    // linear mesher, per point: first vertex id within its block << 3 | edge mask
   uint8_t* kmask = nullptr;    // per cell: bit t set when triangle t of its case row is kept
   int64_t cap_kmask = 0;
+  uint32_t* mlist = nullptr;   // cells that await the degenerate measurement
+  int64_t cap_mlist = 0;
+  uint32_t* d_mcount = nullptr;
   uint16_t* tloc16 = nullptr;  // per active cell first triangle within its block of cells
   uint32_t* blkt = nullptr;    // per block of cells triangle count -> first triangle of the block
   int64_t cap_tloc16 = 0, cap_blkt = 0;
@@ -389,22 +394,69 @@ constexpr float kNearLimit = 1e-6f;
 constexpr double kPrefilterSpacing = 1e-6;
 constexpr double kPrefilterMaxSpacing = 3.0;
 
-// Cell case and kept triangles of every cell. flags[p] is the case of the
-// cell with lowest corner p (0 on the last layer of each axis), kmask[p] bit t
-// is set when triangle t of that case row survives the filter, blkt[block]
-// counts the block's kept triangles. prefilter = 0 measures every triangle.
-// The near test runs on all 12 edges with fixed corner indices (a vertex at
-// place t on a crossing edge has t = |v0| / |v0 - v1| and 1 - t = |v1| /
-// |v0 - v1|, so no division), masked by the case row's edges; a cell with
-// fewer than two near vertices keeps all its triangles without looking at
-// them, and only the others read the per-triangle edge masks. One cell per
-// thread, a block of 256 cells per block of the numbering (four cells per
-// lane, as in the kernels below, measured slower here: the triangle loops of
-// a lane's cells run one after the other).
+// The 12-bit mask of a cell's edges whose vertex may lie within 2e-7 of an
+// end of its edge, from the float corner values (corner k = 4 dx + 2 dy + dz):
+// near unless |v0| h >= 1e-6 |v0 - v1| and |v1| h >= 1e-6 |v0 - v1| (a vertex
+// at place t has t = |v0| / |v0 - v1| and 1 - t = |v1| / |v0 - v1|, so no
+// division); NaN and infinite corners make it near. prefilter = 0: all 12.
+__device__ __forceinline__ unsigned mc_near_edges(const float* v, const GridGeom& g, int prefilter) {
+  if (!prefilter) return 0xfffu;
+  const float h[3] = {fabsf((float)g.dx[0]), fabsf((float)g.dx[1]), fabsf((float)g.dx[2])};
+  unsigned near = 0;
+#pragma unroll
+  for (int e = 0; e < 12; e++) {
+    int dx = 0, dy = 0, dz = 0, axis = 0;
+    edge_parts(e, dx, dy, dz, axis);
+    const int k0 = 4 * dx + 2 * dy + dz, k1 = k0 + (axis == 0 ? 4 : (axis == 1 ? 2 : 1));
+    const float a0 = fabsf(v[k0]), a1 = fabsf(v[k1]), lim = kNearLimit * fabsf(v[k0] - v[k1]);
+    near |= (a0 * h[axis] >= lim && a1 * h[axis] >= lim) ? 0u : (1u << e);
+  }
+  return near;
+}
+
+// The 8 corner values of the cell at point p as floats (CUBE corners are
+// floats, TRIANGLE_MESH.h:283-288; sign test at TRIANGLE_MESH.cpp:1037-1039),
+// and its case.
+__device__ __forceinline__ int mc_corners(const double* __restrict__ field, int64_t p, int64_t X, int64_t XY,
+                                          double iso, float* v) {
+  int flag = 0;
+#pragma unroll
+  for (int k = 0; k < 8; k++) {
+    const int dx = k >> 2, dy = (k >> 1) & 1, dz = k & 1;
+    v[k] = (float)(field[p + dx + dy * X + dz * XY] - iso);
+    flag |= (v[k] > 0.0f) << k;
+  }
+  return flag;
+}
+
+// Bit 7 of kmask: the cell awaits measurement and did not fit the list.
+constexpr unsigned kMeasureMark = 0x80u;
+// The measure list is split in kMeasureLists sub-lists of equal capacity, one
+// counter each (block lb appends to sub-list lb % kMeasureLists): one counter
+// for all blocks serialised the appends (65536 atomics on one address per
+// 256^3 field). Counter kMeasureLists is set when a cell did not fit.
+constexpr int kMeasureLists = 251;
+
+// Cell case and provisional kept triangles of every cell. flags[p] is the
+// case of the cell with lowest corner p (0 on the last layer of each axis),
+// kmask[p] bit t is set when triangle t of that case row is kept, blkt[block]
+// counts the block's kept triangles. A cell with two or more near vertices
+// may hold a degenerate triangle: it keeps all its triangles here and goes to
+// the measure list (mlist: kMeasureLists sub-lists of mcap entries; beyond
+// that its kmask gets kMeasureMark), which k_mc_measure works through with
+// full waves. The FP64
+// measurement is rare but long, and one lane of a wave taking it held up the
+// whole wave here (the borders of a field set by setBorders put such cells in
+// every row). One cell per thread, a block of 256 cells per block of the
+// numbering.
 __global__ __launch_bounds__(kMcBlock) void k_mc_cells(const double* __restrict__ field, GridGeom g, double iso,
                                                        int prefilter, uint8_t* __restrict__ flags,
-                                                       uint8_t* __restrict__ kmask, uint32_t* __restrict__ blkt) {
+                                                       uint8_t* __restrict__ kmask, uint32_t* __restrict__ blkt,
+                                                       uint32_t* __restrict__ mlist, uint32_t* __restrict__ mcount,
+                                                       uint32_t mcap) {
   __shared__ unsigned short s_emn[256];
+  __shared__ uint32_t s_w[kMcBlock / 64], s_m[kMcBlock / 64];
+  __shared__ uint32_t s_mbase;
   s_emn[threadIdx.x] = kMc.emn[threadIdx.x];
   __syncthreads();
   const int64_t X = g.res[0], Y = g.res[1], Z = g.res[2], XY = X * Y, N = XY * Z;
@@ -412,66 +464,119 @@ __global__ __launch_bounds__(kMcBlock) void k_mc_cells(const double* __restrict_
   const int64_t p = lb * kMcBlock + threadIdx.x;
   int flag = 0;
   unsigned keep = 0;
+  bool measure = false;
   if (p < N) {
     int64_t x, y, z;
     point_xyz(g, p, x, y, z);
     if (x < X - 1 && y < Y - 1 && z < Z - 1) {
       float v[8];
-#pragma unroll
-      for (int k = 0; k < 8; k++) {
-        const int dx = k >> 2, dy = (k >> 1) & 1, dz = k & 1;
-        // CUBE corners are floats (TRIANGLE_MESH.h:283-288); test at TRIANGLE_MESH.cpp:1037-1039
-        v[k] = (float)(field[p + dx + dy * X + dz * XY] - iso);
-        flag |= (v[k] > 0.0f) << k;
-      }
+      flag = mc_corners(field, p, X, XY, iso, v);
       const unsigned emn = s_emn[flag];
       const int n = (int)(emn >> 12);
       if (n) {
         keep = (1u << n) - 1u;
-        unsigned near = ~0u;  // no prefilter: every triangle is measured
-        if (prefilter) {
-          const float h[3] = {fabsf((float)g.dx[0]), fabsf((float)g.dx[1]), fabsf((float)g.dx[2])};
-          near = 0;
-#pragma unroll
-          for (int e = 0; e < 12; e++) {
-            int dx = 0, dy = 0, dz = 0, axis = 0;
-            edge_parts(e, dx, dy, dz, axis);
-            const int k0 = 4 * dx + 2 * dy + dz, k1 = k0 + (axis == 0 ? 4 : (axis == 1 ? 2 : 1));
-            const float a0 = fabsf(v[k0]), a1 = fabsf(v[k1]), lim = kNearLimit * fabsf(v[k0] - v[k1]);
-            near |= (a0 * h[axis] >= lim && a1 * h[axis] >= lim) ? 0u : (1u << e);
-          }
-          near &= emn & 0xfffu;
-        }
-        if (__popc(near) >= 2) {
-          for (int t = 0; t < n; t++) {
-            const unsigned tm = kMc.tmask[flag][t];
-            if (__popc(near & tm) < 2) continue;
-            // measured with the FP64 vertices (the corner values read again:
-            // this path is rare, and a register array indexed per lane would
-            // go to scratch)
-            double w[3][3];
-            unsigned r = tm;
-            for (int j = 0; j < 3; j++) {
-              const int e = __ffs(r) - 1;
-              r &= r - 1;
-              int dx, dy, dz, axis;
-              edge_geom(e, dx, dy, dz, axis);
-              const int64_t q0 = p + dx + dy * X + dz * XY;
-              const int64_t q1 = q0 + (axis == 0 ? 1 : (axis == 1 ? X : XY));
-              mc_vertex(g, (int)x + dx, (int)y + dy, (int)z + dz, axis, field[q0], field[q1], iso, w[j]);
-            }
-            const double eps = 1e-7;
-            if (edge_len(w[0], w[1]) < eps || edge_len(w[0], w[2]) < eps || edge_len(w[1], w[2]) < eps)
-              keep &= ~(1u << t);
-          }
-        }
+        measure = __popc(mc_near_edges(v, g, prefilter) & emn & 0xfffu) >= 2;
       }
     }
-    flags[p] = (uint8_t)flag;
-    kmask[p] = (uint8_t)keep;
   }
-  const BlockPrefix tp = mc_block_prefix3<1, 2, 4>((unsigned)__popc(keep));
-  if (threadIdx.x == 0) blkt[lb] = tp.total;
+  // kept-triangle count of the block and this block's share of the measure list
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const unsigned cnt = (unsigned)__popc(keep);
+  const unsigned long long b0 = __ballot(cnt & 1u), b1 = __ballot(cnt & 2u), b2 = __ballot(cnt & 4u);
+  const unsigned long long bm = __ballot(measure);
+  if (lane == 0) {
+    s_w[wave] = (uint32_t)(__popcll(b0) + 2 * __popcll(b1) + 4 * __popcll(b2));
+    s_m[wave] = (uint32_t)__popcll(bm);
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    uint32_t t = 0, m = 0;
+    for (int i = 0; i < kMcBlock / 64; i++) {
+      t += s_w[i];
+      const uint32_t c = s_m[i];
+      s_m[i] = m;
+      m += c;
+    }
+    blkt[lb] = t;
+    const int sub = (int)(lb % kMeasureLists);
+    const uint32_t base = m ? atomicAdd(&mcount[sub], m) : 0u;
+    s_mbase = base;
+    if (m && base + m > mcap) mcount[kMeasureLists] = 1u;  // some cells did not fit
+  }
+  __syncthreads();
+  if (p < N) {
+    const uint32_t slot = s_mbase + s_m[wave] + (uint32_t)__popcll(bm & ((1ull << lane) - 1ull));
+    const bool listed = measure && slot < mcap;
+    if (listed) mlist[(int64_t)(lb % kMeasureLists) * mcap + slot] = (uint32_t)p;
+    flags[p] = (uint8_t)flag;
+    kmask[p] = (uint8_t)(keep | (measure && !listed ? kMeasureMark : 0u));
+  }
+}
+
+// The degenerate filter (TRIANGLE_MESH.cpp:1088-1102) for one listed cell:
+// triangles with two or three near vertices are measured with the FP64
+// vertices (the function the vertex kernel uses, so both see the same bits).
+// Only a side between two near vertices can be short: a vertex that is not
+// near is at least 2e-7 from both ends of its edge, so at least that far from
+// a vertex on an edge through either end, and a grid spacing from vertices on
+// the other edges. Dropped triangles leave kmask and the block's count.
+__device__ __forceinline__ void mc_measure_cell(const double* __restrict__ field, const GridGeom& g, double iso,
+                                                int prefilter, int64_t p, uint8_t* __restrict__ kmask,
+                                                uint32_t* __restrict__ blkt) {
+  const int64_t X = g.res[0], XY = X * g.res[1];
+  int64_t x, y, z;
+  point_xyz(g, p, x, y, z);
+  float v[8];
+  const int flag = mc_corners(field, p, X, XY, iso, v);
+  const unsigned near = mc_near_edges(v, g, prefilter) & kMc.emask[flag];
+  const int n = kMc.ntri[flag];
+  unsigned keep = kmask[p] & 0x1fu, dropped = 0;
+  for (int t = 0; t < n; t++) {
+    const unsigned nm = near & kMc.tmask[flag][t];
+    const int m = __popc(nm);
+    if (m < 2) continue;
+    double w[3][3] = {};
+    unsigned r = nm;
+#pragma unroll
+    for (int j = 0; j < 3; j++) {
+      if (j < m) {  // constant indices into w keep it in registers
+        const int e = __ffs(r) - 1;
+        r &= r - 1;
+        int dx, dy, dz, axis;
+        edge_geom(e, dx, dy, dz, axis);
+        const int64_t q0 = p + dx + dy * X + dz * XY;
+        const int64_t q1 = q0 + (axis == 0 ? 1 : (axis == 1 ? X : XY));
+        mc_vertex(g, (int)x + dx, (int)y + dy, (int)z + dz, axis, field[q0], field[q1], iso, w[j]);
+      }
+    }
+    const double eps = 1e-7;
+    bool degenerate = edge_len(w[0], w[1]) < eps;
+    if (m == 3) degenerate = degenerate || edge_len(w[0], w[2]) < eps || edge_len(w[1], w[2]) < eps;
+    if (degenerate) {
+      keep &= ~(1u << t);
+      dropped++;
+    }
+  }
+  kmask[p] = (uint8_t)keep;
+  if (dropped) atomicSub(&blkt[p / kMcBlock], dropped);
+}
+
+// The listed cells, one per thread; then, if the list overflowed, every cell
+// that carries kMeasureMark.
+__global__ __launch_bounds__(kMcBlock) void k_mc_measure(const double* __restrict__ field, GridGeom g, double iso,
+                                                         int prefilter, const uint32_t* __restrict__ mlist,
+                                                         const uint32_t* __restrict__ mcount, uint32_t mcap,
+                                                         uint8_t* __restrict__ kmask, uint32_t* __restrict__ blkt) {
+  const int64_t stride = (int64_t)gridDim.x * kMcBlock;
+  const int64_t total = (int64_t)kMeasureLists * mcap;
+  for (int64_t i = (int64_t)blockIdx.x * kMcBlock + threadIdx.x; i < total; i += stride) {
+    const int64_t sub = i / mcap, j = i - sub * mcap;
+    if (j < (int64_t)mcount[sub]) mc_measure_cell(field, g, iso, prefilter, (int64_t)mlist[i], kmask, blkt);
+  }
+  if (!mcount[kMeasureLists]) return;  // the same for every thread
+  const int64_t N = (int64_t)g.res[0] * g.res[1] * g.res[2];
+  for (int64_t p = (int64_t)blockIdx.x * kMcBlock + threadIdx.x; p < N; p += stride)
+    if (kmask[p] & kMeasureMark) mc_measure_cell(field, g, iso, prefilter, p, kmask, blkt);
 }
 
 // computeEdgeInterpolations (TRIANGLE_MESH.cpp:2373-2425). Vertex ids follow
@@ -704,6 +809,10 @@ __global__ void k_mc_zero_totals(uint32_t* __restrict__ totals) {
   if (threadIdx.x < 4) totals[threadIdx.x] = 0u;
 }
 
+__global__ void k_mc_zero_count(uint32_t* __restrict__ count, int n) {
+  for (int i = threadIdx.x; i < n; i += blockDim.x) count[i] = 0u;
+}
+
 // Meshes n fields one after another on the stream without waiting on the
 // host: every count stays on the device (running vertex and triangle totals,
 // per-tile counts) and is read back once at the end. The vertex and triangle
@@ -749,6 +858,7 @@ qj_status mc_run_tiles(qj_workspace* ws, int n, const double* const* fields, con
   const int64_t nbmax = nblocks(nmax);
   if ((st = grow(m.flags, m.cap_flags, nmax, s)) || (st = grow(m.kmask, m.cap_kmask, nmax, s)) ||
       (st = grow(m.pk16, m.cap_pk16, nmax, s)) || (st = grow(m.blkv, m.cap_blkv, nbmax, s)) ||
+      (st = grow(m.mlist, m.cap_mlist, std::max(nmax / 16, (int64_t)kMeasureLists * 256), s)) ||
       (st = grow(m.blkt, m.cap_blkt, nbmax, s)) || (st = grow(m.d_counts, m.cap_counts, 2 * (int64_t)n, s)))
     return st;
   const size_t need_tmp = scan_tmp_bytes(nbmax > 4096 ? nbmax : 4096);
@@ -764,6 +874,7 @@ qj_status mc_run_tiles(qj_workspace* ws, int n, const double* const* fields, con
   }
   if (!m.totals) QJ_HIP(hipMalloc(&m.totals, 4 * sizeof(uint32_t)));
   if (!m.d_base) QJ_HIP(hipMalloc(&m.d_base, 2 * sizeof(uint64_t)));
+  if (!m.d_mcount) QJ_HIP(hipMalloc(&m.d_mcount, sizeof(uint32_t) * (kMeasureLists + 1)));
   // first sizes: a quarter of the points as vertices, half as triangles
   // (a 256^3 torus tile holds 0.42 and 0.73 of its points)
   if (m.cap_v == 0 && (st = grow(m.verts, m.cap_v, 3 * (total_pts / 4 + 1024), s))) return st;
@@ -785,7 +896,14 @@ qj_status mc_run_tiles(qj_workspace* ws, int n, const double* const* fields, con
       } else {
         const int pf = prefilter[(size_t)k] ? 1 : 0;
         const dim3 nwg((nb.x + kMcWaves - 1) / kMcWaves);
-        hipLaunchKernelGGL(k_mc_cells, nb, dim3(kMcBlock), 0, s, fields[k], g, iso, pf, m.flags, m.kmask, m.blkt);
+        const uint32_t mcap = (uint32_t)(m.cap_mlist / kMeasureLists);  // entries per sub-list
+        hipLaunchKernelGGL(k_mc_zero_count, dim3(1), dim3(256), 0, s, m.d_mcount, kMeasureLists + 1);
+        QJ_HIP(hipGetLastError());
+        hipLaunchKernelGGL(k_mc_cells, nb, dim3(kMcBlock), 0, s, fields[k], g, iso, pf, m.flags, m.kmask, m.blkt,
+                           m.mlist, m.d_mcount, mcap);
+        QJ_HIP(hipGetLastError());
+        hipLaunchKernelGGL(k_mc_measure, dim3(std::min<unsigned>(nb.x, 1024u)), dim3(kMcBlock), 0, s, fields[k], g,
+                           iso, pf, (const uint32_t*)m.mlist, (const uint32_t*)m.d_mcount, mcap, m.kmask, m.blkt);
         QJ_HIP(hipGetLastError());
         hipLaunchKernelGGL(k_mc_edges, nwg, dim3(kMcGroup), 0, s, m.flags, g, m.pk16, m.blkv);
         QJ_HIP(hipGetLastError());
@@ -853,6 +971,8 @@ void mc_free(qj_workspace* ws) {
   hipFree(m->emask);
   hipFree(m->cnt);
   hipFree(m->pk16);
+  hipFree(m->mlist);
+  hipFree(m->d_mcount);
   hipFree(m->kmask);
   hipFree(m->tloc16);
   hipFree(m->blkt);
