@@ -756,8 +756,10 @@ __global__ __launch_bounds__(kMcGroup) void k_mc_tris(const uint8_t* __restrict_
 
 // After a tile: its vertex and triangle counts (the scans' totals) recorded
 // and added to the running totals.
+// (Both also clear the measure-list counters for the next field.)
 __global__ void k_mc_advance(const uint32_t* __restrict__ totals, uint64_t* __restrict__ base, int tile,
-                             int64_t* __restrict__ counts) {
+                             int64_t* __restrict__ counts, uint32_t* __restrict__ mcount) {
+  for (int i = threadIdx.x; i <= kMeasureLists; i += blockDim.x) mcount[i] = 0u;
   if (threadIdx.x != 0) return;
   const uint64_t nv = totals[1], nt = totals[2];
   counts[2 * tile + 0] = (int64_t)nv;
@@ -766,7 +768,8 @@ __global__ void k_mc_advance(const uint32_t* __restrict__ totals, uint64_t* __re
   base[1] += nt;
 }
 
-__global__ void k_mc_reset(uint64_t* __restrict__ base) {
+__global__ void k_mc_reset(uint64_t* __restrict__ base, uint32_t* __restrict__ mcount) {
+  for (int i = threadIdx.x; i <= kMeasureLists; i += blockDim.x) mcount[i] = 0u;
   if (threadIdx.x < 2) base[threadIdx.x] = 0;
 }
 
@@ -809,9 +812,6 @@ __global__ void k_mc_zero_totals(uint32_t* __restrict__ totals) {
   if (threadIdx.x < 4) totals[threadIdx.x] = 0u;
 }
 
-__global__ void k_mc_zero_count(uint32_t* __restrict__ count, int n) {
-  for (int i = threadIdx.x; i < n; i += blockDim.x) count[i] = 0u;
-}
 
 // Meshes n fields one after another on the stream without waiting on the
 // host: every count stays on the device (running vertex and triangle totals,
@@ -883,7 +883,7 @@ qj_status mc_run_tiles(qj_workspace* ws, int n, const double* const* fields, con
   uint64_t totals[2] = {0, 0};
   for (int pass = 0; pass < 2; pass++) {
     const int64_t vcap = m.cap_v / 3, tcap = m.cap_t / 3;
-    hipLaunchKernelGGL(k_mc_reset, dim3(1), dim3(64), 0, s, m.d_base);
+    hipLaunchKernelGGL(k_mc_reset, dim3(1), dim3(64), 0, s, m.d_base, m.d_mcount);
     QJ_HIP(hipGetLastError());
     for (int k = 0; k < n; k++) {
       const GridGeom& g = geo[(size_t)k];
@@ -897,8 +897,6 @@ qj_status mc_run_tiles(qj_workspace* ws, int n, const double* const* fields, con
         const int pf = prefilter[(size_t)k] ? 1 : 0;
         const dim3 nwg((nb.x + kMcWaves - 1) / kMcWaves);
         const uint32_t mcap = (uint32_t)(m.cap_mlist / kMeasureLists);  // entries per sub-list
-        hipLaunchKernelGGL(k_mc_zero_count, dim3(1), dim3(256), 0, s, m.d_mcount, kMeasureLists + 1);
-        QJ_HIP(hipGetLastError());
         hipLaunchKernelGGL(k_mc_cells, nb, dim3(kMcBlock), 0, s, fields[k], g, iso, pf, m.flags, m.kmask, m.blkt,
                            m.mlist, m.d_mcount, mcap);
         QJ_HIP(hipGetLastError());
@@ -916,7 +914,7 @@ qj_status mc_run_tiles(qj_workspace* ws, int n, const double* const* fields, con
                            (const uint64_t*)m.d_base, m.blkt, tcap, m.tris);
         QJ_HIP(hipGetLastError());
       }
-      hipLaunchKernelGGL(k_mc_advance, dim3(1), dim3(64), 0, s, m.totals, m.d_base, k, m.d_counts);
+      hipLaunchKernelGGL(k_mc_advance, dim3(1), dim3(64), 0, s, m.totals, m.d_base, k, m.d_counts, m.d_mcount);
       QJ_HIP(hipGetLastError());
     }
     QJ_HIP(hipMemcpyAsync(counts.data(), m.d_counts, sizeof(int64_t) * 2 * (size_t)n, hipMemcpyDeviceToHost, s));
