@@ -146,9 +146,8 @@ struct McState {
   uint32_t* mlist = nullptr;   // cells that await the degenerate measurement
   int64_t cap_mlist = 0;
   uint32_t* d_mcount = nullptr;
-  uint16_t* tloc16 = nullptr;  // per active cell first triangle within its block of cells
   uint32_t* blkt = nullptr;    // per block of cells triangle count -> first triangle of the block
-  int64_t cap_tloc16 = 0, cap_blkt = 0;
+  int64_t cap_blkt = 0;
   uint32_t* blkv = nullptr;    // per block vertex count -> first vertex id of the block
   int64_t cap_pk16 = 0, cap_blkv = 0;
   uint32_t* blk = nullptr;     // per block active count -> offset
@@ -972,7 +971,6 @@ void mc_free(qj_workspace* ws) {
   hipFree(m->mlist);
   hipFree(m->d_mcount);
   hipFree(m->kmask);
-  hipFree(m->tloc16);
   hipFree(m->blkt);
   hipFree(m->blkv);
   hipFree(m->blk);
