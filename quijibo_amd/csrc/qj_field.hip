@@ -389,12 +389,20 @@ __global__ __launch_bounds__(kBlock) void k_batch32(const ChunkDesc* __restrict_
       const MapArgs& a = d.a;
       const Consts32 k = d.k;
       const int64_t idx = d.start + local;
-      const Roots32K R = roots32_const(d.r32, a.nt + a.nb);
-      const Res32 r = eval32<kRN, kGen>(a, root_acc(R.q, R.pat, R.e, 0), root_acc(R.q, R.pat, R.e, a.nt), k,
-                                             chunk_start(a, idx));
-      if (r.written) {
-        if (!r.need64) d.out[idx] = (double)r.F;
-        need = r.need64;
+      if (__builtin_isinf(k.band)) {
+        // a band of infinite width (a result escalated to FP64): every cell goes
+        // to the FP64 pass, so the FP32 evaluation is skipped; cells that eval32
+        // leaves unwritten (the skipped origin) stay out of the list
+        const q64 q0 = chunk_start(a, idx);
+        need = !((a.flags & QJ_MAP_SKIP_ORIGIN) && qmag(q0) < 1e-8);
+      } else {
+        const Roots32K R = roots32_const(d.r32, a.nt + a.nb);
+        const Res32 r = eval32<kRN, kGen>(a, root_acc(R.q, R.pat, R.e, 0), root_acc(R.q, R.pat, R.e, a.nt), k,
+                                               chunk_start(a, idx));
+        if (r.written) {
+          if (!r.need64) d.out[idx] = (double)r.F;
+          need = r.need64;
+        }
       }
     }
     const uint32_t e = (uint32_t)(c * chunk_voxels + local);
