@@ -443,11 +443,10 @@ constexpr int kMeasureLists = 251;
 // may hold a degenerate triangle: it keeps all its triangles here and goes to
 // the measure list (mlist: kMeasureLists sub-lists of mcap entries; beyond
 // that its kmask gets kMeasureMark), which k_mc_measure works through with
-// full waves. The FP64
-// measurement is rare but long, and one lane of a wave taking it held up the
-// whole wave here (the borders of a field set by setBorders put such cells in
-// every row). One cell per thread, a block of 256 cells per block of the
-// numbering.
+// full waves. The FP64 measurement is rare but long, and one lane of a wave
+// taking it held up the whole wave here (the borders of a field set by
+// setBorders put such cells in every row). One cell per thread, a block of
+// 256 cells per block of the numbering.
 __global__ __launch_bounds__(kMcBlock) void k_mc_cells(const double* __restrict__ field, GridGeom g, double iso,
                                                        int prefilter, uint8_t* __restrict__ flags,
                                                        uint8_t* __restrict__ kmask, uint32_t* __restrict__ blkt,
