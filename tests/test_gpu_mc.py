@@ -77,6 +77,21 @@ def test_exact_iso_values_and_degenerate_triangles(gpu):
     assert_same_mesh(v, t, ov, ot)
 
 
+def test_measure_list_overflow_is_swept(gpu):
+    """A spacing below 1e-6 sends every active cell to the FP64 degenerate
+    test. On 64^3 random values nearly every cell is active, far more than the
+    measure list's 251 sub-lists hold (a sixteenth of the points, at least 251
+    x 256 entries), so most cells are marked in their mask byte and measured by
+    the sweep instead: the mesh must still be the oracle's."""
+    rng = np.random.default_rng(17)
+    res = (64, 64, 64)
+    f = rng.normal(size=res[0] * res[1] * res[2])
+    f[rng.integers(0, f.size, 200)] = 0.0  # vertices on grid points: coincident corners to drop
+    v, t, ov, ot = mc_both(f, res, lengths=(1e-5, 1e-5, 1e-5))
+    assert_same_mesh(v, t, ov, ot)
+    assert len(t) > 100000
+
+
 def test_iso_shift(gpu):
     rng = np.random.default_rng(3)
     res = (12, 12, 12)
