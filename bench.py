@@ -544,6 +544,7 @@ def c3_tiles(args, log):
     torch.cuda.synchronize()
     t_mc = time.perf_counter() - t0
     tot_v, tot_t = int(v.shape[0]), int(t.shape[0])
+    mc_bytes = len(tiles) * per * 8 + tot_v * 24 + tot_t * 12
     nan_vertices = int((~torch.isfinite(v)).any(dim=1).sum().item())
     log("c3: marching cubes %.1f ms, %d vertices, %d triangles, %d non-finite vertices" % (t_mc * 1e3, tot_v,
                                                                                           tot_t, nan_vertices))
@@ -573,6 +574,12 @@ def c3_tiles(args, log):
             "band_voxels": st["band_voxels"], "nonfinite_cells": nonfinite, "error_statement": worst,
             "clamp_borders_ms": round(t_clamp * 1e3, 2), "mc_ms_all_tiles": round(t_mc * 1e3, 2),
             "mc_host_syncs": 1, "vertices": tot_v, "triangles": tot_t, "nonfinite_vertices": nan_vertices,
+            "mc_roofline": {"bound": "hbm", "algorithmic_bytes": mc_bytes, "achieved": round(mc_bytes / t_mc / 1e9, 1),
+                            "peak": 8000.0, "unit": "GB/s", "frac": round(mc_bytes / t_mc / 8e12, 4),
+                            "note": "the fields read once (8 B per point) and the meshes written once (24 B per "
+                                    "vertex, 12 B per triangle) over the call's time, which includes the copy of "
+                                    "the meshes into the output tensors; the kernels are latency-bound "
+                                    "(profiles/marching_cubes_c3_tile.md)"},
             "mc_note": "one qj_mc_run_tiles call over the 64 clamped tiles and the device-to-device copy of the "
                        "mesh into the output tensors; buffers sized by an untimed call over the same tiles",
             "parity": parity}
