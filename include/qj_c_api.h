@@ -230,6 +230,17 @@ qj_status qj_mc_run(qj_workspace* ws, const double* d_field, const qj_grid* grid
  * Synchronizes `stream`. */
 qj_status qj_mc_run_tiles(qj_workspace* ws, int32_t n, const double* const* d_fields, const qj_grid* grids,
                           double iso, int64_t* n_vertices, int64_t* n_triangles, void* stream);
+/* qj_mc_run_tiles writing the meshes straight into caller device buffers (no
+ * copy out of the workspace): d_vertices holds vertex_capacity x 3 doubles,
+ * d_triangles triangle_capacity x 3 int32, in the same order. When the meshes
+ * need more, returns QJ_ERR_CAPACITY with the counts filled in (the sizes to
+ * pass next time) and the buffers' contents undefined. The workspace keeps no
+ * mesh: qj_mc_copy and the edge cache do not apply. Synchronizes `stream`.
+ * No counterpart in the reference (its meshes live in host memory). */
+qj_status qj_mc_run_tiles_into(qj_workspace* ws, int32_t n, const double* const* d_fields, const qj_grid* grids,
+                               double iso, double* d_vertices, int64_t vertex_capacity, int32_t* d_triangles,
+                               int64_t triangle_capacity, int64_t* n_vertices, int64_t* n_triangles,
+                               void* stream);
 /* Copy the last mesh: vertices n_vertices x 3 doubles, triangles n_triangles x 3
  * int32 (0-based). Pointers may be host or device memory (hipMemcpyDefault). */
 qj_status qj_mc_copy(qj_workspace* ws, double* vertices, int32_t* triangles, void* stream);

@@ -107,6 +107,9 @@ SIGNATURES = [
                                    ctypes.POINTER(ctypes.c_int64), _P]),
     ("qj_mc_run_tiles", ctypes.c_int32, [_P, ctypes.c_int32, _P, ctypes.POINTER(Grid), ctypes.c_double,
                                          ctypes.POINTER(ctypes.c_int64), ctypes.POINTER(ctypes.c_int64), _P]),
+    ("qj_mc_run_tiles_into", ctypes.c_int32, [_P, ctypes.c_int32, _P, ctypes.POINTER(Grid), ctypes.c_double, _P,
+                                              ctypes.c_int64, _P, ctypes.c_int64, ctypes.POINTER(ctypes.c_int64),
+                                              ctypes.POINTER(ctypes.c_int64), _P]),
     ("qj_nlmc_run", ctypes.c_int32, [_P, ctypes.POINTER(Grid), _P, ctypes.POINTER(MapParams), ctypes.c_double,
                                      ctypes.c_int32, ctypes.c_double, ctypes.c_int32, ctypes.POINTER(ctypes.c_int64),
                                      ctypes.POINTER(ctypes.c_int64), _P]),

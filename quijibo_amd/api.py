@@ -331,11 +331,14 @@ def marching_cubes(field, grid, iso=0.0, workspace=None, stream=None, device_out
     return v, t
 
 
-def marching_cubes_tiles(fields, grids, iso=0.0, workspace=None, stream=None):
+def marching_cubes_tiles(fields, grids, iso=0.0, workspace=None, stream=None, out=None):
     """qj_mc_run_tiles: marching cubes of several fields with one read-back of
     the counts. Returns (vertices, triangles, counts) with the meshes one after
     another (triangle indices local to their field) as device tensors and the
-    per-field (vertices, triangles) counts."""
+    per-field (vertices, triangles) counts. out = (vertices (cap_v, 3) float64,
+    triangles (cap_t, 3) int32) CUDA tensors: the meshes are written straight
+    into them (qj_mc_run_tiles_into, no copy) and views of their first rows are
+    returned; QJError (status QJ_ERR_CAPACITY) when they are too small."""
     n = len(fields)
     if len(grids) != n:
         raise ValueError("one grid per field")
@@ -347,6 +350,17 @@ def marching_cubes_tiles(fields, grids, iso=0.0, workspace=None, stream=None):
     nv = (ctypes.c_int64 * max(n, 1))()
     nt = (ctypes.c_int64 * max(n, 1))()
     L = _lib.load()
+    if out is not None:
+        ov, ot = out
+        for name, t, dt in (("out vertices", ov, torch.float64), ("out triangles", ot, torch.int32)):
+            if (not isinstance(t, torch.Tensor) or not t.is_cuda or t.dtype != dt or t.dim() != 2 or t.shape[1] != 3
+                    or not t.is_contiguous()):
+                raise ValueError("%s must be a contiguous (n, 3) %s CUDA tensor" % (name, dt))
+        check(L.qj_mc_run_tiles_into(ws.handle, n, ctypes.cast(ptrs, ctypes.c_void_p), garr, float(iso), _ptr(ov),
+                                     int(ov.shape[0]), _ptr(ot), int(ot.shape[0]), nv, nt, _stream_ptr(stream)))
+        counts = [(nv[k], nt[k]) for k in range(n)]
+        tv, tt = sum(c[0] for c in counts), sum(c[1] for c in counts)
+        return ov[:tv], ot[:tt], counts
     check(L.qj_mc_run_tiles(ws.handle, n, ctypes.cast(ptrs, ctypes.c_void_p), garr, float(iso), nv, nt,
                             _stream_ptr(stream)))
     counts = [(nv[k], nt[k]) for k in range(n)]

@@ -505,6 +505,25 @@ qj_status qj_mc_run_tiles(qj_workspace* ws, int32_t n, const double* const* d_fi
   return mc_run_tiles(ws, n, d_fields, grids, iso, n_vertices, n_triangles, (hipStream_t)stream);
 }
 
+qj_status qj_mc_run_tiles_into(qj_workspace* ws, int32_t n, const double* const* d_fields, const qj_grid* grids,
+                               double iso, double* d_vertices, int64_t vertex_capacity, int32_t* d_triangles,
+                               int64_t triangle_capacity, int64_t* n_vertices, int64_t* n_triangles,
+                               void* stream) {
+  if (!ws || n < 0 || (n > 0 && (!d_fields || !grids || !n_vertices || !n_triangles)) || !d_vertices ||
+      !d_triangles || vertex_capacity < 0 || triangle_capacity < 0) {
+    set_error("qj_mc_run_tiles_into: null argument or negative count or capacity");
+    return QJ_ERR_INVALID;
+  }
+  for (int k = 0; k < n; k++)
+    if (!d_fields[k]) {
+      set_error("qj_mc_run_tiles_into: field %d is null", k);
+      return QJ_ERR_INVALID;
+    }
+  std::lock_guard<std::mutex> lock(ws->mu);
+  return mc_run_tiles(ws, n, d_fields, grids, iso, n_vertices, n_triangles, (hipStream_t)stream, d_vertices,
+                      vertex_capacity, d_triangles, triangle_capacity);
+}
+
 qj_status qj_nlmc_run(qj_workspace* ws, const qj_grid* grid, const qj_rational* f, const qj_map_params* params,
                       double iso, qj_precision precision, double band, int32_t chunk_slices, int64_t* n_vertices,
                       int64_t* n_triangles, void* stream) {

@@ -147,7 +147,8 @@ qj_status launch_batch(const ChunkDesc* d_desc, int nchunks, int chunk_voxels, q
 qj_status mc_run(qj_workspace* ws, const double* d_field, const qj_grid* g, double iso, int64_t* nv,
                  int64_t* nt, hipStream_t s);
 qj_status mc_run_tiles(qj_workspace* ws, int n, const double* const* fields, const qj_grid* grids, double iso,
-                       int64_t* nv, int64_t* nt, hipStream_t s);
+                       int64_t* nv_out, int64_t* nt_out, hipStream_t s, double* ext_v = nullptr,
+                       int64_t ext_vcap = 0, int32_t* ext_t = nullptr, int64_t ext_tcap = 0);
 qj_status mc_copy(qj_workspace* ws, double* v, int32_t* t, hipStream_t s);
 qj_status nlmc_cache(qj_workspace* ws, int64_t* n_edges, int32_t* edges, int64_t* n_cells, int32_t* cells,
                      hipStream_t s);

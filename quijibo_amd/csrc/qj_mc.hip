@@ -817,8 +817,12 @@ __global__ void k_mc_zero_totals(uint32_t* __restrict__ totals) {
 // buffers keep their size from earlier calls; a pass that outgrows them
 // writes nothing beyond them, and the host then sizes them from the totals it
 // read and runs the pass again.
+// ext_v / ext_t (with their capacities in vertices and triangles): the
+// caller's buffers, written directly in one pass; null = the workspace's.
 qj_status mc_run_tiles(qj_workspace* ws, int n, const double* const* fields, const qj_grid* grids, double iso,
-                       int64_t* nv_out, int64_t* nt_out, hipStream_t s) {
+                       int64_t* nv_out, int64_t* nt_out, hipStream_t s, double* ext_v, int64_t ext_vcap,
+                       int32_t* ext_t, int64_t ext_tcap) {
+  const bool ext = ext_v != nullptr;
   if (!ws->mc) ws->mc = new McState();
   McState& m = *(McState*)ws->mc;
   m.nv = m.nt = 0;
@@ -875,12 +879,14 @@ qj_status mc_run_tiles(qj_workspace* ws, int n, const double* const* fields, con
   if (!m.d_mcount) QJ_HIP(hipMalloc(&m.d_mcount, sizeof(uint32_t) * (kMeasureLists + 1)));
   // first sizes: a quarter of the points as vertices, half as triangles
   // (a 256^3 torus tile holds 0.42 and 0.73 of its points)
-  if (m.cap_v == 0 && (st = grow(m.verts, m.cap_v, 3 * (total_pts / 4 + 1024), s))) return st;
-  if (m.cap_t == 0 && (st = grow(m.tris, m.cap_t, 3 * (total_pts / 2 + 1024), s))) return st;
+  if (!ext && m.cap_v == 0 && (st = grow(m.verts, m.cap_v, 3 * (total_pts / 4 + 1024), s))) return st;
+  if (!ext && m.cap_t == 0 && (st = grow(m.tris, m.cap_t, 3 * (total_pts / 2 + 1024), s))) return st;
   std::vector<int64_t> counts(2 * (size_t)n);
   uint64_t totals[2] = {0, 0};
   for (int pass = 0; pass < 2; pass++) {
-    const int64_t vcap = m.cap_v / 3, tcap = m.cap_t / 3;
+    const int64_t vcap = ext ? ext_vcap : m.cap_v / 3, tcap = ext ? ext_tcap : m.cap_t / 3;
+    double* const verts = ext ? ext_v : m.verts;
+    int32_t* const tris = ext ? ext_t : m.tris;
     hipLaunchKernelGGL(k_mc_reset, dim3(1), dim3(64), 0, s, m.d_base, m.d_mcount);
     QJ_HIP(hipGetLastError());
     for (int k = 0; k < n; k++) {
@@ -905,11 +911,11 @@ qj_status mc_run_tiles(qj_workspace* ws, int n, const double* const* fields, con
         QJ_HIP(hipGetLastError());
         if ((st = scan_u32(m.blkv, m.blkv, nb.x, m.totals + 1, m.tmp, m.tmp_bytes, s))) return st;
         hipLaunchKernelGGL(k_mc_vertices, nwg, dim3(kMcGroup), 0, s, fields[k], g, iso, m.pk16, m.blkv,
-                           (const uint64_t*)m.d_base, vcap, m.verts);
+                           (const uint64_t*)m.d_base, vcap, verts);
         QJ_HIP(hipGetLastError());
         if ((st = scan_u32(m.blkt, m.blkt, nb.x, m.totals + 2, m.tmp, m.tmp_bytes, s))) return st;
         hipLaunchKernelGGL(k_mc_tris, nwg, dim3(kMcGroup), 0, s, m.flags, m.kmask, g, m.pk16, m.blkv,
-                           (const uint64_t*)m.d_base, m.blkt, tcap, m.tris);
+                           (const uint64_t*)m.d_base, m.blkt, tcap, tris);
         QJ_HIP(hipGetLastError());
       }
       hipLaunchKernelGGL(k_mc_advance, dim3(1), dim3(64), 0, s, m.totals, m.d_base, k, m.d_counts, m.d_mcount);
@@ -919,6 +925,16 @@ qj_status mc_run_tiles(qj_workspace* ws, int n, const double* const* fields, con
     QJ_HIP(hipMemcpyAsync(totals, m.d_base, sizeof(totals), hipMemcpyDeviceToHost, s));
     QJ_HIP(hipStreamSynchronize(s));
     if ((int64_t)totals[0] <= vcap && (int64_t)totals[1] <= tcap) break;
+    if (ext) {
+      // the caller's buffers are too small: report the sizes needed
+      for (int k = 0; k < n; k++) {
+        if (nv_out) nv_out[k] = counts[2 * (size_t)k];
+        if (nt_out) nt_out[k] = counts[2 * (size_t)k + 1];
+      }
+      set_error("marching cubes: the meshes need %llu vertices and %llu triangles (buffers for %lld and %lld)",
+                (unsigned long long)totals[0], (unsigned long long)totals[1], (long long)vcap, (long long)tcap);
+      return QJ_ERR_CAPACITY;
+    }
     if (pass == 1) {
       set_error("marching cubes: counts changed between two passes over the same fields");
       return QJ_ERR_NUMERIC;
@@ -928,8 +944,8 @@ qj_status mc_run_tiles(qj_workspace* ws, int n, const double* const* fields, con
         (st = grow(m.tris, m.cap_t, 3 * (int64_t)totals[1], s)))
       return st;
   }
-  m.nv = (int64_t)totals[0];
-  m.nt = (int64_t)totals[1];
+  m.nv = ext ? 0 : (int64_t)totals[0];
+  m.nt = ext ? 0 : (int64_t)totals[1];
   for (int k = 0; k < n; k++) {
     if (nv_out) nv_out[k] = counts[2 * (size_t)k];
     if (nt_out) nt_out[k] = counts[2 * (size_t)k + 1];

@@ -7,7 +7,7 @@ import torch
 
 import oracle as O
 from quijibo_amd import api, workloads as W
-from quijibo_amd._lib import FP64, MAP_BLACK_HOLE
+from quijibo_amd._lib import FP64, MAP_BLACK_HOLE, QJError
 
 pytestmark = pytest.mark.gpu
 
@@ -164,3 +164,16 @@ def test_tiles_in_one_call_match_single_calls_and_the_oracle(gpu):
     # the same workspace again: buffers now large enough, one pass
     v2, t2, counts2 = api.marching_cubes_tiles(fields, grids, workspace=ws)
     assert counts2 == counts and torch.equal(v2.cpu(), torch.from_numpy(v)) and np.array_equal(t2.cpu().numpy(), t)
+    # straight into caller buffers (qj_mc_run_tiles_into): larger than needed, the same mesh in their
+    # first rows and the tails untouched; too small, a capacity error that reports the sizes needed
+    bv = torch.full((len(v) + 7, 3), -5.0, dtype=torch.float64, device="cuda")
+    bt = torch.full((len(t) + 5, 3), -9, dtype=torch.int32, device="cuda")
+    v3, t3, counts3 = api.marching_cubes_tiles(fields, grids, workspace=api.Workspace(), out=(bv, bt))
+    assert counts3 == counts and v3.data_ptr() == bv.data_ptr()
+    assert np.array_equal(v3.cpu().numpy(), v) and np.array_equal(t3.cpu().numpy(), t)
+    assert bool((bv[len(v):] == -5.0).all()) and bool((bt[len(t):] == -9).all())
+    small = (torch.empty((len(v) - 1, 3), dtype=torch.float64, device="cuda"),
+             torch.empty((len(t), 3), dtype=torch.int32, device="cuda"))
+    with pytest.raises(QJError) as e:
+        api.marching_cubes_tiles(fields, grids, workspace=ws, out=small)
+    assert e.value.status == 4 and str(len(v)) in str(e.value)
