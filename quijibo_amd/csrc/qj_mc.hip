@@ -1113,14 +1113,17 @@ __global__ __launch_bounds__(kMcBlock) void k_nl_edges(const uint8_t* __restrict
 // neighbour (computeNonlinearEdgeInterpolationsHuge :2034-2123 with
 // midpointSearch :2232-2271). va: value at (x, y, z). Counts edges whose end
 // values do not straddle the surface in *bad (the reference exits there).
+// fb: the field value at the +axis neighbour when the caller has it (the value
+// minus iso is the neighbour's end value), else null and it is evaluated.
 template <int kPow, bool kLag>
 __device__ __forceinline__ void place_vertex(const MapArgs& a, const RootRec64* __restrict__ roots, double iso,
                                              int x, int y, int z, int axis, const double* pa, double va,
-                                             double* __restrict__ out, unsigned int* __restrict__ bad) {
+                                             const double* fb, double* __restrict__ out,
+                                             unsigned int* __restrict__ bad) {
 #pragma clang fp contract(off)
   double pb[3];
   cell_center(a, x + (axis == 0), y + (axis == 1), z + (axis == 2), pb);
-  const double vb = nonlinear_value<kPow, kLag>(a, roots, pb, iso);
+  const double vb = fb ? *fb - iso : nonlinear_value<kPow, kLag>(a, roots, pb, iso);
   // first = lower endpoint; the positive end is the first unless firstValue < 0
   double pp[3], pn[3], vp, vn;
   if (va < 0) {
@@ -1176,14 +1179,16 @@ __global__ __launch_bounds__(kMcBlock) void k_nl_vlist(const uint8_t* __restrict
     }
 }
 
-// Vertices by bisection, one per thread over the list: both end values, then
-// up to 6 midpoints (FP64 map evaluations on full waves; about one point in
+// Vertices by bisection, one per thread over the list: both end values
+// (evaluated, or read from the slab), then up to 6 midpoints (FP64 map evaluations on full waves; about one point in
 // ten owns an edge, so a thread per point would leave most lanes idle). The
 // range's vertex count is the scan total minus the scan at its first point;
-// vertex t gets id base + off[0] + t.
+// vertex t gets id base + off[0] + t. ends: null, or the slab's values from
+// the range's first point on, when they are the end values' own (see
+// slab_holds_end_values); the midpoints are always evaluated.
 template <int kPow, bool kLag>
 __global__ __launch_bounds__(kMcBlock) void k_nl_vertices(MapArgs a, const RootRec64* __restrict__ roots,
-                                                          double iso, int64_t p0,
+                                                          double iso, int64_t p0, const double* __restrict__ ends,
                                                           const uint32_t* __restrict__ vpt,
                                                           const uint8_t* __restrict__ vax,
                                                           const uint32_t* __restrict__ off,
@@ -1201,9 +1206,10 @@ __global__ __launch_bounds__(kMcBlock) void k_nl_vertices(MapArgs a, const RootR
   const int y = (int)(r / X), x = (int)(r - (int64_t)y * X), z = (int)(p0 + lz);
   double pa[3];
   cell_center(a, x, y, z, pa);
-  const double va = nonlinear_value<kPow, kLag>(a, roots, pa, iso);
+  const double va = ends ? ends[q] - iso : nonlinear_value<kPow, kLag>(a, roots, pa, iso);
+  const double* fb = ends ? ends + q + (axis == 0 ? 1 : axis == 1 ? X : XY) : nullptr;
   const int64_t k = (int64_t)base + first + t;
-  place_vertex<kPow, kLag>(a, roots, iso, x, y, z, axis, pa, va, verts + 3 * k, bad);
+  place_vertex<kPow, kLag>(a, roots, iso, x, y, z, axis, pa, va, fb, verts + 3 * k, bad);
   edges[4 * k + 0] = x;
   edges[4 * k + 1] = y;
   edges[4 * k + 2] = z;
@@ -1331,6 +1337,25 @@ static qj_status grow_keep(T*& p, int64_t& cap, int64_t n, int64_t used, hipStre
   return QJ_OK;
 }
 
+// Whether the slab's values can stand in for the bisection's end values
+// (TRIANGLE_MESH.cpp:2063-2067 evaluates them again with nonlinearValue,
+// whose values the reference's slices hold too: computeNonlinearSlice
+// :330-378). In the FP32 + FP64 band field every cell with |F| < band holds
+// the FP64 value of the evaluator nonlinear_value calls (k_list64 and
+// k_band_sweep with the same power mode and range), and every other cell an
+// FP32 value of the FP64 value's sign: with iso 0 the end values enter the
+// bisection only through their signs (the orientation and the straddle
+// tests; products of values of magnitude band or more do not underflow), so
+// the vertices are those of the evaluated end values. Not for the caller's
+// field (its values are not the map's), the Lagrange order (its value has
+// its own infinite case), the plain FP64 slabs (polar powers, where the
+// bisection evaluates exact ones), the AP flags or a band of 0.
+static bool slab_holds_end_values(const MapArgs& a, const double* src_field, double iso, qj_precision prec,
+                                  double band) {
+  return !src_field && prec == QJ_FP32_BAND && iso == 0.0 && band > 0.0 &&
+         !(a.flags & (QJ_MAP_LAGRANGE | QJ_MAP_SKIP_ORIGIN | QJ_MAP_ADAPTIVE));
+}
+
 qj_status nlmc_run(qj_workspace* ws, const MapArgs& a0, const qj_rational* f, double iso, qj_precision prec,
                    double band, int chunk, int64_t* nv_out, int64_t* nt_out, int64_t* n_bad, hipStream_t s,
                    const double* src_field) {
@@ -1372,6 +1397,7 @@ qj_status nlmc_run(qj_workspace* ws, const MapArgs& a0, const qj_rational* f, do
   QJ_HIP(hipMemsetAsync(m.d_bad, 0, sizeof(unsigned int), s));
   // the mesher's map: escape 2000 and the black-hole stop (TRIANGLE_MESH.cpp:97, 1789-1791)
   MapArgs a = a0;
+  const bool reuse_ends = slab_holds_end_values(a, src_field, iso, prec, band);
   uint64_t nv = 0, nt = 0, placed_next = 0;
   for (int64_t z0 = 0; z0 < Z; z0 += chunk) {
     const int64_t z1 = z0 + chunk < Z ? z0 + chunk : Z;
@@ -1387,6 +1413,7 @@ qj_status nlmc_run(qj_workspace* ws, const MapArgs& a0, const qj_rational* f, do
       // the Lagrange order has no FP32 path: its slabs are FP64
       if ((st = launch_field(ws, sa, f, (a.flags & QJ_MAP_LAGRANGE) ? QJ_FP64 : prec, band, m.slab, s))) return st;
     }
+    const double* ends = reuse_ends ? slab : nullptr;
     // 2. cases
     hipLaunchKernelGGL(k_nl_flags, dim3(nblocks(XY * ns)), dim3(kMcBlock), 0, s, slab, X, Y, Z, s0, ns, m.flags);
     QJ_HIP(hipGetLastError());
@@ -1426,7 +1453,8 @@ qj_status nlmc_run(qj_workspace* ws, const MapArgs& a0, const qj_rational* f, do
       hipLaunchKernelGGL(k_nl_vlist, dim3(nblocks(n)), dim3(kMcBlock), 0, s, m.emask + skip * XY,
                          m.cnt + skip * XY, n, m.vpt, m.vax);
       QJ_HIP(hipGetLastError());
-      launch_nl_vertices(a, dim3(nblocks((int64_t)cnt_own)), s, f->d64, iso, z0 + skip, m.vpt, m.vax,
+      launch_nl_vertices(a, dim3(nblocks((int64_t)cnt_own)), s, f->d64, iso, z0 + skip,
+                         ends ? ends + (z0 + skip - s0) * XY : (const double*)nullptr, m.vpt, m.vax,
                          m.cnt + skip * XY, m.totals + 1, (uint32_t)nv, m.verts, m.c_edges, m.d_bad);
       QJ_HIP(hipGetLastError());
     }
@@ -1434,7 +1462,8 @@ qj_status nlmc_run(qj_workspace* ws, const MapArgs& a0, const qj_rational* f, do
       hipLaunchKernelGGL(k_nl_vlist, dim3(nblocks(XY)), dim3(kMcBlock), 0, s, m.emask + nown, m.slice_off, XY,
                          m.vpt + cnt_own, m.vax + cnt_own);
       QJ_HIP(hipGetLastError());
-      launch_nl_vertices(a, dim3(nblocks((int64_t)cnt_next)), s, f->d64, iso, z1, m.vpt + cnt_own,
+      launch_nl_vertices(a, dim3(nblocks((int64_t)cnt_next)), s, f->d64, iso, z1,
+                         ends ? ends + (z1 - s0) * XY : (const double*)nullptr, m.vpt + cnt_own,
                          m.vax + cnt_own, m.slice_off, m.totals + 3, (uint32_t)(nv + cnt_own), m.verts, m.c_edges,
                          m.d_bad);
       QJ_HIP(hipGetLastError());
@@ -1536,7 +1565,7 @@ __global__ __launch_bounds__(kMcBlock) void k_cache_vertices(MapArgs a, const Ro
   double pa[3];
   cell_center(a, x, y, z, pa);
   const double va = nonlinear_value<kPow, kLag>(a, roots, pa, iso);
-  place_vertex<kPow, kLag>(a, roots, iso, x, y, z, axis, pa, va, verts + 3 * i, bad);
+  place_vertex<kPow, kLag>(a, roots, iso, x, y, z, axis, pa, va, nullptr, verts + 3 * i, bad);
 }
 
 template <class... Args>

@@ -53,6 +53,19 @@ def test_chunking_and_precision_do_not_change_the_mesh(gpu):
         assert np.array_equal(t, ref[1]) and np.array_equal(v, ref[0]), (prec, chunk)
 
 
+def test_slab_end_values_give_the_evaluated_vertices(gpu):
+    """With FP32+band slabs and iso 0 the bisection takes its end values from
+    the slab (their signs are the FP64 map's: cells with |F| < band hold the
+    FP64 value itself); any other iso evaluates them. iso = 1e-300 changes no
+    value above 1e-284 in magnitude, so both runs must give the same mesh bit
+    for bit: the headline's 32+32 roots at 96^3 over three slabs."""
+    w = W.headline(96)
+    read = _run(w, w.res, FP32_BAND, chunk=32, iso=0.0)
+    evaluated = _run(w, w.res, FP32_BAND, chunk=32, iso=1e-300)
+    assert len(read[0]) > 50000
+    assert np.array_equal(read[1], evaluated[1]) and np.array_equal(read[0], evaluated[0])
+
+
 def test_device_output_is_the_host_mesh(gpu):
     """device_output=True returns the same mesh as the host arrays, bit for bit,
     including a mesh that spans several slabs (the slab-wide vertex lists)."""
