@@ -1180,25 +1180,43 @@ __global__ __launch_bounds__(kMcBlock) void k_nl_vlist(const uint8_t* __restrict
 }
 
 // Vertices by bisection, one per thread over the list: both end values
-// (evaluated, or read from the slab), then up to 6 midpoints (FP64 map evaluations on full waves; about one point in
-// ten owns an edge, so a thread per point would leave most lanes idle). The
-// range's vertex count is the scan total minus the scan at its first point;
-// vertex t gets id base + off[0] + t. ends: null, or the slab's values from
-// the range's first point on, when they are the end values' own (see
-// slab_holds_end_values); the midpoints are always evaluated.
+// (evaluated, or read from the slab), then up to 6 midpoints (FP64 map
+// evaluations on full waves; about one point in ten owns an edge, so a thread
+// per point would leave most lanes idle). One launch takes two ranges of
+// points: the chunk's own points from slice p0a (list entries [0, nA), nA the
+// scan total minus the scan at the range's first point, off_a null when the
+// range is empty; vertex t gets id base + off_a[0] + t) and the next chunk's
+// first slice p0b (entries [b_start, b_start + nb), ids base + t), so the
+// slice's few vertices do not take a launch of their own. List entries hold
+// the point relative to their range's first point. ends_a / ends_b: null, or
+// the slab's values from the range's first point on, when they are the end
+// values' own (see slab_holds_end_values); the midpoints are always evaluated.
 template <int kPow, bool kLag>
 __global__ __launch_bounds__(kMcBlock) void k_nl_vertices(MapArgs a, const RootRec64* __restrict__ roots,
-                                                          double iso, int64_t p0, const double* __restrict__ ends,
-                                                          const uint32_t* __restrict__ vpt,
-                                                          const uint8_t* __restrict__ vax,
-                                                          const uint32_t* __restrict__ off,
-                                                          const uint32_t* __restrict__ total, uint32_t base,
+                                                          double iso, int64_t p0a, const double* __restrict__ ends_a,
+                                                          const uint32_t* __restrict__ off_a,
+                                                          const uint32_t* __restrict__ total_a, int64_t p0b,
+                                                          const double* __restrict__ ends_b, uint32_t b_start,
+                                                          uint32_t nb, const uint32_t* __restrict__ vpt,
+                                                          const uint8_t* __restrict__ vax, uint32_t base,
                                                           double* __restrict__ verts, int32_t* __restrict__ edges,
                                                           unsigned int* __restrict__ bad) {
-  const uint32_t first = off[0];
-  const uint32_t n = *total - first;
   const uint32_t t = blockIdx.x * kMcBlock + threadIdx.x;
-  if (t >= n) return;
+  int64_t p0, k;
+  const double* ends;
+  if (t >= b_start) {
+    if (t - b_start >= nb) return;
+    p0 = p0b;
+    ends = ends_b;
+    k = (int64_t)base + t;
+  } else {
+    if (!off_a) return;
+    const uint32_t first = off_a[0];
+    if (t >= *total_a - first) return;
+    p0 = p0a;
+    ends = ends_a;
+    k = (int64_t)base + first + t;
+  }
   const int64_t X = a.res[0], XY = X * a.res[1];
   const int64_t q = vpt[t];
   const int axis = vax[t];
@@ -1208,7 +1226,6 @@ __global__ __launch_bounds__(kMcBlock) void k_nl_vertices(MapArgs a, const RootR
   cell_center(a, x, y, z, pa);
   const double va = ends ? ends[q] - iso : nonlinear_value<kPow, kLag>(a, roots, pa, iso);
   const double* fb = ends ? ends + q + (axis == 0 ? 1 : axis == 1 ? X : XY) : nullptr;
-  const int64_t k = (int64_t)base + first + t;
   place_vertex<kPow, kLag>(a, roots, iso, x, y, z, axis, pa, va, fb, verts + 3 * k, bad);
   edges[4 * k + 0] = x;
   edges[4 * k + 1] = y;
@@ -1448,24 +1465,25 @@ qj_status nlmc_run(qj_workspace* ws, const MapArgs& a0, const qj_rational* f, do
     const int64_t skip = z0 > 0 ? 1 : 0;
     if ((st = grow(m.vpt, m.cap_vpt, (int64_t)(cnt_own + cnt_next), s))) return st;
     if ((st = grow(m.vax, m.cap_vax, (int64_t)(cnt_own + cnt_next), s))) return st;
-    if (cnt_own > 0 && z1 - z0 > skip) {
+    const bool own = cnt_own > 0 && z1 - z0 > skip;
+    if (own) {
       const int64_t n = XY * (z1 - z0 - skip);
       hipLaunchKernelGGL(k_nl_vlist, dim3(nblocks(n)), dim3(kMcBlock), 0, s, m.emask + skip * XY,
                          m.cnt + skip * XY, n, m.vpt, m.vax);
-      QJ_HIP(hipGetLastError());
-      launch_nl_vertices(a, dim3(nblocks((int64_t)cnt_own)), s, f->d64, iso, z0 + skip,
-                         ends ? ends + (z0 + skip - s0) * XY : (const double*)nullptr, m.vpt, m.vax,
-                         m.cnt + skip * XY, m.totals + 1, (uint32_t)nv, m.verts, m.c_edges, m.d_bad);
       QJ_HIP(hipGetLastError());
     }
     if (cnt_next > 0) {
       hipLaunchKernelGGL(k_nl_vlist, dim3(nblocks(XY)), dim3(kMcBlock), 0, s, m.emask + nown, m.slice_off, XY,
                          m.vpt + cnt_own, m.vax + cnt_own);
       QJ_HIP(hipGetLastError());
-      launch_nl_vertices(a, dim3(nblocks((int64_t)cnt_next)), s, f->d64, iso, z1,
-                         ends ? ends + (z1 - s0) * XY : (const double*)nullptr, m.vpt + cnt_own,
-                         m.vax + cnt_own, m.slice_off, m.totals + 3, (uint32_t)(nv + cnt_own), m.verts, m.c_edges,
-                         m.d_bad);
+    }
+    if (own || cnt_next > 0) {
+      launch_nl_vertices(a, dim3(nblocks((int64_t)(cnt_own + cnt_next))), s, f->d64, iso, z0 + skip,
+                         ends ? ends + (z0 + skip - s0) * XY : (const double*)nullptr,
+                         own ? (const uint32_t*)(m.cnt + skip * XY) : (const uint32_t*)nullptr,
+                         (const uint32_t*)(m.totals + 1), z1, ends ? ends + (z1 - s0) * XY : (const double*)nullptr,
+                         (uint32_t)cnt_own, (uint32_t)cnt_next, (const uint32_t*)m.vpt, (const uint8_t*)m.vax,
+                         (uint32_t)nv, m.verts, m.c_edges, m.d_bad);
       QJ_HIP(hipGetLastError());
     }
     // 5. active cells of the chunk, in cell order
