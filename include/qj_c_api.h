@@ -257,8 +257,13 @@ qj_status qj_mc_device_ptrs(qj_workspace* ws, const double** d_vertices, const i
  * order and the degenerate filter are the linear mesher's. params: the
  * mesher uses escape 2000 and the black-hole stop (TRIANGLE_MESH.cpp:97).
  * precision selects the slab evaluation (QJ_FP32_BAND or QJ_FP64); vertices are
- * always FP64. Returns QJ_ERR_NUMERIC (mesh still available) when an edge's end
- * values do not straddle the isosurface, where the reference exits.
+ * always FP64 (with QJ_FP32_BAND and iso 0 the end values are the slab's, whose
+ * signs are the FP64 map's; the midpoints are always evaluated). Returns
+ * QJ_ERR_NUMERIC (mesh still available) when an edge's end values do not
+ * straddle the isosurface, where the reference exits. The next chunk's slab is
+ * evaluated on a stream of the workspace's own while the current chunk is
+ * meshed on `stream` (two slab buffers); that stream first waits for the work
+ * queued on `stream` before the call, and `stream` waits for every slab.
  * Results are read with qj_mc_copy / qj_mc_device_ptrs. Synchronizes `stream`. */
 qj_status qj_nlmc_run(qj_workspace* ws, const qj_grid* grid, const qj_rational* f, const qj_map_params* params,
                       double iso, qj_precision precision, double band, int32_t chunk_slices,

@@ -161,9 +161,13 @@ struct McState {
   size_t tmp_bytes = 0;
   uint32_t* totals = nullptr;  // [0] active [1] vertices [2] triangles pre-filter [3] triangles
   int64_t nv = 0, nt = 0;
-  // non-linear mesher: slab of field values, slice offsets, error counter
-  double* slab = nullptr;
-  int64_t cap_slab = 0;
+  // non-linear mesher: two slabs of field values (chunk i's in slab[i % 2]; the
+  // next chunk's is evaluated on stream s2 while this one is meshed), slice
+  // offsets, error counter
+  double* slab[2] = {nullptr, nullptr};
+  int64_t cap_slab[2] = {0, 0};
+  hipStream_t s2 = nullptr;
+  hipEvent_t ev_start = nullptr, ev_field[2] = {nullptr, nullptr}, ev_free[2] = {nullptr, nullptr};
   uint32_t* slice_off = nullptr;
   int64_t cap_slice = 0;
   uint32_t* vpt = nullptr;  // vertices of a range of points: owning point, edge axis
@@ -997,7 +1001,14 @@ void mc_free(qj_workspace* ws) {
   hipFree(m->tris);
   hipFree(m->tmp);
   hipFree(m->totals);
-  hipFree(m->slab);
+  hipFree(m->slab[0]);
+  hipFree(m->slab[1]);
+  if (m->s2) {
+    hipStreamSynchronize(m->s2);
+    hipStreamDestroy(m->s2);
+  }
+  for (hipEvent_t e : {m->ev_start, m->ev_field[0], m->ev_field[1], m->ev_free[0], m->ev_free[1]})
+    if (e) hipEventDestroy(e);
   hipFree(m->slice_off);
   hipFree(m->vpt);
   hipFree(m->vax);
@@ -1392,7 +1403,19 @@ qj_status nlmc_run(qj_workspace* ws, const MapArgs& a0, const qj_rational* f, do
   }
   const int64_t slab_pts = XY * (chunk + 3), own_pts = XY * (chunk + 1);
   qj_status st;
-  if ((!src_field && (st = grow(m.slab, m.cap_slab, slab_pts, s))) || (st = grow(m.flags, m.cap_flags, slab_pts, s)) ||
+  if (!src_field) {
+    if (m.s2) QJ_HIP(hipStreamSynchronize(m.s2));
+    if ((st = grow(m.slab[0], m.cap_slab[0], slab_pts, s)) || (st = grow(m.slab[1], m.cap_slab[1], slab_pts, s)))
+      return st;
+    // the band lists at the largest slab's size, so no slab evaluation regrows them
+    if ((st = ensure_list(ws, slab_pts, (a0.flags & QJ_MAP_ADAPTIVE) != 0, s))) return st;
+    if (!m.s2) {
+      QJ_HIP(hipStreamCreateWithFlags(&m.s2, hipStreamNonBlocking));
+      for (hipEvent_t* e : {&m.ev_start, &m.ev_field[0], &m.ev_field[1], &m.ev_free[0], &m.ev_free[1]})
+        QJ_HIP(hipEventCreateWithFlags(e, hipEventDisableTiming));
+    }
+  }
+  if ((st = grow(m.flags, m.cap_flags, slab_pts, s)) ||
       (st = grow(m.emask, m.cap_emask, own_pts, s)) || (st = grow(m.cnt, m.cap_cnt, own_pts, s)) ||
       (st = grow(m.active, m.cap_active, own_pts, s)) || (st = grow(m.tri_off, m.cap_toff, own_pts, s)) ||
       (st = grow(m.blk, m.cap_blk, (int64_t)nblocks(own_pts), s)) ||
@@ -1415,20 +1438,50 @@ qj_status nlmc_run(qj_workspace* ws, const MapArgs& a0, const qj_rational* f, do
   // the mesher's map: escape 2000 and the black-hole stop (TRIANGLE_MESH.cpp:97, 1789-1791)
   MapArgs a = a0;
   const bool reuse_ends = slab_holds_end_values(a, src_field, iso, prec, band);
+  // slices [s0, s0 + ns) of the slab of the chunk that starts at slice zc: the
+  // chunk's points and one halo slice on each side
+  auto slab_range = [&](int64_t zc, int64_t& s0, int64_t& ns) {
+    const int64_t zc1 = zc + chunk < Z ? zc + chunk : Z;
+    s0 = zc > 0 ? zc - 1 : 0;
+    const int64_t s1 = zc1 + 1 < Z - 1 ? zc1 + 1 : Z - 1;
+    ns = s1 - s0 + 1;
+  };
+  // 1. slab of log|q| (or the caller's field, TRIANGLE_MESH.cpp:152-215): the
+  // chunk at slice zc into slab[b] on stream s2 (the Lagrange order has no FP32
+  // path: its slabs are FP64)
+  auto eval_slab = [&](int64_t zc, int b) -> qj_status {
+    int64_t fs0, fns;
+    slab_range(zc, fs0, fns);
+    MapArgs sa = a;
+    sa.res[2] = (int32_t)fns;
+    sa.zoff = (int32_t)fs0;
+    qj_status e = launch_field(ws, sa, f, (a.flags & QJ_MAP_LAGRANGE) ? QJ_FP64 : prec, band, m.slab[b], m.s2);
+    if (e) return e;
+    QJ_HIP(hipEventRecord(m.ev_field[b], m.s2));
+    return QJ_OK;
+  };
+  if (!src_field) {
+    // the first slab after the work the caller queued on s
+    QJ_HIP(hipEventRecord(m.ev_start, s));
+    QJ_HIP(hipStreamWaitEvent(m.s2, m.ev_start, 0));
+    if ((st = eval_slab(0, 0))) return st;
+  }
   uint64_t nv = 0, nt = 0, placed_next = 0;
-  for (int64_t z0 = 0; z0 < Z; z0 += chunk) {
+  for (int64_t z0 = 0, ci = 0; z0 < Z; z0 += chunk, ci++) {
     const int64_t z1 = z0 + chunk < Z ? z0 + chunk : Z;
-    const int64_t s0 = z0 > 0 ? z0 - 1 : 0;
-    const int64_t s1 = z1 + 1 < Z - 1 ? z1 + 1 : Z - 1;
-    const int64_t ns = s1 - s0 + 1;
-    // 1. slab of log|q| (or the caller's field, TRIANGLE_MESH.cpp:152-215)
-    const double* slab = src_field ? src_field + s0 * XY : m.slab;
+    int64_t s0, ns;
+    slab_range(z0, s0, ns);
+    const int b = (int)(ci & 1);
+    const double* slab = src_field ? src_field + s0 * XY : m.slab[b];
     if (!src_field) {
-      MapArgs sa = a;
-      sa.res[2] = (int32_t)ns;
-      sa.zoff = (int32_t)s0;
-      // the Lagrange order has no FP32 path: its slabs are FP64
-      if ((st = launch_field(ws, sa, f, (a.flags & QJ_MAP_LAGRANGE) ? QJ_FP64 : prec, band, m.slab, s))) return st;
+      // the next chunk's slab goes into the other buffer once the previous
+      // chunk's vertex kernel, its last reader, is done; this chunk's cases
+      // wait for this chunk's slab
+      if (z1 < Z) {
+        if (ci > 0) QJ_HIP(hipStreamWaitEvent(m.s2, m.ev_free[1 - b], 0));
+        if ((st = eval_slab(z1, 1 - b))) return st;
+      }
+      QJ_HIP(hipStreamWaitEvent(s, m.ev_field[b], 0));
     }
     const double* ends = reuse_ends ? slab : nullptr;
     // 2. cases
@@ -1486,6 +1539,7 @@ qj_status nlmc_run(qj_workspace* ws, const MapArgs& a0, const qj_rational* f, do
                          (uint32_t)nv, m.verts, m.c_edges, m.d_bad);
       QJ_HIP(hipGetLastError());
     }
+    if (!src_field) QJ_HIP(hipEventRecord(m.ev_free[b], s));  // the last reader of slab[b] is queued
     // 5. active cells of the chunk, in cell order
     const int64_t cz1 = z1 < Z - 1 ? z1 : Z - 1;  // cells exist for z <= Z-2
     const int64_t nc = cz1 > z0 ? cz1 - z0 : 0;
