@@ -1403,17 +1403,25 @@ qj_status nlmc_run(qj_workspace* ws, const MapArgs& a0, const qj_rational* f, do
   }
   const int64_t slab_pts = XY * (chunk + 3), own_pts = XY * (chunk + 1);
   qj_status st;
+  // the slab stream is idle whenever the call returns, early error returns
+  // included (its slab evaluations read the caller's roots)
+  struct JoinOnExit {
+    hipStream_t s = nullptr;
+    ~JoinOnExit() {
+      if (s) hipStreamSynchronize(s);
+    }
+  } join;
   if (!src_field) {
     if (m.s2) QJ_HIP(hipStreamSynchronize(m.s2));
     if ((st = grow(m.slab[0], m.cap_slab[0], slab_pts, s)) || (st = grow(m.slab[1], m.cap_slab[1], slab_pts, s)))
       return st;
     // the band lists at the largest slab's size, so no slab evaluation regrows them
     if ((st = ensure_list(ws, slab_pts, (a0.flags & QJ_MAP_ADAPTIVE) != 0, s))) return st;
-    if (!m.s2) {
-      QJ_HIP(hipStreamCreateWithFlags(&m.s2, hipStreamNonBlocking));
-      for (hipEvent_t* e : {&m.ev_start, &m.ev_field[0], &m.ev_field[1], &m.ev_free[0], &m.ev_free[1]})
-        QJ_HIP(hipEventCreateWithFlags(e, hipEventDisableTiming));
-    }
+    // created on the first mesh (what a failed creation left out, on the next)
+    if (!m.s2) QJ_HIP(hipStreamCreateWithFlags(&m.s2, hipStreamNonBlocking));
+    join.s = m.s2;
+    for (hipEvent_t* e : {&m.ev_start, &m.ev_field[0], &m.ev_field[1], &m.ev_free[0], &m.ev_free[1]})
+      if (!*e) QJ_HIP(hipEventCreateWithFlags(e, hipEventDisableTiming));
   }
   if ((st = grow(m.flags, m.cap_flags, slab_pts, s)) ||
       (st = grow(m.emask, m.cap_emask, own_pts, s)) || (st = grow(m.cnt, m.cap_cnt, own_pts, s)) ||
