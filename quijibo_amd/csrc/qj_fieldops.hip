@@ -146,10 +146,13 @@ __global__ __launch_bounds__(kOpsBlock) void k_mean_final(double* __restrict__ p
   }
 }
 
+// mean_at: [mean, count of summed cells]; when every cell was summed there is
+// nothing to replace and the field is not read again
 template <bool kNan>
 __global__ __launch_bounds__(kOpsBlock) void k_replace(double* __restrict__ f, int64_t n,
                                                        const double* __restrict__ mean_at) {
-  const double mean = *mean_at;
+  if (mean_at[1] == (double)n) return;  // the same for every thread of the grid
+  const double mean = mean_at[0];
   const int64_t stride = (int64_t)gridDim.x * kOpsBlock;
   for (int64_t i = (int64_t)blockIdx.x * kOpsBlock + threadIdx.x; i < n; i += stride) {
     const double v = f[i];
