@@ -529,7 +529,7 @@ def c3_tiles(args, log):
     t0 = time.perf_counter()
     for k in range(len(tiles)):
         F = api.FIELD_3D(tile, tile, tile, center=centers[k], lengths=lengths[k], data=out[k * per:(k + 1) * per])
-        F.clampInfsToMean(workspace=ws)
+        F.clampInfsToMean(workspace=ws, return_mean=False)  # no host wait per tile
         F.setBorders(F.totalCells())
     torch.cuda.synchronize()
     t_clamp = time.perf_counter() - t0

@@ -194,13 +194,15 @@ class FIELD_3D:
     def numpy(self):
         return self.data.detach().cpu().numpy().reshape(-1)
 
-    def clampInfsToMean(self, workspace=None, stream=None):
-        """FIELD_3D::clampInfsToMean (FIELD_3D.cpp:5182-5207) on the device; returns the mean used."""
-        return _clamp(self, "qj_field_clamp_infs_to_mean", workspace, stream)
+    def clampInfsToMean(self, workspace=None, stream=None, return_mean=True):
+        """FIELD_3D::clampInfsToMean (FIELD_3D.cpp:5182-5207) on the device; returns the mean used
+        (which waits for the stream), or None without waiting when return_mean is False."""
+        return _clamp(self, "qj_field_clamp_infs_to_mean", workspace, stream, return_mean)
 
-    def clampNansToMean(self, workspace=None, stream=None):
-        """FIELD_3D::clampNansToMean (FIELD_3D.cpp:5212-5234) on the device; returns the mean used."""
-        return _clamp(self, "qj_field_clamp_nans_to_mean", workspace, stream)
+    def clampNansToMean(self, workspace=None, stream=None, return_mean=True):
+        """FIELD_3D::clampNansToMean (FIELD_3D.cpp:5212-5234) on the device; returns the mean used
+        (which waits for the stream), or None without waiting when return_mean is False."""
+        return _clamp(self, "qj_field_clamp_nans_to_mean", workspace, stream, return_mean)
 
     def setBorders(self, value, stream=None):
         """FIELD_3D::setBorders (FIELD_3D.cpp:5357-5382): the six faces take value."""
@@ -228,13 +230,13 @@ class FIELD_3D:
         return FIELD_3D(*self.res, center=self.center, lengths=self.lengths, data=out, rotation=self.rotation), a
 
 
-def _clamp(field, fn, workspace, stream):
+def _clamp(field, fn, workspace, stream, return_mean=True):
     _device_tensor(field.data, "field", field.totalCells())
     ws = workspace or default_workspace()
     mean = ctypes.c_double()
-    check(getattr(_lib.load(), fn)(ws.handle, _ptr(field.data), field.totalCells(), ctypes.byref(mean),
-                                   _stream_ptr(stream)))
-    return mean.value
+    check(getattr(_lib.load(), fn)(ws.handle, _ptr(field.data), field.totalCells(),
+                                   ctypes.byref(mean) if return_mean else None, _stream_ptr(stream)))
+    return mean.value if return_mean else None
 
 
 def field_min_max(field, workspace=None, stream=None):

@@ -52,6 +52,20 @@ def test_clamp_infs_matches_the_serial_mean(gpu, n):
     assert np.all(got[~keep] == mean)
 
 
+@pytest.mark.parametrize("n_inf", [0, 37])
+def test_clamp_without_the_mean_gives_the_same_field(gpu, n_inf):
+    """return_mean=False (no mean read back, no wait) clamps exactly as the call that returns the
+    mean, with infinite cells to replace and with none (the replace pass then reads nothing)."""
+    f = _random_field(70001, 11, n_inf=n_inf)
+    a, b = _field((70001, 1, 1), f), _field((70001, 1, 1), f)
+    mean = a.clampInfsToMean()
+    assert b.clampInfsToMean(return_mean=False) is None
+    torch.cuda.synchronize()
+    assert np.array_equal(a.numpy(), b.numpy())
+    assert np.array_equal(a.numpy(), f) == (n_inf == 0)
+    assert n_inf == 0 or np.all(b.numpy()[np.isinf(f)] == mean)
+
+
 def test_clamp_nans_and_the_serial_sum_quirks(gpu):
     f = _random_field(50000, 3, n_nan=400)
     F = _field((100, 50, 10), f)
