@@ -477,27 +477,30 @@ constexpr int kAdjBlock = 128;
 constexpr int kMaxAdjIter = 16;
 // lanes per point: each lane of a group takes a contiguous share of the top
 // roots and of the bottom roots, so 40k points keep every SIMD busy
-constexpr int kAdjLanes = 8;
+template <class T>
+constexpr int adj_lanes() {
+  return sizeof(T) == 4 ? 16 : 8;  // FP64: 8 (twice the registers per value, 16 measured slower)
+}
 
 template <class T>
 __device__ __forceinline__ Q4<T> shfl_up_q(const Q4<T>& a, int d) {
-  return {__shfl_up(a.w, d, kAdjLanes), __shfl_up(a.x, d, kAdjLanes), __shfl_up(a.y, d, kAdjLanes),
-          __shfl_up(a.z, d, kAdjLanes)};
+  return {__shfl_up(a.w, d, adj_lanes<T>()), __shfl_up(a.x, d, adj_lanes<T>()), __shfl_up(a.y, d, adj_lanes<T>()),
+          __shfl_up(a.z, d, adj_lanes<T>())};
 }
 template <class T>
 __device__ __forceinline__ Q4<T> shfl_down_q(const Q4<T>& a, int d) {
-  return {__shfl_down(a.w, d, kAdjLanes), __shfl_down(a.x, d, kAdjLanes), __shfl_down(a.y, d, kAdjLanes),
-          __shfl_down(a.z, d, kAdjLanes)};
+  return {__shfl_down(a.w, d, adj_lanes<T>()), __shfl_down(a.x, d, adj_lanes<T>()), __shfl_down(a.y, d, adj_lanes<T>()),
+          __shfl_down(a.z, d, adj_lanes<T>())};
 }
 template <class T>
 __device__ __forceinline__ Q4<T> shfl_q(const Q4<T>& a, int src) {
-  return {__shfl(a.w, src, kAdjLanes), __shfl(a.x, src, kAdjLanes), __shfl(a.y, src, kAdjLanes),
-          __shfl(a.z, src, kAdjLanes)};
+  return {__shfl(a.w, src, adj_lanes<T>()), __shfl(a.x, src, adj_lanes<T>()), __shfl(a.y, src, adj_lanes<T>()),
+          __shfl(a.z, src, adj_lanes<T>())};
 }
 // sum over the lanes of a group
 template <class T>
 __device__ __forceinline__ T group_sum(T v) {
-  for (int o = kAdjLanes / 2; o > 0; o >>= 1) v += __shfl_xor(v, o);
+  for (int o = adj_lanes<T>() / 2; o > 0; o >>= 1) v += __shfl_xor(v, o);
   return v;
 }
 template <class T>
@@ -505,8 +508,9 @@ __device__ __forceinline__ Q4<T> group_sum_q(const Q4<T>& a) {
   return {group_sum(a.w), group_sum(a.x), group_sum(a.y), group_sum(a.z)};
 }
 // sum over the groups of a wave, lane by lane (the lanes that hold the same roots)
+template <class T>
 __device__ __forceinline__ double across_groups(double v) {
-  for (int o = kAdjLanes; o < 64; o <<= 1) v += __shfl_xor(v, o);
+  for (int o = adj_lanes<T>(); o < 64; o <<= 1) v += __shfl_xor(v, o);
   return v;
 }
 
@@ -533,22 +537,22 @@ __device__ __forceinline__ Share<T> group_product(const RootRec64* __restrict__ 
   p = unit(p);
   // ordered inclusive prefix and suffix products over the group (Hillis-Steele)
   Q4<T> pre = p, suf = p;
-  for (int d = 1; d < kAdjLanes; d <<= 1) {
+  for (int d = 1; d < adj_lanes<T>(); d <<= 1) {
     const Q4<T> y = shfl_up_q(pre, d);
     const Q4<T> z = shfl_down_q(suf, d);
     if (sub >= d) pre = mul(y, pre);
-    if (sub + d < kAdjLanes) suf = mul(suf, z);
+    if (sub + d < adj_lanes<T>()) suf = mul(suf, z);
   }
   Share<T> s;
-  s.all = unit(shfl_q(pre, kAdjLanes - 1));
+  s.all = unit(shfl_q(pre, adj_lanes<T>() - 1));
   const Q4<T> nxt = shfl_down_q(suf, 1);
-  s.after = sub + 1 < kAdjLanes ? unit(nxt) : Q4<T>{1, 0, 0, 0};
+  s.after = sub + 1 < adj_lanes<T>() ? unit(nxt) : Q4<T>{1, 0, 0, 0};
   s.L = group_sum(l);
   return s;
 }
 
 // slot layout per root i (top then bottom): 5 i + {0..3} = d/d root component, 5 i + 4 = d/d e_i
-// One group of kAdjLanes lanes per point; lane `sub` of a group takes top
+// One group of adj_lanes<T>() lanes per point; lane `sub` of a group takes top
 // roots [sub ct, sub ct + ct) and bottom roots [sub cb, sub cb + cb). Control
 // flow is uniform over the wave (every group runs max_iter steps and ct / cb
 // root steps, masked), so the shuffles always see active partners.
@@ -560,11 +564,12 @@ __global__ __launch_bounds__(kAdjBlock) void k_adjoint(MapArgs a, const RootRec6
   extern __shared__ double acc[];  // (kAdjBlock/64) x K
   for (int i = threadIdx.x; i < (kAdjBlock / 64) * K; i += kAdjBlock) acc[i] = 0.0;
   __syncthreads();
+  constexpr int kL = adj_lanes<T>();
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  const int sub = lane & (kAdjLanes - 1);
-  const int64_t pid = ((int64_t)blockIdx.x * kAdjBlock + threadIdx.x) / kAdjLanes;
+  const int sub = lane & (kL - 1);
+  const int64_t pid = ((int64_t)blockIdx.x * kAdjBlock + threadIdx.x) / kL;
   const bool active = pid < n;
-  const int ct = (a.nt + kAdjLanes - 1) / kAdjLanes, cb = (a.nb + kAdjLanes - 1) / kAdjLanes;
+  const int ct = (a.nt + kL - 1) / kL, cb = (a.nb + kL - 1) / kL;
   const int t0 = sub * ct, t1 = min(a.nt, t0 + ct);
   const int b0 = sub * cb, b1 = min(a.nb, b0 + cb);
   Q4<T> qs[kMaxAdjIter + 1];
@@ -627,12 +632,12 @@ __global__ __launch_bounds__(kAdjBlock) void k_adjoint(MapArgs a, const RootRec6
           g0 = -(double)bb.w; g1 = -(double)bb.x; g2 = -(double)bb.y; g3 = -(double)bb.z; g4 = (double)ge;
           if ((j & 7) == 7) S = unit(S);
         }
-        g0 = across_groups(g0);
-        g1 = across_groups(g1);
-        g2 = across_groups(g2);
-        g3 = across_groups(g3);
-        g4 = across_groups(g4);
-        if (lane < kAdjLanes && i < t1) {
+        g0 = across_groups<T>(g0);
+        g1 = across_groups<T>(g1);
+        g2 = across_groups<T>(g2);
+        g3 = across_groups<T>(g3);
+        g4 = across_groups<T>(g4);
+        if (lane < kL && i < t1) {
           double* r = acc + wave * K + 5 * i;
           r[0] += g0; r[1] += g1; r[2] += g2; r[3] += g3; r[4] += g4;
         }
@@ -660,12 +665,12 @@ __global__ __launch_bounds__(kAdjBlock) void k_adjoint(MapArgs a, const RootRec6
           g0 = -(double)bb.w; g1 = -(double)bb.x; g2 = -(double)bb.y; g3 = -(double)bb.z; g4 = (double)ge;
           if ((jj & 7) == 7) S = unit(S);
         }
-        g0 = across_groups(g0);
-        g1 = across_groups(g1);
-        g2 = across_groups(g2);
-        g3 = across_groups(g3);
-        g4 = across_groups(g4);
-        if (lane < kAdjLanes && j < b1) {
+        g0 = across_groups<T>(g0);
+        g1 = across_groups<T>(g1);
+        g2 = across_groups<T>(g2);
+        g3 = across_groups<T>(g3);
+        g4 = across_groups<T>(g4);
+        if (lane < kL && j < b1) {
           double* r = acc + wave * K + 5 * (a.nt + j);
           r[0] += g0; r[1] += g1; r[2] += g2; r[3] += g3; r[4] += g4;
         }
@@ -714,8 +719,8 @@ qj_status points_adjoint(qj_workspace* ws, const MapArgs& a, const qj_rational* 
     set_error("too many roots for the adjoint kernel (%d)", nr);
     return QJ_ERR_CAPACITY;
   }
-  constexpr int64_t kPointsPerBlock = kAdjBlock / kAdjLanes;
-  const int64_t nblocks = n > 0 ? (n + kPointsPerBlock - 1) / kPointsPerBlock : 1;
+  const int64_t per_block = kAdjBlock / (prec == QJ_FP64 ? adj_lanes<double>() : adj_lanes<float>());  // points
+  const int64_t nblocks = n > 0 ? (n + per_block - 1) / per_block : 1;
   qj_status st = ensure_partials(ws, nblocks * K + K, s);
   if (st) return st;
   double* sums = ws->d_partials + nblocks * K;
